@@ -1,0 +1,217 @@
+"""ctypes binding of the FP64 CPU oracle (oracle/build/liboracle_dpmm.so).
+
+Used only by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg, as the checker.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(ROOT, "oracle", "build", "liboracle_dpmm.so")
+
+_lib = None
+
+c_i64 = ctypes.c_int64
+c_u64 = ctypes.c_uint64
+c_u32 = ctypes.c_uint32
+c_dbl = ctypes.c_double
+c_int = ctypes.c_int
+P = ctypes.c_void_p
+
+
+class RunOpts(ctypes.Structure):
+    _fields_ = [
+        ("prior", c_int), ("alpha", c_dbl), ("iters", c_int), ("init_clusters", c_int),
+        ("burnout", c_int), ("argmax_sample_stop", c_int), ("split_stop", c_int),
+        ("max_clusters", c_dbl), ("seed", c_u64), ("full_mu_cov", c_int),
+        ("mult_use_weight", c_int), ("verbose", c_int),
+    ]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle library missing: {LIB_PATH} (run `make -C oracle`)")
+        L = ctypes.CDLL(LIB_PATH)
+        L.orc_lgamma.restype = c_dbl
+        L.orc_log_multivariate_gamma.restype = c_dbl
+        L.orc_logdet_spd.restype = c_dbl
+        L.orc_niw_log_marginal.restype = c_dbl
+        L.orc_mult_log_marginal.restype = c_dbl
+        L.orc_split_log_hr.restype = c_dbl
+        L.orc_split_log_hr.argtypes = [c_dbl, c_i64, c_i64, c_i64, c_dbl, c_dbl, c_dbl]
+        L.orc_merge_log_hr.restype = c_dbl
+        L.orc_merge_log_hr.argtypes = [c_dbl, c_i64, c_i64, c_dbl, c_dbl, c_dbl]
+        L.orc_nmi.restype = c_dbl
+        L.orc_run.restype = c_int
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(P) if a is not None else None
+
+
+def f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def philox(ctr, key):
+    c = (c_u32 * 4)(*ctr)
+    k = (c_u32 * 2)(*key)
+    o = (c_u32 * 4)()
+    lib().orc_philox4x32_10(c, k, o)
+    return list(o)
+
+
+def stream(kind, seed, c0, c2, c3, n, shape=None, alpha=None):
+    out = np.empty(n, np.float64)
+    L = lib()
+    if kind == "uniform":
+        L.orc_stream_uniforms(c_u64(seed), c_u32(c0), c_u32(c2), c_u32(c3), c_int(n), _p(out))
+    elif kind == "normal":
+        L.orc_stream_normals(c_u64(seed), c_u32(c0), c_u32(c2), c_u32(c3), c_int(n), _p(out))
+    elif kind == "gamma":
+        L.orc_stream_gammas(c_u64(seed), c_u32(c0), c_u32(c2), c_u32(c3), c_dbl(shape), c_int(n), _p(out))
+    elif kind == "dirichlet":
+        a = f64(alpha)
+        L.orc_stream_dirichlet(c_u64(seed), c_u32(c0), c_u32(c2), c_u32(c3), c_int(n), _p(a), _p(out))
+    return out
+
+
+def gauss_loglik(X, mu, inv_sigma, logdet, weight=1.0, use_weight=True):
+    X = f64(X)
+    out = np.empty(X.shape[0])
+    lib().orc_gauss_loglik(_p(X), c_i64(X.shape[0]), c_int(X.shape[1]), _p(f64(mu)), _p(f64(inv_sigma)),
+                           c_dbl(logdet), c_dbl(weight), c_int(int(use_weight)), _p(out))
+    return out
+
+
+def mult_loglik(X, logtheta):
+    X = f64(X)
+    out = np.empty(X.shape[0])
+    lib().orc_mult_loglik(_p(X), c_i64(X.shape[0]), c_int(X.shape[1]), _p(f64(logtheta)), _p(out))
+    return out
+
+
+def niw_posterior(k0, m0, v0, psi0, N, s, S):
+    D = len(m0)
+    k, v = c_dbl(), c_dbl()
+    m = np.empty(D)
+    psi = np.empty((D, D))
+    lib().orc_niw_posterior(c_int(D), c_dbl(k0), _p(f64(m0)), c_dbl(v0), _p(f64(psi0)), c_i64(N), _p(f64(s)),
+                            _p(f64(S)), ctypes.byref(k), _p(m), ctypes.byref(v), _p(psi))
+    return k.value, m, v.value, psi
+
+
+def niw_log_marginal(k0, v0, psi0, kn, vn, psin, N):
+    psi0 = f64(psi0)
+    D = psi0.shape[0]
+    return lib().orc_niw_log_marginal(c_int(D), c_dbl(k0), c_dbl(v0), _p(psi0), c_dbl(kn), c_dbl(vn),
+                                      _p(f64(psin)), c_i64(N))
+
+
+def mult_log_marginal(a0, an):
+    a0 = f64(a0)
+    return lib().orc_mult_log_marginal(c_int(a0.size), _p(a0), _p(f64(an)))
+
+
+def split_log_hr(alpha, N, Nl, Nr, L, Ll, Lr):
+    return lib().orc_split_log_hr(alpha, N, Nl, Nr, L, Ll, Lr)
+
+
+def merge_log_hr(alpha, Ni, Nj, L, Li, Lj):
+    return lib().orc_merge_log_hr(alpha, Ni, Nj, L, Li, Lj)
+
+
+def niw_draw(k, m, v, psi, seed, slot, it, full_mu_cov=False):
+    D = len(m)
+    mu = np.empty(D)
+    lam = np.empty((D, D))
+    ld = c_dbl()
+    lib().orc_niw_draw(c_int(D), c_dbl(k), _p(f64(m)), c_dbl(v), _p(f64(psi)), c_u64(seed), c_u32(slot),
+                       c_u32(it), c_int(int(full_mu_cov)), _p(mu), _p(lam), ctypes.byref(ld))
+    return mu, lam, ld.value
+
+
+def label_sweep(prior, X, mu=None, inv_sigma=None, logdet=None, logtheta=None, weights=None, seed=0, it=1,
+                final=False, mult_use_weight=False, want_scores=False):
+    X = f64(X)
+    N, D = X.shape
+    K = len(weights)
+    labels = np.empty(N, np.int32)
+    margin = np.empty(N)
+    scores = np.empty((N, K)) if want_scores else None
+    lib().orc_label_sweep(c_int(prior), _p(X), c_i64(N), c_int(D), c_int(K),
+                          _p(f64(mu) if mu is not None else None),
+                          _p(f64(inv_sigma) if inv_sigma is not None else None),
+                          _p(f64(logdet) if logdet is not None else None),
+                          _p(f64(logtheta) if logtheta is not None else None),
+                          _p(f64(weights)), c_int(int(mult_use_weight)), c_u64(seed), c_u32(it), c_int(int(final)),
+                          _p(labels), _p(margin), _p(scores))
+    return (labels, margin, scores) if want_scores else (labels, margin)
+
+
+def sublabel_sweep(prior, X, labels, mu=None, inv_sigma=None, logdet=None, logtheta=None, lr_weights=None, seed=0,
+                   it=1):
+    X = f64(X)
+    N, D = X.shape
+    K = len(lr_weights) // 2
+    subs = np.empty(N, np.int32)
+    margin = np.empty(N)
+    lib().orc_sublabel_sweep(c_int(prior), _p(X), c_i64(N), c_int(D), c_int(K), _p(i32(labels)),
+                             _p(f64(mu) if mu is not None else None),
+                             _p(f64(inv_sigma) if inv_sigma is not None else None),
+                             _p(f64(logdet) if logdet is not None else None),
+                             _p(f64(logtheta) if logtheta is not None else None),
+                             _p(f64(lr_weights)), c_u64(seed), c_u32(it), _p(subs), _p(margin))
+    return subs, margin
+
+
+def suff_stats(prior, X, labels, subs, K, want_S=True):
+    X = f64(X)
+    N, D = X.shape
+    Nsub = np.empty(2 * K, np.int64)
+    s = np.empty((2 * K, D))
+    S = np.empty((2 * K, D, D)) if (want_S and prior == 0) else None
+    Nc = np.empty(K, np.int64)
+    sc = np.empty((K, D))
+    Sc = np.empty((K, D, D)) if (want_S and prior == 0) else None
+    lib().orc_suff_stats(c_int(prior), _p(X), c_i64(N), c_int(D), c_int(K), _p(i32(labels)), _p(i32(subs)),
+                         _p(Nsub), _p(s), _p(S), _p(Nc), _p(sc), _p(Sc))
+    return Nsub, s, S, Nc, sc, Sc
+
+
+def nmi(a, b):
+    a = i32(a)
+    return lib().orc_nmi(_p(a), _p(i32(b)), c_i64(a.size))
+
+
+def run(X, prior=0, alpha=1.0, iters=100, init_clusters=1, burnout=15, seed=0, k0=1.0, m0=None, v0=None, psi0=None,
+        alpha0=None, max_clusters=float("inf"), full_mu_cov=False, mult_use_weight=False, verbose=False):
+    X = f64(X)
+    N, D = X.shape
+    o = RunOpts(prior=prior, alpha=alpha, iters=iters, init_clusters=init_clusters, burnout=burnout,
+                argmax_sample_stop=5, split_stop=5, max_clusters=max_clusters, seed=seed,
+                full_mu_cov=int(full_mu_cov), mult_use_weight=int(mult_use_weight), verbose=int(verbose))
+    labels = np.empty(N, np.int32)
+    khist = np.zeros(iters, np.int32)
+    weights = np.zeros(4096)
+    secs = np.zeros(iters)
+    if prior == 0:
+        m0 = np.zeros(D) if m0 is None else m0
+        v0 = float(D + 3) if v0 is None else v0
+        psi0 = np.eye(D) if psi0 is None else psi0
+        K = lib().orc_run(ctypes.byref(o), _p(X), c_i64(N), c_int(D), c_dbl(k0), _p(f64(m0)), c_dbl(v0),
+                          _p(f64(psi0)), None, _p(labels), _p(khist), _p(weights), c_int(4096), _p(secs))
+    else:
+        alpha0 = np.ones(D) if alpha0 is None else alpha0
+        K = lib().orc_run(ctypes.byref(o), _p(X), c_i64(N), c_int(D), c_dbl(0), None, c_dbl(0), None,
+                          _p(f64(alpha0)), _p(labels), _p(khist), _p(weights), c_int(4096), _p(secs))
+    return {"K": K, "labels": labels, "k_hist": khist, "weights": weights[:max(K, 0)], "iter_seconds": secs}

@@ -1,0 +1,170 @@
+"""Throughput of the DPMM sub-cluster sampler iteration on one MI355X.
+
+A step is one full sampler iteration (local_clusters_actions::group_step): parameter draws,
+label sweep, sub-label sweep, statistics, split / merge / reset / remove, over the whole synthetic
+data set resident in HBM.  The metric is point-iterations per second.  Warmup iterations let the
+sampler grow from one cluster to the mixture's cluster count before the timed steps.
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 150] [--config c3_d64]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK = {"fp32": 157.3, "fp64": 78.6, "bf16": 2500.0}   # dense TFLOP/s, MI355X
+HBM_PEAK = 8000.0                                       # GB/s
+
+CONFIGS = {
+    # N, D, true clusters, A scale, ridge, Dirichlet weight (None equal), alpha, dtype
+    "c1_readme_2d": dict(N=100_000, D=2, K=2, a=1.0, ridge=1.0, weight=None, alpha=1.0, dtype="float64"),
+    "c2_d32": dict(N=1_000_000, D=32, K=20, a=1.0, ridge=1.0, weight=1.0, alpha=10.0, dtype="float32"),
+    "c3_d64": dict(N=10_000_000, D=64, K=50, a=1.0, ridge=0.5, weight=0.5, alpha=10.0, dtype="float32"),
+}
+
+
+def gen_on_gpu(cfg, seed):
+    """Mixture of Gaussians generated on the GPU (means ~ U(-100,100)^D, covariances A A^T + ridge I)."""
+    import torch
+    rng = np.random.default_rng(seed)
+    N, D, K = cfg["N"], cfg["D"], cfg["K"]
+    means = rng.uniform(-100, 100, size=(K, D))
+    facs = []
+    for _ in range(K):
+        A = rng.normal(0.0, cfg["a"], size=(D, D))
+        facs.append(np.linalg.cholesky(A @ A.T + cfg["ridge"] * np.eye(D)))
+    w = np.full(K, 1.0 / K) if cfg["weight"] is None else rng.dirichlet(np.full(K, cfg["weight"]))
+    dt = torch.float64 if cfg["dtype"] == "float64" else torch.float32
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    gt = torch.multinomial(torch.tensor(w, device="cuda"), N, replacement=True, generator=g)
+    X = torch.empty((N, D), device="cuda", dtype=dt)
+    for k in range(K):
+        idx = (gt == k).nonzero().squeeze(1)
+        if idx.numel() == 0:
+            continue
+        z = torch.randn((idx.numel(), D), device="cuda", dtype=dt, generator=g)
+        Lt = torch.tensor(facs[k].T.copy(), device="cuda", dtype=dt).contiguous()
+        X[idx] = z @ Lt + torch.tensor(means[k], device="cuda", dtype=dt)
+    torch.cuda.synchronize()
+    return X, gt, means, np.stack(facs), w
+
+
+def cpu_baseline(Xs, means, facs, w, seed, target_s=8.0):
+    """FP64 oracle (OpenMP) on a bounded sample: one iteration's label sweep, sub-label sweep and
+    statistics over the sample, with the generating mixture as cluster parameters."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_ctypes as orc
+    K, D = means.shape
+    lam = np.stack([np.linalg.inv(f @ f.T) for f in facs])
+    ld = np.array([2 * np.log(np.diag(f)).sum() for f in facs])
+    smu = np.repeat(means, 2, axis=0)
+    slam = np.repeat(lam, 2, axis=0)
+    sld = np.repeat(ld, 2)
+    lrw = np.full(2 * K, 0.5)
+
+    def one(n):
+        X = np.ascontiguousarray(Xs[:n], dtype=np.float64)
+        t0 = time.perf_counter()
+        lab, _ = orc.label_sweep(0, X, means, lam, ld, weights=w, seed=seed, it=1)
+        sub, _ = orc.sublabel_sweep(0, X, lab, smu, slam, sld, lr_weights=lrw, seed=seed, it=1)
+        orc.suff_stats(0, X, lab, sub, K)
+        return time.perf_counter() - t0
+
+    n = min(len(Xs), 20_000)
+    dt = one(n)
+    n2 = int(min(len(Xs), max(n, n * target_s / max(dt, 1e-3))))
+    dt2 = one(n2)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": n2 / dt2, "unit": "point*iters/s", "cores": threads, "kind": "port",
+            "sample": f"FP64 oracle label sweep + sub-label sweep + statistics of one iteration over {n2} points "
+                      f"(K={K}, D={D}), {dt2:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=150)
+    ap.add_argument("--config", default="c3_d64", choices=sorted(CONFIGS))
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (points would shard across GPUs "
+              "with one FP64 all-reduce of the sufficient statistics per iteration; not built).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import dpmmsubclusters_gpu_amd as dp
+
+    cfg = CONFIGS[args.config]
+    N, D = cfg["N"], cfg["D"]
+    print(f"[bench] generating {args.config}: N={N} D={D} K*={cfg['K']}", file=sys.stderr, flush=True)
+    X, gt, means, facs, w = gen_on_gpu(cfg, args.seed)
+    dtype = np.float64 if cfg["dtype"] == "float64" else np.float32
+    dev = dp.DeviceSweep(None, dp.prior_type.Gaussian, seed=args.seed + 1, device_ptr=X.data_ptr(), N=N, D=D,
+                         dtype=dtype)
+    hyper = dp.niw_hyperparams(1.0, np.zeros(D), D + 3.0, np.eye(D))
+    prec = dp.Precision.FP64 if dtype == np.float64 else dp.Precision.FP32
+    iters = args.warmup + args.steps + 10
+    smp = dp.Sampler(dev, hyper, cfg["alpha"], iters, seed=args.seed + 1, precision=prec)
+    t_w = time.perf_counter()
+    for t in range(1, args.warmup + 1):
+        smp.step(t)
+        if t % 10 == 0 or t == args.warmup:
+            print(f"[bench] warmup {t}/{args.warmup} K={smp.num_clusters()} ({time.perf_counter() - t_w:.1f}s)",
+                  file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    dev.synchronize()
+    lab_ms, sub_ms, st_ms, host_ms, flops = [], [], [], [], []
+    t0 = time.perf_counter()
+    for t in range(args.warmup + 1, args.warmup + args.steps + 1):
+        kc = smp.num_clusters()
+        smp.step(t)
+        a, b, c = dev.kernel_ms()
+        lab_ms.append(a); sub_ms.append(b); st_ms.append(c); host_ms.append(smp.host_ms())
+        flops.append(2.0 * kc * D * D * N)
+    dev.synchronize()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_step = 1000 * elapsed / args.steps
+    value = N * args.steps / elapsed
+    avg_lab = float(np.mean(lab_ms))
+    achieved = float(np.mean(flops)) / (avg_lab * 1e-3) / 1e12
+    peak = PEAK["fp64" if prec == dp.Precision.FP64 else "fp32"]
+    K_final = smp.num_clusters()
+    from dpmmsubclusters_gpu_amd import _nmi
+    lab, _ = dev.get_labels()
+    nmi = _nmi(gt.cpu().numpy(), lab)
+    out = {
+        "metric": "point*iters/sec", "value": value, "unit": "point*iters/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32" if prec != dp.Precision.FP64 else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{args.config}: Gaussian DPMM sampler iteration, N={N}, D={D}, "
+                               f"{cfg['K']} true clusters, alpha={cfg['alpha']}, NIW(k=1, m=0, v=D+3, psi=I)",
+                   "N": N, "D": D, "clusters_timed": K_final, "precision": args.precision,
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "mfma", "kernel": "k_label_gauss", "achieved": achieved, "peak": peak,
+                     "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
+                     "algorithmic": "2*K*D^2 flops per point per launch (dense whitening of K clusters)"},
+        "breakdown_ms": {"label_sweep": avg_lab, "sub_sweep": float(np.mean(sub_ms)),
+                         "statistics": float(np.mean(st_ms)), "host_fp64": float(np.mean(host_ms))},
+        "final_nmi_vs_truth": nmi,
+    }
+    if not args.no_cpu_baseline:
+        print("[bench] cpu baseline (FP64 oracle)", file=sys.stderr, flush=True)
+        Xs = X[: min(N, 2_000_000)].cpu().numpy()
+        out["cpu_baseline"] = cpu_baseline(Xs, means, facs, w, args.seed)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

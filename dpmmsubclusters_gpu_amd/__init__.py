@@ -1,0 +1,370 @@
+"""MI355X-native sub-cluster sweep of the DPMM split/merge sampler (Chang & Fisher).
+
+Python mirror of the reference's interface (BGU-CS-VIL/DPMMSubClusters_GPU):
+``dp_parallel_sampling_class(num_labels, all_data, random_seed, prior_type)`` with
+``dp_parallel(...)`` returning a ``ModelInfo`` (dp_parallel_sampling.h:11-43, modelInfo.h),
+``niw_hyperparams`` / ``multinomial_hyper`` (priors/*_hyperparams.h) and ``prior_type``
+(global_params.h:14-18), over the C ABI of lib/libdpmm_hip.so (include/dpmm_hip.h).
+
+``DeviceSweep`` exposes the device layer itself (the reference's cudaKernel plugin,
+cudaKernel.cuh:31-138) for callers that drive iterations themselves.
+
+The HIP library is required: every entry point raises if lib/libdpmm_hip.so is missing or no
+GPU is visible.  There is no CPU fallback.
+"""
+import ctypes
+import enum
+import os
+import time
+
+import numpy as np
+
+__all__ = ["prior_type", "Precision", "niw_hyperparams", "multinomial_hyper", "ModelInfo",
+           "dp_parallel_sampling_class", "DeviceSweep", "DPMMError", "library_path", "load_library"]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+library_path = os.path.join(_HERE, "lib", "libdpmm_hip.so")
+
+DPMM_OK = 0
+_ERRNAMES = {-1: "EINVAL", -2: "EHIP", -3: "ENOMEM", -4: "ENOTSPD", -5: "ESTATE"}
+
+
+class DPMMError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"dpmm error {code} ({_ERRNAMES.get(code, '?')}): {msg}")
+        self.code = code
+
+
+class prior_type(enum.IntEnum):
+    """prior_type (global_params.h:14-18)."""
+    Gaussian = 0
+    Multinomial = 1
+
+
+class Precision(enum.IntEnum):
+    FP64 = 0
+    FP32 = 1
+    MIXED = 2
+
+
+class _Options(ctypes.Structure):
+    _fields_ = [("prior", ctypes.c_int), ("alpha", ctypes.c_double), ("iters", ctypes.c_int),
+                ("init_clusters", ctypes.c_int), ("burnout", ctypes.c_int), ("argmax_sample_stop", ctypes.c_int),
+                ("split_stop", ctypes.c_int), ("max_clusters", ctypes.c_double), ("seed", ctypes.c_uint64),
+                ("full_mu_cov", ctypes.c_int), ("mult_use_weight", ctypes.c_int), ("precision", ctypes.c_int),
+                ("verbose", ctypes.c_int)]
+
+
+_lib = None
+_P = ctypes.c_void_p
+
+
+def load_library():
+    """Load lib/libdpmm_hip.so; raises if it is missing (build with dpmmsubclusters_gpu_amd.build)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(library_path):
+        raise RuntimeError(f"HIP library not built: {library_path} (run `python -m dpmmsubclusters_gpu_amd.build`)")
+    L = ctypes.CDLL(library_path)
+    L.dpmm_last_error.restype = ctypes.c_char_p
+    i64, u64, u32, dbl, i = ctypes.c_int64, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_double, ctypes.c_int
+    sig = {
+        "dpmm_device_count": [_P],
+        "dpmm_create": [_P, i64, i, i, i, u64, _P],
+        "dpmm_create_device": [_P, i64, i, i, i, u64, _P],
+        "dpmm_destroy": [_P],
+        "dpmm_set_precision": [_P, i],
+        "dpmm_sample_initial_labels": [_P, i],
+        "dpmm_set_labels": [_P, _P, _P],
+        "dpmm_get_labels": [_P, _P, _P],
+        "dpmm_set_gaussian_params": [_P, i, _P, _P, _P, _P, _P, _P, _P, _P],
+        "dpmm_set_gaussian_factors": [_P, i, _P, _P, _P, _P, _P, _P, _P, _P],
+        "dpmm_set_multinomial_params": [_P, i, _P, _P, _P, _P, i],
+        "dpmm_sample_cluster_labels": [_P, u32, i],
+        "dpmm_sample_sub_labels": [_P, u32],
+        "dpmm_sufficient_statistics": [_P, i, _P, _P, _P],
+        "dpmm_relabel": [_P, u32, i, _P, _P],
+        "dpmm_last_kernel_ms": [_P, _P, _P, _P],
+        "dpmm_synchronize": [_P],
+        "dpmm_sampler_create": [_P, _P, dbl, _P, dbl, _P, _P, _P],
+        "dpmm_sampler_step": [_P, i],
+        "dpmm_sampler_num_clusters": [_P],
+        "dpmm_sampler_weights": [_P, _P, _P, i],
+        "dpmm_sampler_host_ms": [_P, _P],
+        "dpmm_sampler_destroy": [_P],
+        "dpmm_run": [_P, _P, dbl, _P, dbl, _P, _P, _P, _P, _P, _P, i, _P],
+    }
+    for name, args in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != DPMM_OK:
+        raise DPMMError(rc, load_library().dpmm_last_error().decode(errors="replace"))
+
+
+def _ptr(a):
+    return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+def _f64(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.int32)
+
+
+class niw_hyperparams:
+    """niw_hyperparams(k, m, v, psi) (priors/niw_hyperparams.h:13); psi is Psi / v as in the reference."""
+
+    def __init__(self, k, m, v, psi):
+        self.k = float(k)
+        self.m = np.asarray(m, dtype=np.float64).ravel()
+        self.v = float(v)
+        self.psi = np.asarray(psi, dtype=np.float64).reshape(self.m.size, self.m.size)
+
+
+class multinomial_hyper:
+    """multinomial_hyper(alpha) (priors/multinomial_hyper.h:13)."""
+
+    def __init__(self, alpha):
+        self.alpha = np.asarray(alpha, dtype=np.float64).ravel()
+
+
+class ModelInfo:
+    """modelInfo.h:8-56: labels (1-based, as the reference), iter_count (seconds per iteration),
+    cluster_count_history, likelihood_history, nmi_score_history, and the final mixture weights."""
+
+    def __init__(self):
+        self.labels = None
+        self.iter_count = []
+        self.cluster_count_history = []
+        self.likelihood_history = []
+        self.nmi_score_history = []
+        self.weights = None
+        self.cluster_counts = None
+
+
+def _nmi(a, b):
+    """2 I(X;Y) / (H(X) + H(Y)) (dp_parallel_sampling.cpp:207-222)."""
+    a = np.asarray(a, np.int64)
+    b = np.asarray(b, np.int64)
+    n = a.size
+    _, ai = np.unique(a, return_inverse=True)
+    _, bi = np.unique(b, return_inverse=True)
+    cont = np.zeros((ai.max() + 1, bi.max() + 1))
+    np.add.at(cont, (ai, bi), 1)
+    pa = cont.sum(1) / n
+    pb = cont.sum(0) / n
+    pab = cont / n
+    nz = pab > 0
+    mi = (pab[nz] * np.log(pab[nz] / np.outer(pa, pb)[nz])).sum()
+    ha = -(pa[pa > 0] * np.log(pa[pa > 0])).sum()
+    hb = -(pb[pb > 0] * np.log(pb[pb > 0])).sum()
+    return 1.0 if ha + hb <= 0 else 2 * mi / (ha + hb)
+
+
+class DeviceSweep:
+    """The device layer (the reference's cudaKernel, cudaKernel.cuh:31-138) over HBM-resident points.
+
+    X is N x D (each point contiguous): float32 / float64 for the Gaussian prior, int32 counts for
+    the multinomial prior.  Labels are 0-based here; sub-labels are 0 / 1.
+    """
+
+    def __init__(self, X, prior=prior_type.Gaussian, seed=0, device_ptr=None, N=None, D=None, dtype=None):
+        L = load_library()
+        self._ctx = ctypes.c_void_p()
+        if device_ptr is not None:
+            self.N, self.D = int(N), int(D)
+            self._dtype = {np.float64: 0, np.float32: 1, np.int32: 2}[np.dtype(dtype).type]
+            _check(L.dpmm_create_device(ctypes.c_void_p(device_ptr), self.N, self.D, self._dtype, int(prior),
+                                        int(seed), ctypes.byref(self._ctx)))
+        else:
+            X = np.ascontiguousarray(X)
+            if X.dtype not in (np.float64, np.float32, np.int32):
+                raise ValueError("X must be float64, float32 or int32")
+            self.N, self.D = X.shape
+            self._dtype = {np.dtype(np.float64): 0, np.dtype(np.float32): 1, np.dtype(np.int32): 2}[X.dtype]
+            _check(L.dpmm_create(_ptr(X), self.N, self.D, self._dtype, int(prior), int(seed), ctypes.byref(self._ctx)))
+        self.prior = prior_type(prior)
+
+    def close(self):
+        if self._ctx:
+            load_library().dpmm_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._ctx
+
+    def set_precision(self, p):
+        _check(load_library().dpmm_set_precision(self._ctx, int(p)))
+
+    def sample_initial_labels(self, init_clusters):
+        _check(load_library().dpmm_sample_initial_labels(self._ctx, int(init_clusters)))
+
+    def set_labels(self, labels=None, sub_labels=None):
+        lab, sub = _i32(labels), _i32(sub_labels)
+        _check(load_library().dpmm_set_labels(self._ctx, _ptr(lab), _ptr(sub)))
+
+    def get_labels(self):
+        lab = np.empty(self.N, np.int32)
+        sub = np.empty(self.N, np.int32)
+        _check(load_library().dpmm_get_labels(self._ctx, _ptr(lab), _ptr(sub)))
+        return lab, sub
+
+    def set_gaussian_params(self, mu, inv_sigma, logdet_sigma, weights, sub_mu, sub_inv_sigma, sub_logdet_sigma,
+                            lr_weights, factors=False):
+        mu, S, ld, w = _f64(mu), _f64(inv_sigma), _f64(logdet_sigma), _f64(weights)
+        smu, sS, sld, lrw = _f64(sub_mu), _f64(sub_inv_sigma), _f64(sub_logdet_sigma), _f64(lr_weights)
+        K = w.size
+        f = load_library().dpmm_set_gaussian_factors if factors else load_library().dpmm_set_gaussian_params
+        _check(f(self._ctx, K, _ptr(mu), _ptr(S), _ptr(ld), _ptr(w), _ptr(smu), _ptr(sS), _ptr(sld), _ptr(lrw)))
+
+    def set_multinomial_params(self, logtheta, weights, sub_logtheta, lr_weights, use_weight=False):
+        lt, w, slt, lrw = _f64(logtheta), _f64(weights), _f64(sub_logtheta), _f64(lr_weights)
+        K = lt.shape[0]
+        _check(load_library().dpmm_set_multinomial_params(self._ctx, K, _ptr(lt), _ptr(w), _ptr(slt), _ptr(lrw),
+                                                         int(use_weight)))
+
+    def sample_cluster_labels(self, iteration, final=False):
+        _check(load_library().dpmm_sample_cluster_labels(self._ctx, int(iteration), int(bool(final))))
+
+    def sample_sub_labels(self, iteration):
+        _check(load_library().dpmm_sample_sub_labels(self._ctx, int(iteration)))
+
+    def sufficient_statistics(self, K, want_S=True):
+        Nsub = np.empty(2 * K, np.int64)
+        s = np.empty((2 * K, self.D))
+        S = np.empty((2 * K, self.D, self.D)) if (want_S and self.prior == prior_type.Gaussian) else None
+        _check(load_library().dpmm_sufficient_statistics(self._ctx, K, _ptr(Nsub), _ptr(s), _ptr(S)))
+        return Nsub, s, S
+
+    def relabel(self, iteration, new_label, sub_action):
+        nl, act = _i32(new_label), _i32(sub_action)
+        _check(load_library().dpmm_relabel(self._ctx, int(iteration), act.size, _ptr(nl), _ptr(act)))
+
+    def kernel_ms(self):
+        a, b, c = ctypes.c_float(), ctypes.c_float(), ctypes.c_float()
+        _check(load_library().dpmm_last_kernel_ms(self._ctx, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, c.value
+
+    def synchronize(self):
+        _check(load_library().dpmm_synchronize(self._ctx))
+
+
+class Sampler:
+    """Stateful driver: init_model + init_first_clusters, then one group_step per step()."""
+
+    def __init__(self, dev, hyper, alpha, iters, init_clusters=1, burnout=15, seed=0, max_clusters=float("inf"),
+                 precision=None, full_mu_cov=False, mult_use_weight=False, verbose=False):
+        L = load_library()
+        self.dev = dev
+        if precision is None:
+            precision = Precision.FP64 if dev._dtype == 0 else Precision.FP32
+        self.opts = _Options(prior=int(dev.prior), alpha=float(alpha), iters=int(iters),
+                             init_clusters=int(init_clusters), burnout=int(burnout), argmax_sample_stop=5,
+                             split_stop=5, max_clusters=float(max_clusters), seed=int(seed),
+                             full_mu_cov=int(full_mu_cov), mult_use_weight=int(mult_use_weight),
+                             precision=int(precision), verbose=int(verbose))
+        self._s = ctypes.c_void_p()
+        if dev.prior == prior_type.Gaussian:
+            self._m0, self._psi0 = _f64(hyper.m), _f64(hyper.psi)
+            _check(L.dpmm_sampler_create(dev.handle, ctypes.byref(self.opts), hyper.k, _ptr(self._m0), hyper.v,
+                                         _ptr(self._psi0), None, ctypes.byref(self._s)))
+        else:
+            self._a0 = _f64(hyper.alpha)
+            _check(L.dpmm_sampler_create(dev.handle, ctypes.byref(self.opts), 0.0, None, 0.0, None, _ptr(self._a0),
+                                         ctypes.byref(self._s)))
+
+    def step(self, iteration):
+        _check(load_library().dpmm_sampler_step(self._s, int(iteration)))
+
+    def num_clusters(self):
+        return load_library().dpmm_sampler_num_clusters(self._s)
+
+    def weights(self):
+        K = self.num_clusters()
+        w = np.zeros(K)
+        n = np.zeros(K, np.int64)
+        _check(load_library().dpmm_sampler_weights(self._s, _ptr(w), _ptr(n), K))
+        return w, n
+
+    def host_ms(self):
+        v = ctypes.c_double()
+        _check(load_library().dpmm_sampler_host_ms(self._s, ctypes.byref(v)))
+        return v.value
+
+    def close(self):
+        if self._s:
+            load_library().dpmm_sampler_destroy(self._s)
+            self._s = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class dp_parallel_sampling_class:
+    """dp_parallel_sampling_class(numLabels, all_data, randomSeed, priorType) (dp_parallel_sampling.h:14).
+
+    all_data is D x N as in the reference (each column a point); float64 or float32 for the Gaussian
+    prior, integer counts for the multinomial prior.
+    """
+
+    def __init__(self, num_labels, all_data, random_seed, prior):
+        all_data = np.asarray(all_data)
+        if all_data.ndim != 2 or all_data.shape[1] != num_labels:
+            raise ValueError("all_data must be D x numLabels")
+        prior = prior_type(prior)
+        X = all_data.T
+        if prior == prior_type.Multinomial:
+            X = np.ascontiguousarray(X, dtype=np.int32)
+        elif X.dtype != np.float32:
+            X = np.ascontiguousarray(X, dtype=np.float64)
+        else:
+            X = np.ascontiguousarray(X)
+        self.prior = prior
+        self.seed = int(random_seed) & 0xFFFFFFFFFFFFFFFF
+        self.dev = DeviceSweep(X, prior, self.seed)
+
+    def dp_parallel(self, local_hyper_params, alpha_param, iters=100, init_clusters=1, verbose=True, draw_labels=False,
+                    save_model=False, burnout=15, gt=None, max_clusters=float("inf"), outlier_weight=0.0,
+                    outlier_params=None, precision=None):
+        """dp_parallel (dp_parallel_sampling.cpp:132-160 + run_model 175-249)."""
+        if outlier_weight:
+            raise NotImplementedError("the outlier cluster is not part of this build")
+        s = Sampler(self.dev, local_hyper_params, alpha_param, iters, init_clusters, burnout, self.seed, max_clusters,
+                    precision)
+        mi = ModelInfo()
+        for it in range(1, iters + 1):
+            t0 = time.perf_counter()
+            s.step(it)
+            mi.iter_count.append(time.perf_counter() - t0)
+            mi.cluster_count_history.append(s.num_clusters())
+            mi.likelihood_history.append(1.0)
+            if verbose and gt is not None:
+                lab, _ = self.dev.get_labels()
+                mi.nmi_score_history.append(_nmi(np.asarray(gt), lab))
+            if verbose:
+                print(f"Iteration: {it} || Clusters count: {mi.cluster_count_history[-1]} || "
+                      f"Iter Time: {mi.iter_count[-1]:.4f}", flush=True)
+        lab, _ = self.dev.get_labels()
+        mi.labels = lab + 1
+        mi.weights, mi.cluster_counts = s.weights()
+        s.close()
+        return mi

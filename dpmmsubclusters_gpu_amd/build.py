@@ -1,0 +1,65 @@
+"""Builds the HIP library lib/libdpmm_hip.so for gfx950 (in-tree, so it travels with the repo).
+
+    python -m dpmmsubclusters_gpu_amd.build        # or build() from Python
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+LIB_DIR = os.path.join(HERE, "lib")
+LIB = os.path.join(LIB_DIR, "libdpmm_hip.so")
+
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+HIPCC = os.path.join(ROCM, "bin", "hipcc")
+ARCH = "gfx950"
+
+# Packed FP32 instructions are disabled for device code: some of their forms have given wrong
+# results on MI355X boxes, silently.
+DEVICE_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-target-feature",
+                "-Xclang", "-packed-fp32-ops", "-Wno-unused-command-line-argument"]
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-fopenmp", "-Wall", "-Wno-unused-function"]
+
+DEVICE_SOURCES = ["dpmm_device.hip", "inst_gauss_small.hip", "inst_gauss_64.hip", "inst_gauss_96.hip",
+                  "inst_gauss_128.hip"]
+HOST_SOURCES = ["sampler.cpp"]
+
+
+def _run(cmd):
+    print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+
+
+def build(verbose=True, jobs=None):
+    os.makedirs(BUILD, exist_ok=True)
+    os.makedirs(LIB_DIR, exist_ok=True)
+    objs = []
+    procs = []
+    for src in DEVICE_SOURCES:
+        obj = os.path.join(BUILD, src + ".o")
+        cmd = [HIPCC] + DEVICE_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
+        procs.append((cmd, subprocess.Popen(cmd)))
+        objs.append(obj)
+    for src in HOST_SOURCES:
+        obj = os.path.join(BUILD, src + ".o")
+        cmd = ["g++"] + HOST_FLAGS + ["-I", os.path.join(ROCM, "include"), "-c", os.path.join(CSRC, src), "-o", obj]
+        procs.append((cmd, subprocess.Popen(cmd)))
+        objs.append(obj)
+    for cmd, p in procs:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        if p.wait() != 0:
+            raise RuntimeError("compile failed: " + " ".join(cmd))
+    tmp = LIB + ".tmp"
+    _run(["g++", "-shared", "-o", tmp] + objs + ["-fopenmp", f"-L{ROCM}/lib", "-lamdhip64",
+                                                 f"-Wl,-rpath,{ROCM}/lib"])
+    shutil.move(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build()
+    sys.exit(0)

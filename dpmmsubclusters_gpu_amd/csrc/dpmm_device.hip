@@ -1,0 +1,920 @@
+// MI355X (gfx950) device layer of the DPMM sub-cluster sweep and its C ABI (include/dpmm_hip.h).
+//
+// One iteration's device work, all on one stream:
+//   label sweep     X read once; every point scored against all K clusters (whitened
+//                   residual || W_k (x - mu_k) ||^2 or x . log theta_k), Gumbel-max draw
+//   partition       block histograms of labels -> scans -> stable scatter of point indices
+//                   (ballot ranks, no global atomics): segments of points per cluster
+//   sub-label sweep per segment chunk: the cluster's two sub-cluster parameters in LDS,
+//                   logistic draw of each point's sub-label
+//   statistics      per (chunk, sub-cluster): count, FP64 sum and FP64 scatter matrix,
+//                   then a fixed-order reduction over the chunks of each cluster
+//   relabel         one table-driven pass for split / merge / reset / remove
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/dpmm_hip.h"
+#include "dpmm_kernels.h"
+#include "dpmm_device.h"
+#include "dpmm_gauss.h"
+
+using namespace dpmm;
+
+namespace dpmm {
+DPMM_GAUSS_EXTERN(float, 4)
+DPMM_GAUSS_EXTERN(float, 8)
+DPMM_GAUSS_EXTERN(float, 16)
+DPMM_GAUSS_EXTERN(float, 32)
+DPMM_GAUSS_EXTERN(float, 48)
+DPMM_GAUSS_EXTERN(float, 64)
+DPMM_GAUSS_EXTERN(float, 96)
+DPMM_GAUSS_EXTERN(float, 128)
+DPMM_GAUSS_EXTERN(double, 4)
+DPMM_GAUSS_EXTERN(double, 8)
+DPMM_GAUSS_EXTERN(double, 16)
+DPMM_GAUSS_EXTERN(double, 32)
+DPMM_GAUSS_EXTERN(double, 48)
+DPMM_GAUSS_EXTERN(double, 64)
+}  // namespace dpmm
+
+namespace {
+thread_local std::string g_err;
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+#define HIPCHK(expr)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            return fail(DPMM_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));            \
+    } while (0)
+
+constexpr int kBlockPts = 4096;    // points per partition block (256 threads x 16)
+constexpr int kMaxK = 2048;        // clusters per context
+}  // namespace
+
+// ============================================================================ kernels ======
+
+__global__ void __launch_bounds__(kThreads) k_init_labels(int32_t* labels, int32_t* subs, int64_t N, int K0,
+                                                          uint32_t key0, uint32_t key1) {
+    const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= N) return;
+    const Ctr4 w = philox4x32_10((uint32_t)i, 0, 0, TAG_INIT, key0, key1);
+    int l = (int)(u01_f64(w.v[0]) * (double)K0);
+    labels[i] = l < K0 - 1 ? l : K0 - 1;
+    subs[i] = (int32_t)(w.v[1] & 1u);
+}
+
+// Multinomial label sweep: 64 points x K clusters per workgroup, x . log theta accumulated over
+// 32-dimension chunks staged in LDS.  Thread (p, g) owns clusters g, g + 4, ...
+template <int KPT>
+__global__ void __launch_bounds__(kThreads) k_label_mult(const int32_t* __restrict__ X, int64_t N, int D, int K,
+                                                         const float* __restrict__ P, int stride, uint32_t key0,
+                                                         uint32_t key1, uint32_t iter, int final_iter,
+                                                         int32_t* __restrict__ labels) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    float* xs = reinterpret_cast<float*>(smem);        // [32][65]
+    float* th = xs + 32 * 65;                           // [K][33]
+    __shared__ float bs[4][64];
+    __shared__ int bk[4][64];
+    const int p = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const int64_t base = (int64_t)blockIdx.x * 64;
+    float acc[KPT];
+#pragma unroll
+    for (int j = 0; j < KPT; ++j) acc[j] = 0.f;
+    for (int d0 = 0; d0 < D; d0 += 32) {
+        __syncthreads();
+        for (int e = threadIdx.x; e < 64 * 32; e += kThreads) {
+            const int pp = e >> 5, dd = e & 31;
+            const int64_t i = base + pp;
+            xs[dd * 65 + pp] = (i < N && d0 + dd < D) ? (float)X[i * D + d0 + dd] : 0.f;
+        }
+        for (int e = threadIdx.x; e < K * 32; e += kThreads) {
+            const int kk = e >> 5, dd = e & 31;
+            th[kk * 33 + dd] = (d0 + dd < D) ? P[(size_t)kk * stride + d0 + dd] : 0.f;
+        }
+        __syncthreads();
+        for (int dd = 0; dd < 32; ++dd) {
+            const float xv = xs[dd * 65 + p];
+#pragma unroll
+            for (int j = 0; j < KPT; ++j) {
+                const int k = g + 4 * j;
+                if (k < K) acc[j] = fmaf(xv, th[k * 33 + dd], acc[j]);
+            }
+        }
+    }
+    const int64_t i = base + p;
+    float best = neg_inf<float>();
+    int bi = 0x7fffffff;
+#pragma unroll
+    for (int j = 0; j < KPT; ++j) {
+        const int k = g + 4 * j;
+        if (k < K) {
+            float s = acc[j] + P[(size_t)k * stride + D];
+            if (!final_iter) s += gumbel_f32(philox4x32_10((uint32_t)i, (uint32_t)j, iter, TAG_LABEL, key0, key1).v[g]);
+            if (s > best || bi == 0x7fffffff) { best = s; bi = k; }
+        }
+    }
+    bs[g][p] = best;
+    bk[g][p] = bi;
+    __syncthreads();
+    if (g == 0 && i < N) {
+        float b = bs[0][p];
+        int kb = bk[0][p];
+        for (int gg = 1; gg < 4; ++gg) {
+            const int k2 = bk[gg][p];
+            if (k2 == 0x7fffffff) continue;
+            const float s2 = bs[gg][p];
+            if (kb == 0x7fffffff || s2 > b || (s2 == b && k2 < kb)) { b = s2; kb = k2; }
+        }
+        labels[i] = kb;
+    }
+}
+
+// ---- deterministic partition of points by label -----------------------------------------------
+__global__ void __launch_bounds__(kThreads) k_block_hist(const int32_t* __restrict__ labels, int64_t N, int K,
+                                                         int nblk, int32_t* __restrict__ hist) {
+    __shared__ int cnt[kMaxK];
+    for (int k = threadIdx.x; k < K; k += kThreads) cnt[k] = 0;
+    __syncthreads();
+    const int64_t b0 = (int64_t)blockIdx.x * kBlockPts;
+    for (int r = 0; r < kBlockPts / kThreads; ++r) {
+        const int64_t i = b0 + r * kThreads + threadIdx.x;
+        if (i < N) {
+            const int L = labels[i];
+            if ((unsigned)L < (unsigned)K) atomicAdd(&cnt[L], 1);   // out-of-range labels are left out
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < K; k += kThreads) hist[(size_t)k * nblk + blockIdx.x] = cnt[k];
+}
+
+// Exclusive scan of one label's row of block counts (one workgroup per label).
+__global__ void __launch_bounds__(kThreads) k_scan_rows(int32_t* __restrict__ hist, int nblk,
+                                                        int32_t* __restrict__ rowtot) {
+    __shared__ int buf[kThreads];
+    __shared__ int carry;
+    int32_t* row = hist + (size_t)blockIdx.x * nblk;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int b0 = 0; b0 < nblk; b0 += kThreads) {
+        const int b = b0 + threadIdx.x;
+        const int v = b < nblk ? row[b] : 0;
+        buf[threadIdx.x] = v;
+        __syncthreads();
+        for (int off = 1; off < kThreads; off <<= 1) {
+            const int t = threadIdx.x >= off ? buf[threadIdx.x - off] : 0;
+            __syncthreads();
+            buf[threadIdx.x] += t;
+            __syncthreads();
+        }
+        const int incl = buf[threadIdx.x];
+        if (b < nblk) row[b] = carry + incl - v;
+        __syncthreads();
+        if (threadIdx.x == kThreads - 1) carry += incl;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) rowtot[blockIdx.x] = carry;
+}
+
+// Segment starts per label and chunk starts (ceil(n_k / CH) chunks per label).
+__global__ void k_scan_labels(const int32_t* __restrict__ rowtot, int K, int CH, int32_t* __restrict__ label_start,
+                              int32_t* __restrict__ chunk_start) {
+    if (threadIdx.x != 0) return;
+    int s = 0, c = 0;
+    for (int k = 0; k < K; ++k) {
+        label_start[k] = s;
+        chunk_start[k] = c;
+        const int n = rowtot[k];
+        s += n;
+        c += (n + CH - 1) / CH;
+    }
+    label_start[K] = s;
+    chunk_start[K] = c;
+}
+
+// Stable scatter: within a block, points keep index order per label (wave ballot ranks,
+// waves in order, rounds in order).
+__global__ void __launch_bounds__(kThreads) k_scatter(const int32_t* __restrict__ labels, int64_t N, int K, int nblk,
+                                                      const int32_t* __restrict__ hist,
+                                                      const int32_t* __restrict__ label_start,
+                                                      int32_t* __restrict__ perm) {
+    __shared__ int base[kMaxK];
+    __shared__ int wc[4][kMaxK];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int k = threadIdx.x; k < K; k += kThreads) base[k] = label_start[k] + hist[(size_t)k * nblk + blockIdx.x];
+    const int64_t b0 = (int64_t)blockIdx.x * kBlockPts;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int r = 0; r < kBlockPts / kThreads; ++r) {
+        for (int e = threadIdx.x; e < 4 * K; e += kThreads) wc[e / K][e % K] = 0;
+        __syncthreads();
+        const int64_t i = b0 + r * kThreads + threadIdx.x;
+        const int Li = i < N ? labels[i] : -1;
+        const bool valid = (unsigned)Li < (unsigned)K;
+        const int L = valid ? Li : -1;
+        uint64_t todo = __ballot(valid);
+        int rank = 0, cnt = 0;
+        while (todo) {
+            const int src = __ffsll((unsigned long long)todo) - 1;
+            const int Ls = __shfl(L, src);
+            const uint64_t m = __ballot(L == Ls);
+            if (L == Ls) { rank = __popcll(m & lt); cnt = __popcll(m); }
+            todo &= ~m;
+        }
+        if (valid && rank == 0) wc[wv][L] = cnt;
+        __syncthreads();
+        if (valid) {
+            int pos = base[L] + rank;
+            for (int w2 = 0; w2 < wv; ++w2) pos += wc[w2][L];
+            perm[pos] = (int32_t)i;
+        }
+        __syncthreads();
+        for (int k = threadIdx.x; k < K; k += kThreads) base[k] += wc[0][k] + wc[1][k] + wc[2][k] + wc[3][k];
+        __syncthreads();
+    }
+}
+
+// ---- sub-label sweeps over segments ----------------------------------------------------------
+// Multinomial: one wavefront per point, lanes split the vocabulary.
+__global__ void __launch_bounds__(kThreads) k_sub_mult(const int32_t* __restrict__ X, int D, int K, int CH,
+                                                       const int32_t* __restrict__ perm,
+                                                       const int32_t* __restrict__ label_start,
+                                                       const int32_t* __restrict__ chunk_start,
+                                                       const float* __restrict__ SP, int stride, uint32_t key0,
+                                                       uint32_t key1, uint32_t iter, int32_t* __restrict__ subs) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    float* th = reinterpret_cast<float*>(smem);   // [2][stride]
+    int k, beg, end;
+    if (!chunk_of(blockIdx.x, K, chunk_start, label_start, CH, k, beg, end)) return;
+    const float* src = SP + (size_t)(2 * k) * stride;
+    for (int e = threadIdx.x; e < 2 * stride; e += kThreads) th[e] = src[e];
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int p = beg + wv; p < end; p += 4) {
+        const int idx = perm[p];
+        const int32_t* xr = X + (int64_t)idx * D;
+        float a0 = 0.f, a1 = 0.f;
+        for (int d = lane; d < D; d += 64) {
+            const float xv = (float)xr[d];
+            if (xv != 0.f) { a0 = fmaf(xv, th[d], a0); a1 = fmaf(xv, th[stride + d], a1); }
+        }
+        for (int off = 32; off > 0; off >>= 1) { a0 += __shfl_xor(a0, off); a1 += __shfl_xor(a1, off); }
+        if (lane == 0) {
+            const float s0 = a0 + th[D], s1 = a1 + th[stride + D];
+            const uint32_t w = philox4x32_10((uint32_t)idx, 0, iter, TAG_SUB, key0, key1).v[0];
+            subs[idx] = (logit_u32<float>(w) < s0 - s1) ? 0 : 1;
+        }
+    }
+}
+
+// ---- sufficient statistics --------------------------------------------------------------------
+// Multinomial: exact integer sums per vocabulary entry.
+__global__ void __launch_bounds__(kThreads) k_stats_mult(const int32_t* __restrict__ X, int D, int K, int CH,
+                                                         const int32_t* __restrict__ perm,
+                                                         const int32_t* __restrict__ label_start,
+                                                         const int32_t* __restrict__ chunk_start,
+                                                         const int32_t* __restrict__ subs, int E,
+                                                         double* __restrict__ partials) {
+    const int c = blockIdx.x >> 1, s = blockIdx.x & 1;
+    int k, beg, end;
+    if (!chunk_of(c, K, chunk_start, label_start, CH, k, beg, end)) return;
+    double* out = partials + (size_t)blockIdx.x * E;
+    int64_t cnt = 0;
+    for (int d0 = 0; d0 < D; d0 += kThreads) {
+        const int d = d0 + threadIdx.x;
+        int64_t acc = 0;
+        int64_t n = 0;
+        for (int p = beg; p < end; ++p) {
+            const int idx = perm[p];
+            if (subs[idx] != s) continue;
+            ++n;
+            if (d < D) acc += X[(int64_t)idx * D + d];
+        }
+        if (d < D) out[1 + d] = (double)acc;
+        cnt = n;
+    }
+    if (threadIdx.x == 0) out[0] = (double)cnt;
+}
+
+// Fixed-order reduction of the chunk partials of each (cluster, sub-cluster).
+__global__ void __launch_bounds__(kThreads) k_stats_reduce(const double* __restrict__ partials,
+                                                           const int32_t* __restrict__ chunk_start, int K, int E,
+                                                           double* __restrict__ stats) {
+    const int q = blockIdx.x, k = q >> 1, s = q & 1;
+    const int c0 = chunk_start[k], c1 = chunk_start[k + 1];
+    for (int e = threadIdx.x; e < E; e += kThreads) {
+        double v = 0.0;
+        for (int c = c0; c < c1; ++c) v += partials[(size_t)(2 * c + s) * E + e];
+        stats[(size_t)q * E + e] = v;
+    }
+}
+
+// ---- relabel -----------------------------------------------------------------------------------
+__global__ void __launch_bounds__(kThreads) k_relabel(int32_t* __restrict__ labels, int32_t* __restrict__ subs,
+                                                      int64_t N, const int32_t* __restrict__ table, int K,
+                                                      uint32_t key0, uint32_t key1, uint32_t iter) {
+    const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= N) return;
+    const int k = labels[i], s = subs[i];
+    if ((unsigned)k >= (unsigned)K || (unsigned)s > 1u) return;
+    const int act = table[2 * K + k];
+    labels[i] = table[2 * k + s];
+    int ns = s;
+    if (act == 1 || act == 2) {
+        ns = (int)(philox4x32_10((uint32_t)i, 0, iter, act == 1 ? TAG_RESET : TAG_SPLITSUB, key0, key1).v[0] & 1u);
+    } else if (act == 3) {
+        ns = 0;
+    } else if (act == 4) {
+        ns = 1;
+    }
+    subs[i] = ns;
+}
+
+// ============================================================================ context ======
+
+struct dpmm_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int64_t N = 0;
+    int D = 0, DP = 0, dtype = 0, prior = 0, precision = DPMM_PREC_FP32;
+    uint64_t seed = 0;
+    void* dX = nullptr;
+    bool owns_X = false;
+    int32_t* dLabels = nullptr;
+    int32_t* dSubs = nullptr;
+    // parameters (compute type: double for FP64 precision, float otherwise)
+    int K = 0, Kcap = 0, stride = 0;
+    bool params_f64 = false;
+    void* dP = nullptr;
+    void* dSP = nullptr;
+    std::vector<unsigned char> hP, hSP;
+    // partition
+    int nblk = 0, CH = 0, Kpart = 0, Kpartition = 0;
+    bool partition_valid = false;
+    int32_t* dHist = nullptr;
+    size_t histCap = 0;
+    int32_t* dRowtot = nullptr;
+    int32_t* dLabelStart = nullptr;
+    int32_t* dChunkStart = nullptr;
+    int32_t* dPerm = nullptr;
+    int32_t* dTable = nullptr;
+    size_t tableCap = 0;
+    // statistics
+    double* dPartials = nullptr;
+    size_t partialsCap = 0;
+    double* dStats = nullptr;
+    size_t statsCap = 0;
+    double* hStats = nullptr;
+    size_t hStatsCap = 0;
+    hipEvent_t ev[6] = {};
+    float ms_label = 0, ms_sub = 0, ms_stats = 0;
+    bool ev_label = false, ev_sub = false, ev_stats = false;
+    uint32_t key0() const { return (uint32_t)seed; }
+    uint32_t key1() const { return (uint32_t)(seed >> 32); }
+    size_t statEntries() const { return prior == DPMM_PRIOR_GAUSSIAN ? 1 + DP + (size_t)DP * DP : 1 + (size_t)D; }
+};
+
+namespace {
+
+int pick_dp(int D) {
+    static const int dps[] = {4, 8, 16, 32, 48, 64, 96, 128};
+    for (int dp : dps)
+        if (D <= dp) return dp;
+    return -1;
+}
+
+template <typename T>
+int grow(T*& p, size_t& cap, size_t n) {
+    if (n <= cap && p) return DPMM_OK;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    if (hipMalloc((void**)&p, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess)
+        return fail(DPMM_ENOMEM, "hipMalloc failed");
+    cap = n;
+    return DPMM_OK;
+}
+
+int ensure_partition(dpmm_ctx* c, int K) {
+    c->nblk = (int)((c->N + kBlockPts - 1) / kBlockPts);
+    int rc = grow(c->dHist, c->histCap, (size_t)K * c->nblk);
+    if (rc) return rc;
+    if (c->Kpart < K || !c->dLabelStart) {
+        if (c->dRowtot) (void)hipFree(c->dRowtot);
+        if (c->dLabelStart) (void)hipFree(c->dLabelStart);
+        if (c->dChunkStart) (void)hipFree(c->dChunkStart);
+        const int cap = std::max(K, 64) + 1;
+        if (hipMalloc((void**)&c->dRowtot, cap * sizeof(int32_t)) != hipSuccess ||
+            hipMalloc((void**)&c->dLabelStart, cap * sizeof(int32_t)) != hipSuccess ||
+            hipMalloc((void**)&c->dChunkStart, cap * sizeof(int32_t)) != hipSuccess)
+            return fail(DPMM_ENOMEM, "hipMalloc failed");
+        c->Kpart = cap - 1;
+    }
+    return DPMM_OK;
+}
+
+int chunk_size(const dpmm_ctx* c) {
+    const int64_t target = c->DP > 64 ? 1024 : 2048;
+    int64_t ch = (c->N + target - 1) / target;
+    ch = std::max<int64_t>(ch, 256);
+    ch = (ch + 63) / 64 * 64;
+    return (int)ch;
+}
+
+int chunks_upper(const dpmm_ctx* c, int K) { return (int)((c->N + c->CH - 1) / c->CH) + K; }
+
+int build_partition(dpmm_ctx* c, int K) {
+    int rc = ensure_partition(c, K);
+    if (rc) return rc;
+    c->CH = chunk_size(c);
+    hipLaunchKernelGGL(k_block_hist, dim3(c->nblk), dim3(kThreads), 0, c->stream, c->dLabels, c->N, K, c->nblk,
+                       c->dHist);
+    hipLaunchKernelGGL(k_scan_rows, dim3(K), dim3(kThreads), 0, c->stream, c->dHist, c->nblk, c->dRowtot);
+    hipLaunchKernelGGL(k_scan_labels, dim3(1), dim3(64), 0, c->stream, c->dRowtot, K, c->CH, c->dLabelStart,
+                       c->dChunkStart);
+    hipLaunchKernelGGL(k_scatter, dim3(c->nblk), dim3(kThreads), 0, c->stream, c->dLabels, c->N, K, c->nblk,
+                       c->dHist, c->dLabelStart, c->dPerm);
+    HIPCHK(hipGetLastError());
+    c->partition_valid = true;
+    c->Kpartition = K;
+    return DPMM_OK;
+}
+
+int create_common(const void* X, int64_t N, int D, int dtype, int prior, uint64_t seed, bool device_ptr,
+                  dpmm_ctx** out) {
+    if (!out) return fail(DPMM_EINVAL, "out is null");
+    *out = nullptr;
+    if (!X || N <= 0 || N >= (int64_t)INT32_MAX || D <= 0) return fail(DPMM_EINVAL, "bad X / N / D");
+    if (prior != DPMM_PRIOR_GAUSSIAN && prior != DPMM_PRIOR_MULTINOMIAL) return fail(DPMM_EINVAL, "bad prior");
+    if (prior == DPMM_PRIOR_GAUSSIAN && dtype != DPMM_DTYPE_F32 && dtype != DPMM_DTYPE_F64)
+        return fail(DPMM_EINVAL, "Gaussian data must be f32 or f64");
+    if (prior == DPMM_PRIOR_MULTINOMIAL && dtype != DPMM_DTYPE_I32)
+        return fail(DPMM_EINVAL, "multinomial data must be int32 counts");
+    if (prior == DPMM_PRIOR_GAUSSIAN && pick_dp(D) < 0) return fail(DPMM_EINVAL, "Gaussian D > 128 unsupported");
+    if (prior == DPMM_PRIOR_GAUSSIAN && dtype == DPMM_DTYPE_F64 && D > 64)
+        return fail(DPMM_EINVAL, "f64 Gaussian data supports D <= 64");
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (ndev <= 0) return fail(DPMM_EHIP, "no HIP device");
+    dpmm_ctx* c = new dpmm_ctx();
+    c->N = N; c->D = D; c->dtype = dtype; c->prior = prior; c->seed = seed;
+    c->DP = prior == DPMM_PRIOR_GAUSSIAN ? pick_dp(D) : D;
+    c->precision = dtype == DPMM_DTYPE_F64 ? DPMM_PREC_FP64 : DPMM_PREC_FP32;
+    (void)hipGetDevice(&c->device);
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return fail(DPMM_EHIP, "stream create failed");
+    }
+    const size_t esz = dtype == DPMM_DTYPE_F64 ? 8 : 4;
+    const size_t bytes = (size_t)N * D * esz;
+    if (device_ptr) {
+        c->dX = const_cast<void*>(X);
+    } else {
+        if (hipMalloc(&c->dX, bytes) != hipSuccess) { dpmm_destroy(c); return fail(DPMM_ENOMEM, "X alloc"); }
+        c->owns_X = true;
+        if (hipMemcpy(c->dX, X, bytes, hipMemcpyHostToDevice) != hipSuccess) {
+            dpmm_destroy(c);
+            return fail(DPMM_EHIP, "X copy");
+        }
+    }
+    if (hipMalloc((void**)&c->dLabels, N * sizeof(int32_t)) != hipSuccess ||
+        hipMalloc((void**)&c->dSubs, N * sizeof(int32_t)) != hipSuccess ||
+        hipMalloc((void**)&c->dPerm, N * sizeof(int32_t)) != hipSuccess) {
+        dpmm_destroy(c);
+        return fail(DPMM_ENOMEM, "label alloc");
+    }
+    (void)hipMemsetAsync(c->dLabels, 0, N * sizeof(int32_t), c->stream);
+    (void)hipMemsetAsync(c->dSubs, 0, N * sizeof(int32_t), c->stream);
+    for (auto& e : c->ev) (void)hipEventCreate(&e);
+    *out = c;
+    return DPMM_OK;
+}
+
+// ---- parameter packing -----------------------------------------------------------------------
+bool chol_lower(int D, const double* A, std::vector<double>& L) {
+    L.assign(A, A + (size_t)D * D);
+    for (int j = 0; j < D; ++j) {
+        double s = L[(size_t)j * D + j];
+        for (int k = 0; k < j; ++k) s -= L[(size_t)j * D + k] * L[(size_t)j * D + k];
+        if (!(s > 0.0) || !std::isfinite(s)) return false;
+        const double d = std::sqrt(s);
+        L[(size_t)j * D + j] = d;
+        for (int i = j + 1; i < D; ++i) {
+            double t = L[(size_t)i * D + j];
+            for (int k = 0; k < j; ++k) t -= L[(size_t)i * D + k] * L[(size_t)j * D + k];
+            L[(size_t)i * D + j] = t / d;
+        }
+    }
+    return true;
+}
+
+template <typename T>
+void pack_gauss_slot(int D, int DP, const double* mu, const double* R, double c, T* dst) {
+    const int stride = gauss_slot_stride(DP);
+    std::fill(dst, dst + stride, T(0));
+    for (int j = 0; j < D; ++j) {
+        const int l0 = j & ~3;
+        T* row = dst + tri_row_offset(DP, j) - l0;
+        for (int l = j; l < D; ++l) row[l] = (T)R[(size_t)l * D + j];   // W = R^T
+    }
+    for (int l = 0; l < D; ++l) dst[gauss_mu_offset(DP) + l] = (T)mu[l];
+    dst[gauss_c_offset(DP)] = (T)c;
+}
+
+int ensure_param_buffers(dpmm_ctx* c, int K, size_t slot_bytes) {
+    if (K > kMaxK) return fail(DPMM_EINVAL, "K exceeds 2048");
+    const size_t need = (size_t)K * slot_bytes;
+    if ((size_t)c->Kcap * c->stride * (c->params_f64 ? 8 : 4) < need || !c->dP) {
+        if (c->dP) (void)hipFree(c->dP);
+        if (c->dSP) (void)hipFree(c->dSP);
+        c->dP = c->dSP = nullptr;
+        const int cap = std::max(K, 64);
+        if (hipMalloc(&c->dP, (size_t)cap * slot_bytes) != hipSuccess ||
+            hipMalloc(&c->dSP, (size_t)2 * cap * slot_bytes) != hipSuccess)
+            return fail(DPMM_ENOMEM, "param alloc");
+        c->Kcap = cap;
+    }
+    return DPMM_OK;
+}
+
+int set_gauss(dpmm_ctx* c, int K, const double* mu, const double* mat, const double* logdet, const double* weights,
+              const double* smu, const double* smat, const double* slogdet, const double* lrw, bool is_factor) {
+    if (!c) return fail(DPMM_EINVAL, "null ctx");
+    if (c->prior != DPMM_PRIOR_GAUSSIAN) return fail(DPMM_EINVAL, "context is not Gaussian");
+    if (K <= 0 || !mu || !mat || !logdet || !weights || !smu || !smat || !slogdet || !lrw)
+        return fail(DPMM_EINVAL, "bad Gaussian params");
+    const int D = c->D, DP = c->DP;
+    const bool f64 = c->precision == DPMM_PREC_FP64;
+    const int stride = gauss_slot_stride(DP);
+    const size_t esz = f64 ? 8 : 4;
+    c->params_f64 = f64;
+    c->stride = stride;
+    int rc = ensure_param_buffers(c, K, (size_t)stride * esz);
+    if (rc) return rc;
+    c->hP.resize((size_t)K * stride * esz);
+    c->hSP.resize((size_t)2 * K * stride * esz);
+    std::vector<double> R;
+    auto one = [&](const double* m, const double* A, double ld, double logw, unsigned char* dst) -> bool {
+        if (is_factor) R.assign(A, A + (size_t)D * D);
+        else if (!chol_lower(D, A, R)) return false;
+        // Score constant without the common -D^2 log(2 pi) / 2 of cudaKernel_gaussian.cu:75,108:
+        // label and sub-label draws only compare scores of the same point.
+        const double cst = -0.5 * ld + logw;
+        if (f64) pack_gauss_slot<double>(D, DP, m, R.data(), cst, reinterpret_cast<double*>(dst));
+        else pack_gauss_slot<float>(D, DP, m, R.data(), cst, reinterpret_cast<float*>(dst));
+        return true;
+    };
+    for (int k = 0; k < K; ++k) {
+        if (!std::isfinite(logdet[k])) return fail(DPMM_EINVAL, "non-finite logdet");
+        if (!one(mu + (size_t)k * D, mat + (size_t)k * D * D, logdet[k], std::log(weights[k]),
+                 c->hP.data() + (size_t)k * stride * esz))
+            return fail(DPMM_ENOTSPD, "cluster precision matrix not SPD");
+    }
+    for (int q = 0; q < 2 * K; ++q) {
+        if (!one(smu + (size_t)q * D, smat + (size_t)q * D * D, slogdet[q], std::log(lrw[q]),
+                 c->hSP.data() + (size_t)q * stride * esz))
+            return fail(DPMM_ENOTSPD, "sub-cluster precision matrix not SPD");
+    }
+    HIPCHK(hipMemcpyAsync(c->dP, c->hP.data(), c->hP.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->dSP, c->hSP.data(), c->hSP.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->K = K;
+    return DPMM_OK;
+}
+
+template <typename T, int DP>
+void run_label_gauss(dpmm_ctx* c, uint32_t iter, int fin) {
+    const int grid = (int)((c->N + kThreads - 1) / kThreads);
+    launch_label_gauss<T, DP>(c->stream, grid, reinterpret_cast<const T*>(c->dX), c->N, c->D, c->K,
+                              reinterpret_cast<const T*>(c->dP), c->key0(), c->key1(), iter, fin, c->dLabels);
+}
+
+template <typename T, int DP>
+void run_sub_gauss(dpmm_ctx* c, uint32_t iter) {
+    launch_sub_gauss<T, DP>(c->stream, chunks_upper(c, c->K), reinterpret_cast<const T*>(c->dX), c->D, c->K, c->CH,
+                            c->dPerm, c->dLabelStart, c->dChunkStart, reinterpret_cast<const T*>(c->dSP), c->key0(),
+                            c->key1(), iter, c->dSubs);
+}
+
+template <typename TX, int DP>
+void run_stats_gauss(dpmm_ctx* c) {
+    const int K = c->Kpartition;
+    launch_stats_gauss<TX, DP>(c->stream, 2 * chunks_upper(c, K), reinterpret_cast<const TX*>(c->dX), c->D, K, c->CH,
+                               c->dPerm, c->dLabelStart, c->dChunkStart, c->dSubs, c->dPartials);
+}
+
+#define DPMM_DISPATCH_DP(DPV, FN, ...)            \
+    switch (DPV) {                                \
+        case 4: FN<4>(__VA_ARGS__); break;        \
+        case 8: FN<8>(__VA_ARGS__); break;        \
+        case 16: FN<16>(__VA_ARGS__); break;      \
+        case 32: FN<32>(__VA_ARGS__); break;      \
+        case 48: FN<48>(__VA_ARGS__); break;      \
+        case 64: FN<64>(__VA_ARGS__); break;      \
+        case 96: FN<96>(__VA_ARGS__); break;      \
+        default: FN<128>(__VA_ARGS__); break;     \
+    }
+
+#define DPMM_DISPATCH_DP64(DPV, FN, ...)          \
+    switch (DPV) {                                \
+        case 4: FN<4>(__VA_ARGS__); break;        \
+        case 8: FN<8>(__VA_ARGS__); break;        \
+        case 16: FN<16>(__VA_ARGS__); break;      \
+        case 32: FN<32>(__VA_ARGS__); break;      \
+        case 48: FN<48>(__VA_ARGS__); break;      \
+        default: FN<64>(__VA_ARGS__); break;      \
+    }
+
+template <int DP> void lg_f32(dpmm_ctx* c, uint32_t it, int f) { run_label_gauss<float, DP>(c, it, f); }
+template <int DP> void lg_f64(dpmm_ctx* c, uint32_t it, int f) { run_label_gauss<double, DP>(c, it, f); }
+template <int DP> void sg_f32(dpmm_ctx* c, uint32_t it) { run_sub_gauss<float, DP>(c, it); }
+template <int DP> void sg_f64(dpmm_ctx* c, uint32_t it) { run_sub_gauss<double, DP>(c, it); }
+template <int DP> void st_f32(dpmm_ctx* c) { run_stats_gauss<float, DP>(c); }
+template <int DP> void st_f64(dpmm_ctx* c) { run_stats_gauss<double, DP>(c); }
+
+template <int KPT>
+void launch_label_mult_k(dpmm_ctx* c, uint32_t iter, int fin) {
+    const int grid = (int)((c->N + 63) / 64);
+    const size_t smem = (size_t)(32 * 65 + c->K * 33) * sizeof(float);
+    hipLaunchKernelGGL((k_label_mult<KPT>), dim3(grid), dim3(kThreads), smem, c->stream,
+                       reinterpret_cast<const int32_t*>(c->dX), c->N, c->D, c->K,
+                       reinterpret_cast<const float*>(c->dP), c->stride, c->key0(), c->key1(), iter, fin,
+                       c->dLabels);
+}
+
+int launch_label_mult(dpmm_ctx* c, uint32_t iter, int fin) {
+    if ((size_t)(32 * 65 + c->K * 33) * sizeof(float) > 65536) return fail(DPMM_EINVAL, "multinomial K > 433");
+    const int kpt = (c->K + 3) / 4;
+    if (kpt <= 4) launch_label_mult_k<4>(c, iter, fin);
+    else if (kpt <= 8) launch_label_mult_k<8>(c, iter, fin);
+    else if (kpt <= 16) launch_label_mult_k<16>(c, iter, fin);
+    else if (kpt <= 32) launch_label_mult_k<32>(c, iter, fin);
+    else if (kpt <= 64) launch_label_mult_k<64>(c, iter, fin);
+    else if (kpt <= 128) launch_label_mult_k<128>(c, iter, fin);
+    else return fail(DPMM_EINVAL, "multinomial K > 512 unsupported");
+    return DPMM_OK;
+}
+
+}  // namespace
+
+// ============================================================================ C ABI ========
+extern "C" {
+
+const char* dpmm_last_error(void) { return g_err.c_str(); }
+
+int dpmm_device_count(int* count) {
+    if (!count) return fail(DPMM_EINVAL, "null");
+    HIPCHK(hipGetDeviceCount(count));
+    return DPMM_OK;
+}
+
+int dpmm_create(const void* X, int64_t N, int D, int dtype, int prior, uint64_t seed, dpmm_ctx** out) {
+    return create_common(X, N, D, dtype, prior, seed, false, out);
+}
+
+int dpmm_create_device(const void* dX, int64_t N, int D, int dtype, int prior, uint64_t seed, dpmm_ctx** out) {
+    return create_common(dX, N, D, dtype, prior, seed, true, out);
+}
+
+int dpmm_destroy(dpmm_ctx* c) {
+    if (!c) return DPMM_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->owns_X && c->dX) (void)hipFree(c->dX);
+    for (void* p : {(void*)c->dLabels, (void*)c->dSubs, (void*)c->dPerm, c->dP, c->dSP, (void*)c->dHist,
+                    (void*)c->dRowtot, (void*)c->dLabelStart, (void*)c->dChunkStart, (void*)c->dTable,
+                    (void*)c->dPartials, (void*)c->dStats})
+        if (p) (void)hipFree(p);
+    if (c->hStats) (void)hipHostFree(c->hStats);
+    for (auto& e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return DPMM_OK;
+}
+
+int dpmm_set_precision(dpmm_ctx* c, int precision) {
+    if (!c) return fail(DPMM_EINVAL, "null ctx");
+    if (precision == DPMM_PREC_FP64 && c->dtype != DPMM_DTYPE_F64)
+        return fail(DPMM_EINVAL, "FP64 sweep needs f64 data");
+    if (precision != DPMM_PREC_FP64 && c->dtype == DPMM_DTYPE_F64)
+        return fail(DPMM_EINVAL, "f64 data sweeps in FP64");
+    if (precision == DPMM_PREC_MIXED) return fail(DPMM_EINVAL, "mixed precision not built in this library");
+    c->precision = precision;
+    c->K = 0;
+    return DPMM_OK;
+}
+
+int dpmm_sample_initial_labels(dpmm_ctx* c, int K0) {
+    if (!c || K0 <= 0) return fail(DPMM_EINVAL, "bad args");
+    const int grid = (int)((c->N + kThreads - 1) / kThreads);
+    hipLaunchKernelGGL(k_init_labels, dim3(grid), dim3(kThreads), 0, c->stream, c->dLabels, c->dSubs, c->N, K0,
+                       c->key0(), c->key1());
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->partition_valid = false;
+    return DPMM_OK;
+}
+
+int dpmm_set_labels(dpmm_ctx* c, const int32_t* labels, const int32_t* subs) {
+    if (!c) return fail(DPMM_EINVAL, "null ctx");
+    if (labels) HIPCHK(hipMemcpy(c->dLabels, labels, c->N * sizeof(int32_t), hipMemcpyHostToDevice));
+    if (subs) HIPCHK(hipMemcpy(c->dSubs, subs, c->N * sizeof(int32_t), hipMemcpyHostToDevice));
+    c->partition_valid = false;
+    return DPMM_OK;
+}
+
+int dpmm_get_labels(dpmm_ctx* c, int32_t* labels, int32_t* subs) {
+    if (!c) return fail(DPMM_EINVAL, "null ctx");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (labels) HIPCHK(hipMemcpy(labels, c->dLabels, c->N * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (subs) HIPCHK(hipMemcpy(subs, c->dSubs, c->N * sizeof(int32_t), hipMemcpyDeviceToHost));
+    return DPMM_OK;
+}
+
+int dpmm_set_gaussian_params(dpmm_ctx* c, int K, const double* mu, const double* inv_sigma,
+                             const double* logdet_sigma, const double* weights, const double* sub_mu,
+                             const double* sub_inv_sigma, const double* sub_logdet_sigma, const double* lr_weights) {
+    return set_gauss(c, K, mu, inv_sigma, logdet_sigma, weights, sub_mu, sub_inv_sigma, sub_logdet_sigma,
+                     lr_weights, false);
+}
+
+int dpmm_set_gaussian_factors(dpmm_ctx* c, int K, const double* mu, const double* chol, const double* logdet_sigma,
+                              const double* weights, const double* sub_mu, const double* sub_chol,
+                              const double* sub_logdet_sigma, const double* lr_weights) {
+    return set_gauss(c, K, mu, chol, logdet_sigma, weights, sub_mu, sub_chol, sub_logdet_sigma, lr_weights, true);
+}
+
+int dpmm_set_multinomial_params(dpmm_ctx* c, int K, const double* logtheta, const double* weights,
+                                const double* sub_logtheta, const double* lr_weights, int use_weight) {
+    if (!c) return fail(DPMM_EINVAL, "null ctx");
+    if (c->prior != DPMM_PRIOR_MULTINOMIAL) return fail(DPMM_EINVAL, "context is not multinomial");
+    if (K <= 0 || !logtheta || !sub_logtheta || !lr_weights || (use_weight && !weights))
+        return fail(DPMM_EINVAL, "bad multinomial params");
+    const int D = c->D;
+    const int stride = (D + 1 + 3) & ~3;
+    c->params_f64 = false;
+    c->stride = stride;
+    int rc = ensure_param_buffers(c, K, (size_t)stride * 4);
+    if (rc) return rc;
+    c->hP.assign((size_t)K * stride * 4, 0);
+    c->hSP.assign((size_t)2 * K * stride * 4, 0);
+    float* P = reinterpret_cast<float*>(c->hP.data());
+    float* SP = reinterpret_cast<float*>(c->hSP.data());
+    for (int k = 0; k < K; ++k) {
+        for (int d = 0; d < D; ++d) P[(size_t)k * stride + d] = (float)logtheta[(size_t)k * D + d];
+        P[(size_t)k * stride + D] = use_weight ? (float)std::log(weights[k]) : 0.f;
+    }
+    for (int q = 0; q < 2 * K; ++q) {
+        for (int d = 0; d < D; ++d) SP[(size_t)q * stride + d] = (float)sub_logtheta[(size_t)q * D + d];
+        SP[(size_t)q * stride + D] = (float)std::log(lr_weights[q]);
+    }
+    HIPCHK(hipMemcpyAsync(c->dP, c->hP.data(), c->hP.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->dSP, c->hSP.data(), c->hSP.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->K = K;
+    return DPMM_OK;
+}
+
+int dpmm_sample_cluster_labels(dpmm_ctx* c, uint32_t iter, int fin) {
+    if (!c) return fail(DPMM_EINVAL, "null ctx");
+    if (c->K <= 0) return fail(DPMM_ESTATE, "parameters not set");
+    (void)hipEventRecord(c->ev[0], c->stream);
+    if (c->prior == DPMM_PRIOR_GAUSSIAN) {
+        if (c->params_f64) { DPMM_DISPATCH_DP64(c->DP, lg_f64, c, iter, fin); }
+        else { DPMM_DISPATCH_DP(c->DP, lg_f32, c, iter, fin); }
+    } else {
+        int rc = launch_label_mult(c, iter, fin);
+        if (rc) return rc;
+    }
+    (void)hipEventRecord(c->ev[1], c->stream);
+    HIPCHK(hipGetLastError());
+    c->ev_label = true;
+    c->partition_valid = false;
+    return DPMM_OK;
+}
+
+int dpmm_sample_sub_labels(dpmm_ctx* c, uint32_t iter) {
+    if (!c) return fail(DPMM_EINVAL, "null ctx");
+    if (c->K <= 0) return fail(DPMM_ESTATE, "parameters not set");
+    if (!c->partition_valid || c->Kpartition != c->K) {
+        int rc = build_partition(c, c->K);
+        if (rc) return rc;
+    }
+    (void)hipEventRecord(c->ev[2], c->stream);
+    if (c->prior == DPMM_PRIOR_GAUSSIAN) {
+        if (c->params_f64) { DPMM_DISPATCH_DP64(c->DP, sg_f64, c, iter); }
+        else { DPMM_DISPATCH_DP(c->DP, sg_f32, c, iter); }
+    } else {
+        hipLaunchKernelGGL(k_sub_mult, dim3(chunks_upper(c, c->K)), dim3(kThreads), (size_t)2 * c->stride * 4,
+                           c->stream, reinterpret_cast<const int32_t*>(c->dX), c->D, c->K, c->CH, c->dPerm,
+                           c->dLabelStart, c->dChunkStart, reinterpret_cast<const float*>(c->dSP), c->stride,
+                           c->key0(), c->key1(), iter, c->dSubs);
+    }
+    (void)hipEventRecord(c->ev[3], c->stream);
+    HIPCHK(hipGetLastError());
+    c->ev_sub = true;
+    return DPMM_OK;
+}
+
+int dpmm_sufficient_statistics(dpmm_ctx* c, int K, int64_t* Nsub, double* sum, double* S) {
+    if (!c || !Nsub || !sum) return fail(DPMM_EINVAL, "bad args");
+    if (K <= 0 || K > kMaxK) return fail(DPMM_EINVAL, "bad K");
+    if (!c->partition_valid || c->Kpartition != K) {
+        int rc = build_partition(c, K);
+        if (rc) return rc;
+    }
+    const size_t E = c->statEntries();
+    const int nch = chunks_upper(c, K);
+    int rc = grow(c->dPartials, c->partialsCap, (size_t)2 * nch * E);
+    if (rc) return rc;
+    rc = grow(c->dStats, c->statsCap, (size_t)2 * K * E);
+    if (rc) return rc;
+    (void)hipEventRecord(c->ev[4], c->stream);
+    if (c->prior == DPMM_PRIOR_GAUSSIAN) {
+        if (c->dtype == DPMM_DTYPE_F64) { DPMM_DISPATCH_DP64(c->DP, st_f64, c); }
+        else { DPMM_DISPATCH_DP(c->DP, st_f32, c); }
+    } else {
+        hipLaunchKernelGGL(k_stats_mult, dim3(2 * nch), dim3(kThreads), 0, c->stream,
+                           reinterpret_cast<const int32_t*>(c->dX), c->D, K, c->CH, c->dPerm, c->dLabelStart,
+                           c->dChunkStart, c->dSubs, (int)E, c->dPartials);
+    }
+    hipLaunchKernelGGL(k_stats_reduce, dim3(2 * K), dim3(kThreads), 0, c->stream, c->dPartials, c->dChunkStart, K,
+                       (int)E, c->dStats);
+    (void)hipEventRecord(c->ev[5], c->stream);
+    HIPCHK(hipGetLastError());
+    c->ev_stats = true;
+    const size_t n = (size_t)2 * K * E;
+    if (c->hStatsCap < n) {
+        if (c->hStats) (void)hipHostFree(c->hStats);
+        c->hStats = nullptr;
+        if (hipHostMalloc((void**)&c->hStats, n * sizeof(double)) != hipSuccess) return fail(DPMM_ENOMEM, "pinned");
+        c->hStatsCap = n;
+    }
+    HIPCHK(hipMemcpyAsync(c->hStats, c->dStats, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const int D = c->D, DP = c->DP;
+    for (int q = 0; q < 2 * K; ++q) {
+        const double* src = c->hStats + (size_t)q * E;
+        Nsub[q] = (int64_t)src[0];
+        for (int d = 0; d < D; ++d) sum[(size_t)q * D + d] = src[1 + d];
+        if (S && c->prior == DPMM_PRIOR_GAUSSIAN)
+            for (int a = 0; a < D; ++a)
+                for (int b = 0; b < D; ++b) S[((size_t)q * D + a) * D + b] = src[1 + DP + (size_t)a * DP + b];
+    }
+    return DPMM_OK;
+}
+
+int dpmm_relabel(dpmm_ctx* c, uint32_t iter, int K_old, const int32_t* new_label, const int32_t* sub_action) {
+    if (!c || !new_label || !sub_action || K_old <= 0 || K_old > kMaxK) return fail(DPMM_EINVAL, "bad args");
+    std::vector<int32_t> table(3 * (size_t)K_old);
+    bool labels_change = false;
+    for (int q = 0; q < 2 * K_old; ++q) {
+        table[q] = new_label[q];
+        if (new_label[q] != q / 2) labels_change = true;
+    }
+    for (int k = 0; k < K_old; ++k) table[2 * K_old + k] = sub_action[k];
+    int rc = grow(c->dTable, c->tableCap, table.size());
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(c->dTable, table.data(), table.size() * 4, hipMemcpyHostToDevice, c->stream));
+    const int grid = (int)((c->N + kThreads - 1) / kThreads);
+    hipLaunchKernelGGL(k_relabel, dim3(grid), dim3(kThreads), 0, c->stream, c->dLabels, c->dSubs, c->N,
+                       c->dTable, K_old, c->key0(), c->key1(), iter);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (labels_change) c->partition_valid = false;
+    return DPMM_OK;
+}
+
+int dpmm_last_kernel_ms(dpmm_ctx* c, float* a, float* b, float* s) {
+    if (!c) return fail(DPMM_EINVAL, "null ctx");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (a) { *a = 0; if (c->ev_label) (void)hipEventElapsedTime(a, c->ev[0], c->ev[1]); }
+    if (b) { *b = 0; if (c->ev_sub) (void)hipEventElapsedTime(b, c->ev[2], c->ev[3]); }
+    if (s) { *s = 0; if (c->ev_stats) (void)hipEventElapsedTime(s, c->ev[4], c->ev[5]); }
+    return DPMM_OK;
+}
+
+int dpmm_synchronize(dpmm_ctx* c) {
+    if (!c) return fail(DPMM_EINVAL, "null ctx");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return DPMM_OK;
+}
+
+}  // extern "C"
+
+// Internal accessors for the host driver (same library).
+namespace dpmm {
+int ctx_dims(const dpmm_ctx* c, int64_t* N, int* D, int* prior, int* precision) {
+    if (!c) return DPMM_EINVAL;
+    *N = c->N; *D = c->D; *prior = c->prior; *precision = c->precision;
+    return DPMM_OK;
+}
+}  // namespace dpmm

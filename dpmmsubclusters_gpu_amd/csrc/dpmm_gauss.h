@@ -1,0 +1,218 @@
+// Templated Gaussian kernels (label sweep, sub-label sweep, FP64 statistics) and their
+// launchers; instantiated per padded dimension DP in inst_gauss_*.hip so the heavily unrolled
+// bodies compile in parallel.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "dpmm_kernels.h"
+
+namespace dpmm {
+
+__device__ __forceinline__ bool chunk_of(int c, int K, const int32_t* __restrict__ chunk_start,
+                                         const int32_t* __restrict__ label_start, int CH, int& k, int& beg, int& end) {
+    if (c >= chunk_start[K]) return false;
+    int lo = 0, hi = K - 1;   // last k with chunk_start[k] <= c
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (chunk_start[mid] <= c) lo = mid; else hi = mid - 1;
+    }
+    k = lo;
+    beg = label_start[k] + (c - chunk_start[k]) * CH;
+    end = min(beg + CH, label_start[k + 1]);
+    return true;
+}
+
+// Gaussian label sweep, one point per thread, point kept in registers; cluster parameters
+// staged through LDS one cluster at a time.
+template <typename T, int DP>
+__global__ void __launch_bounds__(kThreads) k_label_gauss(const T* __restrict__ X, int64_t N, int D, int K,
+                                                          const T* __restrict__ P, uint32_t key0, uint32_t key1,
+                                                          uint32_t iter, int final_iter, int32_t* __restrict__ labels) {
+    constexpr int stride = gauss_slot_stride(DP);
+    __shared__ __align__(16) T ps[stride];
+    const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    T x[DP];
+#pragma unroll
+    for (int l = 0; l < DP; ++l) x[l] = (i < N && l < D) ? X[i * D + l] : T(0);
+    T best = neg_inf<T>();
+    int bi = 0;
+    Ctr4 w = {};
+    for (int k = 0; k < K; ++k) {
+        __syncthreads();
+        const T* src = P + (size_t)k * stride;
+        for (int e = threadIdx.x; e < stride; e += kThreads) ps[e] = src[e];
+        __syncthreads();
+        const T q = whitened_sq<T, DP>(ps, x);
+        T s = ps[gauss_c_offset(DP)] - T(0.5) * q;
+        if (!final_iter) {
+            if ((k & 3) == 0) w = philox4x32_10((uint32_t)i, (uint32_t)(k >> 2), iter, TAG_LABEL, key0, key1);
+            s += gumbel<T>(sel4(w, k & 3));
+        }
+        if (s > best || k == 0) { best = s; bi = k; }
+    }
+    if (i < N) labels[i] = bi;
+}
+
+template <typename T, int DP>
+__global__ void __launch_bounds__(kThreads) k_sub_gauss(const T* __restrict__ X, int D, int K, int CH,
+                                                        const int32_t* __restrict__ perm,
+                                                        const int32_t* __restrict__ label_start,
+                                                        const int32_t* __restrict__ chunk_start,
+                                                        const T* __restrict__ SP, uint32_t key0, uint32_t key1,
+                                                        uint32_t iter, int32_t* __restrict__ subs) {
+    constexpr int stride = gauss_slot_stride(DP);
+    __shared__ __align__(16) T ps[2 * stride];
+    int k, beg, end;
+    if (!chunk_of(blockIdx.x, K, chunk_start, label_start, CH, k, beg, end)) return;
+    const T* src = SP + (size_t)(2 * k) * stride;
+    for (int e = threadIdx.x; e < 2 * stride; e += kThreads) ps[e] = src[e];
+    __syncthreads();
+    for (int p = beg + threadIdx.x; p < end; p += kThreads) {
+        const int idx = perm[p];
+        T x[DP];
+#pragma unroll
+        for (int l = 0; l < DP; ++l) x[l] = l < D ? X[(int64_t)idx * D + l] : T(0);
+        const T s0 = ps[gauss_c_offset(DP)] - T(0.5) * whitened_sq<T, DP>(ps, x);
+        const T s1 = ps[stride + gauss_c_offset(DP)] - T(0.5) * whitened_sq<T, DP>(ps + stride, x);
+        const uint32_t w = philox4x32_10((uint32_t)idx, 0, iter, TAG_SUB, key0, key1).v[0];
+        subs[idx] = (logit_u32<T>(w) < s0 - s1) ? 0 : 1;
+    }
+}
+
+// Partial (count, sum, scatter) of the points of sub-cluster s in one chunk, FP64.  Threads own
+// 4x4 blocks of the upper block triangle; points staged in LDS in batches, compacted by ballot.
+template <typename TX, int DP>
+__global__ void __launch_bounds__(kThreads) k_stats_gauss(const TX* __restrict__ X, int D, int K, int CH,
+                                                          const int32_t* __restrict__ perm,
+                                                          const int32_t* __restrict__ label_start,
+                                                          const int32_t* __restrict__ chunk_start,
+                                                          const int32_t* __restrict__ subs,
+                                                          double* __restrict__ partials) {
+    constexpr int NB = DP / 4;
+    constexpr int NTRI = NB * (NB + 1) / 2;
+    constexpr int R = (NTRI + kThreads - 1) / kThreads;
+    constexpr int BATCH = DP > 64 ? 32 : 64;
+    constexpr size_t E = 1 + DP + (size_t)DP * DP;
+    __shared__ __align__(16) double xs[BATCH * DP];
+    __shared__ int idl[64];
+    __shared__ int nsel;
+    const int c = blockIdx.x >> 1, s = blockIdx.x & 1;
+    int k, beg, end;
+    if (!chunk_of(c, K, chunk_start, label_start, CH, k, beg, end)) return;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int bi[R], bj[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        int b = threadIdx.x + r * kThreads;
+        bi[r] = -1; bj[r] = -1;
+        if (b < NTRI) {   // row-major upper block triangle
+            int row = 0;
+            while (b >= NB - row) { b -= NB - row; ++row; }
+            bi[r] = row; bj[r] = row + b;
+        }
+    }
+    double acc[R][16];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int e = 0; e < 16; ++e) acc[r][e] = 0.0;
+    double sx = 0.0;
+    int64_t cnt = 0;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int b0 = beg; b0 < end; b0 += BATCH) {
+        if (wv == 0) {
+            const int p = b0 + lane;
+            int idx = -1;
+            bool f = false;
+            if (lane < BATCH && p < end) { idx = perm[p]; f = subs[idx] == s; }
+            const uint64_t m = __ballot(f);
+            if (f) idl[__popcll(m & lt)] = idx;
+            if (lane == 0) nsel = __popcll(m);
+        }
+        __syncthreads();
+        const int n = nsel;
+        for (int e = threadIdx.x; e < n * DP; e += kThreads) {
+            const int pp = e / DP, l = e - pp * DP;
+            xs[pp * DP + l] = l < D ? (double)X[(int64_t)idl[pp] * D + l] : 0.0;
+        }
+        __syncthreads();
+        for (int pp = 0; pp < n; ++pp) {
+            const double* xp = xs + pp * DP;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                if (bi[r] < 0) continue;
+                double a[4], b[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) { a[u] = xp[4 * bi[r] + u]; b[u] = xp[4 * bj[r] + u]; }
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) acc[r][u * 4 + v] = fma(a[u], b[v], acc[r][u * 4 + v]);
+            }
+            if (threadIdx.x < DP) sx += xp[threadIdx.x];
+        }
+        cnt += n;
+        __syncthreads();
+    }
+    double* out = partials + (size_t)blockIdx.x * E;
+    if (threadIdx.x == 0) out[0] = (double)cnt;
+    if (threadIdx.x < DP) out[1 + threadIdx.x] = sx;
+    double* S = out + 1 + DP;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        if (bi[r] < 0) continue;
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int a = 4 * bi[r] + u, b = 4 * bj[r] + v;
+                S[(size_t)a * DP + b] = acc[r][u * 4 + v];
+                S[(size_t)b * DP + a] = acc[r][u * 4 + v];
+            }
+    }
+}
+
+
+template <typename T, int DP>
+void launch_label_gauss(hipStream_t st, int grid, const T* X, int64_t N, int D, int K, const T* P, uint32_t key0,
+                        uint32_t key1, uint32_t iter, int fin, int32_t* labels) {
+    hipLaunchKernelGGL((k_label_gauss<T, DP>), dim3(grid), dim3(kThreads), 0, st, X, N, D, K, P, key0, key1, iter,
+                       fin, labels);
+}
+
+template <typename T, int DP>
+void launch_sub_gauss(hipStream_t st, int grid, const T* X, int D, int K, int CH, const int32_t* perm,
+                      const int32_t* label_start, const int32_t* chunk_start, const T* SP, uint32_t key0,
+                      uint32_t key1, uint32_t iter, int32_t* subs) {
+    hipLaunchKernelGGL((k_sub_gauss<T, DP>), dim3(grid), dim3(kThreads), 0, st, X, D, K, CH, perm, label_start,
+                       chunk_start, SP, key0, key1, iter, subs);
+}
+
+template <typename TX, int DP>
+void launch_stats_gauss(hipStream_t st, int grid, const TX* X, int D, int K, int CH, const int32_t* perm,
+                        const int32_t* label_start, const int32_t* chunk_start, const int32_t* subs,
+                        double* partials) {
+    hipLaunchKernelGGL((k_stats_gauss<TX, DP>), dim3(grid), dim3(kThreads), 0, st, X, D, K, CH, perm, label_start,
+                       chunk_start, subs, partials);
+}
+
+#define DPMM_GAUSS_INSTANTIATE(T, DP)                                                                         \
+    template void launch_label_gauss<T, DP>(hipStream_t, int, const T*, int64_t, int, int, const T*, uint32_t, \
+                                            uint32_t, uint32_t, int, int32_t*);                                \
+    template void launch_sub_gauss<T, DP>(hipStream_t, int, const T*, int, int, int, const int32_t*,           \
+                                          const int32_t*, const int32_t*, const T*, uint32_t, uint32_t,        \
+                                          uint32_t, int32_t*);                                                 \
+    template void launch_stats_gauss<T, DP>(hipStream_t, int, const T*, int, int, int, const int32_t*,         \
+                                            const int32_t*, const int32_t*, const int32_t*, double*);
+
+#define DPMM_GAUSS_EXTERN(T, DP)                                                                              \
+    extern template void launch_label_gauss<T, DP>(hipStream_t, int, const T*, int64_t, int, int, const T*,    \
+                                                   uint32_t, uint32_t, uint32_t, int, int32_t*);               \
+    extern template void launch_sub_gauss<T, DP>(hipStream_t, int, const T*, int, int, int, const int32_t*,    \
+                                                 const int32_t*, const int32_t*, const T*, uint32_t, uint32_t, \
+                                                 uint32_t, int32_t*);                                          \
+    extern template void launch_stats_gauss<T, DP>(hipStream_t, int, const T*, int, int, int, const int32_t*,  \
+                                                   const int32_t*, const int32_t*, const int32_t*, double*);
+
+}  // namespace dpmm

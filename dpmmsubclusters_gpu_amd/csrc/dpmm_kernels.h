@@ -1,0 +1,82 @@
+// Device building blocks shared by the sweep, partition, statistics and relabel kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "dpmm_rng.h"
+
+namespace dpmm {
+
+constexpr int kThreads = 256;
+
+// Packed upper-triangular whitening factor W = R^T (R the lower Cholesky factor of the
+// precision matrix), row j stored from column (j & ~3) to DP-1 so every row starts on a
+// 16-byte boundary: entries below the diagonal inside that first group are zero.
+__host__ __device__ constexpr int tri_row_offset(int DP, int j) {
+    int off = 0;
+    for (int r = 0; r < j; ++r) off += DP - (r & ~3);
+    return off;
+}
+__host__ __device__ constexpr int tri_size(int DP) { return tri_row_offset(DP, DP); }
+// Gaussian parameter slot: [W packed | mu (DP) | c | pad] in elements of the compute type.
+__host__ __device__ constexpr int gauss_slot_stride(int DP) { return ((tri_size(DP) + DP + 1) + 3) & ~3; }
+__host__ __device__ constexpr int gauss_mu_offset(int DP) { return tri_size(DP); }
+__host__ __device__ constexpr int gauss_c_offset(int DP) { return tri_size(DP) + DP; }
+
+// -log(-log u), u = (w + 0.5) 2^-32, with -log u taken through log1p on the upper half so
+// the exponential variate keeps full relative precision near u = 1.
+__device__ __forceinline__ float gumbel_f32(uint32_t w) {
+    float e;
+    if (w < 0x80000000u) e = -logf(((float)w + 0.5f) * 0x1p-32f);
+    else e = -log1pf(-(((float)(0xFFFFFFFFu - w)) + 0.5f) * 0x1p-32f);
+    return -logf(e);
+}
+__device__ __forceinline__ double gumbel_f64(uint32_t w) {
+    double e;
+    if (w < 0x80000000u) e = -log(((double)w + 0.5) * 0x1p-32);
+    else e = -log1p(-(((double)(0xFFFFFFFFu - w)) + 0.5) * 0x1p-32);
+    return -log(e);
+}
+template <typename T> __device__ __forceinline__ T gumbel(uint32_t w);
+template <> __device__ __forceinline__ float gumbel<float>(uint32_t w) { return gumbel_f32(w); }
+template <> __device__ __forceinline__ double gumbel<double>(uint32_t w) { return gumbel_f64(w); }
+
+// logit(u) = log(u) - log(1 - u) for u = (w + 0.5) 2^-32.
+template <typename T> __device__ __forceinline__ T logit_u32(uint32_t w);
+template <> __device__ __forceinline__ float logit_u32<float>(uint32_t w) {
+    return logf((float)w + 0.5f) - logf((float)(0xFFFFFFFFu - w) + 0.5f);
+}
+template <> __device__ __forceinline__ double logit_u32<double>(uint32_t w) {
+    return log((double)w + 0.5) - log((double)(0xFFFFFFFFu - w) + 0.5);
+}
+
+__device__ __forceinline__ uint32_t sel4(const Ctr4& c, int u) {
+    const uint32_t a = (u & 1) ? c.v[1] : c.v[0];
+    const uint32_t b = (u & 1) ? c.v[3] : c.v[2];
+    return (u & 2) ? b : a;
+}
+
+template <typename T> __device__ __forceinline__ T neg_inf();
+template <> __device__ __forceinline__ float neg_inf<float>() { return -__builtin_huge_valf(); }
+template <> __device__ __forceinline__ double neg_inf<double>() { return -__builtin_huge_val(); }
+
+// Mahalanobis form q = || W (x - mu) ||^2 with W packed upper triangular in LDS.
+template <typename T, int DP>
+__device__ __forceinline__ T whitened_sq(const T* __restrict__ ps, const T (&x)[DP]) {
+    T d[DP];
+#pragma unroll
+    for (int l = 0; l < DP; ++l) d[l] = x[l] - ps[gauss_mu_offset(DP) + l];
+    T q = 0;
+#pragma unroll
+    for (int j = 0; j < DP; ++j) {
+        const int l0 = j & ~3;
+        const T* row = ps + tri_row_offset(DP, j) - l0;
+        T y = 0;
+#pragma unroll
+        for (int l = l0; l < DP; ++l) y = fma(row[l], d[l], y);
+        q = fma(y, y, q);
+    }
+    return q;
+}
+
+}  // namespace dpmm

@@ -1,0 +1,142 @@
+/*
+ * C ABI of the MI355X (gfx950) sub-cluster sweep: the device layer of the DPMM split/merge
+ * sampler, drop-in for the reference's cudaKernel plugin (prior::get_cuda(),
+ * priors/prior.h:29; cudaKernel.cuh:31-138) plus the host driver that replaces
+ * dp_parallel_sampling_class::dp_parallel() (dp_parallel_sampling.h:17-29).
+ *
+ * Plain pointers and sizes only.  Every function returns DPMM_OK (0) or a negative
+ * DPMM_E* code; dpmm_last_error() describes the last failure.  The library owns no caller
+ * memory.  Conventions: points are row-major N x D (each point contiguous, the reference's
+ * D x N column-major layout, cudaKernel.cu:623-624); labels are 0-based here (the C++
+ * drop-in layer, include/dpmm/, converts to the reference's 1-based labels); sub-labels are
+ * 0 (the reference's 1, "left") or 1 (the reference's 2, "right").
+ */
+#ifndef DPMM_HIP_H
+#define DPMM_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    DPMM_OK = 0,
+    DPMM_EINVAL = -1,   /* bad argument (null pointer, size, K = 0, D mismatch, NaN) */
+    DPMM_EHIP = -2,     /* HIP runtime error */
+    DPMM_ENOMEM = -3,   /* device or host allocation failed */
+    DPMM_ENOTSPD = -4,  /* a covariance / precision matrix is not symmetric positive definite */
+    DPMM_ESTATE = -5    /* call out of order (e.g. statistics before labels) */
+};
+enum { DPMM_PRIOR_GAUSSIAN = 0, DPMM_PRIOR_MULTINOMIAL = 1 };        /* prior_type, global_params.h:14-18 */
+enum { DPMM_DTYPE_F64 = 0, DPMM_DTYPE_F32 = 1, DPMM_DTYPE_I32 = 2 };
+/* Arithmetic of the label sweep: FP64, FP32, or BF16 MFMA with FP32 near-tie recompute. */
+enum { DPMM_PREC_FP64 = 0, DPMM_PREC_FP32 = 1, DPMM_PREC_MIXED = 2 };
+
+typedef struct dpmm_ctx dpmm_ctx;
+typedef struct dpmm_sampler dpmm_sampler;
+
+const char* dpmm_last_error(void);
+int dpmm_device_count(int* count);
+
+/* cudaKernel::init (cudaKernel.cu:537-635): copies X (host) to HBM, N x D of `dtype`. */
+int dpmm_create(const void* X, int64_t N, int D, int dtype, int prior, uint64_t seed, dpmm_ctx** out);
+/* Same, with X already in device memory (not copied; must outlive the context). */
+int dpmm_create_device(const void* dX, int64_t N, int D, int dtype, int prior, uint64_t seed, dpmm_ctx** out);
+/* cudaKernel::release (global_params.cpp:108-114). */
+int dpmm_destroy(dpmm_ctx* ctx);
+int dpmm_set_precision(dpmm_ctx* ctx, int precision);
+
+/* cudaKernel::sample_labels + sample_sub_labels (cudaKernel.cu:1093-1111): uniform labels
+ * in 0..init_clusters-1 and uniform sub-labels. */
+int dpmm_sample_initial_labels(dpmm_ctx* ctx, int init_clusters);
+/* Test hook of the reference's myCudaKernel set_labels / set_sub_labels (test/myCudaKernel.h). */
+int dpmm_set_labels(dpmm_ctx* ctx, const int32_t* labels, const int32_t* sub_labels);
+/* cudaKernel::get_labels / get_sub_labels (cudaKernel.cu:1113-1143). */
+int dpmm_get_labels(dpmm_ctx* ctx, int32_t* labels, int32_t* sub_labels);
+
+/* Per-iteration cluster parameters, the thin_cluster_params broadcast (ds.h:214-227,
+ * local_clusters_actions.cpp:449-458), in the reference's mv_gaussian representation
+ * (distributions/mv_gaussian.h: mu, invSigma, logdetSigma): K clusters with mixture
+ * weights, and 2K sub-clusters (index 2k + s) with lr_weights (K x 2).  The library
+ * factors invSigma = R R^T in FP64.  Returns DPMM_ENOTSPD if a matrix is not SPD. */
+int dpmm_set_gaussian_params(dpmm_ctx* ctx, int K, const double* mu, const double* inv_sigma,
+                             const double* logdet_sigma, const double* weights, const double* sub_mu,
+                             const double* sub_inv_sigma, const double* sub_logdet_sigma,
+                             const double* lr_weights);
+/* Same with the lower Cholesky factor R of invSigma given (the sampler's own draws produce
+ * it directly from the Bartlett decomposition). */
+int dpmm_set_gaussian_factors(dpmm_ctx* ctx, int K, const double* mu, const double* chol_inv_sigma,
+                              const double* logdet_sigma, const double* weights, const double* sub_mu,
+                              const double* sub_chol_inv_sigma, const double* sub_logdet_sigma,
+                              const double* lr_weights);
+/* multinomial_dist (distributions/multinomial_dist.h): log theta per cluster (K x D) and per
+ * sub-cluster (2K x D).  use_weight = 0 reproduces the reference's label log-likelihood,
+ * which leaves out log pi_k (cudaKernel_multinomial.cu:60-76). */
+int dpmm_set_multinomial_params(dpmm_ctx* ctx, int K, const double* logtheta, const double* weights,
+                                const double* sub_logtheta, const double* lr_weights, int use_weight);
+
+/* cudaKernel::create_clusters_labels (cudaKernel.cu:1400-1490): one fused pass over X that
+ * scores every point against all K clusters and draws its label (Gumbel-max on Philox
+ * noise of iteration `iter`), or takes the argmax when final != 0. */
+int dpmm_sample_cluster_labels(dpmm_ctx* ctx, uint32_t iter, int final_iter);
+/* cudaKernel::create_subclusters_labels (cudaKernel.cu:1299-1398): deterministic partition
+ * of the points by label, then one segmented pass drawing each point's sub-label. */
+int dpmm_sample_sub_labels(dpmm_ctx* ctx, uint32_t iter);
+/* cudaKernel::create_sufficient_statistics (cudaKernel.cu:844-941) for all K clusters at
+ * once: per sub-cluster q = 2k + s: count Nsub[q], sum[q*D + d], and (Gaussian) the
+ * uncentred scatter S[q*D*D + a*D + b] in FP64; S may be NULL.  Deterministic: no
+ * atomics on bins; a fixed reduction order. */
+int dpmm_sufficient_statistics(dpmm_ctx* ctx, int K, int64_t* Nsub, double* sum, double* S);
+/* One relabel pass for split / merge / reset / remove (cudaKernel.cu:172-263): a point
+ * with label k and sub-label s gets label new_label[2k + s]; its sub-label becomes
+ * s (sub_action[k] = 0), a Philox draw of tag 4 (reset, 1) or 5 (split, 2), or the
+ * constant 0 (3) / 1 (4).  Tables have K_old entries (new_label 2 K_old). */
+int dpmm_relabel(dpmm_ctx* ctx, uint32_t iter, int K_old, const int32_t* new_label, const int32_t* sub_action);
+
+/* Kernel timings of the last calls (ms, HIP events on the library's stream). */
+int dpmm_last_kernel_ms(dpmm_ctx* ctx, float* label_sweep_ms, float* sub_sweep_ms, float* stats_ms);
+int dpmm_synchronize(dpmm_ctx* ctx);
+
+/* ------------------------------------------------------------------ host driver ----- */
+/* dp_parallel() options (dp_parallel_sampling.h:17-29, global_params.h:36-78). */
+typedef struct {
+    int prior;
+    double alpha;
+    int iters;
+    int init_clusters;
+    int burnout;
+    int argmax_sample_stop;   /* reference default 5 */
+    int split_stop;           /* reference default 5 */
+    double max_clusters;
+    uint64_t seed;
+    int full_mu_cov;          /* 0: the reference's diagonal-only mu draw (niw.cpp:100-103) */
+    int mult_use_weight;      /* 0: the reference's multinomial label log-likelihood */
+    int precision;            /* DPMM_PREC_* of the label sweep */
+    int verbose;
+} dpmm_options;
+
+/* init_model + init_first_clusters (dp_parallel_sampling.cpp:91-130).  Gaussian hyper:
+ * niw_hyperparams(k0, m0, v0, psi0) with psi0 = Psi / v0 (priors/niw_hyperparams.h);
+ * multinomial: multinomial_hyper(alpha0) (priors/multinomial_hyper.h). */
+int dpmm_sampler_create(dpmm_ctx* ctx, const dpmm_options* opt, double k0, const double* m0, double v0,
+                        const double* psi0, const double* alpha0, dpmm_sampler** out);
+/* One iteration, local_clusters_actions::group_step (local_clusters_actions.cpp:460-490),
+ * with the final / no_more_splits flags of run_model (dp_parallel_sampling.cpp:182-193). */
+int dpmm_sampler_step(dpmm_sampler* s, int iter);
+int dpmm_sampler_num_clusters(dpmm_sampler* s);
+/* Mixture weights of the last step (K values) and per-cluster counts. */
+int dpmm_sampler_weights(dpmm_sampler* s, double* weights, int64_t* counts, int max_k);
+/* Host time spent in the last step's FP64 work (draws, posteriors, Hastings), ms. */
+int dpmm_sampler_host_ms(dpmm_sampler* s, double* host_ms);
+int dpmm_sampler_destroy(dpmm_sampler* s);
+
+/* dp_parallel_sampling_class::dp_parallel (dp_parallel_sampling.cpp:132-160) end to end:
+ * labels (N, 0-based), cluster count history (iters), per-iteration seconds (iters). */
+int dpmm_run(dpmm_ctx* ctx, const dpmm_options* opt, double k0, const double* m0, double v0, const double* psi0,
+             const double* alpha0, int32_t* labels, int32_t* k_hist, double* iter_seconds, double* weights,
+             int max_k, int* final_k);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -592,7 +592,8 @@ int set_gauss(dpmm_ctx* c, int K, const double* mu, const double* mat, const dou
 
 template <typename T, int DP>
 void run_label_gauss(dpmm_ctx* c, uint32_t iter, int fin) {
-    const int grid = (int)((c->N + kThreads - 1) / kThreads);
+    constexpr int ppb = label_points_per_block<T, DP>();
+    const int grid = (int)((c->N + ppb - 1) / ppb);
     launch_label_gauss<T, DP>(c->stream, grid, reinterpret_cast<const T*>(c->dX), c->N, c->D, c->K,
                               reinterpret_cast<const T*>(c->dP), c->key0(), c->key1(), iter, fin, c->dLabels);
 }

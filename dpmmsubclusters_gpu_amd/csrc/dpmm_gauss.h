@@ -23,18 +23,25 @@ __device__ __forceinline__ bool chunk_of(int c, int K, const int32_t* __restrict
     return true;
 }
 
-// Gaussian label sweep, one point per thread, point kept in registers; cluster parameters
-// staged through LDS one cluster at a time.
+// Gaussian label sweep, one point per thread (or per lane pair for wide points), the point in
+// registers; cluster parameters staged through LDS one cluster at a time.
 template <typename T, int DP>
 __global__ void __launch_bounds__(kThreads) k_label_gauss(const T* __restrict__ X, int64_t N, int D, int K,
                                                           const T* __restrict__ P, uint32_t key0, uint32_t key1,
                                                           uint32_t iter, int final_iter, int32_t* __restrict__ labels) {
+    constexpr bool kPair = paired_layout<T, DP>();
+    constexpr int kPts = kPair ? kThreads / 2 : kThreads;
+    constexpr int kCols = kPair ? DP / 2 : DP;
     constexpr int stride = gauss_slot_stride(DP);
     __shared__ __align__(16) T ps[stride];
-    const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-    T x[DP];
+    const int par = kPair ? (threadIdx.x & 1) : 0;
+    const int64_t i = (int64_t)blockIdx.x * kPts + (kPair ? (threadIdx.x >> 1) : threadIdx.x);
+    T x[kCols];
 #pragma unroll
-    for (int l = 0; l < DP; ++l) x[l] = (i < N && l < D) ? X[i * D + l] : T(0);
+    for (int c = 0; c < kCols; ++c) {
+        const int l = kPair ? pair_col(c, 0) + (par << 2) : c;
+        x[c] = (i < N && l < D) ? X[i * D + l] : T(0);
+    }
     T best = neg_inf<T>();
     int bi = 0;
     Ctr4 w = {};
@@ -43,7 +50,9 @@ __global__ void __launch_bounds__(kThreads) k_label_gauss(const T* __restrict__ 
         const T* src = P + (size_t)k * stride;
         for (int e = threadIdx.x; e < stride; e += kThreads) ps[e] = src[e];
         __syncthreads();
-        const T q = whitened_sq<T, DP>(ps, x);
+        T q;
+        if constexpr (kPair) q = whitened_sq_pair<T, DP>(ps, x, par);
+        else q = whitened_sq_regs<T, DP>(ps, x);
         T s = ps[gauss_c_offset(DP)] - T(0.5) * q;
         if (!final_iter) {
             if ((k & 3) == 0) w = philox4x32_10((uint32_t)i, (uint32_t)(k >> 2), iter, TAG_LABEL, key0, key1);
@@ -51,7 +60,7 @@ __global__ void __launch_bounds__(kThreads) k_label_gauss(const T* __restrict__ 
         }
         if (s > best || k == 0) { best = s; bi = k; }
     }
-    if (i < N) labels[i] = bi;
+    if (i < N && par == 0) labels[i] = bi;
 }
 
 template <typename T, int DP>
@@ -61,6 +70,9 @@ __global__ void __launch_bounds__(kThreads) k_sub_gauss(const T* __restrict__ X,
                                                         const int32_t* __restrict__ chunk_start,
                                                         const T* __restrict__ SP, uint32_t key0, uint32_t key1,
                                                         uint32_t iter, int32_t* __restrict__ subs) {
+    constexpr bool kPair = paired_layout<T, DP>();
+    constexpr int kPts = kPair ? kThreads / 2 : kThreads;
+    constexpr int kCols = kPair ? DP / 2 : DP;
     constexpr int stride = gauss_slot_stride(DP);
     __shared__ __align__(16) T ps[2 * stride];
     int k, beg, end;
@@ -68,15 +80,32 @@ __global__ void __launch_bounds__(kThreads) k_sub_gauss(const T* __restrict__ X,
     const T* src = SP + (size_t)(2 * k) * stride;
     for (int e = threadIdx.x; e < 2 * stride; e += kThreads) ps[e] = src[e];
     __syncthreads();
-    for (int p = beg + threadIdx.x; p < end; p += kThreads) {
-        const int idx = perm[p];
-        T x[DP];
+    const int par = kPair ? (threadIdx.x & 1) : 0;
+    const int lane_pt = kPair ? (threadIdx.x >> 1) : threadIdx.x;
+    // every lane of a pair runs the loop the same number of times (the DPP swap needs both)
+    for (int p0 = beg; p0 < end; p0 += kPts) {
+        const int p = p0 + lane_pt;
+        const bool valid = p < end;
+        const int idx = valid ? perm[p] : 0;
+        T x[kCols];
 #pragma unroll
-        for (int l = 0; l < DP; ++l) x[l] = l < D ? X[(int64_t)idx * D + l] : T(0);
-        const T s0 = ps[gauss_c_offset(DP)] - T(0.5) * whitened_sq<T, DP>(ps, x);
-        const T s1 = ps[stride + gauss_c_offset(DP)] - T(0.5) * whitened_sq<T, DP>(ps + stride, x);
-        const uint32_t w = philox4x32_10((uint32_t)idx, 0, iter, TAG_SUB, key0, key1).v[0];
-        subs[idx] = (logit_u32<T>(w) < s0 - s1) ? 0 : 1;
+        for (int c = 0; c < kCols; ++c) {
+            const int l = kPair ? pair_col(c, 0) + (par << 2) : c;
+            x[c] = (valid && l < D) ? X[(int64_t)idx * D + l] : T(0);
+        }
+        T sc[2];
+#pragma unroll 1
+        for (int h = 0; h < 2; ++h) {   // one sub-cluster at a time keeps a single residual live
+            const T* pp = ps + h * stride;
+            T q;
+            if constexpr (kPair) q = whitened_sq_pair<T, DP>(pp, x, par);
+            else q = whitened_sq_regs<T, DP>(pp, x);
+            sc[h] = pp[gauss_c_offset(DP)] - T(0.5) * q;
+        }
+        if (valid && par == 0) {
+            const uint32_t w = philox4x32_10((uint32_t)idx, 0, iter, TAG_SUB, key0, key1).v[0];
+            subs[idx] = (logit_u32<T>(w) < sc[0] - sc[1]) ? 0 : 1;
+        }
     }
 }
 
@@ -173,6 +202,9 @@ __global__ void __launch_bounds__(kThreads) k_stats_gauss(const TX* __restrict__
     }
 }
 
+
+template <typename T, int DP>
+constexpr int label_points_per_block() { return paired_layout<T, DP>() ? kThreads / 2 : kThreads; }
 
 template <typename T, int DP>
 void launch_label_gauss(hipStream_t st, int grid, const T* X, int64_t N, int D, int K, const T* P, uint32_t key0,

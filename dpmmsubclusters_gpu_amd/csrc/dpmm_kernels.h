@@ -61,8 +61,9 @@ template <> __device__ __forceinline__ float neg_inf<float>() { return -__builti
 template <> __device__ __forceinline__ double neg_inf<double>() { return -__builtin_huge_val(); }
 
 // Mahalanobis form q = || W (x - mu) ||^2 with W packed upper triangular in LDS.
+// Small points keep the centred residual d = x - mu in registers next to x.
 template <typename T, int DP>
-__device__ __forceinline__ T whitened_sq(const T* __restrict__ ps, const T (&x)[DP]) {
+__device__ __forceinline__ T whitened_sq_regs(const T* __restrict__ ps, const T (&x)[DP]) {
     T d[DP];
 #pragma unroll
     for (int l = 0; l < DP; ++l) d[l] = x[l] - ps[gauss_mu_offset(DP) + l];
@@ -77,6 +78,59 @@ __device__ __forceinline__ T whitened_sq(const T* __restrict__ ps, const T (&x)[
         q = fma(y, y, q);
     }
     return q;
+}
+
+// Wide points (x and the residual would not both fit in registers) are split across a lane
+// pair: lane parity `par` holds the coordinates l = 8 b + 4 par + u (u < 4) of every 8-block
+// (local index c = 4 b + u), forms its half of each row sum, and the pair adds the halves with a
+// DPP swap, so both lanes end with the same q.
+__host__ __device__ constexpr int pair_col(int c, int par) { return ((c >> 2) << 3) + (par << 2) + (c & 3); }
+// Points whose x (DP elements of T) exceeds 256 bytes take the lane-pair layout.
+template <typename T, int DP> __host__ __device__ constexpr bool paired_layout() { return DP * sizeof(T) > 256; }
+
+__device__ __forceinline__ float pair_swap(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, true));   // quad_perm 1,0,3,2
+}
+__device__ __forceinline__ double pair_swap(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffffll), 0xB1, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0xB1, 0xF, 0xF, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+template <typename T, int DP>
+__device__ __forceinline__ T whitened_sq_pair(const T* __restrict__ ps, const T (&xh)[DP / 2], int par) {
+    constexpr int HC = DP / 2;
+    const T* mu = ps + gauss_mu_offset(DP);
+    T d[HC];
+#pragma unroll
+    for (int c = 0; c < HC; ++c) d[c] = xh[c] - mu[pair_col(c, 0) + (par << 2)];
+    const T keep = par ? T(1) : T(0);   // the 4-block just below an odd row block belongs to odd lanes
+    T q = 0;
+#pragma unroll
+    for (int j = 0; j < DP; ++j) {
+        const int jb = j >> 2;
+        const T* row = ps + tri_row_offset(DP, j) - (j & ~3) + (par << 2);
+        T y = 0;
+#pragma unroll
+        for (int b = 0; b < HC / 4; ++b) {
+            if (2 * b + 1 < jb) continue;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                T dv = d[4 * b + u];
+                if (2 * b + 1 == jb) dv *= keep;
+                y = fma(row[(b << 3) + u], dv, y);
+            }
+        }
+        y += pair_swap(y);
+        q = fma(y, y, q);
+    }
+    return q;
+}
+
+template <typename T, int DP>
+__device__ __forceinline__ T whitened_sq(const T* __restrict__ ps, const T (&x)[DP]) {
+    return whitened_sq_regs<T, DP>(ps, x);
 }
 
 }  // namespace dpmm

@@ -95,6 +95,9 @@ def main():
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="copy X once (N*D*4 bytes read + written) before the run: a plain copy kernel of known "
+                         "size that calibrates the HBM byte counters of a profiler")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (points would shard across GPUs "
@@ -113,6 +116,11 @@ def main():
                          dtype=dtype)
     hyper = dp.niw_hyperparams(1.0, np.zeros(D), D + 3.0, np.eye(D))
     prec = dp.Precision.FP64 if dtype == np.float64 else dp.Precision.FP32
+    if args.copy_probe:
+        Y = X.clone()
+        torch.cuda.synchronize()
+        print(f"[bench] copy probe: {X.numel() * X.element_size()} bytes", file=sys.stderr, flush=True)
+        del Y
     iters = args.warmup + args.steps + 10
     smp = dp.Sampler(dev, hyper, cfg["alpha"], iters, seed=args.seed + 1, precision=prec)
     t_w = time.perf_counter()

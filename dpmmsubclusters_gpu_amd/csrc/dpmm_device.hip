@@ -304,17 +304,18 @@ __global__ void __launch_bounds__(kThreads) k_stats_mult(const int32_t* __restri
     if (threadIdx.x == 0) out[0] = (double)cnt;
 }
 
-// Fixed-order reduction of the chunk partials of each (cluster, sub-cluster).
+// Fixed-order reduction of the chunk partials of each (cluster, sub-cluster): one thread per
+// statistic entry, summing that entry over the cluster's chunks in chunk order.
 __global__ void __launch_bounds__(kThreads) k_stats_reduce(const double* __restrict__ partials,
                                                            const int32_t* __restrict__ chunk_start, int K, int E,
                                                            double* __restrict__ stats) {
     const int q = blockIdx.x, k = q >> 1, s = q & 1;
+    const int e = blockIdx.y * kThreads + threadIdx.x;
+    if (e >= E) return;
     const int c0 = chunk_start[k], c1 = chunk_start[k + 1];
-    for (int e = threadIdx.x; e < E; e += kThreads) {
-        double v = 0.0;
-        for (int c = c0; c < c1; ++c) v += partials[(size_t)(2 * c + s) * E + e];
-        stats[(size_t)q * E + e] = v;
-    }
+    double v = 0.0;
+    for (int c = c0; c < c1; ++c) v += partials[(size_t)(2 * c + s) * E + e];
+    stats[(size_t)q * E + e] = v;
 }
 
 // ---- relabel -----------------------------------------------------------------------------------
@@ -847,8 +848,8 @@ int dpmm_sufficient_statistics(dpmm_ctx* c, int K, int64_t* Nsub, double* sum, d
                            reinterpret_cast<const int32_t*>(c->dX), c->D, K, c->CH, c->dPerm, c->dLabelStart,
                            c->dChunkStart, c->dSubs, (int)E, c->dPartials);
     }
-    hipLaunchKernelGGL(k_stats_reduce, dim3(2 * K), dim3(kThreads), 0, c->stream, c->dPartials, c->dChunkStart, K,
-                       (int)E, c->dStats);
+    hipLaunchKernelGGL(k_stats_reduce, dim3(2 * K, (unsigned)((E + kThreads - 1) / kThreads)), dim3(kThreads), 0,
+                       c->stream, c->dPartials, c->dChunkStart, K, (int)E, c->dStats);
     (void)hipEventRecord(c->ev[5], c->stream);
     HIPCHK(hipGetLastError());
     c->ev_stats = true;

@@ -92,7 +92,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=150)
     ap.add_argument("--config", default="c3_d64", choices=sorted(CONFIGS))
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
+    ap.add_argument("--precision", default="mixed", choices=["mixed", "fp32", "fp64"])
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
@@ -115,7 +115,10 @@ def main():
     dev = dp.DeviceSweep(None, dp.prior_type.Gaussian, seed=args.seed + 1, device_ptr=X.data_ptr(), N=N, D=D,
                          dtype=dtype)
     hyper = dp.niw_hyperparams(1.0, np.zeros(D), D + 3.0, np.eye(D))
-    prec = dp.Precision.FP64 if dtype == np.float64 else dp.Precision.FP32
+    if dtype == np.float64:
+        prec = dp.Precision.FP64
+    else:
+        prec = dp.Precision.MIXED if args.precision == "mixed" else dp.Precision.FP32
     if args.copy_probe:
         Y = X.clone()
         torch.cuda.synchronize()
@@ -131,13 +134,15 @@ def main():
                   file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     dev.synchronize()
-    lab_ms, sub_ms, st_ms, host_ms, flops = [], [], [], [], []
+    lab_ms, sub_ms, st_ms, host_ms, flops, rechk = [], [], [], [], [], []
     t0 = time.perf_counter()
     for t in range(args.warmup + 1, args.warmup + args.steps + 1):
         kc = smp.num_clusters()
         smp.step(t)
         a, b, c = dev.kernel_ms()
         lab_ms.append(a); sub_ms.append(b); st_ms.append(c); host_ms.append(smp.host_ms())
+        if prec == dp.Precision.MIXED:
+            rechk.append(dev.last_recheck_count())
         flops.append(2.0 * kc * D * D * N)
     dev.synchronize()
     torch.cuda.synchronize()
@@ -146,7 +151,7 @@ def main():
     value = N * args.steps / elapsed
     avg_lab = float(np.mean(lab_ms))
     achieved = float(np.mean(flops)) / (avg_lab * 1e-3) / 1e12
-    peak = PEAK["fp64" if prec == dp.Precision.FP64 else "fp32"]
+    peak = PEAK[{dp.Precision.FP64: "fp64", dp.Precision.FP32: "fp32", dp.Precision.MIXED: "bf16"}[prec]]
     K_final = smp.num_clusters()
     from dpmmsubclusters_gpu_amd import _nmi
     lab, _ = dev.get_labels()
@@ -154,18 +159,22 @@ def main():
     out = {
         "metric": "point*iters/sec", "value": value, "unit": "point*iters/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32" if prec != dp.Precision.FP64 else "f64",
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": {dp.Precision.FP64: "f64", dp.Precision.FP32: "f32", dp.Precision.MIXED: "bf16+f32"}[prec],
         "data": "synthetic",
         "config": {"workload": f"{args.config}: Gaussian DPMM sampler iteration, N={N}, D={D}, "
                                f"{cfg['K']} true clusters, alpha={cfg['alpha']}, NIW(k=1, m=0, v=D+3, psi=I)",
                    "N": N, "D": D, "clusters_timed": K_final, "precision": args.precision,
                    "parallelism": "single GPU"},
-        "roofline": {"bound": "mfma", "kernel": "k_label_gauss", "achieved": achieved, "peak": peak,
+        "roofline": {"bound": "mfma",
+                     "kernel": "k_label_gauss_mx + k_label_gauss_list" if prec == dp.Precision.MIXED else "k_label_gauss",
+                     "achieved": achieved, "peak": peak,
                      "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
                      "algorithmic": "2*K*D^2 flops per point per launch (dense whitening of K clusters)"},
         "breakdown_ms": {"label_sweep": avg_lab, "sub_sweep": float(np.mean(sub_ms)),
                          "statistics": float(np.mean(st_ms)), "host_fp64": float(np.mean(host_ms))},
         "final_nmi_vs_truth": nmi,
+        "recheck_fraction": (float(np.mean(rechk)) / N) if rechk else None,
     }
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline (FP64 oracle)", file=sys.stderr, flush=True)

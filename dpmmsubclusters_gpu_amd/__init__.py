@@ -86,6 +86,8 @@ def load_library():
         "dpmm_sufficient_statistics": [_P, i, _P, _P, _P],
         "dpmm_relabel": [_P, u32, i, _P, _P],
         "dpmm_last_kernel_ms": [_P, _P, _P, _P],
+        "dpmm_set_mixed_tolerance": [_P, ctypes.c_float],
+        "dpmm_last_recheck_count": [_P, _P],
         "dpmm_synchronize": [_P],
         "dpmm_sampler_create": [_P, _P, dbl, _P, dbl, _P, _P, _P],
         "dpmm_sampler_step": [_P, i],
@@ -255,6 +257,14 @@ class DeviceSweep:
     def relabel(self, iteration, new_label, sub_action):
         nl, act = _i32(new_label), _i32(sub_action)
         _check(load_library().dpmm_relabel(self._ctx, int(iteration), act.size, _ptr(nl), _ptr(act)))
+
+    def set_mixed_tolerance(self, tau):
+        _check(load_library().dpmm_set_mixed_tolerance(self._ctx, float(tau)))
+
+    def last_recheck_count(self):
+        n = ctypes.c_int64()
+        _check(load_library().dpmm_last_recheck_count(self._ctx, ctypes.byref(n)))
+        return n.value
 
     def kernel_ms(self):
         a, b, c = ctypes.c_float(), ctypes.c_float(), ctypes.c_float()

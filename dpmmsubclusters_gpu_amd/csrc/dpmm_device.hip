@@ -24,6 +24,7 @@
 #include "dpmm_kernels.h"
 #include "dpmm_device.h"
 #include "dpmm_gauss.h"
+#include "dpmm_mixed.h"
 
 using namespace dpmm;
 
@@ -42,6 +43,10 @@ DPMM_GAUSS_EXTERN(double, 16)
 DPMM_GAUSS_EXTERN(double, 32)
 DPMM_GAUSS_EXTERN(double, 48)
 DPMM_GAUSS_EXTERN(double, 64)
+DPMM_MX_EXTERN(32)
+DPMM_MX_EXTERN(64)
+DPMM_MX_EXTERN(96)
+DPMM_MX_EXTERN(128)
 }  // namespace dpmm
 
 namespace {
@@ -318,6 +323,11 @@ __global__ void __launch_bounds__(kThreads) k_stats_reduce(const double* __restr
     stats[(size_t)q * E + e] = v;
 }
 
+__global__ void k_zero_int(int* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
 // ---- relabel -----------------------------------------------------------------------------------
 __global__ void __launch_bounds__(kThreads) k_relabel(int32_t* __restrict__ labels, int32_t* __restrict__ subs,
                                                       int64_t N, const int32_t* __restrict__ table, int K,
@@ -375,6 +385,14 @@ struct dpmm_ctx {
     size_t statsCap = 0;
     double* hStats = nullptr;
     size_t hStatsCap = 0;
+    // mixed precision: BF16 screen slots (padded dimension DPM, a multiple of 32) + recheck list
+    int DPM = 0;
+    void* dPB = nullptr;
+    size_t pbCap = 0;
+    std::vector<unsigned char> hPB;
+    int32_t* dRecheck = nullptr;
+    int* dRecheckCount = nullptr;
+    float tau = 1.0f / 32.0f;
     hipEvent_t ev[6] = {};
     float ms_label = 0, ms_sub = 0, ms_stats = 0;
     bool ev_label = false, ev_sub = false, ev_stats = false;
@@ -468,6 +486,7 @@ int create_common(const void* X, int64_t N, int D, int dtype, int prior, uint64_
     dpmm_ctx* c = new dpmm_ctx();
     c->N = N; c->D = D; c->dtype = dtype; c->prior = prior; c->seed = seed;
     c->DP = prior == DPMM_PRIOR_GAUSSIAN ? pick_dp(D) : D;
+    c->DPM = (D + 31) / 32 * 32;
     c->precision = dtype == DPMM_DTYPE_F64 ? DPMM_PREC_FP64 : DPMM_PREC_FP32;
     (void)hipGetDevice(&c->device);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -546,6 +565,24 @@ int ensure_param_buffers(dpmm_ctx* c, int K, size_t slot_bytes) {
     return DPMM_OK;
 }
 
+uint16_t f32_to_bf16_rne(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0;
+    return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+
+// BF16 screen slot of one cluster: W = R^T tiles in MFMA lane order, mu, score constant.
+void pack_mx_slot(int D, int DPM, const double* mu, const double* R, double cst, unsigned char* dst) {
+    std::memset(dst, 0, mx_slot_bytes(DPM));
+    uint16_t* tiles = reinterpret_cast<uint16_t*>(dst);
+    for (int j = 0; j < D; ++j)
+        for (int l = j; l < D; ++l) tiles[mx_tile_index(DPM, j, l)] = f32_to_bf16_rne((float)R[(size_t)l * D + j]);
+    float* m = reinterpret_cast<float*>(dst + mx_tiles_bytes(DPM));
+    for (int l = 0; l < D; ++l) m[l] = (float)mu[l];
+    m[DPM] = (float)cst;
+}
+
 int set_gauss(dpmm_ctx* c, int K, const double* mu, const double* mat, const double* logdet, const double* weights,
               const double* smu, const double* smat, const double* slogdet, const double* lrw, bool is_factor) {
     if (!c) return fail(DPMM_EINVAL, "null ctx");
@@ -573,11 +610,29 @@ int set_gauss(dpmm_ctx* c, int K, const double* mu, const double* mat, const dou
         else pack_gauss_slot<float>(D, DP, m, R.data(), cst, reinterpret_cast<float*>(dst));
         return true;
     };
+    const bool mixed = c->precision == DPMM_PREC_MIXED;
+    const int mslot = mx_slot_bytes(c->DPM);
+    if (mixed) c->hPB.assign((size_t)K * mslot, 0);
     for (int k = 0; k < K; ++k) {
         if (!std::isfinite(logdet[k])) return fail(DPMM_EINVAL, "non-finite logdet");
         if (!one(mu + (size_t)k * D, mat + (size_t)k * D * D, logdet[k], std::log(weights[k]),
                  c->hP.data() + (size_t)k * stride * esz))
             return fail(DPMM_ENOTSPD, "cluster precision matrix not SPD");
+        if (mixed)
+            pack_mx_slot(D, c->DPM, mu + (size_t)k * D, R.data(), -0.5 * logdet[k] + std::log(weights[k]),
+                         c->hPB.data() + (size_t)k * mslot);
+    }
+    if (mixed) {
+        unsigned char* pb = reinterpret_cast<unsigned char*>(c->dPB);
+        int rc2 = grow(pb, c->pbCap, c->hPB.size());
+        c->dPB = pb;
+        if (rc2) return rc2;
+        if (!c->dRecheck) {
+            if (hipMalloc((void**)&c->dRecheck, c->N * sizeof(int32_t)) != hipSuccess ||
+                hipMalloc((void**)&c->dRecheckCount, sizeof(int)) != hipSuccess)
+                return fail(DPMM_ENOMEM, "recheck alloc");
+        }
+        HIPCHK(hipMemcpyAsync(c->dPB, c->hPB.data(), c->hPB.size(), hipMemcpyHostToDevice, c->stream));
     }
     for (int q = 0; q < 2 * K; ++q) {
         if (!one(smu + (size_t)q * D, smat + (size_t)q * D * D, slogdet[q], std::log(lrw[q]),
@@ -642,6 +697,33 @@ template <int DP> void sg_f64(dpmm_ctx* c, uint32_t it) { run_sub_gauss<double, 
 template <int DP> void st_f32(dpmm_ctx* c) { run_stats_gauss<float, DP>(c); }
 template <int DP> void st_f64(dpmm_ctx* c) { run_stats_gauss<double, DP>(c); }
 
+template <int DP> void lgl_f32(dpmm_ctx* c, uint32_t it, int f, int grid) {
+    launch_label_gauss_list<float, DP>(c->stream, grid, reinterpret_cast<const float*>(c->dX), c->N, c->D, c->K,
+                                       reinterpret_cast<const float*>(c->dP), c->key0(), c->key1(), it, f,
+                                       c->dRecheck, c->dRecheckCount, c->dLabels);
+}
+
+// BF16 screen over all points, then the FP32 sweep over the points it could not decide.
+int run_label_mixed(dpmm_ctx* c, uint32_t iter, int fin) {
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, c->dRecheckCount, 1);
+    const float* X = reinterpret_cast<const float*>(c->dX);
+    const unsigned char* PB = reinterpret_cast<const unsigned char*>(c->dPB);
+    switch (c->DPM) {
+        case 32: launch_label_gauss_mx<32>(c->stream, c->N, c->D, c->K, X, PB, c->tau, c->key0(), c->key1(), iter, fin,
+                                           c->dLabels, c->dRecheck, c->dRecheckCount); break;
+        case 64: launch_label_gauss_mx<64>(c->stream, c->N, c->D, c->K, X, PB, c->tau, c->key0(), c->key1(), iter, fin,
+                                           c->dLabels, c->dRecheck, c->dRecheckCount); break;
+        case 96: launch_label_gauss_mx<96>(c->stream, c->N, c->D, c->K, X, PB, c->tau, c->key0(), c->key1(), iter, fin,
+                                           c->dLabels, c->dRecheck, c->dRecheckCount); break;
+        case 128: launch_label_gauss_mx<128>(c->stream, c->N, c->D, c->K, X, PB, c->tau, c->key0(), c->key1(), iter,
+                                             fin, c->dLabels, c->dRecheck, c->dRecheckCount); break;
+        default: return fail(DPMM_EINVAL, "mixed sweep needs D <= 128");
+    }
+    const int grid = (int)std::min<int64_t>(1024, (c->N + kThreads - 1) / kThreads);
+    DPMM_DISPATCH_DP(c->DP, lgl_f32, c, iter, fin, grid);
+    return DPMM_OK;
+}
+
 template <int KPT>
 void launch_label_mult_k(dpmm_ctx* c, uint32_t iter, int fin) {
     const int grid = (int)((c->N + 63) / 64);
@@ -692,7 +774,7 @@ int dpmm_destroy(dpmm_ctx* c) {
     if (c->owns_X && c->dX) (void)hipFree(c->dX);
     for (void* p : {(void*)c->dLabels, (void*)c->dSubs, (void*)c->dPerm, c->dP, c->dSP, (void*)c->dHist,
                     (void*)c->dRowtot, (void*)c->dLabelStart, (void*)c->dChunkStart, (void*)c->dTable,
-                    (void*)c->dPartials, (void*)c->dStats})
+                    (void*)c->dPartials, (void*)c->dStats, c->dPB, (void*)c->dRecheck, (void*)c->dRecheckCount})
         if (p) (void)hipFree(p);
     if (c->hStats) (void)hipHostFree(c->hStats);
     for (auto& e : c->ev)
@@ -708,7 +790,8 @@ int dpmm_set_precision(dpmm_ctx* c, int precision) {
         return fail(DPMM_EINVAL, "FP64 sweep needs f64 data");
     if (precision != DPMM_PREC_FP64 && c->dtype == DPMM_DTYPE_F64)
         return fail(DPMM_EINVAL, "f64 data sweeps in FP64");
-    if (precision == DPMM_PREC_MIXED) return fail(DPMM_EINVAL, "mixed precision not built in this library");
+    if (precision == DPMM_PREC_MIXED && (c->prior != DPMM_PRIOR_GAUSSIAN || c->dtype != DPMM_DTYPE_F32))
+        return fail(DPMM_EINVAL, "mixed precision is the Gaussian f32 sweep");
     c->precision = precision;
     c->K = 0;
     return DPMM_OK;
@@ -789,7 +872,10 @@ int dpmm_sample_cluster_labels(dpmm_ctx* c, uint32_t iter, int fin) {
     if (!c) return fail(DPMM_EINVAL, "null ctx");
     if (c->K <= 0) return fail(DPMM_ESTATE, "parameters not set");
     (void)hipEventRecord(c->ev[0], c->stream);
-    if (c->prior == DPMM_PRIOR_GAUSSIAN) {
+    if (c->prior == DPMM_PRIOR_GAUSSIAN && c->precision == DPMM_PREC_MIXED) {
+        int rc = run_label_mixed(c, iter, fin);
+        if (rc) return rc;
+    } else if (c->prior == DPMM_PRIOR_GAUSSIAN) {
         if (c->params_f64) { DPMM_DISPATCH_DP64(c->DP, lg_f64, c, iter, fin); }
         else { DPMM_DISPATCH_DP(c->DP, lg_f32, c, iter, fin); }
     } else {
@@ -901,6 +987,23 @@ int dpmm_last_kernel_ms(dpmm_ctx* c, float* a, float* b, float* s) {
     if (a) { *a = 0; if (c->ev_label) (void)hipEventElapsedTime(a, c->ev[0], c->ev[1]); }
     if (b) { *b = 0; if (c->ev_sub) (void)hipEventElapsedTime(b, c->ev[2], c->ev[3]); }
     if (s) { *s = 0; if (c->ev_stats) (void)hipEventElapsedTime(s, c->ev[4], c->ev[5]); }
+    return DPMM_OK;
+}
+
+int dpmm_set_mixed_tolerance(dpmm_ctx* c, float tau) {
+    if (!c || !(tau >= 0.0f) || !(tau < 0.5f)) return fail(DPMM_EINVAL, "tolerance must be in [0, 0.5)");
+    c->tau = tau;
+    return DPMM_OK;
+}
+
+int dpmm_last_recheck_count(dpmm_ctx* c, int64_t* n) {
+    if (!c || !n) return fail(DPMM_EINVAL, "bad args");
+    *n = -1;
+    if (!c->dRecheckCount) return DPMM_OK;
+    int v = 0;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(&v, c->dRecheckCount, sizeof(int), hipMemcpyDeviceToHost));
+    *n = v;
     return DPMM_OK;
 }
 

@@ -23,19 +23,16 @@ __device__ __forceinline__ bool chunk_of(int c, int K, const int32_t* __restrict
     return true;
 }
 
-// Gaussian label sweep, one point per thread (or per lane pair for wide points), the point in
-// registers; cluster parameters staged through LDS one cluster at a time.
+// Gaussian label sweep of one point per thread (or per lane pair for wide points), the point in
+// registers; cluster parameters staged through LDS one cluster at a time.  Every thread of the
+// workgroup must call it (it synchronises); threads without a point pass i >= N.
 template <typename T, int DP>
-__global__ void __launch_bounds__(kThreads) k_label_gauss(const T* __restrict__ X, int64_t N, int D, int K,
-                                                          const T* __restrict__ P, uint32_t key0, uint32_t key1,
-                                                          uint32_t iter, int final_iter, int32_t* __restrict__ labels) {
+__device__ __forceinline__ int label_point(const T* __restrict__ X, int64_t N, int D, int K, const T* __restrict__ P,
+                                           int64_t i, int par, T* ps, uint32_t key0, uint32_t key1, uint32_t iter,
+                                           int final_iter) {
     constexpr bool kPair = paired_layout<T, DP>();
-    constexpr int kPts = kPair ? kThreads / 2 : kThreads;
     constexpr int kCols = kPair ? DP / 2 : DP;
     constexpr int stride = gauss_slot_stride(DP);
-    __shared__ __align__(16) T ps[stride];
-    const int par = kPair ? (threadIdx.x & 1) : 0;
-    const int64_t i = (int64_t)blockIdx.x * kPts + (kPair ? (threadIdx.x >> 1) : threadIdx.x);
     T x[kCols];
 #pragma unroll
     for (int c = 0; c < kCols; ++c) {
@@ -56,11 +53,48 @@ __global__ void __launch_bounds__(kThreads) k_label_gauss(const T* __restrict__ 
         T s = ps[gauss_c_offset(DP)] - T(0.5) * q;
         if (!final_iter) {
             if ((k & 3) == 0) w = philox4x32_10((uint32_t)i, (uint32_t)(k >> 2), iter, TAG_LABEL, key0, key1);
-            s += gumbel<T>(sel4(w, k & 3));
+            s += gumbel<T>(w.v[0]);
+            rotate_words(w);
         }
         if (s > best || k == 0) { best = s; bi = k; }
     }
+    return bi;
+}
+
+template <typename T, int DP>
+__global__ void __launch_bounds__(kThreads) k_label_gauss(const T* __restrict__ X, int64_t N, int D, int K,
+                                                          const T* __restrict__ P, uint32_t key0, uint32_t key1,
+                                                          uint32_t iter, int final_iter, int32_t* __restrict__ labels) {
+    constexpr bool kPair = paired_layout<T, DP>();
+    constexpr int kPts = kPair ? kThreads / 2 : kThreads;
+    __shared__ __align__(16) T ps[gauss_slot_stride(DP)];
+    const int par = kPair ? (threadIdx.x & 1) : 0;
+    const int64_t i = (int64_t)blockIdx.x * kPts + (kPair ? (threadIdx.x >> 1) : threadIdx.x);
+    const int bi = label_point<T, DP>(X, N, D, K, P, i, par, ps, key0, key1, iter, final_iter);
     if (i < N && par == 0) labels[i] = bi;
+}
+
+// The same FP32/FP64 sweep over a list of point indices (the points the BF16 screen could not
+// decide); workgroups stride over the list, whose length is read from device memory.
+template <typename T, int DP>
+__global__ void __launch_bounds__(kThreads) k_label_gauss_list(const T* __restrict__ X, int64_t N, int D, int K,
+                                                               const T* __restrict__ P, uint32_t key0, uint32_t key1,
+                                                               uint32_t iter, int final_iter,
+                                                               const int32_t* __restrict__ list,
+                                                               const int* __restrict__ list_count,
+                                                               int32_t* __restrict__ labels) {
+    constexpr bool kPair = paired_layout<T, DP>();
+    constexpr int kPts = kPair ? kThreads / 2 : kThreads;
+    __shared__ __align__(16) T ps[gauss_slot_stride(DP)];
+    const int par = kPair ? (threadIdx.x & 1) : 0;
+    const int local = kPair ? (threadIdx.x >> 1) : threadIdx.x;
+    const int n = *list_count;
+    for (int b0 = blockIdx.x * kPts; b0 < n; b0 += gridDim.x * kPts) {
+        const int pos = b0 + local;
+        const int64_t i = pos < n ? (int64_t)list[pos] : N;
+        const int bi = label_point<T, DP>(X, N, D, K, P, i, par, ps, key0, key1, iter, final_iter);
+        if (i < N && par == 0) labels[i] = bi;
+    }
 }
 
 template <typename T, int DP>
@@ -214,6 +248,14 @@ void launch_label_gauss(hipStream_t st, int grid, const T* X, int64_t N, int D, 
 }
 
 template <typename T, int DP>
+void launch_label_gauss_list(hipStream_t st, int grid, const T* X, int64_t N, int D, int K, const T* P, uint32_t key0,
+                             uint32_t key1, uint32_t iter, int fin, const int32_t* list, const int* count,
+                             int32_t* labels) {
+    hipLaunchKernelGGL((k_label_gauss_list<T, DP>), dim3(grid), dim3(kThreads), 0, st, X, N, D, K, P, key0, key1,
+                       iter, fin, list, count, labels);
+}
+
+template <typename T, int DP>
 void launch_sub_gauss(hipStream_t st, int grid, const T* X, int D, int K, int CH, const int32_t* perm,
                       const int32_t* label_start, const int32_t* chunk_start, const T* SP, uint32_t key0,
                       uint32_t key1, uint32_t iter, int32_t* subs) {
@@ -232,6 +274,9 @@ void launch_stats_gauss(hipStream_t st, int grid, const TX* X, int D, int K, int
 #define DPMM_GAUSS_INSTANTIATE(T, DP)                                                                         \
     template void launch_label_gauss<T, DP>(hipStream_t, int, const T*, int64_t, int, int, const T*, uint32_t, \
                                             uint32_t, uint32_t, int, int32_t*);                                \
+    template void launch_label_gauss_list<T, DP>(hipStream_t, int, const T*, int64_t, int, int, const T*,      \
+                                                 uint32_t, uint32_t, uint32_t, int, const int32_t*, const int*, \
+                                                 int32_t*);                                                    \
     template void launch_sub_gauss<T, DP>(hipStream_t, int, const T*, int, int, int, const int32_t*,           \
                                           const int32_t*, const int32_t*, const T*, uint32_t, uint32_t,        \
                                           uint32_t, int32_t*);                                                 \
@@ -241,6 +286,9 @@ void launch_stats_gauss(hipStream_t st, int grid, const TX* X, int D, int K, int
 #define DPMM_GAUSS_EXTERN(T, DP)                                                                              \
     extern template void launch_label_gauss<T, DP>(hipStream_t, int, const T*, int64_t, int, int, const T*,    \
                                                    uint32_t, uint32_t, uint32_t, int, int32_t*);               \
+    extern template void launch_label_gauss_list<T, DP>(hipStream_t, int, const T*, int64_t, int, int,         \
+                                                        const T*, uint32_t, uint32_t, uint32_t, int,           \
+                                                        const int32_t*, const int*, int32_t*);                 \
     extern template void launch_sub_gauss<T, DP>(hipStream_t, int, const T*, int, int, int, const int32_t*,    \
                                                  const int32_t*, const int32_t*, const T*, uint32_t, uint32_t, \
                                                  uint32_t, int32_t*);                                          \

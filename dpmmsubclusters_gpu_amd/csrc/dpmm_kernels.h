@@ -50,6 +50,13 @@ template <> __device__ __forceinline__ double logit_u32<double>(uint32_t w) {
     return log((double)w + 0.5) - log((double)(0xFFFFFFFFu - w) + 0.5);
 }
 
+// Shift the next word of a Philox block into v[0] (static indices keep the block in registers).
+__device__ __forceinline__ void rotate_words(Ctr4& c) {
+    c.v[0] = c.v[1];
+    c.v[1] = c.v[2];
+    c.v[2] = c.v[3];
+}
+
 __device__ __forceinline__ uint32_t sel4(const Ctr4& c, int u) {
     const uint32_t a = (u & 1) ? c.v[1] : c.v[0];
     const uint32_t b = (u & 1) ? c.v[3] : c.v[2];

@@ -93,6 +93,14 @@ int dpmm_sufficient_statistics(dpmm_ctx* ctx, int K, int64_t* Nsub, double* sum,
  * constant 0 (3) / 1 (4).  Tables have K_old entries (new_label 2 K_old). */
 int dpmm_relabel(dpmm_ctx* ctx, uint32_t iter, int K_old, const int32_t* new_label, const int32_t* sub_action);
 
+/* Mixed precision (DPMM_PREC_MIXED): a point's BF16-screened label is kept when its best score,
+ * lowered by tau * q (q its screened Mahalanobis form), still beats every other cluster's score
+ * raised by tau * q_k; otherwise the point is re-scored exactly in FP32.  Default tau = 1/32.
+ * No counterpart in the reference (whose sweep is FP64 throughout). */
+int dpmm_set_mixed_tolerance(dpmm_ctx* ctx, float tau);
+/* Number of points the last mixed label sweep re-scored in FP32 (-1 if none ran). */
+int dpmm_last_recheck_count(dpmm_ctx* ctx, int64_t* n);
+
 /* Kernel timings of the last calls (ms, HIP events on the library's stream). */
 int dpmm_last_kernel_ms(dpmm_ctx* ctx, float* label_sweep_ms, float* sub_sweep_ms, float* stats_ms);
 int dpmm_synchronize(dpmm_ctx* ctx);

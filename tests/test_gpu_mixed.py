@@ -1,0 +1,89 @@
+"""The BF16-MFMA label sweep with FP32 recheck (DPMM_PREC_MIXED) against the FP32 sweep and the
+FP64 oracle.  Bar: labels identical to the FP32 sweep (the recheck makes the screen exact when
+its error bound holds), hence identical to the oracle wherever its margin exceeds 1e-6."""
+import numpy as np
+import pytest
+
+import dpmmsubclusters_gpu_amd as dp
+from dpmmsubclusters_gpu_amd.synthetic import gaussian_mixture
+import oracle_ctypes as orc
+from helpers import gaussian_params, have_gpu, say
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    if not have_gpu():
+        pytest.fail("no GPU visible to the HIP library")
+
+
+def _fitted_params(X, gt, K, rng):
+    D = X.shape[1]
+    mu = np.stack([X[gt == k].mean(0) for k in range(K)]).astype(np.float64)
+    lam = np.stack([np.linalg.inv(np.cov(X[gt == k].T.astype(np.float64)) + 1e-3 * np.eye(D)) for k in range(K)])
+    ld = np.array([-np.linalg.slogdet(l)[1] for l in lam])
+    w = np.bincount(gt, minlength=K) / len(gt)
+    smu = np.repeat(mu, 2, axis=0) + rng.normal(size=(2 * K, D))
+    return mu, lam, ld, w, smu, np.repeat(lam, 2, axis=0), np.repeat(ld, 2), np.full(2 * K, 0.5)
+
+
+@pytest.mark.parametrize("D,K,close", [(32, 12, False), (64, 20, False), (64, 8, True), (100, 6, False),
+                                       (128, 5, False), (20, 7, True)])
+@pytest.mark.parametrize("final", [False, True])
+def test_mixed_equals_fp32_and_oracle(D, K, close, final):
+    rng = np.random.default_rng(D + K)
+    N = 60_000
+    spread = 8.0 if close else 100.0   # close: overlapping clusters, many near-ties
+    X, gt, _ = gaussian_mixture(N, D, K, seed=D * 3 + K, spread=spread, dtype=np.float32)
+    params = _fitted_params(X, gt, K, rng)
+    say(f"mixed sweep D={D} K={K} close={close} final={final}")
+    dev = dp.DeviceSweep(X, dp.prior_type.Gaussian, seed=9)
+    dev.set_gaussian_params(*params)
+    dev.sample_cluster_labels(3, final)
+    ref, _ = dev.get_labels()
+    dev.set_precision(dp.Precision.MIXED)
+    dev.set_gaussian_params(*params)
+    dev.set_labels(np.full(N, -7, np.int32), None)
+    dev.sample_cluster_labels(3, final)
+    mx, _ = dev.get_labels()
+    nre = dev.last_recheck_count()
+    print(f"D={D} K={K} close={close}: rechecked {nre} of {N} ({nre / N:.3%})")
+    assert (mx >= 0).all(), "a point got no label"
+    np.testing.assert_array_equal(mx, ref)
+    olab, margin = orc.label_sweep(0, X, params[0], params[1], params[2], weights=params[3], seed=9, it=3, final=final)
+    bad = (mx != olab) & (margin >= 1e-6)
+    assert not bad.any()
+
+
+def test_screen_alone_flip_rate():
+    """tau = 0 keeps every screened label: the raw BF16 flip rate against the oracle, reported."""
+    rng = np.random.default_rng(1)
+    D, K, N = 64, 20, 100_000
+    X, gt, _ = gaussian_mixture(N, D, K, seed=5, dtype=np.float32)
+    params = _fitted_params(X, gt, K, rng)
+    dev = dp.DeviceSweep(X, dp.prior_type.Gaussian, seed=2)
+    dev.set_precision(dp.Precision.MIXED)
+    dev.set_mixed_tolerance(0.0)
+    dev.set_gaussian_params(*params)
+    say("mixed screen only D=64 K=20")
+    dev.sample_cluster_labels(4)
+    mx, _ = dev.get_labels()
+    olab, margin = orc.label_sweep(0, X, params[0], params[1], params[2], weights=params[3], seed=2, it=4)
+    rate = (mx != olab).mean()
+    print(f"BF16 screen alone: flip rate {rate:.2e}, rechecked {dev.last_recheck_count()}")
+    assert rate < 1e-2
+
+
+def test_mixed_sampler_end_to_end():
+    X, gt, _ = gaussian_mixture(80_000, 64, 10, seed=3, a=1.0, ridge=0.5, weight=0.5, dtype=np.float32)
+    D = 64
+    say("mixed end to end D=64")
+    model = dp.dp_parallel_sampling_class(X.shape[0], X.T, 4, dp.prior_type.Gaussian)
+    hyper = dp.niw_hyperparams(1.0, np.zeros(D), D + 3.0, np.eye(D))
+    mi = model.dp_parallel(hyper, 10.0, iters=120, verbose=False, precision=dp.Precision.MIXED)
+    ref = model.dp_parallel(hyper, 10.0, iters=120, verbose=False, precision=dp.Precision.FP32)
+    nm, nr = dp._nmi(gt, mi.labels), dp._nmi(gt, ref.labels)
+    print("mixed K", mi.cluster_count_history[-1], "nmi", nm, "fp32 K", ref.cluster_count_history[-1], "nmi", nr)
+    assert abs(nm - nr) < 0.01
+    np.testing.assert_array_equal(mi.labels, ref.labels)   # same draws, exact screen -> same trajectory

@@ -143,6 +143,105 @@ __global__ void __launch_bounds__(kThreads) k_sub_gauss(const T* __restrict__ X,
     }
 }
 
+// FP64 MFMA form of the same partial (v_mfma_f64_16x16x4f64): the scatter matrix of DP x DP is
+// cut into 16 x 16 tiles, the upper-triangle tiles are dealt round robin to the four waves, and
+// each MFMA adds the outer products of four points (A = their 16 coordinates of the tile's row
+// slice, B = those of its column slice; lane L holds point L / 16, coordinate L % 16).  Points of
+// the chunk with sub-label s are staged in LDS (compacted by ballot), zero-padded to a multiple
+// of four.  Output row of lane L, accumulator v: 4 (L / 16) + v, column L % 16.
+template <typename TX, int DP>
+__global__ void __launch_bounds__(kThreads) k_stats_gauss_mfma(const TX* __restrict__ X, int D, int K, int CH,
+                                                               const int32_t* __restrict__ perm,
+                                                               const int32_t* __restrict__ label_start,
+                                                               const int32_t* __restrict__ chunk_start,
+                                                               const int32_t* __restrict__ subs,
+                                                               double* __restrict__ partials) {
+    typedef double f64x4 __attribute__((ext_vector_type(4)));
+    constexpr int T = DP / 16;
+    constexpr int NTILE = T * (T + 1) / 2;
+    constexpr int TPW = (NTILE + 3) / 4;            // tiles per wave
+    constexpr int BATCH = DP > 64 ? 32 : 64;
+    constexpr size_t E = 1 + DP + (size_t)DP * DP;
+    __shared__ __align__(16) double xs[BATCH * DP];
+    __shared__ int idl[64];
+    __shared__ int nsel;
+    const int c = blockIdx.x >> 1, s = blockIdx.x & 1;
+    int k, beg, end;
+    if (!chunk_of(c, K, chunk_start, label_start, CH, k, beg, end)) return;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int kp = lane >> 4, r = lane & 15;
+    int ti[TPW], tj[TPW];
+#pragma unroll
+    for (int u = 0; u < TPW; ++u) {
+        int t = wv + 4 * u;
+        ti[u] = -1; tj[u] = -1;
+        if (t < NTILE) {
+            int row = 0;
+            while (t >= T - row) { t -= T - row; ++row; }
+            ti[u] = row; tj[u] = row + t;
+        }
+    }
+    f64x4 acc[TPW];
+#pragma unroll
+    for (int u = 0; u < TPW; ++u) acc[u] = f64x4{0.0, 0.0, 0.0, 0.0};
+    double sx = 0.0;
+    int64_t cnt = 0;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int b0 = beg; b0 < end; b0 += BATCH) {
+        if (wv == 0) {
+            const int p = b0 + lane;
+            int idx = -1;
+            bool f = false;
+            if (lane < BATCH && p < end) { idx = perm[p]; f = subs[idx] == s; }
+            const uint64_t m = __ballot(f);
+            if (f) idl[__popcll(m & lt)] = idx;
+            if (lane == 0) nsel = __popcll(m);
+        }
+        __syncthreads();
+        const int n = nsel;
+        const int n4 = (n + 3) & ~3;
+        for (int e = threadIdx.x; e < n4 * DP; e += kThreads) {
+            const int pp = e / DP, l = e - pp * DP;
+            xs[pp * DP + l] = (pp < n && l < D) ? (double)X[(int64_t)idl[pp] * D + l] : 0.0;
+        }
+        __syncthreads();
+        for (int g = 0; g < n4; g += 4) {
+            double xr[T];
+#pragma unroll
+            for (int t = 0; t < T; ++t) xr[t] = xs[(g + kp) * DP + 16 * t + r];
+#pragma unroll
+            for (int u = 0; u < TPW; ++u) {
+                if (ti[u] < 0) continue;
+                double a = xr[0], b = xr[0];
+#pragma unroll
+                for (int t = 1; t < T; ++t) {   // register selects keep xr[] out of scratch
+                    a = ti[u] == t ? xr[t] : a;
+                    b = tj[u] == t ? xr[t] : b;
+                }
+                acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[u], 0, 0, 0);
+            }
+        }
+        if (threadIdx.x < DP)
+            for (int pp = 0; pp < n; ++pp) sx += xs[pp * DP + threadIdx.x];
+        cnt += n;
+        __syncthreads();
+    }
+    double* out = partials + (size_t)blockIdx.x * E;
+    if (threadIdx.x == 0) out[0] = (double)cnt;
+    if (threadIdx.x < DP) out[1 + threadIdx.x] = sx;
+    double* S = out + 1 + DP;
+#pragma unroll
+    for (int u = 0; u < TPW; ++u) {
+        if (ti[u] < 0) continue;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int a = 16 * ti[u] + 4 * kp + v, b = 16 * tj[u] + r;
+            S[(size_t)a * DP + b] = acc[u][v];
+            S[(size_t)b * DP + a] = acc[u][v];
+        }
+    }
+}
+
 // Partial (count, sum, scatter) of the points of sub-cluster s in one chunk, FP64.  Threads own
 // 4x4 blocks of the upper block triangle; points staged in LDS in batches, compacted by ballot.
 template <typename TX, int DP>
@@ -267,8 +366,12 @@ template <typename TX, int DP>
 void launch_stats_gauss(hipStream_t st, int grid, const TX* X, int D, int K, int CH, const int32_t* perm,
                         const int32_t* label_start, const int32_t* chunk_start, const int32_t* subs,
                         double* partials) {
-    hipLaunchKernelGGL((k_stats_gauss<TX, DP>), dim3(grid), dim3(kThreads), 0, st, X, D, K, CH, perm, label_start,
-                       chunk_start, subs, partials);
+    if constexpr (DP % 16 == 0)
+        hipLaunchKernelGGL((k_stats_gauss_mfma<TX, DP>), dim3(grid), dim3(kThreads), 0, st, X, D, K, CH, perm,
+                           label_start, chunk_start, subs, partials);
+    else
+        hipLaunchKernelGGL((k_stats_gauss<TX, DP>), dim3(grid), dim3(kThreads), 0, st, X, D, K, CH, perm, label_start,
+                           chunk_start, subs, partials);
 }
 
 #define DPMM_GAUSS_INSTANTIATE(T, DP)                                                                         \

@@ -32,19 +32,32 @@ __host__ __device__ constexpr int mx_tile_index(int DP, int j, int l) {
     return (((j / 32) * (DP / 16) + (l / 16)) * 64 + (j % 32) + 32 * ((l % 16) / 8)) * 8 + (l % 8);
 }
 
+// Screen's Gumbel noise: hardware log2, and -log(u) = v + v^2 / 2 for v = 1 - u < 2^-10.  Its
+// absolute error stays far below kNoiseSlack, which the error bound adds to every cluster.
+constexpr float kNoiseSlack = 1e-3f;
+__device__ __forceinline__ float gumbel_screen(uint32_t w) {
+    const float v = ((float)(0xFFFFFFFFu - w) + 0.5f) * 0x1p-32f;
+    float e;
+    if (v < 0x1p-10f) e = v * (1.0f + 0.5f * v);
+    else e = -__logf(((float)w + 0.5f) * 0x1p-32f);
+    return -__logf(e);
+}
+
 template <int DP>
 __global__ void __launch_bounds__(kThreads, 2)
 k_label_gauss_mx(const float* __restrict__ X, int64_t N, int D, int K, const unsigned char* __restrict__ PB,
                  float tau, uint32_t key0, uint32_t key1, uint32_t iter, int final_iter,
                  int32_t* __restrict__ labels, int32_t* __restrict__ recheck, int* __restrict__ recheck_count) {
     constexpr int RB = DP / 32, KC = DP / 16;
-    constexpr int NB = DP <= 64 ? 2 : 1;          // 32-point blocks per wave
-    constexpr int PTS_W = 32 * NB;
-    constexpr int PTS = 4 * PTS_W;
+    // 32-point blocks per wave: with two, half h of the wave scores block h; with one (wide points,
+    // where two blocks of coordinates would not fit in registers) both halves score the same block.
+    constexpr int NB = DP <= 64 ? 2 : 1;
+    constexpr int PTS = 4 * 32 * NB;
     constexpr int SLOT = mx_slot_bytes(DP);
     __shared__ __align__(16) unsigned char sp[2][SLOT];
     const int lane = threadIdx.x & 63, h = lane >> 5, pl = lane & 31, wv = threadIdx.x >> 6;
-    const int64_t base = (int64_t)blockIdx.x * PTS + wv * PTS_W + pl;
+    const int64_t base = (int64_t)blockIdx.x * PTS + wv * 32 * NB + pl;   // point of block nb: base + 32 nb
+    const int64_t own = NB == 2 ? base + 32 * h : base;                  // the point this lane scores
 
     float x[NB][KC][8];
 #pragma unroll
@@ -58,11 +71,9 @@ k_label_gauss_mx(const float* __restrict__ X, int64_t N, int D, int K, const uns
                 x[nb][kc][e] = (i < N && l < D) ? X[i * D + l] : 0.f;
             }
     }
-    float sb[NB], eb[NB], U[NB];
-    int bi[NB];
-#pragma unroll
-    for (int nb = 0; nb < NB; ++nb) { sb[nb] = neg_inf<float>(); eb[nb] = 0.f; U[nb] = neg_inf<float>(); bi[nb] = 0; }
-    Ctr4 w[NB];
+    float sb = neg_inf<float>(), eb = 0.f, U = neg_inf<float>();
+    int bi = 0;
+    Ctr4 w = {};
 
     auto stage = [&](int k, int buf) {
         const uint4* src = reinterpret_cast<const uint4*>(PB + (size_t)k * SLOT);
@@ -77,6 +88,7 @@ k_label_gauss_mx(const float* __restrict__ X, int64_t N, int D, int K, const uns
         const bf16x8* tiles = reinterpret_cast<const bf16x8*>(slot);
         const float* mu = reinterpret_cast<const float*>(slot + mx_tiles_bytes(DP));
         const float c = mu[DP];
+        float qp[NB];
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) {
             bf16x8 Bv[KC];
@@ -95,39 +107,38 @@ k_label_gauss_mx(const float* __restrict__ X, int64_t N, int D, int K, const uns
 #pragma unroll
                 for (int v = 0; v < 16; ++v) q = fmaf(acc[v], acc[v], q);
             }
-            q += __shfl_xor(q, 32);
-            float s = c - 0.5f * q;
-            const float err = tau * q;
-            if (!final_iter) {
-                if ((k & 3) == 0)
-                    w[nb] = philox4x32_10((uint32_t)(base + 32 * nb), (uint32_t)(k >> 2), iter, TAG_LABEL, key0, key1);
-                s += gumbel_f32(w[nb].v[0]);
-                rotate_words(w[nb]);
-            }
-            if (s > sb[nb] || k == 0) {
-                U[nb] = fmaxf(U[nb], sb[nb] + eb[nb]);
-                sb[nb] = s; eb[nb] = err; bi[nb] = k;
-            } else {
-                U[nb] = fmaxf(U[nb], s + err);
-            }
+            qp[nb] = q;   // this half's rows of block nb
+        }
+        // half h keeps block h: it sends its rows of the other block and receives the other half's rows of its own
+        float q;
+        if constexpr (NB == 2) q = (h ? qp[1] : qp[0]) + __shfl_xor(h ? qp[0] : qp[1], 32);
+        else q = qp[0] + __shfl_xor(qp[0], 32);
+        float s = c - 0.5f * q;
+        const float err = tau * q + kNoiseSlack;
+        if (!final_iter) {
+            if ((k & 3) == 0) w = philox4x32_10((uint32_t)own, (uint32_t)(k >> 2), iter, TAG_LABEL, key0, key1);
+            s += gumbel_screen(w.v[0]);
+            rotate_words(w);
+        }
+        if (s > sb || k == 0) {
+            U = fmaxf(U, sb + eb);
+            sb = s; eb = err; bi = k;
+        } else {
+            U = fmaxf(U, s + err);
         }
         __syncthreads();
     }
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-#pragma unroll
-    for (int nb = 0; nb < NB; ++nb) {
-        const int64_t i = base + 32 * nb;
-        const bool valid = i < N && h == 0;
-        const bool sure = sb[nb] - eb[nb] > U[nb];
-        if (valid && sure) labels[i] = bi[nb];
-        const bool need = valid && !sure;
-        const uint64_t m = __ballot(need);
-        if (m) {
-            int pos0 = 0;
-            if (lane == 0) pos0 = atomicAdd(recheck_count, __popcll(m));   // one atomic per wave
-            pos0 = __shfl(pos0, 0);
-            if (need) recheck[pos0 + __popcll(m & lt)] = (int32_t)i;
-        }
+    const bool valid = own < N && (NB == 2 || h == 0);
+    const bool sure = sb - eb > U;
+    if (valid && sure) labels[own] = bi;
+    const bool need = valid && !sure;
+    const uint64_t m = __ballot(need);
+    if (m) {
+        int pos0 = 0;
+        if (lane == 0) pos0 = atomicAdd(recheck_count, __popcll(m));   // one atomic per wave
+        pos0 = __shfl(pos0, 0);
+        if (need) recheck[pos0 + __popcll(m & lt)] = (int32_t)own;
     }
 }
 

@@ -148,7 +148,7 @@ __global__ void __launch_bounds__(kThreads) k_sub_gauss(const T* __restrict__ X,
 // each MFMA adds the outer products of four points (A = their 16 coordinates of the tile's row
 // slice, B = those of its column slice; lane L holds point L / 16, coordinate L % 16).  Points of
 // the chunk with sub-label s are staged in LDS (compacted by ballot), zero-padded to a multiple
-// of four.  Output row of lane L, accumulator v: 4 (L / 16) + v, column L % 16.
+// of four.  Output row of lane L, accumulator v: 4 v + L / 16, column L % 16 (checked on gfx950).
 template <typename TX, int DP>
 __global__ void __launch_bounds__(kThreads) k_stats_gauss_mfma(const TX* __restrict__ X, int D, int K, int CH,
                                                                const int32_t* __restrict__ perm,
@@ -235,7 +235,7 @@ __global__ void __launch_bounds__(kThreads) k_stats_gauss_mfma(const TX* __restr
         if (ti[u] < 0) continue;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            const int a = 16 * ti[u] + 4 * kp + v, b = 16 * tj[u] + r;
+            const int a = 16 * ti[u] + 4 * v + kp, b = 16 * tj[u] + r;
             S[(size_t)a * DP + b] = acc[u][v];
             S[(size_t)b * DP + a] = acc[u][v];
         }

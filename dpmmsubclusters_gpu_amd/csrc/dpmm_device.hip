@@ -393,6 +393,8 @@ struct dpmm_ctx {
     int32_t* dRecheck = nullptr;
     int* dRecheckCount = nullptr;
     float tau = 1.0f / 32.0f;
+    // statistics already reduced into dStats by the fused sub-label pass, for Kpartition clusters
+    bool stats_cached = false;
     hipEvent_t ev[6] = {};
     float ms_label = 0, ms_sub = 0, ms_stats = 0;
     bool ev_label = false, ev_sub = false, ev_stats = false;
@@ -464,6 +466,7 @@ int build_partition(dpmm_ctx* c, int K) {
     HIPCHK(hipGetLastError());
     c->partition_valid = true;
     c->Kpartition = K;
+    c->stats_cached = false;
     return DPMM_OK;
 }
 
@@ -695,6 +698,14 @@ template <int DP> void lg_f64(dpmm_ctx* c, uint32_t it, int f) { run_label_gauss
 template <int DP> void sg_f32(dpmm_ctx* c, uint32_t it) { run_sub_gauss<float, DP>(c, it); }
 template <int DP> void sg_f64(dpmm_ctx* c, uint32_t it) { run_sub_gauss<double, DP>(c, it); }
 template <int DP> void st_f32(dpmm_ctx* c) { run_stats_gauss<float, DP>(c); }
+template <typename T, int DP>
+void run_sub_stats_gauss(dpmm_ctx* c, uint32_t iter) {
+    launch_sub_stats_gauss<T, DP>(c->stream, chunks_upper(c, c->K), reinterpret_cast<const T*>(c->dX), c->D, c->K,
+                                  c->CH, c->dPerm, c->dLabelStart, c->dChunkStart, reinterpret_cast<const T*>(c->dSP),
+                                  c->key0(), c->key1(), iter, c->dSubs, c->dPartials);
+}
+template <int DP> void ss_f32(dpmm_ctx* c, uint32_t it) { run_sub_stats_gauss<float, DP>(c, it); }
+template <int DP> void ss_f64(dpmm_ctx* c, uint32_t it) { run_sub_stats_gauss<double, DP>(c, it); }
 template <int DP> void st_f64(dpmm_ctx* c) { run_stats_gauss<double, DP>(c); }
 
 template <int DP> void lgl_f32(dpmm_ctx* c, uint32_t it, int f, int grid) {
@@ -805,6 +816,7 @@ int dpmm_sample_initial_labels(dpmm_ctx* c, int K0) {
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));
     c->partition_valid = false;
+    c->stats_cached = false;
     return DPMM_OK;
 }
 
@@ -813,6 +825,7 @@ int dpmm_set_labels(dpmm_ctx* c, const int32_t* labels, const int32_t* subs) {
     if (labels) HIPCHK(hipMemcpy(c->dLabels, labels, c->N * sizeof(int32_t), hipMemcpyHostToDevice));
     if (subs) HIPCHK(hipMemcpy(c->dSubs, subs, c->N * sizeof(int32_t), hipMemcpyHostToDevice));
     c->partition_valid = false;
+    c->stats_cached = false;
     return DPMM_OK;
 }
 
@@ -886,6 +899,7 @@ int dpmm_sample_cluster_labels(dpmm_ctx* c, uint32_t iter, int fin) {
     HIPCHK(hipGetLastError());
     c->ev_label = true;
     c->partition_valid = false;
+    c->stats_cached = false;
     return DPMM_OK;
 }
 
@@ -897,7 +911,21 @@ int dpmm_sample_sub_labels(dpmm_ctx* c, uint32_t iter) {
         if (rc) return rc;
     }
     (void)hipEventRecord(c->ev[2], c->stream);
-    if (c->prior == DPMM_PRIOR_GAUSSIAN) {
+    c->stats_cached = false;
+    const bool fused = c->prior == DPMM_PRIOR_GAUSSIAN && c->DP % 16 == 0;
+    if (fused) {   // sub-labels and their statistics in one pass over the segments
+        const size_t E = c->statEntries();
+        const int K = c->K;
+        int rc = grow(c->dPartials, c->partialsCap, (size_t)2 * chunks_upper(c, K) * E);
+        if (rc) return rc;
+        rc = grow(c->dStats, c->statsCap, (size_t)2 * K * E);
+        if (rc) return rc;
+        if (c->params_f64) { DPMM_DISPATCH_DP64(c->DP, ss_f64, c, iter); }
+        else { DPMM_DISPATCH_DP(c->DP, ss_f32, c, iter); }
+        hipLaunchKernelGGL(k_stats_reduce, dim3(2 * K, (unsigned)((E + kThreads - 1) / kThreads)), dim3(kThreads), 0,
+                           c->stream, c->dPartials, c->dChunkStart, K, (int)E, c->dStats);
+        c->stats_cached = true;
+    } else if (c->prior == DPMM_PRIOR_GAUSSIAN) {
         if (c->params_f64) { DPMM_DISPATCH_DP64(c->DP, sg_f64, c, iter); }
         else { DPMM_DISPATCH_DP(c->DP, sg_f32, c, iter); }
     } else {
@@ -921,21 +949,24 @@ int dpmm_sufficient_statistics(dpmm_ctx* c, int K, int64_t* Nsub, double* sum, d
     }
     const size_t E = c->statEntries();
     const int nch = chunks_upper(c, K);
-    int rc = grow(c->dPartials, c->partialsCap, (size_t)2 * nch * E);
-    if (rc) return rc;
-    rc = grow(c->dStats, c->statsCap, (size_t)2 * K * E);
-    if (rc) return rc;
     (void)hipEventRecord(c->ev[4], c->stream);
-    if (c->prior == DPMM_PRIOR_GAUSSIAN) {
-        if (c->dtype == DPMM_DTYPE_F64) { DPMM_DISPATCH_DP64(c->DP, st_f64, c); }
-        else { DPMM_DISPATCH_DP(c->DP, st_f32, c); }
-    } else {
-        hipLaunchKernelGGL(k_stats_mult, dim3(2 * nch), dim3(kThreads), 0, c->stream,
-                           reinterpret_cast<const int32_t*>(c->dX), c->D, K, c->CH, c->dPerm, c->dLabelStart,
-                           c->dChunkStart, c->dSubs, (int)E, c->dPartials);
+    if (!(c->stats_cached && c->Kpartition == K)) {
+        int rc = grow(c->dPartials, c->partialsCap, (size_t)2 * nch * E);
+        if (rc) return rc;
+        rc = grow(c->dStats, c->statsCap, (size_t)2 * K * E);
+        if (rc) return rc;
+        if (c->prior == DPMM_PRIOR_GAUSSIAN) {
+            if (c->dtype == DPMM_DTYPE_F64) { DPMM_DISPATCH_DP64(c->DP, st_f64, c); }
+            else { DPMM_DISPATCH_DP(c->DP, st_f32, c); }
+        } else {
+            hipLaunchKernelGGL(k_stats_mult, dim3(2 * nch), dim3(kThreads), 0, c->stream,
+                               reinterpret_cast<const int32_t*>(c->dX), c->D, K, c->CH, c->dPerm, c->dLabelStart,
+                               c->dChunkStart, c->dSubs, (int)E, c->dPartials);
+        }
+        hipLaunchKernelGGL(k_stats_reduce, dim3(2 * K, (unsigned)((E + kThreads - 1) / kThreads)), dim3(kThreads), 0,
+                           c->stream, c->dPartials, c->dChunkStart, K, (int)E, c->dStats);
+        c->stats_cached = true;
     }
-    hipLaunchKernelGGL(k_stats_reduce, dim3(2 * K, (unsigned)((E + kThreads - 1) / kThreads)), dim3(kThreads), 0,
-                       c->stream, c->dPartials, c->dChunkStart, K, (int)E, c->dStats);
     (void)hipEventRecord(c->ev[5], c->stream);
     HIPCHK(hipGetLastError());
     c->ev_stats = true;
@@ -978,6 +1009,7 @@ int dpmm_relabel(dpmm_ctx* c, uint32_t iter, int K_old, const int32_t* new_label
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));
     if (labels_change) c->partition_valid = false;
+    c->stats_cached = false;
     return DPMM_OK;
 }
 

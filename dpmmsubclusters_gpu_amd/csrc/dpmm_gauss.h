@@ -243,9 +243,9 @@ __global__ void __launch_bounds__(kThreads) k_stats_gauss_mfma(const TX* __restr
 }
 
 // Fused sub-label sweep and statistics of one chunk (points of one cluster in perm order): each
-// batch of 64 points is staged once in LDS; 256 lanes score the 64 points against both
-// sub-clusters (lane pairs, 128 lanes per sub-cluster), lanes 0-63 draw the sub-labels, then the
-// four waves accumulate the FP64 scatter tiles of both sub-clusters on MFMA (the layout of
+// batch of 128 points is staged once in LDS; 256 lanes score them against both sub-clusters (lane
+// pairs, 128 lanes per sub-cluster, two rounds), wave 0 draws the sub-labels and compacts each
+// sub-cluster's points (index order kept), then the four waves accumulate the FP64 scatter tiles of both sub-clusters on MFMA (the layout of
 // k_stats_gauss_mfma) together with sums and counts.  Writes the partials of (chunk, 0) and
 // (chunk, 1) in the layout k_stats_reduce reads.
 template <typename T, int DP>
@@ -269,7 +269,7 @@ __global__ void __launch_bounds__(kThreads) k_sub_stats_gauss(const T* __restric
     __shared__ __align__(16) T xs[B * RS];
     __shared__ T sc[2][B];
     __shared__ int idb[B];
-    __shared__ int lst[2][B];
+    __shared__ int lst[2][B + 4];
     __shared__ int nls[2];
     int k, beg, end;
     if (!chunk_of(blockIdx.x, K, chunk_start, label_start, CH, k, beg, end)) return;
@@ -277,7 +277,7 @@ __global__ void __launch_bounds__(kThreads) k_sub_stats_gauss(const T* __restric
     for (int e = threadIdx.x; e < 2 * stride; e += kThreads) ps[e] = src[e];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     // scoring role: sub-cluster hs, point pp, lane parity par
-    const int hs = threadIdx.x >> 7, pp = (threadIdx.x & 127) >> 1, par = threadIdx.x & 1;
+    const int hs = threadIdx.x >> 7, pp0 = (threadIdx.x & 127) >> 1, par = threadIdx.x & 1;
     // statistics role: (sub-cluster, tile) pairs of this wave
     const int kp = lane >> 4, r = lane & 15;
     int us[TPW], ti[TPW], tj[TPW];
@@ -296,7 +296,7 @@ __global__ void __launch_bounds__(kThreads) k_sub_stats_gauss(const T* __restric
     f64x4 acc[TPW];
 #pragma unroll
     for (int u = 0; u < TPW; ++u) acc[u] = f64x4{0.0, 0.0, 0.0, 0.0};
-    double sx = 0.0;                  // thread t < 2 DP: sub-cluster t / DP, coordinate t % DP
+    double sx[4] = {0.0, 0.0, 0.0, 0.0};   // thread t < 2 DP: sub-cluster t / DP, coordinate t % DP
     int64_t cnt0 = 0, cnt1 = 0;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     for (int b0 = beg; b0 < end; b0 += B) {
@@ -308,7 +308,9 @@ __global__ void __launch_bounds__(kThreads) k_sub_stats_gauss(const T* __restric
             xs[q * RS + l] = (q < n && l < D) ? X[(int64_t)idb[q] * D + l] : T(0);
         }
         __syncthreads();
-        {   // score point pp against sub-cluster hs
+#pragma unroll 1
+        for (int rd = 0; rd < B / 64; ++rd) {   // score point pp against sub-cluster hs
+            const int pp = pp0 + 64 * rd;
             T xh[HC];
 #pragma unroll
             for (int c = 0; c < HC; ++c) xh[c] = xs[pp * RS + pair_col(c, 0) + (par << 2)];
@@ -317,19 +319,26 @@ __global__ void __launch_bounds__(kThreads) k_sub_stats_gauss(const T* __restric
             if (par == 0) sc[hs][pp] = pps[gauss_c_offset(DP)] - T(0.5) * q;
         }
         __syncthreads();
-        if (threadIdx.x < B) {   // wave 0: draw, write, compact by sub-label
-            const int q = threadIdx.x;
-            int sb = -1;
-            if (q < n) {
-                const int idx = idb[q];
-                const uint32_t w = philox4x32_10((uint32_t)idx, 0, iter, TAG_SUB, key0, key1).v[0];
-                sb = (logit_u32<T>(w) < sc[0][q] - sc[1][q]) ? 0 : 1;
-                subs[idx] = sb;
+        if (wv == 0) {   // draw, write, compact by sub-label (index order kept)
+            int base0 = 0, base1 = 0;
+#pragma unroll 1
+            for (int rd = 0; rd < B / 64; ++rd) {
+                const int q = lane + 64 * rd;
+                int sb = -1;
+                if (q < n) {
+                    const int idx = idb[q];
+                    const uint32_t w = philox4x32_10((uint32_t)idx, 0, iter, TAG_SUB, key0, key1).v[0];
+                    sb = (logit_u32<T>(w) < sc[0][q] - sc[1][q]) ? 0 : 1;
+                    subs[idx] = sb;
+                }
+                const uint64_t m0 = __ballot(sb == 0), m1 = __ballot(sb == 1);
+                if (sb == 0) lst[0][base0 + __popcll(m0 & lt)] = q;
+                if (sb == 1) lst[1][base1 + __popcll(m1 & lt)] = q;
+                base0 += __popcll(m0);
+                base1 += __popcll(m1);
             }
-            const uint64_t m0 = __ballot(sb == 0), m1 = __ballot(sb == 1);
-            if (sb == 0) lst[0][__popcll(m0 & lt)] = q;
-            if (sb == 1) lst[1][__popcll(m1 & lt)] = q;
-            if (q == 0) { nls[0] = __popcll(m0); nls[1] = __popcll(m1); }
+            if (lane < 4) { lst[0][base0 + lane] = B - 1; lst[1][base1 + lane] = B - 1; }   // in-range pads
+            if (lane == 0) { nls[0] = base0; nls[1] = base1; }
         }
         __syncthreads();
         const int n0 = nls[0], n1 = nls[1];
@@ -340,21 +349,26 @@ __global__ void __launch_bounds__(kThreads) k_sub_stats_gauss(const T* __restric
             if (us[u] < 0) continue;
             const int ns = us[u] ? n1 : n0;
             const int* L = lst[us[u]];
+            const int ca = 16 * ti[u] + r, cb = 16 * tj[u] + r;
             for (int g = 0; g < ns; g += 4) {
                 const int q = g + kp;
-                double a = 0.0, bb = 0.0;
-                if (q < ns) {
-                    const T* row = xs + L[q] * RS;
-                    a = (double)row[16 * ti[u] + r];
-                    bb = (double)row[16 * tj[u] + r];
-                }
+                const int row = L[q] * RS;     // pads point at a valid row; masked below
+                const bool in = q < ns;
+                const double a = in ? (double)xs[row + ca] : 0.0;
+                const double bb = in ? (double)xs[row + cb] : 0.0;
                 acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bb, acc[u], 0, 0, 0);
             }
         }
         if (threadIdx.x < 2 * DP) {
             const int s2 = threadIdx.x / DP, l = threadIdx.x - s2 * DP;
             const int ns = s2 ? n1 : n0;
-            for (int q = 0; q < ns; ++q) sx += (double)xs[lst[s2][q] * RS + l];
+            const int* L = lst[s2];
+            int q = 0;
+            for (; q + 4 <= ns; q += 4) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) sx[j] += (double)xs[L[q + j] * RS + l];
+            }
+            for (; q < ns; ++q) sx[0] += (double)xs[L[q] * RS + l];
         }
         __syncthreads();
     }
@@ -362,7 +376,7 @@ __global__ void __launch_bounds__(kThreads) k_sub_stats_gauss(const T* __restric
     if (threadIdx.x == 0) { out0[0] = (double)cnt0; out0[E] = (double)cnt1; }
     if (threadIdx.x < 2 * DP) {
         const int s2 = threadIdx.x / DP, l = threadIdx.x - s2 * DP;
-        out0[s2 * E + 1 + l] = sx;
+        out0[s2 * E + 1 + l] = (sx[0] + sx[1]) + (sx[2] + sx[3]);
     }
 #pragma unroll
     for (int u = 0; u < TPW; ++u) {

@@ -393,7 +393,7 @@ struct dpmm_ctx {
     int32_t* dRecheck = nullptr;
     int* dRecheckCount = nullptr;
     float tau = 1.0f / 32.0f;
-    // statistics already reduced into dStats by the fused sub-label pass, for Kpartition clusters
+    // dStats holds the statistics of the current labels and sub-labels, for Kpartition clusters
     bool stats_cached = false;
     hipEvent_t ev[6] = {};
     float ms_label = 0, ms_sub = 0, ms_stats = 0;
@@ -698,14 +698,6 @@ template <int DP> void lg_f64(dpmm_ctx* c, uint32_t it, int f) { run_label_gauss
 template <int DP> void sg_f32(dpmm_ctx* c, uint32_t it) { run_sub_gauss<float, DP>(c, it); }
 template <int DP> void sg_f64(dpmm_ctx* c, uint32_t it) { run_sub_gauss<double, DP>(c, it); }
 template <int DP> void st_f32(dpmm_ctx* c) { run_stats_gauss<float, DP>(c); }
-template <typename T, int DP>
-void run_sub_stats_gauss(dpmm_ctx* c, uint32_t iter) {
-    launch_sub_stats_gauss<T, DP>(c->stream, chunks_upper(c, c->K), reinterpret_cast<const T*>(c->dX), c->D, c->K,
-                                  c->CH, c->dPerm, c->dLabelStart, c->dChunkStart, reinterpret_cast<const T*>(c->dSP),
-                                  c->key0(), c->key1(), iter, c->dSubs, c->dPartials);
-}
-template <int DP> void ss_f32(dpmm_ctx* c, uint32_t it) { run_sub_stats_gauss<float, DP>(c, it); }
-template <int DP> void ss_f64(dpmm_ctx* c, uint32_t it) { run_sub_stats_gauss<double, DP>(c, it); }
 template <int DP> void st_f64(dpmm_ctx* c) { run_stats_gauss<double, DP>(c); }
 
 template <int DP> void lgl_f32(dpmm_ctx* c, uint32_t it, int f, int grid) {
@@ -912,20 +904,7 @@ int dpmm_sample_sub_labels(dpmm_ctx* c, uint32_t iter) {
     }
     (void)hipEventRecord(c->ev[2], c->stream);
     c->stats_cached = false;
-    const bool fused = c->prior == DPMM_PRIOR_GAUSSIAN && c->DP % 16 == 0;
-    if (fused) {   // sub-labels and their statistics in one pass over the segments
-        const size_t E = c->statEntries();
-        const int K = c->K;
-        int rc = grow(c->dPartials, c->partialsCap, (size_t)2 * chunks_upper(c, K) * E);
-        if (rc) return rc;
-        rc = grow(c->dStats, c->statsCap, (size_t)2 * K * E);
-        if (rc) return rc;
-        if (c->params_f64) { DPMM_DISPATCH_DP64(c->DP, ss_f64, c, iter); }
-        else { DPMM_DISPATCH_DP(c->DP, ss_f32, c, iter); }
-        hipLaunchKernelGGL(k_stats_reduce, dim3(2 * K, (unsigned)((E + kThreads - 1) / kThreads)), dim3(kThreads), 0,
-                           c->stream, c->dPartials, c->dChunkStart, K, (int)E, c->dStats);
-        c->stats_cached = true;
-    } else if (c->prior == DPMM_PRIOR_GAUSSIAN) {
+    if (c->prior == DPMM_PRIOR_GAUSSIAN) {
         if (c->params_f64) { DPMM_DISPATCH_DP64(c->DP, sg_f64, c, iter); }
         else { DPMM_DISPATCH_DP(c->DP, sg_f32, c, iter); }
     } else {

@@ -242,155 +242,6 @@ __global__ void __launch_bounds__(kThreads) k_stats_gauss_mfma(const TX* __restr
     }
 }
 
-// Fused sub-label sweep and statistics of one chunk (points of one cluster in perm order): each
-// batch of 128 points is staged once in LDS; 256 lanes score them against both sub-clusters (lane
-// pairs, 128 lanes per sub-cluster, two rounds), wave 0 draws the sub-labels and compacts each
-// sub-cluster's points (index order kept), then the four waves accumulate the FP64 scatter tiles of both sub-clusters on MFMA (the layout of
-// k_stats_gauss_mfma) together with sums and counts.  Writes the partials of (chunk, 0) and
-// (chunk, 1) in the layout k_stats_reduce reads.
-template <typename T, int DP>
-__global__ void __launch_bounds__(kThreads) k_sub_stats_gauss(const T* __restrict__ X, int D, int K, int CH,
-                                                              const int32_t* __restrict__ perm,
-                                                              const int32_t* __restrict__ label_start,
-                                                              const int32_t* __restrict__ chunk_start,
-                                                              const T* __restrict__ SP, uint32_t key0, uint32_t key1,
-                                                              uint32_t iter, int32_t* __restrict__ subs,
-                                                              double* __restrict__ partials) {
-    typedef double f64x4 __attribute__((ext_vector_type(4)));
-    constexpr int B = 64;                             // points per batch
-    constexpr int RS = DP + 4;                        // padded LDS row (16-byte aligned, spreads banks)
-    constexpr int stride = gauss_slot_stride(DP);
-    constexpr int TT = DP / 16;
-    constexpr int NTILE = TT * (TT + 1) / 2;
-    constexpr int TPW = (2 * NTILE + 3) / 4;          // (sub-cluster, tile) pairs per wave
-    constexpr int HC = DP / 2;
-    constexpr size_t E = 1 + DP + (size_t)DP * DP;
-    __shared__ __align__(16) T ps[2 * stride];
-    __shared__ __align__(16) T xs[B * RS];
-    __shared__ T sc[2][B];
-    __shared__ int idb[B];
-    __shared__ int lst[2][B + 4];
-    __shared__ int nls[2];
-    int k, beg, end;
-    if (!chunk_of(blockIdx.x, K, chunk_start, label_start, CH, k, beg, end)) return;
-    const T* src = SP + (size_t)(2 * k) * stride;
-    for (int e = threadIdx.x; e < 2 * stride; e += kThreads) ps[e] = src[e];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    // scoring role: sub-cluster hs, point pp, lane parity par
-    const int hs = threadIdx.x >> 7, pp0 = (threadIdx.x & 127) >> 1, par = threadIdx.x & 1;
-    // statistics role: (sub-cluster, tile) pairs of this wave
-    const int kp = lane >> 4, r = lane & 15;
-    int us[TPW], ti[TPW], tj[TPW];
-#pragma unroll
-    for (int u = 0; u < TPW; ++u) {
-        int t = wv + 4 * u;
-        us[u] = -1; ti[u] = 0; tj[u] = 0;
-        if (t < 2 * NTILE) {
-            us[u] = t / NTILE;
-            t -= us[u] * NTILE;
-            int row = 0;
-            while (t >= TT - row) { t -= TT - row; ++row; }
-            ti[u] = row; tj[u] = row + t;
-        }
-    }
-    f64x4 acc[TPW];
-#pragma unroll
-    for (int u = 0; u < TPW; ++u) acc[u] = f64x4{0.0, 0.0, 0.0, 0.0};
-    double sx[4] = {0.0, 0.0, 0.0, 0.0};   // thread t < 2 DP: sub-cluster t / DP, coordinate t % DP
-    int64_t cnt0 = 0, cnt1 = 0;
-    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    for (int b0 = beg; b0 < end; b0 += B) {
-        const int n = min(B, end - b0);
-        if (threadIdx.x < B) idb[threadIdx.x] = threadIdx.x < n ? perm[b0 + threadIdx.x] : -1;
-        __syncthreads();
-        for (int e = threadIdx.x; e < B * DP; e += kThreads) {
-            const int q = e / DP, l = e - q * DP;
-            xs[q * RS + l] = (q < n && l < D) ? X[(int64_t)idb[q] * D + l] : T(0);
-        }
-        __syncthreads();
-#pragma unroll 1
-        for (int rd = 0; rd < B / 64; ++rd) {   // score point pp against sub-cluster hs
-            const int pp = pp0 + 64 * rd;
-            T xh[HC];
-#pragma unroll
-            for (int c = 0; c < HC; ++c) xh[c] = xs[pp * RS + pair_col(c, 0) + (par << 2)];
-            const T* pps = ps + hs * stride;
-            const T q = whitened_sq_pair<T, DP>(pps, xh, par);
-            if (par == 0) sc[hs][pp] = pps[gauss_c_offset(DP)] - T(0.5) * q;
-        }
-        __syncthreads();
-        if (wv == 0) {   // draw, write, compact by sub-label (index order kept)
-            int base0 = 0, base1 = 0;
-#pragma unroll 1
-            for (int rd = 0; rd < B / 64; ++rd) {
-                const int q = lane + 64 * rd;
-                int sb = -1;
-                if (q < n) {
-                    const int idx = idb[q];
-                    const uint32_t w = philox4x32_10((uint32_t)idx, 0, iter, TAG_SUB, key0, key1).v[0];
-                    sb = (logit_u32<T>(w) < sc[0][q] - sc[1][q]) ? 0 : 1;
-                    subs[idx] = sb;
-                }
-                const uint64_t m0 = __ballot(sb == 0), m1 = __ballot(sb == 1);
-                if (sb == 0) lst[0][base0 + __popcll(m0 & lt)] = q;
-                if (sb == 1) lst[1][base1 + __popcll(m1 & lt)] = q;
-                base0 += __popcll(m0);
-                base1 += __popcll(m1);
-            }
-            if (lane < 4) { lst[0][base0 + lane] = B - 1; lst[1][base1 + lane] = B - 1; }   // in-range pads
-            if (lane == 0) { nls[0] = base0; nls[1] = base1; }
-        }
-        __syncthreads();
-        const int n0 = nls[0], n1 = nls[1];
-        cnt0 += n0;
-        cnt1 += n1;
-#pragma unroll
-        for (int u = 0; u < TPW; ++u) {
-            if (us[u] < 0) continue;
-            const int ns = us[u] ? n1 : n0;
-            const int* L = lst[us[u]];
-            const int ca = 16 * ti[u] + r, cb = 16 * tj[u] + r;
-            for (int g = 0; g < ns; g += 4) {
-                const int q = g + kp;
-                const int row = L[q] * RS;     // pads point at a valid row; masked below
-                const bool in = q < ns;
-                const double a = in ? (double)xs[row + ca] : 0.0;
-                const double bb = in ? (double)xs[row + cb] : 0.0;
-                acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bb, acc[u], 0, 0, 0);
-            }
-        }
-        if (threadIdx.x < 2 * DP) {
-            const int s2 = threadIdx.x / DP, l = threadIdx.x - s2 * DP;
-            const int ns = s2 ? n1 : n0;
-            const int* L = lst[s2];
-            int q = 0;
-            for (; q + 4 <= ns; q += 4) {
-#pragma unroll
-                for (int j = 0; j < 4; ++j) sx[j] += (double)xs[L[q + j] * RS + l];
-            }
-            for (; q < ns; ++q) sx[0] += (double)xs[L[q] * RS + l];
-        }
-        __syncthreads();
-    }
-    double* out0 = partials + (size_t)(2 * blockIdx.x) * E;
-    if (threadIdx.x == 0) { out0[0] = (double)cnt0; out0[E] = (double)cnt1; }
-    if (threadIdx.x < 2 * DP) {
-        const int s2 = threadIdx.x / DP, l = threadIdx.x - s2 * DP;
-        out0[s2 * E + 1 + l] = (sx[0] + sx[1]) + (sx[2] + sx[3]);
-    }
-#pragma unroll
-    for (int u = 0; u < TPW; ++u) {
-        if (us[u] < 0) continue;
-        double* S = out0 + us[u] * E + 1 + DP;
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int ra = 16 * ti[u] + 4 * v + kp, cb = 16 * tj[u] + r;
-            S[(size_t)ra * DP + cb] = acc[u][v];
-            S[(size_t)cb * DP + ra] = acc[u][v];
-        }
-    }
-}
-
 // Partial (count, sum, scatter) of the points of sub-cluster s in one chunk, FP64.  Threads own
 // 4x4 blocks of the upper block triangle; points staged in LDS in batches, compacted by ballot.
 template <typename TX, int DP>
@@ -511,15 +362,6 @@ void launch_sub_gauss(hipStream_t st, int grid, const T* X, int D, int K, int CH
                        chunk_start, SP, key0, key1, iter, subs);
 }
 
-template <typename T, int DP>
-void launch_sub_stats_gauss(hipStream_t st, int grid, const T* X, int D, int K, int CH, const int32_t* perm,
-                            const int32_t* label_start, const int32_t* chunk_start, const T* SP, uint32_t key0,
-                            uint32_t key1, uint32_t iter, int32_t* subs, double* partials) {
-    if constexpr (DP % 16 == 0)
-        hipLaunchKernelGGL((k_sub_stats_gauss<T, DP>), dim3(grid), dim3(kThreads), 0, st, X, D, K, CH, perm,
-                           label_start, chunk_start, SP, key0, key1, iter, subs, partials);
-}
-
 template <typename TX, int DP>
 void launch_stats_gauss(hipStream_t st, int grid, const TX* X, int D, int K, int CH, const int32_t* perm,
                         const int32_t* label_start, const int32_t* chunk_start, const int32_t* subs,
@@ -542,10 +384,7 @@ void launch_stats_gauss(hipStream_t st, int grid, const TX* X, int D, int K, int
                                           const int32_t*, const int32_t*, const T*, uint32_t, uint32_t,        \
                                           uint32_t, int32_t*);                                                 \
     template void launch_stats_gauss<T, DP>(hipStream_t, int, const T*, int, int, int, const int32_t*,         \
-                                            const int32_t*, const int32_t*, const int32_t*, double*);          \
-    template void launch_sub_stats_gauss<T, DP>(hipStream_t, int, const T*, int, int, int, const int32_t*,     \
-                                                const int32_t*, const int32_t*, const T*, uint32_t, uint32_t,  \
-                                                uint32_t, int32_t*, double*);
+                                            const int32_t*, const int32_t*, const int32_t*, double*);
 
 #define DPMM_GAUSS_EXTERN(T, DP)                                                                              \
     extern template void launch_label_gauss<T, DP>(hipStream_t, int, const T*, int64_t, int, int, const T*,    \
@@ -557,10 +396,6 @@ void launch_stats_gauss(hipStream_t st, int grid, const TX* X, int D, int K, int
                                                  const int32_t*, const int32_t*, const T*, uint32_t, uint32_t, \
                                                  uint32_t, int32_t*);                                          \
     extern template void launch_stats_gauss<T, DP>(hipStream_t, int, const T*, int, int, int, const int32_t*,  \
-                                                   const int32_t*, const int32_t*, const int32_t*, double*);   \
-    extern template void launch_sub_stats_gauss<T, DP>(hipStream_t, int, const T*, int, int, int,              \
-                                                       const int32_t*, const int32_t*, const int32_t*,         \
-                                                       const T*, uint32_t, uint32_t, uint32_t, int32_t*,       \
-                                                       double*);
+                                                   const int32_t*, const int32_t*, const int32_t*, double*);
 
 }  // namespace dpmm

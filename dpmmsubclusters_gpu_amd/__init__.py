@@ -89,6 +89,11 @@ def load_library():
         "dpmm_set_mixed_tolerance": [_P, ctypes.c_float],
         "dpmm_last_recheck_count": [_P, _P],
         "dpmm_synchronize": [_P],
+        "dpmm_set_prior": [_P, dbl, _P, dbl, _P, _P],
+        "dpmm_set_statistics": [_P, i, _P, _P, _P],
+        "dpmm_log_marginal_batch": [_P, i, _P, _P, _P],
+        "dpmm_sample_params": [_P, u32, i, _P, _P, _P, i, i],
+        "dpmm_get_drawn_params": [_P, _P, _P, _P],
         "dpmm_sampler_create": [_P, _P, dbl, _P, dbl, _P, _P, _P],
         "dpmm_sampler_step": [_P, i],
         "dpmm_sampler_num_clusters": [_P],
@@ -116,6 +121,10 @@ def _ptr(a):
 
 def _f64(a):
     return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i64(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.int64)
 
 
 def _i32(a):
@@ -247,12 +256,53 @@ class DeviceSweep:
     def sample_sub_labels(self, iteration):
         _check(load_library().dpmm_sample_sub_labels(self._ctx, int(iteration)))
 
-    def sufficient_statistics(self, K, want_S=True):
+    def sufficient_statistics(self, K, want_S=True, counts_only=False):
         Nsub = np.empty(2 * K, np.int64)
-        s = np.empty((2 * K, self.D))
-        S = np.empty((2 * K, self.D, self.D)) if (want_S and self.prior == prior_type.Gaussian) else None
+        s = None if counts_only else np.empty((2 * K, self.D))
+        S = np.empty((2 * K, self.D, self.D)) if (want_S and not counts_only and self.prior == prior_type.Gaussian) else None
         _check(load_library().dpmm_sufficient_statistics(self._ctx, K, _ptr(Nsub), _ptr(s), _ptr(S)))
         return Nsub, s, S
+
+    def set_statistics(self, Nsub, sums, S=None):
+        """Test hook: statistics of 2K sub-clusters (counts, sums, scatter) uploaded as if computed."""
+        n = _i64(Nsub)
+        s, SS = _f64(sums), _f64(S)
+        _check(load_library().dpmm_set_statistics(self._ctx, n.size // 2, _ptr(n), _ptr(s), _ptr(SS)))
+
+    def set_prior(self, hyper):
+        """niw_hyperparams or multinomial_hyper of the batched posterior work."""
+        if isinstance(hyper, niw_hyperparams):
+            m, psi = _f64(hyper.m), _f64(hyper.psi)
+            _check(load_library().dpmm_set_prior(self._ctx, hyper.k, _ptr(m), hyper.v, _ptr(psi), None))
+        else:
+            a = _f64(hyper.alpha)
+            _check(load_library().dpmm_set_prior(self._ctx, 0.0, None, 0.0, None, _ptr(a)))
+
+    def log_marginal_batch(self, members):
+        """Log marginal likelihood and log det psi_n of each row of members (n x 4 sub-cluster
+        entries of the last statistics, -1 unused)."""
+        mem = _i32(np.asarray(members).reshape(-1, 4))
+        n = mem.shape[0]
+        L, ld = np.empty(n), np.empty(n)
+        _check(load_library().dpmm_log_marginal_batch(self._ctx, n, _ptr(mem), _ptr(L), _ptr(ld)))
+        return L, ld
+
+    def sample_params(self, iteration, members, weights, lr_weights, full_mu_cov=0, use_weight=0):
+        """Device draws of the 3K parts (cluster, left, right) from their posteriors, packed into
+        the sweep's parameter slots."""
+        mem = _i32(np.asarray(members).reshape(-1, 4))
+        K = mem.shape[0] // 3
+        w, lrw = _f64(weights), _f64(lr_weights)
+        _check(load_library().dpmm_sample_params(self._ctx, int(iteration), K, _ptr(mem), _ptr(w), _ptr(lrw),
+                                                  int(full_mu_cov), int(use_weight)))
+        return K
+
+    def drawn_params(self, K):
+        mu = np.empty((3 * K, self.D))
+        R = np.empty((3 * K, self.D, self.D)) if self.prior == prior_type.Gaussian else None
+        ld = np.empty(3 * K) if self.prior == prior_type.Gaussian else None
+        _check(load_library().dpmm_get_drawn_params(self._ctx, _ptr(mu), _ptr(R), _ptr(ld)))
+        return mu, R, ld
 
     def relabel(self, iteration, new_label, sub_action):
         nl, act = _i32(new_label), _i32(sub_action)

@@ -25,6 +25,7 @@
 #include "dpmm_device.h"
 #include "dpmm_gauss.h"
 #include "dpmm_mixed.h"
+#include "dpmm_niw.h"
 
 using namespace dpmm;
 
@@ -395,6 +396,25 @@ struct dpmm_ctx {
     float tau = 1.0f / 32.0f;
     // dStats holds the statistics of the current labels and sub-labels, for Kpartition clusters
     bool stats_cached = false;
+    int Kstats = 0;   // clusters whose statistics dStats holds (entries 0 .. 2 Kstats - 1)
+    // prior and buffers of the batched conjugate work
+    bool has_prior = false;
+    PriorArgs prior_args{};
+    double* dPrior = nullptr;   // m0 (D) | psi0 (D x D) or alpha0 (D)
+    int32_t* dMembers = nullptr;
+    size_t membersCap = 0;
+    double* dLM = nullptr;
+    size_t lmCap = 0;
+    double* dDraws = nullptr;
+    size_t drawsCap = 0;
+    int32_t* dDrawStatus = nullptr;
+    size_t drawStatusCap = 0;
+    double* dLogW = nullptr;
+    size_t logwCap = 0;
+    int Kdraw = 0;
+    int drawStride() const {
+        return prior == DPMM_PRIOR_GAUSSIAN ? ((D + D * (D + 1) / 2 + 1) + 1) & ~1 : (D + 1) & ~1;
+    }
     hipEvent_t ev[6] = {};
     float ms_label = 0, ms_sub = 0, ms_stats = 0;
     bool ev_label = false, ev_sub = false, ev_stats = false;
@@ -586,6 +606,20 @@ void pack_mx_slot(int D, int DPM, const double* mu, const double* R, double cst,
     m[DPM] = (float)cst;
 }
 
+// BF16 screen slots for K clusters and the recheck list.
+int ensure_mixed_buffers(dpmm_ctx* c, int K) {
+    unsigned char* pb = reinterpret_cast<unsigned char*>(c->dPB);
+    int rc = grow(pb, c->pbCap, (size_t)K * mx_slot_bytes(c->DPM));
+    c->dPB = pb;
+    if (rc) return rc;
+    if (!c->dRecheck) {
+        if (hipMalloc((void**)&c->dRecheck, c->N * sizeof(int32_t)) != hipSuccess ||
+            hipMalloc((void**)&c->dRecheckCount, sizeof(int)) != hipSuccess)
+            return fail(DPMM_ENOMEM, "recheck alloc");
+    }
+    return DPMM_OK;
+}
+
 int set_gauss(dpmm_ctx* c, int K, const double* mu, const double* mat, const double* logdet, const double* weights,
               const double* smu, const double* smat, const double* slogdet, const double* lrw, bool is_factor) {
     if (!c) return fail(DPMM_EINVAL, "null ctx");
@@ -626,15 +660,8 @@ int set_gauss(dpmm_ctx* c, int K, const double* mu, const double* mat, const dou
                          c->hPB.data() + (size_t)k * mslot);
     }
     if (mixed) {
-        unsigned char* pb = reinterpret_cast<unsigned char*>(c->dPB);
-        int rc2 = grow(pb, c->pbCap, c->hPB.size());
-        c->dPB = pb;
+        int rc2 = ensure_mixed_buffers(c, K);
         if (rc2) return rc2;
-        if (!c->dRecheck) {
-            if (hipMalloc((void**)&c->dRecheck, c->N * sizeof(int32_t)) != hipSuccess ||
-                hipMalloc((void**)&c->dRecheckCount, sizeof(int)) != hipSuccess)
-                return fail(DPMM_ENOMEM, "recheck alloc");
-        }
         HIPCHK(hipMemcpyAsync(c->dPB, c->hPB.data(), c->hPB.size(), hipMemcpyHostToDevice, c->stream));
     }
     for (int q = 0; q < 2 * K; ++q) {
@@ -777,7 +804,9 @@ int dpmm_destroy(dpmm_ctx* c) {
     if (c->owns_X && c->dX) (void)hipFree(c->dX);
     for (void* p : {(void*)c->dLabels, (void*)c->dSubs, (void*)c->dPerm, c->dP, c->dSP, (void*)c->dHist,
                     (void*)c->dRowtot, (void*)c->dLabelStart, (void*)c->dChunkStart, (void*)c->dTable,
-                    (void*)c->dPartials, (void*)c->dStats, c->dPB, (void*)c->dRecheck, (void*)c->dRecheckCount})
+                    (void*)c->dPartials, (void*)c->dStats, c->dPB, (void*)c->dRecheck, (void*)c->dRecheckCount,
+                    (void*)c->dPrior, (void*)c->dMembers, (void*)c->dLM, (void*)c->dDraws, (void*)c->dDrawStatus,
+                    (void*)c->dLogW})
         if (p) (void)hipFree(p);
     if (c->hStats) (void)hipHostFree(c->hStats);
     for (auto& e : c->ev)
@@ -920,7 +949,7 @@ int dpmm_sample_sub_labels(dpmm_ctx* c, uint32_t iter) {
 }
 
 int dpmm_sufficient_statistics(dpmm_ctx* c, int K, int64_t* Nsub, double* sum, double* S) {
-    if (!c || !Nsub || !sum) return fail(DPMM_EINVAL, "bad args");
+    if (!c || !Nsub) return fail(DPMM_EINVAL, "bad args");
     if (K <= 0 || K > kMaxK) return fail(DPMM_EINVAL, "bad K");
     if (!c->partition_valid || c->Kpartition != K) {
         int rc = build_partition(c, K);
@@ -945,16 +974,24 @@ int dpmm_sufficient_statistics(dpmm_ctx* c, int K, int64_t* Nsub, double* sum, d
         hipLaunchKernelGGL(k_stats_reduce, dim3(2 * K, (unsigned)((E + kThreads - 1) / kThreads)), dim3(kThreads), 0,
                            c->stream, c->dPartials, c->dChunkStart, K, (int)E, c->dStats);
         c->stats_cached = true;
+        c->Kstats = K;
     }
     (void)hipEventRecord(c->ev[5], c->stream);
     HIPCHK(hipGetLastError());
     c->ev_stats = true;
-    const size_t n = (size_t)2 * K * E;
+    const size_t n = sum ? (size_t)2 * K * E : (size_t)2 * K;
     if (c->hStatsCap < n) {
         if (c->hStats) (void)hipHostFree(c->hStats);
         c->hStats = nullptr;
         if (hipHostMalloc((void**)&c->hStats, n * sizeof(double)) != hipSuccess) return fail(DPMM_ENOMEM, "pinned");
         c->hStatsCap = n;
+    }
+    if (!sum) {   // counts only: the first double of every entry
+        HIPCHK(hipMemcpy2DAsync(c->hStats, sizeof(double), c->dStats, E * sizeof(double), sizeof(double),
+                                (size_t)2 * K, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (int q = 0; q < 2 * K; ++q) Nsub[q] = (int64_t)c->hStats[q];
+        return DPMM_OK;
     }
     HIPCHK(hipMemcpyAsync(c->hStats, c->dStats, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -1021,6 +1058,216 @@ int dpmm_last_recheck_count(dpmm_ctx* c, int64_t* n) {
 int dpmm_synchronize(dpmm_ctx* c) {
     if (!c) return fail(DPMM_EINVAL, "null ctx");
     HIPCHK(hipStreamSynchronize(c->stream));
+    return DPMM_OK;
+}
+
+// ---- batched conjugate work ---------------------------------------------------------------------
+int dpmm_set_prior(dpmm_ctx* c, double k0, const double* m0, double v0, const double* psi0, const double* alpha0) {
+    if (!c) return fail(DPMM_EINVAL, "null ctx");
+    const int D = c->D;
+    PriorArgs h{};
+    std::vector<double> host;
+    if (c->prior == DPMM_PRIOR_GAUSSIAN) {
+        if (!m0 || !psi0 || !(k0 > 0.0) || !(v0 > D - 1.0)) return fail(DPMM_EINVAL, "bad NIW hyperparameters");
+        std::vector<double> L;
+        if (!chol_lower(D, psi0, L)) return fail(DPMM_ENOTSPD, "prior psi is not SPD");
+        double ld = 0.0;
+        for (int i = 0; i < D; ++i) ld += std::log(L[(size_t)i * D + i]);
+        h.k0 = k0;
+        h.v0 = v0;
+        h.logdet_psi0 = 2.0 * ld;
+        double lg = (double)D * (D - 1) / 4.0 * std::log(kPiF64);
+        for (int d = 1; d <= D; ++d) lg += std::lgamma(v0 / 2.0 + (1 - d) / 2.0);
+        h.lmvg0 = lg;
+        host.assign(m0, m0 + D);
+        host.insert(host.end(), psi0, psi0 + (size_t)D * D);
+    } else {
+        if (!alpha0) return fail(DPMM_EINVAL, "alpha0 is null");
+        double s = 0.0;
+        for (int d = 0; d < D; ++d) {
+            if (!(alpha0[d] > 0.0)) return fail(DPMM_EINVAL, "alpha0 must be positive");
+            s += alpha0[d];
+        }
+        h.alpha0_sum = s;
+        h.lg_alpha0_sum = std::lgamma(s);
+        host.assign(alpha0, alpha0 + D);
+    }
+    if (c->dPrior) (void)hipFree(c->dPrior);
+    c->dPrior = nullptr;
+    if (hipMalloc((void**)&c->dPrior, host.size() * sizeof(double)) != hipSuccess) return fail(DPMM_ENOMEM, "prior");
+    HIPCHK(hipMemcpy(c->dPrior, host.data(), host.size() * sizeof(double), hipMemcpyHostToDevice));
+    if (c->prior == DPMM_PRIOR_GAUSSIAN) {
+        h.m0 = c->dPrior;
+        h.psi0 = c->dPrior + D;
+    } else {
+        h.alpha0 = c->dPrior;
+    }
+    c->prior_args = h;
+    c->has_prior = true;
+    return DPMM_OK;
+}
+
+namespace {
+int upload_members(dpmm_ctx* c, int n, const int32_t* members) {
+    const int lim = 2 * c->Kstats;
+    for (size_t e = 0; e < (size_t)4 * n; ++e)
+        if (members[e] < -1 || members[e] >= lim) return fail(DPMM_EINVAL, "member outside the current statistics");
+    int rc = grow(c->dMembers, c->membersCap, (size_t)4 * n);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(c->dMembers, members, (size_t)4 * n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    return DPMM_OK;
+}
+
+size_t lds_bytes(size_t doubles) { return std::max<size_t>(doubles, 1) * sizeof(double); }
+
+// Opt a kernel in to more than the default dynamic LDS when a large D needs it.
+int allow_lds(const void* kernel, size_t bytes) {
+    if (bytes <= 65536) return DPMM_OK;
+    HIPCHK(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    return DPMM_OK;
+}
+}  // namespace
+
+int dpmm_set_statistics(dpmm_ctx* c, int K, const int64_t* Nsub, const double* sum, const double* S) {
+    if (!c || K <= 0 || K > kMaxK || !Nsub || !sum) return fail(DPMM_EINVAL, "bad args");
+    const bool gauss = c->prior == DPMM_PRIOR_GAUSSIAN;
+    if (gauss && !S) return fail(DPMM_EINVAL, "Gaussian statistics need S");
+    const size_t E = c->statEntries();
+    const int D = c->D, DP = c->DP;
+    std::vector<double> h((size_t)2 * K * E, 0.0);
+    for (int q = 0; q < 2 * K; ++q) {
+        double* dst = h.data() + (size_t)q * E;
+        if (Nsub[q] < 0) return fail(DPMM_EINVAL, "negative count");
+        dst[0] = (double)Nsub[q];
+        for (int d = 0; d < D; ++d) dst[1 + d] = sum[(size_t)q * D + d];
+        if (gauss)
+            for (int a = 0; a < D; ++a)
+                for (int b = 0; b < D; ++b) dst[1 + DP + (size_t)a * DP + b] = S[((size_t)q * D + a) * D + b];
+    }
+    int rc = grow(c->dStats, c->statsCap, h.size());
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(c->dStats, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->Kstats = K;
+    c->stats_cached = false;   // not the statistics of the current labels
+    return DPMM_OK;
+}
+
+int dpmm_log_marginal_batch(dpmm_ctx* c, int n, const int32_t* members, double* logml, double* logdet_psi) {
+    if (!c || n < 0 || (n > 0 && (!members || !logml))) return fail(DPMM_EINVAL, "bad args");
+    if (!c->has_prior) return fail(DPMM_ESTATE, "prior not set");
+    if (c->Kstats <= 0) return fail(DPMM_ESTATE, "no statistics computed yet");
+    if (n == 0) return DPMM_OK;
+    int rc = upload_members(c, n, members);
+    if (rc) return rc;
+    rc = grow(c->dLM, c->lmCap, (size_t)2 * n);
+    if (rc) return rc;
+    const int D = c->D;
+    const size_t lds = c->prior == DPMM_PRIOR_GAUSSIAN ? lds_bytes((size_t)D * (D + 1) / 2 + D) : lds_bytes(0);
+    rc = allow_lds(reinterpret_cast<const void*>(k_log_marginal), lds);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_log_marginal, dim3(n), dim3(kThreads), lds, c->stream, c->dStats, (int)c->statEntries(),
+                       c->DP, D, c->prior, c->dMembers, c->prior_args, c->dLM);
+    HIPCHK(hipGetLastError());
+    std::vector<double> out((size_t)2 * n);
+    HIPCHK(hipMemcpyAsync(out.data(), c->dLM, out.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int j = 0; j < n; ++j) {
+        logml[j] = out[2 * j];
+        if (logdet_psi) logdet_psi[j] = out[2 * j + 1];
+    }
+    return DPMM_OK;
+}
+
+int dpmm_sample_params(dpmm_ctx* c, uint32_t iter, int K, const int32_t* members, const double* weights,
+                       const double* lr_weights, int full_mu_cov, int use_weight) {
+    if (!c || K <= 0 || !members || !weights || !lr_weights) return fail(DPMM_EINVAL, "bad args");
+    if (K > kMaxK) return fail(DPMM_EINVAL, "K exceeds 2048");
+    if (!c->has_prior) return fail(DPMM_ESTATE, "prior not set");
+    if (c->Kstats <= 0) return fail(DPMM_ESTATE, "no statistics computed yet");
+    const int D = c->D, np = 3 * K;
+    int rc = upload_members(c, np, members);
+    if (rc) return rc;
+    std::vector<double> logw((size_t)3 * K);
+    for (int k = 0; k < K; ++k) {
+        if (!(weights[k] >= 0.0) || !(lr_weights[2 * k] >= 0.0) || !(lr_weights[2 * k + 1] >= 0.0))
+            return fail(DPMM_EINVAL, "weights must be non-negative");
+        logw[k] = std::log(weights[k]);
+        logw[K + 2 * k] = std::log(lr_weights[2 * k]);
+        logw[K + 2 * k + 1] = std::log(lr_weights[2 * k + 1]);
+    }
+    const int dstride = c->drawStride();
+    if ((rc = grow(c->dLogW, c->logwCap, logw.size()))) return rc;
+    if ((rc = grow(c->dDraws, c->drawsCap, (size_t)np * dstride))) return rc;
+    if ((rc = grow(c->dDrawStatus, c->drawStatusCap, (size_t)np))) return rc;
+    HIPCHK(hipMemcpyAsync(c->dLogW, logw.data(), logw.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    const bool gauss = c->prior == DPMM_PRIOR_GAUSSIAN;
+    const size_t lds = gauss ? lds_bytes((size_t)D * (D + 1) + 3 * D) : lds_bytes(D);
+    if ((rc = allow_lds(reinterpret_cast<const void*>(k_draw_params), lds))) return rc;
+    hipLaunchKernelGGL(k_draw_params, dim3(np), dim3(kThreads), lds, c->stream, c->dStats, (int)c->statEntries(),
+                       c->DP, D, c->prior, c->dMembers, c->prior_args, iter, c->key0(), c->key1(), full_mu_cov,
+                       dstride, c->dDraws, c->dDrawStatus);
+    HIPCHK(hipGetLastError());
+    // parameter slots of the sweeps
+    if (gauss) {
+        const bool f64 = c->precision == DPMM_PREC_FP64;
+        const int stride = gauss_slot_stride(c->DP);
+        const size_t esz = f64 ? 8 : 4;
+        c->params_f64 = f64;
+        c->stride = stride;
+        if ((rc = ensure_param_buffers(c, K, (size_t)stride * esz))) return rc;
+        unsigned char* pb = nullptr;
+        if (c->precision == DPMM_PREC_MIXED) {
+            if ((rc = ensure_mixed_buffers(c, K))) return rc;
+            pb = reinterpret_cast<unsigned char*>(c->dPB);
+        }
+        if (f64)
+            hipLaunchKernelGGL(k_pack_gauss<double>, dim3(np), dim3(kThreads), 0, c->stream, c->dDraws, dstride, D,
+                               c->DP, c->DPM, c->dLogW, K, reinterpret_cast<double*>(c->dP),
+                               reinterpret_cast<double*>(c->dSP), pb);
+        else
+            hipLaunchKernelGGL(k_pack_gauss<float>, dim3(np), dim3(kThreads), 0, c->stream, c->dDraws, dstride, D,
+                               c->DP, c->DPM, c->dLogW, K, reinterpret_cast<float*>(c->dP),
+                               reinterpret_cast<float*>(c->dSP), pb);
+    } else {
+        const int stride = (D + 1 + 3) & ~3;
+        c->params_f64 = false;
+        c->stride = stride;
+        if ((rc = ensure_param_buffers(c, K, (size_t)stride * 4))) return rc;
+        hipLaunchKernelGGL(k_pack_mult, dim3(np), dim3(kThreads), 0, c->stream, c->dDraws, dstride, D, stride,
+                           c->dLogW, K, use_weight, reinterpret_cast<float*>(c->dP), reinterpret_cast<float*>(c->dSP));
+    }
+    HIPCHK(hipGetLastError());
+    std::vector<int32_t> st(np);
+    HIPCHK(hipMemcpyAsync(st.data(), c->dDrawStatus, np * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->Kdraw = K;
+    for (int p = 0; p < np; ++p)
+        if (st[p]) return fail(DPMM_ENOTSPD, "posterior draw failed: a factor is not SPD");
+    c->K = K;
+    return DPMM_OK;
+}
+
+int dpmm_get_drawn_params(dpmm_ctx* c, double* mu, double* chol_inv_sigma, double* logdet_sigma) {
+    if (!c || !mu) return fail(DPMM_EINVAL, "bad args");
+    if (c->Kdraw <= 0) return fail(DPMM_ESTATE, "no draws yet");
+    const int D = c->D, np = 3 * c->Kdraw, dstride = c->drawStride();
+    std::vector<double> h((size_t)np * dstride);
+    HIPCHK(hipMemcpyAsync(h.data(), c->dDraws, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const int T = D * (D + 1) / 2;
+    for (int p = 0; p < np; ++p) {
+        const double* src = h.data() + (size_t)p * dstride;
+        std::copy(src, src + D, mu + (size_t)p * D);
+        if (c->prior != DPMM_PRIOR_GAUSSIAN) continue;
+        if (chol_inv_sigma) {
+            double* R = chol_inv_sigma + (size_t)p * D * D;
+            std::fill(R, R + (size_t)D * D, 0.0);
+            for (int i = 0; i < D; ++i)
+                for (int j = 0; j <= i; ++j) R[(size_t)i * D + j] = src[D + i * (i + 1) / 2 + j];
+        }
+        if (logdet_sigma) logdet_sigma[p] = src[D + T];
+    }
     return DPMM_OK;
 }
 

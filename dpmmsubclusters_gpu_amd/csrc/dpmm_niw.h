@@ -1,0 +1,452 @@
+// Batched FP64 conjugate work of the sampler on the device.  One workgroup per job:
+//   k_log_marginal  posterior of a sum of sub-cluster statistics (niw::calc_posterior,
+//                   priors/niw.cpp:23-49; multinomial_prior::calc_posterior, 12-22), its
+//                   Cholesky factor and log marginal likelihood (niw.cpp:122-138;
+//                   multinomial_prior.cpp:56-67).  Every cluster, sub-cluster and merge pair of
+//                   an iteration goes through one launch.
+//   k_draw_params   parameter draw from such a posterior (niw::sample_distribution,
+//                   niw.cpp:82-107, Bartlett form of rinvwish.ipp:35-58; Dirichlet draw of
+//                   multinomial_prior.cpp:24-49) on the part's Philox stream.
+//   k_pack_params   the draws, with log mixture / sub-cluster weights, into the label and
+//                   sub-label sweeps' parameter slots (FP32 or FP64, and the BF16 screen).
+// Matrices live in LDS in packed lower-triangular form, element (i, j), j <= i, at
+// i (i + 1) / 2 + j, so two D = 128 factors fit in one workgroup's LDS.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "dpmm_kernels.h"
+#include "dpmm_mixed.h"
+#include "dpmm_rng.h"
+
+namespace dpmm {
+
+constexpr double kPiF64 = 3.14159265358979323846;
+
+__device__ __forceinline__ int tri(int i, int j) { return i * (i + 1) / 2 + j; }
+
+// (i, j) of packed index e.
+__device__ __forceinline__ void tri_ij(int e, int& i, int& j) {
+    int r = (int)((sqrtf(8.0f * (float)e + 1.0f) - 1.0f) * 0.5f);
+    while (r * (r + 1) / 2 > e) --r;
+    while ((r + 1) * (r + 2) / 2 <= e) ++r;
+    i = r;
+    j = e - r * (r + 1) / 2;
+}
+
+// Closed form of tri_row_offset(DP, j) (row j of the sweep's packed W starts at column j & ~3).
+__device__ __forceinline__ int tri_row_offset_rt(int DP, int j) {
+    const int q = j >> 2, rem = j & 3;
+    return j * DP - 8 * q * (q - 1) - 4 * rem * q;
+}
+
+struct PriorArgs {
+    double k0, v0, logdet_psi0, lmvg0;   // lmvg0 = log Gamma_D(v0 / 2)
+    double alpha0_sum, lg_alpha0_sum;    // multinomial: sum alpha0, lgamma(sum alpha0)
+    const double* m0;                    // D
+    const double* psi0;                  // D x D row-major (the reference's psi = Psi / v0)
+    const double* alpha0;                // D
+};
+
+__device__ double block_sum(double v, double* red) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += red[w];
+    __syncthreads();
+    return s;
+}
+
+// Sequential stream (c0, n, t, tag) from word position w: the words of Philox blocks n = w / 4, ...
+struct DevStream {
+    uint32_t c0, t, tag, k0, k1, w, n;
+    Ctr4 blk;
+    __device__ DevStream(uint32_t c0_, uint32_t t_, uint32_t tag_, uint32_t k0_, uint32_t k1_, uint32_t w0)
+        : c0(c0_), t(t_), tag(tag_), k0(k0_), k1(k1_), w(w0), n(0xFFFFFFFFu) {}
+    __device__ uint32_t word() {
+        const uint32_t b = w >> 2;
+        if (b != n) { blk = philox4x32_10(c0, b, t, tag, k0, k1); n = b; }
+        const uint32_t p = w & 3u;
+        ++w;
+        return p == 0 ? blk.v[0] : p == 1 ? blk.v[1] : p == 2 ? blk.v[2] : blk.v[3];
+    }
+    __device__ double uniform() { return u01_f64(word()); }
+    __device__ double normal() {   // Box-Muller, cosine branch
+        const double a = uniform();
+        const double b = uniform();
+        return sqrt(-2.0 * log(a)) * cos(2.0 * kPiF64 * b);
+    }
+    // Marsaglia & Tsang (2000); a < 1 through Gamma(a + 1) U^(1/a).
+    __device__ double gamma(double a) {
+        if (!(a > 0.0)) return 0.0;
+        const bool boost = a < 1.0;
+        const double ab = boost ? a + 1.0 : a;
+        const double d = ab - 1.0 / 3.0, c = 1.0 / sqrt(9.0 * d);
+        double g;
+        for (;;) {
+            const double z = normal();
+            double v = 1.0 + c * z;
+            if (v <= 0.0) continue;
+            v = v * v * v;
+            if (log(uniform()) < 0.5 * z * z + d - d * v + d * log(v)) { g = d * v; break; }
+        }
+        if (boost) g *= pow(uniform(), 1.0 / a);
+        return g;
+    }
+};
+
+// Normal number m of a stream whose first normals start at word 0: words 2m, 2m + 1.
+__device__ __forceinline__ double stream_normal_at(uint32_t c0, uint32_t t, uint32_t k0, uint32_t k1, uint32_t w) {
+    DevStream s(c0, t, TAG_PARAM, k0, k1, w);
+    return s.normal();
+}
+
+// ---- posterior of one statistics sum ----------------------------------------------------------
+// stats: entries of E doubles [N, sum (DP), S (DP x DP)]; mem: up to 4 entry indices (-1 unused).
+// Fills P (packed lower posterior psi, scaled by `scale`), m (D) and scal = {N, kappa, nu}.
+__device__ void niw_posterior(const double* __restrict__ stats, size_t E, int DP, int D, const int* mem,
+                              const PriorArgs& h, double scale_by_nu, double* P, double* m, double* scal) {
+    if (threadIdx.x == 0) {
+        double n = 0.0;
+        for (int t = 0; t < 4; ++t)
+            if (mem[t] >= 0) n += stats[(size_t)mem[t] * E];
+        scal[0] = n;
+        scal[1] = n == 0.0 ? h.k0 : h.k0 + n;
+        scal[2] = n == 0.0 ? h.v0 : h.v0 + n;
+    }
+    __syncthreads();
+    const double N = scal[0], kn = scal[1], vn = scal[2];
+    const int T = D * (D + 1) / 2;
+    if (N == 0.0) {   // no points: the posterior is the prior itself
+        for (int d = threadIdx.x; d < D; d += blockDim.x) m[d] = h.m0[d];
+        for (int e = threadIdx.x; e < T; e += blockDim.x) {
+            int a, b;
+            tri_ij(e, a, b);
+            const double p = h.psi0[(size_t)a * D + b];
+            P[e] = scale_by_nu != 0.0 ? p * vn : p;
+        }
+        __syncthreads();
+        return;
+    }
+    for (int d = threadIdx.x; d < D; d += blockDim.x) {
+        double s = 0.0;
+        for (int t = 0; t < 4; ++t)
+            if (mem[t] >= 0) s += stats[(size_t)mem[t] * E + 1 + d];
+        m[d] = (h.k0 * h.m0[d] + s) / kn;
+    }
+    __syncthreads();
+    const double iv = 1.0 / vn;
+    for (int e = threadIdx.x; e < T; e += blockDim.x) {
+        int a, b;
+        tri_ij(e, a, b);
+        double s = 0.0;
+        for (int t = 0; t < 4; ++t)
+            if (mem[t] >= 0) s += stats[(size_t)mem[t] * E + 1 + DP + (size_t)b * DP + a];
+        const double p = (h.v0 * h.psi0[(size_t)b * D + a] + h.k0 * h.m0[b] * h.m0[a] + s - kn * m[b] * m[a]) * iv;
+        P[e] = scale_by_nu != 0.0 ? p * vn : p;
+    }
+    __syncthreads();
+}
+
+// In-place Cholesky of packed lower P (right-looking, one column per step).  Uniform result.
+__device__ bool chol_packed(double* P, int D, double* sh) {
+    for (int j = 0; j < D; ++j) {
+        if (threadIdx.x == 0) {
+            const double s = P[tri(j, j)];
+            const bool ok = s > 0.0 && isfinite(s);
+            const double d = ok ? sqrt(s) : 1.0;
+            P[tri(j, j)] = d;
+            sh[0] = 1.0 / d;
+            sh[1] = ok ? 1.0 : 0.0;
+        }
+        __syncthreads();
+        if (sh[1] == 0.0) return false;
+        const double inv = sh[0];
+        for (int i = j + 1 + threadIdx.x; i < D; i += blockDim.x) P[tri(i, j)] *= inv;
+        __syncthreads();
+        const int n = D - j - 1, T = n * (n + 1) / 2;
+        for (int e = threadIdx.x; e < T; e += blockDim.x) {
+            int ii, kk;
+            tri_ij(e, ii, kk);
+            const int i = j + 1 + ii, k = j + 1 + kk;
+            P[tri(i, k)] -= P[tri(i, j)] * P[tri(k, j)];
+        }
+        __syncthreads();
+    }
+    return true;
+}
+
+// X = L^-1 for packed lower L (one thread per column, rows in order).
+__device__ void lower_inverse_packed(const double* L, double* X, int D) {
+    for (int j = threadIdx.x; j < D; j += blockDim.x) {
+        X[tri(j, j)] = 1.0 / L[tri(j, j)];
+        for (int i = j + 1; i < D; ++i) {
+            double s = 0.0;
+            for (int k = j; k < i; ++k) s += L[tri(i, k)] * X[tri(k, j)];
+            X[tri(i, j)] = -s / L[tri(i, i)];
+        }
+    }
+    __syncthreads();
+}
+
+// G = X^T X (lower part) for packed lower X, each entry divided by `div`.
+__device__ void gram_t_packed(const double* X, double* G, int D, double div) {
+    const int T = D * (D + 1) / 2;
+    for (int e = threadIdx.x; e < T; e += blockDim.x) {
+        int a, b;
+        tri_ij(e, a, b);
+        double s = 0.0;
+        for (int k = a; k < D; ++k) s += X[tri(k, a)] * X[tri(k, b)];
+        G[e] = div != 1.0 ? s / div : s;
+    }
+    __syncthreads();
+}
+
+// ---- kernels -------------------------------------------------------------------------------------
+
+// out[2 j] = log marginal likelihood, out[2 j + 1] = log det psi_n (NaN when psi_n is not SPD).
+__global__ void __launch_bounds__(kThreads) k_log_marginal(const double* __restrict__ stats, int E, int DP, int D,
+                                                           int prior, const int32_t* __restrict__ members,
+                                                           PriorArgs h, double* __restrict__ out) {
+    extern __shared__ double lds[];
+    __shared__ double red[8];
+    __shared__ double scal[4];
+    __shared__ double sh[2];
+    __shared__ int mem[4];
+    const int job = blockIdx.x;
+    if (threadIdx.x < 4) mem[threadIdx.x] = members[4 * job + threadIdx.x];
+    __syncthreads();
+    if (prior == DPMM_PRIOR_MULTINOMIAL) {
+        // lgamma(sum a0) - lgamma(sum an) + sum_d [lgamma(an_d) - lgamma(a0_d)]
+        double part = 0.0, an_part = 0.0;
+        for (int d = threadIdx.x; d < D; d += blockDim.x) {
+            double s = 0.0;
+            for (int t = 0; t < 4; ++t)
+                if (mem[t] >= 0) s += stats[(size_t)mem[t] * E + 1 + d];
+            const double an = h.alpha0[d] + s;
+            an_part += an;
+            part += lgamma(an) - lgamma(h.alpha0[d]);
+        }
+        const double sum_an = block_sum(an_part, red);
+        const double tot = block_sum(part, red);
+        if (threadIdx.x == 0) {
+            out[2 * job] = h.lg_alpha0_sum - lgamma(sum_an) + tot;
+            out[2 * job + 1] = 0.0;
+        }
+        return;
+    }
+    double* P = lds;
+    double* m = lds + D * (D + 1) / 2;
+    niw_posterior(stats, (size_t)E, DP, D, mem, h, 0.0, P, m, scal);
+    const bool ok = chol_packed(P, D, sh);
+    double ld = 0.0, lg = 0.0;
+    const double vn = scal[2];
+    for (int d = threadIdx.x; d < D; d += blockDim.x) {
+        if (ok) ld += log(P[tri(d, d)]);
+        lg += lgamma(vn / 2.0 + (double)(1 - (d + 1)) / 2.0);
+    }
+    ld = 2.0 * block_sum(ld, red);
+    lg = block_sum(lg, red) + (double)D * (D - 1) / 4.0 * log(kPiF64);
+    if (threadIdx.x == 0) {
+        const double N = scal[0], kn = scal[1];
+        double L;
+        if (!ok) {
+            L = __builtin_nan("");
+            ld = __builtin_nan("");
+        } else if (N == 0.0) {
+            L = 0.0;   // the posterior is the prior: every term cancels
+            ld = h.logdet_psi0;
+        } else {
+            L = -N * (double)D * 0.5 * log(kPiF64) + lg - h.lmvg0 +
+                (h.v0 / 2.0) * (D * log(h.v0) + h.logdet_psi0) - (vn / 2.0) * (D * log(vn) + ld) +
+                (D / 2.0) * log(h.k0 / kn);
+        }
+        out[2 * job] = L;
+        out[2 * job + 1] = ld;
+    }
+}
+
+// Part p = 3k + r of the draw (r: 0 cluster, 1 left, 2 right): members[4p..], Philox stream
+// (seed, p, iter, TAG_PARAM); output row p of `draws`
+// (Gaussian: [mu (D) | R packed lower (D (D + 1) / 2) | logdet Sigma]; multinomial: log theta (D)).
+// status[p] = 0 on success, 1 if a factorisation failed.
+__global__ void __launch_bounds__(kThreads) k_draw_params(const double* __restrict__ stats, int E, int DP, int D,
+                                                          int prior, const int32_t* __restrict__ members,
+                                                          PriorArgs h,
+                                                          uint32_t iter, uint32_t key0, uint32_t key1,
+                                                          int full_mu_cov, int draw_stride,
+                                                          double* __restrict__ draws, int32_t* __restrict__ status) {
+    extern __shared__ double lds[];
+    __shared__ double red[8];
+    __shared__ double scal[4];
+    __shared__ double sh[2];
+    __shared__ int mem[4];
+    __shared__ uint32_t wpos;
+    const int p = blockIdx.x;
+    const uint32_t c0 = (uint32_t)p;
+    if (threadIdx.x < 4) mem[threadIdx.x] = members[4 * p + threadIdx.x];
+    __syncthreads();
+    double* out = draws + (size_t)p * draw_stride;
+    if (prior == DPMM_PRIOR_MULTINOMIAL) {
+        // Dirichlet(alpha0 + sum): D sequential gammas, normalised, logged
+        double* g = lds;
+        for (int d = threadIdx.x; d < D; d += blockDim.x) {
+            double s = 0.0;
+            for (int t = 0; t < 4; ++t)
+                if (mem[t] >= 0) s += stats[(size_t)mem[t] * E + 1 + d];
+            g[d] = h.alpha0[d] + s;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            DevStream rs(c0, iter, TAG_PARAM, key0, key1, 0);
+            double s = 0.0;
+            for (int d = 0; d < D; ++d) s += (g[d] = rs.gamma(g[d]));
+            sh[0] = s;
+        }
+        __syncthreads();
+        const double s = sh[0];
+        for (int d = threadIdx.x; d < D; d += blockDim.x) out[d] = log(g[d] / s);
+        if (threadIdx.x == 0) status[p] = 0;
+        return;
+    }
+    const int T = D * (D + 1) / 2;
+    double* P = lds;
+    double* Q = lds + T;
+    double* m = lds + 2 * T;
+    double* z = m + D;
+    double* gd = z + D;
+    // C = chol(nu psi_n); B = chol((nu psi_n)^-1) = chol(C^-T C^-1)
+    niw_posterior(stats, (size_t)E, DP, D, mem, h, 1.0, P, m, scal);
+    const double kn = scal[1], vn = scal[2];
+    bool ok = chol_packed(P, D, sh);
+    if (ok) {
+        lower_inverse_packed(P, Q, D);
+        gram_t_packed(Q, P, D, 1.0);
+        ok = chol_packed(P, D, sh);
+    }
+    if (!ok) {
+        if (threadIdx.x == 0) status[p] = 1;
+        return;
+    }
+    // Bartlett factor A (into Q): normals below the diagonal in row order (words 2m, 2m + 1 of
+    // normal m), then the D diagonal chi variates sqrt(2 Gamma((nu - i) / 2)) in sequence, then
+    // the D normals of mu.
+    for (int e = threadIdx.x; e < T; e += blockDim.x) {
+        int i, j;
+        tri_ij(e, i, j);
+        if (j < i) Q[e] = stream_normal_at(c0, iter, key0, key1, 2u * (uint32_t)(i * (i - 1) / 2 + j));
+    }
+    if (threadIdx.x == 0) {
+        DevStream rs(c0, iter, TAG_PARAM, key0, key1, (uint32_t)(D * (D - 1)));
+        for (int i = 0; i < D; ++i) gd[i] = sqrt(2.0 * rs.gamma((vn - i) / 2.0));
+        wpos = rs.w;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < D; i += blockDim.x) {
+        Q[tri(i, i)] = gd[i];
+        z[i] = stream_normal_at(c0, iter, key0, key1, wpos + 2u * (uint32_t)i);
+    }
+    __syncthreads();
+    // R = B A (lower), in place over A: column j from the last row up, so every A[k][j] read is
+    // still unmodified.
+    for (int j = threadIdx.x; j < D; j += blockDim.x) {
+        for (int i = D - 1; i >= j; --i) {
+            double s = 0.0;
+            for (int k = j; k <= i; ++k) s += P[tri(i, k)] * Q[tri(k, j)];
+            Q[tri(i, j)] = s;
+        }
+    }
+    __syncthreads();
+    double ld = 0.0;
+    for (int i = threadIdx.x; i < D; i += blockDim.x) ld += log(Q[tri(i, i)]);
+    ld = block_sum(ld, red);
+    for (int e = threadIdx.x; e < T; e += blockDim.x) out[D + e] = Q[e];
+    if (threadIdx.x == 0) out[D + T] = -2.0 * ld;
+    // mu = m_n + chol(Sigma / kappa) z with Sigma = R^-T R^-1 (diagonal only unless full_mu_cov,
+    // the reference's pdflib.cpp:4007-4014)
+    lower_inverse_packed(Q, P, D);
+    gram_t_packed(P, Q, D, kn);
+    ok = chol_packed(Q, D, sh);
+    if (!ok) {
+        if (threadIdx.x == 0) status[p] = 1;
+        return;
+    }
+    for (int i = threadIdx.x; i < D; i += blockDim.x) {
+        double x = m[i];
+        if (full_mu_cov)
+            for (int j = 0; j <= i; ++j) x += Q[tri(i, j)] * z[j];
+        else
+            x += Q[tri(i, i)] * z[i];
+        out[i] = x;
+    }
+    if (threadIdx.x == 0) status[p] = 0;
+}
+
+// Round-to-nearest-even FP32 -> BF16 bits (NaN kept quiet).
+__device__ __forceinline__ uint16_t bf16_bits_rne(float f) {
+    const uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0;
+    return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+
+// One block per part p = 3k + r: r = 0 writes cluster slot k of P (and the BF16 screen slot k of
+// PB when non-null), r = 1, 2 sub-cluster slot 2k + r - 1 of SP.  logw: K cluster log weights
+// then 2K sub-cluster log weights.
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_pack_gauss(const double* __restrict__ draws, int draw_stride, int D,
+                                                         int DP, int DPM, const double* __restrict__ logw, int K,
+                                                         T* __restrict__ P, T* __restrict__ SP,
+                                                         unsigned char* __restrict__ PB) {
+    const int p = blockIdx.x, k = p / 3, r = p - 3 * k;
+    const double* src = draws + (size_t)p * draw_stride;
+    const int stride = gauss_slot_stride(DP);
+    T* dst = r == 0 ? P + (size_t)k * stride : SP + (size_t)(2 * k + r - 1) * stride;
+    const double lw = r == 0 ? logw[k] : logw[K + 2 * k + r - 1];
+    const int TR = D * (D + 1) / 2;
+    const double cst = -0.5 * src[D + TR] + lw;
+    for (int e = threadIdx.x; e < stride; e += blockDim.x) dst[e] = T(0);
+    uint16_t* tiles = nullptr;
+    const int mslot = mx_slot_bytes(DPM);
+    if (r == 0 && PB) {
+        tiles = reinterpret_cast<uint16_t*>(PB + (size_t)k * mslot);
+        for (int e = threadIdx.x; e < mslot / 2; e += blockDim.x) tiles[e] = 0;
+    }
+    __syncthreads();
+    // W = R^T: W[j][l] = R[l][j] for l >= j
+    for (int e = threadIdx.x; e < TR; e += blockDim.x) {
+        int l, j;
+        tri_ij(e, l, j);
+        const double v = src[D + e];
+        dst[tri_row_offset_rt(DP, j) - (j & ~3) + l] = (T)v;
+        if (tiles) tiles[mx_tile_index(DPM, j, l)] = bf16_bits_rne((float)v);
+    }
+    for (int l = threadIdx.x; l < D; l += blockDim.x) {
+        dst[gauss_mu_offset(DP) + l] = (T)src[l];
+        if (tiles) reinterpret_cast<float*>(PB + (size_t)k * mslot + mx_tiles_bytes(DPM))[l] = (float)src[l];
+    }
+    if (threadIdx.x == 0) {
+        dst[gauss_c_offset(DP)] = (T)cst;
+        if (tiles) reinterpret_cast<float*>(PB + (size_t)k * mslot + mx_tiles_bytes(DPM))[DPM] = (float)cst;
+    }
+}
+
+// Multinomial slots: [log theta (D) | log weight | pad] in FP32; clusters carry log pi_k only when
+// use_weight (the reference's label score leaves it out, cudaKernel_multinomial.cu:60-76).
+__global__ void __launch_bounds__(kThreads) k_pack_mult(const double* __restrict__ draws, int draw_stride, int D,
+                                                        int stride, const double* __restrict__ logw, int K,
+                                                        int use_weight, float* __restrict__ P,
+                                                        float* __restrict__ SP) {
+    const int p = blockIdx.x, k = p / 3, r = p - 3 * k;
+    const double* src = draws + (size_t)p * draw_stride;
+    float* dst = r == 0 ? P + (size_t)k * stride : SP + (size_t)(2 * k + r - 1) * stride;
+    for (int e = threadIdx.x; e < stride; e += blockDim.x) {
+        float v = 0.f;
+        if (e < D) v = (float)src[e];
+        else if (e == D) v = r == 0 ? (use_weight ? (float)logw[k] : 0.f) : (float)logw[K + 2 * k + r - 1];
+        dst[e] = v;
+    }
+}
+
+}  // namespace dpmm

@@ -1,21 +1,21 @@
 // Host driver of the sub-cluster split/merge sampler (Chang & Fisher): the per-iteration
 // control flow of local_clusters_actions::group_step (local_clusters_actions.cpp:460-490) and
 // run_model (dp_parallel_sampling.cpp:175-249), driving the HIP device layer through its C ABI.
-// The FP64 conjugate math (posteriors, log marginal likelihoods, Hastings ratios, parameter
-// draws) runs here, parallel over clusters and merge pairs.
+//
+// The host holds, per cluster part (cluster, left, right), only its point count, the list of
+// sub-cluster statistics entries whose sum it is, and that sum's log marginal likelihood.  All
+// FP64 matrix work runs on the device in batched launches: one log-marginal launch per statistics
+// pass covers every cluster, sub-cluster and candidate merge pair, and one draw launch samples
+// every part's parameters and writes the sweeps' parameter slots.  The host does the scalar work:
+// Dirichlet weights, burn-out history, Hastings ratios and the greedy merge resolution.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
-#include <limits>
 #include <memory>
 #include <string>
 #include <vector>
-
-#ifdef _OPENMP
-#include <omp.h>
-#endif
 
 #include "../../include/dpmm_hip.h"
 #include "dpmm_device.h"
@@ -23,8 +23,6 @@
 
 namespace dpmm {
 namespace {
-
-constexpr double kPi = 3.14159265358979323846;
 
 // ------------------------------------------------------------------ random streams -------
 // Sequential stream (c0, n, t, tag): words of Philox blocks n = 0, 1, 2, ... in order.
@@ -43,7 +41,7 @@ class Stream {
     double normal() {   // Box-Muller, cosine branch
         const double a = uniform();
         const double b = uniform();
-        return std::sqrt(-2.0 * std::log(a)) * std::cos(2.0 * kPi * b);
+        return std::sqrt(-2.0 * std::log(a)) * std::cos(2.0 * 3.14159265358979323846 * b);
     }
     double gamma(double a) {   // Marsaglia & Tsang (2000); a < 1 through Gamma(a + 1) U^(1/a)
         if (!(a > 0.0)) return 0.0;
@@ -73,81 +71,13 @@ class Stream {
     int pos_ = 4;
 };
 
-// ------------------------------------------------------------------ dense FP64 ------------
-using Mat = std::vector<double>;   // row-major D x D
-
-bool cholesky(int D, Mat& A) {   // in place, lower factor, upper zeroed
-    for (int j = 0; j < D; ++j) {
-        double* rj = &A[(size_t)j * D];
-        double s = rj[j];
-        for (int k = 0; k < j; ++k) s -= rj[k] * rj[k];
-        if (!(s > 0.0) || !std::isfinite(s)) return false;
-        const double d = std::sqrt(s);
-        rj[j] = d;
-        const double inv = 1.0 / d;
-        for (int i = j + 1; i < D; ++i) {
-            double* ri = &A[(size_t)i * D];
-            double t = ri[j];
-            for (int k = 0; k < j; ++k) t -= ri[k] * rj[k];
-            ri[j] = t * inv;
-        }
-        for (int k = j + 1; k < D; ++k) rj[k] = 0.0;
-    }
-    return true;
-}
-
-Mat lower_inverse(int D, const Mat& L) {
-    Mat X((size_t)D * D, 0.0);
-    for (int j = 0; j < D; ++j) {
-        X[(size_t)j * D + j] = 1.0 / L[(size_t)j * D + j];
-        for (int i = j + 1; i < D; ++i) {
-            double s = 0.0;
-            for (int k = j; k < i; ++k) s += L[(size_t)i * D + k] * X[(size_t)k * D + j];
-            X[(size_t)i * D + j] = -s / L[(size_t)i * D + i];
-        }
-    }
-    return X;
-}
-
-Mat gram_t(int D, const Mat& X) {   // X^T X for lower-triangular X
-    Mat G((size_t)D * D);
-    for (int a = 0; a < D; ++a)
-        for (int b = 0; b <= a; ++b) {
-            double s = 0.0;
-            for (int k = a; k < D; ++k) s += X[(size_t)k * D + a] * X[(size_t)k * D + b];
-            G[(size_t)a * D + b] = s;
-            G[(size_t)b * D + a] = s;
-        }
-    return G;
-}
-
-double logdet_spd(int D, const Mat& A) {
-    Mat L = A;
-    if (!cholesky(D, L)) return std::numeric_limits<double>::quiet_NaN();
-    double s = 0.0;
-    for (int i = 0; i < D; ++i) s += std::log(L[(size_t)i * D + i]);
-    return 2.0 * s;
-}
-
 // ------------------------------------------------------------------ model ----------------
-struct Hyp {          // niw_hyperparams / multinomial_hyper
-    double k = 0, v = 0;
-    std::vector<double> m, psi, alpha;
-    double logdet_psi = 0;   // cached
-    double alpha_sum = 0;    // cached
-};
-struct Suff {
-    int64_t N = 0;
-    std::vector<double> sum, S;
-};
-struct Draw {         // mv_gaussian through the factor R of invSigma, or log theta
-    std::vector<double> mu, R, logtheta;
-    double logdet_sigma = 0;
-};
+// A cluster part: the sum of up to four sub-cluster statistics entries of the last statistics
+// pass (entry q = 2k + s), its point count and the log marginal likelihood of that sum.
 struct Part {
-    Hyp post;
-    Suff st;
-    Draw dist;
+    int64_t N = 0;
+    int32_t mem[4] = {-1, -1, -1, -1};
+    double lml = 0.0;
 };
 struct Cluster {
     Part c, l, r;
@@ -156,10 +86,12 @@ struct Cluster {
     std::vector<std::pair<bool, double>> hist;
 };
 
-double lmvgamma(double x, int D) {
-    double r = (double)D * (D - 1) / 4.0 * std::log(kPi);
-    for (int d = 1; d <= D; ++d) r += std::lgamma(x + (1 - d) / 2.0);
-    return r;
+Part entry_part(int64_t N, int q0, int q1) {
+    Part p;
+    p.N = N;
+    p.mem[0] = q0;
+    p.mem[1] = q1;
+    return p;
 }
 
 }  // namespace
@@ -170,165 +102,94 @@ struct SamplerImpl {
     int64_t N = 0;
     int D = 0;
     int prior = 0;
-    Hyp h0;
     std::vector<Cluster> cl;
     std::vector<double> weights;
     double host_ms = 0;
-    // scratch
     std::vector<int64_t> Nsub;
-    std::vector<double> ssum, sS;
-
-    bool gauss() const { return prior == DPMM_PRIOR_GAUSSIAN; }
-
-    // niw::calc_posterior (priors/niw.cpp:23-49); multinomial_prior::calc_posterior (12-22).
-    Hyp posterior(const Suff& s) const {
-        if (s.N == 0) return h0;
-        Hyp p;
-        if (!gauss()) {
-            p.alpha.resize(D);
-            p.alpha_sum = 0;
-            for (int d = 0; d < D; ++d) p.alpha_sum += (p.alpha[d] = h0.alpha[d] + s.sum[d]);
-            return p;
-        }
-        const double n = (double)s.N;
-        p.k = h0.k + n;
-        p.v = h0.v + n;
-        p.m.resize(D);
-        for (int d = 0; d < D; ++d) p.m[d] = (h0.k * h0.m[d] + s.sum[d]) / p.k;
-        p.psi.resize((size_t)D * D);
-        const double iv = 1.0 / p.v;
-        for (int a = 0; a < D; ++a)
-            for (int b = a; b < D; ++b) {
-                const double val = (h0.v * h0.psi[(size_t)a * D + b] + h0.k * h0.m[a] * h0.m[b] + s.S[(size_t)a * D + b] -
-                                    p.k * p.m[a] * p.m[b]) * iv;
-                p.psi[(size_t)a * D + b] = val;
-                p.psi[(size_t)b * D + a] = val;
-            }
-        p.logdet_psi = logdet_spd(D, p.psi);
-        return p;
-    }
-
-    // niw::log_marginal_likelihood (priors/niw.cpp:122-138) / multinomial (56-67).
-    double log_marginal(const Hyp& p, const Suff& s) const {
-        if (!gauss()) {
-            double v = std::lgamma(h0.alpha_sum) - std::lgamma(p.alpha_sum);
-            for (int d = 0; d < D; ++d) v += std::lgamma(p.alpha[d]) - std::lgamma(h0.alpha[d]);
-            return v;
-        }
-        return -(double)(s.N * (int64_t)D) * 0.5 * std::log(kPi) + lmvgamma(p.v / 2.0, D) - lmvgamma(h0.v / 2.0, D) +
-               (h0.v / 2.0) * (D * std::log(h0.v) + h0.logdet_psi) - (p.v / 2.0) * (D * std::log(p.v) + p.logdet_psi) +
-               (D / 2.0) * std::log(h0.k / p.k);
-    }
-
-    // niw::sample_distribution (priors/niw.cpp:82-107): invSigma = R R^T with R = B A, B the
-    // Cholesky factor of (v psi)^-1 and A the Bartlett factor (rinvwish.ipp:35-58).  R is
-    // exactly the Cholesky factor the sweep needs, so no factorisation of invSigma follows.
-    bool draw(const Hyp& p, uint32_t slot, uint32_t t, Draw& out) const {
-        Stream rs(o.seed, slot, t, TAG_PARAM);
-        if (!gauss()) {
-            out.logtheta.resize(D);
-            rs.dirichlet(p.alpha.data(), D, out.logtheta.data());
-            for (auto& x : out.logtheta) x = std::log(x);
-            return true;
-        }
-        Mat C(p.psi);
-        for (auto& x : C) x *= p.v;
-        if (!cholesky(D, C)) return false;
-        Mat B = gram_t(D, lower_inverse(D, C));   // (v psi)^-1 = C^-T C^-1
-        if (!cholesky(D, B)) return false;
-        Mat A((size_t)D * D, 0.0);
-        for (int i = 1; i < D; ++i)
-            for (int j = 0; j < i; ++j) A[(size_t)i * D + j] = rs.normal();
-        for (int i = 0; i < D; ++i) A[(size_t)i * D + i] = std::sqrt(2.0 * rs.gamma((p.v - i) / 2.0));
-        out.R.assign((size_t)D * D, 0.0);
-        double ld = 0.0;
-        for (int i = 0; i < D; ++i) {
-            for (int j = 0; j <= i; ++j) {
-                double s = 0.0;
-                for (int k = j; k <= i; ++k) s += B[(size_t)i * D + k] * A[(size_t)k * D + j];
-                out.R[(size_t)i * D + j] = s;
-            }
-            ld += std::log(out.R[(size_t)i * D + i]);
-        }
-        out.logdet_sigma = -2.0 * ld;
-        Mat Sig = gram_t(D, lower_inverse(D, out.R));   // Sigma = R^-T R^-1
-        for (auto& x : Sig) x /= p.k;
-        if (!cholesky(D, Sig)) return false;
-        out.mu.resize(D);
-        std::vector<double> z(D);
-        for (int i = 0; i < D; ++i) z[i] = rs.normal();
-        for (int i = 0; i < D; ++i) {
-            double x = p.m[i];
-            if (o.full_mu_cov)
-                for (int j = 0; j <= i; ++j) x += Sig[(size_t)i * D + j] * z[j];
-            else
-                x += Sig[(size_t)i * D + i] * z[i];   // the reference reads only the diagonal (pdflib.cpp:4007-4014)
-            out.mu[i] = x;
-        }
-        return true;
-    }
+    // merge candidates of the current iteration: clusters and log ML of every pair's union
+    std::vector<int> cand_pos;
+    std::vector<double> cand_L;
+    int ncand = 0;
 
     void reset_hist(Cluster& c) const { c.hist.assign(o.burnout + 5, std::make_pair(false, 0.0)); }
 
-    Suff empty_suff() const {
-        Suff s;
-        s.sum.assign(D, 0.0);
-        if (gauss()) s.S.assign((size_t)D * D, 0.0);
-        return s;
-    }
-
-    static void add_suff(Suff& a, const Suff& b) {
-        a.N += b.N;
-        for (size_t i = 0; i < a.sum.size(); ++i) a.sum[i] += b.sum[i];
-        for (size_t i = 0; i < a.S.size(); ++i) a.S[i] += b.S[i];
-    }
-
     // create_sufficient_statistics + update_suff_stats_posterior (local_clusters_actions.cpp:
-    // 109-168) for the clusters flagged in `which` (empty: all).
-    int update_stats(const std::vector<char>& which) {
+    // 109-168) for the clusters flagged in `which` (empty: all), with the log marginal likelihoods
+    // of their parts from one device launch.  With `pairs`, the same launch also evaluates the
+    // union of every pair of merge candidates (splittable, non-empty clusters).
+    int update_stats(const std::vector<char>& which, bool pairs) {
         const int K = (int)cl.size();
         Nsub.resize(2 * K);
-        ssum.resize((size_t)2 * K * D);
-        if (gauss()) sS.resize((size_t)2 * K * D * D);
-        int rc = dpmm_sufficient_statistics(ctx, K, Nsub.data(), ssum.data(), gauss() ? sS.data() : nullptr);
+        int rc = dpmm_sufficient_statistics(ctx, K, Nsub.data(), nullptr, nullptr);
         if (rc) return rc;
         const auto t0 = std::chrono::steady_clock::now();
-#pragma omp parallel for schedule(dynamic)
+        std::vector<int32_t> mem;
+        std::vector<Part*> dst;
+        auto job = [&](Part* p) {
+            mem.insert(mem.end(), p->mem, p->mem + 4);
+            dst.push_back(p);
+        };
         for (int k = 0; k < K; ++k) {
             if (!which.empty() && !which[k]) continue;
             Cluster& c = cl[k];
-            for (int s = 0; s < 2; ++s) {
-                Suff& st = s ? c.r.st : c.l.st;
-                const int q = 2 * k + s;
-                st.N = Nsub[q];
-                st.sum.assign(ssum.begin() + (size_t)q * D, ssum.begin() + (size_t)(q + 1) * D);
-                if (gauss()) st.S.assign(sS.begin() + (size_t)q * D * D, sS.begin() + (size_t)(q + 1) * D * D);
-            }
-            c.c.st = c.l.st;   // the cluster's statistics aggregate its two sub-clusters (niw.cpp:140-148)
-            add_suff(c.c.st, c.r.st);
-            c.c.post = posterior(c.c.st);
-            c.l.post = posterior(c.l.st);
-            c.r.post = posterior(c.r.st);
+            c.l = entry_part(Nsub[2 * k], 2 * k, -1);
+            c.r = entry_part(Nsub[2 * k + 1], 2 * k + 1, -1);
+            c.c = entry_part(Nsub[2 * k] + Nsub[2 * k + 1], 2 * k, 2 * k + 1);   // cluster = l + r (niw.cpp:140-148)
+            job(&c.c);
+            job(&c.l);
+            job(&c.r);
         }
+        const int nparts = (int)dst.size();
+        if (pairs) {
+            cand_pos.assign(K, -1);
+            std::vector<int> cand;
+            for (int k = 0; k < K; ++k)
+                if (cl[k].splittable && cl[k].c.N > 0) {
+                    cand_pos[k] = (int)cand.size();
+                    cand.push_back(k);
+                }
+            ncand = (int)cand.size();
+            for (int a = 0; a < ncand; ++a)
+                for (int b = a + 1; b < ncand; ++b) {
+                    const int i = cand[a], j = cand[b];
+                    const int32_t m[4] = {2 * i, 2 * i + 1, 2 * j, 2 * j + 1};
+                    mem.insert(mem.end(), m, m + 4);
+                }
+        }
+        const int n = (int)(mem.size() / 4);
+        std::vector<double> L(n);
+        rc = dpmm_log_marginal_batch(ctx, n, mem.data(), L.data(), nullptr);
+        if (rc) return rc;
+        for (int j = 0; j < nparts; ++j) {
+            if (!std::isfinite(L[j])) return DPMM_ENOTSPD;
+            dst[j]->lml = L[j];
+        }
+        if (pairs) cand_L.assign(L.begin() + nparts, L.end());
         host_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         return DPMM_OK;
     }
 
+    double pair_L(int i, int j) const {   // i < j, both candidates
+        const int a = cand_pos[i], b = cand_pos[j];
+        // row a of the upper triangle without diagonal
+        const size_t off = (size_t)a * ncand - (size_t)a * (a + 1) / 2 + (b - a - 1);
+        return cand_L[off];
+    }
+
     // sample_clusters (local_clusters_actions.cpp:341-371) + sample_cluster_params
-    // (shared_actions.cpp:9-55).
+    // (shared_actions.cpp:9-55): weights and burn-out history here, the 3K draws on the device.
     int sample_clusters(uint32_t t) {
         const auto t0 = std::chrono::steady_clock::now();
         const int K = (int)cl.size();
-        int bad = 0;
-#pragma omp parallel for schedule(dynamic) reduction(+ : bad)
+        std::vector<int32_t> mem((size_t)12 * K);
+        std::vector<double> lrw(2 * K);
         for (int k = 0; k < K; ++k) {
             Cluster& c = cl[k];
-            bad += !draw(c.c.post, 3 * k + 0, t, c.c.dist);
-            bad += !draw(c.l.post, 3 * k + 1, t, c.l.dist);
-            bad += !draw(c.r.post, 3 * k + 2, t, c.r.dist);
-            const double a[2] = {(double)c.l.st.N + o.alpha / 2, (double)c.r.st.N + o.alpha / 2};
+            const double a[2] = {(double)c.l.N + o.alpha / 2, (double)c.r.N + o.alpha / 2};
             Stream(o.seed, k, t, TAG_LRW).dirichlet(a, 2, c.lrw);
-            const double L = log_marginal(c.l.post, c.l.st) + log_marginal(c.r.post, c.r.st);
+            lrw[2 * k] = c.lrw[0];
+            lrw[2 * k + 1] = c.lrw[1];
+            const double L = c.l.lml + c.r.lml;
             const int B = o.burnout;
             std::rotate(c.hist.begin(), c.hist.begin() + 1, c.hist.begin() + B);
             c.hist[B - 1] = std::make_pair(true, L);
@@ -339,50 +200,20 @@ struct SamplerImpl {
                 else ready = false;
             }
             if (ready && now - c.hist[B - 1].second < 1e-2) c.splittable = true;
+            const Part* ps[3] = {&c.c, &c.l, &c.r};
+            for (int p = 0; p < 3; ++p) std::copy(ps[p]->mem, ps[p]->mem + 4, mem.begin() + (size_t)4 * (3 * k + p));
         }
         // mixture weights ~ Dir(N_1, ..., N_K, alpha), last entry dropped (357-365)
         std::vector<double> a(K + 1), w(K + 1);
-        for (int k = 0; k < K; ++k) a[k] = (double)cl[k].c.st.N;
+        for (int k = 0; k < K; ++k) a[k] = (double)cl[k].c.N;
         a[K] = o.alpha;
         if (K == 1 && a[0] == 0.0) { w[0] = 0.0; w[1] = 1.0; }   // get_dirichlet_distribution (323-339)
         else Stream(o.seed, 0, t, TAG_MIXW).dirichlet(a.data(), K + 1, w.data());
         weights.assign(w.begin(), w.begin() + K);
+        const int rc = dpmm_sample_params(ctx, t, K, mem.data(), weights.data(), lrw.data(), o.full_mu_cov,
+                                          o.mult_use_weight);
         host_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-        if (bad) {
-            return -4;
-        }
-        return DPMM_OK;
-    }
-
-    int upload_params() {
-        const int K = (int)cl.size();
-        std::vector<double> lrw(2 * K);
-        for (int k = 0; k < K; ++k) { lrw[2 * k] = cl[k].lrw[0]; lrw[2 * k + 1] = cl[k].lrw[1]; }
-        if (gauss()) {
-            std::vector<double> mu((size_t)K * D), R((size_t)K * D * D), ld(K);
-            std::vector<double> smu((size_t)2 * K * D), sR((size_t)2 * K * D * D), sld(2 * K);
-            for (int k = 0; k < K; ++k) {
-                const Draw* ds[3] = {&cl[k].c.dist, &cl[k].l.dist, &cl[k].r.dist};
-                std::copy(ds[0]->mu.begin(), ds[0]->mu.end(), mu.begin() + (size_t)k * D);
-                std::copy(ds[0]->R.begin(), ds[0]->R.end(), R.begin() + (size_t)k * D * D);
-                ld[k] = ds[0]->logdet_sigma;
-                for (int s = 0; s < 2; ++s) {
-                    const int q = 2 * k + s;
-                    std::copy(ds[1 + s]->mu.begin(), ds[1 + s]->mu.end(), smu.begin() + (size_t)q * D);
-                    std::copy(ds[1 + s]->R.begin(), ds[1 + s]->R.end(), sR.begin() + (size_t)q * D * D);
-                    sld[q] = ds[1 + s]->logdet_sigma;
-                }
-            }
-            return dpmm_set_gaussian_factors(ctx, K, mu.data(), R.data(), ld.data(), weights.data(), smu.data(),
-                                             sR.data(), sld.data(), lrw.data());
-        }
-        std::vector<double> lt((size_t)K * D), slt((size_t)2 * K * D);
-        for (int k = 0; k < K; ++k) {
-            std::copy(cl[k].c.dist.logtheta.begin(), cl[k].c.dist.logtheta.end(), lt.begin() + (size_t)k * D);
-            std::copy(cl[k].l.dist.logtheta.begin(), cl[k].l.dist.logtheta.end(), slt.begin() + (size_t)(2 * k) * D);
-            std::copy(cl[k].r.dist.logtheta.begin(), cl[k].r.dist.logtheta.end(), slt.begin() + (size_t)(2 * k + 1) * D);
-        }
-        return dpmm_set_multinomial_params(ctx, K, lt.data(), weights.data(), slt.data(), lrw.data(), o.mult_use_weight);
+        return rc;
     }
 
     int relabel(uint32_t t, const std::vector<int32_t>& nl, const std::vector<int32_t>& act) {
@@ -394,14 +225,8 @@ struct SamplerImpl {
         if (rc) return rc;
         const int K0 = std::max(1, o.init_clusters);
         cl.assign(K0, Cluster());
-        for (auto& c : cl) {
-            c.c.post = h0;
-            c.c.st = empty_suff();
-            c.l = c.c;
-            c.r = c.c;
-            reset_hist(c);
-        }
-        rc = update_stats({});
+        for (auto& c : cl) reset_hist(c);
+        rc = update_stats({}, false);
         if (rc) return rc;
         return sample_clusters(0);
     }
@@ -416,13 +241,13 @@ struct SamplerImpl {
             rc = sample_clusters(it);
             if (rc) return rc;
         }
-        rc = upload_params();
-        if (rc) return rc;
         rc = dpmm_sample_cluster_labels(ctx, it, fin ? 1 : 0);
         if (rc) return rc;
         rc = dpmm_sample_sub_labels(ctx, it);
         if (rc) return rc;
-        rc = update_stats({});
+        // one log-marginal launch: every part, and every pair of merge candidates (a cluster that is
+        // reset or split below leaves the candidates, and the others keep these statistics)
+        rc = update_stats({}, !no_more_splits);
         if (rc) return rc;
 
         // reset_bad_clusters (local_clusters_actions.cpp:429-447)
@@ -431,7 +256,7 @@ struct SamplerImpl {
             std::vector<char> bad(K, 0);
             bool any = false;
             for (int k = 0; k < K; ++k)
-                if (cl[k].l.st.N == 0 || cl[k].r.st.N == 0) {
+                if (cl[k].l.N == 0 || cl[k].r.N == 0) {
                     bad[k] = 1;
                     any = true;
                     reset_hist(cl[k]);
@@ -442,7 +267,7 @@ struct SamplerImpl {
                 for (int q = 0; q < 2 * K; ++q) nl[q] = q / 2;
                 for (int k = 0; k < K; ++k) act[k] = bad[k] ? 1 : 0;
                 if ((rc = relabel(it, nl, act))) return rc;
-                if ((rc = update_stats(bad))) return rc;
+                if ((rc = update_stats(bad, false))) return rc;
             }
         }
 
@@ -451,28 +276,24 @@ struct SamplerImpl {
             // check_and_split (local_clusters_actions.cpp:244-285), should_split_local (215-242)
             const auto t0 = std::chrono::steady_clock::now();
             std::vector<char> acc(K, 0);
-#pragma omp parallel for schedule(dynamic)
             for (int k = 0; k < K; ++k) {
                 const Cluster& c = cl[k];
-                if (!(c.splittable && c.c.st.N > 1) || fin || c.l.st.N == 0 || c.r.st.N == 0) continue;
-                const double L = log_marginal(posterior(c.c.st), c.c.st);
-                const double Ll = log_marginal(posterior(c.l.st), c.l.st);
-                const double Lr = log_marginal(posterior(c.r.st), c.r.st);
-                const double lhr = std::log(o.alpha) + std::lgamma((double)c.l.st.N) + Ll +
-                                   std::lgamma((double)c.r.st.N) + Lr - (std::lgamma((double)c.c.st.N) + L);
+                if (!(c.splittable && c.c.N > 1) || fin || c.l.N == 0 || c.r.N == 0) continue;
+                const double lhr = std::log(o.alpha) + std::lgamma((double)c.l.N) + c.l.lml +
+                                   std::lgamma((double)c.r.N) + c.r.lml - (std::lgamma((double)c.c.N) + c.c.lml);
                 acc[k] = lhr > std::log(Stream(o.seed, k, it, TAG_SPLIT_U).uniform());
             }
             host_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
             std::vector<int32_t> nl(2 * K), act(K, 0);
             for (int q = 0; q < 2 * K; ++q) nl[q] = q / 2;
-            std::vector<char> touched;
             int nsplit = 0;
             for (int k = 0; k < K; ++k) {
                 if (!acc[k]) continue;
                 const int nk = (int)cl.size();
                 // split_cluster_local (183-194): k keeps the left half, nk gets the right half
                 Cluster left, right;
-                left.c = cl[k].l; right.c = cl[k].r;
+                left.c = cl[k].l;
+                right.c = cl[k].r;
                 left.l = left.r = left.c;
                 right.l = right.r = right.c;
                 reset_hist(left);
@@ -485,10 +306,10 @@ struct SamplerImpl {
             }
             if (nsplit) {
                 if ((rc = relabel(it, nl, act))) return rc;
-                touched.assign(cl.size(), 0);
+                std::vector<char> touched(cl.size(), 0);
                 for (int k = 0; k < K; ++k) touched[k] = acc[k];
                 for (size_t k = K; k < cl.size(); ++k) touched[k] = 1;
-                if ((rc = update_stats(touched))) return rc;
+                if ((rc = update_stats(touched, false))) return rc;
             }
 
             // check_and_merge (287-321) + should_merge (shared_actions.cpp:88-111): every
@@ -499,47 +320,42 @@ struct SamplerImpl {
             K = (int)cl.size();
             std::vector<int> elig;
             for (int k = 0; k < K; ++k)
-                if (cl[k].splittable && cl[k].c.st.N > 0) elig.push_back(k);
+                if (cl[k].splittable && cl[k].c.N > 0 && k < (int)cand_pos.size() && cand_pos[k] >= 0)
+                    elig.push_back(k);
             const int ne = (int)elig.size();
-            std::vector<double> Lc(K, 0.0);
-            for (int k : elig) Lc[k] = log_marginal(cl[k].c.post, cl[k].c.st);
-            std::vector<std::pair<int, int>> pairs;
-            for (int a = 0; a < ne; ++a)
-                for (int b = a + 1; b < ne; ++b) pairs.emplace_back(elig[a], elig[b]);
-            std::vector<char> ok(pairs.size(), 0);
             const double al = o.alpha;
             const double cst = -std::log(al) + std::lgamma(al) - 2 * std::lgamma(0.5 * al);
-#pragma omp parallel for schedule(dynamic)
-            for (int64_t p = 0; p < (int64_t)pairs.size(); ++p) {
-                const int i = pairs[p].first, j = pairs[p].second;
-                Suff m = cl[i].c.st;
-                add_suff(m, cl[j].c.st);
-                const double L = log_marginal(posterior(m), m);
-                const double Ni = (double)cl[i].c.st.N, Nj = (double)cl[j].c.st.N, Nm = Ni + Nj;
-                const double lhr = cst + std::lgamma(Nm) - std::lgamma(Nm + al) + std::lgamma(Ni + 0.5 * al) -
-                                   std::lgamma(Ni) - std::lgamma(Nj) + std::lgamma(Nj + 0.5 * al) + L - Lc[i] - Lc[j];
-                const double u = Stream(o.seed, ((uint32_t)i << 16) | (uint32_t)j, it, TAG_MERGE_U).uniform();
-                ok[p] = (lhr > std::log(u)) || (fin && lhr > std::log(0.1));
-            }
             merge_into.assign(K, -1);
             std::vector<char> used(K, 0);
-            for (size_t p = 0; p < pairs.size(); ++p) {
-                const int i = pairs[p].first, j = pairs[p].second;
-                if (!ok[p] || used[i] || used[j]) continue;
-                used[i] = used[j] = 1;
-                merge_into[j] = i;
-                // merge_clusters (204-213) + merge_clusters_to_splittable (shared_actions.cpp:72-86)
-                Cluster& a = cl[i];
-                Cluster& b = cl[j];
-                add_suff(a.c.st, b.c.st);
-                a.c.post = posterior(a.c.st);
-                a.l = a.c;   // the reference clones the merged parameters into the left sub-cluster
-                a.r = b.c;
-                a.splittable = false;
-                reset_hist(a);
-                b.c.st.N = 0;
-                b.splittable = false;
-            }
+            for (int a = 0; a < ne; ++a)
+                for (int b = a + 1; b < ne; ++b) {
+                    const int i = elig[a], j = elig[b];
+                    if (used[i] || used[j]) continue;
+                    const double L = pair_L(i, j);
+                    const double Ni = (double)cl[i].c.N, Nj = (double)cl[j].c.N, Nm = Ni + Nj;
+                    const double lhr = cst + std::lgamma(Nm) - std::lgamma(Nm + al) + std::lgamma(Ni + 0.5 * al) -
+                                       std::lgamma(Ni) - std::lgamma(Nj) + std::lgamma(Nj + 0.5 * al) + L -
+                                       cl[i].c.lml - cl[j].c.lml;
+                    const double u = Stream(o.seed, ((uint32_t)i << 16) | (uint32_t)j, it, TAG_MERGE_U).uniform();
+                    if (!((lhr > std::log(u)) || (fin && lhr > std::log(0.1)))) continue;
+                    used[i] = used[j] = 1;
+                    merge_into[j] = i;
+                    // merge_clusters (204-213) + merge_clusters_to_splittable (shared_actions.cpp:72-86)
+                    Cluster& A = cl[i];
+                    Cluster& Bc = cl[j];
+                    Part merged;
+                    merged.N = A.c.N + Bc.c.N;
+                    merged.mem[0] = 2 * i; merged.mem[1] = 2 * i + 1;
+                    merged.mem[2] = 2 * j; merged.mem[3] = 2 * j + 1;
+                    merged.lml = L;
+                    A.r = Bc.c;
+                    A.c = merged;
+                    A.l = merged;   // the reference clones the merged parameters into the left sub-cluster
+                    A.splittable = false;
+                    reset_hist(A);
+                    Bc.c.N = 0;
+                    Bc.splittable = false;
+                }
             host_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t1).count();
         }
 
@@ -548,7 +364,7 @@ struct SamplerImpl {
         std::vector<int32_t> remap(K, -1);
         int nk = 0;
         for (int k = 0; k < K; ++k)
-            if (cl[k].c.st.N > 0) remap[k] = nk++;
+            if (cl[k].c.N > 0) remap[k] = nk++;
         bool any_merge = false;
         for (int k = 0; k < (int)merge_into.size(); ++k) any_merge |= merge_into[k] >= 0;
         if (any_merge || nk != K) {
@@ -588,6 +404,12 @@ int dpmm_sampler_create(dpmm_ctx* ctx, const dpmm_options* opt, double k0, const
     int D, prior, prec;
     dpmm::ctx_dims(ctx, &N, &D, &prior, &prec);
     if (opt->prior != prior || opt->iters < 0 || opt->burnout < 1 || !(opt->alpha > 0)) return DPMM_EINVAL;
+    int rc = dpmm_set_prior(ctx, k0, m0, v0, psi0, alpha0);
+    if (rc) return rc;
+    if (opt->precision != prec) {
+        rc = dpmm_set_precision(ctx, opt->precision);
+        if (rc) return rc;
+    }
     auto* S = new dpmm_sampler();
     dpmm::SamplerImpl& s = S->s;
     s.ctx = ctx;
@@ -595,28 +417,7 @@ int dpmm_sampler_create(dpmm_ctx* ctx, const dpmm_options* opt, double k0, const
     s.N = N;
     s.D = D;
     s.prior = prior;
-    if (prior == DPMM_PRIOR_GAUSSIAN) {
-        if (!m0 || !psi0 || !(k0 > 0) || !(v0 > D - 1)) { delete S; return DPMM_EINVAL; }
-        s.h0.k = k0;
-        s.h0.v = v0;
-        s.h0.m.assign(m0, m0 + D);
-        s.h0.psi.assign(psi0, psi0 + (size_t)D * D);
-        s.h0.logdet_psi = dpmm::logdet_spd(D, s.h0.psi);
-        if (!std::isfinite(s.h0.logdet_psi)) { delete S; return DPMM_ENOTSPD; }
-    } else {
-        if (!alpha0) { delete S; return DPMM_EINVAL; }
-        s.h0.alpha.assign(alpha0, alpha0 + D);
-        s.h0.alpha_sum = 0;
-        for (double a : s.h0.alpha) {
-            if (!(a > 0)) { delete S; return DPMM_EINVAL; }
-            s.h0.alpha_sum += a;
-        }
-    }
-    if (opt->precision != prec) {
-        int rc = dpmm_set_precision(ctx, opt->precision);
-        if (rc) { delete S; return rc; }
-    }
-    int rc = s.init();
+    rc = s.init();
     if (rc) { delete S; return rc; }
     *out = S;
     return DPMM_OK;
@@ -634,7 +435,7 @@ int dpmm_sampler_weights(dpmm_sampler* S, double* w, int64_t* counts, int max_k)
     const int K = (int)S->s.cl.size();
     for (int k = 0; k < K && k < max_k; ++k) {
         if (w) w[k] = k < (int)S->s.weights.size() ? S->s.weights[k] : 0.0;
-        if (counts) counts[k] = S->s.cl[k].c.st.N;
+        if (counts) counts[k] = S->s.cl[k].c.N;
     }
     return DPMM_OK;
 }

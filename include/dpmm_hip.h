@@ -93,6 +93,40 @@ int dpmm_sufficient_statistics(dpmm_ctx* ctx, int K, int64_t* Nsub, double* sum,
  * constant 0 (3) / 1 (4).  Tables have K_old entries (new_label 2 K_old). */
 int dpmm_relabel(dpmm_ctx* ctx, uint32_t iter, int K_old, const int32_t* new_label, const int32_t* sub_action);
 
+/* --------------------------------------------------- batched conjugate FP64 work on the device ----- */
+/* Prior of the posteriors below: niw_hyperparams(k0, m0 (D), v0, psi0 (D x D row-major, the
+ * reference's psi = Psi / v0)) (priors/niw_hyperparams.h:13, 40-43) for a Gaussian context;
+ * multinomial_hyper(alpha0 (D)) (priors/multinomial_hyper.h:13, 29) for a multinomial one.
+ * DPMM_ENOTSPD if psi0 is not SPD. */
+int dpmm_set_prior(dpmm_ctx* ctx, double k0, const double* m0, double v0, const double* psi0, const double* alpha0);
+/* Test hook beside dpmm_set_labels (the reference's test/myCudaKernel.h): statistics of 2K
+ * sub-clusters in the layout of dpmm_sufficient_statistics, for dpmm_log_marginal_batch and
+ * dpmm_sample_params; the next dpmm_sufficient_statistics call recomputes them from the labels. */
+int dpmm_set_statistics(dpmm_ctx* ctx, int K, const int64_t* Nsub, const double* sum, const double* S);
+/* Log marginal likelihoods of n statistic sums in one launch (one workgroup per sum).  Sum j adds the
+ * sub-cluster entries members[4j .. 4j+3] (-1: unused; entry q = 2k + s) of the statistics of the
+ * last dpmm_sufficient_statistics call, forms the posterior (niw::calc_posterior, niw.cpp:23-49;
+ * multinomial_prior::calc_posterior, multinomial_prior.cpp:12-22), factors it in FP64 and
+ * evaluates niw::log_marginal_likelihood (niw.cpp:122-138) or the Dirichlet-multinomial one
+ * (multinomial_prior.cpp:56-67).  One call serves a whole iteration: the clusters, sub-clusters and
+ * merge pairs of update_suff_stats_posterior / should_split_local / should_merge
+ * (local_clusters_actions.cpp:103-168, 215-242; shared_actions.cpp:88-111).  logdet_psi (may be
+ * NULL) gets log det psi_n; a non-SPD posterior gives NaN. */
+int dpmm_log_marginal_batch(dpmm_ctx* ctx, int n, const int32_t* members, double* logml, double* logdet_psi);
+/* sample_cluster_params (shared_actions.cpp:9-55; local_clusters_actions.cpp:341-371) for K clusters
+ * in one launch: part p = 3k + r (r: 0 cluster, 1 left, 2 right sub-cluster) is drawn from the
+ * posterior of the statistic sum members[4p .. 4p+3] (niw::sample_distribution niw.cpp:82-107, Bartlett
+ * form; multinomial_prior::sample_distribution multinomial_prior.cpp:24-49) on the Philox stream
+ * (seed, p, iter, 6), and the sweeps' parameter slots are written on the device with log weights[k]
+ * (clusters) and log lr_weights[2k + s] (sub-clusters).  full_mu_cov = 0 keeps the reference's
+ * diagonal-only mu draw (pdflib.cpp:4007-4014); use_weight = 0 its multinomial label score without
+ * log pi.  Replaces dpmm_set_*_params for the sampler.  DPMM_ENOTSPD if a draw fails. */
+int dpmm_sample_params(dpmm_ctx* ctx, uint32_t iter, int K, const int32_t* members, const double* weights,
+                       const double* lr_weights, int full_mu_cov, int use_weight);
+/* The FP64 draws of the last dpmm_sample_params, 3K parts: mu (or log theta) 3K x D, the lower
+ * factor R of invSigma 3K x D x D (may be NULL), logdet Sigma 3K (may be NULL). */
+int dpmm_get_drawn_params(dpmm_ctx* ctx, double* mu, double* chol_inv_sigma, double* logdet_sigma);
+
 /* Mixed precision (DPMM_PREC_MIXED): a point's BF16-screened label is kept when its best score,
  * lowered by tau * q (q its screened Mahalanobis form), still beats every other cluster's score
  * raised by tau * q_k; otherwise the point is re-scored exactly in FP32.  Default tau = 1/32.

@@ -694,7 +694,7 @@ void run_sub_gauss(dpmm_ctx* c, uint32_t iter) {
 template <typename TX, int DP>
 void run_stats_gauss(dpmm_ctx* c) {
     const int K = c->Kpartition;
-    launch_stats_gauss<TX, DP>(c->stream, 2 * chunks_upper(c, K), reinterpret_cast<const TX*>(c->dX), c->D, K, c->CH,
+    launch_stats_gauss<TX, DP>(c->stream, chunks_upper(c, K), reinterpret_cast<const TX*>(c->dX), c->D, K, c->CH,
                                c->dPerm, c->dLabelStart, c->dChunkStart, c->dSubs, c->dPartials);
 }
 

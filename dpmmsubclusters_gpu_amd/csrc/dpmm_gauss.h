@@ -143,12 +143,13 @@ __global__ void __launch_bounds__(kThreads) k_sub_gauss(const T* __restrict__ X,
     }
 }
 
-// FP64 MFMA form of the same partial (v_mfma_f64_16x16x4f64): the scatter matrix of DP x DP is
-// cut into 16 x 16 tiles, the upper-triangle tiles are dealt round robin to the four waves, and
-// each MFMA adds the outer products of four points (A = their 16 coordinates of the tile's row
-// slice, B = those of its column slice; lane L holds point L / 16, coordinate L % 16).  Points of
-// the chunk with sub-label s are staged in LDS (compacted by ballot), zero-padded to a multiple
-// of four.  Output row of lane L, accumulator v: 4 v + L / 16, column L % 16 (checked on gfx950).
+// Statistics of both sub-clusters of one chunk on FP64 MFMA (v_mfma_f64_16x16x4f64).  Batches of
+// B points are staged in LDS as FP64; each wave owns (sub-cluster, 16 x 16 tile) pairs of the upper
+// block triangle of the scatter and accumulates 4 points per MFMA: lane L supplies coordinate L % 16
+// of point L / 16 of the sub-cluster's compacted list as both operands, and output element v of
+// lane L is row 4 v + L / 16, column L % 16 of the tile.  The next batch's point indices, sub-labels
+// and coordinates are loaded into registers while the current batch is multiplied, so the gather's
+// latency overlaps the MFMA work.  Writes the partials of (chunk, 0) and (chunk, 1).
 template <typename TX, int DP>
 __global__ void __launch_bounds__(kThreads) k_stats_gauss_mfma(const TX* __restrict__ X, int D, int K, int CH,
                                                                const int32_t* __restrict__ perm,
@@ -159,85 +160,130 @@ __global__ void __launch_bounds__(kThreads) k_stats_gauss_mfma(const TX* __restr
     typedef double f64x4 __attribute__((ext_vector_type(4)));
     constexpr int T = DP / 16;
     constexpr int NTILE = T * (T + 1) / 2;
-    constexpr int TPW = (NTILE + 3) / 4;            // tiles per wave
-    constexpr int BATCH = DP > 64 ? 32 : 64;
+    constexpr int TPW = (2 * NTILE + 3) / 4;          // (sub-cluster, tile) pairs per wave
+    constexpr int B = DP > 64 ? 32 : 64;               // points per batch
+    constexpr int PER = B * DP / kThreads;             // staged coordinates per thread
+    constexpr int SXT = kThreads / (2 * DP) > 0 ? kThreads / (2 * DP) : 1;
     constexpr size_t E = 1 + DP + (size_t)DP * DP;
-    __shared__ __align__(16) double xs[BATCH * DP];
-    __shared__ int idl[64];
-    __shared__ int nsel;
-    const int c = blockIdx.x >> 1, s = blockIdx.x & 1;
+    static_assert(B * DP % kThreads == 0, "batch must tile the workgroup");
+    __shared__ __align__(16) double xs[B * DP];
+    __shared__ int idn[B];          // point indices of the batch being loaded
+    __shared__ int sbb[B];          // sub-labels of the staged batch
+    __shared__ int lst[2][B + 4];
+    __shared__ int nls[2];
+    __shared__ double sxr[kThreads];
     int k, beg, end;
-    if (!chunk_of(c, K, chunk_start, label_start, CH, k, beg, end)) return;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (!chunk_of(blockIdx.x, K, chunk_start, label_start, CH, k, beg, end)) return;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int kp = lane >> 4, r = lane & 15;
-    int ti[TPW], tj[TPW];
+    int us[TPW], ca[TPW], cb[TPW];
 #pragma unroll
     for (int u = 0; u < TPW; ++u) {
         int t = wv + 4 * u;
-        ti[u] = -1; tj[u] = -1;
-        if (t < NTILE) {
+        us[u] = -1; ca[u] = 0; cb[u] = 0;
+        if (t < 2 * NTILE) {
+            us[u] = t / NTILE;
+            t -= us[u] * NTILE;
             int row = 0;
             while (t >= T - row) { t -= T - row; ++row; }
-            ti[u] = row; tj[u] = row + t;
+            ca[u] = 16 * row + r; cb[u] = 16 * (row + t) + r;
         }
     }
     f64x4 acc[TPW];
 #pragma unroll
     for (int u = 0; u < TPW; ++u) acc[u] = f64x4{0.0, 0.0, 0.0, 0.0};
+    const int sx_pair = tid % (2 * DP), sx_part = tid / (2 * DP);
+    const int sx_s = sx_pair / DP, sx_l = sx_pair - sx_s * DP;
     double sx = 0.0;
-    int64_t cnt = 0;
+    int64_t cnt0 = 0, cnt1 = 0;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    for (int b0 = beg; b0 < end; b0 += BATCH) {
-        if (wv == 0) {
-            const int p = b0 + lane;
-            int idx = -1;
-            bool f = false;
-            if (lane < BATCH && p < end) { idx = perm[p]; f = subs[idx] == s; }
-            const uint64_t m = __ballot(f);
-            if (f) idl[__popcll(m & lt)] = idx;
-            if (lane == 0) nsel = __popcll(m);
+
+    // prefetch pipeline: indices two batches ahead, sub-labels and coordinates one batch ahead
+    int id1 = (tid < B && beg + tid < end) ? perm[beg + tid] : -1;
+    int id2 = (tid < B && beg + B + tid < end) ? perm[beg + B + tid] : -1;
+    int sb1 = id1 >= 0 ? subs[id1] : -1;
+    if (tid < B) idn[tid] = id1;
+    __syncthreads();
+    TX xv[PER];
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+        const int e = tid + kThreads * j, q = e / DP, l = e - q * DP;
+        const int id = idn[q];
+        xv[j] = (id >= 0 && l < D) ? X[(int64_t)id * D + l] : TX(0);
+    }
+    for (int b0 = beg; b0 < end; b0 += B) {
+        __syncthreads();   // the previous batch's readers of xs / idn are done
+#pragma unroll
+        for (int j = 0; j < PER; ++j) xs[tid + kThreads * j] = (double)xv[j];
+        if (tid < B) {
+            sbb[tid] = sb1;
+            idn[tid] = id2;   // indices of the next batch
         }
         __syncthreads();
-        const int n = nsel;
-        const int n4 = (n + 3) & ~3;
-        for (int e = threadIdx.x; e < n4 * DP; e += kThreads) {
-            const int pp = e / DP, l = e - pp * DP;
-            xs[pp * DP + l] = (pp < n && l < D) ? (double)X[(int64_t)idl[pp] * D + l] : 0.0;
+        if (wv == 0) {   // compact by sub-label, index order kept
+            const int sb = lane < B ? sbb[lane] : -1;
+            const uint64_t m0 = __ballot(sb == 0), m1 = __ballot(sb == 1);
+            if (sb == 0) lst[0][__popcll(m0 & lt)] = lane;
+            if (sb == 1) lst[1][__popcll(m1 & lt)] = lane;
+            const int c0 = __popcll(m0), c1 = __popcll(m1);
+            if (lane < 4) { lst[0][c0 + lane] = 0; lst[1][c1 + lane] = 0; }   // in-range pads
+            if (lane == 0) { nls[0] = c0; nls[1] = c1; }
         }
+        // next batch: coordinates and sub-labels from the indices now in idn, indices after that
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const int e = tid + kThreads * j, q = e / DP, l = e - q * DP;
+            const int id = idn[q];
+            xv[j] = (id >= 0 && l < D) ? X[(int64_t)id * D + l] : TX(0);
+        }
+        sb1 = id2 >= 0 ? subs[id2] : -1;
+        id2 = (tid < B && b0 + 2 * B + tid < end) ? perm[b0 + 2 * B + tid] : -1;
         __syncthreads();
-        for (int g = 0; g < n4; g += 4) {
-            double xr[T];
+        const int n0 = nls[0], n1 = nls[1];
+        cnt0 += n0;
+        cnt1 += n1;
 #pragma unroll
-            for (int t = 0; t < T; ++t) xr[t] = xs[(g + kp) * DP + 16 * t + r];
+        for (int sub = 0; sub < 2; ++sub) {
+            const int ns = sub ? n1 : n0;
+            const int* L = lst[sub];
+            for (int g = 0; g < ns; g += 4) {
+                const int q = g + kp;
+                const double* row = xs + L[q] * DP;   // pads point at row 0; masked below
+                const bool in = q < ns;
 #pragma unroll
-            for (int u = 0; u < TPW; ++u) {
-                if (ti[u] < 0) continue;
-                double a = xr[0], b = xr[0];
-#pragma unroll
-                for (int t = 1; t < T; ++t) {   // register selects keep xr[] out of scratch
-                    a = ti[u] == t ? xr[t] : a;
-                    b = tj[u] == t ? xr[t] : b;
+                for (int u = 0; u < TPW; ++u) {
+                    if (us[u] != sub) continue;
+                    const double a = in ? row[ca[u]] : 0.0;
+                    const double bb = in ? row[cb[u]] : 0.0;
+                    acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bb, acc[u], 0, 0, 0);
                 }
-                acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[u], 0, 0, 0);
             }
         }
-        if (threadIdx.x < DP)
-            for (int pp = 0; pp < n; ++pp) sx += xs[pp * DP + threadIdx.x];
-        cnt += n;
-        __syncthreads();
+        if (sx_part < SXT) {
+            const int ns = sx_s ? n1 : n0;
+            const int* L = lst[sx_s];
+            for (int q = sx_part; q < ns; q += SXT) sx += xs[L[q] * DP + sx_l];
+        }
     }
-    double* out = partials + (size_t)blockIdx.x * E;
-    if (threadIdx.x == 0) out[0] = (double)cnt;
-    if (threadIdx.x < DP) out[1 + threadIdx.x] = sx;
-    double* S = out + 1 + DP;
+    sxr[tid] = sx;
+    __syncthreads();
+    double* out0 = partials + (size_t)(2 * blockIdx.x) * E;
+    if (tid == 0) { out0[0] = (double)cnt0; out0[E] = (double)cnt1; }
+    if (tid < 2 * DP) {
+        double v = 0.0;
+        for (int j = 0; j < SXT; ++j) v += sxr[tid + j * 2 * DP];
+        out0[sx_s * E + 1 + sx_l] = v;
+    }
 #pragma unroll
     for (int u = 0; u < TPW; ++u) {
-        if (ti[u] < 0) continue;
+        if (us[u] < 0) continue;
+        double* S = out0 + us[u] * E + 1 + DP;
+        const int ti = (ca[u] - r) / 16, tj = (cb[u] - r) / 16;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            const int a = 16 * ti[u] + 4 * v + kp, b = 16 * tj[u] + r;
-            S[(size_t)a * DP + b] = acc[u][v];
-            S[(size_t)b * DP + a] = acc[u][v];
+            const int ra = 16 * ti + 4 * v + kp, cbb = 16 * tj + r;
+            S[(size_t)ra * DP + cbb] = acc[u][v];
+            S[(size_t)cbb * DP + ra] = acc[u][v];
         }
     }
 }
@@ -362,16 +408,18 @@ void launch_sub_gauss(hipStream_t st, int grid, const T* X, int D, int K, int CH
                        chunk_start, SP, key0, key1, iter, subs);
 }
 
+// nchunks: upper bound of the chunk count (one MFMA block per chunk, one vector block per
+// (chunk, sub-cluster)).
 template <typename TX, int DP>
-void launch_stats_gauss(hipStream_t st, int grid, const TX* X, int D, int K, int CH, const int32_t* perm,
+void launch_stats_gauss(hipStream_t st, int nchunks, const TX* X, int D, int K, int CH, const int32_t* perm,
                         const int32_t* label_start, const int32_t* chunk_start, const int32_t* subs,
                         double* partials) {
     if constexpr (DP % 16 == 0)
-        hipLaunchKernelGGL((k_stats_gauss_mfma<TX, DP>), dim3(grid), dim3(kThreads), 0, st, X, D, K, CH, perm,
+        hipLaunchKernelGGL((k_stats_gauss_mfma<TX, DP>), dim3(nchunks), dim3(kThreads), 0, st, X, D, K, CH, perm,
                            label_start, chunk_start, subs, partials);
     else
-        hipLaunchKernelGGL((k_stats_gauss<TX, DP>), dim3(grid), dim3(kThreads), 0, st, X, D, K, CH, perm, label_start,
-                           chunk_start, subs, partials);
+        hipLaunchKernelGGL((k_stats_gauss<TX, DP>), dim3(2 * nchunks), dim3(kThreads), 0, st, X, D, K, CH, perm,
+                           label_start, chunk_start, subs, partials);
 }
 
 #define DPMM_GAUSS_INSTANTIATE(T, DP)                                                                         \

@@ -52,7 +52,7 @@ class _Options(ctypes.Structure):
                 ("init_clusters", ctypes.c_int), ("burnout", ctypes.c_int), ("argmax_sample_stop", ctypes.c_int),
                 ("split_stop", ctypes.c_int), ("max_clusters", ctypes.c_double), ("seed", ctypes.c_uint64),
                 ("full_mu_cov", ctypes.c_int), ("mult_use_weight", ctypes.c_int), ("precision", ctypes.c_int),
-                ("verbose", ctypes.c_int)]
+                ("verbose", ctypes.c_int), ("init_from_labels", ctypes.c_int)]
 
 
 _lib = None
@@ -94,6 +94,18 @@ def load_library():
         "dpmm_log_marginal_batch": [_P, i, _P, _P, _P],
         "dpmm_sample_params": [_P, u32, i, _P, _P, _P, i, i],
         "dpmm_get_drawn_params": [_P, _P, _P, _P],
+        "dpmm_pair_features": [_P, i, _P, i, _P, _P, _P],
+        "dpmm_screener_create": [u64, _P],
+        "dpmm_screener_destroy": [_P],
+        "dpmm_screener_param_count": [],
+        "dpmm_screener_set_params": [_P, _P, _P, _P],
+        "dpmm_screener_get_params": [_P, _P, _P, _P],
+        "dpmm_screener_loss_grad": [_P, i, _P, _P, ctypes.c_float, _P, _P],
+        "dpmm_screener_train_step": [_P, i, _P, _P, ctypes.c_float, ctypes.c_float, _P],
+        "dpmm_screener_forward": [_P, i, _P, _P],
+        "dpmm_sampler_set_screener": [_P, _P, i, ctypes.c_float, i],
+        "dpmm_sampler_screen_stats": [_P, _P, _P, _P, _P],
+        "dpmm_sampler_pair_log": [_P, i, _P, _P, _P, _P],
         "dpmm_sampler_create": [_P, _P, dbl, _P, dbl, _P, _P, _P],
         "dpmm_sampler_step": [_P, i],
         "dpmm_sampler_num_clusters": [_P],
@@ -269,6 +281,15 @@ class DeviceSweep:
         s, SS = _f64(sums), _f64(S)
         _check(load_library().dpmm_set_statistics(self._ctx, n.size // 2, _ptr(n), _ptr(s), _ptr(SS)))
 
+    def pair_features(self, K, cluster_scalars, pi, pj):
+        """16 screener features per pair (pi[p], pj[p]) from the current statistics and K x 7
+        per-cluster scalars (N, N_left, N_right, L, L_left, L_right, log det psi_n)."""
+        sc = _f64(np.asarray(cluster_scalars).reshape(K, 7))
+        a, b = _i32(pi), _i32(pj)
+        out = np.empty((a.size, 16), np.float32)
+        _check(load_library().dpmm_pair_features(self._ctx, int(K), _ptr(sc), a.size, _ptr(a), _ptr(b), _ptr(out)))
+        return out
+
     def set_prior(self, hyper):
         """niw_hyperparams or multinomial_hyper of the batched posterior work."""
         if isinstance(hyper, niw_hyperparams):
@@ -329,16 +350,22 @@ class Sampler:
     """Stateful driver: init_model + init_first_clusters, then one group_step per step()."""
 
     def __init__(self, dev, hyper, alpha, iters, init_clusters=1, burnout=15, seed=0, max_clusters=float("inf"),
-                 precision=None, full_mu_cov=False, mult_use_weight=False, verbose=False):
+                 precision=None, full_mu_cov=False, mult_use_weight=False, verbose=False, init_labels=None):
+        """init_labels (N, 0-based) starts from a given clustering, e.g. an over-split one."""
         L = load_library()
         self.dev = dev
         if precision is None:
             precision = Precision.FP64 if dev._dtype == 0 else Precision.FP32
+        if init_labels is not None:
+            lab = _i32(init_labels)
+            init_clusters = int(lab.max()) + 1
+            dev.set_labels(lab, None)
         self.opts = _Options(prior=int(dev.prior), alpha=float(alpha), iters=int(iters),
                              init_clusters=int(init_clusters), burnout=int(burnout), argmax_sample_stop=5,
                              split_stop=5, max_clusters=float(max_clusters), seed=int(seed),
                              full_mu_cov=int(full_mu_cov), mult_use_weight=int(mult_use_weight),
-                             precision=int(precision), verbose=int(verbose))
+                             precision=int(precision), verbose=int(verbose),
+                             init_from_labels=int(init_labels is not None))
         self._s = ctypes.c_void_p()
         if dev.prior == prior_type.Gaussian:
             self._m0, self._psi0 = _f64(hyper.m), _f64(hyper.psi)
@@ -366,6 +393,27 @@ class Sampler:
         v = ctypes.c_double()
         _check(load_library().dpmm_sampler_host_ms(self._s, ctypes.byref(v)))
         return v.value
+
+    def set_screener(self, screener=None, full_scan_every=10, threshold=0.0, log_pairs=False):
+        """Merge screening with a trained screener (None: off); log_pairs keeps the last step's pairs."""
+        self._screener = screener   # keep it alive while the sampler uses it
+        _check(load_library().dpmm_sampler_set_screener(self._s, screener.handle if screener else None,
+                                                        int(full_scan_every), float(threshold), int(bool(log_pairs))))
+
+    def screen_stats(self):
+        """(exact pair tests, pairs skipped, accepted merges missed on full scans, full scans)."""
+        v = [ctypes.c_int64() for _ in range(4)]
+        _check(load_library().dpmm_sampler_screen_stats(self._s, *[ctypes.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
+    def pair_log(self):
+        """Features, log Hastings ratio and log U of every candidate pair of the last step."""
+        n = ctypes.c_int()
+        _check(load_library().dpmm_sampler_pair_log(self._s, 0, None, None, None, ctypes.byref(n)))
+        f = np.empty((n.value, 16), np.float32)
+        h, u = np.empty(n.value), np.empty(n.value)
+        _check(load_library().dpmm_sampler_pair_log(self._s, n.value, _ptr(f), _ptr(h), _ptr(u), ctypes.byref(n)))
+        return f, h, u
 
     def close(self):
         if self._s:

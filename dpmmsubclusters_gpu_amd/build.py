@@ -24,7 +24,7 @@ DEVICE_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-fopenmp", "-Wall", "-Wno-unused-function"]
 
 DEVICE_SOURCES = ["dpmm_device.hip", "inst_gauss_small.hip", "inst_gauss_64.hip", "inst_gauss_96.hip",
-                  "inst_gauss_128.hip", "inst_mixed.hip"]
+                  "inst_gauss_128.hip", "inst_mixed.hip", "dpmm_screener.hip"]
 HOST_SOURCES = ["sampler.cpp"]
 
 

@@ -6,4 +6,6 @@
 
 namespace dpmm {
 int ctx_dims(const dpmm_ctx* c, int64_t* N, int* D, int* prior, int* precision);
+// Records msg as dpmm_last_error() and returns code.
+int set_error(int code, const char* msg);
 }

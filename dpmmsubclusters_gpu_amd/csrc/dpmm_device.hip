@@ -411,6 +411,8 @@ struct dpmm_ctx {
     size_t drawStatusCap = 0;
     double* dLogW = nullptr;
     size_t logwCap = 0;
+    float* dFeat = nullptr;
+    size_t featCap = 0;
     int Kdraw = 0;
     int drawStride() const {
         return prior == DPMM_PRIOR_GAUSSIAN ? ((D + D * (D + 1) / 2 + 1) + 1) & ~1 : (D + 1) & ~1;
@@ -806,7 +808,7 @@ int dpmm_destroy(dpmm_ctx* c) {
                     (void*)c->dRowtot, (void*)c->dLabelStart, (void*)c->dChunkStart, (void*)c->dTable,
                     (void*)c->dPartials, (void*)c->dStats, c->dPB, (void*)c->dRecheck, (void*)c->dRecheckCount,
                     (void*)c->dPrior, (void*)c->dMembers, (void*)c->dLM, (void*)c->dDraws, (void*)c->dDrawStatus,
-                    (void*)c->dLogW})
+                    (void*)c->dLogW, (void*)c->dFeat})
         if (p) (void)hipFree(p);
     if (c->hStats) (void)hipHostFree(c->hStats);
     for (auto& e : c->ev)
@@ -1248,6 +1250,40 @@ int dpmm_sample_params(dpmm_ctx* c, uint32_t iter, int K, const int32_t* members
     return DPMM_OK;
 }
 
+int dpmm_pair_features(dpmm_ctx* c, int K, const double* scal, int npairs, const int32_t* pi, const int32_t* pj,
+                       float* features) {
+    if (!c || K <= 0 || !scal || npairs < 0 || (npairs > 0 && (!pi || !pj || !features)))
+        return fail(DPMM_EINVAL, "bad args");
+    if (c->prior != DPMM_PRIOR_GAUSSIAN) return fail(DPMM_EINVAL, "pair features need a Gaussian context");
+    if (!c->has_prior) return fail(DPMM_ESTATE, "prior not set");
+    if (K > c->Kstats) return fail(DPMM_ESTATE, "statistics hold fewer clusters");
+    if (npairs == 0) return DPMM_OK;
+    for (int p = 0; p < npairs; ++p)
+        if (pi[p] < 0 || pi[p] >= K || pj[p] < 0 || pj[p] >= K) return fail(DPMM_EINVAL, "pair outside 0..K-1");
+    const int D = c->D;
+    // device scratch: summaries (K x 2D), scalars (K x 7), pairs (2 x npairs), features (npairs x 16)
+    const size_t nd = (size_t)K * 2 * D + (size_t)K * 7;
+    int rc = grow(c->dLM, c->lmCap, std::max<size_t>(nd, (size_t)2 * K));
+    if (rc) return rc;
+    rc = grow(c->dMembers, c->membersCap, (size_t)2 * npairs);
+    if (rc) return rc;
+    rc = grow(c->dFeat, c->featCap, (size_t)16 * npairs);
+    if (rc) return rc;
+    double* dSumm = c->dLM;
+    double* dScal = c->dLM + (size_t)K * 2 * D;
+    HIPCHK(hipMemcpyAsync(dScal, scal, (size_t)K * 7 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->dMembers, pi, (size_t)npairs * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->dMembers + npairs, pj, (size_t)npairs * 4, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_cluster_summary, dim3(K), dim3(kThreads), 0, c->stream, c->dStats, (int)c->statEntries(),
+                       c->DP, D, c->prior_args, dSumm);
+    hipLaunchKernelGGL(k_pair_features, dim3((npairs + 3) / 4), dim3(kThreads), 0, c->stream, dSumm, dScal, D,
+                       npairs, c->dMembers, c->dMembers + npairs, c->dFeat);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(features, c->dFeat, (size_t)16 * npairs * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return DPMM_OK;
+}
+
 int dpmm_get_drawn_params(dpmm_ctx* c, double* mu, double* chol_inv_sigma, double* logdet_sigma) {
     if (!c || !mu) return fail(DPMM_EINVAL, "bad args");
     if (c->Kdraw <= 0) return fail(DPMM_ESTATE, "no draws yet");
@@ -1275,6 +1311,8 @@ int dpmm_get_drawn_params(dpmm_ctx* c, double* mu, double* chol_inv_sigma, doubl
 
 // Internal accessors for the host driver (same library).
 namespace dpmm {
+int set_error(int code, const char* msg) { return fail(code, msg); }
+
 int ctx_dims(const dpmm_ctx* c, int64_t* N, int* D, int* prior, int* precision) {
     if (!c) return DPMM_EINVAL;
     *N = c->N; *D = c->D; *prior = c->prior; *precision = c->precision;

@@ -449,4 +449,79 @@ __global__ void __launch_bounds__(kThreads) k_pack_mult(const double* __restrict
     }
 }
 
+// ---- screener pair features ----------------------------------------------------------------------
+// Cluster k (entries 2k, 2k + 1): posterior mean m (D) and diagonal of psi_n (D), into out[k][0..2D).
+__global__ void __launch_bounds__(kThreads) k_cluster_summary(const double* __restrict__ stats, int E, int DP, int D,
+                                                              PriorArgs h, double* __restrict__ out) {
+    const int k = blockIdx.x;
+    const double* s0 = stats + (size_t)(2 * k) * E;
+    const double* s1 = s0 + E;
+    const double N = s0[0] + s1[0];
+    const double kn = h.k0 + N, vn = h.v0 + N;
+    for (int d = threadIdx.x; d < D; d += blockDim.x) {
+        double m, dp;
+        if (N == 0.0) {
+            m = h.m0[d];
+            dp = h.psi0[(size_t)d * D + d];
+        } else {
+            const double sum = s0[1 + d] + s1[1 + d];
+            const double S = s0[1 + DP + (size_t)d * DP + d] + s1[1 + DP + (size_t)d * DP + d];
+            m = (h.k0 * h.m0[d] + sum) / kn;
+            dp = (h.v0 * h.psi0[(size_t)d * D + d] + h.k0 * h.m0[d] * h.m0[d] + S - kn * m * m) * (1.0 / vn);
+        }
+        out[(size_t)k * 2 * D + d] = m;
+        out[(size_t)k * 2 * D + D + d] = dp;
+    }
+}
+
+// One wave per pair (i, j): the 16 features of the merge screener from the summaries and the
+// per-cluster scalars (N, Nl, Nr, L, Ll, Lr, log det psi_n).
+__global__ void __launch_bounds__(kThreads) k_pair_features(const double* __restrict__ summ,
+                                                            const double* __restrict__ scal, int D, int npairs,
+                                                            const int32_t* __restrict__ pi,
+                                                            const int32_t* __restrict__ pj, float* __restrict__ out) {
+    const int p = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (p >= npairs) return;
+    const int i = pi[p], j = pj[p];
+    const double* mi = summ + (size_t)i * 2 * D;
+    const double* mj = summ + (size_t)j * 2 * D;
+    double di = 0.0, dj = 0.0, dq = 0.0;
+    for (int a = lane; a < D; a += 64) {
+        const double d = mi[a] - mj[a];
+        const double si = mi[D + a], sj = mj[D + a];
+        di += d * d / si;
+        dj += d * d / sj;
+        dq += d * d / (si + sj);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        di += __shfl_xor(di, o);
+        dj += __shfl_xor(dj, o);
+        dq += __shfl_xor(dq, o);
+    }
+    if (lane < 16) {
+        const double* a = scal + (size_t)i * 7;
+        const double* b = scal + (size_t)j * 7;
+        double f;
+        switch (lane) {
+            case 0: f = log(a[0]); break;
+            case 1: f = log(b[0]); break;
+            case 2: f = log(a[0] + b[0]); break;
+            case 3: f = a[3] / a[0]; break;
+            case 4: f = b[3] / b[0]; break;
+            case 5: f = a[6] / D; break;
+            case 6: f = b[6] / D; break;
+            case 7: f = log1p(di / D); break;
+            case 8: f = log1p(dj / D); break;
+            case 9: f = log1p(dq / D); break;
+            case 10: f = (a[4] + a[5] - a[3]) / a[0]; break;
+            case 11: f = (b[4] + b[5] - b[3]) / b[0]; break;
+            case 12: f = fabs(log(a[1] + 1.0) - log(a[2] + 1.0)); break;
+            case 13: f = fabs(log(b[1] + 1.0) - log(b[2] + 1.0)); break;
+            case 14: f = fabs(a[6] / D - b[6] / D); break;
+            default: f = fabs(log(a[0]) - log(b[0])); break;
+        }
+        out[(size_t)p * 16 + lane] = (float)f;
+    }
+}
+
 }  // namespace dpmm

@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <memory>
 #include <string>
 #include <vector>
@@ -78,6 +79,7 @@ struct Part {
     int64_t N = 0;
     int32_t mem[4] = {-1, -1, -1, -1};
     double lml = 0.0;
+    double ld = 0.0;   // log det psi_n of the posterior
 };
 struct Cluster {
     Part c, l, r;
@@ -106,18 +108,30 @@ struct SamplerImpl {
     std::vector<double> weights;
     double host_ms = 0;
     std::vector<int64_t> Nsub;
-    // merge candidates of the current iteration: clusters and log ML of every pair's union
-    std::vector<int> cand_pos;
+    // merge candidates of the current iteration: clusters and log ML of every pair's union (NaN
+    // where the screener skipped the pair)
+    std::vector<int> cand, cand_pos;
     std::vector<double> cand_L;
     int ncand = 0;
+    // merge screener
+    dpmm_screener* scr = nullptr;
+    int scr_every = 10;
+    float scr_thr = 0.f;
+    bool log_pairs = false;
+    bool full_scan = true;
+    std::vector<float> pair_feat, pair_logit;
+    std::vector<float> log_feat;
+    std::vector<double> log_h, log_u;
+    int64_t n_exact = 0, n_skipped = 0, n_missed = 0, n_full = 0;
 
     void reset_hist(Cluster& c) const { c.hist.assign(o.burnout + 5, std::make_pair(false, 0.0)); }
 
     // create_sufficient_statistics + update_suff_stats_posterior (local_clusters_actions.cpp:
     // 109-168) for the clusters flagged in `which` (empty: all), with the log marginal likelihoods
     // of their parts from one device launch.  With `pairs`, the same launch also evaluates the
-    // union of every pair of merge candidates (splittable, non-empty clusters).
-    int update_stats(const std::vector<char>& which, bool pairs) {
+    // union of every pair of merge candidates (splittable, non-empty clusters) that the screener,
+    // when set, passes.
+    int update_stats(const std::vector<char>& which, bool pairs, uint32_t t = 0) {
         const int K = (int)cl.size();
         Nsub.resize(2 * K);
         int rc = dpmm_sufficient_statistics(ctx, K, Nsub.data(), nullptr, nullptr);
@@ -140,40 +154,95 @@ struct SamplerImpl {
             job(&c.r);
         }
         const int nparts = (int)dst.size();
+        std::vector<size_t> tested;   // pair indices whose exact test joins this launch
         if (pairs) {
             cand_pos.assign(K, -1);
-            std::vector<int> cand;
+            cand.clear();
             for (int k = 0; k < K; ++k)
                 if (cl[k].splittable && cl[k].c.N > 0) {
                     cand_pos[k] = (int)cand.size();
                     cand.push_back(k);
                 }
             ncand = (int)cand.size();
+            const size_t np = (size_t)ncand * (ncand - 1) / 2;
+            cand_L.assign(np, std::numeric_limits<double>::quiet_NaN());
+            full_scan = !scr || (scr_every > 0 && t % (uint32_t)scr_every == 0);
+            pair_feat.clear();
+            pair_logit.clear();
+            if ((scr || log_pairs) && np > 0) {
+                // the screener's features need this launch's cluster log MLs: evaluate the parts first
+                rc = run_parts(mem, dst);
+                if (rc) return rc;
+                mem.clear();
+                dst.clear();
+                if ((rc = screen_pairs())) return rc;
+            }
+            size_t p = 0;
             for (int a = 0; a < ncand; ++a)
-                for (int b = a + 1; b < ncand; ++b) {
+                for (int b = a + 1; b < ncand; ++b, ++p) {
+                    if (!full_scan && !(pair_logit[p] > scr_thr)) continue;
                     const int i = cand[a], j = cand[b];
                     const int32_t m[4] = {2 * i, 2 * i + 1, 2 * j, 2 * j + 1};
                     mem.insert(mem.end(), m, m + 4);
+                    tested.push_back(p);
                 }
+            n_exact += (int64_t)tested.size();
+            n_skipped += (int64_t)(np - tested.size());
+            if (scr && full_scan) ++n_full;
         }
         const int n = (int)(mem.size() / 4);
-        std::vector<double> L(n);
-        rc = dpmm_log_marginal_batch(ctx, n, mem.data(), L.data(), nullptr);
+        std::vector<double> L(n), ld(n);
+        rc = dpmm_log_marginal_batch(ctx, n, mem.data(), L.data(), ld.data());
         if (rc) return rc;
-        for (int j = 0; j < nparts; ++j) {
+        const int nd = (int)dst.size();
+        for (int j = 0; j < nd; ++j) {
             if (!std::isfinite(L[j])) return DPMM_ENOTSPD;
             dst[j]->lml = L[j];
+            dst[j]->ld = ld[j];
         }
-        if (pairs) cand_L.assign(L.begin() + nparts, L.end());
+        for (size_t q = 0; q < tested.size(); ++q) cand_L[tested[q]] = L[nd + q];
         host_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         return DPMM_OK;
     }
 
-    double pair_L(int i, int j) const {   // i < j, both candidates
-        const int a = cand_pos[i], b = cand_pos[j];
-        // row a of the upper triangle without diagonal
-        const size_t off = (size_t)a * ncand - (size_t)a * (a + 1) / 2 + (b - a - 1);
-        return cand_L[off];
+    int run_parts(const std::vector<int32_t>& mem, const std::vector<Part*>& dst) {
+        const int n = (int)(mem.size() / 4);
+        std::vector<double> L(n), ld(n);
+        int rc = dpmm_log_marginal_batch(ctx, n, mem.data(), L.data(), ld.data());
+        if (rc) return rc;
+        for (int j = 0; j < n; ++j) {
+            if (!std::isfinite(L[j])) return DPMM_ENOTSPD;
+            dst[j]->lml = L[j];
+            dst[j]->ld = ld[j];
+        }
+        return DPMM_OK;
+    }
+
+    // Features of every candidate pair and, with a screener, their logits.
+    int screen_pairs() {
+        const int K = (int)cl.size();
+        std::vector<double> scal((size_t)K * 7);
+        for (int k = 0; k < K; ++k) {
+            const Cluster& c = cl[k];
+            const double v[7] = {(double)c.c.N, (double)c.l.N, (double)c.r.N, c.c.lml, c.l.lml, c.r.lml, c.c.ld};
+            std::copy(v, v + 7, scal.begin() + (size_t)k * 7);
+        }
+        const size_t np = cand_L.size();
+        std::vector<int32_t> pi(np), pj(np);
+        size_t p = 0;
+        for (int a = 0; a < ncand; ++a)
+            for (int b = a + 1; b < ncand; ++b, ++p) {
+                pi[p] = cand[a];
+                pj[p] = cand[b];
+            }
+        pair_feat.resize(np * 16);
+        int rc = dpmm_pair_features(ctx, K, scal.data(), (int)np, pi.data(), pj.data(), pair_feat.data());
+        if (rc) return rc;
+        if (scr) {
+            pair_logit.resize(np);
+            rc = dpmm_screener_forward(scr, (int)np, pair_feat.data(), pair_logit.data());
+        }
+        return rc;
     }
 
     // sample_clusters (local_clusters_actions.cpp:341-371) + sample_cluster_params
@@ -221,9 +290,20 @@ struct SamplerImpl {
     }
 
     int init() {
-        int rc = dpmm_sample_initial_labels(ctx, std::max(1, o.init_clusters));
-        if (rc) return rc;
         const int K0 = std::max(1, o.init_clusters);
+        int rc;
+        if (o.init_from_labels) {
+            std::vector<int32_t> lab(N);
+            if ((rc = dpmm_get_labels(ctx, lab.data(), nullptr))) return rc;
+            for (int64_t i = 0; i < N; ++i)
+                if (lab[i] < 0 || lab[i] >= K0) return set_error(DPMM_EINVAL, "initial label outside 0..init_clusters-1");
+            // uniform sub-labels (sample_sub_labels, cudaKernel.cu:1100-1111): the reset relabel of every cluster
+            std::vector<int32_t> nl(2 * K0), act(K0, 1);
+            for (int q = 0; q < 2 * K0; ++q) nl[q] = q / 2;
+            if ((rc = relabel(0, nl, act))) return rc;
+        } else if ((rc = dpmm_sample_initial_labels(ctx, K0))) {
+            return rc;
+        }
         cl.assign(K0, Cluster());
         for (auto& c : cl) reset_hist(c);
         rc = update_stats({}, false);
@@ -247,7 +327,7 @@ struct SamplerImpl {
         if (rc) return rc;
         // one log-marginal launch: every part, and every pair of merge candidates (a cluster that is
         // reset or split below leaves the candidates, and the others keep these statistics)
-        rc = update_stats({}, !no_more_splits);
+        rc = update_stats({}, !no_more_splits, it);
         if (rc) return rc;
 
         // reset_bad_clusters (local_clusters_actions.cpp:429-447)
@@ -318,28 +398,38 @@ struct SamplerImpl {
             // outcome identical to the reference's sequential loop.
             const auto t1 = std::chrono::steady_clock::now();
             K = (int)cl.size();
-            std::vector<int> elig;
-            for (int k = 0; k < K; ++k)
-                if (cl[k].splittable && cl[k].c.N > 0 && k < (int)cand_pos.size() && cand_pos[k] >= 0)
-                    elig.push_back(k);
-            const int ne = (int)elig.size();
             const double al = o.alpha;
             const double cst = -std::log(al) + std::lgamma(al) - 2 * std::lgamma(0.5 * al);
             merge_into.assign(K, -1);
             std::vector<char> used(K, 0);
-            for (int a = 0; a < ne; ++a)
-                for (int b = a + 1; b < ne; ++b) {
-                    const int i = elig[a], j = elig[b];
-                    if (used[i] || used[j]) continue;
-                    const double L = pair_L(i, j);
+            const bool logging = log_pairs && !pair_feat.empty();
+            log_feat.clear();
+            log_h.clear();
+            log_u.clear();
+            size_t p = 0;
+            for (int a = 0; a < ncand; ++a)
+                for (int b = a + 1; b < ncand; ++b, ++p) {
+                    const int i = cand[a], j = cand[b];
+                    const double L = cand_L[p];
+                    if (!std::isfinite(L)) continue;   // skipped by the screener
+                    // a cluster split or reset in this iteration is no longer a candidate
+                    const bool elig = cl[i].splittable && cl[j].splittable && cl[i].c.N > 0 && cl[j].c.N > 0;
+                    if (!elig || ((used[i] || used[j]) && !logging)) continue;
                     const double Ni = (double)cl[i].c.N, Nj = (double)cl[j].c.N, Nm = Ni + Nj;
                     const double lhr = cst + std::lgamma(Nm) - std::lgamma(Nm + al) + std::lgamma(Ni + 0.5 * al) -
                                        std::lgamma(Ni) - std::lgamma(Nj) + std::lgamma(Nj + 0.5 * al) + L -
                                        cl[i].c.lml - cl[j].c.lml;
                     const double u = Stream(o.seed, ((uint32_t)i << 16) | (uint32_t)j, it, TAG_MERGE_U).uniform();
+                    if (logging) {
+                        log_feat.insert(log_feat.end(), pair_feat.begin() + p * 16, pair_feat.begin() + p * 16 + 16);
+                        log_h.push_back(lhr);
+                        log_u.push_back(std::log(u));
+                    }
+                    if (used[i] || used[j]) continue;
                     if (!((lhr > std::log(u)) || (fin && lhr > std::log(0.1)))) continue;
                     used[i] = used[j] = 1;
                     merge_into[j] = i;
+                    if (scr && full_scan && !(pair_logit[p] > scr_thr)) ++n_missed;
                     // merge_clusters (204-213) + merge_clusters_to_splittable (shared_actions.cpp:72-86)
                     Cluster& A = cl[i];
                     Cluster& Bc = cl[j];
@@ -443,6 +533,39 @@ int dpmm_sampler_weights(dpmm_sampler* S, double* w, int64_t* counts, int max_k)
 int dpmm_sampler_host_ms(dpmm_sampler* S, double* ms) {
     if (!S || !ms) return DPMM_EINVAL;
     *ms = S->s.host_ms;
+    return DPMM_OK;
+}
+
+int dpmm_sampler_set_screener(dpmm_sampler* S, dpmm_screener* scr, int every, float threshold, int log_pairs) {
+    if (!S || every < 0) return dpmm::set_error(DPMM_EINVAL, "bad args");
+    if ((scr || log_pairs) && S->s.prior != DPMM_PRIOR_GAUSSIAN)
+        return dpmm::set_error(DPMM_EINVAL, "merge screening needs the Gaussian prior");
+    S->s.scr = scr;
+    S->s.scr_every = every;
+    S->s.scr_thr = threshold;
+    S->s.log_pairs = log_pairs != 0;
+    return DPMM_OK;
+}
+
+int dpmm_sampler_screen_stats(dpmm_sampler* S, int64_t* exact, int64_t* skipped, int64_t* missed, int64_t* full) {
+    if (!S) return DPMM_EINVAL;
+    if (exact) *exact = S->s.n_exact;
+    if (skipped) *skipped = S->s.n_skipped;
+    if (missed) *missed = S->s.n_missed;
+    if (full) *full = S->s.n_full;
+    return DPMM_OK;
+}
+
+int dpmm_sampler_pair_log(dpmm_sampler* S, int max_rows, float* feat, double* log_h, double* log_u, int* n_rows) {
+    if (!S || !n_rows || max_rows < 0) return DPMM_EINVAL;
+    const int n = (int)std::min<size_t>(S->s.log_h.size(), (size_t)max_rows);
+    for (int r = 0; r < n; ++r) {
+        if (feat) std::copy(S->s.log_feat.begin() + (size_t)r * 16, S->s.log_feat.begin() + (size_t)r * 16 + 16,
+                            feat + (size_t)r * 16);
+        if (log_h) log_h[r] = S->s.log_h[r];
+        if (log_u) log_u[r] = S->s.log_u[r];
+    }
+    *n_rows = (int)S->s.log_h.size();
     return DPMM_OK;
 }
 

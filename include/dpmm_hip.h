@@ -139,6 +139,39 @@ int dpmm_last_recheck_count(dpmm_ctx* ctx, int64_t* n);
 int dpmm_last_kernel_ms(dpmm_ctx* ctx, float* label_sweep_ms, float* sub_sweep_ms, float* stats_ms);
 int dpmm_synchronize(dpmm_ctx* ctx);
 
+/* ------------------------------------------------------------------ merge screener ----- */
+/* No counterpart in the reference.  A learned screener in front of check_and_merge's all-pairs
+ * Hastings test (local_clusters_actions.cpp:287-321; should_merge, shared_actions.cpp:88-111), for
+ * callers with many clusters: an MLP 16 -> 128 -> 128 -> 1 scores each candidate pair from 16
+ * features of its two clusters, and only pairs scoring above a threshold get the exact test.  It
+ * never accepts a merge by itself; the sampler runs the full all-pairs test every `full_scan_every`
+ * iterations as fallback and counts the accepted merges the screener would have missed. */
+typedef struct dpmm_screener dpmm_screener;
+/* Features (16 floats per pair) of cluster pairs (pi[p], pj[p]) from the current statistics of K
+ * clusters and per-cluster scalars (K x 7: N, N_left, N_right, log ML of cluster / left / right,
+ * log det psi_n of the cluster): log N_i, log N_j, log(N_i + N_j), L_i / N_i, L_j / N_j,
+ * logdet_i / D, logdet_j / D, log1p of the mean distance whitened by diag psi_i, by diag psi_j and
+ * by their sum (each / D), the split gains (L_l + L_r - L) / N of i and j, the sub-cluster count
+ * imbalances |log(N_l + 1) - log(N_r + 1)| of i and j, |logdet_i - logdet_j| / D, |log N_i - log N_j|. */
+int dpmm_pair_features(dpmm_ctx* ctx, int K, const double* cluster_scalars, int npairs, const int32_t* pi,
+                       const int32_t* pj, float* features);
+int dpmm_screener_create(uint64_t seed, dpmm_screener** out);
+int dpmm_screener_destroy(dpmm_screener* s);
+/* Number of parameters: [W1 (128 x 16) | b1 (128) | W2 (128 x 128) | b2 (128) | w3 (128) | b3]. */
+int dpmm_screener_param_count(void);
+/* Parameters (FP32) and, when both are given, the per-feature standardisation (x - mean) / scale. */
+int dpmm_screener_set_params(dpmm_screener* s, const float* params, const float* mean, const float* scale);
+int dpmm_screener_get_params(dpmm_screener* s, float* params, float* mean, float* scale);
+/* Mean binary cross-entropy with logits over n rows (rows with y = 1 weighted pos_weight) and its
+ * gradient, FP32 forward and backward kernels. */
+int dpmm_screener_loss_grad(dpmm_screener* s, int n, const float* X, const float* y, float pos_weight, float* loss,
+                            float* grad);
+/* One Adam step (Kingma & Ba) on a batch; loss of the batch before the step. */
+int dpmm_screener_train_step(dpmm_screener* s, int n, const float* X, const float* y, float pos_weight, float lr,
+                             float* loss);
+/* Logits of n rows, BF16 MFMA with FP32 accumulation. */
+int dpmm_screener_forward(dpmm_screener* s, int n, const float* X, float* logits);
+
 /* ------------------------------------------------------------------ host driver ----- */
 /* dp_parallel() options (dp_parallel_sampling.h:17-29, global_params.h:36-78). */
 typedef struct {
@@ -155,6 +188,10 @@ typedef struct {
     int mult_use_weight;      /* 0: the reference's multinomial label log-likelihood */
     int precision;            /* DPMM_PREC_* of the label sweep */
     int verbose;
+    /* != 0: start from the labels already in the context (dpmm_set_labels, 0 .. init_clusters - 1)
+     * instead of uniform ones, as the reference's model-file constructor continues a saved model
+     * (dp_parallel_sampling.h:15); sub-labels are drawn uniformly. */
+    int init_from_labels;
 } dpmm_options;
 
 /* init_model + init_first_clusters (dp_parallel_sampling.cpp:91-130).  Gaussian hyper:
@@ -171,6 +208,21 @@ int dpmm_sampler_weights(dpmm_sampler* s, double* weights, int64_t* counts, int 
 /* Host time spent in the last step's FP64 work (draws, posteriors, Hastings), ms. */
 int dpmm_sampler_host_ms(dpmm_sampler* s, double* host_ms);
 int dpmm_sampler_destroy(dpmm_sampler* s);
+
+/* Merge screening for a Gaussian sampler (s not owned): from then on, iterations t with
+ * t % full_scan_every != 0 give the exact merge test only to candidate pairs whose logit exceeds
+ * `threshold`; the others run it on all pairs and count accepted merges scored below it.  s = NULL
+ * turns screening off.  log_pairs != 0 keeps every candidate pair's features, log Hastings ratio
+ * and log U of the last step (screening off: all pairs), the training data of the screener. */
+int dpmm_sampler_set_screener(dpmm_sampler* sampler, dpmm_screener* s, int full_scan_every, float threshold,
+                              int log_pairs);
+/* Totals since creation: exact pair tests run, pairs the screener skipped, accepted merges the
+ * screener scored below threshold on full-scan iterations, full scans. */
+int dpmm_sampler_screen_stats(dpmm_sampler* sampler, int64_t* exact_tests, int64_t* skipped, int64_t* missed,
+                              int64_t* full_scans);
+/* The last step's pair log: up to max_rows rows of features (16 floats), log H and log U. */
+int dpmm_sampler_pair_log(dpmm_sampler* sampler, int max_rows, float* features, double* log_h, double* log_u,
+                          int* n_rows);
 
 /* dp_parallel_sampling_class::dp_parallel (dp_parallel_sampling.cpp:132-160) end to end:
  * labels (N, 0-based), cluster count history (iters), per-iteration seconds (iters). */

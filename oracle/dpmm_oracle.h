@@ -102,6 +102,18 @@ int orc_run(const orc_run_opts* opts, const double* X, int64_t N, int D, double 
             double v0, const double* psi0, const double* alpha0, int32_t* labels, int32_t* k_hist,
             double* weights_out, int max_k_out, double* iter_seconds);
 
+/* Merge-pair screener (mlp_oracle.cpp; no reference counterpart): MLP 16 -> 128 -> 128 -> 1 with
+ * parameters [W1 (128x16) | b1 | W2 (128x128) | b2 | w3 | b3], ReLU hidden layers, logit output;
+ * mean weighted binary cross-entropy with logits and its gradient; pair features from cluster
+ * summaries (counts, log marginal likelihoods, log det psi_n, posterior means, diag psi_n). */
+int orc_mlp_param_count(void);
+void orc_mlp_forward(const double* params, const double* X, int n, double* z);
+double orc_mlp_loss_grad(const double* params, const double* X, const double* y, int n, double pos_weight,
+                         double* grad);
+void orc_pair_features(int K, int D, const double* N, const double* Nl, const double* Nr, const double* L,
+                       const double* Ll, const double* Lr, const double* ld, const double* m, const double* dpsi,
+                       int npairs, const int32_t* pi, const int32_t* pj, double* out);
+
 #ifdef __cplusplus
 }
 #endif

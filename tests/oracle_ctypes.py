@@ -46,6 +46,7 @@ def lib():
         L.orc_merge_log_hr.argtypes = [c_dbl, c_i64, c_i64, c_dbl, c_dbl, c_dbl]
         L.orc_nmi.restype = c_dbl
         L.orc_run.restype = c_int
+        L.orc_mlp_loss_grad.restype = c_dbl
         _lib = L
     return _lib
 
@@ -215,3 +216,30 @@ def run(X, prior=0, alpha=1.0, iters=100, init_clusters=1, burnout=15, seed=0, k
         K = lib().orc_run(ctypes.byref(o), _p(X), c_i64(N), c_int(D), c_dbl(0), None, c_dbl(0), None,
                           _p(f64(alpha0)), _p(labels), _p(khist), _p(weights), c_int(4096), _p(secs))
     return {"K": K, "labels": labels, "k_hist": khist, "weights": weights[:max(K, 0)], "iter_seconds": secs}
+
+
+def mlp_param_count():
+    return lib().orc_mlp_param_count()
+
+
+def mlp_forward(params, X):
+    X = f64(X)
+    z = np.empty(X.shape[0])
+    lib().orc_mlp_forward(_p(f64(params)), _p(X), c_int(X.shape[0]), _p(z))
+    return z
+
+
+def mlp_loss_grad(params, X, y, pos_weight=1.0):
+    X, y = f64(X), f64(y)
+    g = np.empty(mlp_param_count())
+    loss = lib().orc_mlp_loss_grad(_p(f64(params)), _p(X), _p(y), c_int(X.shape[0]), c_dbl(pos_weight), _p(g))
+    return loss, g
+
+
+def pair_features(N, Nl, Nr, L, Ll, Lr, ld, m, dpsi, pi, pj):
+    K, D = np.asarray(m).shape
+    out = np.empty((len(pi), 16))
+    lib().orc_pair_features(c_int(K), c_int(D), _p(f64(N)), _p(f64(Nl)), _p(f64(Nr)), _p(f64(L)), _p(f64(Ll)),
+                            _p(f64(Lr)), _p(f64(ld)), _p(f64(m)), _p(f64(dpsi)), c_int(len(pi)), _p(i32(pi)),
+                            _p(i32(pj)), _p(out))
+    return out

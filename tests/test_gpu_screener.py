@@ -1,0 +1,162 @@
+"""The learned merge screener on the device: FP32 training gradients against the FP64 oracle's
+(rtol 1e-4), BF16-MFMA inference against an FP64 evaluation of the same BF16-rounded operands
+(|dz| <= 5e-3 (1 + |z|)), device pair features against the oracle's (rtol 1e-5), training on a
+known rule, and the end-to-end gain: a screener trained on the plain sampler's logged merge
+decisions cuts the exact merge tests while the clustering stays the same."""
+import numpy as np
+import pytest
+
+import dpmmsubclusters_gpu_amd as dp
+from dpmmsubclusters_gpu_amd import screener as scr
+from dpmmsubclusters_gpu_amd.synthetic import gaussian_mixture
+import oracle_ctypes as orc
+from helpers import have_gpu, random_spd, say
+
+pytestmark = pytest.mark.gpu
+F, H = 16, 128
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    if not have_gpu():
+        pytest.fail("no GPU visible to the HIP library")
+
+
+def _bf16(a):
+    u = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    u = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def _unpack(p):
+    o = 0
+    W1 = p[o:o + H * F].reshape(H, F); o += H * F
+    b1 = p[o:o + H]; o += H
+    W2 = p[o:o + H * H].reshape(H, H); o += H * H
+    b2 = p[o:o + H]; o += H
+    w3 = p[o:o + H]; o += H
+    return W1, b1, W2, b2, w3, p[o]
+
+
+def test_gradient_vs_oracle():
+    rng = np.random.default_rng(0)
+    say("screener gradient")
+    s = scr.Screener(seed=3)
+    p, _, _ = s.get_params()
+    p = p + rng.normal(size=p.size).astype(np.float32) * 0.05   # non-zero biases too
+    s.set_params(p)
+    X = rng.normal(size=(300, F)).astype(np.float32)
+    y = (rng.random(300) < 0.3).astype(np.float32)
+    loss, g = s.loss_grad(X, y, pos_weight=2.5)
+    oloss, og = orc.mlp_loss_grad(p.astype(np.float64), X.astype(np.float64), y.astype(np.float64), 2.5)
+    np.testing.assert_allclose(loss, oloss, rtol=1e-5)
+    np.testing.assert_allclose(g, og, rtol=1e-4, atol=1e-6 * np.abs(og).max())
+
+
+def test_bf16_forward_vs_bf16_evaluation():
+    rng = np.random.default_rng(1)
+    say("screener BF16 inference")
+    s = scr.Screener(seed=5)
+    p, _, _ = s.get_params()
+    mean = rng.normal(size=F).astype(np.float32)
+    scale = rng.uniform(0.5, 2.0, F).astype(np.float32)
+    s.set_params(p, mean, scale)
+    X = (rng.normal(size=(1000, F)) * scale + mean).astype(np.float32)   # not a multiple of 128 rows
+    z = s.logits(X)
+    W1, b1, W2, b2, w3, b3 = _unpack(p.astype(np.float64))
+    xh = _bf16((X - mean) / scale)
+    h1 = _bf16(np.maximum(xh @ _bf16(W1).T + b1, 0))
+    h2 = np.maximum(h1 @ _bf16(W2).T + b2, 0)
+    want = h2 @ w3 + b3
+    assert np.all(np.abs(z - want) <= 5e-3 * (1 + np.abs(want)))
+    exact = orc.mlp_forward(p.astype(np.float64), ((X - mean) / scale).astype(np.float64))
+    print("BF16 logit error vs FP64: max", np.abs(z - exact).max(), "mean", np.abs(z - exact).mean())
+
+
+def test_pair_features_vs_oracle():
+    rng = np.random.default_rng(2)
+    N, D, K = 5000, 8, 5
+    X, gt, _ = gaussian_mixture(N, D, K, seed=4, dtype=np.float64)
+    labels = gt.astype(np.int32)
+    subs = rng.integers(0, 2, N).astype(np.int32)
+    k0, v0 = 1.0, D + 3.0
+    m0, psi0 = np.zeros(D), np.eye(D) + random_spd(rng, D, 0.1)
+    dev = dp.DeviceSweep(X, dp.prior_type.Gaussian, seed=1)
+    dev.set_labels(labels, subs)
+    Nsub, _, _ = dev.sufficient_statistics(K, counts_only=True)
+    dev.set_prior(dp.niw_hyperparams(k0, m0, v0, psi0))
+    jobs = []
+    for k in range(K):
+        jobs += [[2 * k, 2 * k + 1, -1, -1], [2 * k, -1, -1, -1], [2 * k + 1, -1, -1, -1]]
+    L, ld = dev.log_marginal_batch(jobs)
+    L, ld = L.reshape(K, 3), ld.reshape(K, 3)
+    Nl, Nr = Nsub[0::2].astype(float), Nsub[1::2].astype(float)
+    scal = np.stack([Nl + Nr, Nl, Nr, L[:, 0], L[:, 1], L[:, 2], ld[:, 0]], 1)
+    pi, pj = np.triu_indices(K, 1)
+    say("pair features")
+    f = dev.pair_features(K, scal, pi, pj)
+    _, _, _, Nc, sc, Sc = orc.suff_stats(0, X, labels, subs, K)
+    m = np.empty((K, D))
+    dpsi = np.empty((K, D))
+    for k in range(K):
+        _, mk, _, psik = orc.niw_posterior(k0, m0, v0, psi0, Nc[k], sc[k], Sc[k])
+        m[k], dpsi[k] = mk, np.diag(psik)
+    want = orc.pair_features(scal[:, 0], Nl, Nr, scal[:, 3], scal[:, 4], scal[:, 5], scal[:, 6], m, dpsi, pi, pj)
+    np.testing.assert_allclose(f, want, rtol=1e-5, atol=1e-6)
+
+
+def test_training_learns_a_rule():
+    rng = np.random.default_rng(3)
+    X = rng.normal(size=(24000, F)).astype(np.float32) * 2 + 1
+    y = ((X[:, 9] < 0.5) & (X[:, 0] > 1.0)).astype(np.float32)
+    say("screener training")
+    s, hist = scr.train(X[:20000], y[:20000], epochs=12, batch=2048, lr=3e-3, seed=1)
+    acc = ((s.logits(X[20000:]) > 0) == (y[20000:] > 0.5)).mean()
+    print("loss per epoch", np.round(hist, 4), "held-out accuracy", acc)
+    assert hist[-1] < 0.5 * hist[0]
+    assert acc > 0.95
+
+
+def test_screener_gain_end_to_end():
+    """Train on merge decisions logged by the plain sampler on two overlapping mixtures started
+    over-split; on a third, the screened sampler runs far fewer exact merge tests, misses no
+    accepted merge, and so follows the plain sampler's trajectory exactly."""
+    say("screener data collection")
+    feats, hs, acc = [], [], []
+    for seed in (11, 12, 14):
+        f, h, u = scr.collect_pairs(60_000, 16, 20, pieces=3, iters=60, seed=seed, spread=10.0)
+        feats.append(f)
+        hs.append(h)
+        acc.append(h > u)
+    X, h, a = np.concatenate(feats), np.concatenate(hs), np.concatenate(acc)
+    y = scr.plausible(h)
+    print("logged pairs", len(y), "plausible", int(y.sum()), "accepted", int(a.sum()))
+    assert len(y) > 5000 and y.sum() >= 20
+    s, hist = scr.train(X, y, epochs=30, batch=4096, lr=2e-3, seed=2)
+    z = s.logits(X)
+    thr = scr.threshold_for_recall(z, y, recall=1.0, margin=2.0)
+    assert np.all(z[a] > thr)
+    Xe, gt, _ = gaussian_mixture(60_000, 16, 20, seed=13, spread=10.0, dtype=np.float32)
+    hyper = dp.niw_hyperparams(1.0, np.zeros(16), 19.0, np.eye(16))
+    out = {}
+    for name, use in (("plain", False), ("screened", True)):
+        say(f"screener gain run {name}")
+        dev = dp.DeviceSweep(Xe, dp.prior_type.Gaussian, seed=13)
+        smp = dp.Sampler(dev, hyper, 10.0, 80, seed=13, init_labels=scr.over_split_labels(gt, 3, 13))
+        smp.set_screener(s if use else None, full_scan_every=10, threshold=thr)
+        hist_k = []
+        for t in range(1, 81):
+            smp.step(t)
+            hist_k.append(smp.num_clusters())
+        lab, _ = dev.get_labels()
+        out[name] = (hist_k, lab, dp._nmi(gt, lab), smp.screen_stats())
+        smp.close()
+        dev.close()
+    st_p, st_s = out["plain"][3], out["screened"][3]
+    print("exact tests plain", st_p[0], "screened", st_s[0], "skipped", st_s[1], "missed", st_s[2],
+          "full scans", st_s[3], "nmi", out["plain"][2], out["screened"][2], "threshold", thr)
+    assert st_s[3] == 7   # full scans at iterations 10, ..., 70 (the last 5 iterations test no merges)
+    assert st_s[0] <= 0.35 * st_p[0], "the screener should cut the exact merge tests"
+    assert st_s[2] == 0, "an accepted merge was screened out"
+    assert out["screened"][0] == out["plain"][0]
+    np.testing.assert_array_equal(out["screened"][1], out["plain"][1])

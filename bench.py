@@ -26,6 +26,8 @@ CONFIGS = {
     "c1_readme_2d": dict(N=100_000, D=2, K=2, a=1.0, ridge=1.0, weight=None, alpha=1.0, dtype="float64"),
     "c2_d32": dict(N=1_000_000, D=32, K=20, a=1.0, ridge=1.0, weight=1.0, alpha=10.0, dtype="float32"),
     "c3_d64": dict(N=10_000_000, D=64, K=50, a=1.0, ridge=0.5, weight=0.5, alpha=10.0, dtype="float32"),
+    # multinomial: N documents of `words` words over a vocabulary of D, topics theta ~ Dir(conc)
+    "c4_mult": dict(N=1_000_000, D=1000, K=100, words=100, conc=0.1, alpha=10.0, dtype="int32", prior=1.0),
 }
 
 
@@ -53,6 +55,58 @@ def gen_on_gpu(cfg, seed):
         X[idx] = z @ Lt + torch.tensor(means[k], device="cuda", dtype=dt)
     torch.cuda.synchronize()
     return X, gt, means, np.stack(facs), w
+
+
+def gen_docs_on_gpu(cfg, seed):
+    """Bag-of-words documents generated on the GPU: topic uniform, `words` draws from theta_topic by
+    inverse CDF, counted into a dense N x D int32 matrix."""
+    import torch
+    rng = np.random.default_rng(seed)
+    N, D, K, W = cfg["N"], cfg["D"], cfg["K"], cfg["words"]
+    theta = rng.dirichlet(np.full(D, cfg["conc"]), size=K)
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    gt = torch.randint(0, K, (N,), device="cuda", generator=g)
+    X = torch.zeros((N, D), device="cuda", dtype=torch.int32)
+    for k in range(K):
+        idx = (gt == k).nonzero().squeeze(1)
+        if idx.numel() == 0:
+            continue
+        cdf = torch.tensor(np.cumsum(theta[k]), device="cuda", dtype=torch.float64)
+        u = torch.rand((idx.numel(), W), device="cuda", dtype=torch.float64, generator=g) * cdf[-1]
+        words = torch.searchsorted(cdf, u).clamp_(max=D - 1)
+        cnt = torch.zeros((idx.numel(), D), device="cuda", dtype=torch.int32)
+        cnt.scatter_add_(1, words, torch.ones_like(words, dtype=torch.int32))
+        X[idx] = cnt
+    torch.cuda.synchronize()
+    return X, gt, theta
+
+
+def cpu_baseline_mult(Xs, theta, seed, target_s=8.0):
+    """FP64 oracle (OpenMP) on a bounded sample: one iteration's label sweep, sub-label sweep and
+    statistics with the generating topics as cluster parameters."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_ctypes as orc
+    K, D = theta.shape
+    lt = np.log(theta + 1e-300)
+    slt = np.repeat(lt, 2, axis=0)
+    lrw = np.full(2 * K, 0.5)
+
+    def one(n):
+        X = np.ascontiguousarray(Xs[:n], dtype=np.float64)
+        t0 = time.perf_counter()
+        lab, _ = orc.label_sweep(1, X, logtheta=lt, weights=np.full(K, 1.0 / K), seed=seed, it=1)
+        sub, _ = orc.sublabel_sweep(1, X, lab, logtheta=slt, lr_weights=lrw, seed=seed, it=1)
+        orc.suff_stats(1, X, lab, sub, K, want_S=False)
+        return time.perf_counter() - t0
+
+    n = min(len(Xs), 5_000)
+    dt = one(n)
+    n2 = int(min(len(Xs), max(n, n * target_s / max(dt, 1e-3))))
+    dt2 = one(n2)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": n2 / dt2, "unit": "point*iters/s", "cores": threads, "kind": "port",
+            "sample": f"FP64 oracle label sweep + sub-label sweep + statistics of one iteration over {n2} documents "
+                      f"(K={K}, D={D}), {dt2:.1f} s"}
 
 
 def cpu_baseline(Xs, means, facs, w, seed, target_s=8.0):
@@ -109,16 +163,24 @@ def main():
 
     cfg = CONFIGS[args.config]
     N, D = cfg["N"], cfg["D"]
+    mult = cfg["dtype"] == "int32"
     print(f"[bench] generating {args.config}: N={N} D={D} K*={cfg['K']}", file=sys.stderr, flush=True)
-    X, gt, means, facs, w = gen_on_gpu(cfg, args.seed)
-    dtype = np.float64 if cfg["dtype"] == "float64" else np.float32
-    dev = dp.DeviceSweep(None, dp.prior_type.Gaussian, seed=args.seed + 1, device_ptr=X.data_ptr(), N=N, D=D,
-                         dtype=dtype)
-    hyper = dp.niw_hyperparams(1.0, np.zeros(D), D + 3.0, np.eye(D))
-    if dtype == np.float64:
-        prec = dp.Precision.FP64
-    else:
+    if mult:
+        X, gt, theta = gen_docs_on_gpu(cfg, args.seed)
+        dev = dp.DeviceSweep(None, dp.prior_type.Multinomial, seed=args.seed + 1, device_ptr=X.data_ptr(), N=N, D=D,
+                             dtype=np.int32)
+        hyper = dp.multinomial_hyper(np.full(D, cfg["prior"]))
         prec = dp.Precision.MIXED if args.precision == "mixed" else dp.Precision.FP32
+    else:
+        X, gt, means, facs, w = gen_on_gpu(cfg, args.seed)
+        dtype = np.float64 if cfg["dtype"] == "float64" else np.float32
+        dev = dp.DeviceSweep(None, dp.prior_type.Gaussian, seed=args.seed + 1, device_ptr=X.data_ptr(), N=N, D=D,
+                             dtype=dtype)
+        hyper = dp.niw_hyperparams(1.0, np.zeros(D), D + 3.0, np.eye(D))
+        if dtype == np.float64:
+            prec = dp.Precision.FP64
+        else:
+            prec = dp.Precision.MIXED if args.precision == "mixed" else dp.Precision.FP32
     if args.copy_probe:
         Y = X.clone()
         torch.cuda.synchronize()
@@ -141,17 +203,29 @@ def main():
         smp.step(t)
         a, b, c = dev.kernel_ms()
         lab_ms.append(a); sub_ms.append(b); st_ms.append(c); host_ms.append(smp.host_ms())
-        if prec == dp.Precision.MIXED:
+        if prec == dp.Precision.MIXED and not mult:
             rechk.append(dev.last_recheck_count())
-        flops.append(2.0 * kc * D * D * N)
+        flops.append(2.0 * kc * D * (1 if mult else D) * N)
     dev.synchronize()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     ms_step = 1000 * elapsed / args.steps
     value = N * args.steps / elapsed
     avg_lab = float(np.mean(lab_ms))
-    achieved = float(np.mean(flops)) / (avg_lab * 1e-3) / 1e12
-    peak = PEAK[{dp.Precision.FP64: "fp64", dp.Precision.FP32: "fp32", dp.Precision.MIXED: "bf16"}[prec]]
+    if mult:   # HBM-bound: one read of the N x D int32 counts, one label write
+        bytes_launch = N * (4.0 * D + 4.0)
+        roof = {"bound": "hbm", "kernel": "k_label_mult", "achieved": bytes_launch / (avg_lab * 1e-3) / 1e9,
+                "peak": HBM_PEAK, "unit": "GB/s", "traffic": None,
+                "algorithmic": "4 D + 4 bytes per document per launch (counts read once, label written)",
+                "flops_TFLOP/s": float(np.mean(flops)) / (avg_lab * 1e-3) / 1e12}
+    else:
+        achieved = float(np.mean(flops)) / (avg_lab * 1e-3) / 1e12
+        peak = PEAK[{dp.Precision.FP64: "fp64", dp.Precision.FP32: "fp32", dp.Precision.MIXED: "bf16"}[prec]]
+        roof = {"bound": "mfma",
+                "kernel": "k_label_gauss_mx + k_label_gauss_list" if prec == dp.Precision.MIXED else "k_label_gauss",
+                "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "traffic": None,
+                "algorithmic": "2*K*D^2 flops per point per launch (dense whitening of K clusters)"}
+    roof["frac"] = roof["achieved"] / roof["peak"]
     K_final = smp.num_clusters()
     from dpmmsubclusters_gpu_amd import _nmi
     lab, _ = dev.get_labels()
@@ -160,26 +234,27 @@ def main():
         "metric": "point*iters/sec", "value": value, "unit": "point*iters/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None,
-        "dtype": {dp.Precision.FP64: "f64", dp.Precision.FP32: "f32", dp.Precision.MIXED: "bf16+f32"}[prec],
+        "dtype": ("i32+f32" if mult else
+                  {dp.Precision.FP64: "f64", dp.Precision.FP32: "f32", dp.Precision.MIXED: "bf16+f32"}[prec]),
         "data": "synthetic",
-        "config": {"workload": f"{args.config}: Gaussian DPMM sampler iteration, N={N}, D={D}, "
-                               f"{cfg['K']} true clusters, alpha={cfg['alpha']}, NIW(k=1, m=0, v=D+3, psi=I)",
+        "config": {"workload": (f"{args.config}: multinomial DPMM sampler iteration, N={N} documents x "
+                                f"{cfg['words']} words, vocabulary D={D}, {cfg['K']} topics theta~Dir({cfg['conc']}), "
+                                f"alpha={cfg['alpha']}, Dirichlet prior {cfg['prior']}") if mult else
+                               (f"{args.config}: Gaussian DPMM sampler iteration, N={N}, D={D}, "
+                                f"{cfg['K']} true clusters, alpha={cfg['alpha']}, NIW(k=1, m=0, v=D+3, psi=I)"),
                    "N": N, "D": D, "clusters_timed": K_final, "precision": args.precision,
                    "parallelism": "single GPU"},
-        "roofline": {"bound": "mfma",
-                     "kernel": "k_label_gauss_mx + k_label_gauss_list" if prec == dp.Precision.MIXED else "k_label_gauss",
-                     "achieved": achieved, "peak": peak,
-                     "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
-                     "algorithmic": "2*K*D^2 flops per point per launch (dense whitening of K clusters)"},
+        "roofline": roof,
         "breakdown_ms": {"label_sweep": avg_lab, "sub_sweep": float(np.mean(sub_ms)),
-                         "statistics": float(np.mean(st_ms)), "host_fp64": float(np.mean(host_ms))},
+                         "statistics": float(np.mean(st_ms)), "conjugate": float(np.mean(host_ms))},
         "final_nmi_vs_truth": nmi,
         "recheck_fraction": (float(np.mean(rechk)) / N) if rechk else None,
     }
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline (FP64 oracle)", file=sys.stderr, flush=True)
-        Xs = X[: min(N, 2_000_000)].cpu().numpy()
-        out["cpu_baseline"] = cpu_baseline(Xs, means, facs, w, args.seed)
+        Xs = X[: min(N, 2_000_000 if not mult else 200_000)].cpu().numpy()
+        out["cpu_baseline"] = (cpu_baseline_mult(Xs, theta, args.seed) if mult else
+                               cpu_baseline(Xs, means, facs, w, args.seed))
     print(json.dumps(out), flush=True)
 
 

@@ -26,6 +26,7 @@
 #include "dpmm_gauss.h"
 #include "dpmm_mixed.h"
 #include "dpmm_niw.h"
+#include "dpmm_mult.h"
 
 using namespace dpmm;
 
@@ -329,6 +330,18 @@ __global__ void k_zero_int(int* p, int n) {
     if (i < n) p[i] = 0;
 }
 
+// Largest value of an int32 array (one atomic per workgroup).
+__global__ void __launch_bounds__(kThreads) k_max_i32(const int32_t* __restrict__ x, int64_t n, int* __restrict__ out) {
+    int m = 0;
+    for (int64_t e = (int64_t)blockIdx.x * kThreads + threadIdx.x; e < n; e += (int64_t)gridDim.x * kThreads)
+        m = max(m, x[e]);
+    for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o));
+    __shared__ int wm[4];
+    if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicMax(out, max(max(wm[0], wm[1]), max(wm[2], wm[3])));
+}
+
 // ---- relabel -----------------------------------------------------------------------------------
 __global__ void __launch_bounds__(kThreads) k_relabel(int32_t* __restrict__ labels, int32_t* __restrict__ subs,
                                                       int64_t N, const int32_t* __restrict__ table, int K,
@@ -413,6 +426,11 @@ struct dpmm_ctx {
     size_t logwCap = 0;
     float* dFeat = nullptr;
     size_t featCap = 0;
+    // mixed multinomial sweep: split-BF16 tile block for clusters in NCB blocks of 32
+    unsigned char* dMB = nullptr;
+    size_t mbCap = 0;
+    int mm_ncb = 0;        // 0: no tile block for the current parameters
+    int max_count = -1;    // largest count in X (multinomial), -1 until measured
     int Kdraw = 0;
     int drawStride() const {
         return prior == DPMM_PRIOR_GAUSSIAN ? ((D + D * (D + 1) / 2 + 1) + 1) & ~1 : (D + 1) & ~1;
@@ -608,17 +626,39 @@ void pack_mx_slot(int D, int DPM, const double* mu, const double* R, double cst,
     m[DPM] = (float)cst;
 }
 
+// Recheck list of the mixed sweeps.
+int ensure_recheck(dpmm_ctx* c) {
+    if (!c->dRecheck) {
+        if (hipMalloc((void**)&c->dRecheck, c->N * sizeof(int32_t)) != hipSuccess ||
+            hipMalloc((void**)&c->dRecheckCount, sizeof(int)) != hipSuccess)
+            return fail(DPMM_ENOMEM, "recheck alloc");
+    }
+    return DPMM_OK;
+}
+
 // BF16 screen slots for K clusters and the recheck list.
 int ensure_mixed_buffers(dpmm_ctx* c, int K) {
     unsigned char* pb = reinterpret_cast<unsigned char*>(c->dPB);
     int rc = grow(pb, c->pbCap, (size_t)K * mx_slot_bytes(c->DPM));
     c->dPB = pb;
     if (rc) return rc;
-    if (!c->dRecheck) {
-        if (hipMalloc((void**)&c->dRecheck, c->N * sizeof(int32_t)) != hipSuccess ||
-            hipMalloc((void**)&c->dRecheckCount, sizeof(int)) != hipSuccess)
-            return fail(DPMM_ENOMEM, "recheck alloc");
-    }
+    return ensure_recheck(c);
+}
+
+// Split-BF16 tiles of the K clusters' log theta (cluster k's D values at src + k * row_stride).
+int pack_mult_mixed(dpmm_ctx* c, int K, const double* src, size_t row_stride, const double* logw, int use_weight) {
+    c->mm_ncb = 0;
+    if (K > 256) return DPMM_OK;   // the FP32 sweep serves larger K
+    const int ncb = (K + 31) / 32;
+    int rc = grow(c->dMB, c->mbCap, mm_block_bytes(c->D, ncb));
+    if (rc) return rc;
+    rc = ensure_recheck(c);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_mult_mx_clear, dim3(256), dim3(kThreads), 0, c->stream, c->dMB, c->D, ncb);
+    hipLaunchKernelGGL(k_pack_mult_mx, dim3(K), dim3(kThreads), 0, c->stream, src, row_stride, c->D, ncb, logw,
+                       use_weight, c->dMB);
+    HIPCHK(hipGetLastError());
+    c->mm_ncb = ncb;
     return DPMM_OK;
 }
 
@@ -766,6 +806,34 @@ void launch_label_mult_k(dpmm_ctx* c, uint32_t iter, int fin) {
                        c->dLabels);
 }
 
+template <int NCB>
+void launch_label_mult_mx_k(dpmm_ctx* c, uint32_t iter, int fin) {
+    const int grid = (int)((c->N + 127) / 128);
+    hipLaunchKernelGGL((k_label_mult_mx<NCB>), dim3(grid), dim3(kThreads), 0, c->stream,
+                       reinterpret_cast<const int32_t*>(c->dX), c->N, c->D, c->dMB, c->key0(), c->key1(), iter, fin,
+                       c->dLabels, c->dRecheck, c->dRecheckCount);
+}
+
+// Split-BF16 MFMA screen over all documents, then the FP32 sweep over the ones it could not decide.
+int run_label_mult_mixed(dpmm_ctx* c, uint32_t iter, int fin) {
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, c->dRecheckCount, 1);
+    switch (c->mm_ncb) {
+        case 1: launch_label_mult_mx_k<1>(c, iter, fin); break;
+        case 2: launch_label_mult_mx_k<2>(c, iter, fin); break;
+        case 3: launch_label_mult_mx_k<3>(c, iter, fin); break;
+        case 4: launch_label_mult_mx_k<4>(c, iter, fin); break;
+        case 5: launch_label_mult_mx_k<5>(c, iter, fin); break;
+        case 6: launch_label_mult_mx_k<6>(c, iter, fin); break;
+        case 7: launch_label_mult_mx_k<7>(c, iter, fin); break;
+        default: launch_label_mult_mx_k<8>(c, iter, fin); break;
+    }
+    const int grid = (int)std::min<int64_t>(1024, (c->N + 3) / 4);
+    hipLaunchKernelGGL(k_label_mult_list, dim3(grid), dim3(kThreads), 0, c->stream,
+                       reinterpret_cast<const int32_t*>(c->dX), c->D, c->K, reinterpret_cast<const float*>(c->dP),
+                       c->stride, c->key0(), c->key1(), iter, fin, c->dRecheck, c->dRecheckCount, c->dLabels);
+    return DPMM_OK;
+}
+
 int launch_label_mult(dpmm_ctx* c, uint32_t iter, int fin) {
     if ((size_t)(32 * 65 + c->K * 33) * sizeof(float) > 65536) return fail(DPMM_EINVAL, "multinomial K > 433");
     const int kpt = (c->K + 3) / 4;
@@ -808,7 +876,7 @@ int dpmm_destroy(dpmm_ctx* c) {
                     (void*)c->dRowtot, (void*)c->dLabelStart, (void*)c->dChunkStart, (void*)c->dTable,
                     (void*)c->dPartials, (void*)c->dStats, c->dPB, (void*)c->dRecheck, (void*)c->dRecheckCount,
                     (void*)c->dPrior, (void*)c->dMembers, (void*)c->dLM, (void*)c->dDraws, (void*)c->dDrawStatus,
-                    (void*)c->dLogW, (void*)c->dFeat})
+                    (void*)c->dLogW, (void*)c->dFeat, (void*)c->dMB})
         if (p) (void)hipFree(p);
     if (c->hStats) (void)hipHostFree(c->hStats);
     for (auto& e : c->ev)
@@ -824,8 +892,23 @@ int dpmm_set_precision(dpmm_ctx* c, int precision) {
         return fail(DPMM_EINVAL, "FP64 sweep needs f64 data");
     if (precision != DPMM_PREC_FP64 && c->dtype == DPMM_DTYPE_F64)
         return fail(DPMM_EINVAL, "f64 data sweeps in FP64");
-    if (precision == DPMM_PREC_MIXED && (c->prior != DPMM_PRIOR_GAUSSIAN || c->dtype != DPMM_DTYPE_F32))
-        return fail(DPMM_EINVAL, "mixed precision is the Gaussian f32 sweep");
+    if (precision == DPMM_PREC_MIXED && c->dtype == DPMM_DTYPE_I32) {
+        // counts enter the MFMA as BF16, exact up to 256
+        if (c->max_count < 0) {
+            int* dm = nullptr;
+            if (hipMalloc((void**)&dm, sizeof(int)) != hipSuccess) return fail(DPMM_ENOMEM, "max count");
+            hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, dm, 1);
+            hipLaunchKernelGGL(k_max_i32, dim3(1024), dim3(kThreads), 0, c->stream,
+                               reinterpret_cast<const int32_t*>(c->dX), c->N * (int64_t)c->D, dm);
+            int m = 0;
+            const hipError_t e1 = hipMemcpyAsync(&m, dm, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+            const hipError_t e2 = hipStreamSynchronize(c->stream);
+            (void)hipFree(dm);
+            if (e1 != hipSuccess || e2 != hipSuccess) return fail(DPMM_EHIP, "max count copy");
+            c->max_count = m;
+        }
+        if (c->max_count > 256) return fail(DPMM_EINVAL, "mixed multinomial sweep needs counts <= 256");
+    }
     c->precision = precision;
     c->K = 0;
     return DPMM_OK;
@@ -899,6 +982,18 @@ int dpmm_set_multinomial_params(dpmm_ctx* c, int K, const double* logtheta, cons
     }
     HIPCHK(hipMemcpyAsync(c->dP, c->hP.data(), c->hP.size(), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->dSP, c->hSP.data(), c->hSP.size(), hipMemcpyHostToDevice, c->stream));
+    c->mm_ncb = 0;
+    if (c->precision == DPMM_PREC_MIXED) {   // split-BF16 tiles from the same log theta (scratch: dLM)
+        rc = grow(c->dLM, c->lmCap, (size_t)K * D + K);
+        if (rc) return rc;
+        std::vector<double> lw(K, 0.0);
+        if (use_weight)
+            for (int k = 0; k < K; ++k) lw[k] = std::log(weights[k]);
+        HIPCHK(hipMemcpyAsync(c->dLM, logtheta, (size_t)K * D * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(c->dLM + (size_t)K * D, lw.data(), K * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        rc = pack_mult_mixed(c, K, c->dLM, (size_t)D, c->dLM + (size_t)K * D, use_weight);
+        if (rc) return rc;
+    }
     HIPCHK(hipStreamSynchronize(c->stream));
     c->K = K;
     return DPMM_OK;
@@ -915,7 +1010,8 @@ int dpmm_sample_cluster_labels(dpmm_ctx* c, uint32_t iter, int fin) {
         if (c->params_f64) { DPMM_DISPATCH_DP64(c->DP, lg_f64, c, iter, fin); }
         else { DPMM_DISPATCH_DP(c->DP, lg_f32, c, iter, fin); }
     } else {
-        int rc = launch_label_mult(c, iter, fin);
+        int rc = (c->precision == DPMM_PREC_MIXED && c->mm_ncb > 0) ? run_label_mult_mixed(c, iter, fin)
+                                                                     : launch_label_mult(c, iter, fin);
         if (rc) return rc;
     }
     (void)hipEventRecord(c->ev[1], c->stream);
@@ -1238,6 +1334,9 @@ int dpmm_sample_params(dpmm_ctx* c, uint32_t iter, int K, const int32_t* members
         if ((rc = ensure_param_buffers(c, K, (size_t)stride * 4))) return rc;
         hipLaunchKernelGGL(k_pack_mult, dim3(np), dim3(kThreads), 0, c->stream, c->dDraws, dstride, D, stride,
                            c->dLogW, K, use_weight, reinterpret_cast<float*>(c->dP), reinterpret_cast<float*>(c->dSP));
+        c->mm_ncb = 0;
+        if (c->precision == DPMM_PREC_MIXED && (rc = pack_mult_mixed(c, K, c->dDraws, (size_t)3 * dstride, c->dLogW, use_weight)))
+            return rc;
     }
     HIPCHK(hipGetLastError());
     std::vector<int32_t> st(np);

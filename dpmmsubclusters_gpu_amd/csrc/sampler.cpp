@@ -153,7 +153,6 @@ struct SamplerImpl {
             job(&c.l);
             job(&c.r);
         }
-        const int nparts = (int)dst.size();
         std::vector<size_t> tested;   // pair indices whose exact test joins this launch
         if (pairs) {
             cand_pos.assign(K, -1);

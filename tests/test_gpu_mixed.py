@@ -87,3 +87,69 @@ def test_mixed_sampler_end_to_end():
     print("mixed K", mi.cluster_count_history[-1], "nmi", nm, "fp32 K", ref.cluster_count_history[-1], "nmi", nr)
     assert abs(nm - nr) < 0.01
     np.testing.assert_array_equal(mi.labels, ref.labels)   # same draws, exact screen -> same trajectory
+
+
+def _docs(N, D, K, seed, words=80):
+    rng = np.random.default_rng(seed)
+    theta = rng.dirichlet(np.full(D, 0.1), K)
+    gt = rng.integers(0, K, N)
+    cdf = np.cumsum(theta, 1)
+    X = np.zeros((N, D), np.int32)
+    for k in range(K):
+        idx = np.nonzero(gt == k)[0]
+        w = np.minimum(np.searchsorted(cdf[k], rng.random((idx.size, words)) * cdf[k, -1]), D - 1)
+        np.add.at(X, (np.repeat(idx, words), w.ravel()), 1)
+    return X, gt, theta
+
+
+@pytest.mark.parametrize("D,K", [(300, 20), (1000, 100), (200, 200)])
+@pytest.mark.parametrize("final", [False, True])
+def test_mixed_multinomial_equals_fp32_and_oracle(D, K, final):
+    """Split-BF16 MFMA screen + FP32 recheck: labels identical to the FP32 sweep."""
+    rng = np.random.default_rng(D + K)
+    X, gt, theta = _docs(20_000, D, K, seed=K)
+    # parameters near the truth, plus a few near-duplicate topics to force close calls
+    lt = np.log(np.stack([rng.dirichlet(theta[k] * 500 + 0.05) for k in range(K)]))
+    lt[1::7] = lt[0::7][:len(lt[1::7])] + rng.normal(size=lt[1::7].shape) * 0.01
+    w = rng.dirichlet(np.ones(K))
+    slt, lrw = np.repeat(lt, 2, axis=0), np.full(2 * K, 0.5)
+    say(f"mixed multinomial D={D} K={K} final={final}")
+    dev = dp.DeviceSweep(X, dp.prior_type.Multinomial, seed=7)
+    dev.set_multinomial_params(lt, w, slt, lrw)
+    dev.sample_cluster_labels(5, final)
+    ref, _ = dev.get_labels()
+    dev.set_precision(dp.Precision.MIXED)
+    dev.set_multinomial_params(lt, w, slt, lrw)
+    dev.set_labels(np.full(len(X), -7, np.int32), None)
+    dev.sample_cluster_labels(5, final)
+    mx, _ = dev.get_labels()
+    nre = dev.last_recheck_count()
+    print(f"D={D} K={K} final={final}: rechecked {nre} of {len(X)}")
+    assert (mx >= 0).all()
+    np.testing.assert_array_equal(mx, ref)
+    olab, margin = orc.label_sweep(1, X, logtheta=lt, weights=w, seed=7, it=5, final=final)
+    assert not ((mx != olab) & (margin >= 1e-6)).any()
+
+
+def test_mixed_multinomial_rejects_large_counts():
+    X = np.zeros((100, 10), np.int32)
+    X[3, 4] = 300
+    dev = dp.DeviceSweep(X, dp.prior_type.Multinomial, seed=1)
+    with pytest.raises(dp.DPMMError):
+        dev.set_precision(dp.Precision.MIXED)
+
+
+def test_mixed_multinomial_sampler_end_to_end():
+    X, gt, _ = _docs(30_000, 200, 8, seed=3)
+    say("mixed multinomial end to end")
+    runs = {}
+    for prec in (dp.Precision.FP32, dp.Precision.MIXED):
+        dev = dp.DeviceSweep(X, dp.prior_type.Multinomial, seed=5)
+        s = dp.Sampler(dev, dp.multinomial_hyper(np.ones(200)), 10.0, 60, seed=5, precision=prec)
+        for t in range(1, 61):
+            s.step(t)
+        runs[prec] = dev.get_labels()[0]
+        print(prec, "K", s.num_clusters(), "nmi", dp._nmi(gt, runs[prec]))
+        s.close()
+        dev.close()
+    np.testing.assert_array_equal(runs[dp.Precision.MIXED], runs[dp.Precision.FP32])

@@ -290,19 +290,20 @@ __global__ void __launch_bounds__(kThreads) k_draw_params(const double* __restri
     __syncthreads();
     double* out = draws + (size_t)p * draw_stride;
     if (prior == DPMM_PRIOR_MULTINOMIAL) {
-        // Dirichlet(alpha0 + sum): D sequential gammas, normalised, logged
+        // Dirichlet(alpha0 + sum): coordinate d's gamma from the sub-stream at word 4096 d (block
+        // 1024 d), all coordinates in parallel; normalised in a fixed order, logged
         double* g = lds;
         for (int d = threadIdx.x; d < D; d += blockDim.x) {
             double s = 0.0;
             for (int t = 0; t < 4; ++t)
                 if (mem[t] >= 0) s += stats[(size_t)mem[t] * E + 1 + d];
-            g[d] = h.alpha0[d] + s;
+            DevStream rs(c0, iter, TAG_PARAM, key0, key1, 4096u * (uint32_t)d);
+            g[d] = rs.gamma(h.alpha0[d] + s);
         }
         __syncthreads();
-        if (threadIdx.x == 0) {
-            DevStream rs(c0, iter, TAG_PARAM, key0, key1, 0);
+        if (threadIdx.x == 0) {   // the oracle's summation order
             double s = 0.0;
-            for (int d = 0; d < D; ++d) s += (g[d] = rs.gamma(g[d]));
+            for (int d = 0; d < D; ++d) s += g[d];
             sh[0] = s;
         }
         __syncthreads();

@@ -1,0 +1,220 @@
+// C++ drop-in for the reference's public surface over the C ABI of dpmm_hip.h (header only, C++17):
+//   prior_type                      global_params.h:14-18
+//   hyperparams, niw_hyperparams    hyperparams.h, priors/niw_hyperparams.h:13, 40-43 (psi is Psi / v)
+//   multinomial_hyper               priors/multinomial_hyper.h:13, 29
+//   ModelInfo                       modelInfo.h:44-56 (labels are 1-based, as in the reference)
+//   dp_parallel_sampling_class      dp_parallel_sampling.h:11-44: the (numLabels, all_data, seed,
+//                                   prior) constructor and dp_parallel() with the reference's
+//                                   parameters and defaults.
+// all_data is D x N column-major (each point contiguous): any matrix type with data(), rows() (= D)
+// and cols() (= N), e.g. Eigen::MatrixXd, or a raw pointer with D.  Gaussian data of D <= 64 is
+// swept in FP64 like the reference; larger D, or set_precision(DPMM_PREC_FP32 / DPMM_PREC_MIXED),
+// sweeps an FP32 copy.  Multinomial data are counts, passed to the device as int32.
+//
+// Differences from the reference, each reported by an exception rather than ignored: the outlier
+// cluster (outlier_weight > 0) and save_model are not built; draw_labels is accepted and ignored
+// (the reference plots with OpenCV).
+#pragma once
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../dpmm_hip.h"
+
+namespace dpmm {
+
+enum prior_type { Gaussian = DPMM_PRIOR_GAUSSIAN, Multinomial = DPMM_PRIOR_MULTINOMIAL };
+typedef int LabelType;
+typedef std::vector<LabelType> LabelsType;
+
+struct hyperparams {
+    virtual ~hyperparams() = default;
+};
+
+struct niw_hyperparams : hyperparams {
+    double k = 0, v = 0;
+    std::vector<double> m, psi;   // psi row-major D x D (symmetric)
+    niw_hyperparams(double k_, std::vector<double> m_, double v_, std::vector<double> psi_)
+        : k(k_), v(v_), m(std::move(m_)), psi(std::move(psi_)) {
+        if (psi.size() != m.size() * m.size()) throw std::invalid_argument("niw_hyperparams: psi must be D x D");
+    }
+    // Eigen-style vector / matrix (data(), size(); data(), rows(), cols())
+    template <class V, class M, class = decltype(std::declval<const V&>().data()),
+              class = decltype(std::declval<const M&>().rows())>
+    niw_hyperparams(double k_, const V& m_, double v_, const M& psi_) : k(k_), v(v_) {
+        m.assign(m_.data(), m_.data() + m_.size());
+        psi.assign(psi_.data(), psi_.data() + (size_t)psi_.rows() * psi_.cols());
+        if (psi.size() != m.size() * m.size()) throw std::invalid_argument("niw_hyperparams: psi must be D x D");
+    }
+};
+
+struct multinomial_hyper : hyperparams {
+    std::vector<double> alpha;
+    explicit multinomial_hyper(std::vector<double> a) : alpha(std::move(a)) {}
+    template <class V, class = decltype(std::declval<const V&>().data())>
+    explicit multinomial_hyper(const V& a) : alpha(a.data(), a.data() + a.size()) {}
+};
+
+struct ModelInfo {
+    std::vector<double> iter_count;           // seconds per iteration
+    std::vector<double> nmi_score_history;    // per iteration when gt is given
+    std::vector<double> likelihood_history;   // not computed (empty)
+    std::vector<int> cluster_count_history;
+    std::shared_ptr<LabelsType> labels;       // 1-based
+    std::vector<double> weights;              // the model's mixture weights (dp_model->group.weights)
+};
+
+class dpmm_error : public std::runtime_error {
+  public:
+    dpmm_error(int code, const std::string& what) : std::runtime_error(what), code_(code) {}
+    int code() const { return code_; }
+
+  private:
+    int code_;
+};
+
+inline void check(int rc) {
+    if (rc != DPMM_OK) throw dpmm_error(rc, std::string("dpmm: ") + dpmm_last_error());
+}
+
+// Normalised mutual information 2 I(a;b) / (H(a) + H(b)), the reference's verbose NMI
+// (dp_parallel_sampling.cpp:170-178).
+inline double nmi(const LabelsType& a, const LabelsType& b) {
+    const size_t n = a.size();
+    if (n == 0 || b.size() != n) return 0.0;
+    std::unordered_map<long long, double> joint;
+    std::unordered_map<int, double> pa, pb;
+    for (size_t i = 0; i < n; ++i) {
+        joint[((long long)a[i] << 32) ^ (unsigned)b[i]] += 1.0;
+        pa[a[i]] += 1.0;
+        pb[b[i]] += 1.0;
+    }
+    double ha = 0, hb = 0, mi = 0;
+    for (auto& e : pa) ha -= e.second / n * std::log(e.second / n);
+    for (auto& e : pb) hb -= e.second / n * std::log(e.second / n);
+    for (auto& e : joint) {
+        const int x = (int)(e.first >> 32), y = (int)(unsigned)(e.first & 0xffffffffu);
+        const double p = e.second / n;
+        mi += p * std::log(p / (pa[x] / n * pb[y] / n));
+    }
+    return (ha + hb > 0) ? 2.0 * mi / (ha + hb) : 1.0;
+}
+
+class dp_parallel_sampling_class {
+  public:
+    template <class Matrix, class = decltype(std::declval<Matrix&>().rows())>
+    dp_parallel_sampling_class(int numLabels, Matrix& all_data, unsigned long long randomSeed, prior_type priorType)
+        : dp_parallel_sampling_class(numLabels, all_data.data(), (int)all_data.rows(), randomSeed, priorType) {
+        if ((long long)all_data.cols() != (long long)numLabels)
+            throw std::invalid_argument("all_data must be D x numLabels");
+    }
+
+    dp_parallel_sampling_class(int numLabels, const double* all_data, int D, unsigned long long randomSeed,
+                               prior_type priorType)
+        : N_(numLabels), D_(D), seed_(randomSeed), prior_(priorType) {
+        if (!all_data || numLabels <= 0 || D <= 0) throw std::invalid_argument("empty data");
+        const size_t n = (size_t)numLabels * D;
+        if (prior_ == Multinomial) {
+            counts_.resize(n);
+            for (size_t i = 0; i < n; ++i) counts_[i] = (int32_t)std::lround(all_data[i]);
+        } else {
+            data64_.assign(all_data, all_data + n);
+        }
+        precision_ = (prior_ == Gaussian && D_ <= 64) ? DPMM_PREC_FP64 : DPMM_PREC_FP32;
+    }
+
+    dp_parallel_sampling_class(const dp_parallel_sampling_class&) = delete;
+    dp_parallel_sampling_class& operator=(const dp_parallel_sampling_class&) = delete;
+    ~dp_parallel_sampling_class() = default;
+
+    // DPMM_PREC_FP64 (Gaussian, D <= 64), DPMM_PREC_FP32 or DPMM_PREC_MIXED for the label sweep.
+    void set_precision(int p) { precision_ = p; }
+
+    ModelInfo dp_parallel(std::shared_ptr<hyperparams>& local_hyper_params, double alpha_param, int iters = 100,
+                          int init_clusters = 1, bool verbose = true, bool draw_labels = false,
+                          bool save_model = false, int burnout = 15, std::shared_ptr<LabelsType> gt = nullptr,
+                          double max_clusters = DBL_MAX, double outlier_weight = 0,
+                          std::shared_ptr<hyperparams> outlier_params = nullptr) {
+        (void)draw_labels;
+        if (outlier_weight > 0 || outlier_params) throw std::invalid_argument("dpmm: the outlier cluster is not built");
+        if (save_model) throw std::invalid_argument("dpmm: save_model is not built");
+        dpmm_ctx* ctx = nullptr;
+        if (prior_ == Multinomial) {
+            check(dpmm_create(counts_.data(), N_, D_, DPMM_DTYPE_I32, DPMM_PRIOR_MULTINOMIAL, seed_, &ctx));
+        } else if (precision_ == DPMM_PREC_FP64) {
+            check(dpmm_create(data64_.data(), N_, D_, DPMM_DTYPE_F64, DPMM_PRIOR_GAUSSIAN, seed_, &ctx));
+        } else {
+            std::vector<float> f(data64_.begin(), data64_.end());
+            check(dpmm_create(f.data(), N_, D_, DPMM_DTYPE_F32, DPMM_PRIOR_GAUSSIAN, seed_, &ctx));
+        }
+        std::unique_ptr<dpmm_ctx, int (*)(dpmm_ctx*)> ctx_guard(ctx, dpmm_destroy);
+        dpmm_options o{};
+        o.prior = prior_;
+        o.alpha = alpha_param;
+        o.iters = iters;
+        o.init_clusters = init_clusters;
+        o.burnout = burnout;
+        o.argmax_sample_stop = 5;
+        o.split_stop = 5;
+        o.max_clusters = max_clusters;
+        o.seed = seed_;
+        o.full_mu_cov = 0;
+        o.mult_use_weight = 0;
+        o.precision = precision_;
+        o.verbose = 0;
+        dpmm_sampler* s = nullptr;
+        if (prior_ == Gaussian) {
+            auto* h = dynamic_cast<niw_hyperparams*>(local_hyper_params.get());
+            if (!h || (int)h->m.size() != D_) throw std::invalid_argument("dpmm: Gaussian prior needs niw_hyperparams of dimension D");
+            check(dpmm_sampler_create(ctx, &o, h->k, h->m.data(), h->v, h->psi.data(), nullptr, &s));
+        } else {
+            auto* h = dynamic_cast<multinomial_hyper*>(local_hyper_params.get());
+            if (!h || (int)h->alpha.size() != D_) throw std::invalid_argument("dpmm: multinomial prior needs multinomial_hyper of dimension D");
+            check(dpmm_sampler_create(ctx, &o, 0.0, nullptr, 0.0, nullptr, h->alpha.data(), &s));
+        }
+        std::unique_ptr<dpmm_sampler, int (*)(dpmm_sampler*)> s_guard(s, dpmm_sampler_destroy);
+        ModelInfo info;
+        LabelsType lab(N_);
+        for (int t = 1; t <= iters; ++t) {
+            const auto t0 = std::chrono::steady_clock::now();
+            check(dpmm_sampler_step(s, t));
+            const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            const int K = dpmm_sampler_num_clusters(s);
+            info.iter_count.push_back(dt);
+            info.cluster_count_history.push_back(K);
+            double score = 0.0;
+            if (gt) {
+                check(dpmm_get_labels(ctx, lab.data(), nullptr));
+                for (auto& l : lab) l += 1;
+                score = nmi(*gt, lab);
+                info.nmi_score_history.push_back(score);
+            }
+            if (verbose)
+                std::printf("Iteration: %d || Clusters count: %d || NMI: %f || Iter Time: %f\n", t, K, score, dt);
+        }
+        check(dpmm_get_labels(ctx, lab.data(), nullptr));
+        for (auto& l : lab) l += 1;   // the reference's labels are 1-based
+        info.labels = std::make_shared<LabelsType>(std::move(lab));
+        const int K = dpmm_sampler_num_clusters(s);
+        info.weights.resize(K);
+        check(dpmm_sampler_weights(s, info.weights.data(), nullptr, K));
+        return info;
+    }
+
+  private:
+    int N_, D_;
+    unsigned long long seed_;
+    prior_type prior_;
+    int precision_;
+    std::vector<double> data64_;
+    std::vector<int32_t> counts_;
+};
+
+}  // namespace dpmm

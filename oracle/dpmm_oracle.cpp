@@ -21,7 +21,9 @@
  *   reset relabel     (i, 0, t, 4)      word0 & 1 -> sub-label
  *   split relabel     (i, 0, t, 5)      word0 & 1 -> sub-label
  *   sequential streams (c0, n, t, tag) with n = 0, 1, 2, ... (4 words per n):
- *     tag 6 parameter draw of slot 3*cluster + {0 cluster, 1 left, 2 right}
+ *     tag 6 parameter draw of slot 3*cluster + {0 cluster, 1 left, 2 right}; a Dirichlet
+ *           (multinomial) draw gives coordinate d the stream starting at n = 1024 d, so the
+ *           D gammas are independent of each other's rejection counts
  *     tag 7 left/right weights of a cluster, tag 8 mixture weights (c0 = 0),
  *     tag 9 split-test uniform (c0 = cluster), tag 10 merge-test uniform (c0 = i<<16 | j).
  * uniform(w) = (w + 0.5) 2^-32; normal = Box-Muller cos branch of two uniforms;
@@ -100,7 +102,8 @@ struct Stream {
     uint32_t c0, c1, c2, c3;
     uint32_t buf[4];
     int pos;
-    Stream(uint64_t seed, uint32_t a, uint32_t t, uint32_t tag) : c0(a), c1(0), c2(t), c3(tag), pos(4) {
+    Stream(uint64_t seed, uint32_t a, uint32_t t, uint32_t tag, uint32_t first_block = 0)
+        : c0(a), c1(first_block), c2(t), c3(tag), pos(4) {
         key_of(seed, key);
     }
     uint32_t next() {
@@ -353,15 +356,27 @@ bool niw_draw(int D, const Hyper& h, Stream& rs, int full_mu_cov, Dist& out) {
 }
 
 /* multinomial_prior::sample_distribution (priors/multinomial_prior.cpp:24-49). */
-void mult_draw(int D, const Hyper& h, Stream& rs, Dist& out) {
+/* Dirichlet(alpha) with coordinate d's gamma from the sub-stream starting at block 1024 d. */
+void dirichlet_substreams(uint64_t seed, uint32_t c0, uint32_t t, uint32_t tag, int D, const double* alpha,
+                          double* out) {
+    double s = 0.0;
+    for (int d = 0; d < D; ++d) {
+        Stream rs(seed, c0, t, tag, (uint32_t)d * 1024u);
+        out[d] = rs.gamma(alpha[d]);
+        s += out[d];
+    }
+    for (int d = 0; d < D; ++d) out[d] /= s;
+}
+
+void mult_draw(int D, const Hyper& h, uint64_t seed, uint32_t slot, uint32_t t, Dist& out) {
     out.logtheta.resize(D);
-    rs.dirichlet(D, h.alpha.data(), out.logtheta.data());
+    dirichlet_substreams(seed, slot, t, TAG_PARAM, D, h.alpha.data(), out.logtheta.data());
     for (int d = 0; d < D; ++d) out.logtheta[d] = std::log(out.logtheta[d]);
 }
 
 bool draw_dist(const Model& M, const Hyper& h, uint32_t slot, uint32_t t, Dist& out) {
+    if (M.prior == 1) { mult_draw(M.D, h, M.seed, slot, t, out); return true; }
     Stream rs(M.seed, slot, t, TAG_PARAM);
-    if (M.prior == 1) { mult_draw(M.D, h, rs, out); return true; }
     return niw_draw(M.D, h, rs, M.full_mu_cov, out);
 }
 
@@ -752,6 +767,11 @@ void orc_stream_dirichlet(uint64_t seed, uint32_t c0, uint32_t c2, uint32_t c3, 
                           double* out) {
     Stream s(seed, c0, c2, c3);
     s.dirichlet(n, alpha, out);
+}
+
+void orc_dirichlet_substreams(uint64_t seed, uint32_t c0, uint32_t c2, uint32_t c3, int n, const double* alpha,
+                              double* out) {
+    dirichlet_substreams(seed, c0, c2, c3, n, alpha, out);
 }
 
 double orc_lgamma(double x) { return std::lgamma(x); }

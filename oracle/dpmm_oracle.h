@@ -28,6 +28,10 @@ void orc_stream_uniforms(uint64_t seed, uint32_t c0, uint32_t c2, uint32_t c3, i
 void orc_stream_normals(uint64_t seed, uint32_t c0, uint32_t c2, uint32_t c3, int n, double* out);
 void orc_stream_gammas(uint64_t seed, uint32_t c0, uint32_t c2, uint32_t c3, double shape, int n, double* out);
 void orc_stream_dirichlet(uint64_t seed, uint32_t c0, uint32_t c2, uint32_t c3, int n, const double* alpha, double* out);
+/* Dirichlet draw of the multinomial parameter draws: coordinate d's gamma from the stream
+ * (c0, n, c2, c3) starting at n = 1024 d. */
+void orc_dirichlet_substreams(uint64_t seed, uint32_t c0, uint32_t c2, uint32_t c3, int n, const double* alpha,
+                              double* out);
 
 double orc_lgamma(double x);
 double orc_log_multivariate_gamma(double x, int D);

@@ -83,6 +83,9 @@ def stream(kind, seed, c0, c2, c3, n, shape=None, alpha=None):
     elif kind == "dirichlet":
         a = f64(alpha)
         L.orc_stream_dirichlet(c_u64(seed), c_u32(c0), c_u32(c2), c_u32(c3), c_int(n), _p(a), _p(out))
+    elif kind == "dirichlet_substreams":
+        a = f64(alpha)
+        L.orc_dirichlet_substreams(c_u64(seed), c_u32(c0), c_u32(c2), c_u32(c3), c_int(n), _p(a), _p(out))
     return out
 
 

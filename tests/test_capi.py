@@ -40,3 +40,13 @@ def test_errors_are_codes_not_exits():
 def test_python_mirror_validates_before_device():
     with pytest.raises(ValueError):
         dp.dp_parallel_sampling_class(5, [[1.0, 2.0]], 0, dp.prior_type.Gaussian)
+
+
+def test_dropin_header_compiles():
+    """The C++ drop-in header is self-contained C++17 over the C ABI header."""
+    import subprocess
+    root = os.path.join(os.path.dirname(__file__), "..")
+    src = "#include \"dpmm/dp_parallel_sampling.hpp\"\nint main() { dpmm::niw_hyperparams h(1.0, std::vector<double>{0.0}, 3.0, std::vector<double>{1.0}); return h.m.size() == 1 ? 0 : 1; }\n"
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Werror", "-I", os.path.join(root, "include"),
+                        "-x", "c++", "-"], input=src, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr

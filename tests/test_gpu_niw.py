@@ -197,7 +197,7 @@ def test_multinomial_draws_vs_oracle_stream():
     lt, _, _ = dev.drawn_params(K)
     Nsub, s, _, _, _, _ = orc.suff_stats(1, X, labels, subs, K, want_S=False)
     for p, (n, su, _) in enumerate(_sum(mem, Nsub, s, None)):
-        want = np.log(orc.stream("dirichlet", 21, p, 5, 6, D, alpha=a0 + su))
+        want = np.log(orc.stream("dirichlet_substreams", 21, p, 5, 6, D, alpha=a0 + su))
         np.testing.assert_allclose(lt[p], want, rtol=1e-10, atol=1e-10)
 
 

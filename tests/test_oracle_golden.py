@@ -368,3 +368,15 @@ def test_end_to_end_multinomial():
     out = orc.run(X, prior=1, alpha=10.0, iters=60, seed=3, alpha0=np.ones(D))
     assert out["K"] == 3
     assert orc.nmi(gt, out["labels"]) > 0.95
+
+
+def test_dirichlet_substreams_moments():
+    """Multinomial parameter draws use one sub-stream per coordinate; the draws are Dirichlet(alpha):
+    means alpha / sum, variances a_d (A - a_d) / (A^2 (A + 1))."""
+    alpha = np.array([0.5, 1.0, 2.0, 4.0, 8.5])
+    A = alpha.sum()
+    draws = np.stack([orc.stream("dirichlet_substreams", 3, c0, 1, 6, len(alpha), alpha=alpha) for c0 in range(4000)])
+    np.testing.assert_allclose(draws.sum(1), 1.0, rtol=1e-12)
+    mean, var = alpha / A, alpha * (A - alpha) / (A * A * (A + 1))
+    assert np.all(np.abs(draws.mean(0) - mean) < 4 * np.sqrt(var / len(draws)))
+    np.testing.assert_allclose(draws.var(0), var, rtol=0.15)

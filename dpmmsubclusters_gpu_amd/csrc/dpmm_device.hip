@@ -404,6 +404,10 @@ struct dpmm_ctx {
     void* dPB = nullptr;
     size_t pbCap = 0;
     std::vector<unsigned char> hPB;
+    void* dSPB = nullptr;   // the 2K sub-clusters' BF16 slots
+    size_t spbCap = 0;
+    std::vector<unsigned char> hSPB;
+    bool spb_valid = false;
     int32_t* dRecheck = nullptr;
     int* dRecheckCount = nullptr;
     float tau = 1.0f / 32.0f;
@@ -636,11 +640,15 @@ int ensure_recheck(dpmm_ctx* c) {
     return DPMM_OK;
 }
 
-// BF16 screen slots for K clusters and the recheck list.
+// BF16 screen slots for K clusters and 2K sub-clusters, and the recheck list.
 int ensure_mixed_buffers(dpmm_ctx* c, int K) {
     unsigned char* pb = reinterpret_cast<unsigned char*>(c->dPB);
     int rc = grow(pb, c->pbCap, (size_t)K * mx_slot_bytes(c->DPM));
     c->dPB = pb;
+    if (rc) return rc;
+    unsigned char* spb = reinterpret_cast<unsigned char*>(c->dSPB);
+    rc = grow(spb, c->spbCap, (size_t)2 * K * mx2_slot_bytes(c->DPM));
+    c->dSPB = spb;
     if (rc) return rc;
     return ensure_recheck(c);
 }
@@ -660,6 +668,27 @@ int pack_mult_mixed(dpmm_ctx* c, int K, const double* src, size_t row_stride, co
     HIPCHK(hipGetLastError());
     c->mm_ncb = ncb;
     return DPMM_OK;
+}
+
+// Split-BF16 slot of one sub-cluster for the sub-label screen: hi and lo tiles of W = R^T (of the
+// FP32 value the FP32 sweep uses), mu, score constant.
+void pack_mx2_slot(int D, int DPM, const double* mu, const double* R, double cst, unsigned char* dst) {
+    std::memset(dst, 0, mx2_slot_bytes(DPM));
+    uint16_t* hi = reinterpret_cast<uint16_t*>(dst);
+    uint16_t* lo = reinterpret_cast<uint16_t*>(dst + mx_tiles_bytes(DPM));
+    for (int j = 0; j < D; ++j)
+        for (int l = j; l < D; ++l) {
+            const float w = (float)R[(size_t)l * D + j];
+            const uint16_t h = f32_to_bf16_rne(w);
+            uint32_t hb = (uint32_t)h << 16;
+            float hf;
+            std::memcpy(&hf, &hb, 4);
+            hi[mx_tile_index(DPM, j, l)] = h;
+            lo[mx_tile_index(DPM, j, l)] = f32_to_bf16_rne(w - hf);
+        }
+    float* m = reinterpret_cast<float*>(dst + 2 * mx_tiles_bytes(DPM));
+    for (int l = 0; l < D; ++l) m[l] = (float)mu[l];
+    m[DPM] = (float)cst;
 }
 
 int set_gauss(dpmm_ctx* c, int K, const double* mu, const double* mat, const double* logdet, const double* weights,
@@ -691,7 +720,11 @@ int set_gauss(dpmm_ctx* c, int K, const double* mu, const double* mat, const dou
     };
     const bool mixed = c->precision == DPMM_PREC_MIXED;
     const int mslot = mx_slot_bytes(c->DPM);
-    if (mixed) c->hPB.assign((size_t)K * mslot, 0);
+    c->spb_valid = false;
+    if (mixed) {
+        c->hPB.assign((size_t)K * mslot, 0);
+        c->hSPB.assign((size_t)2 * K * mx2_slot_bytes(c->DPM), 0);
+    }
     for (int k = 0; k < K; ++k) {
         if (!std::isfinite(logdet[k])) return fail(DPMM_EINVAL, "non-finite logdet");
         if (!one(mu + (size_t)k * D, mat + (size_t)k * D * D, logdet[k], std::log(weights[k]),
@@ -710,6 +743,13 @@ int set_gauss(dpmm_ctx* c, int K, const double* mu, const double* mat, const dou
         if (!one(smu + (size_t)q * D, smat + (size_t)q * D * D, slogdet[q], std::log(lrw[q]),
                  c->hSP.data() + (size_t)q * stride * esz))
             return fail(DPMM_ENOTSPD, "sub-cluster precision matrix not SPD");
+        if (mixed)
+            pack_mx2_slot(D, c->DPM, smu + (size_t)q * D, R.data(), -0.5 * slogdet[q] + std::log(lrw[q]),
+                          c->hSPB.data() + (size_t)q * mx2_slot_bytes(c->DPM));
+    }
+    if (mixed) {
+        HIPCHK(hipMemcpyAsync(c->dSPB, c->hSPB.data(), c->hSPB.size(), hipMemcpyHostToDevice, c->stream));
+        c->spb_valid = true;
     }
     HIPCHK(hipMemcpyAsync(c->dP, c->hP.data(), c->hP.size(), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->dSP, c->hSP.data(), c->hSP.size(), hipMemcpyHostToDevice, c->stream));
@@ -773,6 +813,32 @@ template <int DP> void lgl_f32(dpmm_ctx* c, uint32_t it, int f, int grid) {
     launch_label_gauss_list<float, DP>(c->stream, grid, reinterpret_cast<const float*>(c->dX), c->N, c->D, c->K,
                                        reinterpret_cast<const float*>(c->dP), c->key0(), c->key1(), it, f,
                                        c->dRecheck, c->dRecheckCount, c->dLabels);
+}
+
+template <int DP> void sgl_f32(dpmm_ctx* c, uint32_t it, int grid) {
+    launch_sub_gauss_list<float, DP>(c->stream, grid, reinterpret_cast<const float*>(c->dX), c->D,
+                                     reinterpret_cast<const float*>(c->dSP), c->dLabels, c->dRecheck,
+                                     c->dRecheckCount, c->key0(), c->key1(), it, c->dSubs);
+}
+template <int DPM> void sgm(dpmm_ctx* c, uint32_t it) {
+    launch_sub_gauss_mx<DPM>(c->stream, chunks_upper(c, c->K), reinterpret_cast<const float*>(c->dX), c->D, c->K,
+                             c->CH, c->dPerm, c->dLabelStart, c->dChunkStart,
+                             reinterpret_cast<const unsigned char*>(c->dSPB), c->tau, c->key0(), c->key1(), it,
+                             c->dSubs, c->dRecheck, c->dRecheckCount);
+}
+
+// BF16 sub-label screen over the segments, then the FP32 draw of the points it could not decide.
+int run_sub_mixed(dpmm_ctx* c, uint32_t iter) {
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, c->dRecheckCount, 1);
+    switch (c->DPM) {
+        case 32: sgm<32>(c, iter); break;
+        case 64: sgm<64>(c, iter); break;
+        case 96: sgm<96>(c, iter); break;
+        default: sgm<128>(c, iter); break;
+    }
+    const int grid = (int)std::min<int64_t>(1024, (c->N + kThreads - 1) / kThreads);
+    DPMM_DISPATCH_DP(c->DP, sgl_f32, c, iter, grid);
+    return DPMM_OK;
 }
 
 // BF16 screen over all points, then the FP32 sweep over the points it could not decide.
@@ -876,7 +942,7 @@ int dpmm_destroy(dpmm_ctx* c) {
                     (void*)c->dRowtot, (void*)c->dLabelStart, (void*)c->dChunkStart, (void*)c->dTable,
                     (void*)c->dPartials, (void*)c->dStats, c->dPB, (void*)c->dRecheck, (void*)c->dRecheckCount,
                     (void*)c->dPrior, (void*)c->dMembers, (void*)c->dLM, (void*)c->dDraws, (void*)c->dDrawStatus,
-                    (void*)c->dLogW, (void*)c->dFeat, (void*)c->dMB})
+                    (void*)c->dLogW, (void*)c->dFeat, (void*)c->dMB, c->dSPB})
         if (p) (void)hipFree(p);
     if (c->hStats) (void)hipHostFree(c->hStats);
     for (auto& e : c->ev)
@@ -1031,7 +1097,10 @@ int dpmm_sample_sub_labels(dpmm_ctx* c, uint32_t iter) {
     }
     (void)hipEventRecord(c->ev[2], c->stream);
     c->stats_cached = false;
-    if (c->prior == DPMM_PRIOR_GAUSSIAN) {
+    if (c->prior == DPMM_PRIOR_GAUSSIAN && c->precision == DPMM_PREC_MIXED && c->spb_valid) {
+        int rc = run_sub_mixed(c, iter);
+        if (rc) return rc;
+    } else if (c->prior == DPMM_PRIOR_GAUSSIAN) {
         if (c->params_f64) { DPMM_DISPATCH_DP64(c->DP, sg_f64, c, iter); }
         else { DPMM_DISPATCH_DP(c->DP, sg_f32, c, iter); }
     } else {
@@ -1315,18 +1384,22 @@ int dpmm_sample_params(dpmm_ctx* c, uint32_t iter, int K, const int32_t* members
         c->stride = stride;
         if ((rc = ensure_param_buffers(c, K, (size_t)stride * esz))) return rc;
         unsigned char* pb = nullptr;
+        unsigned char* spb = nullptr;
+        c->spb_valid = false;
         if (c->precision == DPMM_PREC_MIXED) {
             if ((rc = ensure_mixed_buffers(c, K))) return rc;
             pb = reinterpret_cast<unsigned char*>(c->dPB);
+            spb = reinterpret_cast<unsigned char*>(c->dSPB);
+            c->spb_valid = true;
         }
         if (f64)
             hipLaunchKernelGGL(k_pack_gauss<double>, dim3(np), dim3(kThreads), 0, c->stream, c->dDraws, dstride, D,
                                c->DP, c->DPM, c->dLogW, K, reinterpret_cast<double*>(c->dP),
-                               reinterpret_cast<double*>(c->dSP), pb);
+                               reinterpret_cast<double*>(c->dSP), pb, spb);
         else
             hipLaunchKernelGGL(k_pack_gauss<float>, dim3(np), dim3(kThreads), 0, c->stream, c->dDraws, dstride, D,
                                c->DP, c->DPM, c->dLogW, K, reinterpret_cast<float*>(c->dP),
-                               reinterpret_cast<float*>(c->dSP), pb);
+                               reinterpret_cast<float*>(c->dSP), pb, spb);
     } else {
         const int stride = (D + 1 + 3) & ~3;
         c->params_f64 = false;

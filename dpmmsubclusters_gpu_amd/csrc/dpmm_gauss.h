@@ -9,20 +9,6 @@
 
 namespace dpmm {
 
-__device__ __forceinline__ bool chunk_of(int c, int K, const int32_t* __restrict__ chunk_start,
-                                         const int32_t* __restrict__ label_start, int CH, int& k, int& beg, int& end) {
-    if (c >= chunk_start[K]) return false;
-    int lo = 0, hi = K - 1;   // last k with chunk_start[k] <= c
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (chunk_start[mid] <= c) lo = mid; else hi = mid - 1;
-    }
-    k = lo;
-    beg = label_start[k] + (c - chunk_start[k]) * CH;
-    end = min(beg + CH, label_start[k + 1]);
-    return true;
-}
-
 // Gaussian label sweep of one point per thread (or per lane pair for wide points), the point in
 // registers; cluster parameters staged through LDS one cluster at a time.  Every thread of the
 // workgroup must call it (it synchronises); threads without a point pass i >= N.
@@ -288,6 +274,51 @@ __global__ void __launch_bounds__(kThreads) k_stats_gauss_mfma(const TX* __restr
     }
 }
 
+// Exact sub-label draw of listed points (the undecided points of the mixed sub-label screen), with
+// the arithmetic of k_sub_gauss: each point's two sub-cluster slots (of its label) are read from
+// global memory; workgroups stride over the list, whose length is read from device memory.
+template <typename T, int DP>
+__global__ void __launch_bounds__(kThreads) k_sub_gauss_list(const T* __restrict__ X, int D,
+                                                             const T* __restrict__ SP,
+                                                             const int32_t* __restrict__ labels,
+                                                             const int32_t* __restrict__ list,
+                                                             const int* __restrict__ list_count, uint32_t key0,
+                                                             uint32_t key1, uint32_t iter,
+                                                             int32_t* __restrict__ subs) {
+    constexpr bool kPair = paired_layout<T, DP>();
+    constexpr int kPts = kPair ? kThreads / 2 : kThreads;
+    constexpr int kCols = kPair ? DP / 2 : DP;
+    constexpr int stride = gauss_slot_stride(DP);
+    const int par = kPair ? (threadIdx.x & 1) : 0;
+    const int local = kPair ? (threadIdx.x >> 1) : threadIdx.x;
+    const int n = *list_count;
+    for (int b0 = blockIdx.x * kPts; b0 < n; b0 += gridDim.x * kPts) {   // uniform trip count per pair
+        const int pos = b0 + local;
+        const bool valid = pos < n;
+        const int idx = valid ? list[pos] : 0;
+        const int k = valid ? labels[idx] : 0;
+        T x[kCols];
+#pragma unroll
+        for (int c = 0; c < kCols; ++c) {
+            const int l = kPair ? pair_col(c, 0) + (par << 2) : c;
+            x[c] = (valid && l < D) ? X[(int64_t)idx * D + l] : T(0);
+        }
+        T sc[2];
+#pragma unroll 1
+        for (int h = 0; h < 2; ++h) {
+            const T* pp = SP + (size_t)(2 * k + h) * stride;
+            T q;
+            if constexpr (kPair) q = whitened_sq_pair<T, DP>(pp, x, par);
+            else q = whitened_sq_regs<T, DP>(pp, x);
+            sc[h] = pp[gauss_c_offset(DP)] - T(0.5) * q;
+        }
+        if (valid && par == 0) {
+            const uint32_t w = philox4x32_10((uint32_t)idx, 0, iter, TAG_SUB, key0, key1).v[0];
+            subs[idx] = (logit_u32<T>(w) < sc[0] - sc[1]) ? 0 : 1;
+        }
+    }
+}
+
 // Partial (count, sum, scatter) of the points of sub-cluster s in one chunk, FP64.  Threads own
 // 4x4 blocks of the upper block triangle; points staged in LDS in batches, compacted by ballot.
 template <typename TX, int DP>
@@ -401,6 +432,14 @@ void launch_label_gauss_list(hipStream_t st, int grid, const T* X, int64_t N, in
 }
 
 template <typename T, int DP>
+void launch_sub_gauss_list(hipStream_t st, int grid, const T* X, int D, const T* SP, const int32_t* labels,
+                           const int32_t* list, const int* count, uint32_t key0, uint32_t key1, uint32_t iter,
+                           int32_t* subs) {
+    hipLaunchKernelGGL((k_sub_gauss_list<T, DP>), dim3(grid), dim3(kThreads), 0, st, X, D, SP, labels, list, count,
+                       key0, key1, iter, subs);
+}
+
+template <typename T, int DP>
 void launch_sub_gauss(hipStream_t st, int grid, const T* X, int D, int K, int CH, const int32_t* perm,
                       const int32_t* label_start, const int32_t* chunk_start, const T* SP, uint32_t key0,
                       uint32_t key1, uint32_t iter, int32_t* subs) {
@@ -432,7 +471,10 @@ void launch_stats_gauss(hipStream_t st, int nchunks, const TX* X, int D, int K, 
                                           const int32_t*, const int32_t*, const T*, uint32_t, uint32_t,        \
                                           uint32_t, int32_t*);                                                 \
     template void launch_stats_gauss<T, DP>(hipStream_t, int, const T*, int, int, int, const int32_t*,         \
-                                            const int32_t*, const int32_t*, const int32_t*, double*);
+                                            const int32_t*, const int32_t*, const int32_t*, double*);          \
+    template void launch_sub_gauss_list<T, DP>(hipStream_t, int, const T*, int, const T*, const int32_t*,      \
+                                               const int32_t*, const int*, uint32_t, uint32_t, uint32_t,       \
+                                               int32_t*);
 
 #define DPMM_GAUSS_EXTERN(T, DP)                                                                              \
     extern template void launch_label_gauss<T, DP>(hipStream_t, int, const T*, int64_t, int, int, const T*,    \
@@ -444,6 +486,9 @@ void launch_stats_gauss(hipStream_t st, int nchunks, const TX* X, int D, int K, 
                                                  const int32_t*, const int32_t*, const T*, uint32_t, uint32_t, \
                                                  uint32_t, int32_t*);                                          \
     extern template void launch_stats_gauss<T, DP>(hipStream_t, int, const T*, int, int, int, const int32_t*,  \
-                                                   const int32_t*, const int32_t*, const int32_t*, double*);
+                                                   const int32_t*, const int32_t*, const int32_t*, double*);   \
+    extern template void launch_sub_gauss_list<T, DP>(hipStream_t, int, const T*, int, const T*,               \
+                                                      const int32_t*, const int32_t*, const int*, uint32_t,    \
+                                                      uint32_t, uint32_t, int32_t*);
 
 }  // namespace dpmm

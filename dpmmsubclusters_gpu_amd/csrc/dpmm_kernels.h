@@ -140,4 +140,20 @@ __device__ __forceinline__ T whitened_sq(const T* __restrict__ ps, const T (&x)[
     return whitened_sq_regs<T, DP>(ps, x);
 }
 
+// Chunk c of the partition: the cluster k whose segment holds it and its point range [beg, end)
+// in perm.  False past the last chunk.
+__device__ __forceinline__ bool chunk_of(int c, int K, const int32_t* __restrict__ chunk_start,
+                                         const int32_t* __restrict__ label_start, int CH, int& k, int& beg, int& end) {
+    if (c >= chunk_start[K]) return false;
+    int lo = 0, hi = K - 1;   // last k with chunk_start[k] <= c
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (chunk_start[mid] <= c) lo = mid; else hi = mid - 1;
+    }
+    k = lo;
+    beg = label_start[k] + (c - chunk_start[k]) * CH;
+    end = min(beg + CH, label_start[k + 1]);
+    return true;
+}
+
 }  // namespace dpmm

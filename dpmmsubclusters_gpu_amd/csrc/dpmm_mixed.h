@@ -26,6 +26,8 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 // Slot of one cluster for the screen: [tiles RB x KC x 64 lanes x 8 bf16 | mu (DP f32) | c | pad].
 __host__ __device__ constexpr int mx_tiles_bytes(int DP) { return (DP / 32) * (DP / 16) * 64 * 16; }
 __host__ __device__ constexpr int mx_slot_bytes(int DP) { return (mx_tiles_bytes(DP) + DP * 4 + 4 + 15) & ~15; }
+// Sub-cluster slot of the sub-label screen: W split as hi + lo tiles, then mu (DP f32), c, pad.
+__host__ __device__ constexpr int mx2_slot_bytes(int DP) { return (2 * mx_tiles_bytes(DP) + DP * 4 + 4 + 15) & ~15; }
 // Lane-order index of W[j][l] inside the tile array (row j, column l).
 __host__ __device__ constexpr int mx_tile_index(int DP, int j, int l) {
     // tile (rb, kc); lane = (j % 32) + 32 * ((l % 16) / 8); element e = l % 8
@@ -152,12 +154,136 @@ void launch_label_gauss_mx(hipStream_t st, int64_t N, int D, int K, const float*
                        iter, fin, labels, recheck, recheck_count);
 }
 
+// Sub-label screen of one chunk (the points of cluster k): both sub-clusters' split-BF16 slots in
+// LDS, 32 gathered points per wave.  With d = x - mu_h in FP32 split as d_hi + d_lo and W = W_hi + W_lo,
+// y = W_hi d_hi + W_hi d_lo + W_lo d_hi on MFMA (FP32 accumulation), and a fourth MFMA gives
+// a_j = sum_l |W_hi,jl| |d_hi,l|.  Each y_j is then within eps a_j of the FP32 sweep's (eps covers the
+// split residuals and both FP32 accumulations), so q_h is within sum_j (2 |y_j| + eps a_j) eps a_j of
+// the FP32 sweep's q_h.  The exact draw sets sub-label 0 when logit(u) < S_0 - S_1 (k_sub_gauss); a
+// point whose threshold is farther than that bound from s_0 - s_1 gets its sub-label here, the others
+// go to the recheck list.
+template <int DP>
+__global__ void __launch_bounds__(kThreads) k_sub_gauss_mx(const float* __restrict__ X, int D, int K, int CH,
+                                                           const int32_t* __restrict__ perm,
+                                                           const int32_t* __restrict__ label_start,
+                                                           const int32_t* __restrict__ chunk_start,
+                                                           const unsigned char* __restrict__ SPB, float tau,
+                                                           uint32_t key0, uint32_t key1, uint32_t iter,
+                                                           int32_t* __restrict__ subs, int32_t* __restrict__ recheck,
+                                                           int* __restrict__ recheck_count) {
+    typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+    constexpr int RB = DP / 32, KC = DP / 16;
+    constexpr int SLOT = mx2_slot_bytes(DP), TB = mx_tiles_bytes(DP);
+    __shared__ __align__(16) unsigned char sp[2 * SLOT];
+    int k, beg, end;
+    if (!chunk_of(blockIdx.x, K, chunk_start, label_start, CH, k, beg, end)) return;
+    {
+        const uint4* src = reinterpret_cast<const uint4*>(SPB + (size_t)(2 * k) * SLOT);
+        uint4* dst = reinterpret_cast<uint4*>(sp);
+        for (int e = threadIdx.x; e < 2 * SLOT / 16; e += kThreads) dst[e] = src[e];
+    }
+    __syncthreads();
+    (void)tau;
+    // |y_screen - y_fp32| <= eps a_j: split residuals (3 x 2^-18 of |W||d|) and the FP32 sums of the
+    // screen (4 D terms) and of the sweep (D terms), with a_j from BF16 magnitudes inflated by 2^-6
+    const float eps = (3.0f * 0x1p-18f + (5.0f * (float)DP + 16.0f) * 0x1p-24f) * (1.0f + 0x1p-6f);
+    const int lane = threadIdx.x & 63, h = lane >> 5, pl = lane & 31, wv = threadIdx.x >> 6;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int p0 = beg + 32 * wv; p0 < end; p0 += 4 * 32) {   // wave-uniform trip count
+        const int p = p0 + pl;
+        const bool valid = p < end;
+        const int idx = valid ? perm[p] : 0;
+        float x[KC][8];
+#pragma unroll
+        for (int kc = 0; kc < KC; ++kc)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int l = 16 * kc + 8 * h + e;
+                x[kc][e] = (valid && l < D) ? X[(int64_t)idx * D + l] : 0.f;
+            }
+        float s0 = 0.f, s1 = 0.f, dq0 = 0.f, dq1 = 0.f;
+#pragma unroll 1
+        for (int hh = 0; hh < 2; ++hh) {
+            const unsigned char* slot = sp + hh * SLOT;
+            const bf16x8* thi = reinterpret_cast<const bf16x8*>(slot);
+            const bf16x8* tlo = reinterpret_cast<const bf16x8*>(slot + TB);
+            const float* mu = reinterpret_cast<const float*>(slot + 2 * TB);
+            float qq = 0.f, ee = 0.f;
+#pragma unroll
+            for (int rb = 0; rb < RB; ++rb) {
+                f32x16 y = {}, a = {};
+#pragma unroll
+                for (int kc = 2 * rb; kc < KC; ++kc) {
+                    bf16x8 dh, dl;
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) {
+                        const float d = x[kc][e] - mu[16 * kc + 8 * h + e];
+                        const __bf16 hi = (__bf16)d;
+                        dh[e] = hi;
+                        dl[e] = (__bf16)(d - (float)hi);
+                    }
+                    const bf16x8 wh = thi[(rb * KC + kc) * 64 + lane];
+                    const bf16x8 wl = tlo[(rb * KC + kc) * 64 + lane];
+                    const u32x4 wabs = __builtin_bit_cast(u32x4, wh) & 0x7fff7fffu;
+                    const u32x4 dabs = __builtin_bit_cast(u32x4, dh) & 0x7fff7fffu;
+                    y = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wh, dh, y, 0, 0, 0);
+                    y = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wh, dl, y, 0, 0, 0);
+                    y = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wl, dh, y, 0, 0, 0);
+                    a = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, wabs),
+                                                                __builtin_bit_cast(bf16x8, dabs), a, 0, 0, 0);
+                }
+#pragma unroll
+                for (int v = 0; v < 16; ++v) {
+                    qq = fmaf(y[v], y[v], qq);
+                    const float ea = eps * a[v];
+                    ee = fmaf(2.0f * fabsf(y[v]) + ea, ea, ee);
+                }
+            }
+            qq += __shfl_xor(qq, 32);
+            ee += __shfl_xor(ee, 32);
+            const float sv = mu[DP] - 0.5f * qq;
+            const float dv = ee + (2.0f * (float)DP + 4.0f) * 0x1p-24f * qq;   // the two sums of squares
+            if (hh == 0) { s0 = sv; dq0 = dv; } else { s1 = sv; dq1 = dv; }
+        }
+        const float d = s0 - s1;
+        const float err = 0.5f * (dq0 + dq1) * (1.0f + 0x1p-10f) + 0x1p-21f * (fabsf(s0) + fabsf(s1)) + 1e-6f;
+        bool need = false;
+        if (valid && h == 0) {
+            const float lg = logit_u32<float>(philox4x32_10((uint32_t)idx, 0, iter, TAG_SUB, key0, key1).v[0]);
+            if (fabsf(lg - d) > err) subs[idx] = lg < d ? 0 : 1;
+            else need = true;
+        }
+        const uint64_t m = __ballot(need);
+        if (m) {
+            int pos0 = 0;
+            if (lane == 0) pos0 = atomicAdd(recheck_count, __popcll(m));   // one atomic per wave
+            pos0 = __shfl(pos0, 0);
+            if (need) recheck[pos0 + __popcll(m & lt)] = idx;
+        }
+    }
+}
+
+template <int DP>
+void launch_sub_gauss_mx(hipStream_t st, int grid, const float* X, int D, int K, int CH, const int32_t* perm,
+                         const int32_t* label_start, const int32_t* chunk_start, const unsigned char* SPB, float tau,
+                         uint32_t key0, uint32_t key1, uint32_t iter, int32_t* subs, int32_t* recheck,
+                         int* recheck_count) {
+    hipLaunchKernelGGL((k_sub_gauss_mx<DP>), dim3(grid), dim3(kThreads), 0, st, X, D, K, CH, perm, label_start,
+                       chunk_start, SPB, tau, key0, key1, iter, subs, recheck, recheck_count);
+}
+
 #define DPMM_MX_INSTANTIATE(DP)                                                                               \
     template void launch_label_gauss_mx<DP>(hipStream_t, int64_t, int, int, const float*, const unsigned char*, \
-                                            float, uint32_t, uint32_t, uint32_t, int, int32_t*, int32_t*, int*);
+                                            float, uint32_t, uint32_t, uint32_t, int, int32_t*, int32_t*, int*); \
+    template void launch_sub_gauss_mx<DP>(hipStream_t, int, const float*, int, int, int, const int32_t*,       \
+                                          const int32_t*, const int32_t*, const unsigned char*, float, uint32_t, \
+                                          uint32_t, uint32_t, int32_t*, int32_t*, int*);
 #define DPMM_MX_EXTERN(DP)                                                                                    \
     extern template void launch_label_gauss_mx<DP>(hipStream_t, int64_t, int, int, const float*,               \
                                                    const unsigned char*, float, uint32_t, uint32_t, uint32_t,  \
-                                                   int, int32_t*, int32_t*, int*);
+                                                   int, int32_t*, int32_t*, int*);                             \
+    extern template void launch_sub_gauss_mx<DP>(hipStream_t, int, const float*, int, int, int, const int32_t*, \
+                                                 const int32_t*, const int32_t*, const unsigned char*, float,   \
+                                                 uint32_t, uint32_t, uint32_t, int32_t*, int32_t*, int*);
 
 }  // namespace dpmm

@@ -393,13 +393,14 @@ __device__ __forceinline__ uint16_t bf16_bits_rne(float f) {
 }
 
 // One block per part p = 3k + r: r = 0 writes cluster slot k of P (and the BF16 screen slot k of
-// PB when non-null), r = 1, 2 sub-cluster slot 2k + r - 1 of SP.  logw: K cluster log weights
-// then 2K sub-cluster log weights.
+// PB when non-null), r = 1, 2 sub-cluster slot 2k + r - 1 of SP (and of SPB when non-null).
+// logw: K cluster log weights then 2K sub-cluster log weights.
 template <typename T>
 __global__ void __launch_bounds__(kThreads) k_pack_gauss(const double* __restrict__ draws, int draw_stride, int D,
                                                          int DP, int DPM, const double* __restrict__ logw, int K,
                                                          T* __restrict__ P, T* __restrict__ SP,
-                                                         unsigned char* __restrict__ PB) {
+                                                         unsigned char* __restrict__ PB,
+                                                         unsigned char* __restrict__ SPB) {
     const int p = blockIdx.x, k = p / 3, r = p - 3 * k;
     const double* src = draws + (size_t)p * draw_stride;
     const int stride = gauss_slot_stride(DP);
@@ -408,10 +409,18 @@ __global__ void __launch_bounds__(kThreads) k_pack_gauss(const double* __restric
     const int TR = D * (D + 1) / 2;
     const double cst = -0.5 * src[D + TR] + lw;
     for (int e = threadIdx.x; e < stride; e += blockDim.x) dst[e] = T(0);
+    // BF16 slots: a cluster's screen slot (tiles | mu | c) or a sub-cluster's split slot
+    // (hi tiles | lo tiles | mu | c)
     uint16_t* tiles = nullptr;
-    const int mslot = mx_slot_bytes(DPM);
-    if (r == 0 && PB) {
-        tiles = reinterpret_cast<uint16_t*>(PB + (size_t)k * mslot);
+    uint16_t* lo_tiles = nullptr;
+    unsigned char* mslot_base = nullptr;
+    int mslot = 0;
+    if (r == 0 && PB) { mslot = mx_slot_bytes(DPM); mslot_base = PB + (size_t)k * mslot; }
+    if (r > 0 && SPB) { mslot = mx2_slot_bytes(DPM); mslot_base = SPB + (size_t)(2 * k + r - 1) * mslot; }
+    const int mu_off = (r > 0 ? 2 : 1) * mx_tiles_bytes(DPM);
+    if (mslot_base) {
+        tiles = reinterpret_cast<uint16_t*>(mslot_base);
+        if (r > 0) lo_tiles = reinterpret_cast<uint16_t*>(mslot_base + mx_tiles_bytes(DPM));
         for (int e = threadIdx.x; e < mslot / 2; e += blockDim.x) tiles[e] = 0;
     }
     __syncthreads();
@@ -421,15 +430,20 @@ __global__ void __launch_bounds__(kThreads) k_pack_gauss(const double* __restric
         tri_ij(e, l, j);
         const double v = src[D + e];
         dst[tri_row_offset_rt(DP, j) - (j & ~3) + l] = (T)v;
-        if (tiles) tiles[mx_tile_index(DPM, j, l)] = bf16_bits_rne((float)v);
+        if (tiles) {
+            const float w = (float)v;
+            const uint16_t hb = bf16_bits_rne(w);
+            tiles[mx_tile_index(DPM, j, l)] = hb;
+            if (lo_tiles) lo_tiles[mx_tile_index(DPM, j, l)] = bf16_bits_rne(w - __uint_as_float((uint32_t)hb << 16));
+        }
     }
     for (int l = threadIdx.x; l < D; l += blockDim.x) {
         dst[gauss_mu_offset(DP) + l] = (T)src[l];
-        if (tiles) reinterpret_cast<float*>(PB + (size_t)k * mslot + mx_tiles_bytes(DPM))[l] = (float)src[l];
+        if (tiles) reinterpret_cast<float*>(mslot_base + mu_off)[l] = (float)src[l];
     }
     if (threadIdx.x == 0) {
         dst[gauss_c_offset(DP)] = (T)cst;
-        if (tiles) reinterpret_cast<float*>(PB + (size_t)k * mslot + mx_tiles_bytes(DPM))[DPM] = (float)cst;
+        if (tiles) reinterpret_cast<float*>(mslot_base + mu_off)[DPM] = (float)cst;
     }
 }
 

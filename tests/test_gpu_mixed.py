@@ -153,3 +153,31 @@ def test_mixed_multinomial_sampler_end_to_end():
         s.close()
         dev.close()
     np.testing.assert_array_equal(runs[dp.Precision.MIXED], runs[dp.Precision.FP32])
+
+
+@pytest.mark.parametrize("D,K,close", [(32, 12, False), (64, 20, False), (64, 8, True), (100, 6, False),
+                                       (128, 5, False), (20, 7, True)])
+def test_mixed_sublabels_equal_fp32_and_oracle(D, K, close):
+    """BF16-MFMA sub-label screen + FP32 recheck: sub-labels identical to the FP32 sub-label sweep."""
+    rng = np.random.default_rng(D * 7 + K)
+    N = 60_000
+    X, gt, _ = gaussian_mixture(N, D, K, seed=D * 5 + K, spread=8.0 if close else 100.0, dtype=np.float32)
+    mu, lam, ld, w, smu, slam, sld, lrw = _fitted_params(X, gt, K, rng)
+    if close:   # sub-clusters nearly equal: most draws are close calls
+        smu = np.repeat(mu, 2, axis=0) + rng.normal(size=(2 * K, D)) * 0.01
+    labels = gt.astype(np.int32)
+    say(f"mixed sub-labels D={D} K={K} close={close}")
+    out = {}
+    for prec in (dp.Precision.FP32, dp.Precision.MIXED):
+        dev = dp.DeviceSweep(X, dp.prior_type.Gaussian, seed=13)
+        dev.set_precision(prec)
+        dev.set_gaussian_params(mu, lam, ld, w, smu, slam, sld, lrw)
+        dev.set_labels(labels, np.zeros(N, np.int32))
+        dev.sample_sub_labels(6)
+        out[prec] = dev.get_labels()[1]
+        if prec == dp.Precision.MIXED:
+            print(f"D={D} K={K} close={close}: rechecked {dev.last_recheck_count()} of {N}")
+        dev.close()
+    np.testing.assert_array_equal(out[dp.Precision.MIXED], out[dp.Precision.FP32])
+    osub, margin = orc.sublabel_sweep(0, X, labels, smu, slam, sld, lr_weights=lrw, seed=13, it=6)
+    assert not ((out[dp.Precision.MIXED] != osub) & (margin >= 1e-6)).any()

@@ -20,6 +20,9 @@ sys.path.insert(0, ROOT)
 
 PEAK = {"fp32": 157.3, "fp64": 78.6, "bf16": 2500.0}   # dense TFLOP/s, MI355X
 HBM_PEAK = 8000.0                                       # GB/s
+# HBM bytes per launch of the dominant kernel, measured with rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE (scaled by a copy of known size; profiles/hbm_traffic_c3_d64_mixed.md)
+TRAFFIC = {("c3_d64", "mixed"): 2.750e9 + 0.040e9}
 
 CONFIGS = {
     # N, D, true clusters, A scale, ridge, Dirichlet weight (None equal), alpha, dtype
@@ -196,15 +199,17 @@ def main():
                   file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     dev.synchronize()
-    lab_ms, sub_ms, st_ms, host_ms, flops, rechk = [], [], [], [], [], []
+    lab_ms, sub_ms, st_ms, host_ms, flops, rechk, srechk = [], [], [], [], [], [], []
     t0 = time.perf_counter()
     for t in range(args.warmup + 1, args.warmup + args.steps + 1):
         kc = smp.num_clusters()
         smp.step(t)
         a, b, c = dev.kernel_ms()
         lab_ms.append(a); sub_ms.append(b); st_ms.append(c); host_ms.append(smp.host_ms())
-        if prec == dp.Precision.MIXED and not mult:
+        if prec == dp.Precision.MIXED:
             rechk.append(dev.last_recheck_count())
+            if not mult:
+                srechk.append(dev.last_sub_recheck_count())
         flops.append(2.0 * kc * D * (1 if mult else D) * N)
     dev.synchronize()
     torch.cuda.synchronize()
@@ -223,7 +228,8 @@ def main():
         peak = PEAK[{dp.Precision.FP64: "fp64", dp.Precision.FP32: "fp32", dp.Precision.MIXED: "bf16"}[prec]]
         roof = {"bound": "mfma",
                 "kernel": "k_label_gauss_mx + k_label_gauss_list" if prec == dp.Precision.MIXED else "k_label_gauss",
-                "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "traffic": None,
+                "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                "traffic": TRAFFIC.get((args.config, args.precision)),
                 "algorithmic": "2*K*D^2 flops per point per launch (dense whitening of K clusters)"}
     roof["frac"] = roof["achieved"] / roof["peak"]
     K_final = smp.num_clusters()
@@ -249,6 +255,7 @@ def main():
                          "statistics": float(np.mean(st_ms)), "conjugate": float(np.mean(host_ms))},
         "final_nmi_vs_truth": nmi,
         "recheck_fraction": (float(np.mean(rechk)) / N) if rechk else None,
+        "sub_recheck_fraction": (float(np.mean(srechk)) / N) if srechk else None,
     }
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline (FP64 oracle)", file=sys.stderr, flush=True)

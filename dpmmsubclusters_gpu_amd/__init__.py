@@ -88,6 +88,7 @@ def load_library():
         "dpmm_last_kernel_ms": [_P, _P, _P, _P],
         "dpmm_set_mixed_tolerance": [_P, ctypes.c_float],
         "dpmm_last_recheck_count": [_P, _P],
+        "dpmm_last_sub_recheck_count": [_P, _P],
         "dpmm_synchronize": [_P],
         "dpmm_set_prior": [_P, dbl, _P, dbl, _P, _P],
         "dpmm_set_statistics": [_P, i, _P, _P, _P],
@@ -335,6 +336,11 @@ class DeviceSweep:
     def last_recheck_count(self):
         n = ctypes.c_int64()
         _check(load_library().dpmm_last_recheck_count(self._ctx, ctypes.byref(n)))
+        return n.value
+
+    def last_sub_recheck_count(self):
+        n = ctypes.c_int64()
+        _check(load_library().dpmm_last_sub_recheck_count(self._ctx, ctypes.byref(n)))
         return n.value
 
     def kernel_ms(self):

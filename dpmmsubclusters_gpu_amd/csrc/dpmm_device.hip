@@ -634,7 +634,7 @@ void pack_mx_slot(int D, int DPM, const double* mu, const double* R, double cst,
 int ensure_recheck(dpmm_ctx* c) {
     if (!c->dRecheck) {
         if (hipMalloc((void**)&c->dRecheck, c->N * sizeof(int32_t)) != hipSuccess ||
-            hipMalloc((void**)&c->dRecheckCount, sizeof(int)) != hipSuccess)
+            hipMalloc((void**)&c->dRecheckCount, 2 * sizeof(int)) != hipSuccess)
             return fail(DPMM_ENOMEM, "recheck alloc");
     }
     return DPMM_OK;
@@ -818,18 +818,18 @@ template <int DP> void lgl_f32(dpmm_ctx* c, uint32_t it, int f, int grid) {
 template <int DP> void sgl_f32(dpmm_ctx* c, uint32_t it, int grid) {
     launch_sub_gauss_list<float, DP>(c->stream, grid, reinterpret_cast<const float*>(c->dX), c->D,
                                      reinterpret_cast<const float*>(c->dSP), c->dLabels, c->dRecheck,
-                                     c->dRecheckCount, c->key0(), c->key1(), it, c->dSubs);
+                                     c->dRecheckCount + 1, c->key0(), c->key1(), it, c->dSubs);
 }
 template <int DPM> void sgm(dpmm_ctx* c, uint32_t it) {
     launch_sub_gauss_mx<DPM>(c->stream, chunks_upper(c, c->K), reinterpret_cast<const float*>(c->dX), c->D, c->K,
                              c->CH, c->dPerm, c->dLabelStart, c->dChunkStart,
                              reinterpret_cast<const unsigned char*>(c->dSPB), c->tau, c->key0(), c->key1(), it,
-                             c->dSubs, c->dRecheck, c->dRecheckCount);
+                             c->dSubs, c->dRecheck, c->dRecheckCount + 1);
 }
 
 // BF16 sub-label screen over the segments, then the FP32 draw of the points it could not decide.
 int run_sub_mixed(dpmm_ctx* c, uint32_t iter) {
-    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, c->dRecheckCount, 1);
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, c->dRecheckCount + 1, 1);
     switch (c->DPM) {
         case 32: sgm<32>(c, iter); break;
         case 64: sgm<64>(c, iter); break;
@@ -875,9 +875,10 @@ void launch_label_mult_k(dpmm_ctx* c, uint32_t iter, int fin) {
 template <int NCB>
 void launch_label_mult_mx_k(dpmm_ctx* c, uint32_t iter, int fin) {
     const int grid = (int)((c->N + 127) / 128);
+    const int vec = (c->D % 4 == 0 && reinterpret_cast<uintptr_t>(c->dX) % 16 == 0) ? 1 : 0;
     hipLaunchKernelGGL((k_label_mult_mx<NCB>), dim3(grid), dim3(kThreads), 0, c->stream,
-                       reinterpret_cast<const int32_t*>(c->dX), c->N, c->D, c->dMB, c->key0(), c->key1(), iter, fin,
-                       c->dLabels, c->dRecheck, c->dRecheckCount);
+                       reinterpret_cast<const int32_t*>(c->dX), c->N, c->D, vec, c->dMB, c->key0(), c->key1(), iter,
+                       fin, c->dLabels, c->dRecheck, c->dRecheckCount);
 }
 
 // Split-BF16 MFMA screen over all documents, then the FP32 sweep over the ones it could not decide.
@@ -1218,6 +1219,17 @@ int dpmm_last_recheck_count(dpmm_ctx* c, int64_t* n) {
     int v = 0;
     HIPCHK(hipStreamSynchronize(c->stream));
     HIPCHK(hipMemcpy(&v, c->dRecheckCount, sizeof(int), hipMemcpyDeviceToHost));
+    *n = v;
+    return DPMM_OK;
+}
+
+int dpmm_last_sub_recheck_count(dpmm_ctx* c, int64_t* n) {
+    if (!c || !n) return fail(DPMM_EINVAL, "bad args");
+    *n = -1;
+    if (!c->dRecheckCount) return DPMM_OK;
+    int v = 0;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(&v, c->dRecheckCount + 1, sizeof(int), hipMemcpyDeviceToHost));
     *n = v;
     return DPMM_OK;
 }

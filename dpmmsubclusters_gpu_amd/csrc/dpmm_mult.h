@@ -75,7 +75,8 @@ __global__ void __launch_bounds__(kThreads) k_pack_mult_mx(const double* __restr
 // 32 documents per wave, 4 waves per workgroup sharing the LDS-staged tiles of G chunks at a time.
 template <int NCB>
 __global__ void __launch_bounds__(kThreads) k_label_mult_mx(const int32_t* __restrict__ X, int64_t N, int D,
-                                                            const unsigned char* __restrict__ blk, uint32_t key0,
+                                                            int vec, const unsigned char* __restrict__ blk,
+                                                            uint32_t key0,
                                                             uint32_t key1, uint32_t iter, int final_iter,
                                                             int32_t* __restrict__ labels,
                                                             int32_t* __restrict__ recheck,
@@ -94,6 +95,21 @@ __global__ void __launch_bounds__(kThreads) k_label_mult_mx(const int32_t* __res
     int words = 0;
     const int32_t* xr = X + (valid ? doc : 0) * (int64_t)D;
     for (int g0 = 0; g0 < NC; g0 += G) {
+        // this group's counts first, so their latency overlaps the tile staging below
+        int xg[G][8];
+#pragma unroll
+        for (int gc = 0; gc < G; ++gc) {
+            const int d0 = 16 * (g0 + gc) + 8 * half;
+            if (vec && valid && d0 + 8 <= D) {   // rows 16-byte aligned: two int4 loads
+                const int4 a = *reinterpret_cast<const int4*>(xr + d0);
+                const int4 b = *reinterpret_cast<const int4*>(xr + d0 + 4);
+                xg[gc][0] = a.x; xg[gc][1] = a.y; xg[gc][2] = a.z; xg[gc][3] = a.w;
+                xg[gc][4] = b.x; xg[gc][5] = b.y; xg[gc][6] = b.z; xg[gc][7] = b.w;
+            } else {
+#pragma unroll
+                for (int e = 0; e < 8; ++e) xg[gc][e] = (valid && d0 + e < D) ? xr[d0 + e] : 0;
+            }
+        }
         __syncthreads();
         for (int e = threadIdx.x; e < 2 * G * NCB * 64; e += kThreads) {
             const int ln = e & 63, rest = e >> 6;
@@ -104,12 +120,13 @@ __global__ void __launch_bounds__(kThreads) k_label_mult_mx(const int32_t* __res
             sA[hl][gc][cb][ln] = v;
         }
         __syncthreads();
-        for (int gc = 0; gc < G && g0 + gc < NC; ++gc) {
-            const int d0 = 16 * (g0 + gc) + 8 * half;
+#pragma unroll
+        for (int gc = 0; gc < G; ++gc) {
+            if (g0 + gc >= NC) break;
             bf16x8 xb;
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
-                const int v = (valid && d0 + e < D) ? xr[d0 + e] : 0;
+                const int v = xg[gc][e];
                 words += v;
                 xb[e] = (__bf16)(float)v;
             }

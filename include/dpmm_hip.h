@@ -134,6 +134,10 @@ int dpmm_get_drawn_params(dpmm_ctx* ctx, double* mu, double* chol_inv_sigma, dou
 int dpmm_set_mixed_tolerance(dpmm_ctx* ctx, float tau);
 /* Number of points the last mixed label sweep re-scored in FP32 (-1 if none ran). */
 int dpmm_last_recheck_count(dpmm_ctx* ctx, int64_t* n);
+/* Number of points whose sub-label the last mixed sub-label sweep drew in FP32 (-1 if none ran).
+ * The mixed sub-label sweep screens s_left - s_right with split-BF16 MFMA and a per-point error
+ * bound against the FP32 sweep's arithmetic; no counterpart in the reference. */
+int dpmm_last_sub_recheck_count(dpmm_ctx* ctx, int64_t* n);
 
 /* Kernel timings of the last calls (ms, HIP events on the library's stream). */
 int dpmm_last_kernel_ms(dpmm_ctx* ctx, float* label_sweep_ms, float* sub_sweep_ms, float* stats_ms);

@@ -26,11 +26,14 @@ TRAFFIC = {("c3_d64", "mixed"): 2.750e9 + 0.040e9}
 
 CONFIGS = {
     # N, D, true clusters, A scale, ridge, Dirichlet weight (None equal), alpha, dtype
-    "c1_readme_2d": dict(N=100_000, D=2, K=2, a=1.0, ridge=1.0, weight=None, alpha=1.0, dtype="float64"),
-    "c2_d32": dict(N=1_000_000, D=32, K=20, a=1.0, ridge=1.0, weight=1.0, alpha=10.0, dtype="float32"),
-    "c3_d64": dict(N=10_000_000, D=64, K=50, a=1.0, ridge=0.5, weight=0.5, alpha=10.0, dtype="float32"),
+    "c1_readme_2d": dict(N=100_000, D=2, K=2, a=1.0, ridge=1.0, weight=None, alpha=1.0, dtype="float64",
+                         iters=100),
+    "c2_d32": dict(N=1_000_000, D=32, K=20, a=1.0, ridge=1.0, weight=1.0, alpha=10.0, dtype="float32", iters=200),
+    "c3_d64": dict(N=10_000_000, D=64, K=50, a=1.0, ridge=0.5, weight=0.5, alpha=10.0, dtype="float32",
+                   iters=200),
     # multinomial: N documents of `words` words over a vocabulary of D, topics theta ~ Dir(conc)
-    "c4_mult": dict(N=1_000_000, D=1000, K=100, words=100, conc=0.1, alpha=10.0, dtype="int32", prior=1.0),
+    "c4_mult": dict(N=1_000_000, D=1000, K=100, words=100, conc=0.1, alpha=10.0, dtype="int32", prior=1.0,
+                    iters=150),
 }
 
 
@@ -84,63 +87,137 @@ def gen_docs_on_gpu(cfg, seed):
     return X, gt, theta
 
 
-def cpu_baseline_mult(Xs, theta, seed, target_s=8.0):
-    """FP64 oracle (OpenMP) on a bounded sample: one iteration's label sweep, sub-label sweep and
-    statistics with the generating topics as cluster parameters."""
+def _draw_current(dev, smp, K, it):
+    """Draw every cluster's and sub-cluster's parameters from the sampler's current statistics on
+    the device (packed into the sweep's slots), and return them with the mixture weights."""
+    w = smp.weights()[0]
+    lrw = np.full(2 * K, 0.5)
+    dev.sufficient_statistics(K, counts_only=True)
+    mem = np.array([[2 * k, 2 * k + 1, -1, -1] if p == 0 else [2 * k + p - 1, -1, -1, -1]
+                    for k in range(K) for p in range(3)], np.int32)
+    dev.sample_params(it, mem, w, lrw)
+    mu, R, ld = dev.drawn_params(K)
+    return w, lrw, mu, R, ld
+
+
+def cpu_baseline_and_check(dev, smp, X, N, mult, seed, it, target_s=8.0, score_points=20_000):
+    """The FP64 oracle (OpenMP) timed on a bounded sample of the same workload: one iteration's
+    label sweep, sub-label sweep and statistics at parameters the sampler draws from its current
+    state.  The same parameters and Philox noise drive one sweep of the measured GPU path over all
+    N points, and the oracle's labels on the sample give the label-flip rates; the noise-free scores
+    of the first points give the log-likelihood error of the arithmetic that decides the labels."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_ctypes as orc
-    K, D = theta.shape
-    lt = np.log(theta + 1e-300)
-    slt = np.repeat(lt, 2, axis=0)
-    lrw = np.full(2 * K, 0.5)
+    K = smp.num_clusters()
+    w, lrw, mu, R, ld = _draw_current(dev, smp, K, it)
+    c = np.arange(K) * 3
+    sub = np.sort(np.r_[c + 1, c + 2])
+    dev.sample_cluster_labels(it, False)
+    dev.sample_sub_labels(it)
+    glab, gsub = dev.get_labels()
+    gs = dev.log_likelihoods(K, 0, 0, score_points)
+    gss = dev.log_likelihoods(K, 1, 0, score_points)
+    prior = 1 if mult else 0
+    if mult:
+        cl = dict(logtheta=mu[c])
+        sl = dict(logtheta=mu[sub])
+    else:
+        lam = np.einsum("kij,klj->kil", R, R)
+        cl = dict(mu=mu[c], inv_sigma=lam[c], logdet=ld[c])
+        sl = dict(mu=mu[sub], inv_sigma=lam[sub], logdet=ld[sub])
 
     def one(n):
-        X = np.ascontiguousarray(Xs[:n], dtype=np.float64)
+        Xs = np.ascontiguousarray(X[:n].cpu().numpy(), dtype=np.float64)
         t0 = time.perf_counter()
-        lab, _ = orc.label_sweep(1, X, logtheta=lt, weights=np.full(K, 1.0 / K), seed=seed, it=1)
-        sub, _ = orc.sublabel_sweep(1, X, lab, logtheta=slt, lr_weights=lrw, seed=seed, it=1)
-        orc.suff_stats(1, X, lab, sub, K, want_S=False)
-        return time.perf_counter() - t0
+        lab, margin = orc.label_sweep(prior, Xs, weights=w, seed=seed, it=it, **cl)
+        subs, smargin = orc.sublabel_sweep(prior, Xs, lab, lr_weights=lrw, seed=seed, it=it, **sl)
+        orc.suff_stats(prior, Xs, lab, subs, K, want_S=not mult)
+        return time.perf_counter() - t0, Xs, lab, margin, subs, smargin
 
-    n = min(len(Xs), 5_000)
-    dt = one(n)
-    n2 = int(min(len(Xs), max(n, n * target_s / max(dt, 1e-3))))
-    dt2 = one(n2)
+    n = min(N, 5_000 if mult else 20_000)
+    dt = one(n)[0]
+    n2 = int(min(N, max(n, n * target_s / max(dt, 1e-3))))
+    dt2, Xs, olab, margin, osub, smargin = one(n2)
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": n2 / dt2, "unit": "point*iters/s", "cores": threads, "kind": "port",
-            "sample": f"FP64 oracle label sweep + sub-label sweep + statistics of one iteration over {n2} documents "
-                      f"(K={K}, D={D}), {dt2:.1f} s"}
+    what = "documents" if mult else "points"
+    base = {"value": n2 / dt2, "unit": "point*iters/s", "cores": threads, "kind": "port",
+            "sample": f"FP64 oracle label sweep + sub-label sweep + statistics of one iteration over the first "
+                      f"{n2} {what} (K={K}, D={Xs.shape[1]}) at sampler-drawn parameters, {dt2:.1f} s"}
+    flips = np.nonzero(glab[:n2] != olab)[0]
+    same = np.nonzero(glab[:n2] == olab)[0]
+    sflips = same[gsub[:n2][same] != osub[same]]
+    # noise-free oracle scores of the first points, clusters and the sub-clusters of the GPU labels
+    m = min(score_points, n2)
+    _, _, os_ = orc.label_sweep(prior, Xs[:m], weights=w, seed=seed, it=it, final=True, want_scores=True, **cl)
+    _, _, oss = orc.label_sweep(prior, Xs[:m], weights=lrw, seed=seed, it=it, final=True, want_scores=True,
+                                mult_use_weight=True, **sl)
+    k = glab[:m]
+    oss = np.stack([oss[np.arange(m), 2 * k], oss[np.arange(m), 2 * k + 1]], axis=1)
+    err = np.abs(gs[:m] - os_)
+    serr = np.abs(gss[:m] - oss)
+    check = {
+        "points": n2, "iteration": it, "clusters": K,
+        "label_flip_rate": flips.size / n2, "label_flips": int(flips.size),
+        "label_flips_above_rel_margin_1e-6": int(np.sum(margin[flips] >= 1e-6)),
+        "sub_label_flip_rate": sflips.size / max(1, same.size), "sub_label_flips": int(sflips.size),
+        "sub_label_flips_above_rel_margin_1e-6": int(np.sum(smargin[sflips] >= 1e-6)),
+        "score_points": m,
+        "score_max_abs_err": float(err.max()), "score_mean_abs_err": float(err.mean()),
+        "score_max_rel_err": float((err / np.maximum(1.0, np.abs(os_))).max()),
+        "sub_score_max_abs_err": float(serr.max()),
+        "sub_score_max_rel_err": float((serr / np.maximum(1.0, np.abs(oss))).max()),
+    }
+    return base, check
 
 
-def cpu_baseline(Xs, means, facs, w, seed, target_s=8.0):
-    """FP64 oracle (OpenMP) on a bounded sample: one iteration's label sweep, sub-label sweep and
-    statistics over the sample, with the generating mixture as cluster parameters."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_ctypes as orc
-    K, D = means.shape
-    lam = np.stack([np.linalg.inv(f @ f.T) for f in facs])
-    ld = np.array([2 * np.log(np.diag(f)).sum() for f in facs])
-    smu = np.repeat(means, 2, axis=0)
-    slam = np.repeat(lam, 2, axis=0)
-    sld = np.repeat(ld, 2)
-    lrw = np.full(2 * K, 0.5)
-
-    def one(n):
-        X = np.ascontiguousarray(Xs[:n], dtype=np.float64)
-        t0 = time.perf_counter()
-        lab, _ = orc.label_sweep(0, X, means, lam, ld, weights=w, seed=seed, it=1)
-        sub, _ = orc.sublabel_sweep(0, X, lab, smu, slam, sld, lr_weights=lrw, seed=seed, it=1)
-        orc.suff_stats(0, X, lab, sub, K)
-        return time.perf_counter() - t0
-
-    n = min(len(Xs), 20_000)
-    dt = one(n)
-    n2 = int(min(len(Xs), max(n, n * target_s / max(dt, 1e-3))))
-    dt2 = one(n2)
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": n2 / dt2, "unit": "point*iters/s", "cores": threads, "kind": "port",
-            "sample": f"FP64 oracle label sweep + sub-label sweep + statistics of one iteration over {n2} points "
-                      f"(K={K}, D={D}), {dt2:.1f} s"}
+def run_seeds(args, cfg):
+    """Final K and NMI against the generating labels after the configuration's full run, for seeds
+    0..S-1 (data and sampler seeded alike), on each requested path; one JSON line."""
+    import torch
+    import dpmmsubclusters_gpu_amd as dp
+    from dpmmsubclusters_gpu_amd import _nmi
+    mult = cfg["dtype"] == "int32"
+    N, D, iters = cfg["N"], cfg["D"], cfg["iters"]
+    paths = [p for p in args.paths.split(",")
+             if not (mult and p == "fp64") and not (cfg["dtype"] == "float64" and p != "fp64")]
+    res = {p: {"K": [], "nmi": [], "seconds": []} for p in paths}
+    for seed in range(args.seeds):
+        if mult:
+            X, gt, _ = gen_docs_on_gpu(cfg, seed)
+        else:
+            X, gt, _, _, _ = gen_on_gpu(cfg, seed)
+        gtn = gt.cpu().numpy()
+        for path in paths:
+            Xp = X.double() if path == "fp64" and not mult else X
+            dtype = np.int32 if mult else (np.float64 if Xp.dtype == torch.float64 else np.float32)
+            prior = dp.prior_type.Multinomial if mult else dp.prior_type.Gaussian
+            dev = dp.DeviceSweep(None, prior, seed=seed + 1, device_ptr=Xp.data_ptr(), N=N, D=D, dtype=dtype)
+            hyper = (dp.multinomial_hyper(np.full(D, cfg["prior"])) if mult else
+                     dp.niw_hyperparams(1.0, np.zeros(D), D + 3.0, np.eye(D)))
+            prec = {"mixed": dp.Precision.MIXED, "fp32": dp.Precision.FP32, "fp64": dp.Precision.FP64}[path]
+            smp = dp.Sampler(dev, hyper, cfg["alpha"], iters, seed=seed + 1, precision=prec)
+            t0 = time.perf_counter()
+            for t in range(1, iters + 1):
+                smp.step(t)
+            dev.synchronize()
+            dt = time.perf_counter() - t0
+            lab, _ = dev.get_labels()
+            res[path]["K"].append(smp.num_clusters())
+            res[path]["nmi"].append(_nmi(gtn, lab))
+            res[path]["seconds"].append(dt)
+            print(f"[bench] seed {seed} {path}: K={smp.num_clusters()} NMI={res[path]['nmi'][-1]:.4f} "
+                  f"({dt:.1f} s)", file=sys.stderr, flush=True)
+            smp.close()
+            dev.close()
+            del Xp
+        del X
+        torch.cuda.empty_cache()
+    summary = {p: {"K_mean": float(np.mean(r["K"])), "K_std": float(np.std(r["K"])),
+                   "nmi_mean": float(np.mean(r["nmi"])), "nmi_min": float(np.min(r["nmi"])), **r}
+               for p, r in res.items()}
+    print(json.dumps({"metric": "final K and NMI vs generating labels over seeds", "config": args.config,
+                      "N": N, "D": D, "true_clusters": cfg["K"], "iterations": iters, "seeds": args.seeds,
+                      "paths": summary}), flush=True)
 
 
 def main():
@@ -152,6 +229,9 @@ def main():
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp32", "fp64"])
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--seeds", type=int, default=0,
+                    help="instead of timing: final K and NMI over this many seeds on each of --paths")
+    ap.add_argument("--paths", default="mixed,fp32,fp64")
     ap.add_argument("--copy-probe", action="store_true",
                     help="copy X once (N*D*4 bytes read + written) before the run: a plain copy kernel of known "
                          "size that calibrates the HBM byte counters of a profiler")
@@ -160,6 +240,10 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (points would shard across GPUs "
               "with one FP64 all-reduce of the sufficient statistics per iteration; not built).", file=sys.stderr)
         sys.exit(2)
+
+    if args.seeds > 0:
+        run_seeds(args, CONFIGS[args.config])
+        return
 
     import torch
     import dpmmsubclusters_gpu_amd as dp
@@ -258,10 +342,9 @@ def main():
         "sub_recheck_fraction": (float(np.mean(srechk)) / N) if srechk else None,
     }
     if not args.no_cpu_baseline:
-        print("[bench] cpu baseline (FP64 oracle)", file=sys.stderr, flush=True)
-        Xs = X[: min(N, 2_000_000 if not mult else 200_000)].cpu().numpy()
-        out["cpu_baseline"] = (cpu_baseline_mult(Xs, theta, args.seed) if mult else
-                               cpu_baseline(Xs, means, facs, w, args.seed))
+        print("[bench] cpu baseline and flip check (FP64 oracle)", file=sys.stderr, flush=True)
+        out["cpu_baseline"], out["oracle_check"] = cpu_baseline_and_check(
+            dev, smp, X, N, mult, args.seed + 1, args.warmup + args.steps + 1)
     print(json.dumps(out), flush=True)
 
 

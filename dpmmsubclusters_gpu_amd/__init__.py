@@ -83,6 +83,7 @@ def load_library():
         "dpmm_set_multinomial_params": [_P, i, _P, _P, _P, _P, i],
         "dpmm_sample_cluster_labels": [_P, u32, i],
         "dpmm_sample_sub_labels": [_P, u32],
+        "dpmm_log_likelihoods": [_P, i, i, ctypes.c_int64, ctypes.c_int64, _P],
         "dpmm_sufficient_statistics": [_P, i, _P, _P, _P],
         "dpmm_relabel": [_P, u32, i, _P, _P],
         "dpmm_last_kernel_ms": [_P, _P, _P, _P],
@@ -268,6 +269,14 @@ class DeviceSweep:
 
     def sample_sub_labels(self, iteration):
         _check(load_library().dpmm_sample_sub_labels(self._ctx, int(iteration)))
+
+    def log_likelihoods(self, K, which, first, n):
+        """Noise-free scores of points [first, first + n) in the sweeps' arithmetic against the K
+        clusters whose parameters are set: which = 0 the n x K cluster scores, which = 1 the n x 2
+        scores of each point's two sub-clusters."""
+        out = np.empty((n, K if which == 0 else 2))
+        _check(load_library().dpmm_log_likelihoods(self._ctx, int(K), int(which), int(first), int(n), _ptr(out)))
+        return out
 
     def sufficient_statistics(self, K, want_S=True, counts_only=False):
         Nsub = np.empty(2 * K, np.int64)

@@ -282,6 +282,43 @@ __global__ void __launch_bounds__(kThreads) k_sub_mult(const int32_t* __restrict
     }
 }
 
+// Noise-free multinomial scores of documents [first, first + n), one wave per document, with the
+// sweeps' FP32 arithmetic: which = 0 the n x K cluster scores of k_label_mult (per cluster the
+// fmaf chain over the vocabulary in order, then the slot's constant), which = 1 the n x 2 scores of
+// the two sub-clusters of the document's label as k_sub_mult forms them (lane-strided partial
+// sums, butterfly reduction, constant).
+__global__ void __launch_bounds__(kThreads) k_score_mult(const int32_t* __restrict__ X, int D, int K,
+                                                         const float* __restrict__ P, const float* __restrict__ SP,
+                                                         int stride, const int32_t* __restrict__ labels,
+                                                         int64_t first, int64_t n, int which,
+                                                         float* __restrict__ out) {
+    const int lane = threadIdx.x & 63;
+    for (int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); w < n; w += (int64_t)gridDim.x * 4) {
+        const int64_t i = first + w;
+        const int32_t* x = X + i * (int64_t)D;
+        if (which == 0) {
+            for (int k = lane; k < K; k += 64) {
+                const float* th = P + (size_t)k * stride;
+                float a = 0.f;
+                for (int d = 0; d < D; ++d) a = fmaf((float)x[d], th[d], a);
+                out[w * K + k] = a + th[D];
+            }
+        } else {
+            const float* th = SP + (size_t)(2 * labels[i]) * stride;
+            float a0 = 0.f, a1 = 0.f;
+            for (int d = lane; d < D; d += 64) {
+                const float xv = (float)x[d];
+                if (xv != 0.f) { a0 = fmaf(xv, th[d], a0); a1 = fmaf(xv, th[stride + d], a1); }
+            }
+            for (int off = 32; off > 0; off >>= 1) { a0 += __shfl_xor(a0, off); a1 += __shfl_xor(a1, off); }
+            if (lane == 0) {
+                out[w * 2] = a0 + th[D];
+                out[w * 2 + 1] = a1 + th[stride + D];
+            }
+        }
+    }
+}
+
 // ---- sufficient statistics --------------------------------------------------------------------
 // Multinomial: exact integer sums per vocabulary entry.
 __global__ void __launch_bounds__(kThreads) k_stats_mult(const int32_t* __restrict__ X, int D, int K, int CH,
@@ -827,6 +864,17 @@ template <int DPM> void sgm(dpmm_ctx* c, uint32_t it) {
                              c->dSubs, c->dRecheck, c->dRecheckCount + 1);
 }
 
+template <int DP> void score_f32(dpmm_ctx* c, int grid, int64_t first, int64_t n, int which, void* out) {
+    launch_score_gauss<float, DP>(c->stream, grid, reinterpret_cast<const float*>(c->dX), c->D, c->K,
+                                  reinterpret_cast<const float*>(c->dP), reinterpret_cast<const float*>(c->dSP),
+                                  c->dLabels, first, n, which, reinterpret_cast<float*>(out));
+}
+template <int DP> void score_f64(dpmm_ctx* c, int grid, int64_t first, int64_t n, int which, void* out) {
+    launch_score_gauss<double, DP>(c->stream, grid, reinterpret_cast<const double*>(c->dX), c->D, c->K,
+                                   reinterpret_cast<const double*>(c->dP), reinterpret_cast<const double*>(c->dSP),
+                                   c->dLabels, first, n, which, reinterpret_cast<double*>(out));
+}
+
 // BF16 sub-label screen over the segments, then the FP32 draw of the points it could not decide.
 int run_sub_mixed(dpmm_ctx* c, uint32_t iter) {
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, c->dRecheckCount + 1, 1);
@@ -1113,6 +1161,51 @@ int dpmm_sample_sub_labels(dpmm_ctx* c, uint32_t iter) {
     (void)hipEventRecord(c->ev[3], c->stream);
     HIPCHK(hipGetLastError());
     c->ev_sub = true;
+    return DPMM_OK;
+}
+
+int dpmm_log_likelihoods(dpmm_ctx* c, int K, int which, int64_t first, int64_t n, double* out) {
+    if (!c || !out || (which != 0 && which != 1)) return fail(DPMM_EINVAL, "bad args");
+    if (c->K <= 0) return fail(DPMM_ESTATE, "parameters not set");
+    if (K != c->K) return fail(DPMM_EINVAL, "K differs from the cluster count of the parameters set");
+    if (first < 0 || n <= 0 || first + n > c->N) return fail(DPMM_EINVAL, "point range outside [0, N)");
+    const int cols = which == 0 ? c->K : 2;
+    if (which == 1) {   // the sub-cluster slots are indexed by each point's label: check them first
+        std::vector<int32_t> lab((size_t)n);
+        HIPCHK(hipMemcpyAsync(lab.data(), c->dLabels + first, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (int32_t l : lab)
+            if (l < 0 || l >= c->K) return fail(DPMM_ESTATE, "a label is outside the clusters whose parameters are set");
+    }
+    const bool f64 = c->prior == DPMM_PRIOR_GAUSSIAN && c->params_f64;
+    const size_t esz = f64 ? 8 : 4;
+    void* dOut = nullptr;
+    HIPCHK(hipMalloc(&dOut, (size_t)n * cols * esz));
+    if (c->prior == DPMM_PRIOR_GAUSSIAN) {
+        const int64_t ppb = (c->DP * (int)esz > 256) ? kThreads / 2 : kThreads;
+        const int grid = (int)std::min<int64_t>(4096, (n + ppb - 1) / ppb);
+        if (f64) { DPMM_DISPATCH_DP64(c->DP, score_f64, c, grid, first, n, which, dOut); }
+        else { DPMM_DISPATCH_DP(c->DP, score_f32, c, grid, first, n, which, dOut); }
+    } else {
+        const int grid = (int)std::min<int64_t>(4096, (n + 3) / 4);
+        hipLaunchKernelGGL(k_score_mult, dim3(grid), dim3(kThreads), 0, c->stream,
+                           reinterpret_cast<const int32_t*>(c->dX), c->D, c->K, reinterpret_cast<const float*>(c->dP),
+                           reinterpret_cast<const float*>(c->dSP), c->stride, c->dLabels, first, n, which,
+                           reinterpret_cast<float*>(dOut));
+    }
+    hipError_t e = hipGetLastError();
+    std::vector<unsigned char> h((size_t)n * cols * esz);
+    if (e == hipSuccess) e = hipMemcpyAsync(h.data(), dOut, h.size(), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(dOut);
+    if (e != hipSuccess) return fail(DPMM_EHIP, std::string("log likelihoods: ") + hipGetErrorString(e));
+    const size_t m = (size_t)n * cols;
+    if (f64) std::memcpy(out, h.data(), m * 8);
+    else for (size_t j = 0; j < m; ++j) out[j] = (double)reinterpret_cast<const float*>(h.data())[j];
+    if (c->prior == DPMM_PRIOR_GAUSSIAN) {   // the constant the slots leave out (common to every score)
+        const double common = -0.5 * (double)c->D * c->D * std::log(2.0 * kPiF64);
+        for (size_t j = 0; j < m; ++j) out[j] += common;
+    }
     return DPMM_OK;
 }
 

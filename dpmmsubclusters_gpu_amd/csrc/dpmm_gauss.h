@@ -319,6 +319,46 @@ __global__ void __launch_bounds__(kThreads) k_sub_gauss_list(const T* __restrict
     }
 }
 
+// Noise-free scores of points [first, first + n) with the sweeps' arithmetic (the FP32 / FP64
+// expression c - q / 2 of k_label_gauss and k_sub_gauss): which = 0 writes the n x K matrix of
+// cluster scores (log weight + log N(x | mu_k, Sigma_k)), which = 1 the n x 2 scores of the two
+// sub-clusters of each point's label (log sub-weight + log-likelihood).  Slots are read from global
+// memory; workgroups stride over the points with a uniform trip count per lane pair.
+template <typename T, int DP>
+__global__ void __launch_bounds__(kThreads) k_score_gauss(const T* __restrict__ X, int D, int K,
+                                                          const T* __restrict__ P, const T* __restrict__ SP,
+                                                          const int32_t* __restrict__ labels, int64_t first,
+                                                          int64_t n, int which, T* __restrict__ out) {
+    constexpr bool kPair = paired_layout<T, DP>();
+    constexpr int kPts = kPair ? kThreads / 2 : kThreads;
+    constexpr int kCols = kPair ? DP / 2 : DP;
+    constexpr int stride = gauss_slot_stride(DP);
+    const int par = kPair ? (threadIdx.x & 1) : 0;
+    const int local = kPair ? (threadIdx.x >> 1) : threadIdx.x;
+    for (int64_t b0 = (int64_t)blockIdx.x * kPts; b0 < n; b0 += (int64_t)gridDim.x * kPts) {
+        const int64_t pos = b0 + local;
+        const bool valid = pos < n;
+        const int64_t idx = valid ? first + pos : 0;
+        T x[kCols];
+#pragma unroll
+        for (int c = 0; c < kCols; ++c) {
+            const int l = kPair ? pair_col(c, 0) + (par << 2) : c;
+            x[c] = (valid && l < D) ? X[idx * D + l] : T(0);
+        }
+        const int kl = (which == 1 && valid) ? labels[idx] : 0;
+        const int cnt = which == 1 ? 2 : K;
+#pragma unroll 1
+        for (int k = 0; k < cnt; ++k) {
+            const T* pp = which == 1 ? SP + (size_t)(2 * kl + k) * stride : P + (size_t)k * stride;
+            T q;
+            if constexpr (kPair) q = whitened_sq_pair<T, DP>(pp, x, par);
+            else q = whitened_sq_regs<T, DP>(pp, x);
+            const T s = pp[gauss_c_offset(DP)] - T(0.5) * q;
+            if (valid && par == 0) out[pos * cnt + k] = s;
+        }
+    }
+}
+
 // Partial (count, sum, scatter) of the points of sub-cluster s in one chunk, FP64.  Threads own
 // 4x4 blocks of the upper block triangle; points staged in LDS in batches, compacted by ballot.
 template <typename TX, int DP>
@@ -447,6 +487,13 @@ void launch_sub_gauss(hipStream_t st, int grid, const T* X, int D, int K, int CH
                        chunk_start, SP, key0, key1, iter, subs);
 }
 
+template <typename T, int DP>
+void launch_score_gauss(hipStream_t st, int grid, const T* X, int D, int K, const T* P, const T* SP,
+                        const int32_t* labels, int64_t first, int64_t n, int which, T* out) {
+    hipLaunchKernelGGL((k_score_gauss<T, DP>), dim3(grid), dim3(kThreads), 0, st, X, D, K, P, SP, labels, first, n,
+                       which, out);
+}
+
 // nchunks: upper bound of the chunk count (one MFMA block per chunk, one vector block per
 // (chunk, sub-cluster)).
 template <typename TX, int DP>
@@ -474,7 +521,9 @@ void launch_stats_gauss(hipStream_t st, int nchunks, const TX* X, int D, int K, 
                                             const int32_t*, const int32_t*, const int32_t*, double*);          \
     template void launch_sub_gauss_list<T, DP>(hipStream_t, int, const T*, int, const T*, const int32_t*,      \
                                                const int32_t*, const int*, uint32_t, uint32_t, uint32_t,       \
-                                               int32_t*);
+                                               int32_t*);                                                      \
+    template void launch_score_gauss<T, DP>(hipStream_t, int, const T*, int, int, const T*, const T*,          \
+                                            const int32_t*, int64_t, int64_t, int, T*);
 
 #define DPMM_GAUSS_EXTERN(T, DP)                                                                              \
     extern template void launch_label_gauss<T, DP>(hipStream_t, int, const T*, int64_t, int, int, const T*,    \
@@ -489,6 +538,8 @@ void launch_stats_gauss(hipStream_t st, int nchunks, const TX* X, int D, int K, 
                                                    const int32_t*, const int32_t*, const int32_t*, double*);   \
     extern template void launch_sub_gauss_list<T, DP>(hipStream_t, int, const T*, int, const T*,               \
                                                       const int32_t*, const int32_t*, const int*, uint32_t,    \
-                                                      uint32_t, uint32_t, int32_t*);
+                                                      uint32_t, uint32_t, int32_t*);                           \
+    extern template void launch_score_gauss<T, DP>(hipStream_t, int, const T*, int, int, const T*, const T*,   \
+                                                   const int32_t*, int64_t, int64_t, int, T*);
 
 }  // namespace dpmm

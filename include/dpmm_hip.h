@@ -82,6 +82,15 @@ int dpmm_sample_cluster_labels(dpmm_ctx* ctx, uint32_t iter, int final_iter);
 /* cudaKernel::create_subclusters_labels (cudaKernel.cu:1299-1398): deterministic partition
  * of the points by label, then one segmented pass drawing each point's sub-label. */
 int dpmm_sample_sub_labels(dpmm_ctx* ctx, uint32_t iter);
+/* The log-likelihood rows the reference writes into its N x K matrix d_r before the draw
+ * (cudaKernel_gaussian.cu:43-113, cudaKernel_multinomial.cu:37-76, called from cudaKernel.cu:1331
+ * and 1430-1434), for points [first, first + n), in the sweeps' own arithmetic and without noise:
+ * which = 0 the n x K scores log pi_k + log p(x | theta_k) (multinomial: without log pi_k unless
+ * use_weight), which = 1 the n x 2 scores log pi_{k,s} + log p(x | theta_{k,s}) of the two
+ * sub-clusters of each point's current label.  Gaussian scores carry the reference's constant
+ * -D^2 log(2 pi) / 2 (cudaKernel_gaussian.cu:75, 108), added in FP64 to the sweep's value.
+ * Row-major into out (FP64).  K must equal the cluster count of the parameters set. */
+int dpmm_log_likelihoods(dpmm_ctx* ctx, int K, int which, int64_t first, int64_t n, double* out);
 /* cudaKernel::create_sufficient_statistics (cudaKernel.cu:844-941) for all K clusters at
  * once: per sub-cluster q = 2k + s: count Nsub[q], sum[q*D + d], and (Gaussian) the
  * uncentred scatter S[q*D*D + a*D + b] in FP64; S may be NULL.  Deterministic: no

@@ -138,11 +138,15 @@ def cpu_baseline_and_check(dev, smp, X, N, mult, seed, it, target_s=8.0, score_p
     dt = one(n)[0]
     n2 = int(min(N, max(n, n * target_s / max(dt, 1e-3))))
     dt2, Xs, olab, margin, osub, smargin = one(n2)
+    reps = 1
+    if n2 == N and dt2 < target_s:   # the whole data set is under the target: repeat the iteration
+        reps = int(min(200, np.ceil(target_s / max(dt2, 1e-4))))
+        dt2 = sum(one(N)[0] for _ in range(reps))
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     what = "documents" if mult else "points"
-    base = {"value": n2 / dt2, "unit": "point*iters/s", "cores": threads, "kind": "port",
-            "sample": f"FP64 oracle label sweep + sub-label sweep + statistics of one iteration over the first "
-                      f"{n2} {what} (K={K}, D={Xs.shape[1]}) at sampler-drawn parameters, {dt2:.1f} s"}
+    base = {"value": n2 * reps / dt2, "unit": "point*iters/s", "cores": threads, "kind": "port",
+            "sample": f"FP64 oracle label sweep + sub-label sweep + statistics of {reps} iteration(s) over the "
+                      f"first {n2} {what} (K={K}, D={Xs.shape[1]}) at sampler-drawn parameters, {dt2:.1f} s"}
     flips = np.nonzero(glab[:n2] != olab)[0]
     same = np.nonzero(glab[:n2] == olab)[0]
     sflips = same[gsub[:n2][same] != osub[same]]

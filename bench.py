@@ -287,7 +287,7 @@ def main():
                   file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     dev.synchronize()
-    lab_ms, sub_ms, st_ms, host_ms, flops, rechk, srechk = [], [], [], [], [], [], []
+    lab_ms, sub_ms, st_ms, host_ms, flops, rechk, srechk, evfrac = [], [], [], [], [], [], [], []
     t0 = time.perf_counter()
     for t in range(args.warmup + 1, args.warmup + args.steps + 1):
         kc = smp.num_clusters()
@@ -298,6 +298,7 @@ def main():
             rechk.append(dev.last_recheck_count())
             if not mult:
                 srechk.append(dev.last_sub_recheck_count())
+                evfrac.append(dev.last_label_evaluations())
         flops.append(2.0 * kc * D * (1 if mult else D) * N)
     dev.synchronize()
     torch.cuda.synchronize()
@@ -344,6 +345,7 @@ def main():
         "final_nmi_vs_truth": nmi,
         "recheck_fraction": (float(np.mean(rechk)) / N) if rechk else None,
         "sub_recheck_fraction": (float(np.mean(srechk)) / N) if srechk else None,
+        "label_evaluation_fraction": float(np.mean(evfrac)) if evfrac else None,
     }
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline and flip check (FP64 oracle)", file=sys.stderr, flush=True)

@@ -90,6 +90,8 @@ def load_library():
         "dpmm_set_mixed_tolerance": [_P, ctypes.c_float],
         "dpmm_last_recheck_count": [_P, _P],
         "dpmm_last_sub_recheck_count": [_P, _P],
+        "dpmm_set_label_pruning": [_P, i],
+        "dpmm_last_label_evaluations": [_P, _P],
         "dpmm_synchronize": [_P],
         "dpmm_set_prior": [_P, dbl, _P, dbl, _P, _P],
         "dpmm_set_statistics": [_P, i, _P, _P, _P],
@@ -346,6 +348,21 @@ class DeviceSweep:
         n = ctypes.c_int64()
         _check(load_library().dpmm_last_recheck_count(self._ctx, ctypes.byref(n)))
         return n.value
+
+    def set_label_pruning(self, mode):
+        """Exact pruning of the mixed label sweep: False / 0 off, "adaptive" / 1 (the default),
+        True / 2 always."""
+        if mode is True or mode is False:
+            m = 2 if mode else 0
+        else:
+            m = 1 if mode == "adaptive" else int(mode)
+        _check(load_library().dpmm_set_label_pruning(self._ctx, m))
+
+    def last_label_evaluations(self):
+        """Fraction of (wave, cluster) scorings the last pruned label sweep performed, -1 if none."""
+        v = ctypes.c_double()
+        _check(load_library().dpmm_last_label_evaluations(self._ctx, ctypes.byref(v)))
+        return v.value
 
     def last_sub_recheck_count(self):
         n = ctypes.c_int64()

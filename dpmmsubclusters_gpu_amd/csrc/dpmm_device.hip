@@ -147,8 +147,9 @@ __global__ void __launch_bounds__(kThreads) k_label_mult(const int32_t* __restri
 }
 
 // ---- deterministic partition of points by label -----------------------------------------------
+// clamp != 0 files labels outside [0, K) under label 0 instead of leaving them out.
 __global__ void __launch_bounds__(kThreads) k_block_hist(const int32_t* __restrict__ labels, int64_t N, int K,
-                                                         int nblk, int32_t* __restrict__ hist) {
+                                                         int nblk, int clamp, int32_t* __restrict__ hist) {
     __shared__ int cnt[kMaxK];
     for (int k = threadIdx.x; k < K; k += kThreads) cnt[k] = 0;
     __syncthreads();
@@ -156,7 +157,8 @@ __global__ void __launch_bounds__(kThreads) k_block_hist(const int32_t* __restri
     for (int r = 0; r < kBlockPts / kThreads; ++r) {
         const int64_t i = b0 + r * kThreads + threadIdx.x;
         if (i < N) {
-            const int L = labels[i];
+            int L = labels[i];
+            if (clamp && (unsigned)L >= (unsigned)K) L = 0;
             if ((unsigned)L < (unsigned)K) atomicAdd(&cnt[L], 1);   // out-of-range labels are left out
         }
     }
@@ -211,7 +213,7 @@ __global__ void k_scan_labels(const int32_t* __restrict__ rowtot, int K, int CH,
 // Stable scatter: within a block, points keep index order per label (wave ballot ranks,
 // waves in order, rounds in order).
 __global__ void __launch_bounds__(kThreads) k_scatter(const int32_t* __restrict__ labels, int64_t N, int K, int nblk,
-                                                      const int32_t* __restrict__ hist,
+                                                      int clamp, const int32_t* __restrict__ hist,
                                                       const int32_t* __restrict__ label_start,
                                                       int32_t* __restrict__ perm) {
     __shared__ int base[kMaxK];
@@ -224,7 +226,8 @@ __global__ void __launch_bounds__(kThreads) k_scatter(const int32_t* __restrict_
         for (int e = threadIdx.x; e < 4 * K; e += kThreads) wc[e / K][e % K] = 0;
         __syncthreads();
         const int64_t i = b0 + r * kThreads + threadIdx.x;
-        const int Li = i < N ? labels[i] : -1;
+        int Li = i < N ? labels[i] : -1;
+        if (clamp && i < N && (unsigned)Li >= (unsigned)K) Li = 0;
         const bool valid = (unsigned)Li < (unsigned)K;
         const int L = valid ? Li : -1;
         uint64_t todo = __ballot(valid);
@@ -400,6 +403,140 @@ __global__ void __launch_bounds__(kThreads) k_relabel(int32_t* __restrict__ labe
     subs[i] = ns;
 }
 
+// Bounds for the pruned mixed label sweep (k_label_gauss_mxp), from the FP32 slots of the K clusters
+// (their values are exact in FP64): psc[2k] = sig_k >= ||W_k||_2 + g ||W_k||_F, psc[2k + 1] = c_k and
+// pm[a K + k] <= ||W_k (mu_a - mu_k)|| - g ||W_k||_F ||mu_a - mu_k||, with g = (D + 2) u / (1 - (D + 2) u),
+// u = 2^-24, bounding the FP32 sweep's rounding of W (x - mu) by g ||W||_F ||x - mu||.
+// ||W||_2^2 = lambda_max(W^T W): a power iteration gives an estimate s2 from below, and t = f s2
+// (f = 1.05, 1.2, 1.5, 3 in turn) is certified by an FP64 Cholesky of t I - W^T W (its success bounds
+// lambda_max by t plus the factorisation's backward error, added); without a certificate the bound
+// ||W||_1 ||W||_inf is kept.  One workgroup per cluster k; LDS: W (D x D floats),
+// then one FP64 difference vector per wave, two power-iteration vectors and a packed D x D matrix.
+__global__ void __launch_bounds__(kThreads) k_prune_prep(const float* __restrict__ P, int K, int D, int DP,
+                                                         float* __restrict__ pm, float* __restrict__ psc) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int DS = D + 1;   // padded row stride: rows of W start in different LDS banks
+    float* W = reinterpret_cast<float*>(smem);
+    double* diff = reinterpret_cast<double*>(smem + (((size_t)D * DS * 4 + 15) & ~(size_t)15));
+    double* v = diff + 4 * D;
+    double* uu = v + D;
+    double* M = uu + D;
+    __shared__ double red[4], red_c[4], red_r[4];
+    __shared__ double sh[2];
+    const int k = blockIdx.x;
+    const int stride = gauss_slot_stride(DP);
+    const float* slot = P + (size_t)k * stride;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int e = threadIdx.x; e < D * D; e += kThreads) {
+        const int j = e / D, l = e - j * D;
+        W[j * DS + l] = l >= j ? slot[tri_row_offset(DP, j) - (j & ~3) + l] : 0.f;
+    }
+    __syncthreads();
+    // fallback bound ||W||_2^2 <= ||W||_1 ||W||_inf, and ||W||_F^2
+    double f2 = 0.0, c1 = 0.0, r1 = 0.0;
+    for (int i = threadIdx.x; i < D; i += kThreads) {
+        double cs = 0.0, rw = 0.0;
+        for (int j = 0; j < D; ++j) {
+            cs += fabs((double)W[j * DS + i]);
+            rw += fabs((double)W[i * DS + j]);
+            f2 = fma((double)W[i * DS + j], (double)W[i * DS + j], f2);
+        }
+        c1 = fmax(c1, cs);
+        r1 = fmax(r1, rw);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        c1 = fmax(c1, __shfl_xor(c1, o));
+        r1 = fmax(r1, __shfl_xor(r1, o));
+    }
+    if (lane == 0) {
+        red_c[wv] = c1;
+        red_r[wv] = r1;
+    }
+    f2 = block_sum(f2, red);   // synchronises, so the maxima above are visible too
+    c1 = fmax(fmax(red_c[0], red_c[1]), fmax(red_c[2], red_c[3]));
+    r1 = fmax(fmax(red_r[0], red_r[1]), fmax(red_r[2], red_r[3]));
+    double sig2 = c1 * r1 * (1.0 + 1e-12);
+    // power iteration on W^T W (u = W v, v <- W^T u / ||W^T u||, s2 = ||W v||^2 for unit v): TPR
+    // threads per row, each summing every TPR-th term, combined by butterfly shuffles
+    const int TPR = D > 64 ? 2 : (D > 32 ? 4 : 8);
+    const int row = threadIdx.x / TPR, part = threadIdx.x % TPR;
+    {
+        const double v0 = row < D && part == 0 ? 1.0 + 0.125 * (row % 7) : 0.0;
+        const double nv = block_sum(v0 * v0, red);
+        if (row < D && part == 0) v[row] = v0 / sqrt(nv);
+    }
+    __syncthreads();
+    double s2 = 0.0;
+    for (int itp = 0; itp < 14; ++itp) {
+        double uj = 0.0;
+        if (row < D)
+            for (int l = row + part; l < D; l += TPR) uj = fma((double)W[row * DS + l], v[l], uj);
+        for (int o = 1; o < TPR; o <<= 1) uj += __shfl_xor(uj, o);
+        if (row < D && part == 0) uu[row] = uj;
+        s2 = block_sum(row < D && part == 0 ? uj * uj : 0.0, red);   // also publishes uu
+        double zi = 0.0;
+        if (row < D)
+            for (int j = part; j <= row; j += TPR) zi = fma((double)W[j * DS + row], uu[j], zi);
+        for (int o = 1; o < TPR; o <<= 1) zi += __shfl_xor(zi, o);
+        const double nz = block_sum(row < D && part == 0 ? zi * zi : 0.0, red);
+        if (row < D && part == 0 && nz > 0.0) v[row] = zi / sqrt(nz);
+        __syncthreads();
+    }
+    const double fac[4] = {1.05, 1.2, 1.5, 3.0};
+    for (int tr = 0; tr < 4; ++tr) {   // uniform: s2 and sig2 are the same in every thread
+        const double tt = s2 * fac[tr];
+        if (!(tt > 0.0) || tt >= sig2) break;
+        const int T = D * (D + 1) / 2;
+        for (int e = threadIdx.x; e < T; e += kThreads) {
+            int i, j;
+            tri_ij(e, i, j);   // j <= i
+            double lam = 0.0;
+            for (int l = 0; l <= j; ++l) lam = fma((double)W[l * DS + i], (double)W[l * DS + j], lam);
+            M[e] = (i == j ? tt : 0.0) - lam;
+        }
+        __syncthreads();
+        if (chol_packed(M, D, sh)) {
+            sig2 = tt + 1e-12 * (f2 + (double)D * tt);
+            break;
+        }
+    }
+    const double u = 0x1p-24;
+    const double gam = (D + 2) * u / (1.0 - (D + 2) * u);
+    const double F = sqrt(f2) * (1.0 + 1e-12);
+    if (threadIdx.x == 0) {
+        psc[2 * k] = __double2float_ru(sqrt(sig2) * (1.0 + 1e-12) + gam * F);
+        psc[2 * k + 1] = slot[gauss_c_offset(DP)];
+    }
+    const float* muk = slot + gauss_mu_offset(DP);
+    double* dv = diff + (size_t)wv * D;
+    for (int a0 = 0; a0 < K; a0 += 4) {   // the same trip count in every wave: barriers stay uniform
+        const int a = a0 + wv;
+        if (a < K) {
+            const float* mua = P + (size_t)a * stride + gauss_mu_offset(DP);
+            for (int l = lane; l < D; l += 64) dv[l] = (double)mua[l] - (double)muk[l];
+        }
+        __syncthreads();
+        if (a < K) {
+            double sy = 0.0, se = 0.0;
+            for (int j = lane; j < D; j += 64) {
+                double y = 0.0;
+                for (int l = j; l < D; ++l) y = fma((double)W[j * DS + l], dv[l], y);
+                sy = fma(y, y, sy);
+                se = fma(dv[j], dv[j], se);
+            }
+            for (int o = 32; o > 0; o >>= 1) {
+                sy += __shfl_xor(sy, o);
+                se += __shfl_xor(se, o);
+            }
+            if (lane == 0) {
+                const double m = sqrt(sy) * (1.0 - 1e-12) - gam * F * sqrt(se) * (1.0 + 1e-12);
+                pm[(size_t)a * K + k] = a == k ? 0.f : __double2float_rd(fmax(m, 0.0));
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // ============================================================================ context ======
 
 struct dpmm_ctx {
@@ -448,6 +585,21 @@ struct dpmm_ctx {
     int32_t* dRecheck = nullptr;
     int* dRecheckCount = nullptr;
     float tau = 1.0f / 32.0f;
+    // exact pruning of the mixed Gaussian label sweep: bounds of the current parameters
+    int prune_mode = 1;           // 0 off, 1 adaptive, 2 always
+    bool prune_valid = false;
+    int prune_skip = 0;           // adaptive: unpruned sweeps left before pruning is tried again
+    unsigned long long* hEvals = nullptr;   // pinned copy of dEvals, read at the next sweep
+    hipEvent_t evEvals = nullptr;
+    bool evals_pending = false;
+    int evals_K = 0;
+    float* dPruneM = nullptr;     // K x K: row a, column k = lower bound of ||W_k (mu_a - mu_k)||
+    size_t pruneMCap = 0;
+    float* dPruneSC = nullptr;    // 2K: sigma_k, c_k
+    size_t pruneSCCap = 0;
+    unsigned long long* dEvals = nullptr;   // cluster evaluations, wave passes of the last pruned sweep
+    size_t evalsCap = 0;
+    int last_pruned_K = 0;        // K of the last pruned sweep, 0 if the last sweep was not pruned
     // dStats holds the statistics of the current labels and sub-labels, for Kpartition clusters
     bool stats_cached = false;
     int Kstats = 0;   // clusters whose statistics dStats holds (entries 0 .. 2 Kstats - 1)
@@ -533,16 +685,16 @@ int chunk_size(const dpmm_ctx* c) {
 
 int chunks_upper(const dpmm_ctx* c, int K) { return (int)((c->N + c->CH - 1) / c->CH) + K; }
 
-int build_partition(dpmm_ctx* c, int K) {
+int build_partition(dpmm_ctx* c, int K, int clamp) {
     int rc = ensure_partition(c, K);
     if (rc) return rc;
     c->CH = chunk_size(c);
-    hipLaunchKernelGGL(k_block_hist, dim3(c->nblk), dim3(kThreads), 0, c->stream, c->dLabels, c->N, K, c->nblk,
+    hipLaunchKernelGGL(k_block_hist, dim3(c->nblk), dim3(kThreads), 0, c->stream, c->dLabels, c->N, K, c->nblk, clamp,
                        c->dHist);
     hipLaunchKernelGGL(k_scan_rows, dim3(K), dim3(kThreads), 0, c->stream, c->dHist, c->nblk, c->dRowtot);
     hipLaunchKernelGGL(k_scan_labels, dim3(1), dim3(64), 0, c->stream, c->dRowtot, K, c->CH, c->dLabelStart,
                        c->dChunkStart);
-    hipLaunchKernelGGL(k_scatter, dim3(c->nblk), dim3(kThreads), 0, c->stream, c->dLabels, c->N, K, c->nblk,
+    hipLaunchKernelGGL(k_scatter, dim3(c->nblk), dim3(kThreads), 0, c->stream, c->dLabels, c->N, K, c->nblk, clamp,
                        c->dHist, c->dLabelStart, c->dPerm);
     HIPCHK(hipGetLastError());
     c->partition_valid = true;
@@ -792,6 +944,7 @@ int set_gauss(dpmm_ctx* c, int K, const double* mu, const double* mat, const dou
     HIPCHK(hipMemcpyAsync(c->dSP, c->hSP.data(), c->hSP.size(), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     c->K = K;
+    c->prune_valid = false;
     return DPMM_OK;
 }
 
@@ -910,6 +1063,55 @@ int run_label_mixed(dpmm_ctx* c, uint32_t iter, int fin) {
     return DPMM_OK;
 }
 
+int allow_lds(const void* kernel, size_t bytes);
+
+// Pruning bounds of the current FP32 slots (once per parameter set).
+int ensure_prune(dpmm_ctx* c) {
+    if (c->prune_valid) return DPMM_OK;
+    const int K = c->K, D = c->D;
+    int rc = grow(c->dPruneM, c->pruneMCap, (size_t)K * K);
+    if (rc) return rc;
+    if ((rc = grow(c->dPruneSC, c->pruneSCCap, (size_t)2 * K))) return rc;
+    const size_t lds = (((size_t)D * (D + 1) * 4 + 15) & ~(size_t)15) + ((size_t)6 * D + (size_t)D * (D + 1) / 2) * 8;
+    if ((rc = allow_lds(reinterpret_cast<const void*>(k_prune_prep), lds))) return rc;
+    hipLaunchKernelGGL(k_prune_prep, dim3(K), dim3(kThreads), lds, c->stream, reinterpret_cast<const float*>(c->dP),
+                       K, D, c->DP, c->dPruneM, c->dPruneSC);
+    HIPCHK(hipGetLastError());
+    c->prune_valid = true;
+    return DPMM_OK;
+}
+
+// Mixed sweep with exact pruning: partition by the labels before the sweep (labels outside [0, K)
+// filed under 0), the pruned BF16 screen over the partition's chunks, then the FP32 sweep over the
+// points it could not decide.
+int run_label_mixed_pruned(dpmm_ctx* c, uint32_t iter, int fin) {
+    int rc = ensure_prune(c);
+    if (rc) return rc;
+    if ((rc = build_partition(c, c->K, 1))) return rc;
+    if ((rc = grow(c->dEvals, c->evalsCap, 2))) return rc;
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, c->dRecheckCount, 1);
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, reinterpret_cast<int*>(c->dEvals), 4);
+    const int grid = chunks_upper(c, c->K) * kPruneSplit;
+    const float* X = reinterpret_cast<const float*>(c->dX);
+    const unsigned char* PB = reinterpret_cast<const unsigned char*>(c->dPB);
+#define DPMM_MXP(DPMV)                                                                                          \
+    launch_label_gauss_mxp<DPMV>(c->stream, grid, X, c->D, c->K, c->CH, c->dPerm, c->dLabelStart, c->dChunkStart, \
+                                 PB, c->dPruneM, c->dPruneSC, c->tau, c->key0(), c->key1(), iter, fin, c->dLabels, \
+                                 c->dRecheck, c->dRecheckCount, c->dEvals)
+    switch (c->DPM) {
+        case 32: DPMM_MXP(32); break;
+        case 64: DPMM_MXP(64); break;
+        case 96: DPMM_MXP(96); break;
+        case 128: DPMM_MXP(128); break;
+        default: return fail(DPMM_EINVAL, "mixed sweep needs D <= 128");
+    }
+#undef DPMM_MXP
+    const int grid2 = (int)std::min<int64_t>(1024, (c->N + kThreads - 1) / kThreads);
+    DPMM_DISPATCH_DP(c->DP, lgl_f32, c, iter, fin, grid2);
+    c->last_pruned_K = c->K;
+    return DPMM_OK;
+}
+
 template <int KPT>
 void launch_label_mult_k(dpmm_ctx* c, uint32_t iter, int fin) {
     const int grid = (int)((c->N + 63) / 64);
@@ -991,9 +1193,12 @@ int dpmm_destroy(dpmm_ctx* c) {
                     (void*)c->dRowtot, (void*)c->dLabelStart, (void*)c->dChunkStart, (void*)c->dTable,
                     (void*)c->dPartials, (void*)c->dStats, c->dPB, (void*)c->dRecheck, (void*)c->dRecheckCount,
                     (void*)c->dPrior, (void*)c->dMembers, (void*)c->dLM, (void*)c->dDraws, (void*)c->dDrawStatus,
-                    (void*)c->dLogW, (void*)c->dFeat, (void*)c->dMB, c->dSPB})
+                    (void*)c->dLogW, (void*)c->dFeat, (void*)c->dMB, c->dSPB, (void*)c->dPruneM,
+                    (void*)c->dPruneSC, (void*)c->dEvals})
         if (p) (void)hipFree(p);
     if (c->hStats) (void)hipHostFree(c->hStats);
+    if (c->hEvals) (void)hipHostFree(c->hEvals);
+    if (c->evEvals) (void)hipEventDestroy(c->evEvals);
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1111,6 +1316,7 @@ int dpmm_set_multinomial_params(dpmm_ctx* c, int K, const double* logtheta, cons
     }
     HIPCHK(hipStreamSynchronize(c->stream));
     c->K = K;
+    c->prune_valid = false;
     return DPMM_OK;
 }
 
@@ -1118,9 +1324,38 @@ int dpmm_sample_cluster_labels(dpmm_ctx* c, uint32_t iter, int fin) {
     if (!c) return fail(DPMM_EINVAL, "null ctx");
     if (c->K <= 0) return fail(DPMM_ESTATE, "parameters not set");
     (void)hipEventRecord(c->ev[0], c->stream);
+    c->last_pruned_K = 0;
     if (c->prior == DPMM_PRIOR_GAUSSIAN && c->precision == DPMM_PREC_MIXED) {
-        int rc = run_label_mixed(c, iter, fin);
+        bool prune = c->prune_mode != 0 && c->K >= 2 && c->D % 4 == 0;   // 16-byte rows for the gathers
+        // adaptive: small sweeps (N K DP^2 below 4e11 multiply-adds) stay unpruned, the fixed cost
+        // of the bounds and the partition would exceed what pruning saves
+        if (prune && c->prune_mode == 1 && (double)c->N * c->K * c->DP * c->DP < 4e11) prune = false;
+        if (prune && c->prune_mode == 1) {
+            // adaptive: when the previous pruned sweep still scored more than a quarter of the
+            // (wave, cluster) pairs, the next four sweeps run unpruned
+            if (c->evals_pending) {
+                HIPCHK(hipEventSynchronize(c->evEvals));
+                c->evals_pending = false;
+                const double f = c->hEvals[1] ? (double)c->hEvals[0] / ((double)c->hEvals[1] * c->evals_K) : 0.0;
+                if (f > 0.25) c->prune_skip = 4;
+            }
+            if (c->prune_skip > 0) {
+                --c->prune_skip;
+                prune = false;
+            }
+        }
+        int rc = prune ? run_label_mixed_pruned(c, iter, fin) : run_label_mixed(c, iter, fin);
         if (rc) return rc;
+        if (prune && c->prune_mode == 1) {
+            if (!c->hEvals && hipHostMalloc((void**)&c->hEvals, 2 * sizeof(unsigned long long)) != hipSuccess)
+                return fail(DPMM_ENOMEM, "pinned");
+            if (!c->evEvals) HIPCHK(hipEventCreateWithFlags(&c->evEvals, hipEventDisableTiming));
+            HIPCHK(hipMemcpyAsync(c->hEvals, c->dEvals, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                  c->stream));
+            HIPCHK(hipEventRecord(c->evEvals, c->stream));
+            c->evals_pending = true;
+            c->evals_K = c->K;
+        }
     } else if (c->prior == DPMM_PRIOR_GAUSSIAN) {
         if (c->params_f64) { DPMM_DISPATCH_DP64(c->DP, lg_f64, c, iter, fin); }
         else { DPMM_DISPATCH_DP(c->DP, lg_f32, c, iter, fin); }
@@ -1141,7 +1376,7 @@ int dpmm_sample_sub_labels(dpmm_ctx* c, uint32_t iter) {
     if (!c) return fail(DPMM_EINVAL, "null ctx");
     if (c->K <= 0) return fail(DPMM_ESTATE, "parameters not set");
     if (!c->partition_valid || c->Kpartition != c->K) {
-        int rc = build_partition(c, c->K);
+        int rc = build_partition(c, c->K, 0);
         if (rc) return rc;
     }
     (void)hipEventRecord(c->ev[2], c->stream);
@@ -1213,7 +1448,7 @@ int dpmm_sufficient_statistics(dpmm_ctx* c, int K, int64_t* Nsub, double* sum, d
     if (!c || !Nsub) return fail(DPMM_EINVAL, "bad args");
     if (K <= 0 || K > kMaxK) return fail(DPMM_EINVAL, "bad K");
     if (!c->partition_valid || c->Kpartition != K) {
-        int rc = build_partition(c, K);
+        int rc = build_partition(c, K, 0);
         if (rc) return rc;
     }
     const size_t E = c->statEntries();
@@ -1313,6 +1548,24 @@ int dpmm_last_recheck_count(dpmm_ctx* c, int64_t* n) {
     HIPCHK(hipStreamSynchronize(c->stream));
     HIPCHK(hipMemcpy(&v, c->dRecheckCount, sizeof(int), hipMemcpyDeviceToHost));
     *n = v;
+    return DPMM_OK;
+}
+
+int dpmm_set_label_pruning(dpmm_ctx* c, int mode) {
+    if (!c || mode < 0 || mode > 2) return fail(DPMM_EINVAL, "pruning mode is 0, 1 or 2");
+    c->prune_mode = mode;
+    c->prune_skip = 0;
+    return DPMM_OK;
+}
+
+int dpmm_last_label_evaluations(dpmm_ctx* c, double* fraction) {
+    if (!c || !fraction) return fail(DPMM_EINVAL, "bad args");
+    *fraction = -1.0;
+    if (c->last_pruned_K <= 0 || !c->dEvals) return DPMM_OK;
+    unsigned long long v[2] = {0, 0};
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(v, c->dEvals, sizeof(v), hipMemcpyDeviceToHost));
+    if (v[1] > 0) *fraction = (double)v[0] / ((double)v[1] * c->last_pruned_K);
     return DPMM_OK;
 }
 
@@ -1524,6 +1777,7 @@ int dpmm_sample_params(dpmm_ctx* c, uint32_t iter, int K, const int32_t* members
     for (int p = 0; p < np; ++p)
         if (st[p]) return fail(DPMM_ENOTSPD, "posterior draw failed: a factor is not SPD");
     c->K = K;
+    c->prune_valid = false;
     return DPMM_OK;
 }
 

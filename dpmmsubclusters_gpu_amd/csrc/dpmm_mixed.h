@@ -22,6 +22,29 @@ namespace dpmm {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef __attribute__((address_space(3))) void lds_void;
+typedef __attribute__((address_space(1))) const float gfloat;
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) const f32x4 gfloat4;
+typedef __attribute__((address_space(1))) const void glob_cvoid;
+
+// Copy of `bytes` (a multiple of 16) from global memory into LDS by the wave-level LDS DMA
+// (global_load_lds_dwordx4): wave w of the workgroup moves 1 KB chunks w, w + 4, ..., lane L the 16
+// bytes at L * 16 of its chunk, straight into LDS with no register round trip.  The copy is
+// asynchronous: the caller waits for it with lds_dma_wait() before the barrier that publishes it.
+__device__ __forceinline__ void lds_dma_copy(unsigned char* dst, const unsigned char* src, int bytes, int wv,
+                                             int lane) {
+    const int units = bytes >> 4;
+    for (int ch = wv; ch * 64 < units; ch += kThreads / 64) {
+        const int e = ch * 64 + lane;
+        if (e < units)
+            __builtin_amdgcn_global_load_lds((glob_cvoid*)(src + (size_t)e * 16), (lds_void*)(dst + ch * 1024), 16, 0,
+                                             0);
+    }
+}
+// vmcnt(0), expcnt and lgkmcnt left at their maxima: every outstanding vector memory load of the wave,
+// the LDS DMA copies included, has completed.
+__device__ __forceinline__ void lds_dma_wait() { __builtin_amdgcn_s_waitcnt(0x0f70); }
 
 // Slot of one cluster for the screen: [tiles RB x KC x 64 lanes x 8 bf16 | mu (DP f32) | c | pad].
 __host__ __device__ constexpr int mx_tiles_bytes(int DP) { return (DP / 32) * (DP / 16) * 64 * 16; }
@@ -77,12 +100,10 @@ k_label_gauss_mx(const float* __restrict__ X, int64_t N, int D, int K, const uns
     int bi = 0;
     Ctr4 w = {};
 
-    auto stage = [&](int k, int buf) {
-        const uint4* src = reinterpret_cast<const uint4*>(PB + (size_t)k * SLOT);
-        uint4* dst = reinterpret_cast<uint4*>(sp[buf]);
-        for (int e = threadIdx.x; e < SLOT / 16; e += kThreads) dst[e] = src[e];
-    };
+    // cluster k + 1's slot streams into the other LDS buffer while cluster k is scored
+    auto stage = [&](int k, int buf) { lds_dma_copy(sp[buf], PB + (size_t)k * SLOT, SLOT, wv, lane); };
     stage(0, 0);
+    lds_dma_wait();
     __syncthreads();
     for (int k = 0; k < K; ++k) {
         if (k + 1 < K) stage(k + 1, (k + 1) & 1);
@@ -128,6 +149,7 @@ k_label_gauss_mx(const float* __restrict__ X, int64_t N, int D, int K, const uns
         } else {
             U = fmaxf(U, s + err);
         }
+        lds_dma_wait();
         __syncthreads();
     }
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -272,18 +294,233 @@ void launch_sub_gauss_mx(hipStream_t st, int grid, const float* X, int D, int K,
                        chunk_start, SPB, tau, key0, key1, iter, subs, recheck, recheck_count);
 }
 
+// Exact pruning of the mixed label sweep.  The points are visited in the label partition of the
+// previous labels: one chunk holds points of one anchor cluster a, and a wave takes 32 of them.
+// After screening the anchor, the wave knows r >= ||x - mu_a|| for its points and T, the smallest
+// screened lower bound s_a - e_a of their anchor scores.  For any other cluster k,
+//   ||W_k (x - mu_k)|| >= ||W_k (mu_a - mu_k)|| - ||W_k||_2 r   (triangle inequality),
+// and with m_ak <= ||W_k (mu_a - mu_k)|| and sig_k >= ||W_k||_2, both reduced / raised by the FP32
+// sweep's rounding of W (x - mu) (k_prune_prep), the FP32 sweep's q_k is at least 0.98 (m_ak -
+// sig_k r)^2.  A 32-bit uniform's Gumbel noise is below 22.9, so
+//   ub_k = c_k - 0.49 max(0, m_ak - sig_k r)^2 + 1e-6 |c_k| + kPruneNoise
+// bounds the FP32 sweep's noisy score of cluster k for every point of the wave.  When ub_k < T no
+// point can take label k and the cluster is skipped (ub_k joins the runner-up bound U); the other
+// clusters are screened as in k_label_gauss_mx, so the decided labels are the FP32 sweep's.
+constexpr float kPruneNoise = 24.0f;
+
+// Workgroups per partition chunk in the pruned sweep (each takes an equal share of the chunk).
+constexpr int kPruneSplit = 4;
+
+template <int DP>
+__global__ void __launch_bounds__(kThreads, DP <= 64 ? 3 : 2)
+k_label_gauss_mxp(const float* __restrict__ X, int D, int K, int CH, const int32_t* __restrict__ perm,
+                  const int32_t* __restrict__ label_start, const int32_t* __restrict__ chunk_start,
+                  const unsigned char* __restrict__ PB, const float* __restrict__ pm,
+                  const float* __restrict__ psc, float tau, uint32_t key0, uint32_t key1, uint32_t iter,
+                  int final_iter, int32_t* __restrict__ labels, int32_t* __restrict__ recheck,
+                  int* __restrict__ recheck_count, unsigned long long* __restrict__ evals) {
+    constexpr int RB = DP / 32, KC = DP / 16;
+    constexpr int SLOT = mx_slot_bytes(DP);
+    // one 32-point block per wave: lane L scores point L % 32 on coordinate half L / 32
+    const int lane = threadIdx.x & 63, h = lane >> 5, pl = lane & 31, wv = threadIdx.x >> 6;
+    int a, cbeg, cend;
+    if (!chunk_of(blockIdx.x / kPruneSplit, K, chunk_start, label_start, CH, a, cbeg, cend)) return;
+    const int part = blockIdx.x % kPruneSplit, len = cend - cbeg;
+    const int beg = cbeg + (int)((int64_t)len * part / kPruneSplit);
+    const int end = cbeg + (int)((int64_t)len * (part + 1) / kPruneSplit);
+    const float* mrow = pm + (size_t)a * K;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    // the anchor's bound inputs of clusters 0..63, one per lane (the same for every pass)
+    const bool in0 = lane < K && lane != a;
+    const float sg0 = in0 ? psc[2 * lane] : 0.f, ck0 = in0 ? psc[2 * lane + 1] : 0.f, m0 = in0 ? mrow[lane] : 0.f;
+    unsigned long long nev = 0, npass = 0;
+    int idx_next = (beg + wv * 32 + pl < end) ? perm[beg + wv * 32 + pl] : 0;
+    for (int p0 = beg + wv * 32; p0 < end; p0 += 4 * 32) {   // wave-uniform trip count
+        const int pos = p0 + pl;
+        const bool val = pos < end;
+        const int idx = val ? idx_next : 0;
+        idx_next = (pos + 128 < end) ? perm[pos + 128] : 0;   // the next pass's point, loaded ahead
+        const bool own_valid = val && h == 0;
+        // the point's coordinates 16 kc + 8 h .. + 7: unconditional 16-byte loads (D is a multiple of
+        // 4, so rows and these offsets are 16-byte aligned) from addresses that always exist (row 0
+        // for an empty lane, offset 0 past D), zeroed where they lie outside the point
+        auto load_x = [&](const gfloat* Xs, float (&x)[KC][8]) {
+            const gfloat* row = Xs + (int64_t)idx * D;
+#pragma unroll
+            for (int kc = 0; kc < KC; ++kc) {
+                const int off = 16 * kc + 8 * h;
+                const bool in_lo = val && off + 4 <= D, in_hi = val && off + 8 <= D;
+                const f32x4 lo = *reinterpret_cast<const gfloat4*>(row + (in_lo ? off : 0));
+                const f32x4 hi = *reinterpret_cast<const gfloat4*>(row + (in_hi ? off + 4 : 0));
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    x[kc][e] = in_lo ? lo[e] : 0.f;
+                    x[kc][e + 4] = in_hi ? hi[e] : 0.f;
+                }
+            }
+        };
+        // up to 32 dimensions the coordinates stay in registers for the whole pass; wider points are
+        // read again by each scoring (from cache after the anchor's), which keeps the registers for
+        // the MFMA operands
+        constexpr bool kHoldX = DP <= 32;
+        float xh[kHoldX ? KC : 1][8];
+        if constexpr (kHoldX) load_x((const gfloat*)X, xh);
+        // screened q of cluster k for this lane's point, and sum (x - mu_k)^2
+        auto score = [&](int k, float& d2) -> float {
+            float xr[kHoldX ? 1 : KC][8];
+            if constexpr (!kHoldX) {
+                const gfloat* Xs = (const gfloat*)X;
+                asm volatile("" : "+s"(Xs));   // a load per scoring, not one hoisted out of the cluster loop
+                load_x(Xs, xr);
+            }
+            auto& x = [&]() -> float(&)[KC][8] {
+                if constexpr (kHoldX) return xh; else return xr;
+            }();
+            const unsigned char* slot = PB + (size_t)k * SLOT;
+            const bf16x8* tiles = reinterpret_cast<const bf16x8*>(slot);
+            const float4* mu4 = reinterpret_cast<const float4*>(slot + mx_tiles_bytes(DP));
+            bf16x8 Bv[KC];
+            float dd = 0.f;
+#pragma unroll
+            for (int kc = 0; kc < KC; ++kc) {
+                const float4 ma = mu4[4 * kc + 2 * h], mb = mu4[4 * kc + 2 * h + 1];
+                const float mv[8] = {ma.x, ma.y, ma.z, ma.w, mb.x, mb.y, mb.z, mb.w};
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const float d = x[kc][e] - mv[e];
+                    Bv[kc][e] = (__bf16)d;
+                    dd = fmaf(d, d, dd);
+                }
+            }
+            float q = 0.f;
+#pragma unroll
+            for (int rb = 0; rb < RB; ++rb) {
+                f32x16 acc = {};
+#pragma unroll
+                for (int kc = 2 * rb; kc < KC; ++kc)
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(tiles[(rb * KC + kc) * 64 + lane], Bv[kc], acc, 0, 0,
+                                                                  0);
+#pragma unroll
+                for (int v = 0; v < 16; ++v) q = fmaf(acc[v], acc[v], q);
+            }
+            d2 = dd + __shfl_xor(dd, 32);
+            return q + __shfl_xor(q, 32);
+        };
+        auto noisy = [&](int k, float q) -> float {
+            const float c = reinterpret_cast<const float*>(PB + (size_t)k * SLOT + mx_tiles_bytes(DP))[DP];
+            float s = c - 0.5f * q;
+            if (!final_iter) {
+                const Ctr4 w = philox4x32_10((uint32_t)idx, (uint32_t)(k >> 2), iter, TAG_LABEL, key0, key1);
+                s += gumbel_screen(sel4(w, k & 3));
+            }
+            return s;
+        };
+        // the anchor, then the wave bounds: r >= ||x - mu_a|| over the points (the FP32 rounding of
+        // the distance covered) and T = the smallest screened lower bound s_a - e_a
+        float d2 = 0.f;
+        const float qa = score(a, d2);
+        float sb = noisy(a, qa), eb = tau * qa + kNoiseSlack, U = neg_inf<float>();
+        int bi = a;
+        float rr = own_valid ? sqrtf(d2) : 0.f;
+        float tt = own_valid ? sb - eb : __builtin_huge_valf();
+        for (int o = 32; o > 0; o >>= 1) {
+            rr = fmaxf(rr, __shfl_xor(rr, o));
+            tt = fminf(tt, __shfl_xor(tt, o));
+        }
+        const float rad = rr * (1.0f + (float)(D + 8) * 0x1p-23f);
+        const float T = tt;
+        unsigned long long ne = 1;
+        // 64 clusters per step, one per lane: prune test, then the survivors in index order
+        for (int k0 = 0; k0 < K; k0 += 64) {
+            const int kk = k0 + lane;
+            float ub = neg_inf<float>();
+            bool keep = false;
+            if (kk < K && kk != a) {
+                const float sg = k0 == 0 ? sg0 : psc[2 * kk], ck = k0 == 0 ? ck0 : psc[2 * kk + 1];
+                const float lb = fmaxf(0.f, (k0 == 0 ? m0 : mrow[kk]) - sg * rad);
+                ub = ck - 0.49f * lb * lb + 1e-6f * fabsf(ck) + kPruneNoise;
+                keep = !(ub < T);
+            }
+            float upr = keep ? neg_inf<float>() : ub;   // bound of the skipped clusters
+            for (int o = 32; o > 0; o >>= 1) upr = fmaxf(upr, __shfl_xor(upr, o));
+            U = fmaxf(U, upr);
+            uint64_t todo = __ballot(keep);
+            while (todo) {
+                const int k = k0 + __ffsll((unsigned long long)todo) - 1;
+                todo &= todo - 1;
+                ++ne;
+                float unused;
+                const float qk = score(k, unused);
+                const float s = noisy(k, qk);
+                const float err = tau * qk + kNoiseSlack;
+                if (s > sb) {
+                    U = fmaxf(U, sb + eb);
+                    sb = s; eb = err; bi = k;
+                } else {
+                    U = fmaxf(U, s + err);
+                }
+            }
+        }
+        nev += ne;
+        ++npass;
+        const bool sure = sb - eb > U;
+        if (own_valid && sure) labels[idx] = bi;
+        const bool need = own_valid && !sure;
+        const uint64_t m = __ballot(need);
+        if (m) {
+            int pos0 = 0;
+            if (lane == 0) pos0 = atomicAdd(recheck_count, __popcll(m));   // one atomic per wave
+            pos0 = __shfl(pos0, 0);
+            if (need) recheck[pos0 + __popcll(m & lt)] = (int32_t)idx;
+        }
+    }
+    __shared__ unsigned long long cnt[2][4];   // one pair of counter atomics per workgroup
+    if (lane == 0) {
+        cnt[0][wv] = nev;
+        cnt[1][wv] = npass;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned long long e = cnt[0][0] + cnt[0][1] + cnt[0][2] + cnt[0][3];
+        const unsigned long long pp = cnt[1][0] + cnt[1][1] + cnt[1][2] + cnt[1][3];
+        if (pp) {
+            atomicAdd(evals, e);
+            atomicAdd(evals + 1, pp);
+        }
+    }
+}
+
+template <int DP>
+void launch_label_gauss_mxp(hipStream_t st, int grid, const float* X, int D, int K, int CH, const int32_t* perm,
+                            const int32_t* label_start, const int32_t* chunk_start, const unsigned char* PB,
+                            const float* pm, const float* psc, float tau, uint32_t key0, uint32_t key1, uint32_t iter,
+                            int fin, int32_t* labels, int32_t* recheck, int* recheck_count,
+                            unsigned long long* evals) {
+    hipLaunchKernelGGL((k_label_gauss_mxp<DP>), dim3(grid), dim3(kThreads), 0, st, X, D, K, CH, perm, label_start,
+                       chunk_start, PB, pm, psc, tau, key0, key1, iter, fin, labels, recheck, recheck_count, evals);
+}
+
 #define DPMM_MX_INSTANTIATE(DP)                                                                               \
     template void launch_label_gauss_mx<DP>(hipStream_t, int64_t, int, int, const float*, const unsigned char*, \
                                             float, uint32_t, uint32_t, uint32_t, int, int32_t*, int32_t*, int*); \
     template void launch_sub_gauss_mx<DP>(hipStream_t, int, const float*, int, int, int, const int32_t*,       \
                                           const int32_t*, const int32_t*, const unsigned char*, float, uint32_t, \
-                                          uint32_t, uint32_t, int32_t*, int32_t*, int*);
+                                          uint32_t, uint32_t, int32_t*, int32_t*, int*);                       \
+    template void launch_label_gauss_mxp<DP>(hipStream_t, int, const float*, int, int, int, const int32_t*,   \
+                                             const int32_t*, const int32_t*, const unsigned char*, const float*, \
+                                             const float*, float, uint32_t, uint32_t, uint32_t, int, int32_t*,   \
+                                             int32_t*, int*, unsigned long long*);
 #define DPMM_MX_EXTERN(DP)                                                                                    \
     extern template void launch_label_gauss_mx<DP>(hipStream_t, int64_t, int, int, const float*,               \
                                                    const unsigned char*, float, uint32_t, uint32_t, uint32_t,  \
                                                    int, int32_t*, int32_t*, int*);                             \
     extern template void launch_sub_gauss_mx<DP>(hipStream_t, int, const float*, int, int, int, const int32_t*, \
                                                  const int32_t*, const int32_t*, const unsigned char*, float,   \
-                                                 uint32_t, uint32_t, uint32_t, int32_t*, int32_t*, int*);
+                                                 uint32_t, uint32_t, uint32_t, int32_t*, int32_t*, int*);       \
+    extern template void launch_label_gauss_mxp<DP>(hipStream_t, int, const float*, int, int, int,             \
+                                                    const int32_t*, const int32_t*, const int32_t*,            \
+                                                    const unsigned char*, const float*, const float*, float,   \
+                                                    uint32_t, uint32_t, uint32_t, int, int32_t*, int32_t*, int*, \
+                                                    unsigned long long*);
 
 }  // namespace dpmm

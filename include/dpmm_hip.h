@@ -147,6 +147,19 @@ int dpmm_last_recheck_count(dpmm_ctx* ctx, int64_t* n);
  * The mixed sub-label sweep screens s_left - s_right with split-BF16 MFMA and a per-point error
  * bound against the FP32 sweep's arithmetic; no counterpart in the reference. */
 int dpmm_last_sub_recheck_count(dpmm_ctx* ctx, int64_t* n);
+/* Exact pruning of the mixed Gaussian label sweep (on by default; no counterpart in the
+ * reference).  Points are visited grouped by their labels before the sweep; for a wave of points
+ * of cluster a, a cluster k is skipped when a bound from the triangle inequality in k's whitened
+ * metric (||W_k (mu_a - mu_k)|| - ||W_k||_2 max ||x - mu_a||) plus the largest Gumbel noise of a
+ * 32-bit uniform shows that no point of the wave can take label k in the FP32 sweep.  Labels are
+ * identical with pruning on or off.  Pruning needs D to be a multiple of 4 (other D sweep
+ * unpruned).  mode 0: off; 1 (default): adaptive, pruning only sweeps with
+ * N K DP^2 >= 4e11 (DP: D padded) and following a sweep that still scored more than a quarter of
+ * the (wave, cluster) pairs by four unpruned sweeps; 2: always. */
+int dpmm_set_label_pruning(dpmm_ctx* ctx, int mode);
+/* Fraction of (wave, cluster) scorings the last pruned label sweep performed (-1 if the last
+ * sweep was not pruned). */
+int dpmm_last_label_evaluations(dpmm_ctx* ctx, double* fraction);
 
 /* Kernel timings of the last calls (ms, HIP events on the library's stream). */
 int dpmm_last_kernel_ms(dpmm_ctx* ctx, float* label_sweep_ms, float* sub_sweep_ms, float* stats_ms);

@@ -56,6 +56,39 @@ def test_mixed_equals_fp32_and_oracle(D, K, close, final):
     assert not bad.any()
 
 
+@pytest.mark.parametrize("D,K", [(32, 16), (64, 20), (100, 8), (128, 6)])
+@pytest.mark.parametrize("final", [False, True])
+def test_pruned_sweep_equals_unpruned(D, K, final):
+    """Exact pruning: labels identical with pruning on and off, from the mixture's own labels (points
+    grouped by cluster, so most clusters are skipped) and from scrambled labels, some outside [0, K)."""
+    rng = np.random.default_rng(D * K)
+    N = 80_000
+    X, gt, _ = gaussian_mixture(N, D, K, seed=D + 2 * K, dtype=np.float32)
+    params = _fitted_params(X, gt, K, rng)
+    dev = dp.DeviceSweep(X, dp.prior_type.Gaussian, seed=5)
+    dev.set_precision(dp.Precision.MIXED)
+    dev.set_gaussian_params(*params)
+    for start in ("truth", "scrambled"):
+        before = gt.astype(np.int32) if start == "truth" else rng.integers(-2, 2 * K, N).astype(np.int32)
+        out, frac = {}, None
+        for prune in (False, True):
+            say(f"pruned sweep D={D} K={K} final={final} start={start} prune={prune}")
+            dev.set_label_pruning(prune)
+            dev.set_labels(before, None)
+            dev.sample_cluster_labels(6, final)
+            out[prune] = dev.get_labels()[0]
+            if prune:
+                frac = dev.last_label_evaluations()
+            else:
+                assert dev.last_label_evaluations() == -1.0
+        print(f"D={D} K={K} final={final} {start}: evaluated fraction {frac:.3f}")
+        assert (out[True] >= 0).all() and (out[True] < K).all()
+        np.testing.assert_array_equal(out[True], out[False])
+        if start == "truth":   # separated clusters: most of them are skipped
+            assert frac < 0.75
+    dev.close()
+
+
 def test_screen_alone_flip_rate():
     """tau = 0 keeps every screened label: the raw BF16 flip rate against the oracle, reported."""
     rng = np.random.default_rng(1)

@@ -151,8 +151,9 @@ __global__ void __launch_bounds__(kThreads) k_stats_gauss_mfma(const TX* __restr
     constexpr int PER = B * DP / kThreads;             // staged coordinates per thread
     constexpr int SXT = kThreads / (2 * DP) > 0 ? kThreads / (2 * DP) : 1;
     constexpr size_t E = 1 + DP + (size_t)DP * DP;
-    static_assert(B * DP % kThreads == 0, "batch must tile the workgroup");
-    __shared__ __align__(16) double xs[B * DP];
+    static_assert(B * DP % (4 * kThreads) == 0, "batch must tile the workgroup in groups of 4");
+    constexpr int XS = DP + 1;                         // padded row: the 4 points of an MFMA step
+    __shared__ __align__(16) double xs[(B + 1) * XS];  // start in different LDS banks; row B is zero
     __shared__ int idn[B];          // point indices of the batch being loaded
     __shared__ int sbb[B];          // sub-labels of the staged batch
     __shared__ int lst[2][B + 4];
@@ -184,23 +185,34 @@ __global__ void __launch_bounds__(kThreads) k_stats_gauss_mfma(const TX* __restr
     int64_t cnt0 = 0, cnt1 = 0;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 
+    for (int e = tid; e < XS; e += kThreads) xs[B * XS + e] = 0.0;   // the row that pads a group of 4
     // prefetch pipeline: indices two batches ahead, sub-labels and coordinates one batch ahead
     int id1 = (tid < B && beg + tid < end) ? perm[beg + tid] : -1;
     int id2 = (tid < B && beg + B + tid < end) ? perm[beg + B + tid] : -1;
     int sb1 = id1 >= 0 ? subs[id1] : -1;
     if (tid < B) idn[tid] = id1;
     __syncthreads();
-    TX xv[PER];
+    // thread tid stages coordinates 4 e4 .. 4 e4 + 3 of the batch (e4 = tid + kThreads j): point
+    // 4 e4 / DP, coordinates (4 e4) % DP onwards, loaded 4 at a time
+    TX xv[PER / 4][4];
+    auto load_batch = [&]() {
 #pragma unroll
-    for (int j = 0; j < PER; ++j) {
-        const int e = tid + kThreads * j, q = e / DP, l = e - q * DP;
-        const int id = idn[q];
-        xv[j] = (id >= 0 && l < D) ? X[(int64_t)id * D + l] : TX(0);
-    }
+        for (int j = 0; j < PER / 4; ++j) {
+            const int e = 4 * (tid + kThreads * j), q = e / DP, l = e - q * DP;
+            const int id = idn[q];
+            load_coords4(X + (int64_t)(id >= 0 ? id : 0) * D, l, D, id >= 0, xv[j]);
+        }
+    };
+    load_batch();
     for (int b0 = beg; b0 < end; b0 += B) {
         __syncthreads();   // the previous batch's readers of xs / idn are done
 #pragma unroll
-        for (int j = 0; j < PER; ++j) xs[tid + kThreads * j] = (double)xv[j];
+        for (int j = 0; j < PER / 4; ++j)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int f = 4 * (tid + kThreads * j), q = f / DP, l = f - q * DP;
+                xs[q * XS + l + e] = (double)xv[j][e];
+            }
         if (tid < B) {
             sbb[tid] = sb1;
             idn[tid] = id2;   // indices of the next batch
@@ -216,39 +228,39 @@ __global__ void __launch_bounds__(kThreads) k_stats_gauss_mfma(const TX* __restr
             if (lane == 0) { nls[0] = c0; nls[1] = c1; }
         }
         // next batch: coordinates and sub-labels from the indices now in idn, indices after that
-#pragma unroll
-        for (int j = 0; j < PER; ++j) {
-            const int e = tid + kThreads * j, q = e / DP, l = e - q * DP;
-            const int id = idn[q];
-            xv[j] = (id >= 0 && l < D) ? X[(int64_t)id * D + l] : TX(0);
-        }
+        load_batch();
         sb1 = id2 >= 0 ? subs[id2] : -1;
         id2 = (tid < B && b0 + 2 * B + tid < end) ? perm[b0 + 2 * B + tid] : -1;
         __syncthreads();
         const int n0 = nls[0], n1 = nls[1];
         cnt0 += n0;
         cnt1 += n1;
+        // lane (kp, r) holds coordinates r, 16 + r, ... of point kp of each group of 4: operand of
+        // every tile; (sub-cluster, tile) pair p belongs to wave p % 4, accumulator p / 4
 #pragma unroll
         for (int sub = 0; sub < 2; ++sub) {
             const int ns = sub ? n1 : n0;
             const int* L = lst[sub];
             for (int g = 0; g < ns; g += 4) {
                 const int q = g + kp;
-                const double* row = xs + L[q] * DP;   // pads point at row 0; masked below
-                const bool in = q < ns;
+                const double* row = xs + (q < ns ? L[q] : B) * XS + r;
+                double v[T];
 #pragma unroll
-                for (int u = 0; u < TPW; ++u) {
-                    if (us[u] != sub) continue;
-                    const double a = in ? row[ca[u]] : 0.0;
-                    const double bb = in ? row[cb[u]] : 0.0;
-                    acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bb, acc[u], 0, 0, 0);
-                }
+                for (int ti = 0; ti < T; ++ti) v[ti] = row[16 * ti];
+#pragma unroll
+                for (int ti = 0; ti < T; ++ti)
+#pragma unroll
+                    for (int tj = ti; tj < T; ++tj) {
+                        const int pr = sub * NTILE + ti * T - ti * (ti - 1) / 2 + (tj - ti);
+                        if ((pr & 3) == wv)
+                            acc[pr >> 2] = __builtin_amdgcn_mfma_f64_16x16x4f64(v[ti], v[tj], acc[pr >> 2], 0, 0, 0);
+                    }
             }
         }
         if (sx_part < SXT) {
             const int ns = sx_s ? n1 : n0;
             const int* L = lst[sx_s];
-            for (int q = sx_part; q < ns; q += SXT) sx += xs[L[q] * DP + sx_l];
+            for (int q = sx_part; q < ns; q += SXT) sx += xs[L[q] * XS + sx_l];
         }
     }
     sxr[tid] = sx;

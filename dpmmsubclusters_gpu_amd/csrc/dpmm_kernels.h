@@ -140,6 +140,73 @@ __device__ __forceinline__ T whitened_sq(const T* __restrict__ ps, const T (&x)[
     return whitened_sq_regs<T, DP>(ps, x);
 }
 
+// Coordinates l .. l + 7 of a point row of D floats, zero past D or when !valid; `row` must exist
+// either way (callers pass row 0 for an empty lane).  With D a multiple of 4 the row and every
+// 4-aligned l are 16-byte aligned: two unconditional 16-byte loads (offset 0 stands in past D);
+// otherwise eight unconditional 4-byte loads from clamped offsets.  No branches around the loads.
+typedef float f32x4v __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) const float gfloat;        // a float in global memory
+typedef __attribute__((address_space(1))) const f32x4v gf32x4v;
+template <typename F, typename V>   // F: float or gfloat, V: the matching 16-byte vector type
+__device__ __forceinline__ void load_coords8_t(F* __restrict__ row, int l, int D, bool valid, float (&o)[8]) {
+    if ((D & 3) == 0) {
+        const bool a = valid && l + 4 <= D, b = valid && l + 8 <= D;
+        const f32x4v lo = *(V*)(row + (a ? l : 0));
+        const f32x4v hi = *(V*)(row + (b ? l + 4 : 0));
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            o[e] = a ? lo[e] : 0.f;
+            o[e + 4] = b ? hi[e] : 0.f;
+        }
+    } else {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int c = l + e;
+            const float v = row[c < D ? c : 0];
+            o[e] = (valid && c < D) ? v : 0.f;
+        }
+    }
+}
+__device__ __forceinline__ void load_coords8(const float* __restrict__ row, int l, int D, bool valid, float (&o)[8]) {
+    load_coords8_t<const float, const f32x4v>(row, l, D, valid, o);
+}
+// Only for D a multiple of 4: the two 16-byte loads without the other path.
+__device__ __forceinline__ void load_coords8_aligned(gfloat* __restrict__ row, int l, int D, bool valid,
+                                                     float (&o)[8]) {
+    const bool a = valid && l + 4 <= D, b = valid && l + 8 <= D;
+    const f32x4v lo = *(gf32x4v*)(row + (a ? l : 0));
+    const f32x4v hi = *(gf32x4v*)(row + (b ? l + 4 : 0));
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        o[e] = a ? lo[e] : 0.f;
+        o[e + 4] = b ? hi[e] : 0.f;
+    }
+}
+// The same for 4 coordinates l .. l + 3.
+__device__ __forceinline__ void load_coords4(const float* __restrict__ row, int l, int D, bool valid, float (&o)[4]) {
+    if ((D & 3) == 0) {
+        const bool a = valid && l + 4 <= D;
+        const f32x4v v = *reinterpret_cast<const f32x4v*>(row + (a ? l : 0));
+#pragma unroll
+        for (int e = 0; e < 4; ++e) o[e] = a ? v[e] : 0.f;
+    } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int c = l + e;
+            const float v = row[c < D ? c : 0];
+            o[e] = (valid && c < D) ? v : 0.f;
+        }
+    }
+}
+__device__ __forceinline__ void load_coords4(const double* __restrict__ row, int l, int D, bool valid, double (&o)[4]) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int c = l + e;
+        const double v = row[c < D ? c : 0];
+        o[e] = (valid && c < D) ? v : 0.0;
+    }
+}
+
 // Chunk c of the partition: the cluster k whose segment holds it and its point range [beg, end)
 // in perm.  False past the last chunk.
 __device__ __forceinline__ bool chunk_of(int c, int K, const int32_t* __restrict__ chunk_start,

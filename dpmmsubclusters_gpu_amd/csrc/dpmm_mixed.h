@@ -23,9 +23,6 @@ namespace dpmm {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef __attribute__((address_space(3))) void lds_void;
-typedef __attribute__((address_space(1))) const float gfloat;
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-typedef __attribute__((address_space(1))) const f32x4 gfloat4;
 typedef __attribute__((address_space(1))) const void glob_cvoid;
 
 // Copy of `bytes` (a multiple of 16) from global memory into LDS by the wave-level LDS DMA
@@ -88,13 +85,9 @@ k_label_gauss_mx(const float* __restrict__ X, int64_t N, int D, int K, const uns
 #pragma unroll
     for (int nb = 0; nb < NB; ++nb) {
         const int64_t i = base + 32 * nb;
+        const float* row = X + (i < N ? i : 0) * D;
 #pragma unroll
-        for (int kc = 0; kc < KC; ++kc)
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const int l = 16 * kc + 8 * h + e;
-                x[nb][kc][e] = (i < N && l < D) ? X[i * D + l] : 0.f;
-            }
+        for (int kc = 0; kc < KC; ++kc) load_coords8(row, 16 * kc + 8 * h, D, i < N, x[nb][kc]);
     }
     float sb = neg_inf<float>(), eb = 0.f, U = neg_inf<float>();
     int bi = 0;
@@ -216,13 +209,9 @@ __global__ void __launch_bounds__(kThreads) k_sub_gauss_mx(const float* __restri
         const bool valid = p < end;
         const int idx = valid ? perm[p] : 0;
         float x[KC][8];
+        const float* row = X + (int64_t)idx * D;   // idx = 0 for an empty lane
 #pragma unroll
-        for (int kc = 0; kc < KC; ++kc)
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const int l = 16 * kc + 8 * h + e;
-                x[kc][e] = (valid && l < D) ? X[(int64_t)idx * D + l] : 0.f;
-            }
+        for (int kc = 0; kc < KC; ++kc) load_coords8(row, 16 * kc + 8 * h, D, valid, x[kc]);
         float s0 = 0.f, s1 = 0.f, dq0 = 0.f, dq1 = 0.f;
 #pragma unroll 1
         for (int hh = 0; hh < 2; ++hh) {
@@ -341,35 +330,23 @@ k_label_gauss_mxp(const float* __restrict__ X, int D, int K, int CH, const int32
         const int idx = val ? idx_next : 0;
         idx_next = (pos + 128 < end) ? perm[pos + 128] : 0;   // the next pass's point, loaded ahead
         const bool own_valid = val && h == 0;
-        // the point's coordinates 16 kc + 8 h .. + 7: unconditional 16-byte loads (D is a multiple of
-        // 4, so rows and these offsets are 16-byte aligned) from addresses that always exist (row 0
-        // for an empty lane, offset 0 past D), zeroed where they lie outside the point
-        auto load_x = [&](const gfloat* Xs, float (&x)[KC][8]) {
-            const gfloat* row = Xs + (int64_t)idx * D;
+        // the point's coordinates 16 kc + 8 h .. + 7 (row 0 stands in for an empty lane)
+        auto load_x = [&](gfloat* Xs, float (&x)[KC][8]) {
+            gfloat* row = Xs + (int64_t)idx * D;
 #pragma unroll
-            for (int kc = 0; kc < KC; ++kc) {
-                const int off = 16 * kc + 8 * h;
-                const bool in_lo = val && off + 4 <= D, in_hi = val && off + 8 <= D;
-                const f32x4 lo = *reinterpret_cast<const gfloat4*>(row + (in_lo ? off : 0));
-                const f32x4 hi = *reinterpret_cast<const gfloat4*>(row + (in_hi ? off + 4 : 0));
-#pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    x[kc][e] = in_lo ? lo[e] : 0.f;
-                    x[kc][e + 4] = in_hi ? hi[e] : 0.f;
-                }
-            }
+            for (int kc = 0; kc < KC; ++kc) load_coords8_aligned(row, 16 * kc + 8 * h, D, val, x[kc]);   // D % 4 == 0
         };
         // up to 32 dimensions the coordinates stay in registers for the whole pass; wider points are
         // read again by each scoring (from cache after the anchor's), which keeps the registers for
         // the MFMA operands
         constexpr bool kHoldX = DP <= 32;
         float xh[kHoldX ? KC : 1][8];
-        if constexpr (kHoldX) load_x((const gfloat*)X, xh);
+        if constexpr (kHoldX) load_x((gfloat*)X, xh);
         // screened q of cluster k for this lane's point, and sum (x - mu_k)^2
         auto score = [&](int k, float& d2) -> float {
             float xr[kHoldX ? 1 : KC][8];
             if constexpr (!kHoldX) {
-                const gfloat* Xs = (const gfloat*)X;
+                gfloat* Xs = (gfloat*)X;
                 asm volatile("" : "+s"(Xs));   // a load per scoring, not one hoisted out of the cluster loop
                 load_x(Xs, xr);
             }

@@ -20,9 +20,11 @@ sys.path.insert(0, ROOT)
 
 PEAK = {"fp32": 157.3, "fp64": 78.6, "bf16": 2500.0}   # dense TFLOP/s, MI355X
 HBM_PEAK = 8000.0                                       # GB/s
-# HBM bytes per launch of the dominant kernel, measured with rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE (scaled by a copy of known size; profiles/hbm_traffic_c3_d64_mixed.md)
-TRAFFIC = {("c3_d64", "mixed"): 2.750e9 + 0.040e9}
+# HBM bytes per launch of a phase's kernel, measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
+# (scaled by what the counters read for a copy of known size); keyed by (config, precision, phase)
+TRAFFIC = {("c3_d64", "mixed", "statistics"): 3.274e9 + 0.137e9,
+           ("c3_d64", "mixed", "label_sweep"): 2.866e9 + 0.269e9,
+           ("c3_d64", "mixed", "sub_sweep"): 2.621e9 + 0.272e9}
 
 CONFIGS = {
     # N, D, true clusters, A scale, ridge, Dirichlet weight (None equal), alpha, dtype
@@ -306,21 +308,55 @@ def main():
     ms_step = 1000 * elapsed / args.steps
     value = N * args.steps / elapsed
     avg_lab = float(np.mean(lab_ms))
+    avg_sub, avg_st = float(np.mean(sub_ms)), float(np.mean(st_ms))
+    pruned = bool(evfrac) and float(np.mean(evfrac)) >= 0.0
+
+    def hbm(kernel, bytes_launch, ms, what):
+        return {"bound": "hbm", "kernel": kernel, "achieved": bytes_launch / (ms * 1e-3) / 1e9, "peak": HBM_PEAK,
+                "unit": "GB/s", "traffic": None, "algorithmic": what}
+
+    # one roofline per phase; the bench line's `roofline` is the phase that takes the longest
+    phase = {}
     if mult:   # HBM-bound: one read of the N x D int32 counts, one label write
-        bytes_launch = N * (4.0 * D + 4.0)
-        roof = {"bound": "hbm", "kernel": "k_label_mult", "achieved": bytes_launch / (avg_lab * 1e-3) / 1e9,
-                "peak": HBM_PEAK, "unit": "GB/s", "traffic": None,
-                "algorithmic": "4 D + 4 bytes per document per launch (counts read once, label written)",
-                "flops_TFLOP/s": float(np.mean(flops)) / (avg_lab * 1e-3) / 1e12}
+        phase["label_sweep"] = hbm("k_label_mult_mx + k_label_mult_list" if prec == dp.Precision.MIXED else
+                                   "k_label_mult", N * (4.0 * D + 4.0), avg_lab,
+                                   "4 D + 4 bytes per document (counts read once, label written)")
+        phase["label_sweep"]["flops_TFLOP/s"] = float(np.mean(flops)) / (avg_lab * 1e-3) / 1e12
+        phase["sub_sweep"] = hbm("k_sub_mult", N * (4.0 * D + 12.0), avg_sub,
+                                 "4 D + 12 bytes per document (gathered counts, index, label, sub-label)")
+        phase["statistics"] = hbm("k_stats_mult", N * (4.0 * D + 8.0), avg_st,
+                                  "4 D + 8 bytes per document (gathered counts, index, sub-label)")
     else:
-        achieved = float(np.mean(flops)) / (avg_lab * 1e-3) / 1e12
-        peak = PEAK[{dp.Precision.FP64: "fp64", dp.Precision.FP32: "fp32", dp.Precision.MIXED: "bf16"}[prec]]
-        roof = {"bound": "mfma",
+        esz = 8.0 if prec == dp.Precision.FP64 else 4.0
+        if pruned:   # exact pruning leaves about one cluster scored per point: a gather of the points
+            phase["label_sweep"] = hbm("k_prune_prep + partition + k_label_gauss_mxp + k_label_gauss_list",
+                                       N * (esz * D + 12.0), avg_lab,
+                                       "D coordinates + index + label read + label write per point "
+                                       f"({float(np.mean(evfrac)):.3f} of the (wave, cluster) scorings done)")
+        else:
+            peak = PEAK[{dp.Precision.FP64: "fp64", dp.Precision.FP32: "fp32", dp.Precision.MIXED: "bf16"}[prec]]
+            phase["label_sweep"] = {
+                "bound": "mfma",
                 "kernel": "k_label_gauss_mx + k_label_gauss_list" if prec == dp.Precision.MIXED else "k_label_gauss",
-                "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                "traffic": TRAFFIC.get((args.config, args.precision)),
-                "algorithmic": "2*K*D^2 flops per point per launch (dense whitening of K clusters)"}
-    roof["frac"] = roof["achieved"] / roof["peak"]
+                "achieved": float(np.mean(flops)) / (avg_lab * 1e-3) / 1e12, "peak": peak, "unit": "TFLOP/s",
+                "traffic": TRAFFIC.get((args.config, args.precision, "label_sweep")),
+                "algorithmic": "2 K D^2 flops per point (dense whitening of K clusters)"}
+        phase["sub_sweep"] = hbm("k_sub_gauss_mx + k_sub_gauss_list" if prec == dp.Precision.MIXED else
+                                 "k_sub_gauss", N * (esz * D + 12.0), avg_sub,
+                                 "D coordinates + index + label + sub-label write per point (gathered)")
+        phase["sub_sweep"]["traffic"] = TRAFFIC.get((args.config, args.precision, "sub_sweep"))
+        if pruned:
+            phase["label_sweep"]["traffic"] = TRAFFIC.get((args.config, args.precision, "label_sweep"))
+        phase["statistics"] = {
+            "bound": "mfma", "kernel": ("k_stats_gauss_mfma" if D > 8 else "k_stats_gauss") + " + k_stats_reduce",
+            "achieved": N * (D * (D + 1.0) + D) / (avg_st * 1e-3) / 1e12, "peak": PEAK["fp64"], "unit": "TFLOP/s",
+            "traffic": TRAFFIC.get((args.config, args.precision, "statistics")),
+            "algorithmic": "D (D + 1) + D flops per point (upper triangle of x x^T and the sum, FP64)"}
+    for r in phase.values():
+        r["frac"] = r["achieved"] / r["peak"]
+    dominant = max(("label_sweep", "sub_sweep", "statistics"),
+                   key=lambda k: {"label_sweep": avg_lab, "sub_sweep": avg_sub, "statistics": avg_st}[k])
+    roof = dict(phase[dominant], phase=dominant)
     K_final = smp.num_clusters()
     from dpmmsubclusters_gpu_amd import _nmi
     lab, _ = dev.get_labels()
@@ -340,8 +376,9 @@ def main():
                    "N": N, "D": D, "clusters_timed": K_final, "precision": args.precision,
                    "parallelism": "single GPU"},
         "roofline": roof,
-        "breakdown_ms": {"label_sweep": avg_lab, "sub_sweep": float(np.mean(sub_ms)),
-                         "statistics": float(np.mean(st_ms)), "conjugate": float(np.mean(host_ms))},
+        "roofline_by_phase": phase,
+        "breakdown_ms": {"label_sweep": avg_lab, "sub_sweep": avg_sub, "statistics": avg_st,
+                         "conjugate": float(np.mean(host_ms))},
         "final_nmi_vs_truth": nmi,
         "recheck_fraction": (float(np.mean(rechk)) / N) if rechk else None,
         "sub_recheck_fraction": (float(np.mean(srechk)) / N) if srechk else None,

@@ -33,6 +33,9 @@ CONFIGS = {
     "c2_d32": dict(N=1_000_000, D=32, K=20, a=1.0, ridge=1.0, weight=1.0, alpha=10.0, dtype="float32", iters=200),
     "c3_d64": dict(N=10_000_000, D=64, K=50, a=1.0, ridge=0.5, weight=0.5, alpha=10.0, dtype="float32",
                    iters=200),
+    # learned merge screener at scale: started over-split into `pieces` fragments per true cluster
+    "c5_screener": dict(N=5_000_000, D=128, K=200, a=0.5, ridge=1.0, weight=None, alpha=10.0, dtype="float32",
+                        iters=300, pieces=2),
     # multinomial: N documents of `words` words over a vocabulary of D, topics theta ~ Dir(conc)
     "c4_mult": dict(N=1_000_000, D=1000, K=100, words=100, conc=0.1, alpha=10.0, dtype="int32", prior=1.0,
                     iters=150),
@@ -226,6 +229,89 @@ def run_seeds(args, cfg):
                       "paths": summary}), flush=True)
 
 
+def run_screener(args, cfg):
+    """C5: train the merge screener on pairs logged by the plain sampler on small mixtures of the
+    same family, then run the workload from its over-split start twice, every candidate pair tested
+    exactly and screened with a full scan every 10 iterations; one JSON line."""
+    import torch
+    import dpmmsubclusters_gpu_amd as dp
+    from dpmmsubclusters_gpu_amd import _nmi
+    from dpmmsubclusters_gpu_amd import screener as scr
+    N, D, iters = cfg["N"], cfg["D"], cfg["iters"]
+    t0 = time.perf_counter()
+    feats, hs, acc = [], [], []
+    # training mixtures of the workload's family at its cluster sizes (N / (pieces K) points per
+    # fragment), logged by the plain sampler
+    k_train = 12
+    n_train = max(20_000, min(300_000, N // cfg["K"] * k_train))
+    for seed in (101, 102, 103, 104, 105, 106):   # 3 fragments per cluster: more merge decisions per run
+        print(f"[bench] screener training data: mixture seed {seed}", file=sys.stderr, flush=True)
+        f, h, u = scr.collect_pairs(n_train, D, k_train, pieces=3, iters=40, seed=seed, a=cfg["a"],
+                                    precision=dp.Precision.MIXED)
+        feats.append(f); hs.append(h); acc.append(h > u)
+    Xf, h, a = np.concatenate(feats), np.concatenate(hs), np.concatenate(acc)
+    y = scr.plausible(h)
+    s, hist = scr.train(Xf, y, epochs=30, batch=4096, lr=2e-3, seed=2)
+    z = s.logits(Xf)
+    thr = scr.threshold_for_recall(z, y, recall=1.0, margin=8.0)   # wide margin: pairs unlike the training set
+    t_train = time.perf_counter() - t0
+    print(f"[bench] training logits: positives min {z[y > 0.5].min():.2f}, negatives max {z[y < 0.5].max():.2f}, "
+          f"median {np.median(z[y < 0.5]):.2f}", file=sys.stderr, flush=True)
+    print(f"[bench] screener trained on {len(y)} pairs ({int(y.sum())} plausible, {int(a.sum())} accepted) "
+          f"in {t_train:.1f} s, threshold {thr:.3f}", file=sys.stderr, flush=True)
+    print(f"[bench] generating {N} points", file=sys.stderr, flush=True)
+    X, gt, _, _, _ = gen_on_gpu(cfg, args.seed)
+    gtn = gt.cpu().numpy()
+    print("[bench] generated", file=sys.stderr, flush=True)
+    init = scr.over_split_labels(gtn, cfg["pieces"], args.seed)
+    hyper = dp.niw_hyperparams(1.0, np.zeros(D), D + 3.0, np.eye(D))
+    runs = {}
+    for name, use in (("all_pairs", False), ("screened", True)):
+        dev = dp.DeviceSweep(None, dp.prior_type.Gaussian, seed=args.seed + 1, device_ptr=X.data_ptr(), N=N, D=D,
+                             dtype=np.float32)
+        smp = dp.Sampler(dev, hyper, cfg["alpha"], iters, seed=args.seed + 1, precision=dp.Precision.MIXED,
+                         init_labels=init)
+        smp.set_screener(s if use else None, full_scan_every=10, threshold=thr)
+        dev.synchronize()
+        t1 = time.perf_counter()
+        for t in range(1, iters + 1):
+            smp.step(t)
+            if t <= 3 or t % 10 == 0:
+                dev.synchronize()
+                lab_ms, sub_ms, st_ms = dev.kernel_ms()
+                print(f"[bench] {name} iteration {t}/{iters} K={smp.num_clusters()} "
+                      f"({time.perf_counter() - t1:.1f}s; label {lab_ms:.1f} ms, sub {sub_ms:.1f} ms, statistics "
+                      f"{st_ms:.1f} ms, conjugate {smp.host_ms():.1f} ms, rechecked {dev.last_recheck_count()} "
+                      f"+ {dev.last_sub_recheck_count()}, evaluated {dev.last_label_evaluations():.3f})",
+                      file=sys.stderr, flush=True)
+        dev.synchronize()
+        el = time.perf_counter() - t1
+        lab, _ = dev.get_labels()
+        st = smp.screen_stats()
+        runs[name] = {"seconds": el, "ms_per_iteration": 1000 * el / iters, "point_iters_per_s": N * iters / el,
+                      "final_K": smp.num_clusters(), "nmi_vs_truth": _nmi(gtn, lab),
+                      "exact_merge_tests": int(st[0]), "pairs_skipped": int(st[1]), "missed_merges": int(st[2]),
+                      "full_scans": int(st[3]), "labels": lab}
+        smp.close()
+        dev.close()
+    same = bool(np.array_equal(runs["screened"].pop("labels"), runs["all_pairs"].pop("labels")))
+    sr = runs["screened"]
+    print(json.dumps({
+        "metric": "point*iters/sec", "value": sr["point_iters_per_s"], "unit": "point*iters/s", "n_gpus": 1,
+        "steps": iters, "warmup": 0, "ms_per_step": sr["ms_per_iteration"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "bf16+f32", "data": "synthetic",
+        "config": {"workload": f"c5_screener: Gaussian DPMM sampler, N={N}, D={D}, {cfg['K']} true clusters "
+                               f"(A ~ N(0, {cfg['a']}^2)), started over-split into {cfg['pieces'] * cfg['K']} "
+                               f"clusters, alpha={cfg['alpha']}, {iters} iterations, merge screener with a full "
+                               f"scan every 10 iterations", "N": N, "D": D, "precision": "mixed",
+                   "parallelism": "single GPU"},
+        "screener": {"training_pairs": int(len(y)), "plausible": int(y.sum()), "accepted": int(a.sum()),
+                     "training_seconds": t_train, "threshold": thr, "final_loss": hist[-1]},
+        "all_pairs": runs["all_pairs"], "screened": sr,
+        "speedup_vs_all_pairs": runs["all_pairs"]["seconds"] / sr["seconds"],
+        "labels_identical": same}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -238,6 +324,8 @@ def main():
     ap.add_argument("--seeds", type=int, default=0,
                     help="instead of timing: final K and NMI over this many seeds on each of --paths")
     ap.add_argument("--paths", default="mixed,fp32,fp64")
+    ap.add_argument("--n", type=int, default=0, help="override the configuration's N (a reduced run, not its bench line)")
+    ap.add_argument("--iters", type=int, default=0, help="override the screener workload's iteration count")
     ap.add_argument("--copy-probe", action="store_true",
                     help="copy X once (N*D*4 bytes read + written) before the run: a plain copy kernel of known "
                          "size that calibrates the HBM byte counters of a profiler")
@@ -249,6 +337,13 @@ def main():
 
     if args.seeds > 0:
         run_seeds(args, CONFIGS[args.config])
+        return
+    if args.n > 0:
+        CONFIGS[args.config]["N"] = args.n
+    if args.iters > 0:
+        CONFIGS[args.config]["iters"] = args.iters
+    if args.config == "c5_screener":
+        run_screener(args, CONFIGS[args.config])
         return
 
     import torch

@@ -1005,6 +1005,12 @@ template <int DP> void lgl_f32(dpmm_ctx* c, uint32_t it, int f, int grid) {
                                        c->dRecheck, c->dRecheckCount, c->dLabels);
 }
 
+template <int DP> void lglp_f32(dpmm_ctx* c, uint32_t it, int f, int grid) {
+    launch_label_gauss_listp<float, DP>(c->stream, grid, reinterpret_cast<const float*>(c->dX), c->N, c->D, c->K,
+                                        reinterpret_cast<const float*>(c->dP), c->dPruneM, c->dPruneSC, c->key0(),
+                                        c->key1(), it, f, c->dRecheck, c->dRecheckCount, c->dLabels);
+}
+
 template <int DP> void sgl_f32(dpmm_ctx* c, uint32_t it, int grid) {
     launch_sub_gauss_list<float, DP>(c->stream, grid, reinterpret_cast<const float*>(c->dX), c->D,
                                      reinterpret_cast<const float*>(c->dSP), c->dLabels, c->dRecheck,
@@ -1107,7 +1113,7 @@ int run_label_mixed_pruned(dpmm_ctx* c, uint32_t iter, int fin) {
     }
 #undef DPMM_MXP
     const int grid2 = (int)std::min<int64_t>(1024, (c->N + kThreads - 1) / kThreads);
-    DPMM_DISPATCH_DP(c->DP, lgl_f32, c, iter, fin, grid2);
+    DPMM_DISPATCH_DP(c->DP, lglp_f32, c, iter, fin, grid2);   // the recheck pruned with the same bounds
     c->last_pruned_K = c->K;
     return DPMM_OK;
 }

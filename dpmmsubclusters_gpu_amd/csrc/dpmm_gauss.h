@@ -83,6 +83,89 @@ __global__ void __launch_bounds__(kThreads) k_label_gauss_list(const T* __restri
     }
 }
 
+// The FP32 re-scoring of listed points with the bounds of the pruned sweep (pm, psc: whitened mean
+// distances and spectral norms from k_prune_prep).  Each point's anchor a (its label before the sweep,
+// 0 when that lies outside [0, K)) is scored first, with r >= ||x - mu_a||; cluster k is then scored
+// only if some point of the workgroup cannot rule it out, i.e. c_k - 0.49 max(0, m_ak - sig_k r)^2
+// + 1e-6 |c_k| + kPruneNoise is not below that point's noisy anchor score.  A skipped cluster's FP32
+// score lies below the anchor's for every point of the workgroup, so the first maximum in index order
+// over the scored clusters is the FP32 sweep's label (k_label_gauss).  Noise words are taken
+// directly (word k % 4 of block k / 4), the values k_label_gauss consumes in turn.
+template <typename T, int DP>
+__global__ void __launch_bounds__(kThreads) k_label_gauss_listp(const T* __restrict__ X, int64_t N, int D, int K,
+                                                                const T* __restrict__ P, const float* __restrict__ pm,
+                                                                const float* __restrict__ psc, uint32_t key0,
+                                                                uint32_t key1, uint32_t iter, int final_iter,
+                                                                const int32_t* __restrict__ list,
+                                                                const int* __restrict__ list_count,
+                                                                int32_t* __restrict__ labels) {
+    constexpr bool kPair = paired_layout<T, DP>();
+    constexpr int kPts = kPair ? kThreads / 2 : kThreads;
+    constexpr int kCols = kPair ? DP / 2 : DP;
+    constexpr int stride = gauss_slot_stride(DP);
+    __shared__ __align__(16) T ps[stride];
+    const int par = kPair ? (threadIdx.x & 1) : 0;
+    const int local = kPair ? (threadIdx.x >> 1) : threadIdx.x;
+    const int n = *list_count;
+    for (int b0 = blockIdx.x * kPts; b0 < n; b0 += gridDim.x * kPts) {   // uniform trip count
+        const int pos = b0 + local;
+        const int64_t i = pos < n ? (int64_t)list[pos] : N;
+        const bool valid = i < N;
+        T x[kCols];
+#pragma unroll
+        for (int c = 0; c < kCols; ++c) {
+            const int l = kPair ? pair_col(c, 0) + (par << 2) : c;
+            x[c] = (valid && l < D) ? X[i * D + l] : T(0);
+        }
+        int a = valid ? labels[i] : 0;
+        if ((unsigned)a >= (unsigned)K) a = 0;
+        const T* pa = P + (size_t)a * stride;
+        T qa;
+        if constexpr (kPair) qa = whitened_sq_pair<T, DP>(pa, x, par);
+        else qa = whitened_sq_regs<T, DP>(pa, x);
+        T sa = pa[gauss_c_offset(DP)] - T(0.5) * qa;
+        if (!final_iter)
+            sa += gumbel<T>(sel4(philox4x32_10((uint32_t)i, (uint32_t)(a >> 2), iter, TAG_LABEL, key0, key1), a & 3));
+        float d2 = 0.f;
+#pragma unroll
+        for (int c = 0; c < kCols; ++c) {
+            const int l = kPair ? pair_col(c, 0) + (par << 2) : c;
+            const float d = l < D ? (float)(x[c] - pa[gauss_mu_offset(DP) + l]) : 0.f;
+            d2 = fmaf(d, d, d2);
+        }
+        if constexpr (kPair) d2 += __shfl_xor(d2, 1);
+        const float r = sqrtf(d2) * (1.0f + (float)(D + 8) * 0x1p-23f);
+        const float* mrow = pm + (size_t)a * K;
+        T best = neg_inf<T>();
+        int bi = 0;
+        bool first = true;
+        for (int k = 0; k < K; ++k) {
+            bool need = false;
+            if (valid) {
+                if (k == a) {
+                    need = true;
+                } else {
+                    const float ck = psc[2 * k + 1];
+                    const float lb = fmaxf(0.f, mrow[k] - psc[2 * k] * r);
+                    need = !(ck - 0.49f * lb * lb + 1e-6f * fabsf(ck) + kPruneNoise < (float)sa);
+                }
+            }
+            if (!__syncthreads_or(need)) continue;   // also the barrier after the last cluster's readers
+            const T* src = P + (size_t)k * stride;
+            for (int e = threadIdx.x; e < stride; e += kThreads) ps[e] = src[e];
+            __syncthreads();
+            T q;
+            if constexpr (kPair) q = whitened_sq_pair<T, DP>(ps, x, par);
+            else q = whitened_sq_regs<T, DP>(ps, x);
+            T s = ps[gauss_c_offset(DP)] - T(0.5) * q;
+            if (!final_iter)
+                s += gumbel<T>(sel4(philox4x32_10((uint32_t)i, (uint32_t)(k >> 2), iter, TAG_LABEL, key0, key1), k & 3));
+            if (s > best || first) { best = s; bi = k; first = false; }
+        }
+        if (valid && par == 0) labels[i] = bi;
+    }
+}
+
 template <typename T, int DP>
 __global__ void __launch_bounds__(kThreads) k_sub_gauss(const T* __restrict__ X, int D, int K, int CH,
                                                         const int32_t* __restrict__ perm,
@@ -484,6 +567,14 @@ void launch_label_gauss_list(hipStream_t st, int grid, const T* X, int64_t N, in
 }
 
 template <typename T, int DP>
+void launch_label_gauss_listp(hipStream_t st, int grid, const T* X, int64_t N, int D, int K, const T* P,
+                              const float* pm, const float* psc, uint32_t key0, uint32_t key1, uint32_t iter, int fin,
+                              const int32_t* list, const int* count, int32_t* labels) {
+    hipLaunchKernelGGL((k_label_gauss_listp<T, DP>), dim3(grid), dim3(kThreads), 0, st, X, N, D, K, P, pm, psc, key0,
+                       key1, iter, fin, list, count, labels);
+}
+
+template <typename T, int DP>
 void launch_sub_gauss_list(hipStream_t st, int grid, const T* X, int D, const T* SP, const int32_t* labels,
                            const int32_t* list, const int* count, uint32_t key0, uint32_t key1, uint32_t iter,
                            int32_t* subs) {
@@ -535,7 +626,10 @@ void launch_stats_gauss(hipStream_t st, int nchunks, const TX* X, int D, int K, 
                                                const int32_t*, const int*, uint32_t, uint32_t, uint32_t,       \
                                                int32_t*);                                                      \
     template void launch_score_gauss<T, DP>(hipStream_t, int, const T*, int, int, const T*, const T*,          \
-                                            const int32_t*, int64_t, int64_t, int, T*);
+                                            const int32_t*, int64_t, int64_t, int, T*);                        \
+    template void launch_label_gauss_listp<T, DP>(hipStream_t, int, const T*, int64_t, int, int, const T*,     \
+                                                  const float*, const float*, uint32_t, uint32_t, uint32_t, int, \
+                                                  const int32_t*, const int*, int32_t*);
 
 #define DPMM_GAUSS_EXTERN(T, DP)                                                                              \
     extern template void launch_label_gauss<T, DP>(hipStream_t, int, const T*, int64_t, int, int, const T*,    \
@@ -552,6 +646,10 @@ void launch_stats_gauss(hipStream_t st, int nchunks, const TX* X, int D, int K, 
                                                       const int32_t*, const int32_t*, const int*, uint32_t,    \
                                                       uint32_t, uint32_t, int32_t*);                           \
     extern template void launch_score_gauss<T, DP>(hipStream_t, int, const T*, int, int, const T*, const T*,   \
-                                                   const int32_t*, int64_t, int64_t, int, T*);
+                                                   const int32_t*, int64_t, int64_t, int, T*);                 \
+    extern template void launch_label_gauss_listp<T, DP>(hipStream_t, int, const T*, int64_t, int, int,        \
+                                                         const T*, const float*, const float*, uint32_t,       \
+                                                         uint32_t, uint32_t, int, const int32_t*, const int*,  \
+                                                         int32_t*);
 
 }  // namespace dpmm

@@ -207,6 +207,10 @@ __device__ __forceinline__ void load_coords4(const double* __restrict__ row, int
     }
 }
 
+// Upper bound of the Gumbel noise of a 32-bit uniform (22.87) plus slack, used by the exact
+// pruning bounds of the mixed label sweep.
+constexpr float kPruneNoise = 24.0f;
+
 // Chunk c of the partition: the cluster k whose segment holds it and its point range [beg, end)
 // in perm.  False past the last chunk.
 __device__ __forceinline__ bool chunk_of(int c, int K, const int32_t* __restrict__ chunk_start,

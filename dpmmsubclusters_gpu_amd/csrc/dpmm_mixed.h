@@ -295,7 +295,6 @@ void launch_sub_gauss_mx(hipStream_t st, int grid, const float* X, int D, int K,
 // bounds the FP32 sweep's noisy score of cluster k for every point of the wave.  When ub_k < T no
 // point can take label k and the cluster is skipped (ub_k joins the runner-up bound U); the other
 // clusters are screened as in k_label_gauss_mx, so the decided labels are the FP32 sweep's.
-constexpr float kPruneNoise = 24.0f;
 
 // Workgroups per partition chunk in the pruned sweep (each takes an equal share of the chunk).
 constexpr int kPruneSplit = 4;

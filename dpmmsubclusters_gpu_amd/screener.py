@@ -96,11 +96,11 @@ def over_split_labels(gt, pieces, seed):
 
 
 def collect_pairs(n_points, D, K_true, pieces, iters, seed, alpha=10.0, spread=100.0, precision=None,
-                  dtype=np.float32):
+                  dtype=np.float32, a=1.0):
     """Run the plain sampler (every pair gets the exact test) on a synthetic Gaussian mixture from
     an over-split start and log every candidate pair of every iteration: features (rows x 16),
     log H, log U."""
-    X, gt, _ = gaussian_mixture(n_points, D, K_true, seed=seed, spread=spread, dtype=dtype)
+    X, gt, _ = gaussian_mixture(n_points, D, K_true, seed=seed, spread=spread, dtype=dtype, a=a)
     dev = DeviceSweep(X, prior_type.Gaussian, seed=seed)
     hyper = niw_hyperparams(1.0, np.zeros(D), D + 3.0, np.eye(D))
     s = Sampler(dev, hyper, alpha, iters, seed=seed, precision=precision,

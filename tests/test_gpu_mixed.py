@@ -89,6 +89,44 @@ def test_pruned_sweep_equals_unpruned(D, K, final):
     dev.close()
 
 
+@pytest.mark.parametrize("D,K", [(64, 10), (128, 6)])
+def test_pruned_sweep_with_sibling_clusters(D, K):
+    """Clusters in pairs of near-identical siblings (an over-split start): the BF16 screen leaves
+    most points to the FP32 recheck, which is pruned with the same bounds; labels stay identical to
+    the unpruned sweep's and to the plain FP32 sweep's."""
+    rng = np.random.default_rng(D + K)
+    N = 60_000
+    X, gt, _ = gaussian_mixture(N, D, K, seed=D * 7 + K, dtype=np.float32)
+    mu, lam, ld, w, smu, slam, sld, lrw = _fitted_params(X, gt, K, rng)
+    mu2 = np.repeat(mu, 2, axis=0) + rng.normal(scale=0.01, size=(2 * K, D))
+    lam2, ld2 = np.repeat(lam, 2, axis=0), np.repeat(ld, 2)
+    w2 = np.repeat(w, 2) / 2
+    smu2 = np.repeat(smu, 2, axis=0)
+    slam2, sld2, lrw2 = np.repeat(slam, 2, axis=0), np.repeat(sld, 2), np.repeat(lrw, 2)
+    params = (mu2, lam2, ld2, w2, smu2, slam2, sld2, lrw2)
+    before = (2 * gt + rng.integers(0, 2, N)).astype(np.int32)
+    say(f"sibling clusters D={D} K={2 * K}")
+    ref_dev = dp.DeviceSweep(X, dp.prior_type.Gaussian, seed=8)
+    ref_dev.set_gaussian_params(*params)
+    ref_dev.sample_cluster_labels(4, False)
+    ref, _ = ref_dev.get_labels()
+    ref_dev.close()
+    dev = dp.DeviceSweep(X, dp.prior_type.Gaussian, seed=8)
+    dev.set_precision(dp.Precision.MIXED)
+    dev.set_gaussian_params(*params)
+    out = {}
+    for prune in (False, True):
+        dev.set_label_pruning(prune)
+        dev.set_labels(before, None)
+        dev.sample_cluster_labels(4, False)
+        out[prune] = dev.get_labels()[0]
+        print(f"D={D} prune={prune}: rechecked {dev.last_recheck_count()} of {N}, "
+              f"evaluated {dev.last_label_evaluations():.3f}")
+    np.testing.assert_array_equal(out[True], out[False])
+    np.testing.assert_array_equal(out[True], ref)
+    dev.close()
+
+
 def test_screen_alone_flip_rate():
     """tau = 0 keeps every screened label: the raw BF16 flip rate against the oracle, reported."""
     rng = np.random.default_rng(1)

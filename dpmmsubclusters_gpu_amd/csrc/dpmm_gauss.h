@@ -232,7 +232,6 @@ __global__ void __launch_bounds__(kThreads) k_stats_gauss_mfma(const TX* __restr
     constexpr int TPW = (2 * NTILE + 3) / 4;          // (sub-cluster, tile) pairs per wave
     constexpr int B = DP > 64 ? 32 : 64;               // points per batch
     constexpr int PER = B * DP / kThreads;             // staged coordinates per thread
-    constexpr int SXT = kThreads / (2 * DP) > 0 ? kThreads / (2 * DP) : 1;
     constexpr size_t E = 1 + DP + (size_t)DP * DP;
     static_assert(B * DP % (4 * kThreads) == 0, "batch must tile the workgroup in groups of 4");
     constexpr int XS = DP + 1;                         // padded row: the 4 points of an MFMA step
@@ -241,7 +240,6 @@ __global__ void __launch_bounds__(kThreads) k_stats_gauss_mfma(const TX* __restr
     __shared__ int sbb[B];          // sub-labels of the staged batch
     __shared__ int lst[2][B + 4];
     __shared__ int nls[2];
-    __shared__ double sxr[kThreads];
     int k, beg, end;
     if (!chunk_of(blockIdx.x, K, chunk_start, label_start, CH, k, beg, end)) return;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -262,9 +260,13 @@ __global__ void __launch_bounds__(kThreads) k_stats_gauss_mfma(const TX* __restr
     f64x4 acc[TPW];
 #pragma unroll
     for (int u = 0; u < TPW; ++u) acc[u] = f64x4{0.0, 0.0, 0.0, 0.0};
-    const int sx_pair = tid % (2 * DP), sx_part = tid / (2 * DP);
-    const int sx_s = sx_pair / DP, sx_l = sx_pair - sx_s * DP;
-    double sx = 0.0;
+    // first moments: wave w, lane (kp, r) sums coordinate 16 ti + r (ti = w, w + 4, ...) of its points
+    constexpr int TS = (T + 3) / 4;
+    double sums[2][TS];
+#pragma unroll
+    for (int sub = 0; sub < 2; ++sub)
+#pragma unroll
+        for (int u = 0; u < TS; ++u) sums[sub][u] = 0.0;
     int64_t cnt0 = 0, cnt1 = 0;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 
@@ -332,6 +334,9 @@ __global__ void __launch_bounds__(kThreads) k_stats_gauss_mfma(const TX* __restr
                 for (int ti = 0; ti < T; ++ti) v[ti] = row[16 * ti];
 #pragma unroll
                 for (int ti = 0; ti < T; ++ti)
+                    if ((ti & 3) == wv) sums[sub][ti >> 2] += v[ti];
+#pragma unroll
+                for (int ti = 0; ti < T; ++ti)
 #pragma unroll
                     for (int tj = ti; tj < T; ++tj) {
                         const int pr = sub * NTILE + ti * T - ti * (ti - 1) / 2 + (tj - ti);
@@ -340,21 +345,20 @@ __global__ void __launch_bounds__(kThreads) k_stats_gauss_mfma(const TX* __restr
                     }
             }
         }
-        if (sx_part < SXT) {
-            const int ns = sx_s ? n1 : n0;
-            const int* L = lst[sx_s];
-            for (int q = sx_part; q < ns; q += SXT) sx += xs[L[q] * XS + sx_l];
-        }
     }
-    sxr[tid] = sx;
-    __syncthreads();
     double* out0 = partials + (size_t)(2 * blockIdx.x) * E;
     if (tid == 0) { out0[0] = (double)cnt0; out0[E] = (double)cnt1; }
-    if (tid < 2 * DP) {
-        double v = 0.0;
-        for (int j = 0; j < SXT; ++j) v += sxr[tid + j * 2 * DP];
-        out0[sx_s * E + 1 + sx_l] = v;
-    }
+    // sums over the 4 points of each group (lanes r, r + 16, r + 32, r + 48), fixed order
+#pragma unroll
+    for (int sub = 0; sub < 2; ++sub)
+#pragma unroll
+        for (int u = 0; u < TS; ++u) {
+            const int ti = 4 * u + wv;
+            double v = sums[sub][u];
+            v += __shfl_xor(v, 16);
+            v += __shfl_xor(v, 32);
+            if (kp == 0 && ti < T) out0[sub * E + 1 + 16 * ti + r] = v;
+        }
 #pragma unroll
     for (int u = 0; u < TPW; ++u) {
         if (us[u] < 0) continue;

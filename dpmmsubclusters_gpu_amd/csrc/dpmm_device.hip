@@ -252,14 +252,45 @@ __global__ void __launch_bounds__(kThreads) k_scatter(const int32_t* __restrict_
     }
 }
 
+// ---- multinomial rows --------------------------------------------------------------------------
+// A document's counts are read V words at a time: V = 4 (one 16-byte load per lane) when the
+// vocabulary size is a multiple of 4, so every row starts 16-byte aligned, else V = 1.
+typedef int32_t i32x4v __attribute__((ext_vector_type(4)));
+template <int V> struct IRow;
+template <> struct IRow<1> {
+    typedef int32_t T;
+    static __device__ __forceinline__ int32_t at(T x, int) { return x; }
+};
+template <> struct IRow<4> {
+    typedef i32x4v T;
+    static __device__ __forceinline__ int32_t at(T x, int v) { return x[v]; }
+};
+
+// Lane `lane`'s share of the two sub-cluster dot products of one document: words
+// lane*V + 64*V*step + v in increasing (step, v) order, zero counts skipped (so a -inf log theta of
+// an absent word adds nothing).  The sub-label sweep and the score entry point both sum this way.
+template <int V>
+__device__ __forceinline__ void mult_sub_partial(typename IRow<V>::T x, int col, const float* __restrict__ th,
+                                                 int stride, float& a0, float& a1) {
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        const float xv = (float)IRow<V>::at(x, v);
+        if (xv != 0.f) { a0 = fmaf(xv, th[col + v], a0); a1 = fmaf(xv, th[stride + col + v], a1); }
+    }
+}
+
 // ---- sub-label sweeps over segments ----------------------------------------------------------
-// Multinomial: one wavefront per point, lanes split the vocabulary.
+// Multinomial: one wavefront per document, lanes split the vocabulary V words at a time; each wave
+// takes two documents at once and issues all of a 256 V-word stretch of both rows before using
+// them, so eight row loads per wave are in flight.
+template <int V>
 __global__ void __launch_bounds__(kThreads) k_sub_mult(const int32_t* __restrict__ X, int D, int K, int CH,
                                                        const int32_t* __restrict__ perm,
                                                        const int32_t* __restrict__ label_start,
                                                        const int32_t* __restrict__ chunk_start,
                                                        const float* __restrict__ SP, int stride, uint32_t key0,
                                                        uint32_t key1, uint32_t iter, int32_t* __restrict__ subs) {
+    typedef typename IRow<V>::T VT;
     extern __shared__ __align__(16) unsigned char smem[];
     float* th = reinterpret_cast<float*>(smem);   // [2][stride]
     int k, beg, end;
@@ -268,19 +299,42 @@ __global__ void __launch_bounds__(kThreads) k_sub_mult(const int32_t* __restrict
     for (int e = threadIdx.x; e < 2 * stride; e += kThreads) th[e] = src[e];
     __syncthreads();
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    for (int p = beg + wv; p < end; p += 4) {
-        const int idx = perm[p];
-        const int32_t* xr = X + (int64_t)idx * D;
-        float a0 = 0.f, a1 = 0.f;
-        for (int d = lane; d < D; d += 64) {
-            const float xv = (float)xr[d];
-            if (xv != 0.f) { a0 = fmaf(xv, th[d], a0); a1 = fmaf(xv, th[stride + d], a1); }
+    for (int p = beg + wv; p < end; p += 8) {
+        const bool two = p + 4 < end;
+        const int i0 = perm[p];
+        const int i1 = two ? perm[p + 4] : i0;
+        const VT* r0 = reinterpret_cast<const VT*>(X + (int64_t)i0 * D);
+        const VT* r1 = reinterpret_cast<const VT*>(X + (int64_t)i1 * D);
+        float a00 = 0.f, a01 = 0.f, a10 = 0.f, a11 = 0.f;
+        for (int c0 = 0; c0 < D; c0 += 4 * 64 * V) {
+            VT x0[4], x1[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int col = c0 + (u * 64 + lane) * V;
+                const int g = col < D ? col / V : 0;
+                x0[u] = r0[g];
+                x1[u] = r1[g];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int col = c0 + (u * 64 + lane) * V;
+                if (col < D) {
+                    mult_sub_partial<V>(x0[u], col, th, stride, a00, a01);
+                    mult_sub_partial<V>(x1[u], col, th, stride, a10, a11);
+                }
+            }
         }
-        for (int off = 32; off > 0; off >>= 1) { a0 += __shfl_xor(a0, off); a1 += __shfl_xor(a1, off); }
+        for (int off = 32; off > 0; off >>= 1) {
+            a00 += __shfl_xor(a00, off); a01 += __shfl_xor(a01, off);
+            a10 += __shfl_xor(a10, off); a11 += __shfl_xor(a11, off);
+        }
         if (lane == 0) {
-            const float s0 = a0 + th[D], s1 = a1 + th[stride + D];
-            const uint32_t w = philox4x32_10((uint32_t)idx, 0, iter, TAG_SUB, key0, key1).v[0];
-            subs[idx] = (logit_u32<float>(w) < s0 - s1) ? 0 : 1;
+            uint32_t w = philox4x32_10((uint32_t)i0, 0, iter, TAG_SUB, key0, key1).v[0];
+            subs[i0] = (logit_u32<float>(w) < (a00 + th[D]) - (a01 + th[stride + D])) ? 0 : 1;
+            if (two) {
+                w = philox4x32_10((uint32_t)i1, 0, iter, TAG_SUB, key0, key1).v[0];
+                subs[i1] = (logit_u32<float>(w) < (a10 + th[D]) - (a11 + th[stride + D])) ? 0 : 1;
+            }
         }
     }
 }
@@ -288,13 +342,15 @@ __global__ void __launch_bounds__(kThreads) k_sub_mult(const int32_t* __restrict
 // Noise-free multinomial scores of documents [first, first + n), one wave per document, with the
 // sweeps' FP32 arithmetic: which = 0 the n x K cluster scores of k_label_mult (per cluster the
 // fmaf chain over the vocabulary in order, then the slot's constant), which = 1 the n x 2 scores of
-// the two sub-clusters of the document's label as k_sub_mult forms them (lane-strided partial
-// sums, butterfly reduction, constant).
+// the two sub-clusters of the document's label as k_sub_mult<V> forms them (mult_sub_partial's
+// lane partial sums, butterfly reduction, constant).
+template <int V>
 __global__ void __launch_bounds__(kThreads) k_score_mult(const int32_t* __restrict__ X, int D, int K,
                                                          const float* __restrict__ P, const float* __restrict__ SP,
                                                          int stride, const int32_t* __restrict__ labels,
                                                          int64_t first, int64_t n, int which,
                                                          float* __restrict__ out) {
+    typedef typename IRow<V>::T VT;
     const int lane = threadIdx.x & 63;
     for (int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); w < n; w += (int64_t)gridDim.x * 4) {
         const int64_t i = first + w;
@@ -309,10 +365,8 @@ __global__ void __launch_bounds__(kThreads) k_score_mult(const int32_t* __restri
         } else {
             const float* th = SP + (size_t)(2 * labels[i]) * stride;
             float a0 = 0.f, a1 = 0.f;
-            for (int d = lane; d < D; d += 64) {
-                const float xv = (float)x[d];
-                if (xv != 0.f) { a0 = fmaf(xv, th[d], a0); a1 = fmaf(xv, th[stride + d], a1); }
-            }
+            for (int col = lane * V; col < D; col += 64 * V)
+                mult_sub_partial<V>(reinterpret_cast<const VT*>(x)[col / V], col, th, stride, a0, a1);
             for (int off = 32; off > 0; off >>= 1) { a0 += __shfl_xor(a0, off); a1 += __shfl_xor(a1, off); }
             if (lane == 0) {
                 out[w * 2] = a0 + th[D];
@@ -323,32 +377,70 @@ __global__ void __launch_bounds__(kThreads) k_score_mult(const int32_t* __restri
 }
 
 // ---- sufficient statistics --------------------------------------------------------------------
-// Multinomial: exact integer sums per vocabulary entry.
+// Multinomial: exact integer sums per vocabulary entry, both sub-clusters of a chunk in one block.
+// Thread t owns words [t V, t V + V) of each 256 V-word stretch and every wave walks all of the
+// chunk's documents: a wave reads 64 (index, sub-label) pairs with one load each, then gathers the
+// rows eight at a time with the document index broadcast from a lane (readlane, so the row address
+// is scalar) and adds each row into the sub-cluster's int64 accumulators.  Integer sums are exact,
+// so the order does not matter; partials are written as (2 chunk + sub-cluster) rows.
+template <int V>
 __global__ void __launch_bounds__(kThreads) k_stats_mult(const int32_t* __restrict__ X, int D, int K, int CH,
                                                          const int32_t* __restrict__ perm,
                                                          const int32_t* __restrict__ label_start,
                                                          const int32_t* __restrict__ chunk_start,
                                                          const int32_t* __restrict__ subs, int E,
                                                          double* __restrict__ partials) {
-    const int c = blockIdx.x >> 1, s = blockIdx.x & 1;
+    typedef typename IRow<V>::T VT;
     int k, beg, end;
-    if (!chunk_of(c, K, chunk_start, label_start, CH, k, beg, end)) return;
-    double* out = partials + (size_t)blockIdx.x * E;
-    int64_t cnt = 0;
-    for (int d0 = 0; d0 < D; d0 += kThreads) {
-        const int d = d0 + threadIdx.x;
-        int64_t acc = 0;
-        int64_t n = 0;
-        for (int p = beg; p < end; ++p) {
-            const int idx = perm[p];
-            if (subs[idx] != s) continue;
-            ++n;
-            if (d < D) acc += X[(int64_t)idx * D + d];
+    if (!chunk_of(blockIdx.x, K, chunk_start, label_start, CH, k, beg, end)) return;
+    double* out0 = partials + (size_t)(2 * blockIdx.x) * E;
+    double* out1 = out0 + E;
+    const int lane = threadIdx.x & 63;
+    for (int c0 = 0; c0 < D; c0 += kThreads * V) {
+        const int col = c0 + threadIdx.x * V;
+        const int g = col < D ? col / V : 0;
+        int64_t a0[V], a1[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) { a0[v] = 0; a1[v] = 0; }
+        int64_t n0 = 0, n1 = 0;
+        for (int p0 = beg; p0 < end; p0 += 64) {
+            const int p = p0 + lane;
+            const int idx = p < end ? perm[p] : 0;
+            const int s = p < end ? subs[idx] : -1;
+            n0 += __popcll(__ballot(s == 0));
+            n1 += __popcll(__ballot(s == 1));
+            const int n = min(64, end - p0);
+            for (int j = 0; j < n; j += 8) {
+                VT xv[8];
+                int sv[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int jj = min(j + u, n - 1);
+                    const int ij = __builtin_amdgcn_readlane(idx, jj);
+                    const int sj = __builtin_amdgcn_readlane(s, jj);
+                    sv[u] = j + u < n ? sj : -1;
+                    xv[u] = reinterpret_cast<const VT*>(X + (int64_t)ij * D)[g];
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+#pragma unroll
+                    for (int v = 0; v < V; ++v) {
+                        const int64_t x = IRow<V>::at(xv[u], v);
+                        a0[v] += sv[u] == 0 ? x : 0;
+                        a1[v] += sv[u] == 1 ? x : 0;
+                    }
+                }
+            }
         }
-        if (d < D) out[1 + d] = (double)acc;
-        cnt = n;
+        if (col < D) {
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                out0[1 + col + v] = (double)a0[v];
+                out1[1 + col + v] = (double)a1[v];
+            }
+        }
+        if (c0 == 0 && threadIdx.x == 0) { out0[0] = (double)n0; out1[0] = (double)n1; }
     }
-    if (threadIdx.x == 0) out[0] = (double)cnt;
 }
 
 // Fixed-order reduction of the chunk partials of each (cluster, sub-cluster): one thread per
@@ -1394,8 +1486,8 @@ int dpmm_sample_sub_labels(dpmm_ctx* c, uint32_t iter) {
         if (c->params_f64) { DPMM_DISPATCH_DP64(c->DP, sg_f64, c, iter); }
         else { DPMM_DISPATCH_DP(c->DP, sg_f32, c, iter); }
     } else {
-        hipLaunchKernelGGL(k_sub_mult, dim3(chunks_upper(c, c->K)), dim3(kThreads), (size_t)2 * c->stride * 4,
-                           c->stream, reinterpret_cast<const int32_t*>(c->dX), c->D, c->K, c->CH, c->dPerm,
+        hipLaunchKernelGGL(c->D % 4 == 0 ? k_sub_mult<4> : k_sub_mult<1>, dim3(chunks_upper(c, c->K)),
+                           dim3(kThreads), (size_t)2 * c->stride * 4, c->stream, reinterpret_cast<const int32_t*>(c->dX), c->D, c->K, c->CH, c->dPerm,
                            c->dLabelStart, c->dChunkStart, reinterpret_cast<const float*>(c->dSP), c->stride,
                            c->key0(), c->key1(), iter, c->dSubs);
     }
@@ -1429,10 +1521,10 @@ int dpmm_log_likelihoods(dpmm_ctx* c, int K, int which, int64_t first, int64_t n
         else { DPMM_DISPATCH_DP(c->DP, score_f32, c, grid, first, n, which, dOut); }
     } else {
         const int grid = (int)std::min<int64_t>(4096, (n + 3) / 4);
-        hipLaunchKernelGGL(k_score_mult, dim3(grid), dim3(kThreads), 0, c->stream,
-                           reinterpret_cast<const int32_t*>(c->dX), c->D, c->K, reinterpret_cast<const float*>(c->dP),
-                           reinterpret_cast<const float*>(c->dSP), c->stride, c->dLabels, first, n, which,
-                           reinterpret_cast<float*>(dOut));
+        hipLaunchKernelGGL(c->D % 4 == 0 ? k_score_mult<4> : k_score_mult<1>, dim3(grid), dim3(kThreads), 0,
+                           c->stream, reinterpret_cast<const int32_t*>(c->dX), c->D, c->K,
+                           reinterpret_cast<const float*>(c->dP), reinterpret_cast<const float*>(c->dSP), c->stride,
+                           c->dLabels, first, n, which, reinterpret_cast<float*>(dOut));
     }
     hipError_t e = hipGetLastError();
     std::vector<unsigned char> h((size_t)n * cols * esz);
@@ -1469,7 +1561,8 @@ int dpmm_sufficient_statistics(dpmm_ctx* c, int K, int64_t* Nsub, double* sum, d
             if (c->dtype == DPMM_DTYPE_F64) { DPMM_DISPATCH_DP64(c->DP, st_f64, c); }
             else { DPMM_DISPATCH_DP(c->DP, st_f32, c); }
         } else {
-            hipLaunchKernelGGL(k_stats_mult, dim3(2 * nch), dim3(kThreads), 0, c->stream,
+            hipLaunchKernelGGL(c->D % 4 == 0 ? k_stats_mult<4> : k_stats_mult<1>, dim3(nch), dim3(kThreads), 0,
+                               c->stream,
                                reinterpret_cast<const int32_t*>(c->dX), c->D, K, c->CH, c->dPerm, c->dLabelStart,
                                c->dChunkStart, c->dSubs, (int)E, c->dPartials);
         }

@@ -81,16 +81,16 @@ def test_gaussian_scores_match_oracle(D, K, dtype):
     dev.close()
 
 
-@pytest.mark.parametrize("K", [1, 6, 70])
-def test_multinomial_scores_match_oracle(K):
+@pytest.mark.parametrize("K,D", [(1, 300), (6, 300), (70, 300), (6, 301)])
+def test_multinomial_scores_match_oracle(K, D):
     rng = np.random.default_rng(K + 3)
-    D, N = 300, 3000
+    N = 3000
     X, _, _ = multinomial_mixture(N, D, max(K, 2), seed=K)
     lt = np.log(rng.dirichlet(np.ones(D), size=K))
     slt = np.log(rng.dirichlet(np.ones(D), size=2 * K))
     w = rng.dirichlet(np.ones(K))
     lrw = rng.dirichlet([1.0, 1.0], size=K).ravel()
-    say(f"scores multinomial K={K}")
+    say(f"scores multinomial K={K} D={D}")
     dev = dp.DeviceSweep(X, dp.prior_type.Multinomial, seed=4)
     dev.set_multinomial_params(lt, w, slt, lrw, use_weight=False)
     s = dev.log_likelihoods(K, 0, 0, N)

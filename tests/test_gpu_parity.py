@@ -151,16 +151,18 @@ def test_initial_labels_match_oracle_schedule():
         assert sub[i] == w[1] & 1
 
 
-@pytest.mark.parametrize("K", [1, 6, 37])
-def test_multinomial_sweeps(K):
+@pytest.mark.parametrize("K,D", [(1, 60), (6, 60), (37, 60), (6, 61), (5, 1030), (3, 2052)])
+def test_multinomial_sweeps(K, D):
+    """D = 61 and 1030 read rows one word at a time, the others four; D = 1030 and 2052 take
+    several 256-lane column passes in the statistics kernel."""
     rng = np.random.default_rng(K)
-    D, N = 60, 20000
+    N = 20000
     X, gt, theta = multinomial_mixture(N, D, max(K, 2), seed=K)
     lt = np.log(rng.dirichlet(np.ones(D), size=K))
     slt = np.log(rng.dirichlet(np.ones(D), size=2 * K))
     w = rng.dirichlet(np.ones(K))
     lrw = rng.dirichlet([1.0, 1.0], size=K).ravel()
-    say(f"multinomial sweeps K={K}")
+    say(f"multinomial sweeps K={K} D={D}")
     dev = dp.DeviceSweep(X, dp.prior_type.Multinomial, seed=3)
     dev.set_multinomial_params(lt, w, slt, lrw, use_weight=False)
     dev.sample_cluster_labels(2)

@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "dpmm_kernels.h"
 
 namespace dpmm {
@@ -373,6 +375,138 @@ __global__ void __launch_bounds__(kThreads) k_stats_gauss_mfma(const TX* __restr
     }
 }
 
+// Statistics of both sub-clusters of one chunk on FP64 MFMA without LDS staging (DP <= 64).
+// Coordinates are taken in a permuted order: with T = DP / 16, lane (kp, r) reads the T consecutive
+// coordinates T r .. T r + T - 1 of point kp of a group of 4 in one vector load, and its e-th value
+// is element r of "tile" e, the 16 coordinates T r' + e.  Tile pair (e1 <= e2) accumulates
+// sum x[T a + e1] x[T b + e2] in one v_mfma_f64_16x16x4f64 per 4 points; output element v of lane
+// (kp, r) is a = 4 v + kp, b = r, written back to scatter entries (T a + e1, T b + e2) and their
+// mirror.  The sub-clusters are done one after the other (one set of accumulators): each wave
+// owns 64-point batches b = wave, wave + 4, ... of the chunk, moves the batch's points of the
+// sub-cluster to the front with one lane permute (ds_permute of a stable partition), then gathers
+// 4 groups of 4 rows at a time straight from global memory into MFMA operands.  The four waves'
+// accumulators are summed through LDS in fixed order (waves 3, 2, 1, then 0).
+template <typename TX, int DP>
+__global__ void __launch_bounds__(kThreads, sizeof(TX) == 4 ? 3 : 2) k_stats_gauss_wave(const TX* __restrict__ X, int D, int K, int CH,
+                                                               const int32_t* __restrict__ perm,
+                                                               const int32_t* __restrict__ label_start,
+                                                               const int32_t* __restrict__ chunk_start,
+                                                               const int32_t* __restrict__ subs,
+                                                               double* __restrict__ partials) {
+    typedef double f64x4 __attribute__((ext_vector_type(4)));
+    typedef TX txv __attribute__((ext_vector_type(DP / 16 == 4 ? 4 : 2)));
+    constexpr int T = DP / 16;
+    constexpr int NT = T * (T + 1) / 2;
+    constexpr size_t E = 1 + DP + (size_t)DP * DP;
+    constexpr int NV = 4 * NT + T;   // doubles per lane summed across the waves
+    static_assert(DP % 16 == 0 && DP <= 64, "wave statistics kernel covers DP = 16 .. 64");
+    __shared__ double red[NV * 64];
+    __shared__ int64_t cnt_w[4];
+    int k, beg, end;
+    if (!chunk_of(blockIdx.x, K, chunk_start, label_start, CH, k, beg, end)) return;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int kp = lane >> 4, r = lane & 15;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const bool vec = (T == 2 || T == 4) && D % T == 0;   // rows aligned to T elements
+    const int c0 = T * r;
+    double* out = partials + (size_t)(2 * blockIdx.x) * E;
+
+    for (int s = 0; s < 2; ++s, out += E) {
+        f64x4 acc[NT];
+        double sm[T];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[t] = f64x4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int e = 0; e < T; ++e) sm[e] = 0.0;
+        int64_t cnt = 0;
+        for (int b0 = beg + 64 * wv; b0 < end; b0 += 4 * 64) {
+            const int p = b0 + lane;
+            const int idx = p < end ? perm[p] : 0;
+            const bool mine = p < end && subs[idx] == s;
+            const uint64_t m = __ballot(mine);
+            const int ns = __popcll(m);
+            // stable partition: the sub-cluster's points first (the rest are never used)
+            const int dst = mine ? __popcll(m & lt) : ns + __popcll(~m & lt);
+            const int cidx = __builtin_amdgcn_ds_permute(dst << 2, idx);
+            cnt += ns;
+            for (int g = 0; g < ns; g += 16) {
+                TX xv[4][T];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {   // coordinates c0 .. c0 + T - 1 of 4 rows (zero past D)
+                    const TX* row = X + (int64_t)__shfl(cidx, min(g + 4 * u + kp, 63)) * D + c0;
+                    if (vec && c0 + T <= D) {
+                        if constexpr (T == 2 || T == 4) {
+                            const txv w = *reinterpret_cast<const txv*>(row);
+#pragma unroll
+                            for (int e = 0; e < T; ++e) xv[u][e] = w[e];
+                        }
+                    } else {
+#pragma unroll
+                        for (int e = 0; e < T; ++e) xv[u][e] = c0 + e < D ? row[e] : TX(0);
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (g + 4 * u >= ns) break;
+                    const bool ok = g + 4 * u + kp < ns;
+                    double v[T];
+#pragma unroll
+                    for (int e = 0; e < T; ++e) {
+                        v[e] = ok ? (double)xv[u][e] : 0.0;
+                        sm[e] += v[e];
+                    }
+                    int t = 0;
+#pragma unroll
+                    for (int e1 = 0; e1 < T; ++e1)
+#pragma unroll
+                        for (int e2 = e1; e2 < T; ++e2, ++t)
+                            acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(v[e1], v[e2], acc[t], 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();   // the previous sub-cluster's readers of red are done
+        if (lane == 0) cnt_w[wv] = cnt;
+        for (int w = 3; w >= 1; --w) {
+            if (wv == w) {
+                int i = 0;
+#pragma unroll
+                for (int t = 0; t < NT; ++t)
+#pragma unroll
+                    for (int c = 0; c < 4; ++c, ++i)
+                        red[i * 64 + lane] = (w == 3 ? 0.0 : red[i * 64 + lane]) + acc[t][c];
+#pragma unroll
+                for (int e = 0; e < T; ++e, ++i) red[i * 64 + lane] = (w == 3 ? 0.0 : red[i * 64 + lane]) + sm[e];
+            }
+            __syncthreads();
+        }
+        if (wv == 0) {
+            if (lane == 0) out[0] = (double)(cnt_w[3] + cnt_w[2] + cnt_w[1] + cnt_w[0]);
+            double* S = out + 1 + DP;
+            int i = 0, t = 0;
+#pragma unroll
+            for (int e1 = 0; e1 < T; ++e1)
+#pragma unroll
+                for (int e2 = e1; e2 < T; ++e2, ++t) {
+#pragma unroll
+                    for (int c = 0; c < 4; ++c, ++i) {
+                        const double val = red[i * 64 + lane] + acc[t][c];
+                        const int ra = T * (4 * c + kp) + e1, cb = T * r + e2;
+                        S[(size_t)ra * DP + cb] = val;
+                        S[(size_t)cb * DP + ra] = val;
+                    }
+                }
+            // first moments: the 4 points of each group (lanes r, r + 16, r + 32, r + 48), fixed order
+#pragma unroll
+            for (int e = 0; e < T; ++e, ++i) {
+                double v = red[i * 64 + lane] + sm[e];
+                v += __shfl_xor(v, 16);
+                v += __shfl_xor(v, 32);
+                if (kp == 0) out[1 + c0 + e] = v;
+            }
+        }
+    }
+}
+
 // Exact sub-label draw of listed points (the undecided points of the mixed sub-label screen), with
 // the arithmetic of k_sub_gauss: each point's two sub-cluster slots (of its label) are read from
 // global memory; workgroups stride over the list, whose length is read from device memory.
@@ -607,7 +741,10 @@ template <typename TX, int DP>
 void launch_stats_gauss(hipStream_t st, int nchunks, const TX* X, int D, int K, int CH, const int32_t* perm,
                         const int32_t* label_start, const int32_t* chunk_start, const int32_t* subs,
                         double* partials) {
-    if constexpr (DP % 16 == 0)
+    if constexpr (DP % 16 == 0 && DP <= 64)
+        hipLaunchKernelGGL((k_stats_gauss_wave<TX, DP>), dim3(nchunks), dim3(kThreads), 0, st, X, D, K, CH, perm,
+                           label_start, chunk_start, subs, partials);
+    else if constexpr (DP % 16 == 0)
         hipLaunchKernelGGL((k_stats_gauss_mfma<TX, DP>), dim3(nchunks), dim3(kThreads), 0, st, X, D, K, CH, perm,
                            label_start, chunk_start, subs, partials);
     else

@@ -43,6 +43,38 @@ __device__ __forceinline__ void lds_dma_copy(unsigned char* dst, const unsigned 
 // the LDS DMA copies included, has completed.
 __device__ __forceinline__ void lds_dma_wait() { __builtin_amdgcn_s_waitcnt(0x0f70); }
 
+// Wave-private staging of a recheck list: entries are appended by ballot into 64 LDS slots owned by
+// the wave and moved to the global list with one atomicAdd per 64 entries (and once at the end), so
+// waves that loop over many point blocks do not each add to the single global counter per block,
+// where the atomics serialise.  n is wave-uniform; every lane of the wave calls push / flush.
+struct WaveList {
+    int32_t* buf;   // 64 entries of LDS, private to the wave
+    int n;
+    __device__ __forceinline__ void sync_wave() {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    __device__ __forceinline__ void flush(int32_t* list, int* count, int lane) {
+        if (n == 0) return;
+        sync_wave();   // the lanes' appends are visible to every lane
+        int pos0 = 0;
+        if (lane == 0) pos0 = atomicAdd(count, n);
+        pos0 = __shfl(pos0, 0);
+        if (lane < n) list[pos0 + lane] = buf[lane];
+        sync_wave();   // read before the next appends overwrite
+        n = 0;
+    }
+    __device__ __forceinline__ void push(bool need, int32_t v, int32_t* list, int* count, int lane, uint64_t lt) {
+        const uint64_t m = __ballot(need);
+        if (!m) return;
+        const int c = __popcll(m);
+        if (n + c > 64) flush(list, count, lane);
+        if (need) buf[n + __popcll(m & lt)] = v;
+        n += c;
+    }
+};
+
 // Slot of one cluster for the screen: [tiles RB x KC x 64 lanes x 8 bf16 | mu (DP f32) | c | pad].
 __host__ __device__ constexpr int mx_tiles_bytes(int DP) { return (DP / 32) * (DP / 16) * 64 * 16; }
 __host__ __device__ constexpr int mx_slot_bytes(int DP) { return (mx_tiles_bytes(DP) + DP * 4 + 4 + 15) & ~15; }
@@ -204,6 +236,8 @@ __global__ void __launch_bounds__(kThreads) k_sub_gauss_mx(const float* __restri
     const float eps = (3.0f * 0x1p-18f + (5.0f * (float)DP + 16.0f) * 0x1p-24f) * (1.0f + 0x1p-6f);
     const int lane = threadIdx.x & 63, h = lane >> 5, pl = lane & 31, wv = threadIdx.x >> 6;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    __shared__ int32_t wlist[4][64];
+    WaveList rl{wlist[wv], 0};
     for (int p0 = beg + 32 * wv; p0 < end; p0 += 4 * 32) {   // wave-uniform trip count
         const int p = p0 + pl;
         const bool valid = p < end;
@@ -264,14 +298,9 @@ __global__ void __launch_bounds__(kThreads) k_sub_gauss_mx(const float* __restri
             if (fabsf(lg - d) > err) subs[idx] = lg < d ? 0 : 1;
             else need = true;
         }
-        const uint64_t m = __ballot(need);
-        if (m) {
-            int pos0 = 0;
-            if (lane == 0) pos0 = atomicAdd(recheck_count, __popcll(m));   // one atomic per wave
-            pos0 = __shfl(pos0, 0);
-            if (need) recheck[pos0 + __popcll(m & lt)] = idx;
-        }
+        rl.push(need, idx, recheck, recheck_count, lane, lt);
     }
+    rl.flush(recheck, recheck_count, lane);
 }
 
 template <int DP>
@@ -322,6 +351,8 @@ k_label_gauss_mxp(const float* __restrict__ X, int D, int K, int CH, const int32
     const bool in0 = lane < K && lane != a;
     const float sg0 = in0 ? psc[2 * lane] : 0.f, ck0 = in0 ? psc[2 * lane + 1] : 0.f, m0 = in0 ? mrow[lane] : 0.f;
     unsigned long long nev = 0, npass = 0;
+    __shared__ int32_t wlist[4][64];
+    WaveList rl{wlist[wv], 0};
     int idx_next = (beg + wv * 32 + pl < end) ? perm[beg + wv * 32 + pl] : 0;
     for (int p0 = beg + wv * 32; p0 < end; p0 += 4 * 32) {   // wave-uniform trip count
         const int pos = p0 + pl;
@@ -441,15 +472,9 @@ k_label_gauss_mxp(const float* __restrict__ X, int D, int K, int CH, const int32
         ++npass;
         const bool sure = sb - eb > U;
         if (own_valid && sure) labels[idx] = bi;
-        const bool need = own_valid && !sure;
-        const uint64_t m = __ballot(need);
-        if (m) {
-            int pos0 = 0;
-            if (lane == 0) pos0 = atomicAdd(recheck_count, __popcll(m));   // one atomic per wave
-            pos0 = __shfl(pos0, 0);
-            if (need) recheck[pos0 + __popcll(m & lt)] = (int32_t)idx;
-        }
+        rl.push(own_valid && !sure, (int32_t)idx, recheck, recheck_count, lane, lt);
     }
+    rl.flush(recheck, recheck_count, lane);
     __shared__ unsigned long long cnt[2][4];   // one pair of counter atomics per workgroup
     if (lane == 0) {
         cnt[0][wv] = nev;

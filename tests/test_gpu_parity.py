@@ -88,8 +88,13 @@ def test_label_sweep_realistic_margins_fp32():
 
 
 @pytest.mark.parametrize("D,K,dtype", [(2, 1, np.float64), (3, 4, np.float64), (7, 5, np.float32),
-                                       (32, 9, np.float32), (64, 3, np.float32), (96, 2, np.float32)])
+                                       (16, 4, np.float32), (17, 3, np.float32), (32, 9, np.float32),
+                                       (40, 3, np.float64), (48, 2, np.float32), (50, 5, np.float32),
+                                       (64, 3, np.float32), (64, 2, np.float64), (96, 2, np.float32),
+                                       (128, 2, np.float32)])
 def test_sufficient_statistics(D, K, dtype):
+    """Padded widths 16 .. 64 take the wave-local MFMA kernel (vector row loads when D is a multiple
+    of DP / 16, element loads otherwise), 96 and 128 the LDS-staged one."""
     rng = np.random.default_rng(D + 7 * K)
     N = 30007
     X = (rng.normal(size=(N, D)) * 20 + rng.uniform(-100, 100, size=D)).astype(dtype)

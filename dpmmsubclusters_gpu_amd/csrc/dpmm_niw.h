@@ -150,43 +150,92 @@ __device__ void niw_posterior(const double* __restrict__ stats, size_t E, int DP
     __syncthreads();
 }
 
-// In-place Cholesky of packed lower P (right-looking, one column per step).  Uniform result.
-__device__ bool chol_packed(double* P, int D, double* sh) {
-    for (int j = 0; j < D; ++j) {
-        if (threadIdx.x == 0) {
-            const double s = P[tri(j, j)];
-            const bool ok = s > 0.0 && isfinite(s);
-            const double d = ok ? sqrt(s) : 1.0;
-            P[tri(j, j)] = d;
-            sh[0] = 1.0 / d;
-            sh[1] = ok ? 1.0 : 0.0;
+// Row-parallel layout of the triangular routines below (blockDim = kThreads, D <= 128): thread t
+// owns row t / LN and, within it, the columns congruent to t % LN modulo LN.
+__device__ __forceinline__ int row_lanes(int D) { return D <= 64 ? 4 : 2; }
+
+// In-place Cholesky of packed lower P, right-looking with a lazily scaled pivot column: step k
+// subtracts (a_ik / a_kk) a_jk from the trailing entries (k < j <= i) with column k as the
+// earlier steps left it, while the owners of column k - 1, which nothing reads any more, scale it
+// by 1 / sqrt(a_{k-1,k-1}).  One barrier per column.  Uniform result: false when a pivot is not
+// positive and finite (every thread reads the same pivot).
+__device__ bool chol_packed(double* P, int D, double* /*unused*/) {
+    const int LN = row_lanes(D), r = threadIdx.x / LN, q = threadIdx.x % LN;
+    const int rb = r * (r + 1) / 2;
+    const bool own = r < D;
+    double psq = 0.0, pinv = 0.0;   // sqrt and reciprocal of the previous pivot
+    for (int k = 0; k < D; ++k) {
+        const double akk = P[tri(k, k)];
+        if (!(akk > 0.0) || !isfinite(akk)) return false;
+        if (k > 0 && own && q == 0 && r >= k - 1) {
+            if (r == k - 1) P[rb + r] = psq;
+            else P[rb + k - 1] *= pinv;
         }
-        __syncthreads();
-        if (sh[1] == 0.0) return false;
-        const double inv = sh[0];
-        for (int i = j + 1 + threadIdx.x; i < D; i += blockDim.x) P[tri(i, j)] *= inv;
-        __syncthreads();
-        const int n = D - j - 1, T = n * (n + 1) / 2;
-        for (int e = threadIdx.x; e < T; e += blockDim.x) {
-            int ii, kk;
-            tri_ij(e, ii, kk);
-            const int i = j + 1 + ii, k = j + 1 + kk;
-            P[tri(i, k)] -= P[tri(i, j)] * P[tri(k, j)];
+        if (own && r > k) {
+            const double f = P[rb + k] / akk;
+            for (int j = k + 1 + q; j <= r; j += LN) P[rb + j] -= f * P[tri(j, k)];
         }
+        psq = sqrt(akk);
+        pinv = 1.0 / psq;
         __syncthreads();
     }
+    if (own && q == 0 && r == D - 1) P[rb + r] = psq;
+    __syncthreads();
     return true;
 }
 
-// X = L^-1 for packed lower L (one thread per column, rows in order).
+// X = L^-1 for packed lower L, by forward substitution with rows in parallel: step k subtracts
+// (L_ik / L_kk) X_kj (j <= k) from every row i > k, row k as the earlier steps left it, while
+// row k - 1, which nothing reads any more, is scaled by 1 / L_{k-1,k-1}.  One barrier per row.
 __device__ void lower_inverse_packed(const double* L, double* X, int D) {
-    for (int j = threadIdx.x; j < D; j += blockDim.x) {
-        X[tri(j, j)] = 1.0 / L[tri(j, j)];
-        for (int i = j + 1; i < D; ++i) {
-            double s = 0.0;
-            for (int k = j; k < i; ++k) s += L[tri(i, k)] * X[tri(k, j)];
-            X[tri(i, j)] = -s / L[tri(i, i)];
+    const int LN = row_lanes(D), r = threadIdx.x / LN, q = threadIdx.x % LN;
+    const int rb = r * (r + 1) / 2;
+    const bool own = r < D;
+    if (own)
+        for (int j = q; j <= r; j += LN) X[rb + j] = j == r ? 1.0 : 0.0;
+    __syncthreads();
+    for (int k = 0; k < D; ++k) {
+        if (own && r > k) {
+            const double f = L[rb + k] / L[tri(k, k)];
+            const int kb = tri(k, 0);
+            for (int j = q; j <= k; j += LN) X[rb + j] -= f * X[kb + j];
         }
+        if (own && r == k - 1) {
+            const double inv = 1.0 / L[rb + r];
+            for (int j = q; j <= r; j += LN) X[rb + j] *= inv;
+        }
+        __syncthreads();
+    }
+    if (own && r == D - 1) {
+        const double inv = 1.0 / L[rb + r];
+        for (int j = q; j <= r; j += LN) X[rb + j] *= inv;
+    }
+    __syncthreads();
+}
+
+// B A for packed lower B and A (D <= 128), written over A: every thread forms its entries
+// (k from j to i, in order) in registers, then all are stored after a barrier.
+__device__ void lower_product_packed(const double* B, double* A, int D) {
+    constexpr int kMaxE = (128 * 129 / 2 + kThreads - 1) / kThreads;
+    const int T = D * (D + 1) / 2;
+    double acc[kMaxE];
+#pragma unroll
+    for (int m = 0; m < kMaxE; ++m) {
+        const int e = threadIdx.x + m * kThreads;
+        double s = 0.0;
+        if (e < T) {
+            int i, j;
+            tri_ij(e, i, j);
+            const int ib = tri(i, 0);
+            for (int k = j; k <= i; ++k) s += B[ib + k] * A[tri(k, j)];
+        }
+        acc[m] = s;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < kMaxE; ++m) {
+        const int e = threadIdx.x + m * kThreads;
+        if (e < T) A[e] = acc[m];
     }
     __syncthreads();
 }
@@ -283,7 +332,6 @@ __global__ void __launch_bounds__(kThreads) k_draw_params(const double* __restri
     __shared__ double scal[4];
     __shared__ double sh[2];
     __shared__ int mem[4];
-    __shared__ uint32_t wpos;
     const int p = blockIdx.x;
     const uint32_t c0 = (uint32_t)p;
     if (threadIdx.x < 4) mem[threadIdx.x] = members[4 * p + threadIdx.x];
@@ -332,34 +380,22 @@ __global__ void __launch_bounds__(kThreads) k_draw_params(const double* __restri
         return;
     }
     // Bartlett factor A (into Q): normals below the diagonal in row order (words 2m, 2m + 1 of
-    // normal m), then the D diagonal chi variates sqrt(2 Gamma((nu - i) / 2)) in sequence, then
-    // the D normals of mu.
+    // normal m), then the D normals of mu on the same stream; the D diagonal chi variates
+    // sqrt(2 Gamma((nu - i) / 2)) from the streams starting at block 1024 (16 + i), all in parallel.
     for (int e = threadIdx.x; e < T; e += blockDim.x) {
         int i, j;
         tri_ij(e, i, j);
         if (j < i) Q[e] = stream_normal_at(c0, iter, key0, key1, 2u * (uint32_t)(i * (i - 1) / 2 + j));
     }
-    if (threadIdx.x == 0) {
-        DevStream rs(c0, iter, TAG_PARAM, key0, key1, (uint32_t)(D * (D - 1)));
-        for (int i = 0; i < D; ++i) gd[i] = sqrt(2.0 * rs.gamma((vn - i) / 2.0));
-        wpos = rs.w;
-    }
-    __syncthreads();
     for (int i = threadIdx.x; i < D; i += blockDim.x) {
-        Q[tri(i, i)] = gd[i];
-        z[i] = stream_normal_at(c0, iter, key0, key1, wpos + 2u * (uint32_t)i);
+        DevStream rs(c0, iter, TAG_PARAM, key0, key1, 4096u * (16u + (uint32_t)i));
+        gd[i] = sqrt(2.0 * rs.gamma((vn - i) / 2.0));
+        z[i] = stream_normal_at(c0, iter, key0, key1, (uint32_t)(D * (D - 1)) + 2u * (uint32_t)i);
     }
     __syncthreads();
-    // R = B A (lower), in place over A: column j from the last row up, so every A[k][j] read is
-    // still unmodified.
-    for (int j = threadIdx.x; j < D; j += blockDim.x) {
-        for (int i = D - 1; i >= j; --i) {
-            double s = 0.0;
-            for (int k = j; k <= i; ++k) s += P[tri(i, k)] * Q[tri(k, j)];
-            Q[tri(i, j)] = s;
-        }
-    }
+    for (int i = threadIdx.x; i < D; i += blockDim.x) Q[tri(i, i)] = gd[i];
     __syncthreads();
+    lower_product_packed(P, Q, D);   // R = B A
     double ld = 0.0;
     for (int i = threadIdx.x; i < D; i += blockDim.x) ld += log(Q[tri(i, i)]);
     ld = block_sum(ld, red);

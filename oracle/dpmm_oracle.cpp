@@ -22,8 +22,10 @@
  *   split relabel     (i, 0, t, 5)      word0 & 1 -> sub-label
  *   sequential streams (c0, n, t, tag) with n = 0, 1, 2, ... (4 words per n):
  *     tag 6 parameter draw of slot 3*cluster + {0 cluster, 1 left, 2 right}; a Dirichlet
- *           (multinomial) draw gives coordinate d the stream starting at n = 1024 d, so the
- *           D gammas are independent of each other's rejection counts
+ *           (multinomial) draw gives coordinate d the stream starting at n = 1024 d, and the
+ *           NIW draw's D Bartlett chi variates take the streams starting at n = 1024 (16 + i),
+ *           so the D gammas are independent of each other's rejection counts (the NIW draw's
+ *           normals, D (D - 1) / 2 of the Bartlett factor then D of mu, stay on n = 0, 1, ...)
  *     tag 7 left/right weights of a cluster, tag 8 mixture weights (c0 = 0),
  *     tag 9 split-test uniform (c0 = cluster), tag 10 merge-test uniform (c0 = i<<16 | j).
  * uniform(w) = (w + 0.5) 2^-32; normal = Box-Muller cos branch of two uniforms;
@@ -309,7 +311,8 @@ double log_marginal(const Model& M, const Hyper& h, const Hyper& p, const Stats&
  * (stats rinvwish.ipp:35-58); Sigma = inv(Lambda); mu = m + diag(chol(Sigma / k)) o z, the
  * reference's r8vec_multinormal_sample reading only the diagonal (pdflib.cpp:4007-4014),
  * or m + chol(Sigma / k) z when full_mu_cov. */
-bool niw_draw(int D, const Hyper& h, Stream& rs, int full_mu_cov, Dist& out) {
+bool niw_draw(int D, const Hyper& h, uint64_t seed, uint32_t slot, uint32_t t, int full_mu_cov, Dist& out) {
+    Stream rs(seed, slot, t, TAG_PARAM);
     std::vector<double> Psi((size_t)D * D), PsiInv, B;
     for (size_t i = 0; i < Psi.size(); ++i) Psi[i] = h.v * h.psi[i];
     if (!inv_spd(D, Psi, PsiInv)) return false;
@@ -318,7 +321,10 @@ bool niw_draw(int D, const Hyper& h, Stream& rs, int full_mu_cov, Dist& out) {
     std::vector<double> A((size_t)D * D, 0.0);
     for (int i = 1; i < D; ++i)
         for (int j = 0; j < i; ++j) A[(size_t)i * D + j] = rs.normal();
-    for (int i = 0; i < D; ++i) A[(size_t)i * D + i] = std::sqrt(2.0 * rs.gamma((h.v - i) / 2.0));
+    for (int i = 0; i < D; ++i) {
+        Stream gs(seed, slot, t, TAG_PARAM, 1024u * (16u + (uint32_t)i));
+        A[(size_t)i * D + i] = std::sqrt(2.0 * gs.gamma((h.v - i) / 2.0));
+    }
     std::vector<double> T((size_t)D * D, 0.0);   /* T = B A (lower) */
     for (int i = 0; i < D; ++i)
         for (int j = 0; j <= i; ++j) {
@@ -376,8 +382,7 @@ void mult_draw(int D, const Hyper& h, uint64_t seed, uint32_t slot, uint32_t t, 
 
 bool draw_dist(const Model& M, const Hyper& h, uint32_t slot, uint32_t t, Dist& out) {
     if (M.prior == 1) { mult_draw(M.D, h, M.seed, slot, t, out); return true; }
-    Stream rs(M.seed, slot, t, TAG_PARAM);
-    return niw_draw(M.D, h, rs, M.full_mu_cov, out);
+    return niw_draw(M.D, h, M.seed, slot, t, M.full_mu_cov, out);
 }
 
 /* ------------------------------------------------------------ log-likelihoods ------ */
@@ -829,9 +834,8 @@ void orc_niw_draw(int D, double k, const double* m, double v, const double* psi,
                   uint32_t iter, int full_mu_cov, double* mu, double* inv_sigma, double* logdet_sigma) {
     Hyper h;
     h.k = k; h.v = v; h.m.assign(m, m + D); h.psi.assign(psi, psi + (size_t)D * D);
-    Stream rs(seed, slot, iter, TAG_PARAM);
     Dist d;
-    if (!niw_draw(D, h, rs, full_mu_cov, d)) { *logdet_sigma = std::numeric_limits<double>::quiet_NaN(); return; }
+    if (!niw_draw(D, h, seed, slot, iter, full_mu_cov, d)) { *logdet_sigma = std::numeric_limits<double>::quiet_NaN(); return; }
     std::copy(d.mu.begin(), d.mu.end(), mu);
     std::copy(d.lambda.begin(), d.lambda.end(), inv_sigma);
     *logdet_sigma = d.logdet_sigma;

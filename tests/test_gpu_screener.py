@@ -134,7 +134,9 @@ def test_screener_gain_end_to_end():
     assert len(y) > 5000 and y.sum() >= 20
     s, hist = scr.train(X, y, epochs=30, batch=4096, lr=2e-3, seed=2)
     z = s.logits(X)
-    thr = scr.threshold_for_recall(z, y, recall=1.0, margin=2.0)
+    # 8 logits below the least plausible training pair (the margin the C5 workload uses): a pair
+    # the training data did not cover is still tested exactly
+    thr = scr.threshold_for_recall(z, y, recall=1.0, margin=8.0)
     assert np.all(z[a] > thr)
     Xe, gt, _ = gaussian_mixture(60_000, 16, 20, seed=13, spread=10.0, dtype=np.float32)
     hyper = dp.niw_hyperparams(1.0, np.zeros(16), 19.0, np.eye(16))

@@ -210,44 +210,57 @@ __global__ void k_scan_labels(const int32_t* __restrict__ rowtot, int K, int CH,
     chunk_start[K] = c;
 }
 
-// Stable scatter: within a block, points keep index order per label (wave ballot ranks,
-// waves in order, rounds in order).
+// Stable scatter: within a block, points keep index order per label (ranks within a wave, waves in
+// order, rounds in order).  A lane's rank among the wave's lanes of its label comes from a 64-bit
+// lane mask per (wave, label) in LDS that every lane ORs its bit into: rank = popcount of the lower
+// lanes' bits, count = popcount of all, whatever the number of distinct labels in the wave.
+// Dynamic LDS: base[K] ints, wc[4][K] ints, then (8-byte aligned) masks[4][K].
 __global__ void __launch_bounds__(kThreads) k_scatter(const int32_t* __restrict__ labels, int64_t N, int K, int nblk,
                                                       int clamp, const int32_t* __restrict__ hist,
                                                       const int32_t* __restrict__ label_start,
                                                       int32_t* __restrict__ perm) {
-    __shared__ int base[kMaxK];
-    __shared__ int wc[4][kMaxK];
+    extern __shared__ __align__(16) unsigned char smem[];
+    int* base = reinterpret_cast<int*>(smem);
+    int* wc = base + K;   // [4][K]
+    unsigned long long* masks = reinterpret_cast<unsigned long long*>(smem + (((size_t)5 * K * 4 + 15) & ~(size_t)15));
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    unsigned long long* wmask = masks + (size_t)wv * K;
     for (int k = threadIdx.x; k < K; k += kThreads) base[k] = label_start[k] + hist[(size_t)k * nblk + blockIdx.x];
+    for (int e = threadIdx.x; e < 4 * K; e += kThreads) masks[e] = 0ull;
     const int64_t b0 = (int64_t)blockIdx.x * kBlockPts;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     for (int r = 0; r < kBlockPts / kThreads; ++r) {
-        for (int e = threadIdx.x; e < 4 * K; e += kThreads) wc[e / K][e % K] = 0;
+        for (int e = threadIdx.x; e < 4 * K; e += kThreads) wc[e] = 0;
         __syncthreads();
         const int64_t i = b0 + r * kThreads + threadIdx.x;
         int Li = i < N ? labels[i] : -1;
         if (clamp && i < N && (unsigned)Li >= (unsigned)K) Li = 0;
         const bool valid = (unsigned)Li < (unsigned)K;
-        const int L = valid ? Li : -1;
-        uint64_t todo = __ballot(valid);
         int rank = 0, cnt = 0;
-        while (todo) {
-            const int src = __ffsll((unsigned long long)todo) - 1;
-            const int Ls = __shfl(L, src);
-            const uint64_t m = __ballot(L == Ls);
-            if (L == Ls) { rank = __popcll(m & lt); cnt = __popcll(m); }
-            todo &= ~m;
+        if (valid) atomicOr(&wmask[Li], 1ull << lane);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (valid) {
+            const unsigned long long m = wmask[Li];
+            rank = __popcll(m & lt);
+            cnt = __popcll(m);
         }
-        if (valid && rank == 0) wc[wv][L] = cnt;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (valid && rank == 0) {
+            wc[wv * K + Li] = cnt;
+            wmask[Li] = 0ull;   // every lane of the label has read it
+        }
         __syncthreads();
         if (valid) {
-            int pos = base[L] + rank;
-            for (int w2 = 0; w2 < wv; ++w2) pos += wc[w2][L];
+            int pos = base[Li] + rank;
+            for (int w2 = 0; w2 < wv; ++w2) pos += wc[w2 * K + Li];
             perm[pos] = (int32_t)i;
         }
         __syncthreads();
-        for (int k = threadIdx.x; k < K; k += kThreads) base[k] += wc[0][k] + wc[1][k] + wc[2][k] + wc[3][k];
+        for (int k = threadIdx.x; k < K; k += kThreads) base[k] += wc[k] + wc[K + k] + wc[2 * K + k] + wc[3 * K + k];
         __syncthreads();
     }
 }
@@ -453,7 +466,16 @@ __global__ void __launch_bounds__(kThreads) k_stats_reduce(const double* __restr
     if (e >= E) return;
     const int c0 = chunk_start[k], c1 = chunk_start[k + 1];
     double v = 0.0;
-    for (int c = c0; c < c1; ++c) v += partials[(size_t)(2 * c + s) * E + e];
+    const double* src = partials + (size_t)s * E + e;
+    int c = c0;
+    for (; c + 8 <= c1; c += 8) {   // eight loads in flight, added in chunk order
+        double t[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) t[u] = src[(size_t)(2 * (c + u)) * E];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v += t[u];
+    }
+    for (; c < c1; ++c) v += src[(size_t)(2 * c) * E];
     stats[(size_t)q * E + e] = v;
 }
 
@@ -777,6 +799,8 @@ int chunk_size(const dpmm_ctx* c) {
 
 int chunks_upper(const dpmm_ctx* c, int K) { return (int)((c->N + c->CH - 1) / c->CH) + K; }
 
+int allow_lds(const void* kernel, size_t bytes);
+
 int build_partition(dpmm_ctx* c, int K, int clamp) {
     int rc = ensure_partition(c, K);
     if (rc) return rc;
@@ -786,8 +810,11 @@ int build_partition(dpmm_ctx* c, int K, int clamp) {
     hipLaunchKernelGGL(k_scan_rows, dim3(K), dim3(kThreads), 0, c->stream, c->dHist, c->nblk, c->dRowtot);
     hipLaunchKernelGGL(k_scan_labels, dim3(1), dim3(64), 0, c->stream, c->dRowtot, K, c->CH, c->dLabelStart,
                        c->dChunkStart);
-    hipLaunchKernelGGL(k_scatter, dim3(c->nblk), dim3(kThreads), 0, c->stream, c->dLabels, c->N, K, c->nblk, clamp,
-                       c->dHist, c->dLabelStart, c->dPerm);
+    const size_t scatter_lds = (((size_t)5 * K * 4 + 15) & ~(size_t)15) + (size_t)4 * K * 8;
+    int lrc = allow_lds(reinterpret_cast<const void*>(k_scatter), scatter_lds);
+    if (lrc) return lrc;
+    hipLaunchKernelGGL(k_scatter, dim3(c->nblk), dim3(kThreads), scatter_lds, c->stream, c->dLabels, c->N, K, c->nblk,
+                       clamp, c->dHist, c->dLabelStart, c->dPerm);
     HIPCHK(hipGetLastError());
     c->partition_valid = true;
     c->Kpartition = K;

@@ -402,10 +402,12 @@ __global__ void __launch_bounds__(kThreads) k_stats_mult(const int32_t* __restri
                                                          const int32_t* __restrict__ label_start,
                                                          const int32_t* __restrict__ chunk_start,
                                                          const int32_t* __restrict__ subs, int E,
-                                                         double* __restrict__ partials) {
+                                                         double* __restrict__ partials,
+                                                         const int32_t* __restrict__ dirty) {
     typedef typename IRow<V>::T VT;
     int k, beg, end;
     if (!chunk_of(blockIdx.x, K, chunk_start, label_start, CH, k, beg, end)) return;
+    if (dirty && !dirty[k]) return;   // statistics of this cluster are kept from the last pass
     double* out0 = partials + (size_t)(2 * blockIdx.x) * E;
     double* out1 = out0 + E;
     const int lane = threadIdx.x & 63;
@@ -460,10 +462,11 @@ __global__ void __launch_bounds__(kThreads) k_stats_mult(const int32_t* __restri
 // statistic entry, summing that entry over the cluster's chunks in chunk order.
 __global__ void __launch_bounds__(kThreads) k_stats_reduce(const double* __restrict__ partials,
                                                            const int32_t* __restrict__ chunk_start, int K, int E,
-                                                           double* __restrict__ stats) {
+                                                           double* __restrict__ stats,
+                                                           const int32_t* __restrict__ dirty) {
     const int q = blockIdx.x, k = q >> 1, s = q & 1;
     const int e = blockIdx.y * kThreads + threadIdx.x;
-    if (e >= E) return;
+    if (e >= E || (dirty && !dirty[k])) return;
     const int c0 = chunk_start[k], c1 = chunk_start[k + 1];
     double v = 0.0;
     const double* src = partials + (size_t)s * E + e;
@@ -717,6 +720,16 @@ struct dpmm_ctx {
     // dStats holds the statistics of the current labels and sub-labels, for Kpartition clusters
     bool stats_cached = false;
     int Kstats = 0;   // clusters whose statistics dStats holds (entries 0 .. 2 Kstats - 1)
+    // After a relabel that only redraws the sub-labels of some clusters (a reset), the chunk
+    // partials of every other cluster are still those of the current labels: the next statistics
+    // pass recomputes the flagged clusters only.
+    bool partials_valid = false;
+    int partialsK = 0;
+    bool dirty_pending = false;
+    std::vector<int32_t> dirty_host;
+    int32_t* dDirty = nullptr;
+    size_t dirtyCap = 0;
+    const int32_t* stats_mask = nullptr;   // dDirty during a partial pass, else null
     // prior and buffers of the batched conjugate work
     bool has_prior = false;
     PriorArgs prior_args{};
@@ -804,6 +817,7 @@ int allow_lds(const void* kernel, size_t bytes);
 int build_partition(dpmm_ctx* c, int K, int clamp) {
     int rc = ensure_partition(c, K);
     if (rc) return rc;
+    c->partials_valid = false;   // the chunk layout may change
     c->CH = chunk_size(c);
     hipLaunchKernelGGL(k_block_hist, dim3(c->nblk), dim3(kThreads), 0, c->stream, c->dLabels, c->N, K, c->nblk, clamp,
                        c->dHist);
@@ -819,6 +833,7 @@ int build_partition(dpmm_ctx* c, int K, int clamp) {
     c->partition_valid = true;
     c->Kpartition = K;
     c->stats_cached = false;
+    c->partials_valid = false;
     return DPMM_OK;
 }
 
@@ -1086,7 +1101,7 @@ template <typename TX, int DP>
 void run_stats_gauss(dpmm_ctx* c) {
     const int K = c->Kpartition;
     launch_stats_gauss<TX, DP>(c->stream, chunks_upper(c, K), reinterpret_cast<const TX*>(c->dX), c->D, K, c->CH,
-                               c->dPerm, c->dLabelStart, c->dChunkStart, c->dSubs, c->dPartials);
+                               c->dPerm, c->dLabelStart, c->dChunkStart, c->dSubs, c->dPartials, c->stats_mask);
 }
 
 #define DPMM_DISPATCH_DP(DPV, FN, ...)            \
@@ -1319,7 +1334,7 @@ int dpmm_destroy(dpmm_ctx* c) {
                     (void*)c->dPartials, (void*)c->dStats, c->dPB, (void*)c->dRecheck, (void*)c->dRecheckCount,
                     (void*)c->dPrior, (void*)c->dMembers, (void*)c->dLM, (void*)c->dDraws, (void*)c->dDrawStatus,
                     (void*)c->dLogW, (void*)c->dFeat, (void*)c->dMB, c->dSPB, (void*)c->dPruneM,
-                    (void*)c->dPruneSC, (void*)c->dEvals})
+                    (void*)c->dPruneSC, (void*)c->dEvals, (void*)c->dDirty})
         if (p) (void)hipFree(p);
     if (c->hStats) (void)hipHostFree(c->hStats);
     if (c->hEvals) (void)hipHostFree(c->hEvals);
@@ -1368,6 +1383,7 @@ int dpmm_sample_initial_labels(dpmm_ctx* c, int K0) {
     HIPCHK(hipStreamSynchronize(c->stream));
     c->partition_valid = false;
     c->stats_cached = false;
+    c->partials_valid = false;
     return DPMM_OK;
 }
 
@@ -1377,6 +1393,7 @@ int dpmm_set_labels(dpmm_ctx* c, const int32_t* labels, const int32_t* subs) {
     if (subs) HIPCHK(hipMemcpy(c->dSubs, subs, c->N * sizeof(int32_t), hipMemcpyHostToDevice));
     c->partition_valid = false;
     c->stats_cached = false;
+    c->partials_valid = false;
     return DPMM_OK;
 }
 
@@ -1494,6 +1511,7 @@ int dpmm_sample_cluster_labels(dpmm_ctx* c, uint32_t iter, int fin) {
     c->ev_label = true;
     c->partition_valid = false;
     c->stats_cached = false;
+    c->partials_valid = false;
     return DPMM_OK;
 }
 
@@ -1506,6 +1524,7 @@ int dpmm_sample_sub_labels(dpmm_ctx* c, uint32_t iter) {
     }
     (void)hipEventRecord(c->ev[2], c->stream);
     c->stats_cached = false;
+    c->partials_valid = false;
     if (c->prior == DPMM_PRIOR_GAUSSIAN && c->precision == DPMM_PREC_MIXED && c->spb_valid) {
         int rc = run_sub_mixed(c, iter);
         if (rc) return rc;
@@ -1578,12 +1597,21 @@ int dpmm_sufficient_statistics(dpmm_ctx* c, int K, int64_t* Nsub, double* sum, d
     }
     const size_t E = c->statEntries();
     const int nch = chunks_upper(c, K);
-    (void)hipEventRecord(c->ev[4], c->stream);
     if (!(c->stats_cached && c->Kpartition == K)) {
+        const bool partial = c->dirty_pending && c->partials_valid && c->partialsK == K && c->Kstats == K &&
+                             (int)c->dirty_host.size() == K;
         int rc = grow(c->dPartials, c->partialsCap, (size_t)2 * nch * E);
         if (rc) return rc;
         rc = grow(c->dStats, c->statsCap, (size_t)2 * K * E);
         if (rc) return rc;
+        c->stats_mask = nullptr;
+        if (partial) {
+            if ((rc = grow(c->dDirty, c->dirtyCap, (size_t)K))) return rc;
+            HIPCHK(hipMemcpyAsync(c->dDirty, c->dirty_host.data(), (size_t)K * 4, hipMemcpyHostToDevice, c->stream));
+            c->stats_mask = c->dDirty;
+        } else {
+            (void)hipEventRecord(c->ev[4], c->stream);   // the phase timing covers full passes only
+        }
         if (c->prior == DPMM_PRIOR_GAUSSIAN) {
             if (c->dtype == DPMM_DTYPE_F64) { DPMM_DISPATCH_DP64(c->DP, st_f64, c); }
             else { DPMM_DISPATCH_DP(c->DP, st_f32, c); }
@@ -1591,16 +1619,22 @@ int dpmm_sufficient_statistics(dpmm_ctx* c, int K, int64_t* Nsub, double* sum, d
             hipLaunchKernelGGL(c->D % 4 == 0 ? k_stats_mult<4> : k_stats_mult<1>, dim3(nch), dim3(kThreads), 0,
                                c->stream,
                                reinterpret_cast<const int32_t*>(c->dX), c->D, K, c->CH, c->dPerm, c->dLabelStart,
-                               c->dChunkStart, c->dSubs, (int)E, c->dPartials);
+                               c->dChunkStart, c->dSubs, (int)E, c->dPartials, c->stats_mask);
         }
         hipLaunchKernelGGL(k_stats_reduce, dim3(2 * K, (unsigned)((E + kThreads - 1) / kThreads)), dim3(kThreads), 0,
-                           c->stream, c->dPartials, c->dChunkStart, K, (int)E, c->dStats);
+                           c->stream, c->dPartials, c->dChunkStart, K, (int)E, c->dStats, c->stats_mask);
+        if (!partial) {
+            (void)hipEventRecord(c->ev[5], c->stream);
+            c->ev_stats = true;
+        }
+        c->stats_mask = nullptr;
         c->stats_cached = true;
         c->Kstats = K;
+        c->partials_valid = true;
+        c->partialsK = K;
+        c->dirty_pending = false;
     }
-    (void)hipEventRecord(c->ev[5], c->stream);
     HIPCHK(hipGetLastError());
-    c->ev_stats = true;
     const size_t n = sum ? (size_t)2 * K * E : (size_t)2 * K;
     if (c->hStatsCap < n) {
         if (c->hStats) (void)hipHostFree(c->hStats);
@@ -1648,6 +1682,16 @@ int dpmm_relabel(dpmm_ctx* c, uint32_t iter, int K_old, const int32_t* new_label
     HIPCHK(hipStreamSynchronize(c->stream));
     if (labels_change) c->partition_valid = false;
     c->stats_cached = false;
+    bool resets_only = !labels_change;
+    for (int k = 0; k < K_old; ++k) resets_only = resets_only && (sub_action[k] == 0 || sub_action[k] == 1);
+    if (resets_only && c->partials_valid && c->partialsK == K_old) {
+        if (!c->dirty_pending) c->dirty_host.assign(K_old, 0);
+        for (int k = 0; k < K_old; ++k) c->dirty_host[k] |= sub_action[k] != 0 ? 1 : 0;
+        c->dirty_pending = true;
+    } else {
+        c->partials_valid = false;
+        c->dirty_pending = false;
+    }
     return DPMM_OK;
 }
 
@@ -1800,7 +1844,8 @@ int dpmm_set_statistics(dpmm_ctx* c, int K, const int64_t* Nsub, const double* s
     HIPCHK(hipMemcpyAsync(c->dStats, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     c->Kstats = K;
-    c->stats_cached = false;   // not the statistics of the current labels
+    c->stats_cached = false;
+    c->partials_valid = false;   // not the statistics of the current labels
     return DPMM_OK;
 }
 

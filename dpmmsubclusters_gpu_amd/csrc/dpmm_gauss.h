@@ -227,7 +227,8 @@ __global__ void __launch_bounds__(kThreads) k_stats_gauss_mfma(const TX* __restr
                                                                const int32_t* __restrict__ label_start,
                                                                const int32_t* __restrict__ chunk_start,
                                                                const int32_t* __restrict__ subs,
-                                                               double* __restrict__ partials) {
+                                                               double* __restrict__ partials,
+                                                               const int32_t* __restrict__ dirty) {
     typedef double f64x4 __attribute__((ext_vector_type(4)));
     constexpr int T = DP / 16;
     constexpr int NTILE = T * (T + 1) / 2;
@@ -244,6 +245,7 @@ __global__ void __launch_bounds__(kThreads) k_stats_gauss_mfma(const TX* __restr
     __shared__ int nls[2];
     int k, beg, end;
     if (!chunk_of(blockIdx.x, K, chunk_start, label_start, CH, k, beg, end)) return;
+    if (dirty && !dirty[k]) return;   // statistics of this cluster are kept from the last pass
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int kp = lane >> 4, r = lane & 15;
     int us[TPW], ca[TPW], cb[TPW];
@@ -392,7 +394,8 @@ __global__ void __launch_bounds__(kThreads, sizeof(TX) == 4 ? 3 : 2) k_stats_gau
                                                                const int32_t* __restrict__ label_start,
                                                                const int32_t* __restrict__ chunk_start,
                                                                const int32_t* __restrict__ subs,
-                                                               double* __restrict__ partials) {
+                                                               double* __restrict__ partials,
+                                                               const int32_t* __restrict__ dirty) {
     typedef double f64x4 __attribute__((ext_vector_type(4)));
     typedef TX txv __attribute__((ext_vector_type(DP / 16 == 4 ? 4 : 2)));
     constexpr int T = DP / 16;
@@ -404,6 +407,7 @@ __global__ void __launch_bounds__(kThreads, sizeof(TX) == 4 ? 3 : 2) k_stats_gau
     __shared__ int64_t cnt_w[4];
     int k, beg, end;
     if (!chunk_of(blockIdx.x, K, chunk_start, label_start, CH, k, beg, end)) return;
+    if (dirty && !dirty[k]) return;   // statistics of this cluster are kept from the last pass
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int kp = lane >> 4, r = lane & 15;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -600,7 +604,8 @@ __global__ void __launch_bounds__(kThreads) k_stats_gauss(const TX* __restrict__
                                                           const int32_t* __restrict__ label_start,
                                                           const int32_t* __restrict__ chunk_start,
                                                           const int32_t* __restrict__ subs,
-                                                          double* __restrict__ partials) {
+                                                          double* __restrict__ partials,
+                                                               const int32_t* __restrict__ dirty) {
     constexpr int NB = DP / 4;
     constexpr int NTRI = NB * (NB + 1) / 2;
     constexpr int R = (NTRI + kThreads - 1) / kThreads;
@@ -612,6 +617,7 @@ __global__ void __launch_bounds__(kThreads) k_stats_gauss(const TX* __restrict__
     const int c = blockIdx.x >> 1, s = blockIdx.x & 1;
     int k, beg, end;
     if (!chunk_of(c, K, chunk_start, label_start, CH, k, beg, end)) return;
+    if (dirty && !dirty[k]) return;   // statistics of this cluster are kept from the last pass
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     int bi[R], bj[R];
 #pragma unroll
@@ -740,16 +746,16 @@ void launch_score_gauss(hipStream_t st, int grid, const T* X, int D, int K, cons
 template <typename TX, int DP>
 void launch_stats_gauss(hipStream_t st, int nchunks, const TX* X, int D, int K, int CH, const int32_t* perm,
                         const int32_t* label_start, const int32_t* chunk_start, const int32_t* subs,
-                        double* partials) {
+                        double* partials, const int32_t* dirty) {
     if constexpr (DP % 16 == 0 && DP <= 64)
         hipLaunchKernelGGL((k_stats_gauss_wave<TX, DP>), dim3(nchunks), dim3(kThreads), 0, st, X, D, K, CH, perm,
-                           label_start, chunk_start, subs, partials);
+                           label_start, chunk_start, subs, partials, dirty);
     else if constexpr (DP % 16 == 0)
         hipLaunchKernelGGL((k_stats_gauss_mfma<TX, DP>), dim3(nchunks), dim3(kThreads), 0, st, X, D, K, CH, perm,
-                           label_start, chunk_start, subs, partials);
+                           label_start, chunk_start, subs, partials, dirty);
     else
         hipLaunchKernelGGL((k_stats_gauss<TX, DP>), dim3(2 * nchunks), dim3(kThreads), 0, st, X, D, K, CH, perm,
-                           label_start, chunk_start, subs, partials);
+                           label_start, chunk_start, subs, partials, dirty);
 }
 
 #define DPMM_GAUSS_INSTANTIATE(T, DP)                                                                         \
@@ -762,7 +768,8 @@ void launch_stats_gauss(hipStream_t st, int nchunks, const TX* X, int D, int K, 
                                           const int32_t*, const int32_t*, const T*, uint32_t, uint32_t,        \
                                           uint32_t, int32_t*);                                                 \
     template void launch_stats_gauss<T, DP>(hipStream_t, int, const T*, int, int, int, const int32_t*,         \
-                                            const int32_t*, const int32_t*, const int32_t*, double*);          \
+                                            const int32_t*, const int32_t*, const int32_t*, double*,           \
+                                            const int32_t*);                                                   \
     template void launch_sub_gauss_list<T, DP>(hipStream_t, int, const T*, int, const T*, const int32_t*,      \
                                                const int32_t*, const int*, uint32_t, uint32_t, uint32_t,       \
                                                int32_t*);                                                      \
@@ -782,7 +789,8 @@ void launch_stats_gauss(hipStream_t st, int nchunks, const TX* X, int D, int K, 
                                                  const int32_t*, const int32_t*, const T*, uint32_t, uint32_t, \
                                                  uint32_t, int32_t*);                                          \
     extern template void launch_stats_gauss<T, DP>(hipStream_t, int, const T*, int, int, int, const int32_t*,  \
-                                                   const int32_t*, const int32_t*, const int32_t*, double*);   \
+                                                   const int32_t*, const int32_t*, const int32_t*, double*,    \
+                                                   const int32_t*);                                            \
     extern template void launch_sub_gauss_list<T, DP>(hipStream_t, int, const T*, int, const T*,               \
                                                       const int32_t*, const int32_t*, const int*, uint32_t,    \
                                                       uint32_t, uint32_t, int32_t*);                           \

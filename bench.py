@@ -22,9 +22,9 @@ PEAK = {"fp32": 157.3, "fp64": 78.6, "bf16": 2500.0}   # dense TFLOP/s, MI355X
 HBM_PEAK = 8000.0                                       # GB/s
 # HBM bytes per launch of a phase's kernel, measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
 # (scaled by what the counters read for a copy of known size); keyed by (config, precision, phase)
-TRAFFIC = {("c3_d64", "mixed", "statistics"): 3.274e9 + 0.137e9,
-           ("c3_d64", "mixed", "label_sweep"): 2.866e9 + 0.269e9,
-           ("c3_d64", "mixed", "sub_sweep"): 2.621e9 + 0.272e9}
+TRAFFIC = {("c3_d64", "mixed", "statistics"): 3.996e9 + 0.140e9,
+           ("c3_d64", "mixed", "label_sweep"): 2.805e9 + 0.269e9,
+           ("c3_d64", "mixed", "sub_sweep"): 2.617e9 + 0.267e9}
 
 CONFIGS = {
     # N, D, true clusters, A scale, ridge, Dirichlet weight (None equal), alpha, dtype
@@ -443,7 +443,8 @@ def main():
         if pruned:
             phase["label_sweep"]["traffic"] = TRAFFIC.get((args.config, args.precision, "label_sweep"))
         phase["statistics"] = {
-            "bound": "mfma", "kernel": ("k_stats_gauss_mfma" if D > 8 else "k_stats_gauss") + " + k_stats_reduce",
+            "bound": "mfma", "kernel": ("k_stats_gauss" if D <= 8 else "k_stats_gauss_wave" if D <= 64 else
+                                        "k_stats_gauss_mfma") + " + k_stats_reduce",
             "achieved": N * (D * (D + 1.0) + D) / (avg_st * 1e-3) / 1e12, "peak": PEAK["fp64"], "unit": "TFLOP/s",
             "traffic": TRAFFIC.get((args.config, args.precision, "statistics")),
             "algorithmic": "D (D + 1) + D flops per point (upper triangle of x x^T and the sum, FP64)"}

@@ -437,7 +437,10 @@ __global__ void __launch_bounds__(kThreads, sizeof(TX) == 4 ? 3 : 2) k_stats_gau
                 TX xv[4][T];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {   // coordinates c0 .. c0 + T - 1 of 4 rows (zero past D)
-                    const TX* row = X + (int64_t)__shfl(cidx, min(g + 4 * u + kp, 63)) * D + c0;
+                    // a lane past the sub-cluster's last point reads a row of the batch that this pass
+                    // reads anyway when its group holds points (cached), else the batch's first row
+                    const int q = g + 4 * u + kp;
+                    const TX* row = X + (int64_t)__shfl(cidx, q < ns ? q : (g + 4 * u < ns ? ns - 1 : 0)) * D + c0;
                     if (vec && c0 + T <= D) {
                         if constexpr (T == 2 || T == 4) {
                             const txv w = *reinterpret_cast<const txv*>(row);

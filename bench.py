@@ -321,6 +321,8 @@ def main():
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp32", "fp64"])
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--graph", action="store_true",
+                    help="record each iteration's sweep segment into a hipGraph (no per-phase timings)")
     ap.add_argument("--seeds", type=int, default=0,
                     help="instead of timing: final K and NMI over this many seeds on each of --paths")
     ap.add_argument("--paths", default="mixed,fp32,fp64")
@@ -375,6 +377,7 @@ def main():
         print(f"[bench] copy probe: {X.numel() * X.element_size()} bytes", file=sys.stderr, flush=True)
         del Y
     iters = args.warmup + args.steps + 10
+    dev.set_graph_mode(args.graph)
     smp = dp.Sampler(dev, hyper, cfg["alpha"], iters, seed=args.seed + 1, precision=prec)
     t_w = time.perf_counter()
     for t in range(1, args.warmup + 1):
@@ -402,6 +405,15 @@ def main():
     elapsed = time.perf_counter() - t0
     ms_step = 1000 * elapsed / args.steps
     value = N * args.steps / elapsed
+    if args.graph:   # phase events are not recorded inside a graph: the whole step only
+        inst, upd = dev.graph_counts()
+        print(json.dumps({"metric": "point*iters/sec", "value": value, "unit": "point*iters/s", "n_gpus": 1,
+                          "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
+                          "higher_is_better": True, "config": {"workload": args.config, "precision": args.precision,
+                                                               "graph_mode": True},
+                          "graphs_instantiated": inst, "graphs_updated_in_place": upd,
+                          "final_clusters": smp.num_clusters()}), flush=True)
+        return
     avg_lab = float(np.mean(lab_ms))
     avg_sub, avg_st = float(np.mean(sub_ms)), float(np.mean(st_ms))
     pruned = bool(evfrac) and float(np.mean(evfrac)) >= 0.0

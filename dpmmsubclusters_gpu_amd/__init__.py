@@ -83,6 +83,9 @@ def load_library():
         "dpmm_set_multinomial_params": [_P, i, _P, _P, _P, _P, i],
         "dpmm_sample_cluster_labels": [_P, u32, i],
         "dpmm_sample_sub_labels": [_P, u32],
+        "dpmm_sweep_iteration": [_P, u32, i, _P],
+        "dpmm_set_graph_mode": [_P, i],
+        "dpmm_graph_counts": [_P, _P, _P],
         "dpmm_log_likelihoods": [_P, i, i, ctypes.c_int64, ctypes.c_int64, _P],
         "dpmm_sufficient_statistics": [_P, i, _P, _P, _P],
         "dpmm_relabel": [_P, u32, i, _P, _P],
@@ -376,6 +379,22 @@ class DeviceSweep:
 
     def synchronize(self):
         _check(load_library().dpmm_synchronize(self._ctx))
+
+    def sweep_iteration(self, K, iteration, final=False):
+        """Label sweep, sub-label sweep and statistics of the K clusters whose parameters are set,
+        in one call; returns the 2K sub-cluster counts."""
+        cnt = np.empty(2 * K, np.int64)
+        _check(load_library().dpmm_sweep_iteration(self._ctx, int(iteration), 1 if final else 0, _ptr(cnt)))
+        return cnt
+
+    def set_graph_mode(self, enable=True):
+        """Record the sweep segment of every iteration into a hipGraph and replay it."""
+        _check(load_library().dpmm_set_graph_mode(self._ctx, 1 if enable else 0))
+
+    def graph_counts(self):
+        a, b = ctypes.c_int64(), ctypes.c_int64()
+        _check(load_library().dpmm_graph_counts(self._ctx, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
 
 
 class Sampler:

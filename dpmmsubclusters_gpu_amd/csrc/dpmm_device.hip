@@ -710,6 +710,12 @@ struct dpmm_ctx {
     hipEvent_t evEvals = nullptr;
     bool evals_pending = false;
     int evals_K = 0;
+    bool evals_copy_due = false;   // a captured pruned sweep's counters still to be copied back
+    // graph mode: the sweep segment recorded into a hipGraph each iteration, replayed through one
+    // executable graph that is updated in place while the recorded topology stays the same
+    bool graph_mode = false;
+    hipGraphExec_t gexec = nullptr;
+    int64_t graph_instantiations = 0, graph_updates = 0;
     float* dPruneM = nullptr;     // K x K: row a, column k = lower bound of ||W_k (mu_a - mu_k)||
     size_t pruneMCap = 0;
     float* dPruneSC = nullptr;    // 2K: sigma_k, c_k
@@ -772,9 +778,14 @@ int pick_dp(int D) {
     return -1;
 }
 
+// Set while dpmm_sweep_iteration records a sweep into a hipGraph: nothing may allocate, synchronise
+// or record timing events meanwhile.
+static thread_local bool t_capturing = false;
+
 template <typename T>
 int grow(T*& p, size_t& cap, size_t n) {
     if (n <= cap && p) return DPMM_OK;
+    if (t_capturing) return fail(DPMM_ESTATE, "buffer growth while a sweep is being captured");
     if (p) (void)hipFree(p);
     p = nullptr;
     cap = 0;
@@ -789,6 +800,7 @@ int ensure_partition(dpmm_ctx* c, int K) {
     int rc = grow(c->dHist, c->histCap, (size_t)K * c->nblk);
     if (rc) return rc;
     if (c->Kpart < K || !c->dLabelStart) {
+        if (t_capturing) return fail(DPMM_ESTATE, "buffer growth while a sweep is being captured");
         if (c->dRowtot) (void)hipFree(c->dRowtot);
         if (c->dLabelStart) (void)hipFree(c->dLabelStart);
         if (c->dChunkStart) (void)hipFree(c->dChunkStart);
@@ -1339,6 +1351,7 @@ int dpmm_destroy(dpmm_ctx* c) {
     if (c->hStats) (void)hipHostFree(c->hStats);
     if (c->hEvals) (void)hipHostFree(c->hEvals);
     if (c->evEvals) (void)hipEventDestroy(c->evEvals);
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1462,10 +1475,32 @@ int dpmm_set_multinomial_params(dpmm_ctx* c, int K, const double* logtheta, cons
     return DPMM_OK;
 }
 
+// The adaptive pruning's decision input: the (wave, cluster) scoring counters of the last pruned
+// sweep, copied to pinned memory behind it.
+static int resolve_evals(dpmm_ctx* c) {
+    if (!c->evals_pending) return DPMM_OK;
+    HIPCHK(hipEventSynchronize(c->evEvals));
+    c->evals_pending = false;
+    const double f = c->hEvals[1] ? (double)c->hEvals[0] / ((double)c->hEvals[1] * c->evals_K) : 0.0;
+    if (f > 0.25) c->prune_skip = 4;
+    return DPMM_OK;
+}
+
+static int copy_evals_back(dpmm_ctx* c) {
+    if (!c->hEvals && hipHostMalloc((void**)&c->hEvals, 2 * sizeof(unsigned long long)) != hipSuccess)
+        return fail(DPMM_ENOMEM, "pinned");
+    if (!c->evEvals) HIPCHK(hipEventCreateWithFlags(&c->evEvals, hipEventDisableTiming));
+    HIPCHK(hipMemcpyAsync(c->hEvals, c->dEvals, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipEventRecord(c->evEvals, c->stream));
+    c->evals_pending = true;
+    c->evals_copy_due = false;
+    return DPMM_OK;
+}
+
 int dpmm_sample_cluster_labels(dpmm_ctx* c, uint32_t iter, int fin) {
     if (!c) return fail(DPMM_EINVAL, "null ctx");
     if (c->K <= 0) return fail(DPMM_ESTATE, "parameters not set");
-    (void)hipEventRecord(c->ev[0], c->stream);
+    if (!t_capturing) (void)hipEventRecord(c->ev[0], c->stream);
     c->last_pruned_K = 0;
     if (c->prior == DPMM_PRIOR_GAUSSIAN && c->precision == DPMM_PREC_MIXED) {
         bool prune = c->prune_mode != 0 && c->K >= 2 && c->D % 4 == 0;   // 16-byte rows for the gathers
@@ -1475,11 +1510,9 @@ int dpmm_sample_cluster_labels(dpmm_ctx* c, uint32_t iter, int fin) {
         if (prune && c->prune_mode == 1) {
             // adaptive: when the previous pruned sweep still scored more than a quarter of the
             // (wave, cluster) pairs, the next four sweeps run unpruned
-            if (c->evals_pending) {
-                HIPCHK(hipEventSynchronize(c->evEvals));
-                c->evals_pending = false;
-                const double f = c->hEvals[1] ? (double)c->hEvals[0] / ((double)c->hEvals[1] * c->evals_K) : 0.0;
-                if (f > 0.25) c->prune_skip = 4;
+            if (!t_capturing) {   // a capture resolves them before it starts
+                int rc = resolve_evals(c);
+                if (rc) return rc;
             }
             if (c->prune_skip > 0) {
                 --c->prune_skip;
@@ -1489,14 +1522,9 @@ int dpmm_sample_cluster_labels(dpmm_ctx* c, uint32_t iter, int fin) {
         int rc = prune ? run_label_mixed_pruned(c, iter, fin) : run_label_mixed(c, iter, fin);
         if (rc) return rc;
         if (prune && c->prune_mode == 1) {
-            if (!c->hEvals && hipHostMalloc((void**)&c->hEvals, 2 * sizeof(unsigned long long)) != hipSuccess)
-                return fail(DPMM_ENOMEM, "pinned");
-            if (!c->evEvals) HIPCHK(hipEventCreateWithFlags(&c->evEvals, hipEventDisableTiming));
-            HIPCHK(hipMemcpyAsync(c->hEvals, c->dEvals, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                                  c->stream));
-            HIPCHK(hipEventRecord(c->evEvals, c->stream));
-            c->evals_pending = true;
             c->evals_K = c->K;
+            if (t_capturing) c->evals_copy_due = true;   // copied once the graph has been launched
+            else if ((rc = copy_evals_back(c))) return rc;
         }
     } else if (c->prior == DPMM_PRIOR_GAUSSIAN) {
         if (c->params_f64) { DPMM_DISPATCH_DP64(c->DP, lg_f64, c, iter, fin); }
@@ -1506,9 +1534,9 @@ int dpmm_sample_cluster_labels(dpmm_ctx* c, uint32_t iter, int fin) {
                                                                      : launch_label_mult(c, iter, fin);
         if (rc) return rc;
     }
-    (void)hipEventRecord(c->ev[1], c->stream);
+    if (!t_capturing) (void)hipEventRecord(c->ev[1], c->stream);
     HIPCHK(hipGetLastError());
-    c->ev_label = true;
+    c->ev_label = !t_capturing;
     c->partition_valid = false;
     c->stats_cached = false;
     c->partials_valid = false;
@@ -1522,7 +1550,7 @@ int dpmm_sample_sub_labels(dpmm_ctx* c, uint32_t iter) {
         int rc = build_partition(c, c->K, 0);
         if (rc) return rc;
     }
-    (void)hipEventRecord(c->ev[2], c->stream);
+    if (!t_capturing) (void)hipEventRecord(c->ev[2], c->stream);
     c->stats_cached = false;
     c->partials_valid = false;
     if (c->prior == DPMM_PRIOR_GAUSSIAN && c->precision == DPMM_PREC_MIXED && c->spb_valid) {
@@ -1537,9 +1565,9 @@ int dpmm_sample_sub_labels(dpmm_ctx* c, uint32_t iter) {
                            c->dLabelStart, c->dChunkStart, reinterpret_cast<const float*>(c->dSP), c->stride,
                            c->key0(), c->key1(), iter, c->dSubs);
     }
-    (void)hipEventRecord(c->ev[3], c->stream);
+    if (!t_capturing) (void)hipEventRecord(c->ev[3], c->stream);
     HIPCHK(hipGetLastError());
-    c->ev_sub = true;
+    c->ev_sub = !t_capturing;
     return DPMM_OK;
 }
 
@@ -1588,9 +1616,8 @@ int dpmm_log_likelihoods(dpmm_ctx* c, int K, int which, int64_t first, int64_t n
     return DPMM_OK;
 }
 
-int dpmm_sufficient_statistics(dpmm_ctx* c, int K, int64_t* Nsub, double* sum, double* S) {
-    if (!c || !Nsub) return fail(DPMM_EINVAL, "bad args");
-    if (K <= 0 || K > kMaxK) return fail(DPMM_EINVAL, "bad K");
+// Statistics kernels for K clusters (partition first if needed), nothing read back.
+static int enqueue_stats(dpmm_ctx* c, int K) {
     if (!c->partition_valid || c->Kpartition != K) {
         int rc = build_partition(c, K, 0);
         if (rc) return rc;
@@ -1599,7 +1626,7 @@ int dpmm_sufficient_statistics(dpmm_ctx* c, int K, int64_t* Nsub, double* sum, d
     const int nch = chunks_upper(c, K);
     if (!(c->stats_cached && c->Kpartition == K)) {
         const bool partial = c->dirty_pending && c->partials_valid && c->partialsK == K && c->Kstats == K &&
-                             (int)c->dirty_host.size() == K;
+                             (int)c->dirty_host.size() == K && !t_capturing;
         int rc = grow(c->dPartials, c->partialsCap, (size_t)2 * nch * E);
         if (rc) return rc;
         rc = grow(c->dStats, c->statsCap, (size_t)2 * K * E);
@@ -1609,7 +1636,7 @@ int dpmm_sufficient_statistics(dpmm_ctx* c, int K, int64_t* Nsub, double* sum, d
             if ((rc = grow(c->dDirty, c->dirtyCap, (size_t)K))) return rc;
             HIPCHK(hipMemcpyAsync(c->dDirty, c->dirty_host.data(), (size_t)K * 4, hipMemcpyHostToDevice, c->stream));
             c->stats_mask = c->dDirty;
-        } else {
+        } else if (!t_capturing) {
             (void)hipEventRecord(c->ev[4], c->stream);   // the phase timing covers full passes only
         }
         if (c->prior == DPMM_PRIOR_GAUSSIAN) {
@@ -1623,7 +1650,7 @@ int dpmm_sufficient_statistics(dpmm_ctx* c, int K, int64_t* Nsub, double* sum, d
         }
         hipLaunchKernelGGL(k_stats_reduce, dim3(2 * K, (unsigned)((E + kThreads - 1) / kThreads)), dim3(kThreads), 0,
                            c->stream, c->dPartials, c->dChunkStart, K, (int)E, c->dStats, c->stats_mask);
-        if (!partial) {
+        if (!partial && !t_capturing) {
             (void)hipEventRecord(c->ev[5], c->stream);
             c->ev_stats = true;
         }
@@ -1635,6 +1662,15 @@ int dpmm_sufficient_statistics(dpmm_ctx* c, int K, int64_t* Nsub, double* sum, d
         c->dirty_pending = false;
     }
     HIPCHK(hipGetLastError());
+    return DPMM_OK;
+}
+
+int dpmm_sufficient_statistics(dpmm_ctx* c, int K, int64_t* Nsub, double* sum, double* S) {
+    if (!c || !Nsub) return fail(DPMM_EINVAL, "bad args");
+    if (K <= 0 || K > kMaxK) return fail(DPMM_EINVAL, "bad K");
+    int rc = enqueue_stats(c, K);
+    if (rc) return rc;
+    const size_t E = c->statEntries();
     const size_t n = sum ? (size_t)2 * K * E : (size_t)2 * K;
     if (c->hStatsCap < n) {
         if (c->hStats) (void)hipHostFree(c->hStats);
@@ -1661,6 +1697,90 @@ int dpmm_sufficient_statistics(dpmm_ctx* c, int K, int64_t* Nsub, double* sum, d
                 for (int b = 0; b < D; ++b) S[((size_t)q * D + a) * D + b] = src[1 + DP + (size_t)a * DP + b];
     }
     return DPMM_OK;
+}
+
+// Every buffer the sweep segment (label sweep, sub-label sweep, statistics) can grow for the
+// current K, grown before a capture so that nothing allocates while it records.
+static int ensure_sweep_buffers(dpmm_ctx* c) {
+    const int K = c->K;
+    int rc = ensure_partition(c, K);
+    if (rc) return rc;
+    c->CH = chunk_size(c);
+    const size_t E = c->statEntries();
+    if ((rc = grow(c->dPartials, c->partialsCap, (size_t)2 * chunks_upper(c, K) * E))) return rc;
+    if ((rc = grow(c->dStats, c->statsCap, (size_t)2 * K * E))) return rc;
+    if (c->prior == DPMM_PRIOR_GAUSSIAN && c->precision == DPMM_PREC_MIXED) {
+        if ((rc = grow(c->dPruneM, c->pruneMCap, (size_t)K * K))) return rc;
+        if ((rc = grow(c->dPruneSC, c->pruneSCCap, (size_t)2 * K))) return rc;
+        if ((rc = grow(c->dEvals, c->evalsCap, 2))) return rc;
+    }
+    return DPMM_OK;
+}
+
+int dpmm_set_graph_mode(dpmm_ctx* c, int enable) {
+    if (!c) return fail(DPMM_EINVAL, "null ctx");
+    c->graph_mode = enable != 0;
+    return DPMM_OK;
+}
+
+int dpmm_graph_counts(dpmm_ctx* c, int64_t* instantiations, int64_t* updates) {
+    if (!c) return fail(DPMM_EINVAL, "null ctx");
+    if (instantiations) *instantiations = c->graph_instantiations;
+    if (updates) *updates = c->graph_updates;
+    return DPMM_OK;
+}
+
+int dpmm_sweep_iteration(dpmm_ctx* c, uint32_t iter, int final_iter, int64_t* Nsub) {
+    if (!c || !Nsub) return fail(DPMM_EINVAL, "bad args");
+    if (c->K <= 0) return fail(DPMM_ESTATE, "parameters not set");
+    int rc;
+    if (!c->graph_mode) {
+        if ((rc = dpmm_sample_cluster_labels(c, iter, final_iter))) return rc;
+        if ((rc = dpmm_sample_sub_labels(c, iter))) return rc;
+        return dpmm_sufficient_statistics(c, c->K, Nsub, nullptr, nullptr);
+    }
+    if ((rc = resolve_evals(c))) return rc;   // the adaptive pruning decides from these, on the host
+    if ((rc = ensure_sweep_buffers(c))) return rc;
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+    t_capturing = true;
+    rc = dpmm_sample_cluster_labels(c, iter, final_iter);
+    if (!rc) rc = dpmm_sample_sub_labels(c, iter);
+    if (!rc) rc = enqueue_stats(c, c->K);
+    t_capturing = false;
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(c->stream, &g);
+    if (rc || ec != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        c->partition_valid = false;   // nothing of the recorded work ran
+        c->stats_cached = false;
+        c->partials_valid = false;
+        return rc ? rc : fail(DPMM_EHIP, std::string("graph capture: ") + hipGetErrorString(ec));
+    }
+    bool updated = false;
+    if (c->gexec) {
+        hipGraphNode_t bad = nullptr;
+        hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+        if (hipGraphExecUpdate(c->gexec, g, &bad, &res) == hipSuccess && res == hipGraphExecUpdateSuccess) {
+            updated = true;
+        } else {
+            (void)hipGetLastError();
+            (void)hipGraphExecDestroy(c->gexec);
+            c->gexec = nullptr;
+        }
+    }
+    hipError_t e = hipSuccess;
+    if (!updated) e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        c->gexec = nullptr;
+        return fail(DPMM_EHIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+    }
+    if (updated) ++c->graph_updates;
+    else ++c->graph_instantiations;
+    HIPCHK(hipGraphLaunch(c->gexec, c->stream));
+    if (c->evals_copy_due && (rc = copy_evals_back(c))) return rc;
+    return dpmm_sufficient_statistics(c, c->K, Nsub, nullptr, nullptr);   // the counts (statistics are current)
 }
 
 int dpmm_relabel(dpmm_ctx* c, uint32_t iter, int K_old, const int32_t* new_label, const int32_t* sub_action) {

@@ -123,6 +123,7 @@ struct SamplerImpl {
     std::vector<float> log_feat;
     std::vector<double> log_h, log_u;
     int64_t n_exact = 0, n_skipped = 0, n_missed = 0, n_full = 0;
+    bool counts_fresh = false;   // Nsub holds the counts of the current labels (from the sweep call)
 
     void reset_hist(Cluster& c) const { c.hist.assign(o.burnout + 5, std::make_pair(false, 0.0)); }
 
@@ -133,9 +134,12 @@ struct SamplerImpl {
     // when set, passes.
     int update_stats(const std::vector<char>& which, bool pairs, uint32_t t = 0) {
         const int K = (int)cl.size();
-        Nsub.resize(2 * K);
-        int rc = dpmm_sufficient_statistics(ctx, K, Nsub.data(), nullptr, nullptr);
-        if (rc) return rc;
+        int rc;
+        if (!counts_fresh) {
+            Nsub.resize(2 * K);
+            if ((rc = dpmm_sufficient_statistics(ctx, K, Nsub.data(), nullptr, nullptr))) return rc;
+        }
+        counts_fresh = false;
         const auto t0 = std::chrono::steady_clock::now();
         std::vector<int32_t> mem;
         std::vector<Part*> dst;
@@ -320,10 +324,11 @@ struct SamplerImpl {
             rc = sample_clusters(it);
             if (rc) return rc;
         }
-        rc = dpmm_sample_cluster_labels(ctx, it, fin ? 1 : 0);
+        // label sweep, sub-label sweep and statistics in one device call (one hipGraph in graph mode)
+        Nsub.resize(2 * cl.size());
+        rc = dpmm_sweep_iteration(ctx, it, fin ? 1 : 0, Nsub.data());
         if (rc) return rc;
-        rc = dpmm_sample_sub_labels(ctx, it);
-        if (rc) return rc;
+        counts_fresh = true;
         // one log-marginal launch: every part, and every pair of merge candidates (a cluster that is
         // reset or split below leaves the candidates, and the others keep these statistics)
         rc = update_stats({}, !no_more_splits, it);

@@ -96,6 +96,19 @@ int dpmm_log_likelihoods(dpmm_ctx* ctx, int K, int which, int64_t first, int64_t
  * uncentred scatter S[q*D*D + a*D + b] in FP64; S may be NULL.  Deterministic: no
  * atomics on bins; a fixed reduction order. */
 int dpmm_sufficient_statistics(dpmm_ctx* ctx, int K, int64_t* Nsub, double* sum, double* S);
+/* The device part of local_clusters_actions::group_step's first half (local_clusters_actions.cpp:
+ * 462-467: create_clusters_labels, create_subclusters_labels, then create_sufficient_statistics;
+ * cudaKernel.cu:1400-1490, 1299-1398, 844-941) in one call: label sweep, sub-label sweep and the
+ * statistics of the new labels, with their 2K counts returned in Nsub (the statistics stay on
+ * the device for dpmm_log_marginal_batch).  With graph mode on (dpmm_set_graph_mode), the call
+ * records the whole segment into a hipGraph, updates one executable graph in place with it while
+ * the recorded work keeps its shape (else instantiates it anew), and launches it; results are
+ * identical to the eager calls.  Graph mode has no counterpart in the reference; in it the
+ * per-phase timings of dpmm_last_kernel_ms are not recorded. */
+int dpmm_sweep_iteration(dpmm_ctx* ctx, uint32_t iter, int final_iter, int64_t* Nsub);
+int dpmm_set_graph_mode(dpmm_ctx* ctx, int enable);
+/* Executable graphs instantiated and updated in place so far in graph mode. */
+int dpmm_graph_counts(dpmm_ctx* ctx, int64_t* instantiations, int64_t* updates);
 /* One relabel pass for split / merge / reset / remove (cudaKernel.cu:172-263): a point
  * with label k and sub-label s gets label new_label[2k + s]; its sub-label becomes
  * s (sub_action[k] = 0), a Philox draw of tag 4 (reset, 1) or 5 (split, 2), or the

@@ -544,9 +544,20 @@ __global__ void __launch_bounds__(kThreads) k_prune_prep(const float* __restrict
     const int stride = gauss_slot_stride(DP);
     const float* slot = P + (size_t)k * stride;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    for (int e = threadIdx.x; e < D * D; e += kThreads) {
-        const int j = e / D, l = e - j * D;
-        W[j * DS + l] = l >= j ? slot[tri_row_offset(DP, j) - (j & ~3) + l] : 0.f;
+    // row j of W from the slot (its columns j & ~3 onwards are stored): thread t takes rows
+    // t / 64, t / 64 + 4, ..., lane l its columns l, l + 64; every load of a row is issued before
+    // the stores
+    for (int j = wv; j < D; j += 4) {
+        const float* src = slot + tri_row_offset(DP, j) - (j & ~3);
+        float w[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int l = lane + 64 * u;
+            w[u] = (l < D && l >= j) ? src[l] : 0.f;
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+            if (lane + 64 * u < D) W[j * DS + lane + 64 * u] = w[u];
     }
     __syncthreads();
     // fallback bound ||W||_2^2 <= ||W||_1 ||W||_inf, and ||W||_F^2
@@ -590,7 +601,8 @@ __global__ void __launch_bounds__(kThreads) k_prune_prep(const float* __restrict
             for (int l = row + part; l < D; l += TPR) uj = fma((double)W[row * DS + l], v[l], uj);
         for (int o = 1; o < TPR; o <<= 1) uj += __shfl_xor(uj, o);
         if (row < D && part == 0) uu[row] = uj;
-        s2 = block_sum(row < D && part == 0 ? uj * uj : 0.0, red);   // also publishes uu
+        if (itp == 13) s2 = block_sum(row < D && part == 0 ? uj * uj : 0.0, red);   // also publishes uu
+        else __syncthreads();                                                      // publishes uu
         double zi = 0.0;
         if (row < D)
             for (int j = part; j <= row; j += TPR) zi = fma((double)W[j * DS + row], uu[j], zi);
@@ -626,31 +638,48 @@ __global__ void __launch_bounds__(kThreads) k_prune_prep(const float* __restrict
     }
     const float* muk = slot + gauss_mu_offset(DP);
     double* dv = diff + (size_t)wv * D;
-    for (int a0 = 0; a0 < K; a0 += 4) {   // the same trip count in every wave: barriers stay uniform
-        const int a = a0 + wv;
-        if (a < K) {
-            const float* mua = P + (size_t)a * stride + gauss_mu_offset(DP);
-            for (int l = lane; l < D; l += 64) dv[l] = (double)mua[l] - (double)muk[l];
+    // anchors a = wave, wave + 4, ...: each wave works on its own difference vector with wave-level
+    // ordering only, and loads the next anchor's mean while it finishes the current one
+    auto sync_wave = []() {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
+    float mk[2], ma[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const int l = lane + 64 * u;
+        mk[u] = l < D ? muk[l] : 0.f;
+        ma[u] = (wv < K && l < D) ? P[(size_t)wv * stride + gauss_mu_offset(DP) + l] : 0.f;
+    }
+    for (int a = wv; a < K; a += 4) {
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+            if (lane + 64 * u < D) dv[lane + 64 * u] = (double)ma[u] - (double)mk[u];
+        if (a + 4 < K) {
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const int l = lane + 64 * u;
+                ma[u] = l < D ? P[(size_t)(a + 4) * stride + gauss_mu_offset(DP) + l] : 0.f;
+            }
         }
-        __syncthreads();
-        if (a < K) {
-            double sy = 0.0, se = 0.0;
-            for (int j = lane; j < D; j += 64) {
-                double y = 0.0;
-                for (int l = j; l < D; ++l) y = fma((double)W[j * DS + l], dv[l], y);
-                sy = fma(y, y, sy);
-                se = fma(dv[j], dv[j], se);
-            }
-            for (int o = 32; o > 0; o >>= 1) {
-                sy += __shfl_xor(sy, o);
-                se += __shfl_xor(se, o);
-            }
-            if (lane == 0) {
-                const double m = sqrt(sy) * (1.0 - 1e-12) - gam * F * sqrt(se) * (1.0 + 1e-12);
-                pm[(size_t)a * K + k] = a == k ? 0.f : __double2float_rd(fmax(m, 0.0));
-            }
+        sync_wave();
+        double sy = 0.0, se = 0.0;
+        for (int j = lane; j < D; j += 64) {
+            double y = 0.0;
+            for (int l = j; l < D; ++l) y = fma((double)W[j * DS + l], dv[l], y);
+            sy = fma(y, y, sy);
+            se = fma(dv[j], dv[j], se);
         }
-        __syncthreads();
+        for (int o = 32; o > 0; o >>= 1) {
+            sy += __shfl_xor(sy, o);
+            se += __shfl_xor(se, o);
+        }
+        if (lane == 0) {
+            const double m = sqrt(sy) * (1.0 - 1e-12) - gam * F * sqrt(se) * (1.0 + 1e-12);
+            pm[(size_t)a * K + k] = a == k ? 0.f : __double2float_rd(fmax(m, 0.0));
+        }
+        sync_wave();   // dv is read before the next anchor's difference overwrites it
     }
 }
 

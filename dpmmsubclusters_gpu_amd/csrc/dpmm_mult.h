@@ -74,29 +74,29 @@ __global__ void __launch_bounds__(kThreads) k_pack_mult_mx(const double* __restr
 
 // 32 documents per wave, 4 waves per workgroup sharing the LDS-staged tiles of G chunks at a time.
 template <int NCB>
-__global__ void __launch_bounds__(kThreads) k_label_mult_mx(const int32_t* __restrict__ X, int64_t N, int D,
+__global__ void __launch_bounds__(kThreads, NCB >= 4 ? 3 : 2) k_label_mult_mx(const int32_t* __restrict__ X, int64_t N, int D,
                                                             int vec, const unsigned char* __restrict__ blk,
                                                             uint32_t key0,
                                                             uint32_t key1, uint32_t iter, int final_iter,
                                                             int32_t* __restrict__ labels,
                                                             int32_t* __restrict__ recheck,
                                                             int* __restrict__ recheck_count) {
-    constexpr int G = NCB > 4 ? 4 : 8;   // chunks per staged group
-    __shared__ bf16x8 sA[2][G][NCB][64];
+    // groups of G chunks (16 G words): the tiles of group g + 1 stream into the other LDS buffer by
+    // LDS DMA and its counts into registers while group g runs on MFMA
+    constexpr int G = NCB >= 4 ? 2 : 4;
+    __shared__ __align__(16) bf16x8 sA[2][2][G][NCB][64];   // [buffer][hi / lo][chunk][cluster block][lane]
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, half = lane >> 5;
     const int64_t doc = ((int64_t)blockIdx.x * 4 + wv) * 32 + (lane & 31);
     const bool valid = doc < N;
     const int NC = mm_chunks(D);
-    const bf16x8* hiT = reinterpret_cast<const bf16x8*>(blk);
-    const bf16x8* loT = reinterpret_cast<const bf16x8*>(blk + mm_tiles_bytes(D, NCB));
+    const unsigned char* hiB = blk;
+    const unsigned char* loB = blk + mm_tiles_bytes(D, NCB);
     f32x16 acc[NCB];
 #pragma unroll
     for (int cb = 0; cb < NCB; ++cb) acc[cb] = f32x16{};
     int words = 0;
     const int32_t* xr = X + (valid ? doc : 0) * (int64_t)D;
-    for (int g0 = 0; g0 < NC; g0 += G) {
-        // this group's counts first, so their latency overlaps the tile staging below
-        int xg[G][8];
+    auto load_counts = [&](int g0, int (&xg)[G][8]) {
 #pragma unroll
         for (int gc = 0; gc < G; ++gc) {
             const int d0 = 16 * (g0 + gc) + 8 * half;
@@ -110,16 +110,26 @@ __global__ void __launch_bounds__(kThreads) k_label_mult_mx(const int32_t* __res
                 for (int e = 0; e < 8; ++e) xg[gc][e] = (valid && d0 + e < D) ? xr[d0 + e] : 0;
             }
         }
-        __syncthreads();
-        for (int e = threadIdx.x; e < 2 * G * NCB * 64; e += kThreads) {
-            const int ln = e & 63, rest = e >> 6;
-            const int cb = rest % NCB, gc = (rest / NCB) % G, hl = rest / (NCB * G);
-            const int c = g0 + gc;
-            bf16x8 v = {};
-            if (c < NC) v = (hl ? loT : hiT)[((size_t)c * NCB + cb) * 64 + ln];
-            sA[hl][gc][cb][ln] = v;
+    };
+    // tiles of chunks g0 .. g0 + G - 1 (those that exist): contiguous in the parameter block
+    auto stage = [&](int g0, int buf) {
+        const int bytes = min(G, NC - g0) * NCB * 1024;
+        lds_dma_copy(reinterpret_cast<unsigned char*>(&sA[buf][0][0][0][0]), hiB + (size_t)g0 * NCB * 1024, bytes, wv,
+                     lane);
+        lds_dma_copy(reinterpret_cast<unsigned char*>(&sA[buf][1][0][0][0]), loB + (size_t)g0 * NCB * 1024, bytes, wv,
+                     lane);
+    };
+    int xg[G][8], xn[G][8];
+    load_counts(0, xg);
+    stage(0, 0);
+    lds_dma_wait();
+    __syncthreads();
+    for (int g0 = 0, buf = 0; g0 < NC; g0 += G, buf ^= 1) {
+        const bool more = g0 + G < NC;
+        if (more) {   // the previous group's readers of sA[buf ^ 1] passed the barrier below
+            load_counts(g0 + G, xn);
+            stage(g0 + G, buf ^ 1);
         }
-        __syncthreads();
 #pragma unroll
         for (int gc = 0; gc < G; ++gc) {
             if (g0 + gc >= NC) break;
@@ -132,10 +142,16 @@ __global__ void __launch_bounds__(kThreads) k_label_mult_mx(const int32_t* __res
             }
 #pragma unroll
             for (int cb = 0; cb < NCB; ++cb) {
-                acc[cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(sA[0][gc][cb][lane], xb, acc[cb], 0, 0, 0);
-                acc[cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(sA[1][gc][cb][lane], xb, acc[cb], 0, 0, 0);
+                acc[cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(sA[buf][0][gc][cb][lane], xb, acc[cb], 0, 0, 0);
+                acc[cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(sA[buf][1][gc][cb][lane], xb, acc[cb], 0, 0, 0);
             }
         }
+        lds_dma_wait();   // the next group's tiles (this wave's share) and counts have landed
+        __syncthreads();  // every wave's share is in, and every wave is done with sA[buf]
+#pragma unroll
+        for (int gc = 0; gc < G; ++gc)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) xg[gc][e] = xn[gc][e];
     }
     words += __shfl_xor(words, 32);
     const float lmax = *reinterpret_cast<const float*>(blk + mm_lmax_offset(D, NCB));

@@ -85,6 +85,10 @@ def load_library():
         "dpmm_sample_sub_labels": [_P, u32],
         "dpmm_sweep_iteration": [_P, u32, i, _P],
         "dpmm_set_graph_mode": [_P, i],
+        "dpmm_set_ground_truth": [_P, _P, i],
+        "dpmm_label_contingency": [_P, i, _P],
+        "dpmm_label_nmi": [_P, i, _P],
+        "dpmm_sampler_log_posterior": [_P, _P],
         "dpmm_graph_counts": [_P, _P, _P],
         "dpmm_log_likelihoods": [_P, i, i, ctypes.c_int64, ctypes.c_int64, _P],
         "dpmm_sufficient_statistics": [_P, i, _P, _P, _P],
@@ -182,7 +186,7 @@ class ModelInfo:
 
 
 def _nmi(a, b):
-    """2 I(X;Y) / (H(X) + H(Y)) (dp_parallel_sampling.cpp:207-222)."""
+    """2 I(X;Y) / (H(X) + H(Y)) (dp_parallel_sampling.cpp:164-180)."""
     a = np.asarray(a, np.int64)
     b = np.asarray(b, np.int64)
     n = a.size
@@ -391,6 +395,24 @@ class DeviceSweep:
         """Record the sweep segment of every iteration into a hipGraph and replay it."""
         _check(load_library().dpmm_set_graph_mode(self._ctx, 1 if enable else 0))
 
+    def set_ground_truth(self, classes):
+        """Ground-truth class of every point (values 0 .. C-1), for the device NMI."""
+        c = np.ascontiguousarray(classes, dtype=np.int32)
+        if c.shape != (self.N,):
+            raise ValueError("one class per point")
+        _check(load_library().dpmm_set_ground_truth(self._ctx, _ptr(c), int(c.max()) + 1))
+
+    def label_contingency(self, K, num_classes):
+        t = np.empty((K, num_classes), np.int64)
+        _check(load_library().dpmm_label_contingency(self._ctx, int(K), _ptr(t)))
+        return t
+
+    def label_nmi(self, K):
+        """NMI of the current labels (0 .. K-1) against the ground truth, on the device."""
+        v = ctypes.c_double()
+        _check(load_library().dpmm_label_nmi(self._ctx, int(K), ctypes.byref(v)))
+        return v.value
+
     def graph_counts(self):
         a, b = ctypes.c_int64(), ctypes.c_int64()
         _check(load_library().dpmm_graph_counts(self._ctx, ctypes.byref(a), ctypes.byref(b)))
@@ -443,6 +465,12 @@ class Sampler:
     def host_ms(self):
         v = ctypes.c_double()
         _check(load_library().dpmm_sampler_host_ms(self._s, ctypes.byref(v)))
+        return v.value
+
+    def log_posterior(self):
+        """calculate_posterior (dp_parallel_sampling.cpp:218-236) after the last step."""
+        v = ctypes.c_double()
+        _check(load_library().dpmm_sampler_log_posterior(self._s, ctypes.byref(v)))
         return v.value
 
     def set_screener(self, screener=None, full_scan_every=10, threshold=0.0, log_pairs=False):
@@ -510,18 +538,23 @@ class dp_parallel_sampling_class:
         s = Sampler(self.dev, local_hyper_params, alpha_param, iters, init_clusters, burnout, self.seed, max_clusters,
                     precision)
         mi = ModelInfo()
+        if verbose and gt is not None:   # classes 0 .. C-1 in order of first appearance
+            _, cls = np.unique(np.asarray(gt), return_inverse=True)
+            self.dev.set_ground_truth(cls.astype(np.int32))
         for it in range(1, iters + 1):
             t0 = time.perf_counter()
             s.step(it)
             mi.iter_count.append(time.perf_counter() - t0)
             mi.cluster_count_history.append(s.num_clusters())
-            mi.likelihood_history.append(1.0)
+            # the reference records the log posterior only in verbose mode, else 1 (dp_parallel_sampling.cpp:192)
+            mi.likelihood_history.append(s.log_posterior() if verbose else 1.0)
             if verbose and gt is not None:
-                lab, _ = self.dev.get_labels()
-                mi.nmi_score_history.append(_nmi(np.asarray(gt), lab))
+                mi.nmi_score_history.append(self.dev.label_nmi(s.num_clusters()))
             if verbose:
+                nmi = mi.nmi_score_history[-1] if mi.nmi_score_history else -1
                 print(f"Iteration: {it} || Clusters count: {mi.cluster_count_history[-1]} || "
-                      f"Iter Time: {mi.iter_count[-1]:.4f}", flush=True)
+                      f"Log posterior: {mi.likelihood_history[-1]:f} || NMI score: {nmi:f} || "
+                      f"Iter Time: {mi.iter_count[-1]:.4f} || Total time: {sum(mi.iter_count):.4f}", flush=True)
         lab, _ = self.dev.get_labels()
         mi.labels = lab + 1
         mi.weights, mi.cluster_counts = s.weights()

@@ -683,6 +683,35 @@ __global__ void __launch_bounds__(kThreads) k_prune_prep(const float* __restrict
     }
 }
 
+// ---- verbose diagnostics: label x class contingency table ------------------------------------
+// Counts of (label, ground-truth class) pairs for the NMI that dp_parallel reports per iteration in
+// verbose mode (dp_parallel_sampling.cpp:164-180).  Each workgroup histograms a grid-strided share of
+// the points into an LDS table (LDS atomics only) and writes it as one partial row; k_table_reduce
+// sums the rows in workgroup order, so the table is exact and deterministic.
+__global__ void __launch_bounds__(kThreads) k_contingency(const int32_t* __restrict__ labels,
+                                                          const int32_t* __restrict__ classes, int64_t N, int K,
+                                                          int G, int32_t* __restrict__ partials) {
+    extern __shared__ int tab[];   // K x G
+    const int KG = K * G;
+    for (int e = threadIdx.x; e < KG; e += kThreads) tab[e] = 0;
+    __syncthreads();
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < N; i += (int64_t)gridDim.x * kThreads) {
+        const int L = labels[i], g = classes[i];
+        if ((unsigned)L < (unsigned)K && (unsigned)g < (unsigned)G) atomicAdd(&tab[L * G + g], 1);
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < KG; e += kThreads) partials[(size_t)blockIdx.x * KG + e] = tab[e];
+}
+
+__global__ void __launch_bounds__(kThreads) k_table_reduce(const int32_t* __restrict__ partials, int rows, int KG,
+                                                           int64_t* __restrict__ table) {
+    const int e = blockIdx.x * kThreads + threadIdx.x;
+    if (e >= KG) return;
+    int64_t s = 0;
+    for (int b = 0; b < rows; ++b) s += partials[(size_t)b * KG + e];
+    table[e] = s;
+}
+
 // ============================================================================ context ======
 
 struct dpmm_ctx {
@@ -743,6 +772,8 @@ struct dpmm_ctx {
     // graph mode: the sweep segment recorded into a hipGraph each iteration, replayed through one
     // executable graph that is updated in place while the recorded topology stays the same
     bool graph_mode = false;
+    int32_t* dClasses = nullptr;   // ground-truth classes of the points (verbose NMI)
+    int nClasses = 0;
     hipGraphExec_t gexec = nullptr;
     int64_t graph_instantiations = 0, graph_updates = 0;
     float* dPruneM = nullptr;     // K x K: row a, column k = lower bound of ||W_k (mu_a - mu_k)||
@@ -1375,7 +1406,7 @@ int dpmm_destroy(dpmm_ctx* c) {
                     (void*)c->dPartials, (void*)c->dStats, c->dPB, (void*)c->dRecheck, (void*)c->dRecheckCount,
                     (void*)c->dPrior, (void*)c->dMembers, (void*)c->dLM, (void*)c->dDraws, (void*)c->dDrawStatus,
                     (void*)c->dLogW, (void*)c->dFeat, (void*)c->dMB, c->dSPB, (void*)c->dPruneM,
-                    (void*)c->dPruneSC, (void*)c->dEvals, (void*)c->dDirty})
+                    (void*)c->dPruneSC, (void*)c->dEvals, (void*)c->dDirty, (void*)c->dClasses})
         if (p) (void)hipFree(p);
     if (c->hStats) (void)hipHostFree(c->hStats);
     if (c->hEvals) (void)hipHostFree(c->hEvals);
@@ -1810,6 +1841,76 @@ int dpmm_sweep_iteration(dpmm_ctx* c, uint32_t iter, int final_iter, int64_t* Ns
     HIPCHK(hipGraphLaunch(c->gexec, c->stream));
     if (c->evals_copy_due && (rc = copy_evals_back(c))) return rc;
     return dpmm_sufficient_statistics(c, c->K, Nsub, nullptr, nullptr);   // the counts (statistics are current)
+}
+
+int dpmm_set_ground_truth(dpmm_ctx* c, const int32_t* classes, int num_classes) {
+    if (!c || !classes || num_classes <= 0) return fail(DPMM_EINVAL, "bad args");
+    for (int64_t i = 0; i < c->N; ++i)
+        if (classes[i] < 0 || classes[i] >= num_classes) return fail(DPMM_EINVAL, "a class outside [0, num_classes)");
+    if (!c->dClasses && hipMalloc((void**)&c->dClasses, c->N * sizeof(int32_t)) != hipSuccess)
+        return fail(DPMM_ENOMEM, "hipMalloc failed");
+    HIPCHK(hipMemcpyAsync(c->dClasses, classes, c->N * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->nClasses = num_classes;
+    return DPMM_OK;
+}
+
+int dpmm_label_contingency(dpmm_ctx* c, int K, int64_t* table) {
+    if (!c || !table || K <= 0) return fail(DPMM_EINVAL, "bad args");
+    if (!c->dClasses) return fail(DPMM_ESTATE, "no ground truth set");
+    const int G = c->nClasses, KG = K * G;
+    if ((int64_t)K * G > 16384) return fail(DPMM_EINVAL, "clusters x classes above 16384");
+    const int rows = (int)std::min<int64_t>(256, (c->N + kThreads - 1) / kThreads);
+    int32_t* dPart = nullptr;
+    int64_t* dTab = nullptr;
+    if (hipMalloc((void**)&dPart, (size_t)rows * KG * sizeof(int32_t)) != hipSuccess ||
+        hipMalloc((void**)&dTab, (size_t)KG * sizeof(int64_t)) != hipSuccess) {
+        if (dPart) (void)hipFree(dPart);
+        return fail(DPMM_ENOMEM, "hipMalloc failed");
+    }
+    hipLaunchKernelGGL(k_contingency, dim3(rows), dim3(kThreads), (size_t)KG * sizeof(int), c->stream, c->dLabels,
+                       c->dClasses, c->N, K, G, dPart);
+    hipLaunchKernelGGL(k_table_reduce, dim3((KG + kThreads - 1) / kThreads), dim3(kThreads), 0, c->stream, dPart, rows,
+                       KG, dTab);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(table, dTab, (size_t)KG * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(dPart);
+    (void)hipFree(dTab);
+    if (e != hipSuccess) return fail(DPMM_EHIP, std::string("contingency: ") + hipGetErrorString(e));
+    return DPMM_OK;
+}
+
+int dpmm_label_nmi(dpmm_ctx* c, int K, double* nmi) {
+    if (!c || !nmi) return fail(DPMM_EINVAL, "bad args");
+    if (!c->dClasses) return fail(DPMM_ESTATE, "no ground truth set");
+    const int G = c->nClasses;
+    std::vector<int64_t> t((size_t)K * G);
+    int rc = dpmm_label_contingency(c, K, t.data());
+    if (rc) return rc;
+    // 2 I(X;Y) / (H(X) + H(Y)), natural logarithms, over the points whose label lies in [0, K)
+    std::vector<double> pr(K, 0.0), pc(G, 0.0);
+    double n = 0.0;
+    for (int k = 0; k < K; ++k)
+        for (int g = 0; g < G; ++g) {
+            const double v = (double)t[(size_t)k * G + g];
+            pr[k] += v;
+            pc[g] += v;
+            n += v;
+        }
+    if (n <= 0.0) return fail(DPMM_ESTATE, "no labelled points");
+    double I = 0.0, hr = 0.0, hc = 0.0;
+    for (int k = 0; k < K; ++k)
+        for (int g = 0; g < G; ++g) {
+            const double v = (double)t[(size_t)k * G + g];
+            if (v > 0.0) I += v / n * std::log(v * n / (pr[k] * pc[g]));
+        }
+    for (int k = 0; k < K; ++k)
+        if (pr[k] > 0.0) hr -= pr[k] / n * std::log(pr[k] / n);
+    for (int g = 0; g < G; ++g)
+        if (pc[g] > 0.0) hc -= pc[g] / n * std::log(pc[g] / n);
+    *nmi = hr + hc <= 0.0 ? 1.0 : 2.0 * I / (hr + hc);
+    return DPMM_OK;
 }
 
 int dpmm_relabel(dpmm_ctx* c, uint32_t iter, int K_old, const int32_t* new_label, const int32_t* sub_action) {

@@ -74,7 +74,7 @@ __global__ void __launch_bounds__(kThreads) k_pack_mult_mx(const double* __restr
 
 // 32 documents per wave, 4 waves per workgroup sharing the LDS-staged tiles of G chunks at a time.
 template <int NCB>
-__global__ void __launch_bounds__(kThreads, NCB >= 4 ? 3 : 2) k_label_mult_mx(const int32_t* __restrict__ X, int64_t N, int D,
+__global__ void __launch_bounds__(kThreads, NCB >= 4 && NCB <= 6 ? 3 : 2) k_label_mult_mx(const int32_t* __restrict__ X, int64_t N, int D,
                                                             int vec, const unsigned char* __restrict__ blk,
                                                             uint32_t key0,
                                                             uint32_t key1, uint32_t iter, int final_iter,

@@ -534,6 +534,23 @@ int dpmm_sampler_weights(dpmm_sampler* S, double* w, int64_t* counts, int max_k)
     return DPMM_OK;
 }
 
+// dp_parallel_sampling_class::calculate_posterior (dp_parallel_sampling.cpp:218-236):
+// lgamma(alpha) - lgamma(N + alpha) + sum over non-empty clusters of
+// log ML_k + log alpha + lgamma(N_k), from the cluster scalars the last step left (the log ML of a
+// cluster split or merged in the step is its part's, as the reference keeps it).
+int dpmm_sampler_log_posterior(dpmm_sampler* S, double* lp) {
+    if (!S || !lp) return DPMM_EINVAL;
+    const dpmm::SamplerImpl& s = S->s;
+    const double a = s.o.alpha;
+    double v = std::lgamma(a) - std::lgamma((double)s.N + a);
+    for (const auto& c : s.cl) {
+        if (c.c.N == 0) continue;
+        v += c.c.lml + std::log(a) + std::lgamma((double)c.c.N);
+    }
+    *lp = v;
+    return DPMM_OK;
+}
+
 int dpmm_sampler_host_ms(dpmm_sampler* S, double* ms) {
     if (!S || !ms) return DPMM_EINVAL;
     *ms = S->s.host_ms;

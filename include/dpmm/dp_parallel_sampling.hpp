@@ -64,8 +64,8 @@ struct multinomial_hyper : hyperparams {
 
 struct ModelInfo {
     std::vector<double> iter_count;           // seconds per iteration
-    std::vector<double> nmi_score_history;    // per iteration when gt is given
-    std::vector<double> likelihood_history;   // not computed (empty)
+    std::vector<double> nmi_score_history;    // per iteration when gt is given (verbose or not)
+    std::vector<double> likelihood_history;   // log posterior per iteration in verbose mode, else 1
     std::vector<int> cluster_count_history;
     std::shared_ptr<LabelsType> labels;       // 1-based
     std::vector<double> weights;              // the model's mixture weights (dp_model->group.weights)
@@ -182,22 +182,36 @@ class dp_parallel_sampling_class {
         std::unique_ptr<dpmm_sampler, int (*)(dpmm_sampler*)> s_guard(s, dpmm_sampler_destroy);
         ModelInfo info;
         LabelsType lab(N_);
+        if (gt) {   // classes 0 .. C-1 in order of first appearance, for the device contingency table
+            if ((int64_t)gt->size() != N_) throw std::invalid_argument("dpmm: gt needs one label per point");
+            std::unordered_map<int, int> cls;
+            std::vector<int32_t> c(N_);
+            for (int64_t i = 0; i < N_; ++i) {
+                auto it = cls.emplace((*gt)[i], (int)cls.size()).first;
+                c[i] = it->second;
+            }
+            check(dpmm_set_ground_truth(ctx, c.data(), (int)cls.size()));
+        }
+        double total = 0.0;
         for (int t = 1; t <= iters; ++t) {
             const auto t0 = std::chrono::steady_clock::now();
             check(dpmm_sampler_step(s, t));
             const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
             const int K = dpmm_sampler_num_clusters(s);
+            total += dt;
             info.iter_count.push_back(dt);
             info.cluster_count_history.push_back(K);
-            double score = 0.0;
-            if (gt) {
-                check(dpmm_get_labels(ctx, lab.data(), nullptr));
-                for (auto& l : lab) l += 1;
-                score = nmi(*gt, lab);
+            double lp = 1.0;   // the reference records 1 when not verbose (dp_parallel_sampling.cpp:192)
+            if (verbose) check(dpmm_sampler_log_posterior(s, &lp));
+            info.likelihood_history.push_back(lp);
+            double score = -1.0;
+            if (gt) {   // the label / class table is built on the device; only K x C counts come back
+                check(dpmm_label_nmi(ctx, K, &score));
                 info.nmi_score_history.push_back(score);
             }
             if (verbose)
-                std::printf("Iteration: %d || Clusters count: %d || NMI: %f || Iter Time: %f\n", t, K, score, dt);
+                std::printf("Iteration: %d || Clusters count: %d || Log posterior: %f || NMI score: %f || "
+                            "Iter Time: %f || Total time: %f\n", t, K, lp, score, dt, total);
         }
         check(dpmm_get_labels(ctx, lab.data(), nullptr));
         for (auto& l : lab) l += 1;   // the reference's labels are 1-based

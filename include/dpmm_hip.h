@@ -174,6 +174,16 @@ int dpmm_set_label_pruning(dpmm_ctx* ctx, int mode);
  * sweep was not pruned). */
 int dpmm_last_label_evaluations(dpmm_ctx* ctx, double* fraction);
 
+/* Verbose diagnostics of dp_parallel (dp_parallel_sampling.cpp:164-180: the NMI of the labels against
+ * the ground truth, which the reference computes on the host with MIToolbox after copying all labels
+ * back): the ground-truth class of every point is set once; dpmm_label_contingency returns the K x C
+ * table of (label, class) counts (row-major, labels outside [0, K) left out; K x C <= 16384), built on
+ * the device with LDS histograms and a fixed-order reduction; dpmm_label_nmi reduces it to
+ * 2 I(X;Y) / (H(X) + H(Y)) (natural logarithms; 1 when both entropies are 0). */
+int dpmm_set_ground_truth(dpmm_ctx* ctx, const int32_t* classes, int num_classes);
+int dpmm_label_contingency(dpmm_ctx* ctx, int K, int64_t* table);
+int dpmm_label_nmi(dpmm_ctx* ctx, int K, double* nmi);
+
 /* Kernel timings of the last calls (ms, HIP events on the library's stream). */
 int dpmm_last_kernel_ms(dpmm_ctx* ctx, float* label_sweep_ms, float* sub_sweep_ms, float* stats_ms);
 int dpmm_synchronize(dpmm_ctx* ctx);
@@ -246,6 +256,10 @@ int dpmm_sampler_num_clusters(dpmm_sampler* s);
 int dpmm_sampler_weights(dpmm_sampler* s, double* weights, int64_t* counts, int max_k);
 /* Host time spent in the last step's FP64 work (draws, posteriors, Hastings), ms. */
 int dpmm_sampler_host_ms(dpmm_sampler* s, double* host_ms);
+/* dp_parallel_sampling_class::calculate_posterior (dp_parallel_sampling.cpp:218-236) after the last
+ * step: lgamma(alpha) - lgamma(N + alpha) + sum over clusters with points of
+ * log ML_k + log alpha + lgamma(N_k), the entry of ModelInfo::likelihood_history in verbose mode. */
+int dpmm_sampler_log_posterior(dpmm_sampler* s, double* log_posterior);
 int dpmm_sampler_destroy(dpmm_sampler* s);
 
 /* Merge screening for a Gaussian sampler (s not owned): from then on, iterations t with

@@ -253,10 +253,44 @@ __device__ void gram_t_packed(const double* X, double* G, int D, double div) {
     __syncthreads();
 }
 
+// In place, packed lower X becomes J X^T J (J the reversal permutation): element (i, j) takes
+// X[D-1-j][D-1-i].  For a symmetric X that is J X J.  The map is an involution on the packed
+// triangle, so every pair is swapped by the thread owning its smaller index.
+__device__ void reverse_packed(double* X, int D) {
+    const int T = D * (D + 1) / 2;
+    for (int e = threadIdx.x; e < T; e += blockDim.x) {
+        int i, j;
+        tri_ij(e, i, j);
+        const int e2 = tri(D - 1 - j, D - 1 - i);
+        if (e < e2) {
+            const double a = X[e];
+            X[e] = X[e2];
+            X[e2] = a;
+        }
+    }
+    __syncthreads();
+}
+
+// G = J (R R^T) J for packed lower R: entry (i, j) is sum_{k <= a} R[a][k] R[b][k] with
+// a = D-1-i <= b = D-1-j (k in increasing order).
+__device__ void gram_rev_packed(const double* R, double* G, int D) {
+    const int T = D * (D + 1) / 2;
+    for (int e = threadIdx.x; e < T; e += blockDim.x) {
+        int i, j;
+        tri_ij(e, i, j);
+        const int a = D - 1 - i, b = D - 1 - j;
+        const int ab = tri(a, 0), bb = tri(b, 0);
+        double s = 0.0;
+        for (int k = 0; k <= a; ++k) s += R[ab + k] * R[bb + k];
+        G[e] = s;
+    }
+    __syncthreads();
+}
+
 // ---- kernels -------------------------------------------------------------------------------------
 
 // out[2 j] = log marginal likelihood, out[2 j + 1] = log det psi_n (NaN when psi_n is not SPD).
-__global__ void __launch_bounds__(kThreads) k_log_marginal(const double* __restrict__ stats, int E, int DP, int D,
+__global__ void __launch_bounds__(kThreads, 4) k_log_marginal(const double* __restrict__ stats, int E, int DP, int D,
                                                            int prior, const int32_t* __restrict__ members,
                                                            PriorArgs h, double* __restrict__ out) {
     extern __shared__ double lds[];
@@ -366,26 +400,26 @@ __global__ void __launch_bounds__(kThreads) k_draw_params(const double* __restri
     double* m = lds + 2 * T;
     double* z = m + D;
     double* gd = z + D;
-    // C = chol(nu psi_n); B = chol((nu psi_n)^-1) = chol(C^-T C^-1)
+    // B = chol((nu psi_n)^-1).  With Cr = chol(J nu psi_n J) (J reverses the index order),
+    // nu psi_n = U U^T for the upper U = J Cr J, so (nu psi_n)^-1 = U^-T U^-1 and, U^-T being lower
+    // with a positive diagonal, B = U^-T = J Cr^-T J: one Cholesky and one triangular inverse.
     niw_posterior(stats, (size_t)E, DP, D, mem, h, 1.0, P, m, scal);
     const double kn = scal[1], vn = scal[2];
+    reverse_packed(P, D);
     bool ok = chol_packed(P, D, sh);
-    if (ok) {
-        lower_inverse_packed(P, Q, D);
-        gram_t_packed(Q, P, D, 1.0);
-        ok = chol_packed(P, D, sh);
-    }
     if (!ok) {
         if (threadIdx.x == 0) status[p] = 1;
         return;
     }
-    // Bartlett factor A (into Q): normals below the diagonal in row order (words 2m, 2m + 1 of
+    lower_inverse_packed(P, Q, D);   // Cr^-1
+    reverse_packed(Q, D);            // B
+    // Bartlett factor A (into P): normals below the diagonal in row order (words 2m, 2m + 1 of
     // normal m), then the D normals of mu on the same stream; the D diagonal chi variates
     // sqrt(2 Gamma((nu - i) / 2)) from the streams starting at block 1024 (16 + i), all in parallel.
     for (int e = threadIdx.x; e < T; e += blockDim.x) {
         int i, j;
         tri_ij(e, i, j);
-        if (j < i) Q[e] = stream_normal_at(c0, iter, key0, key1, 2u * (uint32_t)(i * (i - 1) / 2 + j));
+        if (j < i) P[e] = stream_normal_at(c0, iter, key0, key1, 2u * (uint32_t)(i * (i - 1) / 2 + j));
     }
     for (int i = threadIdx.x; i < D; i += blockDim.x) {
         DevStream rs(c0, iter, TAG_PARAM, key0, key1, 4096u * (16u + (uint32_t)i));
@@ -393,30 +427,40 @@ __global__ void __launch_bounds__(kThreads) k_draw_params(const double* __restri
         z[i] = stream_normal_at(c0, iter, key0, key1, (uint32_t)(D * (D - 1)) + 2u * (uint32_t)i);
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < D; i += blockDim.x) Q[tri(i, i)] = gd[i];
+    for (int i = threadIdx.x; i < D; i += blockDim.x) P[tri(i, i)] = gd[i];
     __syncthreads();
-    lower_product_packed(P, Q, D);   // R = B A
+    lower_product_packed(Q, P, D);   // R = B A, into P
     double ld = 0.0;
-    for (int i = threadIdx.x; i < D; i += blockDim.x) ld += log(Q[tri(i, i)]);
+    for (int i = threadIdx.x; i < D; i += blockDim.x) ld += log(P[tri(i, i)]);
     ld = block_sum(ld, red);
-    for (int e = threadIdx.x; e < T; e += blockDim.x) out[D + e] = Q[e];
+    for (int e = threadIdx.x; e < T; e += blockDim.x) out[D + e] = P[e];
     if (threadIdx.x == 0) out[D + T] = -2.0 * ld;
-    // mu = m_n + chol(Sigma / kappa) z with Sigma = R^-T R^-1 (diagonal only unless full_mu_cov,
-    // the reference's pdflib.cpp:4007-4014)
-    lower_inverse_packed(Q, P, D);
-    gram_t_packed(P, Q, D, kn);
-    ok = chol_packed(Q, D, sh);
-    if (!ok) {
-        if (threadIdx.x == 0) status[p] = 1;
-        return;
-    }
-    for (int i = threadIdx.x; i < D; i += blockDim.x) {
-        double x = m[i];
-        if (full_mu_cov)
-            for (int j = 0; j <= i; ++j) x += Q[tri(i, j)] * z[j];
-        else
-            x += Q[tri(i, i)] * z[i];
-        out[i] = x;
+    // mu = m_n + chol(Sigma / kappa) z with Sigma = (R R^T)^-1 (diagonal only unless full_mu_cov, the
+    // reference's pdflib.cpp:4007-4014).  With Cm = chol(J R R^T J), R R^T = U U^T for U = J Cm J, and
+    // chol(Sigma / kappa) = U^-T / sqrt(kappa), whose diagonal is 1 / (Cm[D-1-i][D-1-i] sqrt(kappa)).
+    if (!full_mu_cov) {
+        gram_rev_packed(P, Q, D);
+        ok = chol_packed(Q, D, sh);
+        if (!ok) {
+            if (threadIdx.x == 0) status[p] = 1;
+            return;
+        }
+        const double rk = 1.0 / sqrt(kn);
+        for (int i = threadIdx.x; i < D; i += blockDim.x)
+            out[i] = m[i] + z[i] * (rk / Q[tri(D - 1 - i, D - 1 - i)]);
+    } else {
+        lower_inverse_packed(P, Q, D);   // R^-1
+        gram_t_packed(Q, P, D, kn);      // R^-T R^-1 / kappa
+        ok = chol_packed(P, D, sh);
+        if (!ok) {
+            if (threadIdx.x == 0) status[p] = 1;
+            return;
+        }
+        for (int i = threadIdx.x; i < D; i += blockDim.x) {
+            double x = m[i];
+            for (int j = 0; j <= i; ++j) x += P[tri(i, j)] * z[j];
+            out[i] = x;
+        }
     }
     if (threadIdx.x == 0) status[p] = 0;
 }

@@ -52,7 +52,7 @@ class _Options(ctypes.Structure):
                 ("init_clusters", ctypes.c_int), ("burnout", ctypes.c_int), ("argmax_sample_stop", ctypes.c_int),
                 ("split_stop", ctypes.c_int), ("max_clusters", ctypes.c_double), ("seed", ctypes.c_uint64),
                 ("full_mu_cov", ctypes.c_int), ("mult_use_weight", ctypes.c_int), ("precision", ctypes.c_int),
-                ("verbose", ctypes.c_int), ("init_from_labels", ctypes.c_int)]
+                ("verbose", ctypes.c_int), ("init_from_labels", ctypes.c_int), ("hard_clustering", ctypes.c_int)]
 
 
 _lib = None
@@ -423,8 +423,10 @@ class Sampler:
     """Stateful driver: init_model + init_first_clusters, then one group_step per step()."""
 
     def __init__(self, dev, hyper, alpha, iters, init_clusters=1, burnout=15, seed=0, max_clusters=float("inf"),
-                 precision=None, full_mu_cov=False, mult_use_weight=False, verbose=False, init_labels=None):
-        """init_labels (N, 0-based) starts from a given clustering, e.g. an over-split one."""
+                 precision=None, full_mu_cov=False, mult_use_weight=False, verbose=False, init_labels=None,
+                 hard_clustering=False):
+        """init_labels (N, 0-based) starts from a given clustering, e.g. an over-split one;
+        hard_clustering takes the argmax in every label sweep (global_params.h:38)."""
         L = load_library()
         self.dev = dev
         if precision is None:
@@ -438,7 +440,7 @@ class Sampler:
                              split_stop=5, max_clusters=float(max_clusters), seed=int(seed),
                              full_mu_cov=int(full_mu_cov), mult_use_weight=int(mult_use_weight),
                              precision=int(precision), verbose=int(verbose),
-                             init_from_labels=int(init_labels is not None))
+                             init_from_labels=int(init_labels is not None), hard_clustering=int(bool(hard_clustering)))
         self._s = ctypes.c_void_p()
         if dev.prior == prior_type.Gaussian:
             self._m0, self._psi0 = _f64(hyper.m), _f64(hyper.psi)

@@ -326,7 +326,7 @@ struct SamplerImpl {
         }
         // label sweep, sub-label sweep and statistics in one device call (one hipGraph in graph mode)
         Nsub.resize(2 * cl.size());
-        rc = dpmm_sweep_iteration(ctx, it, fin ? 1 : 0, Nsub.data());
+        rc = dpmm_sweep_iteration(ctx, it, (fin || o.hard_clustering) ? 1 : 0, Nsub.data());
         if (rc) return rc;
         counts_fresh = true;
         // one log-marginal launch: every part, and every pair of merge candidates (a cluster that is

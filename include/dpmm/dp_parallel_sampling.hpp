@@ -136,6 +136,8 @@ class dp_parallel_sampling_class {
 
     // DPMM_PREC_FP64 (Gaussian, D <= 64), DPMM_PREC_FP32 or DPMM_PREC_MIXED for the label sweep.
     void set_precision(int p) { precision_ = p; }
+    // global_params::hard_clustering (global_params.h:38): argmax labels in every sweep.
+    void set_hard_clustering(bool h) { hard_clustering_ = h; }
 
     ModelInfo dp_parallel(std::shared_ptr<hyperparams>& local_hyper_params, double alpha_param, int iters = 100,
                           int init_clusters = 1, bool verbose = true, bool draw_labels = false,
@@ -169,6 +171,7 @@ class dp_parallel_sampling_class {
         o.mult_use_weight = 0;
         o.precision = precision_;
         o.verbose = 0;
+        o.hard_clustering = hard_clustering_ ? 1 : 0;
         dpmm_sampler* s = nullptr;
         if (prior_ == Gaussian) {
             auto* h = dynamic_cast<niw_hyperparams*>(local_hyper_params.get());
@@ -227,6 +230,7 @@ class dp_parallel_sampling_class {
     unsigned long long seed_;
     prior_type prior_;
     int precision_;
+    bool hard_clustering_ = false;
     std::vector<double> data64_;
     std::vector<int32_t> counts_;
 };

@@ -241,6 +241,10 @@ typedef struct {
      * instead of uniform ones, as the reference's model-file constructor continues a saved model
      * (dp_parallel_sampling.h:15); sub-labels are drawn uniformly. */
     int init_from_labels;
+    /* != 0: hard assignments, every label sweep takes the argmax (global_params.h:38,
+     * local_clusters_actions.cpp:466: sample_labels(group, hard_clustering ? true : final, ...));
+     * split / merge decisions still use the final flag of the iteration. */
+    int hard_clustering;
 } dpmm_options;
 
 /* init_model + init_first_clusters (dp_parallel_sampling.cpp:91-130).  Gaussian hyper:

@@ -108,3 +108,31 @@ def test_dp_parallel_verbose_histories():
     assert len(mi.likelihood_history) == 40 and len(mi.nmi_score_history) == 40
     assert all(np.isfinite(mi.likelihood_history)) and all(v != 1.0 for v in mi.likelihood_history)
     assert abs(mi.nmi_score_history[-1] - dp._nmi(gt, mi.labels)) <= 1e-12
+
+
+def test_hard_clustering_takes_the_argmax():
+    """global_params::hard_clustering (local_clusters_actions.cpp:466): every label sweep takes the
+    argmax.  After a step that changed no cluster, the labels are the argmax of the noise-free
+    scores of the parameters that step drew (the soft sampler's are not, on overlapping clusters)."""
+    N, D, K = 60_000, 8, 5
+    X, _, _ = gaussian_mixture(N, D, K, seed=31, a=2.0, ridge=0.3, dtype=np.float32)
+    hyper = dp.niw_hyperparams(1.0, np.zeros(D), D + 3.0, np.eye(D))
+    say("hard clustering")
+    mism = {}
+    for hard in (True, False):
+        dev = dp.DeviceSweep(X, dp.prior_type.Gaussian, seed=9)
+        smp = dp.Sampler(dev, hyper, 10.0, 200, seed=9, precision=dp.Precision.FP32, hard_clustering=hard)
+        kprev = 0
+        for t in range(1, 61):
+            smp.step(t)
+            k = smp.num_clusters()
+            if t >= 30 and k == kprev:
+                lab, _ = dev.get_labels()
+                s = dev.log_likelihoods(k, 0, 0, N)
+                mism[hard] = int(np.count_nonzero(lab != np.argmax(s, axis=1)))
+                break
+            kprev = k
+        smp.close()
+        dev.close()
+    print("labels off the argmax: hard", mism.get(True), "soft", mism.get(False))
+    assert mism[True] == 0

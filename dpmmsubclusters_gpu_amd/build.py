@@ -26,6 +26,9 @@ HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-fopenmp", "-Wall", "-Wno-unused-fu
 DEVICE_SOURCES = ["dpmm_device.hip", "inst_gauss_small.hip", "inst_gauss_64.hip", "inst_gauss_96.hip",
                   "inst_gauss_128.hip", "inst_mixed.hip", "dpmm_screener.hip"]
 HOST_SOURCES = ["sampler.cpp"]
+# The BF16 screens read every MFMA result back into VALU arithmetic: the VGPR form of the MFMA
+# instructions saves one v_accvgpr_read per result element.
+SOURCE_FLAGS = {"inst_mixed.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 
 
 def _run(cmd):
@@ -40,7 +43,7 @@ def build(verbose=True, jobs=None):
     procs = []
     for src in DEVICE_SOURCES:
         obj = os.path.join(BUILD, src + ".o")
-        cmd = [HIPCC] + DEVICE_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = [HIPCC] + DEVICE_FLAGS + SOURCE_FLAGS.get(src, []) + ["-c", os.path.join(CSRC, src), "-o", obj]
         procs.append((cmd, subprocess.Popen(cmd)))
         objs.append(obj)
     for src in HOST_SOURCES:

@@ -234,6 +234,7 @@ __global__ void __launch_bounds__(kThreads) k_sub_gauss_mx(const float* __restri
     // |y_screen - y_fp32| <= eps a_j: split residuals (3 x 2^-18 of |W||d|) and the FP32 sums of the
     // screen (4 D terms) and of the sweep (D terms), with a_j from BF16 magnitudes inflated by 2^-6
     const float eps = (3.0f * 0x1p-18f + (5.0f * (float)DP + 16.0f) * 0x1p-24f) * (1.0f + 0x1p-6f);
+    const float heps = 0.5f * eps;
     const int lane = threadIdx.x & 63, h = lane >> 5, pl = lane & 31, wv = threadIdx.x >> 6;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     __shared__ int32_t wlist[4][64];
@@ -253,39 +254,53 @@ __global__ void __launch_bounds__(kThreads) k_sub_gauss_mx(const float* __restri
             const bf16x8* thi = reinterpret_cast<const bf16x8*>(slot);
             const bf16x8* tlo = reinterpret_cast<const bf16x8*>(slot + TB);
             const float* mu = reinterpret_cast<const float*>(slot + 2 * TB);
+            // d = x - mu split as d_hi + d_lo: one packed conversion per coordinate pair gives both
+            // BF16 highs, read back as FP32 by shifting / masking the packed word
+            u32x4 dh[KC], dl[KC];
+#pragma unroll
+            for (int kc = 0; kc < KC; ++kc)
+#pragma unroll
+                for (int e = 0; e < 8; e += 2) {
+                    typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+                    const float d0 = x[kc][e] - mu[16 * kc + 8 * h + e];
+                    const float d1 = x[kc][e + 1] - mu[16 * kc + 8 * h + e + 1];
+                    bf16x2 hp;
+                    hp[0] = (__bf16)d0;
+                    hp[1] = (__bf16)d1;
+                    const uint32_t hb = __builtin_bit_cast(uint32_t, hp);
+                    bf16x2 lp;
+                    lp[0] = (__bf16)(d0 - __uint_as_float(hb << 16));
+                    lp[1] = (__bf16)(d1 - __uint_as_float(hb & 0xffff0000u));
+                    dh[kc][e / 2] = hb;
+                    dl[kc][e / 2] = __builtin_bit_cast(uint32_t, lp);
+                }
+            // qq = sum y_j^2; ee = sum a_j (|y_j| + eps a_j / 2), so that the bound
+            // sum (2 |y_j| + eps a_j) eps a_j is 2 eps ee
             float qq = 0.f, ee = 0.f;
 #pragma unroll
             for (int rb = 0; rb < RB; ++rb) {
                 f32x16 y = {}, a = {};
 #pragma unroll
                 for (int kc = 2 * rb; kc < KC; ++kc) {
-                    bf16x8 dh, dl;
-#pragma unroll
-                    for (int e = 0; e < 8; ++e) {
-                        const float d = x[kc][e] - mu[16 * kc + 8 * h + e];
-                        const __bf16 hi = (__bf16)d;
-                        dh[e] = hi;
-                        dl[e] = (__bf16)(d - (float)hi);
-                    }
                     const bf16x8 wh = thi[(rb * KC + kc) * 64 + lane];
                     const bf16x8 wl = tlo[(rb * KC + kc) * 64 + lane];
+                    const bf16x8 bh = __builtin_bit_cast(bf16x8, dh[kc]), bl = __builtin_bit_cast(bf16x8, dl[kc]);
                     const u32x4 wabs = __builtin_bit_cast(u32x4, wh) & 0x7fff7fffu;
-                    const u32x4 dabs = __builtin_bit_cast(u32x4, dh) & 0x7fff7fffu;
-                    y = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wh, dh, y, 0, 0, 0);
-                    y = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wh, dl, y, 0, 0, 0);
-                    y = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wl, dh, y, 0, 0, 0);
+                    const u32x4 dabs = dh[kc] & 0x7fff7fffu;
+                    y = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wh, bh, y, 0, 0, 0);
+                    y = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wh, bl, y, 0, 0, 0);
+                    y = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wl, bh, y, 0, 0, 0);
                     a = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, wabs),
                                                                 __builtin_bit_cast(bf16x8, dabs), a, 0, 0, 0);
                 }
 #pragma unroll
                 for (int v = 0; v < 16; ++v) {
                     qq = fmaf(y[v], y[v], qq);
-                    const float ea = eps * a[v];
-                    ee = fmaf(2.0f * fabsf(y[v]) + ea, ea, ee);
+                    ee = fmaf(a[v], fmaf(heps, a[v], fabsf(y[v])), ee);
                 }
             }
             qq += __shfl_xor(qq, 32);
-            ee += __shfl_xor(ee, 32);
+            ee = 2.0f * eps * (ee + __shfl_xor(ee, 32));
             const float sv = mu[DP] - 0.5f * qq;
             const float dv = ee + (2.0f * (float)DP + 4.0f) * 0x1p-24f * qq;   // the two sums of squares
             if (hh == 0) { s0 = sv; dq0 = dv; } else { s1 = sv; dq1 = dv; }
@@ -347,6 +362,15 @@ k_label_gauss_mxp(const float* __restrict__ X, int D, int K, int CH, const int32
     const int end = cbeg + (int)((int64_t)len * (part + 1) / kPruneSplit);
     const float* mrow = pm + (size_t)a * K;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    // the anchor's slot, scored by every pass of the workgroup, is staged in LDS once; the other
+    // clusters' slots are read from global memory by the few passes that score them
+    __shared__ __align__(16) unsigned char sa[SLOT];
+    {
+        const uint4* src = reinterpret_cast<const uint4*>(PB + (size_t)a * SLOT);
+        uint4* dst = reinterpret_cast<uint4*>(sa);
+        for (int e = threadIdx.x; e < SLOT / 16; e += kThreads) dst[e] = src[e];
+    }
+    __syncthreads();
     // the anchor's bound inputs of clusters 0..63, one per lane (the same for every pass)
     const bool in0 = lane < K && lane != a;
     const float sg0 = in0 ? psc[2 * lane] : 0.f, ck0 = in0 ? psc[2 * lane + 1] : 0.f, m0 = in0 ? mrow[lane] : 0.f;
@@ -358,7 +382,9 @@ k_label_gauss_mxp(const float* __restrict__ X, int D, int K, int CH, const int32
         const int pos = p0 + pl;
         const bool val = pos < end;
         const int idx = val ? idx_next : 0;
-        idx_next = (pos + 128 < end) ? perm[pos + 128] : 0;   // the next pass's point, loaded ahead
+        // the next pass's point, loaded ahead; past the end the load repeats the last entry (a
+        // branch-free load, so that waiting for it is left to its first use in the next pass)
+        idx_next = perm[pos + 128 < end ? pos + 128 : end - 1];
         const bool own_valid = val && h == 0;
         // the point's coordinates 16 kc + 8 h .. + 7 (row 0 stands in for an empty lane)
         auto load_x = [&](gfloat* Xs, float (&x)[KC][8]) {
@@ -372,8 +398,9 @@ k_label_gauss_mxp(const float* __restrict__ X, int D, int K, int CH, const int32
         constexpr bool kHoldX = DP <= 32;
         float xh[kHoldX ? KC : 1][8];
         if constexpr (kHoldX) load_x((gfloat*)X, xh);
-        // screened q of cluster k for this lane's point, and sum (x - mu_k)^2
-        auto score = [&](int k, float& d2) -> float {
+        // screened q of the cluster whose slot is `slot` (the anchor's in LDS, or one in global
+        // memory) for this lane's point, and sum (x - mu_k)^2
+        auto score = [&](const unsigned char* slot, float& d2) -> float {
             float xr[kHoldX ? 1 : KC][8];
             if constexpr (!kHoldX) {
                 gfloat* Xs = (gfloat*)X;
@@ -383,7 +410,6 @@ k_label_gauss_mxp(const float* __restrict__ X, int D, int K, int CH, const int32
             auto& x = [&]() -> float(&)[KC][8] {
                 if constexpr (kHoldX) return xh; else return xr;
             }();
-            const unsigned char* slot = PB + (size_t)k * SLOT;
             const bf16x8* tiles = reinterpret_cast<const bf16x8*>(slot);
             const float4* mu4 = reinterpret_cast<const float4*>(slot + mx_tiles_bytes(DP));
             bf16x8 Bv[KC];
@@ -413,8 +439,8 @@ k_label_gauss_mxp(const float* __restrict__ X, int D, int K, int CH, const int32
             d2 = dd + __shfl_xor(dd, 32);
             return q + __shfl_xor(q, 32);
         };
-        auto noisy = [&](int k, float q) -> float {
-            const float c = reinterpret_cast<const float*>(PB + (size_t)k * SLOT + mx_tiles_bytes(DP))[DP];
+        auto noisy = [&](const unsigned char* slot, int k, float q) -> float {
+            const float c = reinterpret_cast<const float*>(slot + mx_tiles_bytes(DP))[DP];
             float s = c - 0.5f * q;
             if (!final_iter) {
                 const Ctr4 w = philox4x32_10((uint32_t)idx, (uint32_t)(k >> 2), iter, TAG_LABEL, key0, key1);
@@ -425,8 +451,8 @@ k_label_gauss_mxp(const float* __restrict__ X, int D, int K, int CH, const int32
         // the anchor, then the wave bounds: r >= ||x - mu_a|| over the points (the FP32 rounding of
         // the distance covered) and T = the smallest screened lower bound s_a - e_a
         float d2 = 0.f;
-        const float qa = score(a, d2);
-        float sb = noisy(a, qa), eb = tau * qa + kNoiseSlack, U = neg_inf<float>();
+        const float qa = score(sa, d2);
+        float sb = noisy(sa, a, qa), eb = tau * qa + kNoiseSlack, U = neg_inf<float>();
         int bi = a;
         float rr = own_valid ? sqrtf(d2) : 0.f;
         float tt = own_valid ? sb - eb : __builtin_huge_valf();
@@ -457,8 +483,9 @@ k_label_gauss_mxp(const float* __restrict__ X, int D, int K, int CH, const int32
                 todo &= todo - 1;
                 ++ne;
                 float unused;
-                const float qk = score(k, unused);
-                const float s = noisy(k, qk);
+                const unsigned char* slot = PB + (size_t)k * SLOT;
+                const float qk = score(slot, unused);
+                const float s = noisy(slot, k, qk);
                 const float err = tau * qk + kNoiseSlack;
                 if (s > sb) {
                     U = fmaxf(U, sb + eb);

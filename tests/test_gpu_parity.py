@@ -266,3 +266,30 @@ def test_end_to_end_d32_fp32_matches_oracle_distribution():
     print("gpu K", ks, "nmi", nmis, "oracle K", oks, "nmi", onmis)
     assert abs(np.mean(nmis) - np.mean(onmis)) < 0.05
     assert abs(np.mean(ks) - np.mean(oks)) <= 2
+
+
+def _splits_after_cap(khist, k_start, cap):
+    """Iterations that began with at least `cap` clusters and ended with more than they began with."""
+    starts = [k_start] + list(khist[:-1])
+    return [t + 1 for t, (a, b) in enumerate(zip(starts, khist)) if a >= cap and b > a]
+
+
+def test_max_clusters_stops_splits():
+    """max_clusters (dp_parallel_sampling.cpp:147): an iteration that starts with at least
+    max_clusters clusters runs with no_more_splits, so from then on the count only falls (merges,
+    removals); splits accepted within the iteration that first reaches the cap may overshoot it.
+    Checked on the device and on the oracle with the same data and seed."""
+    D, cap = 8, 3
+    X, gt, _ = gaussian_mixture(60_000, D, 8, seed=5, dtype=np.float64)
+    say("max_clusters")
+    model = dp.dp_parallel_sampling_class(X.shape[0], X.T, 4, dp.prior_type.Gaussian)
+    hyper = dp.niw_hyperparams(1.0, np.zeros(D), D + 3.0, np.eye(D))
+    mi = model.dp_parallel(hyper, 10.0, iters=60, verbose=False, max_clusters=cap)
+    kd = list(mi.cluster_count_history)
+    ref = orc.run(X, prior=0, alpha=10.0, iters=60, seed=4, k0=1.0, v0=D + 3.0, max_clusters=cap)
+    ko = [int(k) for k in ref["k_hist"]]
+    free = model.dp_parallel(hyper, 10.0, iters=60, verbose=False).cluster_count_history
+    print("device", kd, "\noracle", ko, "\nuncapped device", list(free))
+    for k in (kd, ko):
+        assert max(k) >= cap and _splits_after_cap(k, 1, cap) == []
+    assert kd[-1] < free[-1]

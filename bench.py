@@ -23,7 +23,7 @@ HBM_PEAK = 8000.0                                       # GB/s
 # HBM bytes per launch of a phase's kernel, measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
 # (scaled by what the counters read for a copy of known size); keyed by (config, precision, phase)
 TRAFFIC = {("c3_d64", "mixed", "statistics"): 3.996e9 + 0.140e9,
-           ("c3_d64", "mixed", "label_sweep"): 2.805e9 + 0.269e9,
+           ("c3_d64", "mixed", "label_sweep"): 2.733e9 + 0.270e9,
            ("c3_d64", "mixed", "sub_sweep"): 2.617e9 + 0.267e9}
 
 CONFIGS = {

@@ -765,6 +765,15 @@ struct dpmm_ctx {
     bool prune_valid = false;
     int prune_skip = 0;           // adaptive: unpruned sweeps left before pruning is tried again
     unsigned long long* hEvals = nullptr;   // pinned copy of dEvals, read at the next sweep
+    // pinned staging of the per-iteration transfers of the log-ML batch and the parameter draws
+    // (members, log weights, results, draw status): a copy from pageable memory blocks the host
+    // and goes through a driver bounce buffer.  Each call that uses them ends synchronised.
+    void* pinMembers = nullptr;
+    size_t pinMembersCap = 0;
+    void* pinLogW = nullptr;
+    size_t pinLogWCap = 0;
+    void* pinOut = nullptr;
+    size_t pinOutCap = 0;
     hipEvent_t evEvals = nullptr;
     bool evals_pending = false;
     int evals_K = 0;
@@ -1410,6 +1419,8 @@ int dpmm_destroy(dpmm_ctx* c) {
         if (p) (void)hipFree(p);
     if (c->hStats) (void)hipHostFree(c->hStats);
     if (c->hEvals) (void)hipHostFree(c->hEvals);
+    for (void* p : {c->pinMembers, c->pinLogW, c->pinOut})
+        if (p) (void)hipHostFree(p);
     if (c->evEvals) (void)hipEventDestroy(c->evEvals);
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     for (auto& e : c->ev)
@@ -2053,13 +2064,29 @@ int dpmm_set_prior(dpmm_ctx* c, double k0, const double* m0, double v0, const do
 }
 
 namespace {
+// Pinned host buffer of at least `bytes`; a larger one replaces it (never while a sweep is being
+// captured).  Callers synchronise the stream before returning, so no copy uses the old buffer.
+int pin_reserve(void*& p, size_t& cap, size_t bytes) {
+    if (cap >= bytes) return DPMM_OK;
+    if (t_capturing) return fail(DPMM_ESTATE, "buffer growth while a sweep is being captured");
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+    if (hipHostMalloc(&p, bytes) != hipSuccess) return fail(DPMM_ENOMEM, "pinned host buffer");
+    cap = bytes;
+    return DPMM_OK;
+}
+
 int upload_members(dpmm_ctx* c, int n, const int32_t* members) {
     const int lim = 2 * c->Kstats;
     for (size_t e = 0; e < (size_t)4 * n; ++e)
         if (members[e] < -1 || members[e] >= lim) return fail(DPMM_EINVAL, "member outside the current statistics");
     int rc = grow(c->dMembers, c->membersCap, (size_t)4 * n);
     if (rc) return rc;
-    HIPCHK(hipMemcpyAsync(c->dMembers, members, (size_t)4 * n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    const size_t bytes = (size_t)4 * n * sizeof(int32_t);
+    if ((rc = pin_reserve(c->pinMembers, c->pinMembersCap, bytes))) return rc;
+    std::memcpy(c->pinMembers, members, bytes);
+    HIPCHK(hipMemcpyAsync(c->dMembers, c->pinMembers, bytes, hipMemcpyHostToDevice, c->stream));
     return DPMM_OK;
 }
 
@@ -2115,8 +2142,9 @@ int dpmm_log_marginal_batch(dpmm_ctx* c, int n, const int32_t* members, double* 
     hipLaunchKernelGGL(k_log_marginal, dim3(n), dim3(kThreads), lds, c->stream, c->dStats, (int)c->statEntries(),
                        c->DP, D, c->prior, c->dMembers, c->prior_args, c->dLM);
     HIPCHK(hipGetLastError());
-    std::vector<double> out((size_t)2 * n);
-    HIPCHK(hipMemcpyAsync(out.data(), c->dLM, out.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if ((rc = pin_reserve(c->pinOut, c->pinOutCap, (size_t)2 * n * sizeof(double)))) return rc;
+    const double* out = static_cast<const double*>(c->pinOut);
+    HIPCHK(hipMemcpyAsync(c->pinOut, c->dLM, (size_t)2 * n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     for (int j = 0; j < n; ++j) {
         logml[j] = out[2 * j];
@@ -2134,7 +2162,8 @@ int dpmm_sample_params(dpmm_ctx* c, uint32_t iter, int K, const int32_t* members
     const int D = c->D, np = 3 * K;
     int rc = upload_members(c, np, members);
     if (rc) return rc;
-    std::vector<double> logw((size_t)3 * K);
+    if ((rc = pin_reserve(c->pinLogW, c->pinLogWCap, (size_t)3 * K * sizeof(double)))) return rc;
+    double* logw = static_cast<double*>(c->pinLogW);
     for (int k = 0; k < K; ++k) {
         if (!(weights[k] >= 0.0) || !(lr_weights[2 * k] >= 0.0) || !(lr_weights[2 * k + 1] >= 0.0))
             return fail(DPMM_EINVAL, "weights must be non-negative");
@@ -2143,10 +2172,10 @@ int dpmm_sample_params(dpmm_ctx* c, uint32_t iter, int K, const int32_t* members
         logw[K + 2 * k + 1] = std::log(lr_weights[2 * k + 1]);
     }
     const int dstride = c->drawStride();
-    if ((rc = grow(c->dLogW, c->logwCap, logw.size()))) return rc;
+    if ((rc = grow(c->dLogW, c->logwCap, (size_t)3 * K))) return rc;
     if ((rc = grow(c->dDraws, c->drawsCap, (size_t)np * dstride))) return rc;
     if ((rc = grow(c->dDrawStatus, c->drawStatusCap, (size_t)np))) return rc;
-    HIPCHK(hipMemcpyAsync(c->dLogW, logw.data(), logw.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->dLogW, logw, (size_t)3 * K * sizeof(double), hipMemcpyHostToDevice, c->stream));
     const bool gauss = c->prior == DPMM_PRIOR_GAUSSIAN;
     const size_t lds = gauss ? lds_bytes((size_t)D * (D + 1) + 3 * D) : lds_bytes(D);
     if ((rc = allow_lds(reinterpret_cast<const void*>(k_draw_params), lds))) return rc;
@@ -2191,8 +2220,9 @@ int dpmm_sample_params(dpmm_ctx* c, uint32_t iter, int K, const int32_t* members
             return rc;
     }
     HIPCHK(hipGetLastError());
-    std::vector<int32_t> st(np);
-    HIPCHK(hipMemcpyAsync(st.data(), c->dDrawStatus, np * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    if ((rc = pin_reserve(c->pinOut, c->pinOutCap, np * sizeof(int32_t)))) return rc;
+    const int32_t* st = static_cast<const int32_t*>(c->pinOut);
+    HIPCHK(hipMemcpyAsync(c->pinOut, c->dDrawStatus, np * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     c->Kdraw = K;
     for (int p = 0; p < np; ++p)

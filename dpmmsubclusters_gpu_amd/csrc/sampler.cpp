@@ -410,6 +410,13 @@ struct SamplerImpl {
             log_feat.clear();
             log_h.clear();
             log_u.clear();
+            // per-cluster terms of the ratio, computed once (the same values as inline calls)
+            std::vector<double> lg_n(K), lg_nh(K);
+            for (int k = 0; k < K; ++k) {
+                const double Nk = (double)cl[k].c.N;
+                lg_n[k] = cl[k].c.N > 0 ? std::lgamma(Nk) : 0.0;
+                lg_nh[k] = std::lgamma(Nk + 0.5 * al);
+            }
             size_t p = 0;
             for (int a = 0; a < ncand; ++a)
                 for (int b = a + 1; b < ncand; ++b, ++p) {
@@ -419,10 +426,16 @@ struct SamplerImpl {
                     // a cluster split or reset in this iteration is no longer a candidate
                     const bool elig = cl[i].splittable && cl[j].splittable && cl[i].c.N > 0 && cl[j].c.N > 0;
                     if (!elig || ((used[i] || used[j]) && !logging)) continue;
+                    if (!logging) {
+                        // lgamma(Nm) - lgamma(Nm + alpha) < 0 for Nm >= 2, so without it the ratio is an
+                        // upper bound; far below log(2^-32) (the smallest uniform) and log(0.1) the pair
+                        // is rejected whatever its uniform, which is then not drawn (keyed streams)
+                        const double ub = cst + lg_nh[i] - lg_n[i] - lg_n[j] + lg_nh[j] + L - cl[i].c.lml - cl[j].c.lml;
+                        if (ub < -100.0) continue;
+                    }
                     const double Ni = (double)cl[i].c.N, Nj = (double)cl[j].c.N, Nm = Ni + Nj;
-                    const double lhr = cst + std::lgamma(Nm) - std::lgamma(Nm + al) + std::lgamma(Ni + 0.5 * al) -
-                                       std::lgamma(Ni) - std::lgamma(Nj) + std::lgamma(Nj + 0.5 * al) + L -
-                                       cl[i].c.lml - cl[j].c.lml;
+                    const double lhr = cst + std::lgamma(Nm) - std::lgamma(Nm + al) + lg_nh[i] - lg_n[i] - lg_n[j] +
+                                       lg_nh[j] + L - cl[i].c.lml - cl[j].c.lml;
                     const double u = Stream(o.seed, ((uint32_t)i << 16) | (uint32_t)j, it, TAG_MERGE_U).uniform();
                     if (logging) {
                         log_feat.insert(log_feat.end(), pair_feat.begin() + p * 16, pair_feat.begin() + p * 16 + 16);

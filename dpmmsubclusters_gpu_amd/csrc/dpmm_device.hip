@@ -530,7 +530,8 @@ __global__ void __launch_bounds__(kThreads) k_relabel(int32_t* __restrict__ labe
 // ||W||_1 ||W||_inf is kept.  One workgroup per cluster k; LDS: W (D x D floats),
 // then one FP64 difference vector per wave, two power-iteration vectors and a packed D x D matrix.
 __global__ void __launch_bounds__(kThreads) k_prune_prep(const float* __restrict__ P, int K, int D, int DP,
-                                                         float* __restrict__ pm, float* __restrict__ psc) {
+                                                         float* __restrict__ pm, float* __restrict__ psc,
+                                                         double* __restrict__ pv, int kwarm) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int DS = D + 1;   // padded row stride: rows of W start in different LDS banks
     float* W = reinterpret_cast<float*>(smem);
@@ -588,20 +589,28 @@ __global__ void __launch_bounds__(kThreads) k_prune_prep(const float* __restrict
     // threads per row, each summing every TPR-th term, combined by butterfly shuffles
     const int TPR = D > 64 ? 2 : (D > 32 ? 4 : 8);
     const int row = threadIdx.x / TPR, part = threadIdx.x % TPR;
+    // start: this slot's vector of the last call (the parameters of a cluster change little between
+    // iterations, so its dominant direction does too) with fewer steps, else a fixed vector; the
+    // estimate only chooses the certificate's t, so a poor start costs time, not correctness
+    bool warm = k < kwarm;
     {
-        const double v0 = row < D && part == 0 ? 1.0 + 0.125 * (row % 7) : 0.0;
-        const double nv = block_sum(v0 * v0, red);
+        const double w0 = warm && row < D && part == 0 ? pv[(size_t)k * DP + row] : 0.0;
+        const double nw = block_sum(w0 * w0, red);
+        warm = warm && nw > 0.0 && nw < 1e300;   // uniform: every thread holds the same sum
+        const double v0 = row < D && part == 0 ? (warm ? w0 : 1.0 + 0.125 * (row % 7)) : 0.0;
+        const double nv = warm ? nw : block_sum(v0 * v0, red);
         if (row < D && part == 0) v[row] = v0 / sqrt(nv);
     }
     __syncthreads();
     double s2 = 0.0;
-    for (int itp = 0; itp < 14; ++itp) {
+    const int nit = warm ? 6 : 14;
+    for (int itp = 0; itp < nit; ++itp) {
         double uj = 0.0;
         if (row < D)
             for (int l = row + part; l < D; l += TPR) uj = fma((double)W[row * DS + l], v[l], uj);
         for (int o = 1; o < TPR; o <<= 1) uj += __shfl_xor(uj, o);
         if (row < D && part == 0) uu[row] = uj;
-        if (itp == 13) s2 = block_sum(row < D && part == 0 ? uj * uj : 0.0, red);   // also publishes uu
+        if (itp == nit - 1) s2 = block_sum(row < D && part == 0 ? uj * uj : 0.0, red);   // also publishes uu
         else __syncthreads();                                                      // publishes uu
         double zi = 0.0;
         if (row < D)
@@ -611,6 +620,7 @@ __global__ void __launch_bounds__(kThreads) k_prune_prep(const float* __restrict
         if (row < D && part == 0 && nz > 0.0) v[row] = zi / sqrt(nz);
         __syncthreads();
     }
+    if (row < D && part == 0) pv[(size_t)k * DP + row] = v[row];
     const double fac[4] = {1.05, 1.2, 1.5, 3.0};
     for (int tr = 0; tr < 4; ++tr) {   // uniform: s2 and sig2 are the same in every thread
         const double tt = s2 * fac[tr];
@@ -789,6 +799,9 @@ struct dpmm_ctx {
     size_t pruneMCap = 0;
     float* dPruneSC = nullptr;    // 2K: sigma_k, c_k
     size_t pruneSCCap = 0;
+    double* dPruneV = nullptr;    // kMaxK x DP: each slot's last power-iteration vector (warm start)
+    size_t pruneVCap = 0;
+    int prune_v_k = 0;            // slots holding a vector
     unsigned long long* dEvals = nullptr;   // cluster evaluations, wave passes of the last pruned sweep
     size_t evalsCap = 0;
     int last_pruned_K = 0;        // K of the last pruned sweep, 0 if the last sweep was not pruned
@@ -1293,11 +1306,13 @@ int ensure_prune(dpmm_ctx* c) {
     int rc = grow(c->dPruneM, c->pruneMCap, (size_t)K * K);
     if (rc) return rc;
     if ((rc = grow(c->dPruneSC, c->pruneSCCap, (size_t)2 * K))) return rc;
+    if ((rc = grow(c->dPruneV, c->pruneVCap, (size_t)kMaxK * c->DP))) return rc;   // once, at full size
     const size_t lds = (((size_t)D * (D + 1) * 4 + 15) & ~(size_t)15) + ((size_t)6 * D + (size_t)D * (D + 1) / 2) * 8;
     if ((rc = allow_lds(reinterpret_cast<const void*>(k_prune_prep), lds))) return rc;
     hipLaunchKernelGGL(k_prune_prep, dim3(K), dim3(kThreads), lds, c->stream, reinterpret_cast<const float*>(c->dP),
-                       K, D, c->DP, c->dPruneM, c->dPruneSC);
+                       K, D, c->DP, c->dPruneM, c->dPruneSC, c->dPruneV, std::min(c->prune_v_k, K));
     HIPCHK(hipGetLastError());
+    c->prune_v_k = std::max(c->prune_v_k, K);
     c->prune_valid = true;
     return DPMM_OK;
 }
@@ -1415,7 +1430,7 @@ int dpmm_destroy(dpmm_ctx* c) {
                     (void*)c->dPartials, (void*)c->dStats, c->dPB, (void*)c->dRecheck, (void*)c->dRecheckCount,
                     (void*)c->dPrior, (void*)c->dMembers, (void*)c->dLM, (void*)c->dDraws, (void*)c->dDrawStatus,
                     (void*)c->dLogW, (void*)c->dFeat, (void*)c->dMB, c->dSPB, (void*)c->dPruneM,
-                    (void*)c->dPruneSC, (void*)c->dEvals, (void*)c->dDirty, (void*)c->dClasses})
+                    (void*)c->dPruneSC, (void*)c->dEvals, (void*)c->dDirty, (void*)c->dClasses, (void*)c->dPruneV})
         if (p) (void)hipFree(p);
     if (c->hStats) (void)hipHostFree(c->hStats);
     if (c->hEvals) (void)hipHostFree(c->hEvals);
@@ -1783,6 +1798,7 @@ static int ensure_sweep_buffers(dpmm_ctx* c) {
     if (c->prior == DPMM_PRIOR_GAUSSIAN && c->precision == DPMM_PREC_MIXED) {
         if ((rc = grow(c->dPruneM, c->pruneMCap, (size_t)K * K))) return rc;
         if ((rc = grow(c->dPruneSC, c->pruneSCCap, (size_t)2 * K))) return rc;
+        if ((rc = grow(c->dPruneV, c->pruneVCap, (size_t)kMaxK * c->DP))) return rc;
         if ((rc = grow(c->dEvals, c->evalsCap, 2))) return rc;
     }
     return DPMM_OK;

@@ -896,12 +896,13 @@ int ensure_partition(dpmm_ctx* c, int K) {
     return DPMM_OK;
 }
 
-// Points per partition chunk, aiming at about 4096 chunks (2048 for DP > 64): more, smaller
-// workgroups shorten the tail of the per-chunk kernels, fewer keep the chunk reduction small.
-// Measured at N = 1e7, D = 64: 4.03-4.09 ms per sampler iteration with 4096 chunks, 4.13 with 2048,
-// 4.33 with 8192; at N = 5e6, D = 128: 39.0 ms with 2048, 39.6 with 1024.
+// Points per partition chunk, aiming at about 4096 chunks from 2^22 points on (2048 below, and for
+// DP > 64): more, smaller workgroups shorten the tail of the per-chunk kernels, fewer keep the chunk
+// reduction small.  Measured per sampler iteration: N = 1e7, D = 64: 4.03-4.09 ms with 4096 chunks,
+// 4.13 with 2048, 4.33 with 8192; N = 1e6, D = 32: 0.576 ms with 2048, 0.597 with 4096; N = 5e6,
+// D = 128: 39.0 ms with 2048, 39.6 with 1024.
 int chunk_size(const dpmm_ctx* c) {
-    const int64_t target = c->DP > 64 ? 2048 : 4096;
+    const int64_t target = c->DP > 64 || c->N < (int64_t(1) << 22) ? 2048 : 4096;
     int64_t ch = (c->N + target - 1) / target;
     ch = std::max<int64_t>(ch, 256);
     ch = (ch + 63) / 64 * 64;

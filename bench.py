@@ -2,10 +2,13 @@
 
 A step is one full sampler iteration (local_clusters_actions::group_step): parameter draws,
 label sweep, sub-label sweep, statistics, split / merge / reset / remove, over the whole synthetic
-data set resident in HBM.  The metric is point-iterations per second.  Warmup iterations let the
-sampler grow from one cluster to the mixture's cluster count before the timed steps.
+data set resident in HBM.  The metric is point-iterations per second at steady state: an untimed
+burn-in (at least 150 iterations, then until the cluster count has not changed for 20 iterations)
+lets the sampler grow from one cluster to the mixture's cluster count, whatever --warmup says;
+--warmup iterations follow it untimed, then --steps iterations are timed.  The run exits with
+status 3 if fewer than 0.9 x the true cluster count are present when the timed steps start.
 
-    python bench.py [--gpus 1] [--steps 20] [--warmup 150] [--config c3_d64]
+    python bench.py [--gpus 1] [--steps 20] [--warmup 5] [--config c3_d64]
 """
 import argparse
 import json
@@ -20,11 +23,22 @@ sys.path.insert(0, ROOT)
 
 PEAK = {"fp32": 157.3, "fp64": 78.6, "bf16": 2500.0}   # dense TFLOP/s, MI355X
 HBM_PEAK = 8000.0                                       # GB/s
-# HBM bytes per launch of a phase's kernel, measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
-# (scaled by what the counters read for a copy of known size); keyed by (config, precision, phase)
-TRAFFIC = {("c3_d64", "mixed", "statistics"): 3.996e9 + 0.273e9,
-           ("c3_d64", "mixed", "label_sweep"): 2.733e9 + 0.270e9,
-           ("c3_d64", "mixed", "sub_sweep"): 2.617e9 + 0.267e9}
+# HBM bytes per launch of each phase's kernels, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of
+# this script (scaled by what the counters read for a copy of known size), with the workload they were
+# measured on; a line carries them only when its own workload (config, precision, cluster count) matches.
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+
+
+def measured_traffic(config, precision, clusters, phase):
+    try:
+        with open(TRAFFIC_FILE) as f:
+            rows = json.load(f)["rows"]
+    except (OSError, ValueError, KeyError):
+        return None
+    for r in rows:
+        if r["config"] == config and r["precision"] == precision and r["clusters"] == clusters:
+            return r["bytes_per_launch"].get(phase)
+    return None
 
 CONFIGS = {
     # N, D, true clusters, A scale, ridge, Dirichlet weight (None equal), alpha, dtype
@@ -316,7 +330,10 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=150)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--burn-in", type=int, default=150,
+                    help="minimum untimed iterations before --warmup; burn-in then continues until the cluster "
+                         "count has not changed for 20 iterations (at most 400 in all)")
     ap.add_argument("--config", default="c3_d64", choices=sorted(CONFIGS))
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp32", "fp64"])
     ap.add_argument("--seed", type=int, default=0)
@@ -376,21 +393,40 @@ def main():
         torch.cuda.synchronize()
         print(f"[bench] copy probe: {X.numel() * X.element_size()} bytes", file=sys.stderr, flush=True)
         del Y
-    iters = args.warmup + args.steps + 10
+    max_burn = max(400, args.burn_in)
+    # the schedule's final / no-more-splits iterations lie beyond every iteration run here
+    iters = max_burn + args.warmup + args.steps + 10
     dev.set_graph_mode(args.graph)
     smp = dp.Sampler(dev, hyper, cfg["alpha"], iters, seed=args.seed + 1, precision=prec)
     t_w = time.perf_counter()
-    for t in range(1, args.warmup + 1):
+    t, last_change, k_prev = 0, 0, smp.num_clusters()
+    while t < max_burn and (t < args.burn_in or t - last_change < 20):
+        t += 1
         smp.step(t)
-        if t % 10 == 0 or t == args.warmup:
-            print(f"[bench] warmup {t}/{args.warmup} K={smp.num_clusters()} ({time.perf_counter() - t_w:.1f}s)",
-                  file=sys.stderr, flush=True)
+        k = smp.num_clusters()
+        if k != k_prev:
+            last_change, k_prev = t, k
+        if t % 25 == 0:
+            print(f"[bench] burn-in {t} K={k} ({time.perf_counter() - t_w:.1f}s)", file=sys.stderr, flush=True)
+    burn_in = t
+    print(f"[bench] burn-in done after {burn_in} iterations, K={smp.num_clusters()} (unchanged since iteration "
+          f"{last_change})", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        t += 1
+        smp.step(t)
     torch.cuda.synchronize()
     dev.synchronize()
-    lab_ms, sub_ms, st_ms, host_ms, flops, rechk, srechk, evfrac = [], [], [], [], [], [], [], []
+    k_start = smp.num_clusters()
+    if k_start < 0.9 * cfg["K"]:
+        print(f"[bench] only {k_start} clusters after burn-in and warmup ({cfg['K']} true): not the steady-state "
+              "workload, no bench line", file=sys.stderr, flush=True)
+        sys.exit(3)
+    first = t + 1
+    lab_ms, sub_ms, st_ms, host_ms, flops, rechk, srechk, evfrac, ks = [], [], [], [], [], [], [], [], []
     t0 = time.perf_counter()
-    for t in range(args.warmup + 1, args.warmup + args.steps + 1):
+    for t in range(first, first + args.steps):
         kc = smp.num_clusters()
+        ks.append(kc)
         smp.step(t)
         a, b, c = dev.kernel_ms()
         lab_ms.append(a); sub_ms.append(b); st_ms.append(c); host_ms.append(smp.host_ms())
@@ -446,22 +482,21 @@ def main():
                 "bound": "mfma",
                 "kernel": "k_label_gauss_mx + k_label_gauss_list" if prec == dp.Precision.MIXED else "k_label_gauss",
                 "achieved": float(np.mean(flops)) / (avg_lab * 1e-3) / 1e12, "peak": peak, "unit": "TFLOP/s",
-                "traffic": TRAFFIC.get((args.config, args.precision, "label_sweep")),
+                "traffic": None,
                 "algorithmic": "2 K D^2 flops per point (dense whitening of K clusters)"}
         phase["sub_sweep"] = hbm("k_sub_gauss_mx + k_sub_gauss_list" if prec == dp.Precision.MIXED else
                                  "k_sub_gauss", N * (esz * D + 12.0), avg_sub,
                                  "D coordinates + index + label + sub-label write per point (gathered)")
-        phase["sub_sweep"]["traffic"] = TRAFFIC.get((args.config, args.precision, "sub_sweep"))
-        if pruned:
-            phase["label_sweep"]["traffic"] = TRAFFIC.get((args.config, args.precision, "label_sweep"))
         phase["statistics"] = {
             "bound": "mfma", "kernel": ("k_stats_gauss" if D <= 8 else "k_stats_gauss_wave" if D <= 64 else
                                         "k_stats_gauss_mfma") + " + k_stats_reduce",
             "achieved": N * (D * (D + 1.0) + D) / (avg_st * 1e-3) / 1e12, "peak": PEAK["fp64"], "unit": "TFLOP/s",
-            "traffic": TRAFFIC.get((args.config, args.precision, "statistics")),
+            "traffic": None,
             "algorithmic": "D (D + 1) + D flops per point (upper triangle of x x^T and the sum, FP64)"}
-    for r in phase.values():
+    k_timed = int(round(float(np.mean(ks))))
+    for name, r in phase.items():
         r["frac"] = r["achieved"] / r["peak"]
+        r["traffic"] = measured_traffic(args.config, args.precision, k_timed, name)
     dominant = max(("label_sweep", "sub_sweep", "statistics"),
                    key=lambda k: {"label_sweep": avg_lab, "sub_sweep": avg_sub, "statistics": avg_st}[k])
     roof = dict(phase[dominant], phase=dominant)
@@ -481,8 +516,8 @@ def main():
                                 f"alpha={cfg['alpha']}, Dirichlet prior {cfg['prior']}") if mult else
                                (f"{args.config}: Gaussian DPMM sampler iteration, N={N}, D={D}, "
                                 f"{cfg['K']} true clusters, alpha={cfg['alpha']}, NIW(k=1, m=0, v=D+3, psi=I)"),
-                   "N": N, "D": D, "clusters_timed": K_final, "precision": args.precision,
-                   "parallelism": "single GPU"},
+                   "N": N, "D": D, "clusters_timed": k_timed, "clusters_after": K_final,
+                   "burn_in_iterations": burn_in, "precision": args.precision, "parallelism": "single GPU"},
         "roofline": roof,
         "roofline_by_phase": phase,
         "breakdown_ms": {"label_sweep": avg_lab, "sub_sweep": avg_sub, "statistics": avg_st,
@@ -495,7 +530,7 @@ def main():
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline and flip check (FP64 oracle)", file=sys.stderr, flush=True)
         out["cpu_baseline"], out["oracle_check"] = cpu_baseline_and_check(
-            dev, smp, X, N, mult, args.seed + 1, args.warmup + args.steps + 1)
+            dev, smp, X, N, mult, args.seed + 1, first + args.steps)
     print(json.dumps(out), flush=True)
 
 

@@ -75,6 +75,8 @@ def load_library():
         "dpmm_create_device": [_P, i64, i, i, i, u64, _P],
         "dpmm_destroy": [_P],
         "dpmm_set_precision": [_P, i],
+        "dpmm_max_clusters": [_P, _P],
+        "dpmm_limit_clusters": [_P, i],
         "dpmm_sample_initial_labels": [_P, i],
         "dpmm_set_labels": [_P, _P, _P],
         "dpmm_get_labels": [_P, _P, _P],
@@ -122,6 +124,7 @@ def load_library():
         "dpmm_sampler_num_clusters": [_P],
         "dpmm_sampler_weights": [_P, _P, _P, i],
         "dpmm_sampler_host_ms": [_P, _P],
+        "dpmm_sampler_trace_sweep": [_P, _P, _P],
         "dpmm_sampler_destroy": [_P],
         "dpmm_run": [_P, _P, dbl, _P, dbl, _P, _P, _P, _P, _P, _P, i, _P],
     }
@@ -245,6 +248,16 @@ class DeviceSweep:
 
     def set_precision(self, p):
         _check(load_library().dpmm_set_precision(self._ctx, int(p)))
+
+    def max_clusters(self):
+        """Largest cluster count the kernels take (lowered by limit_clusters)."""
+        v = ctypes.c_int()
+        _check(load_library().dpmm_max_clusters(self._ctx, ctypes.byref(v)))
+        return v.value
+
+    def limit_clusters(self, kmax):
+        """Lower the cluster cap at which the sampler stops splitting (0: the kernels' own)."""
+        _check(load_library().dpmm_limit_clusters(self._ctx, int(kmax)))
 
     def sample_initial_labels(self, init_clusters):
         _check(load_library().dpmm_sample_initial_labels(self._ctx, int(init_clusters)))
@@ -463,6 +476,19 @@ class Sampler:
         n = np.zeros(K, np.int64)
         _check(load_library().dpmm_sampler_weights(self._s, _ptr(w), _ptr(n), K))
         return w, n
+
+    def trace_sweeps(self, enable=True):
+        """Keep the labels and sub-labels each step's sweeps drew (before resets, splits, merges)."""
+        if enable:
+            self._trace = (np.empty(self.dev.N, np.int32), np.empty(self.dev.N, np.int32))
+            _check(load_library().dpmm_sampler_trace_sweep(self._s, _ptr(self._trace[0]), _ptr(self._trace[1])))
+        else:
+            _check(load_library().dpmm_sampler_trace_sweep(self._s, None, None))
+            self._trace = None
+
+    def sweep_trace(self):
+        """(labels, sub-labels) of the last step's sweeps; trace_sweeps must be on."""
+        return self._trace[0].copy(), self._trace[1].copy()
 
     def host_ms(self):
         v = ctypes.c_double()

@@ -66,6 +66,9 @@ int fail(int code, const std::string& msg) {
 
 constexpr int kBlockPts = 4096;    // points per partition block (256 threads x 16)
 constexpr int kMaxK = 2048;        // clusters per context
+// clusters of a multinomial context: the FP32 label sweep keeps 32 documents' counts (65 floats
+// apart) and K x 33 floats of cluster terms in 64 KiB of LDS
+constexpr int kMaxKMult = (16384 - 32 * 65) / 33;
 }  // namespace
 
 // ============================================================================ kernels ======
@@ -736,6 +739,7 @@ struct dpmm_ctx {
     int32_t* dSubs = nullptr;
     // parameters (compute type: double for FP64 precision, float otherwise)
     int K = 0, Kcap = 0, stride = 0;
+    int kcap_limit = 0;   // dpmm_limit_clusters (0: the kernels' own cap)
     bool params_f64 = false;
     void* dP = nullptr;
     void* dSP = nullptr;
@@ -1393,7 +1397,7 @@ int run_label_mult_mixed(dpmm_ctx* c, uint32_t iter, int fin) {
 }
 
 int launch_label_mult(dpmm_ctx* c, uint32_t iter, int fin) {
-    if ((size_t)(32 * 65 + c->K * 33) * sizeof(float) > 65536) return fail(DPMM_EINVAL, "multinomial K > 433");
+    if (c->K > kMaxKMult) return fail(DPMM_EINVAL, "multinomial K > 433");
     const int kpt = (c->K + 3) / 4;
     if (kpt <= 4) launch_label_mult_k<4>(c, iter, fin);
     else if (kpt <= 8) launch_label_mult_k<8>(c, iter, fin);
@@ -1447,6 +1451,19 @@ int dpmm_destroy(dpmm_ctx* c) {
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
+    return DPMM_OK;
+}
+
+int dpmm_max_clusters(dpmm_ctx* c, int* kmax) {
+    if (!c || !kmax) return fail(DPMM_EINVAL, "bad args");
+    const int hw = c->prior == DPMM_PRIOR_GAUSSIAN ? kMaxK : kMaxKMult;
+    *kmax = c->kcap_limit > 0 ? std::min(c->kcap_limit, hw) : hw;
+    return DPMM_OK;
+}
+
+int dpmm_limit_clusters(dpmm_ctx* c, int kmax) {
+    if (!c || kmax < 0) return fail(DPMM_EINVAL, "bad args");
+    c->kcap_limit = kmax;
     return DPMM_OK;
 }
 
@@ -1842,11 +1859,20 @@ int dpmm_sweep_iteration(dpmm_ctx* c, uint32_t iter, int final_iter, int64_t* Ns
     t_capturing = false;
     hipGraph_t g = nullptr;
     const hipError_t ec = hipStreamEndCapture(c->stream, &g);
-    if (rc || ec != hipSuccess) {
-        if (g) (void)hipGraphDestroy(g);
-        c->partition_valid = false;   // nothing of the recorded work ran
+    // Nothing of the recorded work runs when capture or instantiation fails: every flag that the
+    // recording set on the promise of that work (partition, statistics, pruning bounds, the pruning
+    // counters' copy) is cleared again.
+    auto drop_recorded = [c]() {
+        c->partition_valid = false;
         c->stats_cached = false;
         c->partials_valid = false;
+        c->prune_valid = false;
+        c->evals_copy_due = false;
+        c->last_pruned_K = 0;
+    };
+    if (rc || ec != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        drop_recorded();
         return rc ? rc : fail(DPMM_EHIP, std::string("graph capture: ") + hipGetErrorString(ec));
     }
     bool updated = false;
@@ -1866,6 +1892,7 @@ int dpmm_sweep_iteration(dpmm_ctx* c, uint32_t iter, int final_iter, int64_t* Ns
     (void)hipGraphDestroy(g);
     if (e != hipSuccess) {
         c->gexec = nullptr;
+        drop_recorded();
         return fail(DPMM_EHIP, std::string("graph instantiate: ") + hipGetErrorString(e));
     }
     if (updated) ++c->graph_updates;

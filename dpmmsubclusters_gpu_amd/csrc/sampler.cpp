@@ -104,6 +104,7 @@ struct SamplerImpl {
     int64_t N = 0;
     int D = 0;
     int prior = 0;
+    int kcap = 0;   // the device kernels' largest cluster count (dpmm_max_clusters)
     std::vector<Cluster> cl;
     std::vector<double> weights;
     double host_ms = 0;
@@ -124,6 +125,8 @@ struct SamplerImpl {
     std::vector<double> log_h, log_u;
     int64_t n_exact = 0, n_skipped = 0, n_missed = 0, n_full = 0;
     bool counts_fresh = false;   // Nsub holds the counts of the current labels (from the sweep call)
+    int32_t* trace_labels = nullptr;   // dpmm_sampler_trace_sweep: host copies of each step's sweep output
+    int32_t* trace_subs = nullptr;
 
     void reset_hist(Cluster& c) const { c.hist.assign(o.burnout + 5, std::make_pair(false, 0.0)); }
 
@@ -243,7 +246,11 @@ struct SamplerImpl {
         if (rc) return rc;
         if (scr) {
             pair_logit.resize(np);
-            rc = dpmm_screener_forward(scr, (int)np, pair_feat.data(), pair_logit.data());
+            // the forward pass takes at most 2^20 rows per call
+            const size_t chunk = (size_t)1 << 20;
+            for (size_t p0 = 0; p0 < np && !rc; p0 += chunk)
+                rc = dpmm_screener_forward(scr, (int)std::min(chunk, np - p0), pair_feat.data() + p0 * 16,
+                                           pair_logit.data() + p0);
         }
         return rc;
     }
@@ -316,7 +323,9 @@ struct SamplerImpl {
 
     int step(int t) {
         const bool fin = t >= o.iters - o.argmax_sample_stop;
-        const bool no_more_splits = t >= o.iters - o.split_stop || (double)cl.size() >= o.max_clusters;
+        // run_model's flags (dp_parallel_sampling.cpp:143-150); the device's cluster cap ends splitting too
+        const bool no_more_splits =
+            t >= o.iters - o.split_stop || (double)cl.size() >= o.max_clusters || (int)cl.size() >= kcap;
         const uint32_t it = (uint32_t)t;
         int rc;
         host_ms = 0;
@@ -329,6 +338,7 @@ struct SamplerImpl {
         rc = dpmm_sweep_iteration(ctx, it, (fin || o.hard_clustering) ? 1 : 0, Nsub.data());
         if (rc) return rc;
         counts_fresh = true;
+        if ((trace_labels || trace_subs) && (rc = dpmm_get_labels(ctx, trace_labels, trace_subs))) return rc;
         // one log-marginal launch: every part, and every pair of merge candidates (a cluster that is
         // reset or split below leaves the candidates, and the others keep these statistics)
         rc = update_stats({}, !no_more_splits, it);
@@ -371,6 +381,12 @@ struct SamplerImpl {
             std::vector<int32_t> nl(2 * K), act(K, 0);
             for (int q = 0; q < 2 * K; ++q) nl[q] = q / 2;
             int nsplit = 0;
+            // accepted splits beyond the device's cluster cap are dropped, the highest indices first
+            int room = kcap - K;
+            for (int k = 0; k < K; ++k) {
+                if (acc[k] && room <= 0) acc[k] = 0;
+                else if (acc[k]) --room;
+            }
             for (int k = 0; k < K; ++k) {
                 if (!acc[k]) continue;
                 const int nk = (int)cl.size();
@@ -524,6 +540,7 @@ int dpmm_sampler_create(dpmm_ctx* ctx, const dpmm_options* opt, double k0, const
     s.N = N;
     s.D = D;
     s.prior = prior;
+    if ((rc = dpmm_max_clusters(ctx, &s.kcap))) { delete S; return rc; }
     rc = s.init();
     if (rc) { delete S; return rc; }
     *out = S;
@@ -561,6 +578,13 @@ int dpmm_sampler_log_posterior(dpmm_sampler* S, double* lp) {
         v += c.c.lml + std::log(a) + std::lgamma((double)c.c.N);
     }
     *lp = v;
+    return DPMM_OK;
+}
+
+int dpmm_sampler_trace_sweep(dpmm_sampler* S, int32_t* labels, int32_t* sub_labels) {
+    if (!S) return DPMM_EINVAL;
+    S->s.trace_labels = labels;
+    S->s.trace_subs = sub_labels;
     return DPMM_OK;
 }
 

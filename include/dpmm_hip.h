@@ -45,6 +45,13 @@ int dpmm_create_device(const void* dX, int64_t N, int D, int dtype, int prior, u
 /* cudaKernel::release (global_params.cpp:108-114). */
 int dpmm_destroy(dpmm_ctx* ctx);
 int dpmm_set_precision(dpmm_ctx* ctx, int precision);
+/* Largest cluster count the context's kernels take (2048 Gaussian, 433 multinomial).  The
+ * reference has no such limit; the host driver stops splitting at it (as run_model stops at
+ * max_clusters, dp_parallel_sampling.cpp:147) and accepts splits only up to it. */
+int dpmm_max_clusters(dpmm_ctx* ctx, int* kmax);
+/* Lowers the cap dpmm_max_clusters reports (0 restores the kernels' own), e.g. to bound the
+ * memory of a run; no counterpart in the reference. */
+int dpmm_limit_clusters(dpmm_ctx* ctx, int kmax);
 
 /* cudaKernel::sample_labels + sample_sub_labels (cudaKernel.cu:1093-1111): uniform labels
  * in 0..init_clusters-1 and uniform sub-labels. */
@@ -258,6 +265,11 @@ int dpmm_sampler_step(dpmm_sampler* s, int iter);
 int dpmm_sampler_num_clusters(dpmm_sampler* s);
 /* Mixture weights of the last step (K values) and per-cluster counts. */
 int dpmm_sampler_weights(dpmm_sampler* s, double* weights, int64_t* counts, int max_k);
+/* Test hook: from the next step on, copy the labels and sub-labels the step's two sweeps drew
+ * (before resets, splits and merges relabel them) into the caller's host arrays of N entries
+ * (either may be NULL; both NULL turns the copies off).  No counterpart in the reference; it lets a
+ * test follow the sampler iteration by iteration against an FP64 restatement. */
+int dpmm_sampler_trace_sweep(dpmm_sampler* s, int32_t* labels, int32_t* sub_labels);
 /* Host time spent in the last step's FP64 work (draws, posteriors, Hastings), ms. */
 int dpmm_sampler_host_ms(dpmm_sampler* s, double* host_ms);
 /* dp_parallel_sampling_class::calculate_posterior (dp_parallel_sampling.cpp:218-236) after the last

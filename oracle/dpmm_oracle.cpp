@@ -242,6 +242,11 @@ struct Model {
     std::vector<Cluster> cl;
     std::vector<double> weights;
     std::vector<int32_t> labels, subs;
+    /* trace of the last iteration's sweeps (orc_model_trace): labels and sub-labels as the two
+     * sweeps drew them, before resets, splits and merges, with the sweeps' relative margins */
+    bool trace = false;
+    std::vector<int32_t> tr_labels, tr_subs;
+    std::vector<double> tr_margin, tr_smargin;
 };
 
 Stats empty_stats(int prior, int D) {
@@ -611,13 +616,16 @@ void group_step(Model& M, uint32_t t, bool no_more_splits, bool final_iter) {
     int K = (int)M.cl.size();
     std::vector<double> mu, lam, ld, lt;
     gather_params(M, mu, lam, ld, lt, 0);
+    if (M.trace) { M.tr_margin.resize(M.N); M.tr_smargin.resize(M.N); }
     label_sweep(M.prior, M.X, M.N, D, K, mu.data(), lam.data(), ld.data(), lt.data(), M.weights.data(),
-                M.mult_use_weight, M.seed, t, final_iter ? 1 : 0, M.labels.data(), nullptr, nullptr);
+                M.mult_use_weight, M.seed, t, final_iter ? 1 : 0, M.labels.data(),
+                M.trace ? M.tr_margin.data() : nullptr, nullptr);
     gather_params(M, mu, lam, ld, lt, 1);
     std::vector<double> lrw(2 * K);
     for (int k = 0; k < K; ++k) { lrw[2 * k] = M.cl[k].lrw[0]; lrw[2 * k + 1] = M.cl[k].lrw[1]; }
     sublabel_sweep(M.prior, M.X, M.N, D, K, M.labels.data(), mu.data(), lam.data(), ld.data(), lt.data(),
-                   lrw.data(), M.seed, t, M.subs.data(), nullptr);
+                   lrw.data(), M.seed, t, M.subs.data(), M.trace ? M.tr_smargin.data() : nullptr);
+    if (M.trace) { M.tr_labels = M.labels; M.tr_subs = M.subs; }
     std::vector<int> all(K);
     for (int k = 0; k < K; ++k) all[k] = k;
     update_stats(M, all);
@@ -882,10 +890,12 @@ double orc_nmi(const int32_t* a, const int32_t* b, int64_t N) {
     return 2 * I / (ha + hb);
 }
 
-int orc_run(const orc_run_opts* o, const double* X, int64_t N, int D, double k0, const double* m0, double v0,
-            const double* psi0, const double* alpha0, int32_t* labels, int32_t* k_hist, double* weights_out,
-            int max_k_out, double* iter_seconds) {
-    Model M;
+}  // extern "C"
+
+namespace {
+/* init_model + init_first_clusters (dp_parallel_sampling.cpp:48-87) into M; X is not copied. */
+void model_init(Model& M, const orc_run_opts* o, const double* X, int64_t N, int D, double k0, const double* m0,
+                double v0, const double* psi0, const double* alpha0) {
     M.prior = o->prior; M.D = D; M.N = N; M.X = X; M.alpha = o->alpha; M.seed = o->seed;
     M.burnout = o->burnout; M.full_mu_cov = o->full_mu_cov; M.mult_use_weight = o->mult_use_weight;
     if (M.prior == 0) {
@@ -916,11 +926,26 @@ int orc_run(const orc_run_opts* o, const double* X, int64_t N, int D, double k0,
     for (int k = 0; k < K0; ++k) all[k] = k;
     update_stats(M, all);
     sample_clusters(M, 0);
+}
+
+/* One iteration of run_model (dp_parallel_sampling.cpp:139-154) with its final / no_more_splits flags. */
+void model_step(Model& M, const orc_run_opts* o, int t) {
+    const bool fin = t >= o->iters - o->argmax_sample_stop;
+    const bool nms = t >= o->iters - o->split_stop || (double)M.cl.size() >= o->max_clusters;
+    group_step(M, (uint32_t)t, nms, fin);
+}
+}  // namespace
+
+extern "C" {
+
+int orc_run(const orc_run_opts* o, const double* X, int64_t N, int D, double k0, const double* m0, double v0,
+            const double* psi0, const double* alpha0, int32_t* labels, int32_t* k_hist, double* weights_out,
+            int max_k_out, double* iter_seconds) {
+    Model M;
+    model_init(M, o, X, N, D, k0, m0, v0, psi0, alpha0);
     for (int t = 1; t <= o->iters; ++t) {
         const auto t0 = std::chrono::steady_clock::now();
-        const bool fin = t >= o->iters - o->argmax_sample_stop;
-        const bool nms = t >= o->iters - o->split_stop || (double)M.cl.size() >= o->max_clusters;
-        group_step(M, (uint32_t)t, nms, fin);
+        model_step(M, o, t);
         const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         if (iter_seconds) iter_seconds[t - 1] = dt;
         if (k_hist) k_hist[t - 1] = (int32_t)M.cl.size();
@@ -930,6 +955,52 @@ int orc_run(const orc_run_opts* o, const double* X, int64_t N, int D, double k0,
     if (weights_out)
         for (int k = 0; k < (int)M.weights.size() && k < max_k_out; ++k) weights_out[k] = M.weights[k];
     return (int)M.cl.size();
+}
+
+struct orc_model {
+    Model M;
+    orc_run_opts o;
+    std::vector<double> X;
+};
+
+orc_model* orc_model_create(const orc_run_opts* o, const double* X, int64_t N, int D, double k0, const double* m0,
+                            double v0, const double* psi0, const double* alpha0) {
+    orc_model* m = new orc_model();
+    m->o = *o;
+    m->X.assign(X, X + (size_t)N * D);
+    model_init(m->M, o, m->X.data(), N, D, k0, m0, v0, psi0, alpha0);
+    return m;
+}
+
+void orc_model_destroy(orc_model* m) { delete m; }
+
+void orc_model_trace(orc_model* m, int enable) { m->M.trace = enable != 0; }
+
+int orc_model_step(orc_model* m, int t) {
+    model_step(m->M, &m->o, t);
+    return (int)m->M.cl.size();
+}
+
+int orc_model_num_clusters(const orc_model* m) { return (int)m->M.cl.size(); }
+
+void orc_model_state(const orc_model* m, int32_t* labels, int32_t* subs, int64_t* counts, double* weights) {
+    const Model& M = m->M;
+    if (labels) std::copy(M.labels.begin(), M.labels.end(), labels);
+    if (subs) std::copy(M.subs.begin(), M.subs.end(), subs);
+    for (size_t k = 0; k < M.cl.size(); ++k) {
+        if (counts) counts[k] = M.cl[k].c.st.N;
+        if (weights) weights[k] = k < M.weights.size() ? M.weights[k] : 0.0;
+    }
+}
+
+int orc_model_sweep_trace(const orc_model* m, int32_t* labels, int32_t* subs, double* margin, double* smargin) {
+    const Model& M = m->M;
+    if (!M.trace || M.tr_labels.size() != (size_t)M.N) return -1;
+    std::copy(M.tr_labels.begin(), M.tr_labels.end(), labels);
+    std::copy(M.tr_subs.begin(), M.tr_subs.end(), subs);
+    std::copy(M.tr_margin.begin(), M.tr_margin.end(), margin);
+    std::copy(M.tr_smargin.begin(), M.tr_smargin.end(), smargin);
+    return 0;
 }
 
 }  // extern "C"

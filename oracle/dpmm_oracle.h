@@ -106,6 +106,22 @@ int orc_run(const orc_run_opts* opts, const double* X, int64_t N, int D, double 
             double v0, const double* psi0, const double* alpha0, int32_t* labels, int32_t* k_hist,
             double* weights_out, int max_k_out, double* iter_seconds);
 
+/* The same sampler one iteration at a time: orc_model_create runs init_model + init_first_clusters
+ * (dp_parallel_sampling.cpp:48-87) on a copy of X, orc_model_step(m, t) one run_model iteration t
+ * (139-154, group_step local_clusters_actions.cpp:460-490) and returns the cluster count.  With
+ * orc_model_trace on, each step keeps the labels and sub-labels its two sweeps drew (before resets,
+ * splits and merges) and their relative margins, read by orc_model_sweep_trace. */
+typedef struct orc_model orc_model;
+orc_model* orc_model_create(const orc_run_opts* opts, const double* X, int64_t N, int D, double k0, const double* m0,
+                            double v0, const double* psi0, const double* alpha0);
+void orc_model_destroy(orc_model* m);
+void orc_model_trace(orc_model* m, int enable);
+int orc_model_step(orc_model* m, int t);
+int orc_model_num_clusters(const orc_model* m);
+/* labels / sub-labels (N), per-cluster counts and mixture weights (K); any may be NULL. */
+void orc_model_state(const orc_model* m, int32_t* labels, int32_t* subs, int64_t* counts, double* weights);
+int orc_model_sweep_trace(const orc_model* m, int32_t* labels, int32_t* subs, double* margin, double* smargin);
+
 /* Merge-pair screener (mlp_oracle.cpp; no reference counterpart): MLP 16 -> 128 -> 128 -> 1 with
  * parameters [W1 (128x16) | b1 | W2 (128x128) | b2 | w3 | b3], ReLU hidden layers, logit output;
  * mean weighted binary cross-entropy with logits and its gradient; pair features from cluster

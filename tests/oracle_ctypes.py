@@ -47,6 +47,13 @@ def lib():
         L.orc_nmi.restype = c_dbl
         L.orc_run.restype = c_int
         L.orc_mlp_loss_grad.restype = c_dbl
+        L.orc_model_create.restype = P
+        L.orc_model_destroy.argtypes = [P]
+        L.orc_model_trace.argtypes = [P, c_int]
+        L.orc_model_step.argtypes = [P, c_int]
+        L.orc_model_num_clusters.argtypes = [P]
+        L.orc_model_state.argtypes = [P, P, P, P, P]
+        L.orc_model_sweep_trace.argtypes = [P, P, P, P, P]
         _lib = L
     return _lib
 
@@ -219,6 +226,61 @@ def run(X, prior=0, alpha=1.0, iters=100, init_clusters=1, burnout=15, seed=0, k
         K = lib().orc_run(ctypes.byref(o), _p(X), c_i64(N), c_int(D), c_dbl(0), None, c_dbl(0), None,
                           _p(f64(alpha0)), _p(labels), _p(khist), _p(weights), c_int(4096), _p(secs))
     return {"K": K, "labels": labels, "k_hist": khist, "weights": weights[:max(K, 0)], "iter_seconds": secs}
+
+
+class Model:
+    """The oracle sampler one iteration at a time (orc_model_*): same options and defaults as run()."""
+
+    def __init__(self, X, prior=0, alpha=1.0, iters=100, init_clusters=1, burnout=15, seed=0, k0=1.0, m0=None,
+                 v0=None, psi0=None, alpha0=None, max_clusters=float("inf"), full_mu_cov=False,
+                 mult_use_weight=False, trace=False):
+        X = f64(X)
+        self.N, D = X.shape
+        o = RunOpts(prior=prior, alpha=alpha, iters=iters, init_clusters=init_clusters, burnout=burnout,
+                    argmax_sample_stop=5, split_stop=5, max_clusters=max_clusters, seed=seed,
+                    full_mu_cov=int(full_mu_cov), mult_use_weight=int(mult_use_weight), verbose=0)
+        L = lib()
+        if prior == 0:
+            m0 = np.zeros(D) if m0 is None else m0
+            v0 = float(D + 3) if v0 is None else v0
+            psi0 = np.eye(D) if psi0 is None else psi0
+            self._m = L.orc_model_create(ctypes.byref(o), _p(X), c_i64(self.N), c_int(D), c_dbl(k0), _p(f64(m0)),
+                                         c_dbl(v0), _p(f64(psi0)), None)
+        else:
+            alpha0 = np.ones(D) if alpha0 is None else alpha0
+            self._m = L.orc_model_create(ctypes.byref(o), _p(X), c_i64(self.N), c_int(D), c_dbl(0), None, c_dbl(0),
+                                         None, _p(f64(alpha0)))
+        L.orc_model_trace(self._m, c_int(int(trace)))
+
+    def step(self, t):
+        return lib().orc_model_step(self._m, c_int(t))
+
+    def num_clusters(self):
+        return lib().orc_model_num_clusters(self._m)
+
+    def state(self):
+        """(labels, sub-labels, per-cluster counts, mixture weights)."""
+        K = self.num_clusters()
+        lab, sub = np.empty(self.N, np.int32), np.empty(self.N, np.int32)
+        cnt, w = np.empty(K, np.int64), np.empty(K)
+        lib().orc_model_state(self._m, _p(lab), _p(sub), _p(cnt), _p(w))
+        return lab, sub, cnt, w
+
+    def sweep_trace(self):
+        """(labels, sub-labels, label margin, sub-label margin) of the last step's sweeps."""
+        lab, sub = np.empty(self.N, np.int32), np.empty(self.N, np.int32)
+        m, sm = np.empty(self.N), np.empty(self.N)
+        if lib().orc_model_sweep_trace(self._m, _p(lab), _p(sub), _p(m), _p(sm)) != 0:
+            raise RuntimeError("trace is off")
+        return lab, sub, m, sm
+
+    def close(self):
+        if self._m:
+            lib().orc_model_destroy(self._m)
+            self._m = None
+
+    def __del__(self):
+        self.close()
 
 
 def mlp_param_count():

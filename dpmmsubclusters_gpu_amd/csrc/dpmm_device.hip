@@ -773,7 +773,7 @@ struct dpmm_ctx {
     bool spb_valid = false;
     int32_t* dRecheck = nullptr;
     int* dRecheckCount = nullptr;
-    float tau = 1.0f / 32.0f;
+    float tau = 1.0f;   // scale of the mixed label screens' error bound (1: the derived bound)
     // exact pruning of the mixed Gaussian label sweep: bounds of the current parameters
     int prune_mode = 1;           // 0 off, 1 adaptive, 2 always
     bool prune_valid = false;
@@ -2014,7 +2014,7 @@ int dpmm_last_kernel_ms(dpmm_ctx* c, float* a, float* b, float* s) {
 }
 
 int dpmm_set_mixed_tolerance(dpmm_ctx* c, float tau) {
-    if (!c || !(tau >= 0.0f) || !(tau < 0.5f)) return fail(DPMM_EINVAL, "tolerance must be in [0, 0.5)");
+    if (!c || !(tau >= 0.0f) || !(tau <= 16.0f)) return fail(DPMM_EINVAL, "bound scale must be in [0, 16]");
     c->tau = tau;
     return DPMM_OK;
 }

@@ -9,9 +9,23 @@
 // 16 kc + 8 (L / 32) + e; the tile's 32 output rows of that point are split between lanes L and
 // L ^ 32, so q = sum of squares is one in-lane sum plus one cross-half add.
 //
-// Each point keeps its best screened score s_b (with error bound e_b = tau q_b) and the largest
-// s_k + e_k of every other cluster.  The label is certain when s_b - e_b exceeds that maximum;
-// other points are appended to a recheck list that the FP32 sweep processes exactly.
+// Error bound against the FP32 sweep (k_label_gauss, which forms y_j = sum_l W_jl d_l by FMAs in
+// FP32 and q = sum_j y_j^2 the same way, with the same FP32 W, mu and c): BF16 rounding of W and of
+// d = x - mu is relative 2^-9 each, so every product W_jl d_l is off by at most (2^-8 + 2^-18)
+// |W_jl| |d_l|, and the screen's and the sweep's FP32 sums add at most (2 DP + 16) 2^-24 of
+// a_j = sum_l |W_jl| |d_l|.  A second MFMA per tile, on the magnitudes |bf16(W)| |bf16(d)|, gives a_j
+// (inflated by 2^-6 for its own rounding), so |y_screen,j - y_fp32,j| <= eps a_j and, with
+// A = sum_j a_j^2 and sum_j |y_j| a_j <= sqrt(q A) (Cauchy-Schwarz),
+//   |q_screen - q_fp32| <= sum_j (2 |y_j| + eps a_j) eps a_j + (2 DP + 4) 2^-24 q
+//                       <= 2 eps sqrt(q A) + eps^2 A + (2 DP + 4) 2^-24 q
+// (the last term: both sums of squares).  Half of it, the rounding of c - q / 2 in both and the
+// screen's fast Gumbel noise (kNoiseSlack) bound |s_screen - s_fp32|; this is e_k.  The bound scales
+// with |W| |d|, not with |W d|, so it holds whatever the conditioning of the covariance.
+//
+// Each point keeps its best screened score s_b (with error bound e_b) and the largest s_k + e_k of
+// every other cluster.  The label is the FP32 sweep's when s_b - e_b exceeds that maximum; other
+// points are appended to a recheck list that the FP32 sweep processes exactly.  `tau` scales the
+// bound: 1 is the derived bound; 0 keeps every screened label (a diagnostic of the raw screen).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -89,6 +103,22 @@ __host__ __device__ constexpr int mx_tile_index(int DP, int j, int l) {
 // Screen's Gumbel noise: hardware log2, and -log(u) = v + v^2 / 2 for v = 1 - u < 2^-10.  Its
 // absolute error stays far below kNoiseSlack, which the error bound adds to every cluster.
 constexpr float kNoiseSlack = 1e-3f;
+
+// eps of the label screen's per-row bound |y_screen,j - y_fp32,j| <= eps a_j (see the header).
+template <int DP> __device__ __forceinline__ constexpr float label_screen_eps() {
+    return (0x1p-8f + 0x1p-18f + (2.0f * (float)DP + 16.0f) * 0x1p-24f) * (1.0f + 0x1p-6f);
+}
+// e_k from q = sum y_j^2, A = sum a_j^2 and the score constant c (sqrt rounding covered by 2^-10).
+template <int DP> __device__ __forceinline__ float label_screen_err(float tau, float q, float A, float c) {
+    constexpr float eps = label_screen_eps<DP>();
+    const float dq = (eps * (sqrtf(q * A) + 0.5f * eps * A) + ((float)DP + 2.0f) * 0x1p-24f * q) * (1.0f + 0x1p-10f);
+    return tau * (dq + 0x1p-22f * (fabsf(c) + 0.5f * q)) + kNoiseSlack;
+}
+// |v| of eight packed BF16 values
+__device__ __forceinline__ bf16x8 bf16x8_abs(bf16x8 v) {
+    typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+    return __builtin_bit_cast(bf16x8, __builtin_bit_cast(u32x4, v) & 0x7fff7fffu);
+}
 __device__ __forceinline__ float gumbel_screen(uint32_t w) {
     const float v = ((float)(0xFFFFFFFFu - w) + 0.5f) * 0x1p-32f;
     float e;
@@ -136,7 +166,7 @@ k_label_gauss_mx(const float* __restrict__ X, int64_t N, int D, int K, const uns
         const bf16x8* tiles = reinterpret_cast<const bf16x8*>(slot);
         const float* mu = reinterpret_cast<const float*>(slot + mx_tiles_bytes(DP));
         const float c = mu[DP];
-        float qp[NB];
+        float qp[NB], ep[NB];
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) {
             bf16x8 Bv[KC];
@@ -144,25 +174,36 @@ k_label_gauss_mx(const float* __restrict__ X, int64_t N, int D, int K, const uns
             for (int kc = 0; kc < KC; ++kc)
 #pragma unroll
                 for (int e = 0; e < 8; ++e) Bv[kc][e] = (__bf16)(x[nb][kc][e] - mu[16 * kc + 8 * h + e]);
-            float q = 0.f;
+            float q = 0.f, ee = 0.f;
 #pragma unroll
             for (int rb = 0; rb < RB; ++rb) {
-                f32x16 acc = {};
+                f32x16 acc = {}, aa = {};
 #pragma unroll
-                for (int kc = 2 * rb; kc < KC; ++kc)
-                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(tiles[(rb * KC + kc) * 64 + lane], Bv[kc], acc, 0,
-                                                                  0, 0);
+                for (int kc = 2 * rb; kc < KC; ++kc) {
+                    const bf16x8 wt = tiles[(rb * KC + kc) * 64 + lane];
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wt, Bv[kc], acc, 0, 0, 0);
+                    aa = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bf16x8_abs(wt), bf16x8_abs(Bv[kc]), aa, 0, 0, 0);
+                }
 #pragma unroll
-                for (int v = 0; v < 16; ++v) q = fmaf(acc[v], acc[v], q);
+                for (int v = 0; v < 16; ++v) {
+                    q = fmaf(acc[v], acc[v], q);
+                    ee = fmaf(aa[v], aa[v], ee);
+                }
             }
             qp[nb] = q;   // this half's rows of block nb
+            ep[nb] = ee;
         }
         // half h keeps block h: it sends its rows of the other block and receives the other half's rows of its own
-        float q;
-        if constexpr (NB == 2) q = (h ? qp[1] : qp[0]) + __shfl_xor(h ? qp[0] : qp[1], 32);
-        else q = qp[0] + __shfl_xor(qp[0], 32);
+        float q, ee;
+        if constexpr (NB == 2) {
+            q = (h ? qp[1] : qp[0]) + __shfl_xor(h ? qp[0] : qp[1], 32);
+            ee = (h ? ep[1] : ep[0]) + __shfl_xor(h ? ep[0] : ep[1], 32);
+        } else {
+            q = qp[0] + __shfl_xor(qp[0], 32);
+            ee = ep[0] + __shfl_xor(ep[0], 32);
+        }
         float s = c - 0.5f * q;
-        const float err = tau * q + kNoiseSlack;
+        const float err = label_screen_err<DP>(tau, q, ee, c);
         if (!final_iter) {
             if ((k & 3) == 0) w = philox4x32_10((uint32_t)own, (uint32_t)(k >> 2), iter, TAG_LABEL, key0, key1);
             s += gumbel_screen(w.v[0]);
@@ -399,8 +440,8 @@ k_label_gauss_mxp(const float* __restrict__ X, int D, int K, int CH, const int32
         float xh[kHoldX ? KC : 1][8];
         if constexpr (kHoldX) load_x((gfloat*)X, xh);
         // screened q of the cluster whose slot is `slot` (the anchor's in LDS, or one in global
-        // memory) for this lane's point, and sum (x - mu_k)^2
-        auto score = [&](const unsigned char* slot, float& d2) -> float {
+        // memory) for this lane's point, sum (x - mu_k)^2 and the bound's sum ee
+        auto score = [&](const unsigned char* slot, float& d2, float& eo) -> float {
             float xr[kHoldX ? 1 : KC][8];
             if constexpr (!kHoldX) {
                 gfloat* Xs = (gfloat*)X;
@@ -425,18 +466,28 @@ k_label_gauss_mxp(const float* __restrict__ X, int D, int K, int CH, const int32
                     dd = fmaf(d, d, dd);
                 }
             }
-            float q = 0.f;
+            float q = 0.f, ee = 0.f;
 #pragma unroll
             for (int rb = 0; rb < RB; ++rb) {
-                f32x16 acc = {};
+                f32x16 acc = {}, aa = {};
 #pragma unroll
-                for (int kc = 2 * rb; kc < KC; ++kc)
-                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(tiles[(rb * KC + kc) * 64 + lane], Bv[kc], acc, 0, 0,
-                                                                  0);
+                for (int kc = 2 * rb; kc < KC; ++kc) {
+                    const bf16x8 wt = tiles[(rb * KC + kc) * 64 + lane];
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wt, Bv[kc], acc, 0, 0, 0);
+                    bf16x8 bab = bf16x8_abs(Bv[kc]);
+                    // at DP = 128 the magnitudes are formed at each use: kept for all row blocks they
+                    // would not fit in registers with the operands
+                    if constexpr (DP > 96) asm volatile("" : "+v"(bab));
+                    aa = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bf16x8_abs(wt), bab, aa, 0, 0, 0);
+                }
 #pragma unroll
-                for (int v = 0; v < 16; ++v) q = fmaf(acc[v], acc[v], q);
+                for (int v = 0; v < 16; ++v) {
+                    q = fmaf(acc[v], acc[v], q);
+                    ee = fmaf(aa[v], aa[v], ee);
+                }
             }
             d2 = dd + __shfl_xor(dd, 32);
+            eo = ee + __shfl_xor(ee, 32);
             return q + __shfl_xor(q, 32);
         };
         auto noisy = [&](const unsigned char* slot, int k, float q) -> float {
@@ -450,9 +501,10 @@ k_label_gauss_mxp(const float* __restrict__ X, int D, int K, int CH, const int32
         };
         // the anchor, then the wave bounds: r >= ||x - mu_a|| over the points (the FP32 rounding of
         // the distance covered) and T = the smallest screened lower bound s_a - e_a
-        float d2 = 0.f;
-        const float qa = score(sa, d2);
-        float sb = noisy(sa, a, qa), eb = tau * qa + kNoiseSlack, U = neg_inf<float>();
+        float d2 = 0.f, ea = 0.f;
+        const float qa = score(sa, d2, ea);
+        const float* ca = reinterpret_cast<const float*>(sa + mx_tiles_bytes(DP)) + DP;
+        float sb = noisy(sa, a, qa), eb = label_screen_err<DP>(tau, qa, ea, *ca), U = neg_inf<float>();
         int bi = a;
         float rr = own_valid ? sqrtf(d2) : 0.f;
         float tt = own_valid ? sb - eb : __builtin_huge_valf();
@@ -482,11 +534,12 @@ k_label_gauss_mxp(const float* __restrict__ X, int D, int K, int CH, const int32
                 const int k = k0 + __ffsll((unsigned long long)todo) - 1;
                 todo &= todo - 1;
                 ++ne;
-                float unused;
+                float unused, ek;
                 const unsigned char* slot = PB + (size_t)k * SLOT;
-                const float qk = score(slot, unused);
+                const float qk = score(slot, unused, ek);
                 const float s = noisy(slot, k, qk);
-                const float err = tau * qk + kNoiseSlack;
+                const float err =
+                    label_screen_err<DP>(tau, qk, ek, reinterpret_cast<const float*>(slot + mx_tiles_bytes(DP))[DP]);
                 if (s > sb) {
                     U = fmaxf(U, sb + eb);
                     sb = s; eb = err; bi = k;

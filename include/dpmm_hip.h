@@ -157,9 +157,12 @@ int dpmm_sample_params(dpmm_ctx* ctx, uint32_t iter, int K, const int32_t* membe
 int dpmm_get_drawn_params(dpmm_ctx* ctx, double* mu, double* chol_inv_sigma, double* logdet_sigma);
 
 /* Mixed precision (DPMM_PREC_MIXED): a point's BF16-screened label is kept when its best score,
- * lowered by tau * q (q its screened Mahalanobis form), still beats every other cluster's score
- * raised by tau * q_k; otherwise the point is re-scored exactly in FP32.  Default tau = 1/32.
- * No counterpart in the reference (whose sweep is FP64 throughout). */
+ * lowered by its error bound e_b, still beats every other cluster's score raised by e_k; otherwise
+ * the point is re-scored exactly in FP32.  e_k bounds the difference between the screened score and
+ * the FP32 sweep's score of the same point and cluster, from sum_l |W_jl| |x_l - mu_l| per row (a
+ * second MFMA), so the kept labels are the FP32 sweep's whatever the covariance's conditioning.  tau
+ * scales the bound: 1 (default) the derived bound, 0 keeps every screened label (a diagnostic of the
+ * raw BF16 screen), up to 16 widens it.  No counterpart in the reference (whose sweep is FP64). */
 int dpmm_set_mixed_tolerance(dpmm_ctx* ctx, float tau);
 /* Number of points the last mixed label sweep re-scored in FP32 (-1 if none ran). */
 int dpmm_last_recheck_count(dpmm_ctx* ctx, int64_t* n);

@@ -7,7 +7,7 @@ import pytest
 import dpmmsubclusters_gpu_amd as dp
 from dpmmsubclusters_gpu_amd.synthetic import gaussian_mixture
 import oracle_ctypes as orc
-from helpers import gaussian_params, have_gpu, say
+from helpers import adversarial_mixture, gaussian_params, have_gpu, say
 
 pytestmark = pytest.mark.gpu
 
@@ -127,8 +127,49 @@ def test_pruned_sweep_with_sibling_clusters(D, K):
     dev.close()
 
 
+@pytest.mark.parametrize("D", [64, 128])
+@pytest.mark.parametrize("final", [False, True])
+def test_mixed_equals_fp32_on_ill_conditioned_near_ties(D, final):
+    """Adversarial data for the BF16 screen: covariances with ridge 1e-3 and a dominant correlated
+    direction (condition number ~4e5) and means that differ only along high-variance directions, so
+    clusters overlap and W (x - mu) cancels large terms (tests/helpers.adversarial_mixture).  The
+    screen's bound scales with |W| |x - mu| and keeps only labels the FP32 sweep would draw: mixed
+    labels, pruned or not, equal the FP32 sweep's; both agree with the FP64 oracle wherever its margin
+    exceeds 1e-6.  The recheck fraction is reported."""
+    K, N = 8, 40_000
+    X, gt, mus, sig = adversarial_mixture(N, D, K, seed=D)
+    rng = np.random.default_rng(D)
+    lam = np.linalg.inv(sig)
+    ld = np.linalg.slogdet(sig)[1]
+    w = np.full(K, 1.0 / K)
+    smu = np.repeat(mus, 2, axis=0) + rng.normal(scale=0.1, size=(2 * K, D))
+    params = (mus, lam, ld, w, smu, np.repeat(lam, 2, axis=0), np.repeat(ld, 2), np.full(2 * K, 0.5))
+    say(f"ill-conditioned near-ties D={D} K={K} final={final}: FP32 sweep")
+    dev = dp.DeviceSweep(X, dp.prior_type.Gaussian, seed=13)
+    dev.set_gaussian_params(*params)
+    dev.sample_cluster_labels(7, final)
+    ref, _ = dev.get_labels()
+    dev.set_precision(dp.Precision.MIXED)
+    dev.set_gaussian_params(*params)
+    for prune in (False, True):
+        say(f"ill-conditioned near-ties D={D} K={K} final={final}: mixed, pruning={prune}")
+        dev.set_label_pruning(prune)
+        dev.set_labels(gt, None)
+        dev.sample_cluster_labels(7, final)
+        mx, _ = dev.get_labels()
+        nre = dev.last_recheck_count()
+        print(f"D={D} final={final} pruning={prune}: rechecked {nre} of {N} ({nre / N:.1%})")
+        np.testing.assert_array_equal(mx, ref)
+    olab, margin = orc.label_sweep(0, X, mus, lam, ld, weights=w, seed=13, it=7, final=final)
+    flips = np.nonzero(ref != olab)[0]
+    print(f"D={D}: {flips.size} labels differ from the FP64 oracle, largest margin "
+          f"{margin[flips].max() if flips.size else 0:.2e}")
+    assert not ((ref != olab) & (margin >= 1e-6)).any()
+    dev.close()
+
+
 def test_screen_alone_flip_rate():
-    """tau = 0 keeps every screened label: the raw BF16 flip rate against the oracle, reported."""
+    """A bound scale of 0 keeps every screened label: the raw BF16 flip rate against the oracle, reported."""
     rng = np.random.default_rng(1)
     D, K, N = 64, 20, 100_000
     X, gt, _ = gaussian_mixture(N, D, K, seed=5, dtype=np.float32)

@@ -1940,6 +1940,54 @@ int dpmm_label_contingency(dpmm_ctx* c, int K, int64_t* table) {
     return DPMM_OK;
 }
 
+// ---- the allocation-free contingency table -------------------------------------------------------
+// Scratch of dpmm_label_contingency_static: the per-workgroup partial tables of k_contingency, in a
+// file-scope device array (4 MiB per GPU, held until the process ends).  Host code never passes the
+// array's name to a kernel: on each device, on the first call made with it current, its address is
+// taken once with hipGetSymbolAddress and kept per device.
+constexpr size_t kStaticScratchInts = (size_t)1 << 20;
+static __device__ __align__(256) int32_t scratch[kStaticScratchInts];
+constexpr int kMaxDevices = 64;
+static int32_t* g_scratch_ptr[kMaxDevices];
+static hipError_t g_scratch_err[kMaxDevices];
+static std::once_flag g_scratch_once[kMaxDevices];
+
+static int scratch_of_current_device(int32_t** out) {
+    int dev = -1;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(DPMM_EINVAL, "device index above 63");
+    std::call_once(g_scratch_once[dev], [dev]() {
+        void* p = nullptr;
+        g_scratch_err[dev] = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        g_scratch_ptr[dev] = static_cast<int32_t*>(p);
+    });
+    if (g_scratch_err[dev] != hipSuccess || !g_scratch_ptr[dev])
+        return fail(DPMM_EHIP, std::string("scratch address: ") + hipGetErrorString(g_scratch_err[dev]));
+    *out = g_scratch_ptr[dev];
+    return DPMM_OK;
+}
+
+int dpmm_label_contingency_static(const int32_t* d_labels, const int32_t* d_classes, int64_t N, int K, int C,
+                                  int64_t* d_table, void* stream) {
+    if (!d_labels || !d_classes || !d_table || N <= 0 || K <= 0 || C <= 0)
+        return fail(DPMM_EINVAL, "bad args");
+    if ((int64_t)K * C > 16384) return fail(DPMM_EINVAL, "clusters x classes above 16384");
+    int32_t* part = nullptr;
+    int rc = scratch_of_current_device(&part);
+    if (rc) return rc;
+    const int KG = K * C;
+    // every partial row is written in full, so nothing left by an earlier call is read
+    const int rows = (int)std::min<int64_t>({256, (N + kThreads - 1) / kThreads, (int64_t)(kStaticScratchInts / KG)});
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(k_contingency, dim3(rows), dim3(kThreads), (size_t)KG * sizeof(int), st, d_labels, d_classes, N,
+                       K, C, part);
+    hipLaunchKernelGGL(k_table_reduce, dim3((KG + kThreads - 1) / kThreads), dim3(kThreads), 0, st, part, rows, KG,
+                       d_table);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(DPMM_EHIP, std::string("contingency: ") + hipGetErrorString(e));
+    return DPMM_OK;
+}
+
 int dpmm_label_nmi(dpmm_ctx* c, int K, double* nmi) {
     if (!c || !nmi) return fail(DPMM_EINVAL, "bad args");
     if (!c->dClasses) return fail(DPMM_ESTATE, "no ground truth set");

@@ -193,6 +193,15 @@ int dpmm_last_label_evaluations(dpmm_ctx* ctx, double* fraction);
 int dpmm_set_ground_truth(dpmm_ctx* ctx, const int32_t* classes, int num_classes);
 int dpmm_label_contingency(dpmm_ctx* ctx, int K, int64_t* table);
 int dpmm_label_nmi(dpmm_ctx* ctx, int K, double* nmi);
+/* No counterpart in the reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The table of
+ * dpmm_label_contingency from device arrays: labels and classes (N each, device memory), K x C
+ * (<= 16384) int64 counts into d_table (device memory), launched on `stream` (a hipStream_t; NULL for
+ * the default stream) without synchronising.  It calls no hipMalloc / hipFree: its partial tables
+ * live in a 4 MiB array that the library holds on each GPU it runs on until the process ends.
+ * Calls that overlap on one GPU share that array and race; calls in stream order do not. */
+int dpmm_label_contingency_static(const int32_t* d_labels, const int32_t* d_classes, int64_t N, int K, int C,
+                                  int64_t* d_table, void* stream);
 
 /* Kernel timings of the last calls (ms, HIP events on the library's stream). */
 int dpmm_last_kernel_ms(dpmm_ctx* ctx, float* label_sweep_ms, float* sub_sweep_ms, float* stats_ms);

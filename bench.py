@@ -244,8 +244,8 @@ def run_seeds(args, cfg):
 
 
 def run_screener(args, cfg):
-    """C5: train the merge screener on pairs logged by the plain sampler on small mixtures of the
-    same family, then run the workload from its over-split start twice, every candidate pair tested
+    """C5: train the merge screener on pairs logged by the plain sampler on mixtures like the two small
+    configurations, then run the workload from its over-split start twice, every candidate pair tested
     exactly and screened with a full scan every 10 iterations; one JSON line."""
     import torch
     import dpmmsubclusters_gpu_amd as dp
@@ -253,17 +253,13 @@ def run_screener(args, cfg):
     from dpmmsubclusters_gpu_amd import screener as scr
     N, D, iters = cfg["N"], cfg["D"], cfg["iters"]
     t0 = time.perf_counter()
-    feats, hs, acc = [], [], []
-    # training mixtures of the workload's family at its cluster sizes (N / (pieces K) points per
-    # fragment), logged by the plain sampler
-    k_train = 12
-    n_train = max(20_000, min(300_000, N // cfg["K"] * k_train))
-    for seed in (101, 102, 103, 104, 105, 106):   # 3 fragments per cluster: more merge decisions per run
-        print(f"[bench] screener training data: mixture seed {seed}", file=sys.stderr, flush=True)
-        f, h, u = scr.collect_pairs(n_train, D, k_train, pieces=3, iters=40, seed=seed, a=cfg["a"],
-                                    precision=dp.Precision.MIXED)
-        feats.append(f); hs.append(h); acc.append(h > u)
-    Xf, h, a = np.concatenate(feats), np.concatenate(hs), np.concatenate(acc)
+    # training data: merge decisions the plain sampler logs on mixtures of the two small configurations'
+    # families (D = 2 with 2 clusters, D = 32 with 20 clusters), started over-split into 3 fragments
+    train_runs = [("c1", 101), ("c1", 102), ("c2", 103), ("c2", 104), ("c2", 105), ("c2", 106)]
+    Xf, h, u = scr.collect_config_style_pairs(train_runs, n_points=200_000, pieces=3, iters=60,
+                                              precision=dp.Precision.MIXED,
+                                              log=lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True))
+    a = h > u
     y = scr.plausible(h)
     s, hist = scr.train(Xf, y, epochs=30, batch=4096, lr=2e-3, seed=2)
     z = s.logits(Xf)
@@ -319,7 +315,11 @@ def run_screener(args, cfg):
                                f"clusters, alpha={cfg['alpha']}, {iters} iterations, merge screener with a full "
                                f"scan every 10 iterations", "N": N, "D": D, "precision": "mixed",
                    "parallelism": "single GPU"},
-        "screener": {"training_pairs": int(len(y)), "plausible": int(y.sum()), "accepted": int(a.sum()),
+        "screener": {"training_data": "pairs logged by the plain sampler on 2 mixtures like the D=2 README "
+                                      "example (2 clusters) and 4 like the D=32 configuration (20 clusters, "
+                                      "Dirichlet(1) weights), 200,000 points each, over-split into 3 fragments per "
+                                      "cluster, 60 iterations; no D=128 data",
+                     "training_pairs": int(len(y)), "plausible": int(y.sum()), "accepted": int(a.sum()),
                      "training_seconds": t_train, "threshold": thr, "final_loss": hist[-1]},
         "all_pairs": runs["all_pairs"], "screened": sr,
         "speedup_vs_all_pairs": runs["all_pairs"]["seconds"] / sr["seconds"],

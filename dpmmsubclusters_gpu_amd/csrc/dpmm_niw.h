@@ -601,15 +601,15 @@ __global__ void __launch_bounds__(kThreads) k_pair_features(const double* __rest
             case 0: f = log(a[0]); break;
             case 1: f = log(b[0]); break;
             case 2: f = log(a[0] + b[0]); break;
-            case 3: f = a[3] / a[0]; break;
-            case 4: f = b[3] / b[0]; break;
+            case 3: f = a[3] / (a[0] * D); break;
+            case 4: f = b[3] / (b[0] * D); break;
             case 5: f = a[6] / D; break;
             case 6: f = b[6] / D; break;
             case 7: f = log1p(di / D); break;
             case 8: f = log1p(dj / D); break;
             case 9: f = log1p(dq / D); break;
-            case 10: f = (a[4] + a[5] - a[3]) / a[0]; break;
-            case 11: f = (b[4] + b[5] - b[3]) / b[0]; break;
+            case 10: f = (a[4] + a[5] - a[3]) / (a[0] * D); break;
+            case 11: f = (b[4] + b[5] - b[3]) / (b[0] * D); break;
             case 12: f = fabs(log(a[1] + 1.0) - log(a[2] + 1.0)); break;
             case 13: f = fabs(log(b[1] + 1.0) - log(b[2] + 1.0)); break;
             case 14: f = fabs(a[6] / D - b[6] / D); break;

@@ -96,11 +96,12 @@ def over_split_labels(gt, pieces, seed):
 
 
 def collect_pairs(n_points, D, K_true, pieces, iters, seed, alpha=10.0, spread=100.0, precision=None,
-                  dtype=np.float32, a=1.0):
+                  dtype=np.float32, a=1.0, ridge=1.0, weight=None):
     """Run the plain sampler (every pair gets the exact test) on a synthetic Gaussian mixture from
     an over-split start and log every candidate pair of every iteration: features (rows x 16),
     log H, log U."""
-    X, gt, _ = gaussian_mixture(n_points, D, K_true, seed=seed, spread=spread, dtype=dtype, a=a)
+    X, gt, _ = gaussian_mixture(n_points, D, K_true, seed=seed, spread=spread, dtype=dtype, a=a, ridge=ridge,
+                                weight=weight)
     dev = DeviceSweep(X, prior_type.Gaussian, seed=seed)
     hyper = niw_hyperparams(1.0, np.zeros(D), D + 3.0, np.eye(D))
     s = Sampler(dev, hyper, alpha, iters, seed=seed, precision=precision,
@@ -119,6 +120,33 @@ def collect_pairs(n_points, D, K_true, pieces, iters, seed, alpha=10.0, spread=1
     if not feats:
         return np.empty((0, N_FEATURES), np.float32), np.empty(0), np.empty(0)
     return np.concatenate(feats), np.concatenate(lh), np.concatenate(lu)
+
+
+# Mixtures of the two small benchmark configurations' families (means ~ U(-100, 100)^D, covariances
+# A A^T + I with A_ij ~ N(0, 1)): the README example's (D = 2, 2 clusters, equal weights, float64) and
+# the D = 32 one's (20 clusters, weights ~ Dirichlet(1), float32), each started over-split.
+CONFIG_STYLES = {
+    "c1": dict(D=2, K_true=2, weight=None, dtype=np.float64, alpha=1.0),
+    "c2": dict(D=32, K_true=20, weight=1.0, dtype=np.float32, alpha=10.0),
+}
+
+
+def collect_config_style_pairs(runs, n_points=60_000, pieces=3, iters=60, precision=None, log=None):
+    """Training data from the plain sampler on configs-1-2-style mixtures: `runs` is a list of
+    (style, seed) with style "c1" or "c2"; returns features, log H, log U of every logged pair.
+    `precision` applies to the float32 mixtures; the float64 ones sweep in FP64."""
+    feats, hs, us = [], [], []
+    for style, seed in runs:
+        c = CONFIG_STYLES[style]
+        if log:
+            log(f"screener training data: {style}-style mixture, seed {seed}")
+        f, h, u = collect_pairs(n_points, c["D"], c["K_true"], pieces, iters, seed, alpha=c["alpha"],
+                                precision=precision if c["dtype"] == np.float32 else None, dtype=c["dtype"],
+                                weight=c["weight"])
+        feats.append(f)
+        hs.append(h)
+        us.append(u)
+    return np.concatenate(feats), np.concatenate(hs), np.concatenate(us)
 
 
 def train(X, y, epochs=30, batch=4096, lr=2e-3, pos_weight=None, seed=0):

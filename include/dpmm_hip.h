@@ -217,10 +217,11 @@ int dpmm_synchronize(dpmm_ctx* ctx);
 typedef struct dpmm_screener dpmm_screener;
 /* Features (16 floats per pair) of cluster pairs (pi[p], pj[p]) from the current statistics of K
  * clusters and per-cluster scalars (K x 7: N, N_left, N_right, log ML of cluster / left / right,
- * log det psi_n of the cluster): log N_i, log N_j, log(N_i + N_j), L_i / N_i, L_j / N_j,
+ * log det psi_n of the cluster): log N_i, log N_j, log(N_i + N_j), L_i / (N_i D), L_j / (N_j D),
  * logdet_i / D, logdet_j / D, log1p of the mean distance whitened by diag psi_i, by diag psi_j and
- * by their sum (each / D), the split gains (L_l + L_r - L) / N of i and j, the sub-cluster count
- * imbalances |log(N_l + 1) - log(N_r + 1)| of i and j, |logdet_i - logdet_j| / D, |log N_i - log N_j|. */
+ * by their sum (each / D), the split gains (L_l + L_r - L) / (N D) of i and j, the sub-cluster count
+ * imbalances |log(N_l + 1) - log(N_r + 1)| of i and j, |logdet_i - logdet_j| / D, |log N_i - log N_j|.
+ * Every feature is per point and per dimension, so a screener trained at one D applies at another. */
 int dpmm_pair_features(dpmm_ctx* ctx, int K, const double* cluster_scalars, int npairs, const int32_t* pi,
                        const int32_t* pj, float* features);
 int dpmm_screener_create(uint64_t seed, dpmm_screener** out);

@@ -102,15 +102,15 @@ void orc_pair_features(int K, int D, const double* N, const double* Nl, const do
         f[0] = std::log(N[i]);
         f[1] = std::log(N[j]);
         f[2] = std::log(N[i] + N[j]);
-        f[3] = L[i] / N[i];
-        f[4] = L[j] / N[j];
+        f[3] = L[i] / (N[i] * D);
+        f[4] = L[j] / (N[j] * D);
         f[5] = ld[i] / D;
         f[6] = ld[j] / D;
         f[7] = std::log1p(di / D);
         f[8] = std::log1p(dj / D);
         f[9] = std::log1p(dpool / D);
-        f[10] = (Ll[i] + Lr[i] - L[i]) / N[i];
-        f[11] = (Ll[j] + Lr[j] - L[j]) / N[j];
+        f[10] = (Ll[i] + Lr[i] - L[i]) / (N[i] * D);
+        f[11] = (Ll[j] + Lr[j] - L[j]) / (N[j] * D);
         f[12] = std::fabs(std::log(Nl[i] + 1.0) - std::log(Nr[i] + 1.0));
         f[13] = std::fabs(std::log(Nl[j] + 1.0) - std::log(Nr[j] + 1.0));
         f[14] = std::fabs(f[5] - f[6]);

@@ -82,4 +82,5 @@ def test_pair_features_invariants():
     N, Nl, Nr, L, Ll, Lr, ld, m, dpsi = S
     d = m[0] - m[3]
     np.testing.assert_allclose(f[1, 9], np.log1p(np.sum(d * d / (dpsi[0] + dpsi[3])) / D), rtol=1e-14)
-    np.testing.assert_allclose(f[1, 10], (Ll[0] + Lr[0] - L[0]) / N[0], rtol=1e-14)
+    np.testing.assert_allclose(f[1, 10], (Ll[0] + Lr[0] - L[0]) / (N[0] * D), rtol=1e-14)
+    np.testing.assert_allclose(f[1, 3], L[0] / (N[0] * D), rtol=1e-14)   # per point and dimension

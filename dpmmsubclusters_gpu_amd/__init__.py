@@ -52,7 +52,8 @@ class _Options(ctypes.Structure):
                 ("init_clusters", ctypes.c_int), ("burnout", ctypes.c_int), ("argmax_sample_stop", ctypes.c_int),
                 ("split_stop", ctypes.c_int), ("max_clusters", ctypes.c_double), ("seed", ctypes.c_uint64),
                 ("full_mu_cov", ctypes.c_int), ("mult_use_weight", ctypes.c_int), ("precision", ctypes.c_int),
-                ("verbose", ctypes.c_int), ("init_from_labels", ctypes.c_int), ("hard_clustering", ctypes.c_int)]
+                ("verbose", ctypes.c_int), ("init_from_labels", ctypes.c_int), ("hard_clustering", ctypes.c_int),
+                ("outlier_mod", ctypes.c_double)]
 
 
 _lib = None
@@ -103,6 +104,9 @@ def load_library():
         "dpmm_last_label_evaluations": [_P, _P],
         "dpmm_synchronize": [_P],
         "dpmm_set_prior": [_P, dbl, _P, dbl, _P, _P],
+        "dpmm_set_outlier_prior": [_P, dbl, _P, dbl, _P, _P],
+        "dpmm_log_marginal_batch_ex": [_P, i, _P, _P, _P, _P],
+        "dpmm_sample_params_ex": [_P, u32, i, _P, _P, _P, _P, i, i],
         "dpmm_set_statistics": [_P, i, _P, _P, _P],
         "dpmm_log_marginal_batch": [_P, i, _P, _P, _P],
         "dpmm_sample_params": [_P, u32, i, _P, _P, _P, i, i],
@@ -331,13 +335,24 @@ class DeviceSweep:
             a = _f64(hyper.alpha)
             _check(load_library().dpmm_set_prior(self._ctx, 0.0, None, 0.0, None, _ptr(a)))
 
-    def log_marginal_batch(self, members):
+    def set_outlier_prior(self, hyper):
+        """Prior of the outlier cluster (niw_hyperparams or multinomial_hyper)."""
+        if isinstance(hyper, niw_hyperparams):
+            m, psi = _f64(hyper.m), _f64(hyper.psi)
+            _check(load_library().dpmm_set_outlier_prior(self._ctx, hyper.k, _ptr(m), hyper.v, _ptr(psi), None))
+        else:
+            a = _f64(hyper.alpha)
+            _check(load_library().dpmm_set_outlier_prior(self._ctx, 0.0, None, 0.0, None, _ptr(a)))
+
+    def log_marginal_batch(self, members, prior_sel=None):
         """Log marginal likelihood and log det psi_n of each row of members (n x 4 sub-cluster
-        entries of the last statistics, -1 unused)."""
+        entries of the last statistics, -1 unused); prior_sel (n) picks, bit 0, the posterior's
+        prior and, bit 1, the marginal's prior terms (0 the model's, 1 the outlier prior)."""
         mem = _i32(np.asarray(members).reshape(-1, 4))
         n = mem.shape[0]
         L, ld = np.empty(n), np.empty(n)
-        _check(load_library().dpmm_log_marginal_batch(self._ctx, n, _ptr(mem), _ptr(L), _ptr(ld)))
+        sel = None if prior_sel is None else _i32(prior_sel)
+        _check(load_library().dpmm_log_marginal_batch_ex(self._ctx, n, _ptr(mem), _ptr(sel), _ptr(L), _ptr(ld)))
         return L, ld
 
     def sample_params(self, iteration, members, weights, lr_weights, full_mu_cov=0, use_weight=0):
@@ -437,11 +452,17 @@ class Sampler:
 
     def __init__(self, dev, hyper, alpha, iters, init_clusters=1, burnout=15, seed=0, max_clusters=float("inf"),
                  precision=None, full_mu_cov=False, mult_use_weight=False, verbose=False, init_labels=None,
-                 hard_clustering=False):
+                 hard_clustering=False, outlier_weight=0.0, outlier_params=None):
         """init_labels (N, 0-based) starts from a given clustering, e.g. an over-split one;
-        hard_clustering takes the argmax in every label sweep (global_params.h:38)."""
+        hard_clustering takes the argmax in every label sweep (global_params.h:38); outlier_weight > 0
+        with outlier_params (a hyperparameter object of the prior's kind) runs the reference's outlier
+        mode, cluster 0 being the outlier cluster (dp_parallel_sampling.cpp:68-72)."""
         L = load_library()
         self.dev = dev
+        if outlier_weight > 0:
+            if outlier_params is None:
+                raise ValueError("outlier_weight > 0 needs outlier_params")
+            dev.set_outlier_prior(outlier_params)
         if precision is None:
             precision = Precision.FP64 if dev._dtype == 0 else Precision.FP32
         if init_labels is not None:
@@ -453,7 +474,8 @@ class Sampler:
                              split_stop=5, max_clusters=float(max_clusters), seed=int(seed),
                              full_mu_cov=int(full_mu_cov), mult_use_weight=int(mult_use_weight),
                              precision=int(precision), verbose=int(verbose),
-                             init_from_labels=int(init_labels is not None), hard_clustering=int(bool(hard_clustering)))
+                             init_from_labels=int(init_labels is not None), hard_clustering=int(bool(hard_clustering)),
+                             outlier_mod=float(outlier_weight))
         self._s = ctypes.c_void_p()
         if dev.prior == prior_type.Gaussian:
             self._m0, self._psi0 = _f64(hyper.m), _f64(hyper.psi)
@@ -561,10 +583,8 @@ class dp_parallel_sampling_class:
                     save_model=False, burnout=15, gt=None, max_clusters=float("inf"), outlier_weight=0.0,
                     outlier_params=None, precision=None):
         """dp_parallel (dp_parallel_sampling.cpp:132-160 + run_model 175-249)."""
-        if outlier_weight:
-            raise NotImplementedError("the outlier cluster is not part of this build")
         s = Sampler(self.dev, local_hyper_params, alpha_param, iters, init_clusters, burnout, self.seed, max_clusters,
-                    precision)
+                    precision, outlier_weight=outlier_weight, outlier_params=outlier_params)
         mi = ModelInfo()
         if verbose and gt is not None:   # classes 0 .. C-1 in order of first appearance
             _, cls = np.unique(np.asarray(gt), return_inverse=True)

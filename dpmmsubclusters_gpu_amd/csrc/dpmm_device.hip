@@ -826,6 +826,16 @@ struct dpmm_ctx {
     bool has_prior = false;
     PriorArgs prior_args{};
     double* dPrior = nullptr;   // m0 (D) | psi0 (D x D) or alpha0 (D)
+    // the outlier cluster's prior (dpmm_set_outlier_prior), selected per job / part
+    bool has_prior1 = false;
+    PriorArgs prior_args1{};
+    double* dPrior1 = nullptr;
+    int32_t* dPsel = nullptr;     // per-job prior selectors of a log-ML batch
+    size_t pselCap = 0;
+    int32_t* dCtl = nullptr;      // per-part draw control of a parameter draw
+    size_t ctlCap = 0;
+    double* dDrawsPrev = nullptr; // the previous draw call's rows, for parts that keep their parameters
+    size_t drawsPrevCap = 0;
     int32_t* dMembers = nullptr;
     size_t membersCap = 0;
     double* dLM = nullptr;
@@ -1439,7 +1449,8 @@ int dpmm_destroy(dpmm_ctx* c) {
                     (void*)c->dPartials, (void*)c->dStats, c->dPB, (void*)c->dRecheck, (void*)c->dRecheckCount,
                     (void*)c->dPrior, (void*)c->dMembers, (void*)c->dLM, (void*)c->dDraws, (void*)c->dDrawStatus,
                     (void*)c->dLogW, (void*)c->dFeat, (void*)c->dMB, c->dSPB, (void*)c->dPruneM,
-                    (void*)c->dPruneSC, (void*)c->dEvals, (void*)c->dDirty, (void*)c->dClasses, (void*)c->dPruneV})
+                    (void*)c->dPruneSC, (void*)c->dEvals, (void*)c->dDirty, (void*)c->dClasses, (void*)c->dPruneV,
+                    (void*)c->dPrior1, (void*)c->dPsel, (void*)c->dCtl, (void*)c->dDrawsPrev})
         if (p) (void)hipFree(p);
     if (c->hStats) (void)hipHostFree(c->hStats);
     if (c->hEvals) (void)hipHostFree(c->hEvals);
@@ -2114,8 +2125,8 @@ int dpmm_synchronize(dpmm_ctx* c) {
 }
 
 // ---- batched conjugate work ---------------------------------------------------------------------
-int dpmm_set_prior(dpmm_ctx* c, double k0, const double* m0, double v0, const double* psi0, const double* alpha0) {
-    if (!c) return fail(DPMM_EINVAL, "null ctx");
+static int make_prior(dpmm_ctx* c, double k0, const double* m0, double v0, const double* psi0, const double* alpha0,
+                      PriorArgs& out, double*& dbuf) {
     const int D = c->D;
     PriorArgs h{};
     std::vector<double> host;
@@ -2144,18 +2155,34 @@ int dpmm_set_prior(dpmm_ctx* c, double k0, const double* m0, double v0, const do
         h.lg_alpha0_sum = std::lgamma(s);
         host.assign(alpha0, alpha0 + D);
     }
-    if (c->dPrior) (void)hipFree(c->dPrior);
-    c->dPrior = nullptr;
-    if (hipMalloc((void**)&c->dPrior, host.size() * sizeof(double)) != hipSuccess) return fail(DPMM_ENOMEM, "prior");
-    HIPCHK(hipMemcpy(c->dPrior, host.data(), host.size() * sizeof(double), hipMemcpyHostToDevice));
+    if (dbuf) (void)hipFree(dbuf);
+    dbuf = nullptr;
+    if (hipMalloc((void**)&dbuf, host.size() * sizeof(double)) != hipSuccess) return fail(DPMM_ENOMEM, "prior");
+    HIPCHK(hipMemcpy(dbuf, host.data(), host.size() * sizeof(double), hipMemcpyHostToDevice));
     if (c->prior == DPMM_PRIOR_GAUSSIAN) {
-        h.m0 = c->dPrior;
-        h.psi0 = c->dPrior + D;
+        h.m0 = dbuf;
+        h.psi0 = dbuf + D;
     } else {
-        h.alpha0 = c->dPrior;
+        h.alpha0 = dbuf;
     }
-    c->prior_args = h;
+    out = h;
+    return DPMM_OK;
+}
+
+int dpmm_set_prior(dpmm_ctx* c, double k0, const double* m0, double v0, const double* psi0, const double* alpha0) {
+    if (!c) return fail(DPMM_EINVAL, "null ctx");
+    const int rc = make_prior(c, k0, m0, v0, psi0, alpha0, c->prior_args, c->dPrior);
+    if (rc) return rc;
     c->has_prior = true;
+    return DPMM_OK;
+}
+
+int dpmm_set_outlier_prior(dpmm_ctx* c, double k0, const double* m0, double v0, const double* psi0,
+                           const double* alpha0) {
+    if (!c) return fail(DPMM_EINVAL, "null ctx");
+    const int rc = make_prior(c, k0, m0, v0, psi0, alpha0, c->prior_args1, c->dPrior1);
+    if (rc) return rc;
+    c->has_prior1 = true;
     return DPMM_OK;
 }
 
@@ -2223,12 +2250,31 @@ int dpmm_set_statistics(dpmm_ctx* c, int K, const int64_t* Nsub, const double* s
 }
 
 int dpmm_log_marginal_batch(dpmm_ctx* c, int n, const int32_t* members, double* logml, double* logdet_psi) {
+    return dpmm_log_marginal_batch_ex(c, n, members, nullptr, logml, logdet_psi);
+}
+
+int dpmm_log_marginal_batch_ex(dpmm_ctx* c, int n, const int32_t* members, const int32_t* prior_sel, double* logml,
+                               double* logdet_psi) {
     if (!c || n < 0 || (n > 0 && (!members || !logml))) return fail(DPMM_EINVAL, "bad args");
     if (!c->has_prior) return fail(DPMM_ESTATE, "prior not set");
     if (c->Kstats <= 0) return fail(DPMM_ESTATE, "no statistics computed yet");
     if (n == 0) return DPMM_OK;
     int rc = upload_members(c, n, members);
     if (rc) return rc;
+    const int32_t* dsel = nullptr;
+    if (prior_sel) {
+        bool any = false;
+        for (int j = 0; j < n; ++j) {
+            if (prior_sel[j] < 0 || prior_sel[j] > 3) return fail(DPMM_EINVAL, "prior selector outside 0..3");
+            any |= prior_sel[j] != 0;
+        }
+        if (any && !c->has_prior1) return fail(DPMM_ESTATE, "outlier prior not set");
+        if (any) {
+            if ((rc = grow(c->dPsel, c->pselCap, (size_t)n))) return rc;
+            HIPCHK(hipMemcpyAsync(c->dPsel, prior_sel, (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+            dsel = c->dPsel;
+        }
+    }
     rc = grow(c->dLM, c->lmCap, (size_t)2 * n);
     if (rc) return rc;
     const int D = c->D;
@@ -2236,7 +2282,8 @@ int dpmm_log_marginal_batch(dpmm_ctx* c, int n, const int32_t* members, double* 
     rc = allow_lds(reinterpret_cast<const void*>(k_log_marginal), lds);
     if (rc) return rc;
     hipLaunchKernelGGL(k_log_marginal, dim3(n), dim3(kThreads), lds, c->stream, c->dStats, (int)c->statEntries(),
-                       c->DP, D, c->prior, c->dMembers, c->prior_args, c->dLM);
+                       c->DP, D, c->prior, c->dMembers, dsel, c->prior_args, dsel ? c->prior_args1 : c->prior_args,
+                       c->dLM);
     HIPCHK(hipGetLastError());
     if ((rc = pin_reserve(c->pinOut, c->pinOutCap, (size_t)2 * n * sizeof(double)))) return rc;
     const double* out = static_cast<const double*>(c->pinOut);
@@ -2251,6 +2298,11 @@ int dpmm_log_marginal_batch(dpmm_ctx* c, int n, const int32_t* members, double* 
 
 int dpmm_sample_params(dpmm_ctx* c, uint32_t iter, int K, const int32_t* members, const double* weights,
                        const double* lr_weights, int full_mu_cov, int use_weight) {
+    return dpmm_sample_params_ex(c, iter, K, members, nullptr, weights, lr_weights, full_mu_cov, use_weight);
+}
+
+int dpmm_sample_params_ex(dpmm_ctx* c, uint32_t iter, int K, const int32_t* members, const int32_t* ctl,
+                          const double* weights, const double* lr_weights, int full_mu_cov, int use_weight) {
     if (!c || K <= 0 || !members || !weights || !lr_weights) return fail(DPMM_EINVAL, "bad args");
     if (K > kMaxK) return fail(DPMM_EINVAL, "K exceeds 2048");
     if (!c->has_prior) return fail(DPMM_ESTATE, "prior not set");
@@ -2258,6 +2310,26 @@ int dpmm_sample_params(dpmm_ctx* c, uint32_t iter, int K, const int32_t* members
     const int D = c->D, np = 3 * K;
     int rc = upload_members(c, np, members);
     if (rc) return rc;
+    const int dstride0 = c->drawStride();
+    const int32_t* dctl = nullptr;
+    if (ctl) {
+        bool keep = false;
+        for (int p = 0; p < np; ++p) {
+            const int32_t* q = ctl + 4 * p;
+            if (q[0] >= 3 * c->Kdraw || q[0] < -1) return fail(DPMM_EINVAL, "kept row outside the previous draws");
+            if (q[1] < -1 || q[2] < -1 || q[3] < 0 || q[3] > 1) return fail(DPMM_EINVAL, "bad draw control");
+            if (q[0] < 0 && q[3] == 1 && !c->has_prior1) return fail(DPMM_ESTATE, "outlier prior not set");
+            keep |= q[0] >= 0;
+        }
+        if (keep) {   // the previous rows, before the draw buffer is reused (or reallocated) below
+            if ((rc = grow(c->dDrawsPrev, c->drawsPrevCap, (size_t)3 * c->Kdraw * dstride0))) return rc;
+            HIPCHK(hipMemcpyAsync(c->dDrawsPrev, c->dDraws, (size_t)3 * c->Kdraw * dstride0 * sizeof(double),
+                                  hipMemcpyDeviceToDevice, c->stream));
+        }
+        if ((rc = grow(c->dCtl, c->ctlCap, (size_t)4 * np))) return rc;
+        HIPCHK(hipMemcpyAsync(c->dCtl, ctl, (size_t)4 * np * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+        dctl = c->dCtl;
+    }
     if ((rc = pin_reserve(c->pinLogW, c->pinLogWCap, (size_t)3 * K * sizeof(double)))) return rc;
     double* logw = static_cast<double*>(c->pinLogW);
     for (int k = 0; k < K; ++k) {
@@ -2276,7 +2348,8 @@ int dpmm_sample_params(dpmm_ctx* c, uint32_t iter, int K, const int32_t* members
     const size_t lds = gauss ? lds_bytes((size_t)D * (D + 1) + 3 * D) : lds_bytes(D);
     if ((rc = allow_lds(reinterpret_cast<const void*>(k_draw_params), lds))) return rc;
     hipLaunchKernelGGL(k_draw_params, dim3(np), dim3(kThreads), lds, c->stream, c->dStats, (int)c->statEntries(),
-                       c->DP, D, c->prior, c->dMembers, c->prior_args, iter, c->key0(), c->key1(), full_mu_cov,
+                       c->DP, D, c->prior, c->dMembers, dctl, c->dDrawsPrev, c->prior_args,
+                       c->has_prior1 ? c->prior_args1 : c->prior_args, iter, c->key0(), c->key1(), full_mu_cov,
                        dstride, c->dDraws, c->dDrawStatus);
     HIPCHK(hipGetLastError());
     // parameter slots of the sweeps
@@ -2390,6 +2463,8 @@ int dpmm_get_drawn_params(dpmm_ctx* c, double* mu, double* chol_inv_sigma, doubl
 // Internal accessors for the host driver (same library).
 namespace dpmm {
 int set_error(int code, const char* msg) { return fail(code, msg); }
+
+bool ctx_has_outlier_prior(const dpmm_ctx* c) { return c && c->has_prior1; }
 
 int ctx_dims(const dpmm_ctx* c, int64_t* N, int* D, int* prior, int* precision) {
     if (!c) return DPMM_EINVAL;

@@ -290,9 +290,15 @@ __device__ void gram_rev_packed(const double* R, double* G, int D) {
 // ---- kernels -------------------------------------------------------------------------------------
 
 // out[2 j] = log marginal likelihood, out[2 j + 1] = log det psi_n (NaN when psi_n is not SPD).
+// Two priors: h (the model's) and h1 (the outlier cluster's); psel[j] (NULL: 0) picks, bit 0, the prior
+// the posterior is formed from and, bit 1, the prior of the log marginal likelihood's prior terms (the
+// reference's log_marginal_likelihood(prior, posterior, stats) takes them separately, niw.cpp:122-138,
+// and its split test forms sub-cluster posteriors from the cluster's prior,
+// local_clusters_actions.cpp:221-228).
 __global__ void __launch_bounds__(kThreads, 4) k_log_marginal(const double* __restrict__ stats, int E, int DP, int D,
                                                            int prior, const int32_t* __restrict__ members,
-                                                           PriorArgs h, double* __restrict__ out) {
+                                                           const int32_t* __restrict__ psel, PriorArgs h,
+                                                           PriorArgs h1, double* __restrict__ out) {
     extern __shared__ double lds[];
     __shared__ double red[8];
     __shared__ double scal[4];
@@ -301,6 +307,9 @@ __global__ void __launch_bounds__(kThreads, 4) k_log_marginal(const double* __re
     const int job = blockIdx.x;
     if (threadIdx.x < 4) mem[threadIdx.x] = members[4 * job + threadIdx.x];
     __syncthreads();
+    const int sel = psel ? psel[job] : 0;
+    const PriorArgs& hp = (sel & 1) ? h1 : h;
+    const PriorArgs& hm = (sel & 2) ? h1 : h;
     if (prior == DPMM_PRIOR_MULTINOMIAL) {
         // lgamma(sum a0) - lgamma(sum an) + sum_d [lgamma(an_d) - lgamma(a0_d)]
         double part = 0.0, an_part = 0.0;
@@ -308,21 +317,21 @@ __global__ void __launch_bounds__(kThreads, 4) k_log_marginal(const double* __re
             double s = 0.0;
             for (int t = 0; t < 4; ++t)
                 if (mem[t] >= 0) s += stats[(size_t)mem[t] * E + 1 + d];
-            const double an = h.alpha0[d] + s;
+            const double an = hp.alpha0[d] + s;
             an_part += an;
-            part += lgamma(an) - lgamma(h.alpha0[d]);
+            part += lgamma(an) - lgamma(hm.alpha0[d]);
         }
         const double sum_an = block_sum(an_part, red);
         const double tot = block_sum(part, red);
         if (threadIdx.x == 0) {
-            out[2 * job] = h.lg_alpha0_sum - lgamma(sum_an) + tot;
+            out[2 * job] = hm.lg_alpha0_sum - lgamma(sum_an) + tot;
             out[2 * job + 1] = 0.0;
         }
         return;
     }
     double* P = lds;
     double* m = lds + D * (D + 1) / 2;
-    niw_posterior(stats, (size_t)E, DP, D, mem, h, 0.0, P, m, scal);
+    niw_posterior(stats, (size_t)E, DP, D, mem, hp, 0.0, P, m, scal);
     const bool ok = chol_packed(P, D, sh);
     double ld = 0.0, lg = 0.0;
     const double vn = scal[2];
@@ -338,13 +347,13 @@ __global__ void __launch_bounds__(kThreads, 4) k_log_marginal(const double* __re
         if (!ok) {
             L = __builtin_nan("");
             ld = __builtin_nan("");
-        } else if (N == 0.0) {
+        } else if (N == 0.0 && (sel & 1) == ((sel >> 1) & 1)) {
             L = 0.0;   // the posterior is the prior: every term cancels
-            ld = h.logdet_psi0;
+            ld = hp.logdet_psi0;
         } else {
-            L = -N * (double)D * 0.5 * log(kPiF64) + lg - h.lmvg0 +
-                (h.v0 / 2.0) * (D * log(h.v0) + h.logdet_psi0) - (vn / 2.0) * (D * log(vn) + ld) +
-                (D / 2.0) * log(h.k0 / kn);
+            L = -N * (double)D * 0.5 * log(kPiF64) + lg - hm.lmvg0 +
+                (hm.v0 / 2.0) * (D * log(hm.v0) + hm.logdet_psi0) - (vn / 2.0) * (D * log(vn) + ld) +
+                (D / 2.0) * log(hm.k0 / kn);
         }
         out[2 * job] = L;
         out[2 * job + 1] = ld;
@@ -354,12 +363,16 @@ __global__ void __launch_bounds__(kThreads, 4) k_log_marginal(const double* __re
 // Part p = 3k + r of the draw (r: 0 cluster, 1 left, 2 right): members[4p..], Philox stream
 // (seed, p, iter, TAG_PARAM); output row p of `draws`
 // (Gaussian: [mu (D) | R packed lower (D (D + 1) / 2) | logdet Sigma]; multinomial: log theta (D)).
-// status[p] = 0 on success, 1 if a factorisation failed.
+// status[p] = 0 on success, 1 if a factorisation failed.  ctl (NULL: every part drawn as above) has 4
+// entries per part: a row of `prev` (the previous call's draws) whose parameters the part keeps (-1:
+// draw), the Philox stream's slot and iteration (-1: p and iter), and the prior the posterior is formed
+// from (0: h, 1: h1).
 __global__ void __launch_bounds__(kThreads) k_draw_params(const double* __restrict__ stats, int E, int DP, int D,
                                                           int prior, const int32_t* __restrict__ members,
-                                                          PriorArgs h,
-                                                          uint32_t iter, uint32_t key0, uint32_t key1,
-                                                          int full_mu_cov, int draw_stride,
+                                                          const int32_t* __restrict__ ctl,
+                                                          const double* __restrict__ prev, PriorArgs h0,
+                                                          PriorArgs h1, uint32_t iter0, uint32_t key0,
+                                                          uint32_t key1, int full_mu_cov, int draw_stride,
                                                           double* __restrict__ draws, int32_t* __restrict__ status) {
     extern __shared__ double lds[];
     __shared__ double red[8];
@@ -367,10 +380,24 @@ __global__ void __launch_bounds__(kThreads) k_draw_params(const double* __restri
     __shared__ double sh[2];
     __shared__ int mem[4];
     const int p = blockIdx.x;
-    const uint32_t c0 = (uint32_t)p;
+    double* out = draws + (size_t)p * draw_stride;
+    uint32_t c0 = (uint32_t)p, iter = iter0;
+    int sel = 0;
+    if (ctl) {
+        const int src = ctl[4 * p];
+        if (src >= 0) {   // the part keeps parameters drawn before
+            const double* in = prev + (size_t)src * draw_stride;
+            for (int e = threadIdx.x; e < draw_stride; e += blockDim.x) out[e] = in[e];
+            if (threadIdx.x == 0) status[p] = 0;
+            return;
+        }
+        if (ctl[4 * p + 1] >= 0) c0 = (uint32_t)ctl[4 * p + 1];
+        if (ctl[4 * p + 2] >= 0) iter = (uint32_t)ctl[4 * p + 2];
+        sel = ctl[4 * p + 3] & 1;
+    }
+    const PriorArgs& h = sel ? h1 : h0;
     if (threadIdx.x < 4) mem[threadIdx.x] = members[4 * p + threadIdx.x];
     __syncthreads();
-    double* out = draws + (size_t)p * draw_stride;
     if (prior == DPMM_PRIOR_MULTINOMIAL) {
         // Dirichlet(alpha0 + sum): coordinate d's gamma from the sub-stream at word 4096 d (block
         // 1024 d), all coordinates in parallel; normalised in a fixed order, logged

@@ -9,8 +9,8 @@
 //   reset relabel      (i, 0, t, TAG_RESET)   word0 & 1
 //   split relabel      (i, 0, t, TAG_SPLITSUB) word0 & 1
 //   sequential streams (c0, n, t, tag), n = 0, 1, ... : parameter draws, Dirichlet weights,
-//   split / merge test uniforms; a multinomial parameter draw gives coordinate d the stream that
-//   starts at n = 1024 d.
+//   split / merge test uniforms, the sub-cluster weights a split or a merge sets; a multinomial
+//   parameter draw gives coordinate d the stream that starts at n = 1024 d.
 // The categorical draws are Gumbel-max (labels) and a logistic threshold (sub-labels), which
 // have the distributions of the reference's softmax + alias sampling (cudaKernel.cu:27-124,
 // 352-403).
@@ -28,7 +28,8 @@ namespace dpmm {
 
 enum RngTag : uint32_t {
     TAG_INIT = 1, TAG_LABEL = 2, TAG_SUB = 3, TAG_RESET = 4, TAG_SPLITSUB = 5,
-    TAG_PARAM = 6, TAG_LRW = 7, TAG_MIXW = 8, TAG_SPLIT_U = 9, TAG_MERGE_U = 10
+    TAG_PARAM = 6, TAG_LRW = 7, TAG_MIXW = 8, TAG_SPLIT_U = 9, TAG_MERGE_U = 10,
+    TAG_SPLIT_LRW = 11, TAG_MERGE_LRW = 12   // sub-cluster weights set by a split / a merge
 };
 
 struct Ctr4 { uint32_t v[4]; };

@@ -79,21 +79,46 @@ struct Part {
     int64_t N = 0;
     int32_t mem[4] = {-1, -1, -1, -1};
     double lml = 0.0;
-    double ld = 0.0;   // log det psi_n of the posterior
+    double ld = 0.0;     // log det psi_n of the posterior
+    int prior = 0;       // 0: the model's prior, 1: the outlier cluster's (outlier mode)
+    double lml_split = 0.0;   // log ML with the posterior formed from the cluster's prior (the split test)
+};
+// Where a part's parameters come from when the draw call does not redraw them (outlier mode leaves
+// cluster index 1 undrawn): a row of the last draw call, or a draw on the Philox stream (slot, iter) from
+// the posterior of the statistics entries `mem` (parameters the reference drew at a split).
+struct ParamRef {
+    int32_t row = -1;
+    int32_t slot = -1, iter = -1;
+    int32_t mem[4] = {-1, -1, -1, -1};
+    int prior = 0;
 };
 struct Cluster {
     Part c, l, r;
     double lrw[2] = {0.5, 0.5};
     bool splittable = false;
     std::vector<std::pair<bool, double>> hist;
+    ParamRef pr[3];
 };
 
-Part entry_part(int64_t N, int q0, int q1) {
-    Part p;
+// A part's statistics entries and count after a statistics pass (its prior stays).
+void set_entries(Part& p, int64_t N, int q0, int q1) {
     p.N = N;
     p.mem[0] = q0;
     p.mem[1] = q1;
-    return p;
+    p.mem[2] = p.mem[3] = -1;
+}
+ParamRef keep_row(int row) {
+    ParamRef r;
+    r.row = row;
+    return r;
+}
+ParamRef draw_from(const int32_t* mem, int prior, int slot, int iter) {
+    ParamRef r;
+    r.slot = slot;
+    r.iter = iter;
+    std::copy(mem, mem + 4, r.mem);
+    r.prior = prior;
+    return r;
 }
 
 }  // namespace
@@ -105,6 +130,7 @@ struct SamplerImpl {
     int D = 0;
     int prior = 0;
     int kcap = 0;   // the device kernels' largest cluster count (dpmm_max_clusters)
+    bool outlier = false;   // outlier mode (cluster 0 is the outlier cluster)
     std::vector<Cluster> cl;
     std::vector<double> weights;
     double host_ms = 0;
@@ -130,11 +156,46 @@ struct SamplerImpl {
 
     void reset_hist(Cluster& c) const { c.hist.assign(o.burnout + 5, std::make_pair(false, 0.0)); }
 
+    // Log-ML jobs of one device launch: up to four statistics entries each, a prior selector (bit 0 the
+    // posterior's prior, bit 1 the prior of the marginal's terms) and where the results go.
+    struct Jobs {
+        std::vector<int32_t> mem, sel;
+        std::vector<double*> lml, ld;
+        void add(const int32_t* m, int s, double* l, double* d) {
+            mem.insert(mem.end(), m, m + 4);
+            sel.push_back(s);
+            lml.push_back(l);
+            ld.push_back(d);
+        }
+        int size() const { return (int)sel.size(); }
+        void clear() { mem.clear(); sel.clear(); lml.clear(); ld.clear(); }
+    };
+
+    int run_jobs(Jobs& jb) {
+        const int n = jb.size();
+        if (n == 0) return DPMM_OK;
+        bool any_sel = false;
+        for (int s : jb.sel) any_sel |= s != 0;
+        std::vector<double> L(n), ld(n);
+        int rc = dpmm_log_marginal_batch_ex(ctx, n, jb.mem.data(), any_sel ? jb.sel.data() : nullptr, L.data(),
+                                            ld.data());
+        if (rc) return rc;
+        for (int j = 0; j < n; ++j) {
+            if (jb.ld[j]) {   // a part: its log ML must exist (pairs keep NaN for a failed factorisation)
+                if (!std::isfinite(L[j])) return DPMM_ENOTSPD;
+                *jb.ld[j] = ld[j];
+            }
+            *jb.lml[j] = L[j];
+        }
+        jb.clear();
+        return DPMM_OK;
+    }
+
     // create_sufficient_statistics + update_suff_stats_posterior (local_clusters_actions.cpp:
     // 109-168) for the clusters flagged in `which` (empty: all), with the log marginal likelihoods
     // of their parts from one device launch.  With `pairs`, the same launch also evaluates the
-    // union of every pair of merge candidates (splittable, non-empty clusters) that the screener,
-    // when set, passes.
+    // union of every pair of merge candidates (splittable, non-empty clusters; in outlier mode not
+    // the outlier cluster, local_clusters_actions.cpp:295) that the screener, when set, passes.
     int update_stats(const std::vector<char>& which, bool pairs, uint32_t t = 0) {
         const int K = (int)cl.size();
         int rc;
@@ -144,27 +205,26 @@ struct SamplerImpl {
         }
         counts_fresh = false;
         const auto t0 = std::chrono::steady_clock::now();
-        std::vector<int32_t> mem;
-        std::vector<Part*> dst;
-        auto job = [&](Part* p) {
-            mem.insert(mem.end(), p->mem, p->mem + 4);
-            dst.push_back(p);
-        };
+        Jobs jb;
         for (int k = 0; k < K; ++k) {
             if (!which.empty() && !which[k]) continue;
             Cluster& c = cl[k];
-            c.l = entry_part(Nsub[2 * k], 2 * k, -1);
-            c.r = entry_part(Nsub[2 * k + 1], 2 * k + 1, -1);
-            c.c = entry_part(Nsub[2 * k] + Nsub[2 * k + 1], 2 * k, 2 * k + 1);   // cluster = l + r (niw.cpp:140-148)
-            job(&c.c);
-            job(&c.l);
-            job(&c.r);
+            set_entries(c.l, Nsub[2 * k], 2 * k, -1);
+            set_entries(c.r, Nsub[2 * k + 1], 2 * k + 1, -1);
+            set_entries(c.c, Nsub[2 * k] + Nsub[2 * k + 1], 2 * k, 2 * k + 1);   // cluster = l + r (niw.cpp:140-148)
+            for (Part* pp : {&c.c, &c.l, &c.r}) jb.add(pp->mem, pp->prior * 3, &pp->lml, &pp->ld);
+            // the split test forms the sub-clusters' posteriors from the cluster's prior
+            // (local_clusters_actions.cpp:221-228): a job of its own where a sub-cluster has another prior
+            for (Part* sp : {&c.l, &c.r}) {
+                if (sp->prior == c.c.prior) continue;
+                jb.add(sp->mem, c.c.prior | (sp->prior << 1), &sp->lml_split, nullptr);
+            }
         }
         std::vector<size_t> tested;   // pair indices whose exact test joins this launch
         if (pairs) {
             cand_pos.assign(K, -1);
             cand.clear();
-            for (int k = 0; k < K; ++k)
+            for (int k = outlier ? 1 : 0; k < K; ++k)
                 if (cl[k].splittable && cl[k].c.N > 0) {
                     cand_pos[k] = (int)cand.size();
                     cand.push_back(k);
@@ -177,10 +237,7 @@ struct SamplerImpl {
             pair_logit.clear();
             if ((scr || log_pairs) && np > 0) {
                 // the screener's features need this launch's cluster log MLs: evaluate the parts first
-                rc = run_parts(mem, dst);
-                if (rc) return rc;
-                mem.clear();
-                dst.clear();
+                if ((rc = run_jobs(jb))) return rc;
                 if ((rc = screen_pairs())) return rc;
             }
             size_t p = 0;
@@ -189,38 +246,21 @@ struct SamplerImpl {
                     if (!full_scan && !(pair_logit[p] > scr_thr)) continue;
                     const int i = cand[a], j = cand[b];
                     const int32_t m[4] = {2 * i, 2 * i + 1, 2 * j, 2 * j + 1};
-                    mem.insert(mem.end(), m, m + 4);
+                    // the merged cluster keeps cluster i's prior (shared_actions.cpp:90-92)
+                    jb.add(m, cl[i].c.prior * 3, &cand_L[p], nullptr);
                     tested.push_back(p);
                 }
             n_exact += (int64_t)tested.size();
             n_skipped += (int64_t)(np - tested.size());
             if (scr && full_scan) ++n_full;
         }
-        const int n = (int)(mem.size() / 4);
-        std::vector<double> L(n), ld(n);
-        rc = dpmm_log_marginal_batch(ctx, n, mem.data(), L.data(), ld.data());
-        if (rc) return rc;
-        const int nd = (int)dst.size();
-        for (int j = 0; j < nd; ++j) {
-            if (!std::isfinite(L[j])) return DPMM_ENOTSPD;
-            dst[j]->lml = L[j];
-            dst[j]->ld = ld[j];
+        if ((rc = run_jobs(jb))) return rc;
+        for (int k = 0; k < K; ++k) {
+            if (!which.empty() && !which[k]) continue;
+            for (Part* sp : {&cl[k].l, &cl[k].r})
+                if (sp->prior == cl[k].c.prior) sp->lml_split = sp->lml;
         }
-        for (size_t q = 0; q < tested.size(); ++q) cand_L[tested[q]] = L[nd + q];
         host_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-        return DPMM_OK;
-    }
-
-    int run_parts(const std::vector<int32_t>& mem, const std::vector<Part*>& dst) {
-        const int n = (int)(mem.size() / 4);
-        std::vector<double> L(n), ld(n);
-        int rc = dpmm_log_marginal_batch(ctx, n, mem.data(), L.data(), ld.data());
-        if (rc) return rc;
-        for (int j = 0; j < n; ++j) {
-            if (!std::isfinite(L[j])) return DPMM_ENOTSPD;
-            dst[j]->lml = L[j];
-            dst[j]->ld = ld[j];
-        }
         return DPMM_OK;
     }
 
@@ -256,41 +296,69 @@ struct SamplerImpl {
     }
 
     // sample_clusters (local_clusters_actions.cpp:341-371) + sample_cluster_params
-    // (shared_actions.cpp:9-55): weights and burn-out history here, the 3K draws on the device.
+    // (shared_actions.cpp:9-55): weights and burn-out history here, the 3K draws on the device.  In
+    // outlier mode cluster index 1 is skipped (348-351): its parameters, sub-cluster weights and history
+    // stay, it is left out of the mixture's Dirichlet draw, and the weights are shifted behind the
+    // outlier cluster's fixed weight outlier_mod (362-370).
     int sample_clusters(uint32_t t) {
         const auto t0 = std::chrono::steady_clock::now();
         const int K = (int)cl.size();
-        std::vector<int32_t> mem((size_t)12 * K);
+        std::vector<int32_t> mem((size_t)12 * K), ctl((size_t)12 * K);
         std::vector<double> lrw(2 * K);
+        bool keep_any = false;
         for (int k = 0; k < K; ++k) {
             Cluster& c = cl[k];
-            const double a[2] = {(double)c.l.N + o.alpha / 2, (double)c.r.N + o.alpha / 2};
-            Stream(o.seed, k, t, TAG_LRW).dirichlet(a, 2, c.lrw);
+            const Part* ps[3] = {&c.c, &c.l, &c.r};
+            if (outlier && k == 1) {
+                for (int p = 0; p < 3; ++p) {
+                    const ParamRef& r = c.pr[p];
+                    int32_t* q = ctl.data() + (size_t)4 * (3 * k + p);
+                    q[0] = r.row; q[1] = r.slot; q[2] = r.iter; q[3] = r.prior;
+                    std::copy(r.mem, r.mem + 4, mem.begin() + (size_t)4 * (3 * k + p));
+                }
+                keep_any = true;
+            } else {
+                const double a[2] = {(double)c.l.N + o.alpha / 2, (double)c.r.N + o.alpha / 2};
+                Stream(o.seed, k, t, TAG_LRW).dirichlet(a, 2, c.lrw);
+                const double L = c.l.lml + c.r.lml;
+                const int B = o.burnout;
+                std::rotate(c.hist.begin(), c.hist.begin() + 1, c.hist.begin() + B);
+                c.hist[B - 1] = std::make_pair(true, L);
+                double now = 0.0;
+                bool ready = true;
+                for (int i = 0; i < B; ++i) {
+                    if (c.hist[i].first) now += c.hist[i].second * (1 / (B - 0.1));
+                    else ready = false;
+                }
+                if (ready && now - c.hist[B - 1].second < 1e-2) c.splittable = true;
+                for (int p = 0; p < 3; ++p) {
+                    std::copy(ps[p]->mem, ps[p]->mem + 4, mem.begin() + (size_t)4 * (3 * k + p));
+                    int32_t* q = ctl.data() + (size_t)4 * (3 * k + p);
+                    q[0] = q[1] = q[2] = -1;
+                    q[3] = ps[p]->prior;
+                }
+            }
             lrw[2 * k] = c.lrw[0];
             lrw[2 * k + 1] = c.lrw[1];
-            const double L = c.l.lml + c.r.lml;
-            const int B = o.burnout;
-            std::rotate(c.hist.begin(), c.hist.begin() + 1, c.hist.begin() + B);
-            c.hist[B - 1] = std::make_pair(true, L);
-            double now = 0.0;
-            bool ready = true;
-            for (int i = 0; i < B; ++i) {
-                if (c.hist[i].first) now += c.hist[i].second * (1 / (B - 0.1));
-                else ready = false;
-            }
-            if (ready && now - c.hist[B - 1].second < 1e-2) c.splittable = true;
-            const Part* ps[3] = {&c.c, &c.l, &c.r};
-            for (int p = 0; p < 3; ++p) std::copy(ps[p]->mem, ps[p]->mem + 4, mem.begin() + (size_t)4 * (3 * k + p));
         }
         // mixture weights ~ Dir(N_1, ..., N_K, alpha), last entry dropped (357-365)
-        std::vector<double> a(K + 1), w(K + 1);
-        for (int k = 0; k < K; ++k) a[k] = (double)cl[k].c.N;
-        a[K] = o.alpha;
-        if (K == 1 && a[0] == 0.0) { w[0] = 0.0; w[1] = 1.0; }   // get_dirichlet_distribution (323-339)
-        else Stream(o.seed, 0, t, TAG_MIXW).dirichlet(a.data(), K + 1, w.data());
-        weights.assign(w.begin(), w.begin() + K);
-        const int rc = dpmm_sample_params(ctx, t, K, mem.data(), weights.data(), lrw.data(), o.full_mu_cov,
-                                          o.mult_use_weight);
+        std::vector<double> a, w;
+        for (int k = 0; k < K; ++k)
+            if (!(outlier && k == 1)) a.push_back((double)cl[k].c.N);
+        a.push_back(o.alpha);
+        w.resize(a.size());
+        if (a.size() == 2 && a[0] == 0.0) { w[0] = 0.0; w[1] = 1.0; }   // get_dirichlet_distribution (323-339)
+        else Stream(o.seed, 0, t, TAG_MIXW).dirichlet(a.data(), (int)a.size(), w.data());
+        weights.clear();
+        if (outlier) weights.push_back(o.outlier_mod);
+        for (size_t i = 0; i + 1 < w.size(); ++i) weights.push_back(w[i] * (1.0 - o.outlier_mod));
+        bool any_prior = false;
+        for (size_t p = 0; p < ctl.size(); p += 4) any_prior |= ctl[p + 3] != 0;
+        const int rc = dpmm_sample_params_ex(ctx, t, K, mem.data(), (keep_any || any_prior) ? ctl.data() : nullptr,
+                                             weights.data(), lrw.data(), o.full_mu_cov, o.mult_use_weight);
+        // every part's parameters are now row 3k + p of this call's draws
+        for (int k = 0; k < K; ++k)
+            for (int p = 0; p < 3; ++p) cl[k].pr[p] = keep_row(3 * k + p);
         host_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         return rc;
     }
@@ -302,6 +370,9 @@ struct SamplerImpl {
     int init() {
         const int K0 = std::max(1, o.init_clusters);
         int rc;
+        outlier = o.outlier_mod > 0;
+        if (outlier && o.init_from_labels)
+            return set_error(DPMM_EINVAL, "outlier mode starts from sampled labels");
         if (o.init_from_labels) {
             std::vector<int32_t> lab(N);
             if ((rc = dpmm_get_labels(ctx, lab.data(), nullptr))) return rc;
@@ -314,7 +385,22 @@ struct SamplerImpl {
         } else if ((rc = dpmm_sample_initial_labels(ctx, K0))) {
             return rc;
         }
-        cl.assign(K0, Cluster());
+        if (outlier) {
+            // sample_labels with outlier_mod > 0 (cudaKernel.cu:158-161): the initial labels skip the outlier
+            // cluster, 1 .. K0; the outlier cluster (dp_parallel_sampling.cpp:68-72) takes index 0 with its
+            // own prior (create_outlier_local_cluster, local_clusters_actions.cpp:48-80)
+            std::vector<int32_t> nl(2 * K0), act(K0, 0);
+            for (int q = 0; q < 2 * K0; ++q) nl[q] = q / 2 + 1;
+            if ((rc = relabel(0, nl, act))) return rc;
+            cl.assign(K0 + 1, Cluster());
+            cl[0].c.prior = cl[0].l.prior = cl[0].r.prior = 1;
+            // cluster 1 is never redrawn by sample_clusters: it keeps the parameters drawn from the prior when
+            // it was created, the sub-clusters' the same ones (create_first_local_cluster, 14-46)
+            const int32_t none[4] = {-1, -1, -1, -1};
+            for (int p = 0; p < 3; ++p) cl[1].pr[p] = draw_from(none, 0, 3, 0);
+        } else {
+            cl.assign(K0, Cluster());
+        }
         for (auto& c : cl) reset_hist(c);
         rc = update_stats({}, false);
         if (rc) return rc;
@@ -372,9 +458,11 @@ struct SamplerImpl {
             std::vector<char> acc(K, 0);
             for (int k = 0; k < K; ++k) {
                 const Cluster& c = cl[k];
+                if (outlier && k == 1) continue;   // local_clusters_actions.cpp:250-253
                 if (!(c.splittable && c.c.N > 1) || fin || c.l.N == 0 || c.r.N == 0) continue;
-                const double lhr = std::log(o.alpha) + std::lgamma((double)c.l.N) + c.l.lml +
-                                   std::lgamma((double)c.r.N) + c.r.lml - (std::lgamma((double)c.c.N) + c.c.lml);
+                const double lhr = std::log(o.alpha) + std::lgamma((double)c.l.N) + c.l.lml_split +
+                                   std::lgamma((double)c.r.N) + c.r.lml_split -
+                                   (std::lgamma((double)c.c.N) + c.c.lml);
                 acc[k] = lhr > std::log(Stream(o.seed, k, it, TAG_SPLIT_U).uniform());
             }
             host_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -390,7 +478,12 @@ struct SamplerImpl {
             for (int k = 0; k < K; ++k) {
                 if (!acc[k]) continue;
                 const int nk = (int)cl.size();
-                // split_cluster_local (183-194): k keeps the left half, nk gets the right half
+                // split_cluster_local (183-194): k keeps the left half, nk gets the right half; each half is
+                // create_splittable_from_params of the old sub-cluster (shared_actions.cpp:57-70): its
+                // parameters, fresh sub-cluster parameters drawn from its posterior (after the relabel below,
+                // the statistics of the half's own entries), sub-cluster weights ~ Dir(alpha / 2, alpha / 2).
+                // These are what the sampler uses only where sample_clusters does not redraw them (cluster
+                // index 1 in outlier mode).
                 Cluster left, right;
                 left.c = cl[k].l;
                 right.c = cl[k].r;
@@ -398,6 +491,16 @@ struct SamplerImpl {
                 right.l = right.r = right.c;
                 reset_hist(left);
                 reset_hist(right);
+                left.pr[0] = cl[k].pr[1];
+                right.pr[0] = cl[k].pr[2];
+                const int32_t ml[4] = {2 * k, 2 * k + 1, -1, -1}, mr[4] = {2 * nk, 2 * nk + 1, -1, -1};
+                for (int p = 1; p < 3; ++p) {
+                    left.pr[p] = draw_from(ml, left.c.prior, 3 * k + p, (int)it);
+                    right.pr[p] = draw_from(mr, right.c.prior, 3 * nk + p, (int)it);
+                }
+                const double half[2] = {o.alpha / 2, o.alpha / 2};
+                Stream(o.seed, k, it, TAG_SPLIT_LRW).dirichlet(half, 2, left.lrw);
+                Stream(o.seed, nk, it, TAG_SPLIT_LRW).dirichlet(half, 2, right.lrw);
                 cl[k] = left;
                 cl.push_back(right);
                 nl[2 * k + 1] = nk;
@@ -470,7 +573,14 @@ struct SamplerImpl {
                     merged.N = A.c.N + Bc.c.N;
                     merged.mem[0] = 2 * i; merged.mem[1] = 2 * i + 1;
                     merged.mem[2] = 2 * j; merged.mem[3] = 2 * j + 1;
-                    merged.lml = L;
+                    merged.lml = merged.lml_split = L;
+                    merged.prior = A.c.prior;
+                    // sub-cluster weights ~ Dir(N_i + alpha / 2, N_j + alpha / 2), the left sub-cluster a clone
+                    // of the merged cluster (its parameters: cluster i's), the right one cluster j's
+                    const double mw[2] = {(double)A.c.N + 0.5 * al, (double)Bc.c.N + 0.5 * al};
+                    Stream(o.seed, i, it, TAG_MERGE_LRW).dirichlet(mw, 2, A.lrw);
+                    A.pr[1] = A.pr[0];
+                    A.pr[2] = Bc.pr[0];
                     A.r = Bc.c;
                     A.c = merged;
                     A.l = merged;   // the reference clones the merged parameters into the left sub-cluster
@@ -486,8 +596,9 @@ struct SamplerImpl {
         K = (int)cl.size();
         std::vector<int32_t> remap(K, -1);
         int nk = 0;
+        // outlier mode keeps the outlier cluster, and cluster 1 while there are two (408-411)
         for (int k = 0; k < K; ++k)
-            if (cl[k].c.N > 0) remap[k] = nk++;
+            if (cl[k].c.N > 0 || (outlier && (k == 0 || (k == 1 && K == 2)))) remap[k] = nk++;
         bool any_merge = false;
         for (int k = 0; k < (int)merge_into.size(); ++k) any_merge |= merge_into[k] >= 0;
         if (any_merge || nk != K) {
@@ -526,7 +637,12 @@ int dpmm_sampler_create(dpmm_ctx* ctx, const dpmm_options* opt, double k0, const
     int64_t N;
     int D, prior, prec;
     dpmm::ctx_dims(ctx, &N, &D, &prior, &prec);
-    if (opt->prior != prior || opt->iters < 0 || opt->burnout < 1 || !(opt->alpha > 0)) return DPMM_EINVAL;
+    if (opt->prior != prior || opt->iters < 0 || opt->burnout < 1 || !(opt->alpha > 0))
+        return dpmm::set_error(DPMM_EINVAL, "bad sampler options");
+    if (!(opt->outlier_mod >= 0.0 && opt->outlier_mod < 1.0))
+        return dpmm::set_error(DPMM_EINVAL, "outlier_mod must lie in [0, 1)");
+    if (opt->outlier_mod > 0.0 && !dpmm::ctx_has_outlier_prior(ctx))
+        return dpmm::set_error(DPMM_ESTATE, "outlier mode needs dpmm_set_outlier_prior first");
     int rc = dpmm_set_prior(ctx, k0, m0, v0, psi0, alpha0);
     if (rc) return rc;
     if (opt->precision != prec) {

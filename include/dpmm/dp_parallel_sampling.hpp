@@ -11,9 +11,13 @@
 // swept in FP64 like the reference; larger D, or set_precision(DPMM_PREC_FP32 / DPMM_PREC_MIXED),
 // sweeps an FP32 copy.  Multinomial data are counts, passed to the device as int32.
 //
-// Differences from the reference, each reported by an exception rather than ignored: the outlier
-// cluster (outlier_weight > 0) and save_model are not built; draw_labels is accepted and ignored
-// (the reference plots with OpenCV).
+// The types are declared in namespace dpmm and, unless DPMM_NO_GLOBAL_NAMES is defined, also in the
+// global namespace, where the reference declares them, so its callers compile unchanged.
+//
+// Differences from the reference, each reported by an exception rather than ignored: save_model is not
+// built; draw_labels is accepted and ignored (the reference plots with OpenCV).  ModelInfo::dp_model
+// carries the model's mixture weights (dp_model->group.weights, as main.cpp:76-80 reads them), not the
+// whole host-side model.
 #pragma once
 #include <cfloat>
 #include <chrono>
@@ -62,13 +66,22 @@ struct multinomial_hyper : hyperparams {
     explicit multinomial_hyper(const V& a) : alpha(a.data(), a.data() + a.size()) {}
 };
 
+// The parts of the reference's model (ds.h:330-378) that its callers read after a run.
+struct local_group {
+    std::vector<double> weights;   // mixture weights of the final clusters (ds.h:339)
+};
+struct dp_parallel_sampling {
+    local_group group;
+};
+
 struct ModelInfo {
+    std::shared_ptr<dp_parallel_sampling> dp_model;   // dp_model->group.weights (main.cpp:76-80)
     std::vector<double> iter_count;           // seconds per iteration
-    std::vector<double> nmi_score_history;    // per iteration when gt is given (verbose or not)
+    std::vector<double> nmi_score_history;    // per iteration in verbose mode when gt is given
     std::vector<double> likelihood_history;   // log posterior per iteration in verbose mode, else 1
     std::vector<int> cluster_count_history;
     std::shared_ptr<LabelsType> labels;       // 1-based
-    std::vector<double> weights;              // the model's mixture weights (dp_model->group.weights)
+    std::vector<double> weights;              // the same mixture weights as dp_model->group.weights
 };
 
 class dpmm_error : public std::runtime_error {
@@ -145,7 +158,8 @@ class dp_parallel_sampling_class {
                           double max_clusters = DBL_MAX, double outlier_weight = 0,
                           std::shared_ptr<hyperparams> outlier_params = nullptr) {
         (void)draw_labels;
-        if (outlier_weight > 0 || outlier_params) throw std::invalid_argument("dpmm: the outlier cluster is not built");
+        if (outlier_weight > 0 && !outlier_params)
+            throw std::invalid_argument("dpmm: outlier_weight > 0 needs outlier_params");
         if (save_model) throw std::invalid_argument("dpmm: save_model is not built");
         dpmm_ctx* ctx = nullptr;
         if (prior_ == Multinomial) {
@@ -172,6 +186,20 @@ class dp_parallel_sampling_class {
         o.precision = precision_;
         o.verbose = 0;
         o.hard_clustering = hard_clustering_ ? 1 : 0;
+        o.outlier_mod = outlier_weight > 0 ? outlier_weight : 0.0;
+        if (outlier_weight > 0) {   // the outlier cluster's own prior (local_clusters_actions.cpp:48-61)
+            if (prior_ == Gaussian) {
+                auto* h = dynamic_cast<niw_hyperparams*>(outlier_params.get());
+                if (!h || (int)h->m.size() != D_)
+                    throw std::invalid_argument("dpmm: outlier_params must be niw_hyperparams of dimension D");
+                check(dpmm_set_outlier_prior(ctx, h->k, h->m.data(), h->v, h->psi.data(), nullptr));
+            } else {
+                auto* h = dynamic_cast<multinomial_hyper*>(outlier_params.get());
+                if (!h || (int)h->alpha.size() != D_)
+                    throw std::invalid_argument("dpmm: outlier_params must be multinomial_hyper of dimension D");
+                check(dpmm_set_outlier_prior(ctx, 0.0, nullptr, 0.0, nullptr, h->alpha.data()));
+            }
+        }
         dpmm_sampler* s = nullptr;
         if (prior_ == Gaussian) {
             auto* h = dynamic_cast<niw_hyperparams*>(local_hyper_params.get());
@@ -208,7 +236,7 @@ class dp_parallel_sampling_class {
             if (verbose) check(dpmm_sampler_log_posterior(s, &lp));
             info.likelihood_history.push_back(lp);
             double score = -1.0;
-            if (gt) {   // the label / class table is built on the device; only K x C counts come back
+            if (verbose && gt) {   // the label / class table is built on the device; only K x C counts come back
                 check(dpmm_label_nmi(ctx, K, &score));
                 info.nmi_score_history.push_back(score);
             }
@@ -222,6 +250,8 @@ class dp_parallel_sampling_class {
         const int K = dpmm_sampler_num_clusters(s);
         info.weights.resize(K);
         check(dpmm_sampler_weights(s, info.weights.data(), nullptr, K));
+        info.dp_model = std::make_shared<dp_parallel_sampling>();
+        info.dp_model->group.weights = info.weights;
         return info;
     }
 
@@ -236,3 +266,16 @@ class dp_parallel_sampling_class {
 };
 
 }  // namespace dpmm
+
+#ifndef DPMM_NO_GLOBAL_NAMES
+using dpmm::dp_parallel_sampling;
+using dpmm::dp_parallel_sampling_class;
+using dpmm::hyperparams;
+using dpmm::LabelsType;
+using dpmm::LabelType;
+using dpmm::local_group;
+using dpmm::ModelInfo;
+using dpmm::multinomial_hyper;
+using dpmm::niw_hyperparams;
+using dpmm::prior_type;
+#endif

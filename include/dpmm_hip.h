@@ -128,6 +128,11 @@ int dpmm_relabel(dpmm_ctx* ctx, uint32_t iter, int K_old, const int32_t* new_lab
  * multinomial_hyper(alpha0 (D)) (priors/multinomial_hyper.h:13, 29) for a multinomial one.
  * DPMM_ENOTSPD if psi0 is not SPD. */
 int dpmm_set_prior(dpmm_ctx* ctx, double k0, const double* m0, double v0, const double* psi0, const double* alpha0);
+/* The outlier cluster's prior, outlier_params of dp_parallel (dp_parallel_sampling.h:28-29,
+ * local_clusters_actions.cpp:48-61), in the form of dpmm_set_prior: jobs and parts below select it
+ * with their prior selectors. */
+int dpmm_set_outlier_prior(dpmm_ctx* ctx, double k0, const double* m0, double v0, const double* psi0,
+                           const double* alpha0);
 /* Test hook beside dpmm_set_labels (the reference's test/myCudaKernel.h): statistics of 2K
  * sub-clusters in the layout of dpmm_sufficient_statistics, for dpmm_log_marginal_batch and
  * dpmm_sample_params; the next dpmm_sufficient_statistics call recomputes them from the labels. */
@@ -142,6 +147,13 @@ int dpmm_set_statistics(dpmm_ctx* ctx, int K, const int64_t* Nsub, const double*
  * (local_clusters_actions.cpp:103-168, 215-242; shared_actions.cpp:88-111).  logdet_psi (may be
  * NULL) gets log det psi_n; a non-SPD posterior gives NaN. */
 int dpmm_log_marginal_batch(dpmm_ctx* ctx, int n, const int32_t* members, double* logml, double* logdet_psi);
+/* Same, with a prior selector per sum (prior_sel may be NULL: all 0): bit 0 picks the prior the posterior
+ * is formed from, bit 1 the prior whose terms the log marginal likelihood subtracts (0 the model's, 1 the
+ * outlier's).  The reference takes both separately (niw::log_marginal_likelihood(prior, posterior, stats),
+ * niw.cpp:122-138) and its split test forms a sub-cluster's posterior from the cluster's prior
+ * (local_clusters_actions.cpp:221-228). */
+int dpmm_log_marginal_batch_ex(dpmm_ctx* ctx, int n, const int32_t* members, const int32_t* prior_sel, double* logml,
+                               double* logdet_psi);
 /* sample_cluster_params (shared_actions.cpp:9-55; local_clusters_actions.cpp:341-371) for K clusters
  * in one launch: part p = 3k + r (r: 0 cluster, 1 left, 2 right sub-cluster) is drawn from the
  * posterior of the statistic sum members[4p .. 4p+3] (niw::sample_distribution niw.cpp:82-107, Bartlett
@@ -152,6 +164,13 @@ int dpmm_log_marginal_batch(dpmm_ctx* ctx, int n, const int32_t* members, double
  * log pi.  Replaces dpmm_set_*_params for the sampler.  DPMM_ENOTSPD if a draw fails. */
 int dpmm_sample_params(dpmm_ctx* ctx, uint32_t iter, int K, const int32_t* members, const double* weights,
                        const double* lr_weights, int full_mu_cov, int use_weight);
+/* Same, with 4 control entries per part (ctl may be NULL): [0] a row of the previous call's draws whose
+ * parameters the part keeps (-1: draw it), [1], [2] the slot and iteration of its Philox stream (-1: p and
+ * iter), [3] the prior of its posterior (0 the model's, 1 the outlier's).  Kept rows serve the reference's
+ * outlier mode, whose sample_clusters leaves cluster index 1 undrawn (local_clusters_actions.cpp:346-352),
+ * and draws keyed elsewhere reproduce parameters drawn at a split (shared_actions.cpp:57-70). */
+int dpmm_sample_params_ex(dpmm_ctx* ctx, uint32_t iter, int K, const int32_t* members, const int32_t* ctl,
+                          const double* weights, const double* lr_weights, int full_mu_cov, int use_weight);
 /* The FP64 draws of the last dpmm_sample_params, 3K parts: mu (or log theta) 3K x D, the lower
  * factor R of invSigma 3K x D x D (may be NULL), logdet Sigma 3K (may be NULL). */
 int dpmm_get_drawn_params(dpmm_ctx* ctx, double* mu, double* chol_inv_sigma, double* logdet_sigma);
@@ -265,6 +284,10 @@ typedef struct {
      * local_clusters_actions.cpp:466: sample_labels(group, hard_clustering ? true : final, ...));
      * split / merge decisions still use the final flag of the iteration. */
     int hard_clustering;
+    /* > 0: the reference's outlier mode (dp_parallel's outlier_weight, dp_parallel_sampling.h:28): cluster 0
+     * is the outlier cluster, with the prior set by dpmm_set_outlier_prior and mixture weight outlier_mod, and
+     * the reference's index rules hold (local_clusters_actions.cpp:250, 295, 348, 364-370, 409-410). */
+    double outlier_mod;
 } dpmm_options;
 
 /* init_model + init_first_clusters (dp_parallel_sampling.cpp:91-130).  Gaussian hyper:

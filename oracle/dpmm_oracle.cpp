@@ -55,7 +55,8 @@ constexpr double kPi = 3.14159265358979323846;
 
 enum : uint32_t {
     TAG_INIT = 1, TAG_LABEL = 2, TAG_SUB = 3, TAG_RESET = 4, TAG_SPLITSUB = 5,
-    TAG_PARAM = 6, TAG_LRW = 7, TAG_MIXW = 8, TAG_SPLIT_U = 9, TAG_MERGE_U = 10
+    TAG_PARAM = 6, TAG_LRW = 7, TAG_MIXW = 8, TAG_SPLIT_U = 9, TAG_MERGE_U = 10,
+    TAG_SPLIT_LRW = 11, TAG_MERGE_LRW = 12   // sub-cluster weights set by a split / a merge
 };
 
 /* ---------------------------------------------------------------- Philox ---------- */
@@ -242,6 +243,9 @@ struct Model {
     std::vector<Cluster> cl;
     std::vector<double> weights;
     std::vector<int32_t> labels, subs;
+    /* outlier mode (dp_parallel's outlier_weight > 0): cluster 0 is the outlier cluster */
+    double outlier_mod = 0.0;
+    bool outlier() const { return outlier_mod > 0.0; }
     /* trace of the last iteration's sweeps (orc_model_trace): labels and sub-labels as the two
      * sweeps drew them, before resets, splits and merges, with the sweeps' relative margins */
     bool trace = false;
@@ -550,23 +554,30 @@ void update_stats(Model& M, const std::vector<int>& which) {
 
 void dirichlet_mix(Model& M, uint32_t t) {   /* sample_clusters tail (local_clusters_actions.cpp:357-370) */
     const int K = (int)M.cl.size();
-    std::vector<double> counts(K + 1), out(K + 1);
-    for (int k = 0; k < K; ++k) counts[k] = (double)M.cl[k].c.st.N;
-    counts[K] = M.alpha;
-    if (K + 1 == 2 && counts[0] == 0.0) {   /* get_dirichlet_distribution special case (323-339) */
+    std::vector<double> counts;
+    for (int k = 0; k < K; ++k)
+        if (!(M.outlier() && k == 1)) counts.push_back((double)M.cl[k].c.st.N);   /* index 1 skipped (348) */
+    counts.push_back(M.alpha);
+    std::vector<double> out(counts.size());
+    if (counts.size() == 2 && counts[0] == 0.0) {   /* get_dirichlet_distribution special case (323-339) */
         out[0] = 0; out[1] = 1;
     } else {
         Stream rs(M.seed, 0, t, TAG_MIXW);
-        rs.dirichlet(K + 1, counts.data(), out.data());
+        rs.dirichlet((int)counts.size(), counts.data(), out.data());
     }
-    M.weights.assign(out.begin(), out.begin() + K);
+    /* weights scaled by 1 - outlier_mod, the outlier cluster's weight in front (362-370) */
+    M.weights.clear();
+    if (M.outlier()) M.weights.push_back(M.outlier_mod);
+    for (size_t i = 0; i + 1 < out.size(); ++i) M.weights.push_back(out[i] * (1.0 - M.outlier_mod));
 }
 
-/* sample_clusters + shared_actions::sample_cluster_params (shared_actions.cpp:9-55). */
+/* sample_clusters + shared_actions::sample_cluster_params (shared_actions.cpp:9-55); in outlier mode
+ * cluster index 1 keeps its parameters, sub-cluster weights and history (348-351). */
 bool sample_clusters(Model& M, uint32_t t) {
     const int K = (int)M.cl.size();
     bool ok = true;
     for (int k = 0; k < K; ++k) {
+        if (M.outlier() && k == 1) continue;
         Cluster& c = M.cl[k];
         ok &= draw_dist(M, c.c.post, 3 * k + 0, t, c.c.dist);
         ok &= draw_dist(M, c.l.post, 3 * k + 1, t, c.l.dist);
@@ -657,6 +668,7 @@ void group_step(Model& M, uint32_t t, bool no_more_splits, bool final_iter) {
         std::vector<int> accepted;
         for (int k = 0; k < K; ++k) {
             Cluster& c = M.cl[k];
+            if (M.outlier() && k == 1) continue;   /* local_clusters_actions.cpp:250-253 */
             if (!(c.splittable && c.c.st.N > 1)) continue;
             if (final_iter || c.l.st.N == 0 || c.r.st.N == 0) continue;
             const Hyper post = posterior(M.prior, D, c.c.prior, c.c.st);
@@ -676,6 +688,10 @@ void group_step(Model& M, uint32_t t, bool no_more_splits, bool final_iter) {
             Cluster nl, nr;
             nl.c = old.l; nl.l = old.l; nl.r = old.l; reset_hist(M, nl); nl.splittable = false;
             nr.c = old.r; nr.l = old.r; nr.r = old.r; reset_hist(M, nr); nr.splittable = false;
+            /* sub-cluster weights ~ Dir(alpha / 2, alpha / 2) (shared_actions.cpp:65-67) */
+            const double half[2] = {M.alpha / 2, M.alpha / 2};
+            Stream(M.seed, (uint32_t)k, t, TAG_SPLIT_LRW).dirichlet(2, half, nl.lrw);
+            Stream(M.seed, (uint32_t)new_index, t, TAG_SPLIT_LRW).dirichlet(2, half, nr.lrw);
             M.cl[k] = nl;
             M.cl.push_back(nr);
             split_of[k] = new_index;
@@ -695,12 +711,22 @@ void group_step(Model& M, uint32_t t, bool no_more_splits, bool final_iter) {
                 }
             }
             update_stats(M, touched);
+            /* fresh sub-cluster parameters of both halves, drawn from the half's posterior
+             * (create_splittable_from_params, shared_actions.cpp:57-64), on the streams of the half's
+             * sub-cluster slots; observable only in outlier mode, where cluster index 1 keeps them */
+            if (M.outlier())
+                for (int k : touched) {
+                    Cluster& c = M.cl[k];
+                    const Hyper post = posterior(M.prior, D, c.c.prior, c.c.st);
+                    draw_dist(M, post, 3 * k + 1, t, c.l.dist);
+                    draw_dist(M, post, 3 * k + 2, t, c.r.dist);
+                }
         }
 
         /* check_and_merge (local_clusters_actions.cpp:287-321) + should_merge (shared_actions.cpp:88-111) */
         K = (int)M.cl.size();
         std::vector<std::pair<int, int>> merges;
-        for (int i = 0; i < K; ++i)
+        for (int i = M.outlier() ? 1 : 0; i < K; ++i)   /* the outlier cluster never merges (295-298) */
             for (int j = i + 1; j < K; ++j) {
                 Cluster& a = M.cl[i];
                 Cluster& b = M.cl[j];
@@ -722,6 +748,9 @@ void group_step(Model& M, uint32_t t, bool no_more_splits, bool final_iter) {
                 const double u = rs.uniform();
                 if ((lhr > std::log(u)) || (final_iter && lhr > std::log(0.1))) {
                     /* merge_clusters (204-213) + merge_clusters_to_splittable (shared_actions.cpp:72-86) */
+                    /* sub-cluster weights ~ Dir(N_i + alpha / 2, N_j + alpha / 2) (shared_actions.cpp:77-81) */
+                    const double mw[2] = {Ni + 0.5 * al, Nj + 0.5 * al};
+                    Stream(M.seed, (uint32_t)i, t, TAG_MERGE_LRW).dirichlet(2, mw, a.lrw);
                     a.c.st = agg;
                     a.c.post = p;
                     a.l = a.c;   /* cluster_params_l = clone of the merged cluster (reference behaviour) */
@@ -749,8 +778,11 @@ void group_step(Model& M, uint32_t t, bool no_more_splits, bool final_iter) {
     K = (int)M.cl.size();
     std::vector<int> remap(K, -1);
     std::vector<Cluster> kept;
-    for (int k = 0; k < K; ++k)
-        if (M.cl[k].c.st.N > 0) { remap[k] = (int)kept.size(); kept.push_back(M.cl[k]); }
+    for (int k = 0; k < K; ++k)   /* outlier mode keeps the outlier cluster, and cluster 1 while K = 2 (408-411) */
+        if (M.cl[k].c.st.N > 0 || (M.outlier() && (k == 0 || (k == 1 && K == 2)))) {
+            remap[k] = (int)kept.size();
+            kept.push_back(M.cl[k]);
+        }
     if ((int)kept.size() != K) {
         for (int64_t i = 0; i < M.N; ++i) M.labels[i] = remap[M.labels[i]];
         M.cl.swap(kept);
@@ -895,8 +927,9 @@ double orc_nmi(const int32_t* a, const int32_t* b, int64_t N) {
 namespace {
 /* init_model + init_first_clusters (dp_parallel_sampling.cpp:48-87) into M; X is not copied. */
 void model_init(Model& M, const orc_run_opts* o, const double* X, int64_t N, int D, double k0, const double* m0,
-                double v0, const double* psi0, const double* alpha0) {
+                double v0, const double* psi0, const double* alpha0, const Hyper* outlier_hyper = nullptr) {
     M.prior = o->prior; M.D = D; M.N = N; M.X = X; M.alpha = o->alpha; M.seed = o->seed;
+    M.outlier_mod = outlier_hyper ? o->outlier_mod : 0.0;
     M.burnout = o->burnout; M.full_mu_cov = o->full_mu_cov; M.mult_use_weight = o->mult_use_weight;
     if (M.prior == 0) {
         M.hyper.k = k0; M.hyper.v = v0; M.hyper.m.assign(m0, m0 + D); M.hyper.psi.assign(psi0, psi0 + (size_t)D * D);
@@ -910,20 +943,32 @@ void model_init(Model& M, const orc_run_opts* o, const double* X, int64_t N, int
         uint32_t w[4];
         philox_at(M.seed, (uint32_t)i, 0, 0, TAG_INIT, w);
         int l = (int)(u01(w[0]) * K0);
-        M.labels[i] = std::min(l, K0 - 1);
+        /* with an outlier cluster the initial labels skip it (cudaKernel.cu:158-161) */
+        M.labels[i] = std::min(l, K0 - 1) + (M.outlier() ? 1 : 0);
         M.subs[i] = (int32_t)(w[1] & 1u);
     }
-    /* init_first_clusters (dp_parallel_sampling.cpp:65-87; create_first_local_cluster 14-46) */
-    M.cl.resize(K0);
-    for (int k = 0; k < K0; ++k) {
+    /* init_first_clusters (dp_parallel_sampling.cpp:65-87; create_first_local_cluster 14-46; the outlier
+     * cluster first, with its own prior, create_outlier_local_cluster 48-80) */
+    const int off = M.outlier() ? 1 : 0, K = K0 + off;
+    M.cl.resize(K);
+    for (int k = 0; k < K; ++k) {
         Cluster& c = M.cl[k];
-        c.c.prior = M.hyper; c.c.post = M.hyper; c.c.st = empty_stats(M.prior, D);
+        const Hyper& h = (k < off) ? *outlier_hyper : M.hyper;
+        c.c.prior = h; c.c.post = h; c.c.st = empty_stats(M.prior, D);
         c.l = c.c; c.r = c.c;
         c.splittable = false;
         reset_hist(M, c);
     }
-    std::vector<int> all(K0);
-    for (int k = 0; k < K0; ++k) all[k] = k;
+    if (M.outlier()) {
+        /* cluster 1 keeps the parameters create_first_local_cluster drew from the prior (its sub-clusters
+         * clones of them): sample_clusters never redraws index 1 in outlier mode */
+        Cluster& c1 = M.cl[1];
+        draw_dist(M, M.hyper, 3, 0, c1.c.dist);
+        c1.l.dist = c1.c.dist;
+        c1.r.dist = c1.c.dist;
+    }
+    std::vector<int> all(K);
+    for (int k = 0; k < K; ++k) all[k] = k;
     update_stats(M, all);
     sample_clusters(M, 0);
 }
@@ -969,6 +1014,22 @@ orc_model* orc_model_create(const orc_run_opts* o, const double* X, int64_t N, i
     m->o = *o;
     m->X.assign(X, X + (size_t)N * D);
     model_init(m->M, o, m->X.data(), N, D, k0, m0, v0, psi0, alpha0);
+    return m;
+}
+
+orc_model* orc_model_create_outlier(const orc_run_opts* o, const double* X, int64_t N, int D, double k0,
+                                    const double* m0, double v0, const double* psi0, const double* alpha0, double ok0,
+                                    const double* om0, double ov0, const double* opsi0, const double* oalpha0) {
+    orc_model* m = new orc_model();
+    m->o = *o;
+    m->X.assign(X, X + (size_t)N * D);
+    Hyper oh;
+    if (o->prior == 0) {
+        oh.k = ok0; oh.v = ov0; oh.m.assign(om0, om0 + D); oh.psi.assign(opsi0, opsi0 + (size_t)D * D);
+    } else {
+        oh.alpha.assign(oalpha0, oalpha0 + D);
+    }
+    model_init(m->M, o, m->X.data(), N, D, k0, m0, v0, psi0, alpha0, o->outlier_mod > 0 ? &oh : nullptr);
     return m;
 }
 

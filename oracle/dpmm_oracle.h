@@ -100,6 +100,7 @@ typedef struct {
     int full_mu_cov;      /* 0 reproduces the reference's diagonal-only mu draw */
     int mult_use_weight;  /* 0 reproduces the reference's multinomial label LL without log pi */
     int verbose;
+    double outlier_mod;   /* > 0: outlier mode (orc_model_create_outlier); orc_run ignores it */
 } orc_run_opts;
 
 int orc_run(const orc_run_opts* opts, const double* X, int64_t N, int D, double k0, const double* m0,
@@ -114,6 +115,12 @@ int orc_run(const orc_run_opts* opts, const double* X, int64_t N, int D, double 
 typedef struct orc_model orc_model;
 orc_model* orc_model_create(const orc_run_opts* opts, const double* X, int64_t N, int D, double k0, const double* m0,
                             double v0, const double* psi0, const double* alpha0);
+/* The same with the reference's outlier cluster (opts->outlier_mod > 0; dp_parallel_sampling.cpp:68-72):
+ * cluster 0 with the prior (ok0, om0, ov0, opsi0) or oalpha0, mixture weight outlier_mod, and the index
+ * rules of local_clusters_actions.cpp:250, 295, 348, 364-370, 409-410. */
+orc_model* orc_model_create_outlier(const orc_run_opts* opts, const double* X, int64_t N, int D, double k0,
+                                    const double* m0, double v0, const double* psi0, const double* alpha0, double ok0,
+                                    const double* om0, double ov0, const double* opsi0, const double* oalpha0);
 void orc_model_destroy(orc_model* m);
 void orc_model_trace(orc_model* m, int enable);
 int orc_model_step(orc_model* m, int t);

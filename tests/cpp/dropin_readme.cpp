@@ -10,7 +10,7 @@
 
 #include "dpmm/dp_parallel_sampling.hpp"
 
-using namespace dpmm;
+// The reference's names, in the global namespace as its callers use them (no dpmm:: prefix).
 
 static void gaussian_run() {
     const int N = 20000, D = 2, K = 2;
@@ -38,19 +38,19 @@ static void gaussian_run() {
         X[(size_t)i * D] = mu[k][0] + L[k][0][0] * z0;
         X[(size_t)i * D + 1] = mu[k][1] + L[k][1][0] * z0 + L[k][1][1] * z1;
     }
-    dp_parallel_sampling_class model(N, X.data(), D, 5, Gaussian);
+    dp_parallel_sampling_class model(N, X.data(), D, 5, prior_type::Gaussian);
     std::shared_ptr<hyperparams> hyper =
         std::make_shared<niw_hyperparams>(1.0, std::vector<double>{0.0, 0.0}, 5.0, std::vector<double>{1, 0, 0, 1});
     ModelInfo info = model.dp_parallel(hyper, 1.0, 400, 1, false, false, false, 15, gt);
     int lo = 1 << 30, hi = -1;
     for (int l : *info.labels) { lo = std::min(lo, l); hi = std::max(hi, l); }
     double ws = 0;
-    for (double w : info.weights) ws += w;
+    for (double w : info.dp_model->group.weights) ws += w;   // as main.cpp:76-80 reads them
     size_t first = info.cluster_count_history.size();
     while (first > 0 && info.cluster_count_history[first - 1] == info.cluster_count_history.back()) --first;
-    std::printf("gaussian K=%d nmi=%.6f labels=%d..%d weights=%.6f iters=%zu reached_at=%zu\n",
-                info.cluster_count_history.back(), info.nmi_score_history.back(), lo, hi, ws, info.iter_count.size(),
-                first + 1);
+    std::printf("gaussian K=%d nmi=%.6f labels=%d..%d weights=%.6f iters=%zu reached_at=%zu nmi_history=%zu\n",
+                info.cluster_count_history.back(), dpmm::nmi(*info.labels, *gt), lo, hi, ws, info.iter_count.size(),
+                first + 1, info.nmi_score_history.size());
 }
 
 static void multinomial_run() {
@@ -64,13 +64,13 @@ static void multinomial_run() {
         std::uniform_int_distribution<int> word(10 * k, 10 * k + 9);   // disjoint vocabularies
         for (int w = 0; w < W; ++w) X[(size_t)i * D + word(rng)] += 1.0;
     }
-    dp_parallel_sampling_class model(N, X.data(), D, 3, Multinomial);
+    dp_parallel_sampling_class model(N, X.data(), D, 3, prior_type::Multinomial);
     std::shared_ptr<hyperparams> hyper = std::make_shared<multinomial_hyper>(std::vector<double>(D, 1.0));
     ModelInfo info = model.dp_parallel(hyper, 10.0, 80, 1, false, false, false, 15, gt);
     int lo = 1 << 30, hi = -1;
     for (int l : *info.labels) { lo = std::min(lo, l); hi = std::max(hi, l); }
     std::printf("multinomial K=%d nmi=%.6f labels=%d..%d iters=%zu\n", info.cluster_count_history.back(),
-                info.nmi_score_history.back(), lo, hi, info.iter_count.size());
+                dpmm::nmi(*info.labels, *gt), lo, hi, info.iter_count.size());
 }
 
 int main() {

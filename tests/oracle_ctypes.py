@@ -25,7 +25,7 @@ class RunOpts(ctypes.Structure):
         ("prior", c_int), ("alpha", c_dbl), ("iters", c_int), ("init_clusters", c_int),
         ("burnout", c_int), ("argmax_sample_stop", c_int), ("split_stop", c_int),
         ("max_clusters", c_dbl), ("seed", c_u64), ("full_mu_cov", c_int),
-        ("mult_use_weight", c_int), ("verbose", c_int),
+        ("mult_use_weight", c_int), ("verbose", c_int), ("outlier_mod", c_dbl),
     ]
 
 
@@ -48,6 +48,7 @@ def lib():
         L.orc_run.restype = c_int
         L.orc_mlp_loss_grad.restype = c_dbl
         L.orc_model_create.restype = P
+        L.orc_model_create_outlier.restype = P
         L.orc_model_destroy.argtypes = [P]
         L.orc_model_trace.argtypes = [P, c_int]
         L.orc_model_step.argtypes = [P, c_int]
@@ -233,14 +234,30 @@ class Model:
 
     def __init__(self, X, prior=0, alpha=1.0, iters=100, init_clusters=1, burnout=15, seed=0, k0=1.0, m0=None,
                  v0=None, psi0=None, alpha0=None, max_clusters=float("inf"), full_mu_cov=False,
-                 mult_use_weight=False, trace=False):
+                 mult_use_weight=False, trace=False, outlier_weight=0.0, outlier_params=None):
+        """outlier_params: (k0, m0, v0, psi0) for the Gaussian prior, alpha0 for the multinomial one."""
         X = f64(X)
         self.N, D = X.shape
         o = RunOpts(prior=prior, alpha=alpha, iters=iters, init_clusters=init_clusters, burnout=burnout,
                     argmax_sample_stop=5, split_stop=5, max_clusters=max_clusters, seed=seed,
-                    full_mu_cov=int(full_mu_cov), mult_use_weight=int(mult_use_weight), verbose=0)
+                    full_mu_cov=int(full_mu_cov), mult_use_weight=int(mult_use_weight), verbose=0,
+                    outlier_mod=float(outlier_weight))
         L = lib()
-        if prior == 0:
+        if outlier_weight > 0:
+            m0 = np.zeros(D) if m0 is None else m0
+            v0 = float(D + 3) if v0 is None else v0
+            psi0 = np.eye(D) if psi0 is None else psi0
+            alpha0 = np.ones(D) if alpha0 is None else alpha0
+            if prior == 0:
+                ok0, om0, ov0, opsi0 = outlier_params
+                self._m = L.orc_model_create_outlier(ctypes.byref(o), _p(X), c_i64(self.N), c_int(D), c_dbl(k0),
+                                                     _p(f64(m0)), c_dbl(v0), _p(f64(psi0)), None, c_dbl(ok0),
+                                                     _p(f64(om0)), c_dbl(ov0), _p(f64(opsi0)), None)
+            else:
+                self._m = L.orc_model_create_outlier(ctypes.byref(o), _p(X), c_i64(self.N), c_int(D), c_dbl(0), None,
+                                                     c_dbl(0), None, _p(f64(alpha0)), c_dbl(0), None, c_dbl(0), None,
+                                                     _p(f64(outlier_params)))
+        elif prior == 0:
             m0 = np.zeros(D) if m0 is None else m0
             v0 = float(D + 3) if v0 is None else v0
             psi0 = np.eye(D) if psi0 is None else psi0

@@ -27,4 +27,5 @@ def test_dropin_readme_example():
     assert int(g.group(1)) == 2 and float(g.group(2)) > 0.99
     assert (int(g.group(3)), int(g.group(4))) == (1, 2)   # 1-based labels
     assert 0.0 < float(g.group(5)) <= 1.0 and int(g.group(6)) == 400
+    assert "nmi_history=0" in out.stdout   # the reference records NMI only in verbose mode
     assert int(m.group(1)) == 3 and float(m.group(2)) > 0.99 and int(m.group(5)) == 80

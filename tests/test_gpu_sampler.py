@@ -35,18 +35,21 @@ CASES = {
 }
 
 
-@pytest.mark.parametrize("case", sorted(CASES))
-def test_lockstep_fp64_against_oracle(case):
-    c = CASES[case]
-    iters = 100
-    X, gt, _ = gaussian_mixture(c["N"], c["D"], c["K"], seed=c["data_seed"], dtype=np.float64)
-    D = c["D"]
-    say(f"lockstep {case}: N={c['N']} D={D} FP64, {iters} iterations")
-    dev = dp.DeviceSweep(X, dp.prior_type.Gaussian, seed=c["seed"])
-    hyper = dp.niw_hyperparams(1.0, np.zeros(D), c["v0"], np.eye(D))
-    smp = dp.Sampler(dev, hyper, c["alpha"], iters, seed=c["seed"], precision=dp.Precision.FP64)
+def _lockstep(name, X, gt, seed, alpha, v0, iters=100, init_clusters=1, outlier=None, min_nmi=0.95):
+    """Device (FP64) and oracle samplers side by side; see the module docstring."""
+    D = X.shape[1]
+    dev = dp.DeviceSweep(X, dp.prior_type.Gaussian, seed=seed)
+    hyper = dp.niw_hyperparams(1.0, np.zeros(D), v0, np.eye(D))
+    kw, okw = {}, {}
+    if outlier is not None:
+        w, (ok0, om0, ov0, opsi0) = outlier
+        kw = dict(outlier_weight=w, outlier_params=dp.niw_hyperparams(ok0, om0, ov0, opsi0))
+        okw = dict(outlier_weight=w, outlier_params=(ok0, om0, ov0, opsi0))
+    smp = dp.Sampler(dev, hyper, alpha, iters, seed=seed, precision=dp.Precision.FP64, init_clusters=init_clusters,
+                     **kw)
     smp.trace_sweeps(True)
-    om = orc.Model(X, prior=0, alpha=c["alpha"], iters=iters, seed=c["seed"], k0=1.0, v0=c["v0"], trace=True)
+    om = orc.Model(X, prior=0, alpha=alpha, iters=iters, seed=seed, k0=1.0, v0=v0, init_clusters=init_clusters,
+                   trace=True, **okw)
     khist = []
     for t in range(1, iters + 1):
         smp.step(t)
@@ -55,14 +58,14 @@ def test_lockstep_fp64_against_oracle(case):
         olab, osub, margin, smargin = om.sweep_trace()
         flips = np.nonzero(glab != olab)[0]
         if flips.size:
-            print(f"{case}: iteration {t}: {flips.size} label flips, largest oracle margin {margin[flips].max():.3e}")
+            print(f"{name}: iteration {t}: {flips.size} label flips, largest oracle margin {margin[flips].max():.3e}")
             assert margin[flips].max() < FP64_MARGIN
-            return
+            return None
         sflips = np.nonzero(gsub != osub)[0]
         if sflips.size:
-            print(f"{case}: iteration {t}: {sflips.size} sub-label flips, largest margin {smargin[sflips].max():.3e}")
+            print(f"{name}: iteration {t}: {sflips.size} sub-label flips, largest margin {smargin[sflips].max():.3e}")
             assert smargin[sflips].max() < FP64_MARGIN
-            return
+            return None
         K = smp.num_clusters()
         assert K == om.num_clusters(), f"iteration {t}: {K} clusters on the device, {om.num_clusters()} in the oracle"
         w, cnt = smp.weights()
@@ -73,11 +76,48 @@ def test_lockstep_fp64_against_oracle(case):
         np.testing.assert_array_equal(sub, osub2, err_msg=f"iteration {t}: sub-labels after the step")
         np.testing.assert_allclose(w, ow, rtol=1e-12, atol=0, err_msg=f"iteration {t}: mixture weights")
         khist.append(K)
-    print(f"{case}: identical for all {iters} iterations; K history {khist}")
-    assert dp._nmi(gt, dev.get_labels()[0]) > 0.95
+    print(f"{name}: identical for all {iters} iterations; K history {khist}")
+    lab = dev.get_labels()[0]
+    if min_nmi is not None:
+        assert dp._nmi(gt, lab) > min_nmi
     smp.close()
     dev.close()
     om.close()
+    return khist, lab, w
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_lockstep_fp64_against_oracle(case):
+    c = CASES[case]
+    X, gt, _ = gaussian_mixture(c["N"], c["D"], c["K"], seed=c["data_seed"], dtype=np.float64)
+    say(f"lockstep {case}: N={c['N']} D={c['D']} FP64, 100 iterations")
+    _lockstep(case, X, gt, c["seed"], c["alpha"], c["v0"])
+
+
+@pytest.mark.parametrize("init_clusters", [1, 3])
+def test_lockstep_outlier_mode_against_oracle(init_clusters):
+    """The reference's outlier mode (dp_parallel's outlier_weight, dp_parallel_sampling.cpp:68-72): cluster
+    0 is the outlier cluster with its own broad prior and weight 0.05; cluster index 1 is never redrawn and
+    is left out of the mixture's Dirichlet draw (local_clusters_actions.cpp:348, 364-370), index 1 never
+    splits (250), the outlier never merges (295), and the outlier (and cluster 1 while K = 2) is never
+    removed (409-410).  The data carry 2 % of points spread uniformly over the whole box; with one initial
+    cluster the outlier cluster first takes every point (cluster 1's weight is 0) and the true clusters are
+    split off it, the spread points staying in it."""
+    D = 4
+    X, gt, _ = gaussian_mixture(20_000, D, 3, seed=1, dtype=np.float64)
+    rng = np.random.default_rng(0)
+    idx = rng.choice(len(X), 400, replace=False)
+    X[idx] = rng.uniform(-300, 300, size=(400, D))
+    gt = gt.copy()
+    gt[idx] = 3
+    say(f"lockstep outlier mode: N=20000 D={D} FP64, init_clusters={init_clusters}")
+    out = _lockstep(f"outlier K0={init_clusters}", X, gt, 3, 10.0, D + 3.0, iters=80, init_clusters=init_clusters,
+                    outlier=(0.05, (1.0, np.zeros(D), D + 3.0, 100.0 * np.eye(D))), min_nmi=None)
+    if out is not None:
+        khist, lab, w = out
+        assert w[0] == 0.05   # the outlier cluster's weight is outlier_mod, fixed
+        if init_clusters == 1:   # the spread points end in the outlier cluster, the true clusters split off it
+            assert khist[-1] == 4 and dp._nmi(gt, lab) > 0.95
 
 
 def test_cluster_cap_stops_splitting_and_run_continues():

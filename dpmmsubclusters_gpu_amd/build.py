@@ -60,13 +60,16 @@ def build(verbose=True, jobs=None):
     _run(["g++", "-shared", "-o", tmp] + objs + ["-fopenmp", f"-L{ROCM}/lib", "-lamdhip64",
                                                  f"-Wl,-rpath,{ROCM}/lib"])
     shutil.move(tmp, LIB)
-    # the C++ drop-in header's example program (run by the GPU tests), linked against the library
     repo = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    link = [f"-L{LIB_DIR}", "-ldpmm_hip", "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath,{ROCM}/lib"]
+    # the command-line front end (the reference's main: --params_path, --model_path, --result_path)
+    _run(["g++", "-O2", "-std=c++17", "-Wall", os.path.join(CSRC, "dpmm_main.cpp"), "-o",
+          os.path.join(LIB_DIR, "dpmm_main")] + link)
+    # the C++ drop-in header's example program (run by the GPU tests), linked against the library
     src = os.path.join(repo, "tests", "cpp", "dropin_readme.cpp")
     if os.path.exists(src):
         _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(repo, "include"), src, "-o",
-              os.path.join(LIB_DIR, "dropin_readme"), f"-L{LIB_DIR}", "-ldpmm_hip", "-Wl,-rpath,$ORIGIN",
-              f"-Wl,-rpath,{ROCM}/lib"])
+              os.path.join(LIB_DIR, "dropin_readme")] + link)
     return LIB
 
 

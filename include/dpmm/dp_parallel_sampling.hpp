@@ -31,6 +31,7 @@
 #include <vector>
 
 #include "../dpmm_hip.h"
+#include "io.hpp"
 
 namespace dpmm {
 
@@ -142,6 +143,63 @@ class dp_parallel_sampling_class {
         }
         precision_ = (prior_ == Gaussian && D_ <= 64) ? DPMM_PREC_FP64 : DPMM_PREC_FP32;
     }
+
+    // The model-file constructor (dp_parallel_sampling.h:15; dp_parallel_sampling.cpp:25-31): points from a
+    // .npy file of N x D values (".npy" appended when missing; read as D x N, utils.cpp:12-57) and the run's
+    // settings from a JSON params file with global_params::init's keys and defaults (global_params.cpp:8-70,
+    // global_params.h:36-78): alpha 10, iterations 100, initial_clusters 1, use_verbose false,
+    // burnout_period 20, max_num_of_clusters unbounded, outlier_mod 0.05, hyper_params and
+    // outlier_hyper_params {"k", "m", "v", "psi"} or {"alpha"} (default niw(1, 0, 5, I), or alpha = 0 for
+    // the multinomial prior, which the sampler rejects), randomSeed (0: seeded from the clock), gt.
+    dp_parallel_sampling_class(const std::string& modelDataFileName, const std::string& modelParamsFileName,
+                               prior_type priorType)
+        : prior_(priorType) {
+        std::string path = modelDataFileName;
+        if (path.find(".npy") == std::string::npos) path += ".npy";
+        const NpyArray a = load_npy(path);
+        N_ = (int)a.rows;
+        D_ = (int)a.cols;
+        if (N_ <= 0 || D_ <= 0) throw std::invalid_argument("dpmm: empty data file");
+        if (prior_ == Multinomial) {
+            counts_.resize(a.data.size());
+            for (size_t i = 0; i < a.data.size(); ++i) counts_[i] = (int32_t)std::lround(a.data[i]);
+        } else {
+            data64_ = a.data;   // point i at i * D: the reference's D x N column-major layout
+        }
+        precision_ = (prior_ == Gaussian && D_ <= 64) ? DPMM_PREC_FP64 : DPMM_PREC_FP32;
+        const json::Value root = json::parse_file(modelParamsFileName);
+        file_.alpha = root.get("alpha", 10.0);
+        file_.iterations = (int)root.get("iterations", 100.0);
+        file_.initial_clusters = (int)root.get("initial_clusters", 1.0);
+        file_.verbose = root.get("use_verbose", false);
+        file_.draw_labels = root.get("draw_labels", false);
+        file_.save_model = root.get("should_save_model", false);
+        file_.burnout = (int)root.get("burnout_period", 20.0);
+        file_.max_clusters = root.get("max_num_of_clusters", DBL_MAX);
+        file_.outlier_mod = root.get("outlier_mod", 0.05);
+        file_.hyper = hyper_from_json(root["hyper_params"]);
+        file_.outlier_hyper = hyper_from_json(root["outlier_hyper_params"]);
+        const double s = root.get("randomSeed", 0.0);
+        seed_ = s != 0.0 ? (unsigned long long)s
+                         : (unsigned long long)std::chrono::system_clock::now().time_since_epoch().count();
+        if (root.has("gt")) {
+            file_.gt = std::make_shared<LabelsType>();
+            for (double v : root["gt"].numbers()) file_.gt->push_back((LabelType)v);
+        }
+        from_file_ = true;
+    }
+
+    // dp_parallel_from_file (dp_parallel_sampling.h:31): dp_parallel with the params file's settings.
+    ModelInfo dp_parallel_from_file() {
+        if (!from_file_) throw std::logic_error("dpmm: dp_parallel_from_file needs the model-file constructor");
+        // checkpoints are written every 1000 iterations (global_params.h:58); shorter runs write none
+        const bool save = file_.save_model && file_.iterations >= 1000;
+        return dp_parallel(file_.hyper, file_.alpha, file_.iterations, file_.initial_clusters, file_.verbose,
+                           file_.draw_labels, save, file_.burnout, file_.gt, file_.max_clusters, file_.outlier_mod,
+                           file_.outlier_hyper);
+    }
+    int num_points() const { return N_; }
+    int dimension() const { return D_; }
 
     dp_parallel_sampling_class(const dp_parallel_sampling_class&) = delete;
     dp_parallel_sampling_class& operator=(const dp_parallel_sampling_class&) = delete;
@@ -256,6 +314,33 @@ class dp_parallel_sampling_class {
     }
 
   private:
+    // niw::create_hyperparams / multinomial_prior::create_hyperparams (niw.cpp:155-198,
+    // multinomial_prior.cpp:80-102): from the JSON object, or the defaults when it is absent.
+    std::shared_ptr<hyperparams> hyper_from_json(const json::Value& v) const {
+        if (prior_ == Gaussian) {
+            if (v.is_null())
+                return std::make_shared<niw_hyperparams>(1.0, std::vector<double>(D_, 0.0), 5.0, identity(D_));
+            std::vector<double> m = v["m"].numbers(), psi;
+            for (const auto& row : v["psi"].arr)
+                for (double x : row.numbers()) psi.push_back(x);
+            return std::make_shared<niw_hyperparams>(v["k"].as_double(), m, v["v"].as_double(), psi);
+        }
+        if (v.is_null()) return std::make_shared<multinomial_hyper>(std::vector<double>(D_, 0.0));
+        return std::make_shared<multinomial_hyper>(v["alpha"].numbers());
+    }
+    static std::vector<double> identity(int D) {
+        std::vector<double> I((size_t)D * D, 0.0);
+        for (int i = 0; i < D; ++i) I[(size_t)i * D + i] = 1.0;
+        return I;
+    }
+    struct FileParams {
+        double alpha = 10.0, max_clusters = DBL_MAX, outlier_mod = 0.05;
+        int iterations = 100, initial_clusters = 1, burnout = 20;
+        bool verbose = false, draw_labels = false, save_model = false;
+        std::shared_ptr<hyperparams> hyper, outlier_hyper;
+        std::shared_ptr<LabelsType> gt;
+    } file_;
+    bool from_file_ = false;
     int N_, D_;
     unsigned long long seed_;
     prior_type prior_;

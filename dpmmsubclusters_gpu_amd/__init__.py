@@ -72,6 +72,7 @@ def load_library():
     i64, u64, u32, dbl, i = ctypes.c_int64, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_double, ctypes.c_int
     sig = {
         "dpmm_device_count": [_P],
+        "dpmm_debug_fail_allocations": [ctypes.c_int64],
         "dpmm_create": [_P, i64, i, i, i, u64, _P],
         "dpmm_create_device": [_P, i64, i, i, i, u64, _P],
         "dpmm_destroy": [_P],

@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -52,6 +53,18 @@ DPMM_MX_EXTERN(128)
 }  // namespace dpmm
 
 namespace {
+// Allocation fault injection (dpmm_debug_fail_allocations): the number of device allocations that still
+// succeed before every further one fails with hipErrorOutOfMemory; -1: off.
+std::atomic<int64_t> g_alloc_budget{-1};
+template <typename T>
+hipError_t dev_malloc(T** p, size_t bytes) {
+    int64_t b = g_alloc_budget.load();
+    while (b >= 0) {
+        if (b == 0) return hipErrorOutOfMemory;
+        if (g_alloc_budget.compare_exchange_weak(b, b - 1)) break;
+    }
+    return hipMalloc(reinterpret_cast<void**>(p), bytes);
+}
 thread_local std::string g_err;
 int fail(int code, const std::string& msg) {
     g_err = msg;
@@ -739,6 +752,7 @@ struct dpmm_ctx {
     int32_t* dSubs = nullptr;
     // parameters (compute type: double for FP64 precision, float otherwise)
     int K = 0, Kcap = 0, stride = 0;
+    size_t capBytes = 0;   // bytes of dP (dSP holds twice as many)
     int kcap_limit = 0;   // dpmm_limit_clusters (0: the kernels' own cap)
     bool params_f64 = false;
     void* dP = nullptr;
@@ -878,15 +892,17 @@ int pick_dp(int D) {
 // or record timing events meanwhile.
 static thread_local bool t_capturing = false;
 
+// Grows a device buffer to n elements.  The new buffer is allocated before the old one is released:
+// when the allocation fails, the old buffer and its capacity stay, so no state of the context ever
+// refers to a released or missing buffer.
 template <typename T>
 int grow(T*& p, size_t& cap, size_t n) {
     if (n <= cap && p) return DPMM_OK;
     if (t_capturing) return fail(DPMM_ESTATE, "buffer growth while a sweep is being captured");
+    T* q = nullptr;
+    if (dev_malloc(&q, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) return fail(DPMM_ENOMEM, "hipMalloc failed");
     if (p) (void)hipFree(p);
-    p = nullptr;
-    cap = 0;
-    if (hipMalloc((void**)&p, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess)
-        return fail(DPMM_ENOMEM, "hipMalloc failed");
+    p = q;
     cap = n;
     return DPMM_OK;
 }
@@ -897,14 +913,20 @@ int ensure_partition(dpmm_ctx* c, int K) {
     if (rc) return rc;
     if (c->Kpart < K || !c->dLabelStart) {
         if (t_capturing) return fail(DPMM_ESTATE, "buffer growth while a sweep is being captured");
-        if (c->dRowtot) (void)hipFree(c->dRowtot);
-        if (c->dLabelStart) (void)hipFree(c->dLabelStart);
-        if (c->dChunkStart) (void)hipFree(c->dChunkStart);
+        // all three new buffers first; the old ones stay when any allocation fails
         const int cap = std::max(K, 64) + 1;
-        if (hipMalloc((void**)&c->dRowtot, cap * sizeof(int32_t)) != hipSuccess ||
-            hipMalloc((void**)&c->dLabelStart, cap * sizeof(int32_t)) != hipSuccess ||
-            hipMalloc((void**)&c->dChunkStart, cap * sizeof(int32_t)) != hipSuccess)
-            return fail(DPMM_ENOMEM, "hipMalloc failed");
+        int32_t* nb[3] = {nullptr, nullptr, nullptr};
+        for (auto& q : nb)
+            if (dev_malloc(&q, cap * sizeof(int32_t)) != hipSuccess) {
+                for (auto& r : nb)
+                    if (r) (void)hipFree(r);
+                return fail(DPMM_ENOMEM, "hipMalloc failed");
+            }
+        for (int32_t* old : {c->dRowtot, c->dLabelStart, c->dChunkStart})
+            if (old) (void)hipFree(old);
+        c->dRowtot = nb[0];
+        c->dLabelStart = nb[1];
+        c->dChunkStart = nb[2];
         c->Kpart = cap - 1;
     }
     return DPMM_OK;
@@ -981,16 +1003,16 @@ int create_common(const void* X, int64_t N, int D, int dtype, int prior, uint64_
     if (device_ptr) {
         c->dX = const_cast<void*>(X);
     } else {
-        if (hipMalloc(&c->dX, bytes) != hipSuccess) { dpmm_destroy(c); return fail(DPMM_ENOMEM, "X alloc"); }
+        if (dev_malloc(&c->dX, bytes) != hipSuccess) { dpmm_destroy(c); return fail(DPMM_ENOMEM, "X alloc"); }
         c->owns_X = true;
         if (hipMemcpy(c->dX, X, bytes, hipMemcpyHostToDevice) != hipSuccess) {
             dpmm_destroy(c);
             return fail(DPMM_EHIP, "X copy");
         }
     }
-    if (hipMalloc((void**)&c->dLabels, N * sizeof(int32_t)) != hipSuccess ||
-        hipMalloc((void**)&c->dSubs, N * sizeof(int32_t)) != hipSuccess ||
-        hipMalloc((void**)&c->dPerm, N * sizeof(int32_t)) != hipSuccess) {
+    if (dev_malloc(&c->dLabels, N * sizeof(int32_t)) != hipSuccess ||
+        dev_malloc(&c->dSubs, N * sizeof(int32_t)) != hipSuccess ||
+        dev_malloc(&c->dPerm, N * sizeof(int32_t)) != hipSuccess) {
         dpmm_destroy(c);
         return fail(DPMM_ENOMEM, "label alloc");
     }
@@ -1035,15 +1057,22 @@ void pack_gauss_slot(int D, int DP, const double* mu, const double* R, double c,
 int ensure_param_buffers(dpmm_ctx* c, int K, size_t slot_bytes) {
     if (K > kMaxK) return fail(DPMM_EINVAL, "K exceeds 2048");
     const size_t need = (size_t)K * slot_bytes;
-    if ((size_t)c->Kcap * c->stride * (c->params_f64 ? 8 : 4) < need || !c->dP) {
+    if (c->capBytes < need || !c->dP) {
+        // both new buffers first; the old ones (and the parameters in them) stay when either fails
+        const int cap = std::max(K, 64);
+        void* np_ = nullptr;
+        void* nsp = nullptr;
+        if (dev_malloc(&np_, (size_t)cap * slot_bytes) != hipSuccess ||
+            dev_malloc(&nsp, (size_t)2 * cap * slot_bytes) != hipSuccess) {
+            if (np_) (void)hipFree(np_);
+            return fail(DPMM_ENOMEM, "param alloc");
+        }
         if (c->dP) (void)hipFree(c->dP);
         if (c->dSP) (void)hipFree(c->dSP);
-        c->dP = c->dSP = nullptr;
-        const int cap = std::max(K, 64);
-        if (hipMalloc(&c->dP, (size_t)cap * slot_bytes) != hipSuccess ||
-            hipMalloc(&c->dSP, (size_t)2 * cap * slot_bytes) != hipSuccess)
-            return fail(DPMM_ENOMEM, "param alloc");
+        c->dP = np_;
+        c->dSP = nsp;
         c->Kcap = cap;
+        c->capBytes = (size_t)cap * slot_bytes;
     }
     return DPMM_OK;
 }
@@ -1068,10 +1097,15 @@ void pack_mx_slot(int D, int DPM, const double* mu, const double* R, double cst,
 
 // Recheck list of the mixed sweeps.
 int ensure_recheck(dpmm_ctx* c) {
-    if (!c->dRecheck) {
-        if (hipMalloc((void**)&c->dRecheck, c->N * sizeof(int32_t)) != hipSuccess ||
-            hipMalloc((void**)&c->dRecheckCount, 2 * sizeof(int)) != hipSuccess)
+    if (!c->dRecheck || !c->dRecheckCount) {
+        if (dev_malloc(&c->dRecheck, c->N * sizeof(int32_t)) != hipSuccess ||
+            dev_malloc(&c->dRecheckCount, 2 * sizeof(int)) != hipSuccess) {
+            if (c->dRecheck) (void)hipFree(c->dRecheck);
+            if (c->dRecheckCount) (void)hipFree(c->dRecheckCount);
+            c->dRecheck = nullptr;
+            c->dRecheckCount = nullptr;
             return fail(DPMM_ENOMEM, "recheck alloc");
+        }
     }
     return DPMM_OK;
 }
@@ -1134,6 +1168,13 @@ int set_gauss(dpmm_ctx* c, int K, const double* mu, const double* mat, const dou
     if (K <= 0 || !mu || !mat || !logdet || !weights || !smu || !smat || !slogdet || !lrw)
         return fail(DPMM_EINVAL, "bad Gaussian params");
     const int D = c->D, DP = c->DP;
+    for (size_t e = 0; e < (size_t)K * D; ++e)
+        if (!std::isfinite(mu[e]) || !std::isfinite(smu[e]) || !std::isfinite(smu[(size_t)K * D + e]))
+            return fail(DPMM_EINVAL, "a mean is NaN or infinite");
+    for (int k = 0; k < K; ++k)
+        if (!std::isfinite(logdet[k]) || !std::isfinite(slogdet[2 * k]) || !std::isfinite(slogdet[2 * k + 1]) ||
+            !(weights[k] >= 0.0) || !(lrw[2 * k] >= 0.0) || !(lrw[2 * k + 1] >= 0.0) || std::isinf(weights[k]))
+            return fail(DPMM_EINVAL, "a log determinant or weight is NaN, infinite or negative");
     const bool f64 = c->precision == DPMM_PREC_FP64;
     const int stride = gauss_slot_stride(DP);
     const size_t esz = f64 ? 8 : 4;
@@ -1426,6 +1467,11 @@ extern "C" {
 
 const char* dpmm_last_error(void) { return g_err.c_str(); }
 
+int dpmm_debug_fail_allocations(int64_t after) {
+    g_alloc_budget.store(after < 0 ? -1 : after);
+    return DPMM_OK;
+}
+
 int dpmm_device_count(int* count) {
     if (!count) return fail(DPMM_EINVAL, "null");
     HIPCHK(hipGetDeviceCount(count));
@@ -1488,7 +1534,7 @@ int dpmm_set_precision(dpmm_ctx* c, int precision) {
         // counts enter the MFMA as BF16, exact up to 256
         if (c->max_count < 0) {
             int* dm = nullptr;
-            if (hipMalloc((void**)&dm, sizeof(int)) != hipSuccess) return fail(DPMM_ENOMEM, "max count");
+            if (dev_malloc(&dm, sizeof(int)) != hipSuccess) return fail(DPMM_ENOMEM, "max count");
             hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, dm, 1);
             hipLaunchKernelGGL(k_max_i32, dim3(1024), dim3(kThreads), 0, c->stream,
                                reinterpret_cast<const int32_t*>(c->dX), c->N * (int64_t)c->D, dm);
@@ -1706,7 +1752,7 @@ int dpmm_log_likelihoods(dpmm_ctx* c, int K, int which, int64_t first, int64_t n
     const bool f64 = c->prior == DPMM_PRIOR_GAUSSIAN && c->params_f64;
     const size_t esz = f64 ? 8 : 4;
     void* dOut = nullptr;
-    HIPCHK(hipMalloc(&dOut, (size_t)n * cols * esz));
+    HIPCHK(dev_malloc(&dOut, (size_t)n * cols * esz));
     if (c->prior == DPMM_PRIOR_GAUSSIAN) {
         const int64_t ppb = (c->DP * (int)esz > 256) ? kThreads / 2 : kThreads;
         const int grid = (int)std::min<int64_t>(4096, (n + ppb - 1) / ppb);
@@ -1744,12 +1790,16 @@ static int enqueue_stats(dpmm_ctx* c, int K) {
     const size_t E = c->statEntries();
     const int nch = chunks_upper(c, K);
     if (!(c->stats_cached && c->Kpartition == K)) {
+        const double* partials_before = c->dPartials;
+        int rc = grow(c->dPartials, c->partialsCap, (size_t)2 * nch * E);
+        if (c->dPartials != partials_before) c->partials_valid = false;   // a new buffer holds no partials
+        if (rc) return rc;
+        const double* stats_before = c->dStats;
+        rc = grow(c->dStats, c->statsCap, (size_t)2 * K * E);
+        if (c->dStats != stats_before) c->Kstats = 0;   // nor statistics
+        if (rc) return rc;
         const bool partial = c->dirty_pending && c->partials_valid && c->partialsK == K && c->Kstats == K &&
                              (int)c->dirty_host.size() == K && !t_capturing;
-        int rc = grow(c->dPartials, c->partialsCap, (size_t)2 * nch * E);
-        if (rc) return rc;
-        rc = grow(c->dStats, c->statsCap, (size_t)2 * K * E);
-        if (rc) return rc;
         c->stats_mask = nullptr;
         if (partial) {
             if ((rc = grow(c->dDirty, c->dirtyCap, (size_t)K))) return rc;
@@ -1917,7 +1967,7 @@ int dpmm_set_ground_truth(dpmm_ctx* c, const int32_t* classes, int num_classes) 
     if (!c || !classes || num_classes <= 0) return fail(DPMM_EINVAL, "bad args");
     for (int64_t i = 0; i < c->N; ++i)
         if (classes[i] < 0 || classes[i] >= num_classes) return fail(DPMM_EINVAL, "a class outside [0, num_classes)");
-    if (!c->dClasses && hipMalloc((void**)&c->dClasses, c->N * sizeof(int32_t)) != hipSuccess)
+    if (!c->dClasses && dev_malloc(&c->dClasses, c->N * sizeof(int32_t)) != hipSuccess)
         return fail(DPMM_ENOMEM, "hipMalloc failed");
     HIPCHK(hipMemcpyAsync(c->dClasses, classes, c->N * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -1933,8 +1983,8 @@ int dpmm_label_contingency(dpmm_ctx* c, int K, int64_t* table) {
     const int rows = (int)std::min<int64_t>(256, (c->N + kThreads - 1) / kThreads);
     int32_t* dPart = nullptr;
     int64_t* dTab = nullptr;
-    if (hipMalloc((void**)&dPart, (size_t)rows * KG * sizeof(int32_t)) != hipSuccess ||
-        hipMalloc((void**)&dTab, (size_t)KG * sizeof(int64_t)) != hipSuccess) {
+    if (dev_malloc(&dPart, (size_t)rows * KG * sizeof(int32_t)) != hipSuccess ||
+        dev_malloc(&dTab, (size_t)KG * sizeof(int64_t)) != hipSuccess) {
         if (dPart) (void)hipFree(dPart);
         return fail(DPMM_ENOMEM, "hipMalloc failed");
     }
@@ -2155,10 +2205,15 @@ static int make_prior(dpmm_ctx* c, double k0, const double* m0, double v0, const
         h.lg_alpha0_sum = std::lgamma(s);
         host.assign(alpha0, alpha0 + D);
     }
+    // the new prior's buffer first: a failure leaves the previous prior in place
+    double* nb = nullptr;
+    if (dev_malloc(&nb, host.size() * sizeof(double)) != hipSuccess) return fail(DPMM_ENOMEM, "prior");
+    if (hipMemcpy(nb, host.data(), host.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(nb);
+        return fail(DPMM_EHIP, "prior copy");
+    }
     if (dbuf) (void)hipFree(dbuf);
-    dbuf = nullptr;
-    if (hipMalloc((void**)&dbuf, host.size() * sizeof(double)) != hipSuccess) return fail(DPMM_ENOMEM, "prior");
-    HIPCHK(hipMemcpy(dbuf, host.data(), host.size() * sizeof(double), hipMemcpyHostToDevice));
+    dbuf = nb;
     if (c->prior == DPMM_PRIOR_GAUSSIAN) {
         h.m0 = dbuf;
         h.psi0 = dbuf + D;

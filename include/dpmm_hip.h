@@ -36,6 +36,11 @@ typedef struct dpmm_ctx dpmm_ctx;
 typedef struct dpmm_sampler dpmm_sampler;
 
 const char* dpmm_last_error(void);
+/* Test hook (no counterpart in the reference, whose device layer exits on any failure): after `after`
+ * further successful device allocations of the contexts, every allocation fails as out of memory, so
+ * that tests can check each entry point reports DPMM_ENOMEM and leaves the context usable; -1 turns the
+ * injection off. */
+int dpmm_debug_fail_allocations(int64_t after);
 int dpmm_device_count(int* count);
 
 /* cudaKernel::init (cudaKernel.cu:537-635): copies X (host) to HBM, N x D of `dtype`. */

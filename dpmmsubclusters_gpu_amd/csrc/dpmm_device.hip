@@ -750,6 +750,11 @@ struct dpmm_ctx {
     bool owns_X = false;
     int32_t* dLabels = nullptr;
     int32_t* dSubs = nullptr;
+    // sub-labels in the order of the current partition (dPerm), left by the mixed sub-label sweep for the
+    // statistics pass; valid until the labels, the sub-labels or the partition change
+    int32_t* dSubsPerm = nullptr;
+    size_t subsPermCap = 0;
+    bool subs_perm_valid = false;
     // parameters (compute type: double for FP64 precision, float otherwise)
     int K = 0, Kcap = 0, stride = 0;
     size_t capBytes = 0;   // bytes of dP (dSP holds twice as many)
@@ -969,6 +974,7 @@ int build_partition(dpmm_ctx* c, int K, int clamp) {
     c->Kpartition = K;
     c->stats_cached = false;
     c->partials_valid = false;
+    c->subs_perm_valid = false;   // positions changed
     return DPMM_OK;
 }
 
@@ -1255,7 +1261,8 @@ template <typename TX, int DP>
 void run_stats_gauss(dpmm_ctx* c) {
     const int K = c->Kpartition;
     launch_stats_gauss<TX, DP>(c->stream, chunks_upper(c, K), reinterpret_cast<const TX*>(c->dX), c->D, K, c->CH,
-                               c->dPerm, c->dLabelStart, c->dChunkStart, c->dSubs, c->dPartials, c->stats_mask);
+                               c->dPerm, c->dLabelStart, c->dChunkStart, c->dSubs,
+                               c->subs_perm_valid ? c->dSubsPerm : nullptr, c->dPartials, c->stats_mask);
 }
 
 #define DPMM_DISPATCH_DP(DPV, FN, ...)            \
@@ -1302,13 +1309,14 @@ template <int DP> void lglp_f32(dpmm_ctx* c, uint32_t it, int f, int grid) {
 template <int DP> void sgl_f32(dpmm_ctx* c, uint32_t it, int grid) {
     launch_sub_gauss_list<float, DP>(c->stream, grid, reinterpret_cast<const float*>(c->dX), c->D,
                                      reinterpret_cast<const float*>(c->dSP), c->dLabels, c->dRecheck,
-                                     c->dRecheckCount + 1, c->key0(), c->key1(), it, c->dSubs);
+                                     c->dRecheckCount + 1, c->key0(), c->key1(), it, c->dSubs, c->dPerm,
+                                     c->dSubsPerm);
 }
 template <int DPM> void sgm(dpmm_ctx* c, uint32_t it) {
     launch_sub_gauss_mx<DPM>(c->stream, chunks_upper(c, c->K), reinterpret_cast<const float*>(c->dX), c->D, c->K,
                              c->CH, c->dPerm, c->dLabelStart, c->dChunkStart,
                              reinterpret_cast<const unsigned char*>(c->dSPB), c->tau, c->key0(), c->key1(), it,
-                             c->dSubs, c->dRecheck, c->dRecheckCount + 1);
+                             c->dSubs, c->dSubsPerm, c->dRecheck, c->dRecheckCount + 1);
 }
 
 template <int DP> void score_f32(dpmm_ctx* c, int grid, int64_t first, int64_t n, int which, void* out) {
@@ -1324,6 +1332,8 @@ template <int DP> void score_f64(dpmm_ctx* c, int grid, int64_t first, int64_t n
 
 // BF16 sub-label screen over the segments, then the FP32 draw of the points it could not decide.
 int run_sub_mixed(dpmm_ctx* c, uint32_t iter) {
+    int rc = grow(c->dSubsPerm, c->subsPermCap, (size_t)c->N);
+    if (rc) return rc;
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, c->dRecheckCount + 1, 1);
     switch (c->DPM) {
         case 32: sgm<32>(c, iter); break;
@@ -1333,6 +1343,7 @@ int run_sub_mixed(dpmm_ctx* c, uint32_t iter) {
     }
     const int grid = (int)std::min<int64_t>(1024, (c->N + kThreads - 1) / kThreads);
     DPMM_DISPATCH_DP(c->DP, sgl_f32, c, iter, grid);
+    c->subs_perm_valid = true;   // every position of the partition got its sub-label in both orders
     return DPMM_OK;
 }
 
@@ -1496,7 +1507,7 @@ int dpmm_destroy(dpmm_ctx* c) {
                     (void*)c->dPrior, (void*)c->dMembers, (void*)c->dLM, (void*)c->dDraws, (void*)c->dDrawStatus,
                     (void*)c->dLogW, (void*)c->dFeat, (void*)c->dMB, c->dSPB, (void*)c->dPruneM,
                     (void*)c->dPruneSC, (void*)c->dEvals, (void*)c->dDirty, (void*)c->dClasses, (void*)c->dPruneV,
-                    (void*)c->dPrior1, (void*)c->dPsel, (void*)c->dCtl, (void*)c->dDrawsPrev})
+                    (void*)c->dPrior1, (void*)c->dPsel, (void*)c->dCtl, (void*)c->dDrawsPrev, (void*)c->dSubsPerm})
         if (p) (void)hipFree(p);
     if (c->hStats) (void)hipHostFree(c->hStats);
     if (c->hEvals) (void)hipHostFree(c->hEvals);
@@ -1560,6 +1571,7 @@ int dpmm_sample_initial_labels(dpmm_ctx* c, int K0) {
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));
     c->partition_valid = false;
+    c->subs_perm_valid = false;
     c->stats_cached = false;
     c->partials_valid = false;
     return DPMM_OK;
@@ -1570,6 +1582,7 @@ int dpmm_set_labels(dpmm_ctx* c, const int32_t* labels, const int32_t* subs) {
     if (labels) HIPCHK(hipMemcpy(c->dLabels, labels, c->N * sizeof(int32_t), hipMemcpyHostToDevice));
     if (subs) HIPCHK(hipMemcpy(c->dSubs, subs, c->N * sizeof(int32_t), hipMemcpyHostToDevice));
     c->partition_valid = false;
+    c->subs_perm_valid = false;
     c->stats_cached = false;
     c->partials_valid = false;
     return DPMM_OK;
@@ -1718,6 +1731,7 @@ int dpmm_sample_sub_labels(dpmm_ctx* c, uint32_t iter) {
     if (!t_capturing) (void)hipEventRecord(c->ev[2], c->stream);
     c->stats_cached = false;
     c->partials_valid = false;
+    c->subs_perm_valid = false;
     if (c->prior == DPMM_PRIOR_GAUSSIAN && c->precision == DPMM_PREC_MIXED && c->spb_valid) {
         int rc = run_sub_mixed(c, iter);
         if (rc) return rc;
@@ -1883,6 +1897,7 @@ static int ensure_sweep_buffers(dpmm_ctx* c) {
         if ((rc = grow(c->dPruneSC, c->pruneSCCap, (size_t)2 * K))) return rc;
         if ((rc = grow(c->dPruneV, c->pruneVCap, (size_t)kMaxK * c->DP))) return rc;
         if ((rc = grow(c->dEvals, c->evalsCap, 2))) return rc;
+        if ((rc = grow(c->dSubsPerm, c->subsPermCap, (size_t)c->N))) return rc;
     }
     return DPMM_OK;
 }
@@ -1925,6 +1940,7 @@ int dpmm_sweep_iteration(dpmm_ctx* c, uint32_t iter, int final_iter, int64_t* Ns
     // counters' copy) is cleared again.
     auto drop_recorded = [c]() {
         c->partition_valid = false;
+        c->subs_perm_valid = false;
         c->stats_cached = false;
         c->partials_valid = false;
         c->prune_valid = false;
@@ -2099,6 +2115,7 @@ int dpmm_relabel(dpmm_ctx* c, uint32_t iter, int K_old, const int32_t* new_label
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));
     if (labels_change) c->partition_valid = false;
+    c->subs_perm_valid = false;   // sub-labels were rewritten by index
     c->stats_cached = false;
     bool resets_only = !labels_change;
     for (int k = 0; k < K_old; ++k) resets_only = resets_only && (sub_action[k] == 0 || sub_action[k] == 1);

@@ -227,6 +227,7 @@ __global__ void __launch_bounds__(kThreads) k_stats_gauss_mfma(const TX* __restr
                                                                const int32_t* __restrict__ label_start,
                                                                const int32_t* __restrict__ chunk_start,
                                                                const int32_t* __restrict__ subs,
+                                                               const int32_t* __restrict__ subs_perm,
                                                                double* __restrict__ partials,
                                                                const int32_t* __restrict__ dirty) {
     typedef double f64x4 __attribute__((ext_vector_type(4)));
@@ -278,7 +279,9 @@ __global__ void __launch_bounds__(kThreads) k_stats_gauss_mfma(const TX* __restr
     // prefetch pipeline: indices two batches ahead, sub-labels and coordinates one batch ahead
     int id1 = (tid < B && beg + tid < end) ? perm[beg + tid] : -1;
     int id2 = (tid < B && beg + B + tid < end) ? perm[beg + B + tid] : -1;
-    int sb1 = id1 >= 0 ? subs[id1] : -1;
+    // sub-labels in partition order when the sub-label sweep left them (coalesced), else gathered
+    auto sub_at = [&](int pos, int id) { return id < 0 ? -1 : (subs_perm ? subs_perm[pos] : subs[id]); };
+    int sb1 = sub_at(beg + tid, id1);
     if (tid < B) idn[tid] = id1;
     __syncthreads();
     // thread tid stages coordinates 4 e4 .. 4 e4 + 3 of the batch (e4 = tid + kThreads j): point
@@ -318,7 +321,7 @@ __global__ void __launch_bounds__(kThreads) k_stats_gauss_mfma(const TX* __restr
         }
         // next batch: coordinates and sub-labels from the indices now in idn, indices after that
         load_batch();
-        sb1 = id2 >= 0 ? subs[id2] : -1;
+        sb1 = sub_at(b0 + B + tid, id2);
         id2 = (tid < B && b0 + 2 * B + tid < end) ? perm[b0 + 2 * B + tid] : -1;
         __syncthreads();
         const int n0 = nls[0], n1 = nls[1];
@@ -394,6 +397,7 @@ __global__ void __launch_bounds__(kThreads, sizeof(TX) == 4 ? 3 : 2) k_stats_gau
                                                                const int32_t* __restrict__ label_start,
                                                                const int32_t* __restrict__ chunk_start,
                                                                const int32_t* __restrict__ subs,
+                                                               const int32_t* __restrict__ subs_perm,
                                                                double* __restrict__ partials,
                                                                const int32_t* __restrict__ dirty) {
     typedef double f64x4 __attribute__((ext_vector_type(4)));
@@ -426,7 +430,9 @@ __global__ void __launch_bounds__(kThreads, sizeof(TX) == 4 ? 3 : 2) k_stats_gau
         for (int b0 = beg + 64 * wv; b0 < end; b0 += 4 * 64) {
             const int p = b0 + lane;
             const int idx = p < end ? perm[p] : 0;
-            const bool mine = p < end && subs[idx] == s;
+            // sub-labels in partition order when the sub-label sweep left them (a coalesced read that
+            // does not wait for perm), else gathered by index
+            const bool mine = p < end && (subs_perm ? subs_perm[p] : subs[idx]) == s;
             const uint64_t m = __ballot(mine);
             const int ns = __popcll(m);
             // stable partition: the sub-cluster's points first (the rest are never used)
@@ -524,7 +530,9 @@ __global__ void __launch_bounds__(kThreads) k_sub_gauss_list(const T* __restrict
                                                              const int32_t* __restrict__ list,
                                                              const int* __restrict__ list_count, uint32_t key0,
                                                              uint32_t key1, uint32_t iter,
-                                                             int32_t* __restrict__ subs) {
+                                                             int32_t* __restrict__ subs,
+                                                             const int32_t* __restrict__ perm,
+                                                             int32_t* __restrict__ subs_perm) {
     constexpr bool kPair = paired_layout<T, DP>();
     constexpr int kPts = kPair ? kThreads / 2 : kThreads;
     constexpr int kCols = kPair ? DP / 2 : DP;
@@ -535,7 +543,9 @@ __global__ void __launch_bounds__(kThreads) k_sub_gauss_list(const T* __restrict
     for (int b0 = blockIdx.x * kPts; b0 < n; b0 += gridDim.x * kPts) {   // uniform trip count per pair
         const int pos = b0 + local;
         const bool valid = pos < n;
-        const int idx = valid ? list[pos] : 0;
+        // with perm, list entries are positions in the partition (the sub-label is also written there)
+        const int ent = valid ? list[pos] : 0;
+        const int idx = perm ? perm[ent] : ent;
         const int k = valid ? labels[idx] : 0;
         T x[kCols];
 #pragma unroll
@@ -554,7 +564,9 @@ __global__ void __launch_bounds__(kThreads) k_sub_gauss_list(const T* __restrict
         }
         if (valid && par == 0) {
             const uint32_t w = philox4x32_10((uint32_t)idx, 0, iter, TAG_SUB, key0, key1).v[0];
-            subs[idx] = (logit_u32<T>(w) < sc[0] - sc[1]) ? 0 : 1;
+            const int v = (logit_u32<T>(w) < sc[0] - sc[1]) ? 0 : 1;
+            subs[idx] = v;
+            if (subs_perm) subs_perm[ent] = v;
         }
     }
 }
@@ -607,8 +619,9 @@ __global__ void __launch_bounds__(kThreads) k_stats_gauss(const TX* __restrict__
                                                           const int32_t* __restrict__ label_start,
                                                           const int32_t* __restrict__ chunk_start,
                                                           const int32_t* __restrict__ subs,
+                                                          const int32_t* __restrict__ subs_perm,
                                                           double* __restrict__ partials,
-                                                               const int32_t* __restrict__ dirty) {
+                                                          const int32_t* __restrict__ dirty) {
     constexpr int NB = DP / 4;
     constexpr int NTRI = NB * (NB + 1) / 2;
     constexpr int R = (NTRI + kThreads - 1) / kThreads;
@@ -646,7 +659,7 @@ __global__ void __launch_bounds__(kThreads) k_stats_gauss(const TX* __restrict__
             const int p = b0 + lane;
             int idx = -1;
             bool f = false;
-            if (lane < BATCH && p < end) { idx = perm[p]; f = subs[idx] == s; }
+            if (lane < BATCH && p < end) { idx = perm[p]; f = (subs_perm ? subs_perm[p] : subs[idx]) == s; }
             const uint64_t m = __ballot(f);
             if (f) idl[__popcll(m & lt)] = idx;
             if (lane == 0) nsel = __popcll(m);
@@ -724,9 +737,9 @@ void launch_label_gauss_listp(hipStream_t st, int grid, const T* X, int64_t N, i
 template <typename T, int DP>
 void launch_sub_gauss_list(hipStream_t st, int grid, const T* X, int D, const T* SP, const int32_t* labels,
                            const int32_t* list, const int* count, uint32_t key0, uint32_t key1, uint32_t iter,
-                           int32_t* subs) {
+                           int32_t* subs, const int32_t* perm, int32_t* subs_perm) {
     hipLaunchKernelGGL((k_sub_gauss_list<T, DP>), dim3(grid), dim3(kThreads), 0, st, X, D, SP, labels, list, count,
-                       key0, key1, iter, subs);
+                       key0, key1, iter, subs, perm, subs_perm);
 }
 
 template <typename T, int DP>
@@ -749,16 +762,16 @@ void launch_score_gauss(hipStream_t st, int grid, const T* X, int D, int K, cons
 template <typename TX, int DP>
 void launch_stats_gauss(hipStream_t st, int nchunks, const TX* X, int D, int K, int CH, const int32_t* perm,
                         const int32_t* label_start, const int32_t* chunk_start, const int32_t* subs,
-                        double* partials, const int32_t* dirty) {
+                        const int32_t* subs_perm, double* partials, const int32_t* dirty) {
     if constexpr (DP % 16 == 0 && DP <= 64)
         hipLaunchKernelGGL((k_stats_gauss_wave<TX, DP>), dim3(nchunks), dim3(kThreads), 0, st, X, D, K, CH, perm,
-                           label_start, chunk_start, subs, partials, dirty);
+                           label_start, chunk_start, subs, subs_perm, partials, dirty);
     else if constexpr (DP % 16 == 0)
         hipLaunchKernelGGL((k_stats_gauss_mfma<TX, DP>), dim3(nchunks), dim3(kThreads), 0, st, X, D, K, CH, perm,
-                           label_start, chunk_start, subs, partials, dirty);
+                           label_start, chunk_start, subs, subs_perm, partials, dirty);
     else
         hipLaunchKernelGGL((k_stats_gauss<TX, DP>), dim3(2 * nchunks), dim3(kThreads), 0, st, X, D, K, CH, perm,
-                           label_start, chunk_start, subs, partials, dirty);
+                           label_start, chunk_start, subs, subs_perm, partials, dirty);
 }
 
 #define DPMM_GAUSS_INSTANTIATE(T, DP)                                                                         \
@@ -771,11 +784,11 @@ void launch_stats_gauss(hipStream_t st, int nchunks, const TX* X, int D, int K, 
                                           const int32_t*, const int32_t*, const T*, uint32_t, uint32_t,        \
                                           uint32_t, int32_t*);                                                 \
     template void launch_stats_gauss<T, DP>(hipStream_t, int, const T*, int, int, int, const int32_t*,         \
-                                            const int32_t*, const int32_t*, const int32_t*, double*,           \
-                                            const int32_t*);                                                   \
+                                            const int32_t*, const int32_t*, const int32_t*, const int32_t*,    \
+                                            double*, const int32_t*);                                          \
     template void launch_sub_gauss_list<T, DP>(hipStream_t, int, const T*, int, const T*, const int32_t*,      \
                                                const int32_t*, const int*, uint32_t, uint32_t, uint32_t,       \
-                                               int32_t*);                                                      \
+                                               int32_t*, const int32_t*, int32_t*);                            \
     template void launch_score_gauss<T, DP>(hipStream_t, int, const T*, int, int, const T*, const T*,          \
                                             const int32_t*, int64_t, int64_t, int, T*);                        \
     template void launch_label_gauss_listp<T, DP>(hipStream_t, int, const T*, int64_t, int, int, const T*,     \
@@ -792,11 +805,11 @@ void launch_stats_gauss(hipStream_t st, int nchunks, const TX* X, int D, int K, 
                                                  const int32_t*, const int32_t*, const T*, uint32_t, uint32_t, \
                                                  uint32_t, int32_t*);                                          \
     extern template void launch_stats_gauss<T, DP>(hipStream_t, int, const T*, int, int, int, const int32_t*,  \
-                                                   const int32_t*, const int32_t*, const int32_t*, double*,    \
-                                                   const int32_t*);                                            \
+                                                   const int32_t*, const int32_t*, const int32_t*,             \
+                                                   const int32_t*, double*, const int32_t*);                   \
     extern template void launch_sub_gauss_list<T, DP>(hipStream_t, int, const T*, int, const T*,               \
                                                       const int32_t*, const int32_t*, const int*, uint32_t,    \
-                                                      uint32_t, uint32_t, int32_t*);                           \
+                                                      uint32_t, uint32_t, int32_t*, const int32_t*, int32_t*); \
     extern template void launch_score_gauss<T, DP>(hipStream_t, int, const T*, int, int, const T*, const T*,   \
                                                    const int32_t*, int64_t, int64_t, int, T*);                 \
     extern template void launch_label_gauss_listp<T, DP>(hipStream_t, int, const T*, int64_t, int, int,        \

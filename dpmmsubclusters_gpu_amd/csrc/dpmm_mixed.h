@@ -257,7 +257,8 @@ __global__ void __launch_bounds__(kThreads) k_sub_gauss_mx(const float* __restri
                                                            const int32_t* __restrict__ chunk_start,
                                                            const unsigned char* __restrict__ SPB, float tau,
                                                            uint32_t key0, uint32_t key1, uint32_t iter,
-                                                           int32_t* __restrict__ subs, int32_t* __restrict__ recheck,
+                                                           int32_t* __restrict__ subs, int32_t* __restrict__ subs_perm,
+                                                           int32_t* __restrict__ recheck,
                                                            int* __restrict__ recheck_count) {
     typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
     constexpr int RB = DP / 32, KC = DP / 16;
@@ -351,10 +352,15 @@ __global__ void __launch_bounds__(kThreads) k_sub_gauss_mx(const float* __restri
         bool need = false;
         if (valid && h == 0) {
             const float lg = logit_u32<float>(philox4x32_10((uint32_t)idx, 0, iter, TAG_SUB, key0, key1).v[0]);
-            if (fabsf(lg - d) > err) subs[idx] = lg < d ? 0 : 1;
-            else need = true;
+            if (fabsf(lg - d) > err) {
+                const int v = lg < d ? 0 : 1;
+                subs[idx] = v;
+                subs_perm[p] = v;   // partition order, for the statistics pass (coalesced)
+            } else {
+                need = true;
+            }
         }
-        rl.push(need, idx, recheck, recheck_count, lane, lt);
+        rl.push(need, p, recheck, recheck_count, lane, lt);   // positions: the recheck writes both orders
     }
     rl.flush(recheck, recheck_count, lane);
 }
@@ -362,10 +368,10 @@ __global__ void __launch_bounds__(kThreads) k_sub_gauss_mx(const float* __restri
 template <int DP>
 void launch_sub_gauss_mx(hipStream_t st, int grid, const float* X, int D, int K, int CH, const int32_t* perm,
                          const int32_t* label_start, const int32_t* chunk_start, const unsigned char* SPB, float tau,
-                         uint32_t key0, uint32_t key1, uint32_t iter, int32_t* subs, int32_t* recheck,
-                         int* recheck_count) {
+                         uint32_t key0, uint32_t key1, uint32_t iter, int32_t* subs, int32_t* subs_perm,
+                         int32_t* recheck, int* recheck_count) {
     hipLaunchKernelGGL((k_sub_gauss_mx<DP>), dim3(grid), dim3(kThreads), 0, st, X, D, K, CH, perm, label_start,
-                       chunk_start, SPB, tau, key0, key1, iter, subs, recheck, recheck_count);
+                       chunk_start, SPB, tau, key0, key1, iter, subs, subs_perm, recheck, recheck_count);
 }
 
 // Exact pruning of the mixed label sweep.  The points are visited in the label partition of the
@@ -586,7 +592,7 @@ void launch_label_gauss_mxp(hipStream_t st, int grid, const float* X, int D, int
                                             float, uint32_t, uint32_t, uint32_t, int, int32_t*, int32_t*, int*); \
     template void launch_sub_gauss_mx<DP>(hipStream_t, int, const float*, int, int, int, const int32_t*,       \
                                           const int32_t*, const int32_t*, const unsigned char*, float, uint32_t, \
-                                          uint32_t, uint32_t, int32_t*, int32_t*, int*);                       \
+                                          uint32_t, uint32_t, int32_t*, int32_t*, int32_t*, int*);             \
     template void launch_label_gauss_mxp<DP>(hipStream_t, int, const float*, int, int, int, const int32_t*,   \
                                              const int32_t*, const int32_t*, const unsigned char*, const float*, \
                                              const float*, float, uint32_t, uint32_t, uint32_t, int, int32_t*,   \
@@ -597,7 +603,8 @@ void launch_label_gauss_mxp(hipStream_t st, int grid, const float* X, int D, int
                                                    int, int32_t*, int32_t*, int*);                             \
     extern template void launch_sub_gauss_mx<DP>(hipStream_t, int, const float*, int, int, int, const int32_t*, \
                                                  const int32_t*, const int32_t*, const unsigned char*, float,   \
-                                                 uint32_t, uint32_t, uint32_t, int32_t*, int32_t*, int*);       \
+                                                 uint32_t, uint32_t, uint32_t, int32_t*, int32_t*, int32_t*,    \
+                                                 int*);                                                         \
     extern template void launch_label_gauss_mxp<DP>(hipStream_t, int, const float*, int, int, int,             \
                                                     const int32_t*, const int32_t*, const int32_t*,            \
                                                     const unsigned char*, const float*, const float*, float,   \

@@ -53,7 +53,7 @@ class _Options(ctypes.Structure):
                 ("split_stop", ctypes.c_int), ("max_clusters", ctypes.c_double), ("seed", ctypes.c_uint64),
                 ("full_mu_cov", ctypes.c_int), ("mult_use_weight", ctypes.c_int), ("precision", ctypes.c_int),
                 ("verbose", ctypes.c_int), ("init_from_labels", ctypes.c_int), ("hard_clustering", ctypes.c_int),
-                ("outlier_mod", ctypes.c_double)]
+                ("outlier_mod", ctypes.c_double), ("host_decisions", ctypes.c_int)]
 
 
 _lib = None
@@ -123,6 +123,7 @@ def load_library():
         "dpmm_screener_forward": [_P, i, _P, _P],
         "dpmm_sampler_set_screener": [_P, _P, i, ctypes.c_float, i],
         "dpmm_sampler_screen_stats": [_P, _P, _P, _P, _P],
+        "dpmm_sampler_events": [_P, _P, _P, _P, _P],
         "dpmm_sampler_pair_log": [_P, i, _P, _P, _P, _P],
         "dpmm_sampler_create": [_P, _P, dbl, _P, dbl, _P, _P, _P],
         "dpmm_sampler_step": [_P, i],
@@ -453,11 +454,13 @@ class Sampler:
 
     def __init__(self, dev, hyper, alpha, iters, init_clusters=1, burnout=15, seed=0, max_clusters=float("inf"),
                  precision=None, full_mu_cov=False, mult_use_weight=False, verbose=False, init_labels=None,
-                 hard_clustering=False, outlier_weight=0.0, outlier_params=None):
+                 hard_clustering=False, outlier_weight=0.0, outlier_params=None, host_decisions=False):
         """init_labels (N, 0-based) starts from a given clustering, e.g. an over-split one;
         hard_clustering takes the argmax in every label sweep (global_params.h:38); outlier_weight > 0
         with outlier_params (a hyperparameter object of the prior's kind) runs the reference's outlier
-        mode, cluster 0 being the outlier cluster (dp_parallel_sampling.cpp:68-72)."""
+        mode, cluster 0 being the outlier cluster (dp_parallel_sampling.cpp:68-72).  host_decisions keeps
+        the reset / split / merge decisions on the host between device launches instead of running each
+        iteration on the device with one synchronisation (both take the same decisions)."""
         L = load_library()
         self.dev = dev
         if outlier_weight > 0:
@@ -476,7 +479,7 @@ class Sampler:
                              full_mu_cov=int(full_mu_cov), mult_use_weight=int(mult_use_weight),
                              precision=int(precision), verbose=int(verbose),
                              init_from_labels=int(init_labels is not None), hard_clustering=int(bool(hard_clustering)),
-                             outlier_mod=float(outlier_weight))
+                             outlier_mod=float(outlier_weight), host_decisions=int(bool(host_decisions)))
         self._s = ctypes.c_void_p()
         if dev.prior == prior_type.Gaussian:
             self._m0, self._psi0 = _f64(hyper.m), _f64(hyper.psi)
@@ -544,6 +547,12 @@ class Sampler:
         h, u = np.empty(n.value), np.empty(n.value)
         _check(load_library().dpmm_sampler_pair_log(self._s, n.value, _ptr(f), _ptr(h), _ptr(u), ctypes.byref(n)))
         return f, h, u
+
+    def events(self):
+        """Decisions since creation: {"splits", "merges", "resets", "removed"}."""
+        v = [ctypes.c_int64() for _ in range(4)]
+        _check(load_library().dpmm_sampler_events(self._s, *[ctypes.byref(x) for x in v]))
+        return dict(zip(("splits", "merges", "resets", "removed"), (x.value for x in v)))
 
     def close(self):
         if self._s:

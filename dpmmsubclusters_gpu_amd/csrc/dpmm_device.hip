@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -28,6 +29,7 @@
 #include "dpmm_mixed.h"
 #include "dpmm_niw.h"
 #include "dpmm_mult.h"
+#include "dpmm_step.h"
 
 using namespace dpmm;
 
@@ -873,6 +875,25 @@ struct dpmm_ctx {
     int mm_ncb = 0;        // 0: no tile block for the current parameters
     int max_count = -1;    // largest count in X (multinomial), -1 until measured
     int Kdraw = 0;
+    // device-resident sampler iteration (dpmm_step.h): the cluster table, its history rings (step_B
+    // doubles per cluster) and their compaction copies, sized for kMaxK clusters; the log-ML job list and
+    // the accepted pairs grow with K
+    StepCluster* dStepCl = nullptr;
+    StepCluster* dStepClTmp = nullptr;
+    double* dStepHist = nullptr;
+    double* dStepHistTmp = nullptr;
+    int step_B = 0;
+    double* dStepW = nullptr;         // mixture weights of the last draw
+    int32_t* dStepMem = nullptr;      // statistics entries of the draw's 3K parts
+    int32_t* dStepCand = nullptr;     // merge candidates, then the touched clusters
+    int32_t* dStepCounts = nullptr;   // jobs, candidates
+    int32_t* dStepTab = nullptr;      // relabel tables: resets + splits (3 kMaxK), merges + removals (3 kMaxK), aux (2 kMaxK)
+    int32_t* dStepJobs = nullptr;
+    size_t stepJobsCap = 0;
+    int32_t* dStepAcc = nullptr;
+    size_t stepAccCap = 0;
+    StepSummary* dStepSum = nullptr;
+    StepSummary* hStepSum = nullptr;  // pinned
     int drawStride() const {
         return prior == DPMM_PRIOR_GAUSSIAN ? ((D + D * (D + 1) / 2 + 1) + 1) & ~1 : (D + 1) & ~1;
     }
@@ -1507,8 +1528,12 @@ int dpmm_destroy(dpmm_ctx* c) {
                     (void*)c->dPrior, (void*)c->dMembers, (void*)c->dLM, (void*)c->dDraws, (void*)c->dDrawStatus,
                     (void*)c->dLogW, (void*)c->dFeat, (void*)c->dMB, c->dSPB, (void*)c->dPruneM,
                     (void*)c->dPruneSC, (void*)c->dEvals, (void*)c->dDirty, (void*)c->dClasses, (void*)c->dPruneV,
-                    (void*)c->dPrior1, (void*)c->dPsel, (void*)c->dCtl, (void*)c->dDrawsPrev, (void*)c->dSubsPerm})
+                    (void*)c->dPrior1, (void*)c->dPsel, (void*)c->dCtl, (void*)c->dDrawsPrev, (void*)c->dSubsPerm,
+                    (void*)c->dStepCl, (void*)c->dStepClTmp, (void*)c->dStepHist, (void*)c->dStepHistTmp,
+                    (void*)c->dStepW, (void*)c->dStepMem, (void*)c->dStepCand, (void*)c->dStepCounts,
+                    (void*)c->dStepTab, (void*)c->dStepJobs, (void*)c->dStepAcc, (void*)c->dStepSum})
         if (p) (void)hipFree(p);
+    if (c->hStepSum) (void)hipHostFree(c->hStepSum);
     if (c->hStats) (void)hipHostFree(c->hStats);
     if (c->hEvals) (void)hipHostFree(c->hEvals);
     for (void* p : {c->pinMembers, c->pinLogW, c->pinOut})
@@ -1902,36 +1927,14 @@ static int ensure_sweep_buffers(dpmm_ctx* c) {
     return DPMM_OK;
 }
 
-int dpmm_set_graph_mode(dpmm_ctx* c, int enable) {
-    if (!c) return fail(DPMM_EINVAL, "null ctx");
-    c->graph_mode = enable != 0;
-    return DPMM_OK;
-}
-
-int dpmm_graph_counts(dpmm_ctx* c, int64_t* instantiations, int64_t* updates) {
-    if (!c) return fail(DPMM_EINVAL, "null ctx");
-    if (instantiations) *instantiations = c->graph_instantiations;
-    if (updates) *updates = c->graph_updates;
-    return DPMM_OK;
-}
-
-int dpmm_sweep_iteration(dpmm_ctx* c, uint32_t iter, int final_iter, int64_t* Nsub) {
-    if (!c || !Nsub) return fail(DPMM_EINVAL, "bad args");
-    if (c->K <= 0) return fail(DPMM_ESTATE, "parameters not set");
-    int rc;
-    if (!c->graph_mode) {
-        if ((rc = dpmm_sample_cluster_labels(c, iter, final_iter))) return rc;
-        if ((rc = dpmm_sample_sub_labels(c, iter))) return rc;
-        return dpmm_sufficient_statistics(c, c->K, Nsub, nullptr, nullptr);
-    }
-    if ((rc = resolve_evals(c))) return rc;   // the adaptive pruning decides from these, on the host
-    if ((rc = ensure_sweep_buffers(c))) return rc;
+// Records what `record` enqueues into a hipGraph and launches it through the context's executable
+// graph, updated in place while the recorded topology stays the same.  Nothing may allocate,
+// synchronise or record timing events meanwhile.
+static int run_captured(dpmm_ctx* c, const std::function<int()>& record) {
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
     t_capturing = true;
-    rc = dpmm_sample_cluster_labels(c, iter, final_iter);
-    if (!rc) rc = dpmm_sample_sub_labels(c, iter);
-    if (!rc) rc = enqueue_stats(c, c->K);
+    int rc = record();
     t_capturing = false;
     hipGraph_t g = nullptr;
     const hipError_t ec = hipStreamEndCapture(c->stream, &g);
@@ -1976,6 +1979,40 @@ int dpmm_sweep_iteration(dpmm_ctx* c, uint32_t iter, int final_iter, int64_t* Ns
     else ++c->graph_instantiations;
     HIPCHK(hipGraphLaunch(c->gexec, c->stream));
     if (c->evals_copy_due && (rc = copy_evals_back(c))) return rc;
+    return DPMM_OK;
+}
+
+int dpmm_set_graph_mode(dpmm_ctx* c, int enable) {
+    if (!c) return fail(DPMM_EINVAL, "null ctx");
+    c->graph_mode = enable != 0;
+    return DPMM_OK;
+}
+
+int dpmm_graph_counts(dpmm_ctx* c, int64_t* instantiations, int64_t* updates) {
+    if (!c) return fail(DPMM_EINVAL, "null ctx");
+    if (instantiations) *instantiations = c->graph_instantiations;
+    if (updates) *updates = c->graph_updates;
+    return DPMM_OK;
+}
+
+int dpmm_sweep_iteration(dpmm_ctx* c, uint32_t iter, int final_iter, int64_t* Nsub) {
+    if (!c || !Nsub) return fail(DPMM_EINVAL, "bad args");
+    if (c->K <= 0) return fail(DPMM_ESTATE, "parameters not set");
+    int rc;
+    if (!c->graph_mode) {
+        if ((rc = dpmm_sample_cluster_labels(c, iter, final_iter))) return rc;
+        if ((rc = dpmm_sample_sub_labels(c, iter))) return rc;
+        return dpmm_sufficient_statistics(c, c->K, Nsub, nullptr, nullptr);
+    }
+    if ((rc = resolve_evals(c))) return rc;   // the adaptive pruning decides from these, on the host
+    if ((rc = ensure_sweep_buffers(c))) return rc;
+    rc = run_captured(c, [&]() {
+        int r = dpmm_sample_cluster_labels(c, iter, final_iter);
+        if (!r) r = dpmm_sample_sub_labels(c, iter);
+        if (!r) r = enqueue_stats(c, c->K);
+        return r;
+    });
+    if (rc) return rc;
     return dpmm_sufficient_statistics(c, c->K, Nsub, nullptr, nullptr);   // the counts (statistics are current)
 }
 
@@ -2355,7 +2392,7 @@ int dpmm_log_marginal_batch_ex(dpmm_ctx* c, int n, const int32_t* members, const
     if (rc) return rc;
     hipLaunchKernelGGL(k_log_marginal, dim3(n), dim3(kThreads), lds, c->stream, c->dStats, (int)c->statEntries(),
                        c->DP, D, c->prior, c->dMembers, dsel, c->prior_args, dsel ? c->prior_args1 : c->prior_args,
-                       c->dLM);
+                       c->dLM, nullptr);
     HIPCHK(hipGetLastError());
     if ((rc = pin_reserve(c->pinOut, c->pinOutCap, (size_t)2 * n * sizeof(double)))) return rc;
     const double* out = static_cast<const double*>(c->pinOut);
@@ -2368,59 +2405,21 @@ int dpmm_log_marginal_batch_ex(dpmm_ctx* c, int n, const int32_t* members, const
     return DPMM_OK;
 }
 
-int dpmm_sample_params(dpmm_ctx* c, uint32_t iter, int K, const int32_t* members, const double* weights,
-                       const double* lr_weights, int full_mu_cov, int use_weight) {
-    return dpmm_sample_params_ex(c, iter, K, members, nullptr, weights, lr_weights, full_mu_cov, use_weight);
-}
-
-int dpmm_sample_params_ex(dpmm_ctx* c, uint32_t iter, int K, const int32_t* members, const int32_t* ctl,
-                          const double* weights, const double* lr_weights, int full_mu_cov, int use_weight) {
-    if (!c || K <= 0 || !members || !weights || !lr_weights) return fail(DPMM_EINVAL, "bad args");
-    if (K > kMaxK) return fail(DPMM_EINVAL, "K exceeds 2048");
-    if (!c->has_prior) return fail(DPMM_ESTATE, "prior not set");
-    if (c->Kstats <= 0) return fail(DPMM_ESTATE, "no statistics computed yet");
+// The parameter draw of 3K parts and the packing of the sweeps' parameter slots, enqueued without
+// a synchronisation.  dmem (4 statistics entries per part), the draw control dctl (or null) and the
+// log weights at c->dLogW (K mixture weights, then 2K sub-cluster weights) are on the device.
+static int enqueue_draws(dpmm_ctx* c, uint32_t iter, int K, const int32_t* dmem, const int32_t* dctl,
+                         int full_mu_cov, int use_weight) {
     const int D = c->D, np = 3 * K;
-    int rc = upload_members(c, np, members);
-    if (rc) return rc;
-    const int dstride0 = c->drawStride();
-    const int32_t* dctl = nullptr;
-    if (ctl) {
-        bool keep = false;
-        for (int p = 0; p < np; ++p) {
-            const int32_t* q = ctl + 4 * p;
-            if (q[0] >= 3 * c->Kdraw || q[0] < -1) return fail(DPMM_EINVAL, "kept row outside the previous draws");
-            if (q[1] < -1 || q[2] < -1 || q[3] < 0 || q[3] > 1) return fail(DPMM_EINVAL, "bad draw control");
-            if (q[0] < 0 && q[3] == 1 && !c->has_prior1) return fail(DPMM_ESTATE, "outlier prior not set");
-            keep |= q[0] >= 0;
-        }
-        if (keep) {   // the previous rows, before the draw buffer is reused (or reallocated) below
-            if ((rc = grow(c->dDrawsPrev, c->drawsPrevCap, (size_t)3 * c->Kdraw * dstride0))) return rc;
-            HIPCHK(hipMemcpyAsync(c->dDrawsPrev, c->dDraws, (size_t)3 * c->Kdraw * dstride0 * sizeof(double),
-                                  hipMemcpyDeviceToDevice, c->stream));
-        }
-        if ((rc = grow(c->dCtl, c->ctlCap, (size_t)4 * np))) return rc;
-        HIPCHK(hipMemcpyAsync(c->dCtl, ctl, (size_t)4 * np * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
-        dctl = c->dCtl;
-    }
-    if ((rc = pin_reserve(c->pinLogW, c->pinLogWCap, (size_t)3 * K * sizeof(double)))) return rc;
-    double* logw = static_cast<double*>(c->pinLogW);
-    for (int k = 0; k < K; ++k) {
-        if (!(weights[k] >= 0.0) || !(lr_weights[2 * k] >= 0.0) || !(lr_weights[2 * k + 1] >= 0.0))
-            return fail(DPMM_EINVAL, "weights must be non-negative");
-        logw[k] = std::log(weights[k]);
-        logw[K + 2 * k] = std::log(lr_weights[2 * k]);
-        logw[K + 2 * k + 1] = std::log(lr_weights[2 * k + 1]);
-    }
     const int dstride = c->drawStride();
-    if ((rc = grow(c->dLogW, c->logwCap, (size_t)3 * K))) return rc;
+    int rc;
     if ((rc = grow(c->dDraws, c->drawsCap, (size_t)np * dstride))) return rc;
     if ((rc = grow(c->dDrawStatus, c->drawStatusCap, (size_t)np))) return rc;
-    HIPCHK(hipMemcpyAsync(c->dLogW, logw, (size_t)3 * K * sizeof(double), hipMemcpyHostToDevice, c->stream));
     const bool gauss = c->prior == DPMM_PRIOR_GAUSSIAN;
     const size_t lds = gauss ? lds_bytes((size_t)D * (D + 1) + 3 * D) : lds_bytes(D);
     if ((rc = allow_lds(reinterpret_cast<const void*>(k_draw_params), lds))) return rc;
     hipLaunchKernelGGL(k_draw_params, dim3(np), dim3(kThreads), lds, c->stream, c->dStats, (int)c->statEntries(),
-                       c->DP, D, c->prior, c->dMembers, dctl, c->dDrawsPrev, c->prior_args,
+                       c->DP, D, c->prior, dmem, dctl, c->dDrawsPrev, c->prior_args,
                        c->has_prior1 ? c->prior_args1 : c->prior_args, iter, c->key0(), c->key1(), full_mu_cov,
                        dstride, c->dDraws, c->dDrawStatus);
     HIPCHK(hipGetLastError());
@@ -2461,6 +2460,55 @@ int dpmm_sample_params_ex(dpmm_ctx* c, uint32_t iter, int K, const int32_t* memb
             return rc;
     }
     HIPCHK(hipGetLastError());
+    return DPMM_OK;
+}
+
+int dpmm_sample_params(dpmm_ctx* c, uint32_t iter, int K, const int32_t* members, const double* weights,
+                       const double* lr_weights, int full_mu_cov, int use_weight) {
+    return dpmm_sample_params_ex(c, iter, K, members, nullptr, weights, lr_weights, full_mu_cov, use_weight);
+}
+
+int dpmm_sample_params_ex(dpmm_ctx* c, uint32_t iter, int K, const int32_t* members, const int32_t* ctl,
+                          const double* weights, const double* lr_weights, int full_mu_cov, int use_weight) {
+    if (!c || K <= 0 || !members || !weights || !lr_weights) return fail(DPMM_EINVAL, "bad args");
+    if (K > kMaxK) return fail(DPMM_EINVAL, "K exceeds 2048");
+    if (!c->has_prior) return fail(DPMM_ESTATE, "prior not set");
+    if (c->Kstats <= 0) return fail(DPMM_ESTATE, "no statistics computed yet");
+    const int np = 3 * K;
+    int rc = upload_members(c, np, members);
+    if (rc) return rc;
+    const int dstride0 = c->drawStride();
+    const int32_t* dctl = nullptr;
+    if (ctl) {
+        bool keep = false;
+        for (int p = 0; p < np; ++p) {
+            const int32_t* q = ctl + 4 * p;
+            if (q[0] >= 3 * c->Kdraw || q[0] < -1) return fail(DPMM_EINVAL, "kept row outside the previous draws");
+            if (q[1] < -1 || q[2] < -1 || q[3] < 0 || q[3] > 1) return fail(DPMM_EINVAL, "bad draw control");
+            if (q[0] < 0 && q[3] == 1 && !c->has_prior1) return fail(DPMM_ESTATE, "outlier prior not set");
+            keep |= q[0] >= 0;
+        }
+        if (keep) {   // the previous rows, before the draw buffer is reused (or reallocated) below
+            if ((rc = grow(c->dDrawsPrev, c->drawsPrevCap, (size_t)3 * c->Kdraw * dstride0))) return rc;
+            HIPCHK(hipMemcpyAsync(c->dDrawsPrev, c->dDraws, (size_t)3 * c->Kdraw * dstride0 * sizeof(double),
+                                  hipMemcpyDeviceToDevice, c->stream));
+        }
+        if ((rc = grow(c->dCtl, c->ctlCap, (size_t)4 * np))) return rc;
+        HIPCHK(hipMemcpyAsync(c->dCtl, ctl, (size_t)4 * np * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+        dctl = c->dCtl;
+    }
+    if ((rc = pin_reserve(c->pinLogW, c->pinLogWCap, (size_t)3 * K * sizeof(double)))) return rc;
+    double* logw = static_cast<double*>(c->pinLogW);
+    for (int k = 0; k < K; ++k) {
+        if (!(weights[k] >= 0.0) || !(lr_weights[2 * k] >= 0.0) || !(lr_weights[2 * k + 1] >= 0.0))
+            return fail(DPMM_EINVAL, "weights must be non-negative");
+        logw[k] = std::log(weights[k]);
+        logw[K + 2 * k] = std::log(lr_weights[2 * k]);
+        logw[K + 2 * k + 1] = std::log(lr_weights[2 * k + 1]);
+    }
+    if ((rc = grow(c->dLogW, c->logwCap, (size_t)3 * K))) return rc;
+    HIPCHK(hipMemcpyAsync(c->dLogW, logw, (size_t)3 * K * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if ((rc = enqueue_draws(c, iter, K, c->dMembers, dctl, full_mu_cov, use_weight))) return rc;
     if ((rc = pin_reserve(c->pinOut, c->pinOutCap, np * sizeof(int32_t)))) return rc;
     const int32_t* st = static_cast<const int32_t*>(c->pinOut);
     HIPCHK(hipMemcpyAsync(c->pinOut, c->dDrawStatus, np * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
@@ -2531,6 +2579,225 @@ int dpmm_get_drawn_params(dpmm_ctx* c, double* mu, double* chol_inv_sigma, doubl
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- device-resident iteration ----
+namespace {
+
+int grid_relabel(const dpmm_ctx* c) { return (int)std::min<int64_t>((c->N + kThreads - 1) / kThreads, 1024); }
+
+// Every cache derived from the labels or sub-labels, after a relabel pass on the device.
+void labels_rewritten(dpmm_ctx* c) {
+    c->partition_valid = false;
+    c->subs_perm_valid = false;
+    c->stats_cached = false;
+    c->partials_valid = false;
+    c->dirty_pending = false;
+}
+
+// k_log_marginal over the device-built job list jobs (4 statistics entries each), the grid sized for
+// `grid` jobs and the count read from njobs; results (log ML, log det psi_n) at c->dLM.
+int launch_log_marginal_dev(dpmm_ctx* c, int grid, const int32_t* jobs, const int32_t* njobs) {
+    const int D = c->D;
+    const size_t lds = c->prior == DPMM_PRIOR_GAUSSIAN ? lds_bytes((size_t)D * (D + 1) / 2 + D) : lds_bytes(0);
+    int rc = allow_lds(reinterpret_cast<const void*>(k_log_marginal), lds);
+    if (rc) return rc;
+    if (grid <= 0) return DPMM_OK;
+    hipLaunchKernelGGL(k_log_marginal, dim3(grid), dim3(kThreads), lds, c->stream, c->dStats, (int)c->statEntries(),
+                       c->DP, D, c->prior, jobs, nullptr, c->prior_args, c->prior_args, c->dLM, njobs);
+    HIPCHK(hipGetLastError());
+    return DPMM_OK;
+}
+
+// What enqueue_draws grows for K clusters, grown ahead of a capture.
+int ensure_draw_buffers(dpmm_ctx* c, int K) {
+    const int np = 3 * K, D = c->D;
+    int rc;
+    if ((rc = grow(c->dDraws, c->drawsCap, (size_t)np * c->drawStride()))) return rc;
+    if ((rc = grow(c->dDrawStatus, c->drawStatusCap, (size_t)np))) return rc;
+    if (c->prior == DPMM_PRIOR_GAUSSIAN) {
+        const size_t esz = c->precision == DPMM_PREC_FP64 ? 8 : 4;
+        if ((rc = ensure_param_buffers(c, K, (size_t)gauss_slot_stride(c->DP) * esz))) return rc;
+        if (c->precision == DPMM_PREC_MIXED && (rc = ensure_mixed_buffers(c, K))) return rc;
+    } else {
+        if ((rc = ensure_param_buffers(c, K, (size_t)((D + 1 + 3) & ~3) * 4))) return rc;
+        if (c->precision == DPMM_PREC_MIXED && K <= 256) {
+            if ((rc = grow(c->dMB, c->mbCap, mm_block_bytes(D, (K + 31) / 32)))) return rc;
+            if ((rc = ensure_recheck(c))) return rc;
+        }
+    }
+    return DPMM_OK;
+}
+
+// One iteration from the parameter draw to the summary copy, enqueued without a synchronisation.
+int enqueue_step(dpmm_ctx* c, const StepArgs& a, int sweep_final, int full_mu_cov, int use_weight, int grid_jobs,
+                 int32_t* trace_labels, int32_t* trace_subs) {
+    const int K = a.K;
+    hipLaunchKernelGGL(k_step_sample, dim3(1), dim3(kStepThreads), 0, c->stream, a, c->dStepCl, c->dStepHist,
+                       c->dStepW, c->dStepMem, c->dLogW);
+    HIPCHK(hipGetLastError());
+    int rc = enqueue_draws(c, a.t, K, c->dStepMem, nullptr, full_mu_cov, use_weight);
+    if (rc) return rc;
+    c->Kdraw = K;
+    c->K = K;
+    c->prune_valid = false;
+    if ((rc = dpmm_sample_cluster_labels(c, a.t, sweep_final))) return rc;
+    if ((rc = dpmm_sample_sub_labels(c, a.t))) return rc;
+    if ((rc = enqueue_stats(c, K))) return rc;
+    if (trace_labels)
+        HIPCHK(hipMemcpyAsync(trace_labels, c->dLabels, (size_t)c->N * 4, hipMemcpyDeviceToHost, c->stream));
+    if (trace_subs) HIPCHK(hipMemcpyAsync(trace_subs, c->dSubs, (size_t)c->N * 4, hipMemcpyDeviceToHost, c->stream));
+    hipLaunchKernelGGL(k_step_jobs, dim3(1), dim3(kStepThreads), 0, c->stream, a, c->dStepCl, c->dStats,
+                       c->dStepCand, c->dStepJobs, c->dStepCounts);
+    if ((rc = launch_log_marginal_dev(c, grid_jobs, c->dStepJobs, c->dStepCounts))) return rc;
+    int32_t* tab1 = c->dStepTab;
+    int32_t* tab2 = tab1 + 3 * kMaxK;
+    int32_t* aux = tab2 + 3 * kMaxK;
+    hipLaunchKernelGGL(k_step_decide, dim3(1), dim3(kStepThreads), 0, c->stream, a, c->dStepCl, c->dLM, c->dStepCand,
+                       c->dStepCounts, c->dDrawStatus, 3 * K, c->dStepAcc, tab1, tab2, aux, c->dStepSum);
+    hipLaunchKernelGGL(k_step_relabel, dim3(grid_relabel(c)), dim3(kThreads), 0, c->stream, c->dLabels, c->dSubs,
+                       c->N, tab1, K, c->dStepSum, 1, c->key0(), c->key1(), a.t);
+    hipLaunchKernelGGL(k_step_relabel, dim3(grid_relabel(c)), dim3(kThreads), 0, c->stream, c->dLabels, c->dSubs,
+                       c->N, tab2, 0, c->dStepSum, 2, c->key0(), c->key1(), a.t);
+    hipLaunchKernelGGL(k_step_compact, dim3(1), dim3(kStepThreads), 0, c->stream, c->dStepCl, c->dStepHist,
+                       c->dStepClTmp, c->dStepHistTmp, aux, c->dStepSum, 2, a.B);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(c->hStepSum, c->dStepSum, sizeof(StepSummary), hipMemcpyDeviceToHost, c->stream));
+    return DPMM_OK;
+}
+
+int step_error(const StepSummary& s) {
+    if (s.error & 1) return fail(DPMM_ENOTSPD, "posterior draw failed: a factor is not SPD");
+    if (s.error & 2) return fail(DPMM_ENOTSPD, "a log marginal likelihood is not finite");
+    return DPMM_OK;
+}
+
+}  // namespace
+
+namespace dpmm {
+
+int step_upload(dpmm_ctx* c, int B, int K, const StepCluster* cl, const double* hist) {
+    if (!c || B < 1 || K <= 0 || K > kMaxK || !cl || !hist) return fail(DPMM_EINVAL, "bad args");
+    size_t cap;
+    int rc;
+    // fixed-size buffers (grow keeps a buffer that exists); the history rings follow the burn-out length
+    cap = c->dStepCl ? (size_t)kMaxK : 0;
+    if ((rc = grow(c->dStepCl, cap, (size_t)kMaxK))) return rc;
+    cap = c->dStepClTmp ? (size_t)kMaxK : 0;
+    if ((rc = grow(c->dStepClTmp, cap, (size_t)kMaxK))) return rc;
+    cap = c->dStepW ? (size_t)kMaxK : 0;
+    if ((rc = grow(c->dStepW, cap, (size_t)kMaxK))) return rc;
+    cap = c->dStepMem ? (size_t)12 * kMaxK : 0;
+    if ((rc = grow(c->dStepMem, cap, (size_t)12 * kMaxK))) return rc;
+    cap = c->dStepCand ? (size_t)kMaxK : 0;
+    if ((rc = grow(c->dStepCand, cap, (size_t)kMaxK))) return rc;
+    cap = c->dStepCounts ? (size_t)4 : 0;
+    if ((rc = grow(c->dStepCounts, cap, (size_t)4))) return rc;
+    cap = c->dStepTab ? (size_t)8 * kMaxK : 0;
+    if ((rc = grow(c->dStepTab, cap, (size_t)8 * kMaxK))) return rc;
+    cap = c->dStepSum ? (size_t)1 : 0;
+    if ((rc = grow(c->dStepSum, cap, (size_t)1))) return rc;
+    if (c->step_B != B) {
+        cap = 0;
+        double* h = nullptr;
+        double* ht = nullptr;
+        if ((rc = grow(h, cap, (size_t)kMaxK * B))) return rc;
+        cap = 0;
+        if ((rc = grow(ht, cap, (size_t)kMaxK * B))) {
+            (void)hipFree(h);
+            return rc;
+        }
+        if (c->dStepHist) (void)hipFree(c->dStepHist);
+        if (c->dStepHistTmp) (void)hipFree(c->dStepHistTmp);
+        c->dStepHist = h;
+        c->dStepHistTmp = ht;
+        c->step_B = B;
+    }
+    if (!c->hStepSum && hipHostMalloc((void**)&c->hStepSum, sizeof(StepSummary)) != hipSuccess) {
+        c->hStepSum = nullptr;
+        return fail(DPMM_ENOMEM, "pinned");
+    }
+    HIPCHK(hipMemcpyAsync(c->dStepCl, cl, (size_t)K * sizeof(StepCluster), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->dStepHist, hist, (size_t)K * B * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return DPMM_OK;
+}
+
+int step_download(dpmm_ctx* c, int K, int Kw, StepCluster* cl, double* hist, double* weights) {
+    if (!c || K <= 0 || K > kMaxK || Kw < 0 || Kw > kMaxK) return fail(DPMM_EINVAL, "bad args");
+    if (!c->dStepCl) return fail(DPMM_ESTATE, "no cluster table on the device");
+    if (cl) HIPCHK(hipMemcpyAsync(cl, c->dStepCl, (size_t)K * sizeof(StepCluster), hipMemcpyDeviceToHost, c->stream));
+    if (hist)
+        HIPCHK(hipMemcpyAsync(hist, c->dStepHist, (size_t)K * c->step_B * sizeof(double), hipMemcpyDeviceToHost,
+                              c->stream));
+    if (weights && Kw > 0)
+        HIPCHK(hipMemcpyAsync(weights, c->dStepW, (size_t)Kw * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return DPMM_OK;
+}
+
+int step_iteration(dpmm_ctx* c, const StepArgs& args, int sweep_final, int full_mu_cov, int use_weight,
+                   int32_t* trace_labels, int32_t* trace_subs, StepSummary* out) {
+    if (!c || !out) return fail(DPMM_EINVAL, "bad args");
+    if (!c->dStepCl || !c->hStepSum) return fail(DPMM_ESTATE, "no cluster table on the device");
+    if (!c->has_prior) return fail(DPMM_ESTATE, "prior not set");
+    StepArgs a = args;
+    a.E = (int)c->statEntries();
+    a.B = c->step_B;
+    const int K = a.K;
+    if (K <= 0 || K > kMaxK || a.kcap < 1 || a.kcap > kMaxK) return fail(DPMM_EINVAL, "bad cluster count");
+    const int64_t maxpairs = a.no_more_splits ? 0 : (int64_t)K * (K - 1) / 2;
+    const int64_t maxjobs = 3 * (int64_t)K + maxpairs;
+    int rc;
+    if ((rc = grow(c->dStepJobs, c->stepJobsCap, (size_t)4 * maxjobs))) return rc;
+    if ((rc = grow(c->dStepAcc, c->stepAccCap, (size_t)std::max<int64_t>(maxpairs, 1)))) return rc;
+    if ((rc = grow(c->dLM, c->lmCap, (size_t)2 * maxjobs))) return rc;
+    if ((rc = grow(c->dLogW, c->logwCap, (size_t)3 * K))) return rc;
+    if (c->graph_mode && !trace_labels && !trace_subs) {
+        // the whole iteration recorded into one hipGraph
+        if ((rc = resolve_evals(c))) return rc;
+        if ((rc = ensure_draw_buffers(c, K))) return rc;
+        c->K = K;
+        if ((rc = ensure_sweep_buffers(c))) return rc;
+        rc = run_captured(c, [&]() {
+            return enqueue_step(c, a, sweep_final, full_mu_cov, use_weight, (int)maxjobs, nullptr, nullptr);
+        });
+    } else {
+        rc = enqueue_step(c, a, sweep_final, full_mu_cov, use_weight, (int)maxjobs, trace_labels, trace_subs);
+    }
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    StepSummary s = *c->hStepSum;
+    if (s.followup || s.relabel2) labels_rewritten(c);
+    if ((rc = step_error(s))) return rc;
+    if (s.followup) {
+        // the statistics and log ML of the reset and split clusters, then the merges and removals
+        const int Ks = s.K_split;
+        if ((rc = grow(c->dStepJobs, c->stepJobsCap, (size_t)12 * Ks))) return rc;
+        if ((rc = grow(c->dLM, c->lmCap, (size_t)6 * Ks))) return rc;
+        if ((rc = enqueue_stats(c, Ks))) return rc;
+        hipLaunchKernelGGL(k_step_touched_jobs, dim3(1), dim3(kStepThreads), 0, c->stream, Ks, a.E, c->dStepCl,
+                           c->dStats, c->dStepCand, c->dStepJobs, c->dStepCounts);
+        if ((rc = launch_log_marginal_dev(c, 3 * Ks, c->dStepJobs, c->dStepCounts))) return rc;
+        hipLaunchKernelGGL(k_step_touched_apply, dim3(1), dim3(kStepThreads), 0, c->stream, c->dStepCl, c->dLM,
+                           c->dStepCand, c->dStepCounts, c->dStepSum);
+        int32_t* tab2 = c->dStepTab + 3 * kMaxK;
+        int32_t* aux = tab2 + 3 * kMaxK;
+        hipLaunchKernelGGL(k_step_relabel, dim3(grid_relabel(c)), dim3(kThreads), 0, c->stream, c->dLabels, c->dSubs,
+                           c->N, tab2, 0, c->dStepSum, 3, c->key0(), c->key1(), a.t);
+        hipLaunchKernelGGL(k_step_compact, dim3(1), dim3(kStepThreads), 0, c->stream, c->dStepCl, c->dStepHist,
+                           c->dStepClTmp, c->dStepHistTmp, aux, c->dStepSum, 3, a.B);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(c->hStepSum, c->dStepSum, sizeof(StepSummary), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        s = *c->hStepSum;
+        if (s.relabel2) labels_rewritten(c);
+        if ((rc = step_error(s))) return rc;
+    }
+    *out = s;
+    return DPMM_OK;
+}
+
+}  // namespace dpmm
 
 // Internal accessors for the host driver (same library).
 namespace dpmm {

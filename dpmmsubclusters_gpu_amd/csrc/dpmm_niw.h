@@ -298,13 +298,15 @@ __device__ void gram_rev_packed(const double* R, double* G, int D) {
 __global__ void __launch_bounds__(kThreads, 4) k_log_marginal(const double* __restrict__ stats, int E, int DP, int D,
                                                            int prior, const int32_t* __restrict__ members,
                                                            const int32_t* __restrict__ psel, PriorArgs h,
-                                                           PriorArgs h1, double* __restrict__ out) {
+                                                           PriorArgs h1, double* __restrict__ out,
+                                                           const int32_t* __restrict__ njobs) {
     extern __shared__ double lds[];
     __shared__ double red[8];
     __shared__ double scal[4];
     __shared__ double sh[2];
     __shared__ int mem[4];
     const int job = blockIdx.x;
+    if (njobs && job >= *njobs) return;   // a grid sized for the most jobs a device-built list can hold
     if (threadIdx.x < 4) mem[threadIdx.x] = members[4 * job + threadIdx.x];
     __syncthreads();
     const int sel = psel ? psel[job] : 0;

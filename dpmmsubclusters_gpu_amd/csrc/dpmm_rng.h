@@ -10,7 +10,8 @@
 //   split relabel      (i, 0, t, TAG_SPLITSUB) word0 & 1
 //   sequential streams (c0, n, t, tag), n = 0, 1, ... : parameter draws, Dirichlet weights,
 //   split / merge test uniforms, the sub-cluster weights a split or a merge sets; a multinomial
-//   parameter draw gives coordinate d the stream that starts at n = 1024 d.
+//   parameter draw gives coordinate d the stream that starts at n = 1024 d, and the mixture
+//   weights' Dirichlet gives component i the stream (i, n, t, TAG_MIXW).
 // The categorical draws are Gumbel-max (labels) and a logistic threshold (sub-labels), which
 // have the distributions of the reference's softmax + alias sampling (cudaKernel.cu:27-124,
 // 352-403).

@@ -1,0 +1,506 @@
+// Kernels of the device-resident sampler iteration: the scalar work of group_step
+// (local_clusters_actions.cpp:460-490) that otherwise runs on the host between device launches.
+//   k_step_sample    sample_clusters (local_clusters_actions.cpp:341-371): sub-cluster weights
+//                    ~ Dir(N_l + alpha / 2, N_r + alpha / 2), the burn-out history and splittable rule
+//                    (shared_actions.cpp:39-54), mixture weights ~ Dir(N_1, ..., N_K, alpha) with the
+//                    last entry dropped (357-365); the rows and log weights of the parameter draw.
+//   k_step_jobs      each part's count and statistics entries (cluster = l + r, niw.cpp:140-148),
+//                    the merge candidates and the log-ML job list: every part, every candidate pair.
+//   k_step_decide    reset_bad_clusters (429-447), check_and_split (244-285) within the cluster cap,
+//                    check_and_merge (287-321) + should_merge (shared_actions.cpp:88-111) with the
+//                    greedy lexicographic resolution, remove_empty_clusters (400-419): the new cluster
+//                    records and the two relabel tables.
+//   k_step_relabel   one relabel pass: resets and splits, or merges and removals.
+//   k_step_compact   the cluster table renumbered after merges and removals.
+//   k_step_touched_* statistics entries and log ML of the clusters a reset or a split touched.
+// Every cluster loop runs in one workgroup (K <= 2048).  The decisions read the Philox counters the
+// host driver reads and evaluate the same expressions in the same order with contraction off, so the
+// two take the same decisions (lgamma, log and the gamma draws' transcendentals may differ from the
+// host library's in the last bit).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "dpmm_niw.h"
+#include "dpmm_rng.h"
+#include "dpmm_step_state.h"
+
+namespace dpmm {
+
+constexpr int kStepThreads = 1024;
+constexpr int kStepMaxK = 2048;
+
+// Rank of this thread's flag among the set flags of the block, in thread order, and their total.
+// Every thread of the block calls it.
+__device__ __forceinline__ int step_rank(bool flag, int* ws, int& total) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const unsigned long long b = __ballot(flag);
+    const int below = __popcll(b & ((1ull << lane) - 1ull));
+    if (lane == 0) ws[w] = __popcll(b);
+    __syncthreads();
+    int off = 0, tot = 0;
+    const int nw = (int)(blockDim.x >> 6);
+    for (int i = 0; i < nw; ++i) {
+        if (i < w) off += ws[i];
+        tot += ws[i];
+    }
+    __syncthreads();
+    total = tot;
+    return off + below;
+}
+
+__device__ __forceinline__ void step_set_part(StepPart& p, int64_t N, int q0, int q1) {
+    p.N = N;
+    p.mem[0] = q0;
+    p.mem[1] = q1;
+    p.mem[2] = p.mem[3] = -1;
+}
+
+// Pair p of n candidates in lexicographic order: (ra, rb), ra < rb; row ra starts at ra n - ra (ra + 1) / 2.
+__device__ __forceinline__ void step_pair(int64_t p, int n, int& ra, int& rb) {
+    const double b = 2.0 * n - 1.0;
+    int r = (int)((b - sqrt(fmax(b * b - 8.0 * (double)p, 0.0))) * 0.5);
+    r = max(0, min(r, n - 2));
+    auto row0 = [n](int q) { return (int64_t)q * n - (int64_t)q * (q + 1) / 2; };
+    while (r > 0 && row0(r) > p) --r;
+    while (r + 1 < n - 1 && row0(r + 1) <= p) ++r;
+    ra = r;
+    rb = (int)(p - row0(r)) + r + 1;
+}
+
+// Two-component Dirichlet on stream (c0, t, tag): the gammas in order, each divided by their sum.
+__device__ __forceinline__ void step_dirichlet2(uint32_t c0, uint32_t t, uint32_t tag, uint32_t k0, uint32_t k1,
+                                                double a0, double a1, double* out) {
+#pragma clang fp contract(off)
+    DevStream rs(c0, t, tag, k0, k1, 0);
+    const double g0 = rs.gamma(a0);
+    const double g1 = rs.gamma(a1);
+    const double s = g0 + g1;
+    out[0] = g0 / s;
+    out[1] = g1 / s;
+}
+
+// sample_clusters for clusters 0 .. K-1.  members: 3K rows of 4 statistics entries (cluster, l, r of
+// each cluster); logw: log mixture weight of each cluster, then the 2K log sub-cluster weights.
+// Mixture component i (clusters, then alpha) draws its gamma from stream (i, t, TAG_MIXW).
+__global__ void __launch_bounds__(kStepThreads) k_step_sample(StepArgs a, StepCluster* __restrict__ cl,
+                                                             double* __restrict__ hist,
+                                                             double* __restrict__ weights,
+                                                             int32_t* __restrict__ members,
+                                                             double* __restrict__ logw) {
+#pragma clang fp contract(off)
+    __shared__ double g[kStepMaxK + 1];
+    __shared__ double s_sum;
+    const int K = a.K, B = a.B;
+    for (int k = threadIdx.x; k < K; k += blockDim.x) {
+        StepCluster& c = cl[k];
+        step_dirichlet2((uint32_t)k, a.t, TAG_LRW, a.key0, a.key1, (double)c.l.N + a.alpha / 2,
+                        (double)c.r.N + a.alpha / 2, c.lrw);
+        // burn-out history: push L_l + L_r; splittable once B entries average within 1e-2 of the last
+        const double L = c.l.lml + c.r.lml;
+        double* h = hist + (size_t)k * B;
+        h[c.hist_head] = L;
+        c.hist_head = c.hist_head + 1 == B ? 0 : c.hist_head + 1;
+        if (c.hist_n < B) ++c.hist_n;
+        if (c.hist_n == B) {
+            const double w = 1 / (B - 0.1);
+            double now = 0.0;
+            for (int j = 0; j < B; ++j) {   // oldest to newest: the ring is full, the oldest at the head
+                int q = c.hist_head + j;
+                if (q >= B) q -= B;
+                now += h[q] * w;
+            }
+            if (now - L < 1e-2) c.splittable = 1;
+        }
+        const StepPart* ps[3] = {&c.c, &c.l, &c.r};
+        for (int p = 0; p < 3; ++p)
+            for (int e = 0; e < 4; ++e) members[4 * (3 * k + p) + e] = ps[p]->mem[e];
+        logw[K + 2 * k] = log(c.lrw[0]);
+        logw[K + 2 * k + 1] = log(c.lrw[1]);
+    }
+    for (int i = threadIdx.x; i <= K; i += blockDim.x) {
+        DevStream rs((uint32_t)i, a.t, TAG_MIXW, a.key0, a.key1, 0);
+        g[i] = rs.gamma(i < K ? (double)cl[i].c.N : a.alpha);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int i = 0; i <= K; ++i) s += g[i];
+        s_sum = s;
+    }
+    __syncthreads();
+    // get_dirichlet_distribution (local_clusters_actions.cpp:323-339): one empty cluster gets weight 0
+    const bool degenerate = K == 1 && cl[0].c.N == 0;
+    for (int k = threadIdx.x; k < K; k += blockDim.x) {
+        const double w = degenerate ? 0.0 : g[k] / s_sum;
+        weights[k] = w;
+        logw[k] = log(w);
+    }
+}
+
+// The parts of clusters 0 .. K-1 from the statistics pass (entry q = 2k + s holds sub-cluster s of
+// cluster k; its first double is the count), the merge candidates (splittable, non-empty) and the
+// log-ML jobs: 3k + {0, 1, 2} the parts of cluster k, then 3K + p the candidate pairs in
+// lexicographic order.  counts[0] = jobs, counts[1] = candidates.
+__global__ void __launch_bounds__(kStepThreads) k_step_jobs(StepArgs a, StepCluster* __restrict__ cl,
+                                                           const double* __restrict__ stats,
+                                                           int32_t* __restrict__ cand, int32_t* __restrict__ jobs,
+                                                           int32_t* __restrict__ counts) {
+    __shared__ int ws[16];
+    const int K = a.K;
+    int ncand = 0;
+    for (int base = 0; base < K; base += blockDim.x) {
+        const int k = base + threadIdx.x;
+        bool f = false;
+        if (k < K) {
+            StepCluster& c = cl[k];
+            const int64_t Nl = (int64_t)stats[(size_t)(2 * k) * a.E];
+            const int64_t Nr = (int64_t)stats[(size_t)(2 * k + 1) * a.E];
+            step_set_part(c.l, Nl, 2 * k, -1);
+            step_set_part(c.r, Nr, 2 * k + 1, -1);
+            step_set_part(c.c, Nl + Nr, 2 * k, 2 * k + 1);
+            const StepPart* ps[3] = {&c.c, &c.l, &c.r};
+            for (int p = 0; p < 3; ++p)
+                for (int e = 0; e < 4; ++e) jobs[4 * (3 * k + p) + e] = ps[p]->mem[e];
+            f = !a.no_more_splits && c.splittable && Nl + Nr > 0;
+        }
+        int tot;
+        const int r = step_rank(f, ws, tot);
+        if (f) cand[ncand + r] = k;
+        ncand += tot;
+    }
+    __syncthreads();
+    for (int ra = 0; ra + 1 < ncand; ++ra) {
+        const int64_t row0 = (int64_t)ra * ncand - (int64_t)ra * (ra + 1) / 2;
+        const int i = cand[ra];
+        for (int rb = ra + 1 + (int)threadIdx.x; rb < ncand; rb += blockDim.x) {
+            const int j = cand[rb];
+            int32_t* m = jobs + 4 * ((int64_t)3 * K + row0 + (rb - ra - 1));
+            m[0] = 2 * i;
+            m[1] = 2 * i + 1;
+            m[2] = 2 * j;
+            m[3] = 2 * j + 1;
+        }
+    }
+    if (threadIdx.x == 0) {
+        counts[0] = 3 * K + (int)((int64_t)ncand * (ncand - 1) / 2);
+        counts[1] = ncand;
+    }
+}
+
+// Decisions of the iteration over the jobs' log MLs lm (2 doubles per job: log ML, log det psi_n).
+// tab1: the reset + split relabel table over the K clusters (new label of (k, s) at 2k + s, action of
+// cluster k at 2K + k: 1 reset, 2 split); tab2: the merge + removal table over the K_split clusters
+// (actions 3 merged into, 4 merged away); aux: merge target (K_split) then new index (K_split) of
+// every cluster; accl: scratch for the accepted pairs.  draw_status: the parameter draw's per-part
+// status (n_status entries).
+__global__ void __launch_bounds__(kStepThreads) k_step_decide(StepArgs a, StepCluster* __restrict__ cl,
+                                                             const double* __restrict__ lm,
+                                                             const int32_t* __restrict__ cand,
+                                                             const int32_t* __restrict__ counts,
+                                                             const int32_t* __restrict__ draw_status, int n_status,
+                                                             int32_t* __restrict__ accl, int32_t* __restrict__ tab1,
+                                                             int32_t* __restrict__ tab2, int32_t* __restrict__ aux,
+                                                             StepSummary* __restrict__ sum) {
+#pragma clang fp contract(off)
+    __shared__ int ws[16];
+    __shared__ int s_err, s_nreset, s_nmerge;
+    __shared__ unsigned char used[kStepMaxK];
+    const int K = a.K;
+    if (threadIdx.x == 0) {
+        s_err = 0;
+        s_nreset = 0;
+        s_nmerge = 0;
+    }
+    for (int k = threadIdx.x; k < kStepMaxK; k += blockDim.x) used[k] = 0;
+    __syncthreads();
+    int err = 0, nreset = 0;
+    for (int p = threadIdx.x; p < n_status; p += blockDim.x)
+        if (draw_status[p]) err |= 1;
+    // the parts' log MLs; reset_bad_clusters: a cluster with an empty sub-cluster gets new sub-labels,
+    // an empty history and is not splittable
+    for (int k = threadIdx.x; k < K; k += blockDim.x) {
+        StepCluster& c = cl[k];
+        StepPart* ps[3] = {&c.c, &c.l, &c.r};
+        for (int p = 0; p < 3; ++p) {
+            const double L = lm[2 * (3 * k + p)];
+            if (!isfinite(L)) err |= 2;
+            ps[p]->lml = L;
+            ps[p]->ld = lm[2 * (3 * k + p) + 1];
+        }
+        c.touched = 0;
+        tab1[2 * k] = k;
+        tab1[2 * k + 1] = k;
+        tab1[2 * K + k] = 0;
+        if (c.l.N == 0 || c.r.N == 0) {
+            c.hist_n = 0;
+            c.hist_head = 0;
+            c.splittable = 0;
+            c.touched = 1;
+            tab1[2 * K + k] = 1;
+            ++nreset;
+        }
+    }
+    if (err) atomicOr(&s_err, err);
+    if (nreset) atomicAdd(&s_nreset, nreset);
+    __syncthreads();
+    // check_and_split: log H = log alpha + lgamma(N_l) + L_l + lgamma(N_r) + L_r - lgamma(N) - L against
+    // log U; accepted splits beyond the cluster cap are dropped, the highest indices first
+    int nsplit = 0;
+    if (!a.no_more_splits) {
+        const int room = a.kcap - K;
+        for (int base = 0; base < K; base += blockDim.x) {
+            const int k = base + threadIdx.x;
+            bool acc = false;
+            if (k < K) {
+                const StepCluster& c = cl[k];
+                if (c.splittable && c.c.N > 1 && !a.fin && c.l.N != 0 && c.r.N != 0) {
+                    const double lhr = log(a.alpha) + lgamma((double)c.l.N) + c.l.lml + lgamma((double)c.r.N) +
+                                       c.r.lml - (lgamma((double)c.c.N) + c.c.lml);
+                    const double u =
+                        u01_f64(philox4x32_10((uint32_t)k, 0, a.t, TAG_SPLIT_U, a.key0, a.key1).v[0]);
+                    acc = lhr > log(u);
+                }
+            }
+            int tot;
+            const int r = step_rank(acc, ws, tot);
+            if (acc && nsplit + r < room) {
+                // split_cluster_local: k keeps the left half, K + rank gets the right half; each half is
+                // create_splittable_from_params of the old sub-cluster with sub-cluster weights
+                // ~ Dir(alpha / 2, alpha / 2); its parts are recomputed from its new sub-labels
+                const int nk = K + nsplit + r;
+                StepCluster& c = cl[k];
+                StepCluster L = {}, R = {};
+                L.c = L.l = L.r = c.l;
+                R.c = R.l = R.r = c.r;
+                L.touched = R.touched = 1;
+                step_dirichlet2((uint32_t)k, a.t, TAG_SPLIT_LRW, a.key0, a.key1, a.alpha / 2, a.alpha / 2, L.lrw);
+                step_dirichlet2((uint32_t)nk, a.t, TAG_SPLIT_LRW, a.key0, a.key1, a.alpha / 2, a.alpha / 2,
+                                R.lrw);
+                c = L;
+                cl[nk] = R;
+                tab1[2 * k + 1] = nk;
+                tab1[2 * K + k] = 2;
+            }
+            nsplit = min(nsplit + tot, max(room, 0));
+        }
+    }
+    __syncthreads();
+    const int Ks = K + nsplit;
+    int32_t* merge_into = aux;
+    int32_t* remap = aux + Ks;
+    for (int k = threadIdx.x; k < Ks; k += blockDim.x) merge_into[k] = -1;
+    __syncthreads();
+    // check_and_merge: every candidate pair on the statistics before the splits; accepted pairs are
+    // resolved greedily in lexicographic order, a cluster merging at most once (a merged cluster is
+    // not splittable), which gives the reference's sequential loop's outcome
+    if (!a.no_more_splits) {
+        const int ncand = counts[1];
+        const int64_t np = (int64_t)ncand * (ncand - 1) / 2;
+        const double al = a.alpha;
+        const double cst = -log(al) + lgamma(al) - 2 * lgamma(0.5 * al);
+        int nacc = 0;
+        for (int64_t base = 0; base < np; base += blockDim.x) {
+            const int64_t p = base + threadIdx.x;
+            bool acc = false;
+            if (p < np) {
+                int ra, rb;
+                step_pair(p, ncand, ra, rb);
+                const int i = cand[ra], j = cand[rb];
+                const StepCluster& ci = cl[i];
+                const StepCluster& cj = cl[j];
+                const double L = lm[2 * (3 * (int64_t)K + p)];
+                if (isfinite(L) && ci.splittable && cj.splittable && ci.c.N > 0 && cj.c.N > 0) {
+                    const double Ni = (double)ci.c.N, Nj = (double)cj.c.N, Nm = Ni + Nj;
+                    const double lhr = cst + lgamma(Nm) - lgamma(Nm + al) + lgamma(Ni + 0.5 * al) - lgamma(Ni) -
+                                       lgamma(Nj) + lgamma(Nj + 0.5 * al) + L - ci.c.lml - cj.c.lml;
+                    const double u = u01_f64(
+                        philox4x32_10(((uint32_t)i << 16) | (uint32_t)j, 0, a.t, TAG_MERGE_U, a.key0, a.key1).v[0]);
+                    acc = lhr > log(u) || (a.fin && lhr > log(0.1));
+                }
+            }
+            int tot;
+            const int r = step_rank(acc, ws, tot);
+            if (acc) accl[nacc + r] = (int32_t)p;
+            nacc += tot;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int nm = 0;
+            for (int q = 0; q < nacc; ++q) {
+                const int64_t p = accl[q];
+                int ra, rb;
+                step_pair(p, ncand, ra, rb);
+                const int i = cand[ra], j = cand[rb];
+                if (used[i] || used[j]) continue;
+                used[i] = used[j] = 1;
+                merge_into[j] = i;
+                ++nm;
+                // merge_clusters + merge_clusters_to_splittable: i becomes the merged cluster, its
+                // sub-clusters the old i (a clone of the merged part, as the reference leaves it) and j
+                StepCluster& A = cl[i];
+                StepCluster& Bc = cl[j];
+                StepPart m;
+                m.N = A.c.N + Bc.c.N;
+                m.mem[0] = 2 * i;
+                m.mem[1] = 2 * i + 1;
+                m.mem[2] = 2 * j;
+                m.mem[3] = 2 * j + 1;
+                m.lml = lm[2 * (3 * (int64_t)K + p)];
+                m.ld = 0.0;
+                step_dirichlet2((uint32_t)i, a.t, TAG_MERGE_LRW, a.key0, a.key1, (double)A.c.N + 0.5 * al,
+                                (double)Bc.c.N + 0.5 * al, A.lrw);
+                A.r = Bc.c;
+                A.c = m;
+                A.l = m;
+                A.splittable = 0;
+                A.hist_n = 0;
+                A.hist_head = 0;
+                Bc.c.N = 0;
+                Bc.splittable = 0;
+            }
+            s_nmerge = nm;
+        }
+    }
+    __syncthreads();
+    // remove_empty_clusters: the clusters with points keep their order
+    int nk = 0;
+    for (int base = 0; base < Ks; base += blockDim.x) {
+        const int k = base + threadIdx.x;
+        const bool keep = k < Ks && cl[k].c.N > 0;
+        int tot;
+        const int r = step_rank(keep, ws, tot);
+        if (k < Ks) remap[k] = keep ? nk + r : -1;
+        nk += tot;
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < Ks; k += blockDim.x) {
+        const int tgt = merge_into[k] >= 0 ? merge_into[k] : k;
+        const int lab = remap[tgt] >= 0 ? remap[tgt] : 0;
+        tab2[2 * k] = lab;
+        tab2[2 * k + 1] = lab;
+        tab2[2 * Ks + k] = merge_into[k] >= 0 ? 4 : 0;
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < Ks; k += blockDim.x)
+        if (merge_into[k] >= 0) tab2[2 * Ks + merge_into[k]] = 3;
+    if (threadIdx.x == 0) {
+        sum->K_split = Ks;
+        sum->K_new = nk;
+        sum->nsplit = nsplit;
+        sum->nreset = s_nreset;
+        sum->nmerge = s_nmerge;
+        sum->followup = nsplit + s_nreset > 0;
+        sum->relabel2 = s_nmerge > 0 || nk != Ks;
+        sum->error = s_err;
+    }
+}
+
+// One relabel pass (the table layout of k_step_decide).  mode 1: the reset + split table over the
+// K_arg clusters, when resets or splits were accepted; mode 2: the merge + removal table over K_split
+// clusters, when there are merges or removals and no reset or split waits for its statistics; mode 3:
+// the same table after that follow-up.
+__global__ void __launch_bounds__(kThreads) k_step_relabel(int32_t* __restrict__ labels, int32_t* __restrict__ subs,
+                                                           int64_t N, const int32_t* __restrict__ table, int K_arg,
+                                                           const StepSummary* __restrict__ sum, int mode,
+                                                           uint32_t key0, uint32_t key1, uint32_t iter) {
+    if (mode == 1 ? !sum->followup : (!sum->relabel2 || (mode == 2 && sum->followup))) return;
+    const int K = mode == 1 ? K_arg : sum->K_split;
+    // grid-stride over the points: a pass that is gated off costs a few workgroups only
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < N; i += (int64_t)gridDim.x * kThreads) {
+        const int k = labels[i], s = subs[i];
+        if ((unsigned)k >= (unsigned)K || (unsigned)s > 1u) continue;
+        const int act = table[2 * K + k];
+        labels[i] = table[2 * k + s];
+        int ns = s;
+        if (act == 1 || act == 2)
+            ns = (int)(philox4x32_10((uint32_t)i, 0, iter, act == 1 ? TAG_RESET : TAG_SPLITSUB, key0, key1).v[0] & 1u);
+        else if (act == 3)
+            ns = 0;
+        else if (act == 4)
+            ns = 1;
+        subs[i] = ns;
+    }
+}
+
+// The cluster records and histories renumbered by aux's new indices (gated as k_step_relabel's
+// mode 2 / 3), through tmp.
+__global__ void __launch_bounds__(kStepThreads) k_step_compact(StepCluster* __restrict__ cl,
+                                                              double* __restrict__ hist,
+                                                              StepCluster* __restrict__ tmp,
+                                                              double* __restrict__ htmp,
+                                                              const int32_t* __restrict__ aux,
+                                                              const StepSummary* __restrict__ sum, int mode, int B) {
+    if (!sum->relabel2 || (mode == 2 && sum->followup)) return;
+    const int Ks = sum->K_split, Kn = sum->K_new;
+    const int32_t* remap = aux + Ks;
+    for (int k = threadIdx.x; k < Ks; k += blockDim.x)
+        if (remap[k] >= 0) tmp[remap[k]] = cl[k];
+    for (int e = threadIdx.x; e < Ks * B; e += blockDim.x) {
+        const int k = e / B;
+        if (remap[k] >= 0) htmp[(size_t)remap[k] * B + (e - k * B)] = hist[e];
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < Kn; k += blockDim.x) cl[k] = tmp[k];
+    for (int e = threadIdx.x; e < Kn * B; e += blockDim.x) hist[e] = htmp[e];
+}
+
+// The parts of the clusters flagged touched (of K_split) from a statistics pass over the relabelled
+// points, as log-ML jobs 3r + {0, 1, 2} for the r-th touched cluster (list[r]); counts[0] = jobs.
+__global__ void __launch_bounds__(kStepThreads) k_step_touched_jobs(int Ks, int E, StepCluster* __restrict__ cl,
+                                                                   const double* __restrict__ stats,
+                                                                   int32_t* __restrict__ list,
+                                                                   int32_t* __restrict__ jobs,
+                                                                   int32_t* __restrict__ counts) {
+    __shared__ int ws[16];
+    int n = 0;
+    for (int base = 0; base < Ks; base += blockDim.x) {
+        const int k = base + threadIdx.x;
+        const bool f = k < Ks && cl[k].touched;
+        int tot;
+        const int r = step_rank(f, ws, tot);
+        if (f) {
+            StepCluster& c = cl[k];
+            const int64_t Nl = (int64_t)stats[(size_t)(2 * k) * E];
+            const int64_t Nr = (int64_t)stats[(size_t)(2 * k + 1) * E];
+            step_set_part(c.l, Nl, 2 * k, -1);
+            step_set_part(c.r, Nr, 2 * k + 1, -1);
+            step_set_part(c.c, Nl + Nr, 2 * k, 2 * k + 1);
+            const StepPart* ps[3] = {&c.c, &c.l, &c.r};
+            const int q = n + r;
+            list[q] = k;
+            for (int p = 0; p < 3; ++p)
+                for (int e = 0; e < 4; ++e) jobs[4 * (3 * q + p) + e] = ps[p]->mem[e];
+        }
+        n += tot;
+    }
+    if (threadIdx.x == 0) counts[0] = 3 * n;
+}
+
+__global__ void __launch_bounds__(kStepThreads) k_step_touched_apply(StepCluster* __restrict__ cl,
+                                                                    const double* __restrict__ lm,
+                                                                    const int32_t* __restrict__ list,
+                                                                    const int32_t* __restrict__ counts,
+                                                                    StepSummary* __restrict__ sum) {
+    __shared__ int s_err;
+    if (threadIdx.x == 0) s_err = 0;
+    __syncthreads();
+    const int n = counts[0] / 3;
+    int err = 0;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) {
+        StepCluster& c = cl[list[q]];
+        StepPart* ps[3] = {&c.c, &c.l, &c.r};
+        for (int p = 0; p < 3; ++p) {
+            const double L = lm[2 * (3 * q + p)];
+            if (!isfinite(L)) err = 2;
+            ps[p]->lml = L;
+            ps[p]->ld = lm[2 * (3 * q + p) + 1];
+        }
+        c.touched = 0;
+    }
+    if (err) atomicOr(&s_err, err);
+    __syncthreads();
+    if (threadIdx.x == 0) sum->error |= s_err;
+}
+
+}  // namespace dpmm

@@ -72,6 +72,14 @@ class Stream {
     int pos_ = 4;
 };
 
+// Mixture weights ~ Dir(a_0, ..., a_{n-1}): component i draws its gamma from stream (i, t, TAG_MIXW),
+// so that the device draws the components in parallel; the sum runs in index order.
+void mixture_dirichlet(uint64_t seed, uint32_t t, const double* a, int n, double* out) {
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += (out[i] = Stream(seed, (uint32_t)i, t, TAG_MIXW).gamma(a[i]));
+    for (int i = 0; i < n; ++i) out[i] /= s;
+}
+
 // ------------------------------------------------------------------ model ----------------
 // A cluster part: the sum of up to four sub-cluster statistics entries of the last statistics
 // pass (entry q = 2k + s), its point count and the log marginal likelihood of that sum.
@@ -150,9 +158,125 @@ struct SamplerImpl {
     std::vector<float> log_feat;
     std::vector<double> log_h, log_u;
     int64_t n_exact = 0, n_skipped = 0, n_missed = 0, n_full = 0;
+    // totals since creation: splits, merges, resets, clusters removed (merged away or emptied)
+    int64_t ev_split = 0, ev_merge = 0, ev_reset = 0, ev_removed = 0;
     bool counts_fresh = false;   // Nsub holds the counts of the current labels (from the sweep call)
     int32_t* trace_labels = nullptr;   // dpmm_sampler_trace_sweep: host copies of each step's sweep output
     int32_t* trace_subs = nullptr;
+    // Device-resident iterations: while dev_on, the cluster list lives in the device table (Kdev
+    // records, the last draw's Kw_dev mixture weights); cl and weights are refreshed from it when host
+    // code reads them (host_fresh).
+    bool dev_on = false;
+    bool host_fresh = true;
+    int Kdev = 0, Kw_dev = 0;
+
+    bool device_step_eligible() const { return !o.host_decisions && !outlier && !scr && !log_pairs; }
+
+    int num_clusters() const { return dev_on ? Kdev : (int)cl.size(); }
+
+    // The host cluster list into the device table (history: the valid entries, oldest first).
+    int push_to_device() {
+        const int K = (int)cl.size(), B = o.burnout;
+        std::vector<StepCluster> v(K);
+        std::vector<double> h((size_t)K * B, 0.0);
+        auto put = [](const Part& p, StepPart& q) {
+            q.N = p.N;
+            std::copy(p.mem, p.mem + 4, q.mem);
+            q.lml = p.lml;
+            q.ld = p.ld;
+        };
+        for (int k = 0; k < K; ++k) {
+            const Cluster& c = cl[k];
+            StepCluster& d = v[k];
+            std::memset(&d, 0, sizeof d);
+            put(c.c, d.c);
+            put(c.l, d.l);
+            put(c.r, d.r);
+            d.lrw[0] = c.lrw[0];
+            d.lrw[1] = c.lrw[1];
+            d.splittable = c.splittable ? 1 : 0;
+            int n = 0;
+            for (int i = 0; i < B; ++i)
+                if (c.hist[i].first) h[(size_t)k * B + n++] = c.hist[i].second;
+            d.hist_n = n;
+            d.hist_head = n % B;
+        }
+        int rc = step_upload(ctx, B, K, v.data(), h.data());
+        if (rc) return rc;
+        dev_on = true;
+        host_fresh = true;
+        Kdev = K;
+        Kw_dev = 0;
+        return DPMM_OK;
+    }
+
+    // The device table into the host cluster list and weights, when they are stale.
+    int pull_from_device() {
+        if (!dev_on || host_fresh) return DPMM_OK;
+        const int K = Kdev, B = o.burnout;
+        std::vector<StepCluster> v(K);
+        std::vector<double> h((size_t)K * B), w(Kw_dev);
+        int rc = step_download(ctx, K, Kw_dev, v.data(), h.data(), w.data());
+        if (rc) return rc;
+        auto get = [](const StepPart& q, Part& p) {
+            p.N = q.N;
+            std::copy(q.mem, q.mem + 4, p.mem);
+            p.lml = p.lml_split = q.lml;
+            p.ld = q.ld;
+            p.prior = 0;
+        };
+        cl.assign(K, Cluster());
+        for (int k = 0; k < K; ++k) {
+            Cluster& c = cl[k];
+            const StepCluster& d = v[k];
+            get(d.c, c.c);
+            get(d.l, c.l);
+            get(d.r, c.r);
+            c.lrw[0] = d.lrw[0];
+            c.lrw[1] = d.lrw[1];
+            c.splittable = d.splittable != 0;
+            reset_hist(c);
+            for (int j = 0; j < d.hist_n; ++j) {
+                int q = d.hist_head - d.hist_n + j;
+                if (q < 0) q += B;
+                c.hist[B - d.hist_n + j] = std::make_pair(true, h[(size_t)k * B + q]);
+            }
+        }
+        weights = w;
+        host_fresh = true;
+        return DPMM_OK;
+    }
+
+    // group_step on the device (step_iteration): the same flags as step() below, one synchronisation.
+    int step_device(int t) {
+        int rc;
+        if (!dev_on && (rc = push_to_device())) return rc;
+        const int K = Kdev;
+        const bool fin = t >= o.iters - o.argmax_sample_stop;
+        const bool no_more_splits = t >= o.iters - o.split_stop || (double)K >= o.max_clusters || K >= kcap;
+        StepArgs a{};
+        a.key0 = (uint32_t)o.seed;
+        a.key1 = (uint32_t)(o.seed >> 32);
+        a.t = (uint32_t)t;
+        a.K = K;
+        a.fin = fin ? 1 : 0;
+        a.no_more_splits = no_more_splits ? 1 : 0;
+        a.kcap = kcap;
+        a.alpha = o.alpha;
+        StepSummary sm{};
+        host_ms = 0;
+        rc = step_iteration(ctx, a, (fin || o.hard_clustering) ? 1 : 0, o.full_mu_cov, o.mult_use_weight, trace_labels,
+                            trace_subs, &sm);
+        if (rc) return rc;
+        ev_split += sm.nsplit;
+        ev_merge += sm.nmerge;
+        ev_reset += sm.nreset;
+        ev_removed += sm.K_split - sm.K_new;
+        Kw_dev = K;
+        Kdev = sm.K_new;
+        host_fresh = false;
+        return DPMM_OK;
+    }
 
     void reset_hist(Cluster& c) const { c.hist.assign(o.burnout + 5, std::make_pair(false, 0.0)); }
 
@@ -348,7 +472,7 @@ struct SamplerImpl {
         a.push_back(o.alpha);
         w.resize(a.size());
         if (a.size() == 2 && a[0] == 0.0) { w[0] = 0.0; w[1] = 1.0; }   // get_dirichlet_distribution (323-339)
-        else Stream(o.seed, 0, t, TAG_MIXW).dirichlet(a.data(), (int)a.size(), w.data());
+        else mixture_dirichlet(o.seed, t, a.data(), (int)a.size(), w.data());
         weights.clear();
         if (outlier) weights.push_back(o.outlier_mod);
         for (size_t i = 0; i + 1 < w.size(); ++i) weights.push_back(w[i] * (1.0 - o.outlier_mod));
@@ -408,6 +532,12 @@ struct SamplerImpl {
     }
 
     int step(int t) {
+        if (device_step_eligible()) return step_device(t);
+        if (dev_on) {   // back to host decisions (a screener was set): the device table into the host list
+            int rc = pull_from_device();
+            if (rc) return rc;
+            dev_on = false;
+        }
         const bool fin = t >= o.iters - o.argmax_sample_stop;
         // run_model's flags (dp_parallel_sampling.cpp:143-150); the device's cluster cap ends splitting too
         const bool no_more_splits =
@@ -443,6 +573,7 @@ struct SamplerImpl {
                     cl[k].splittable = false;
                 }
             if (any) {
+                for (int k = 0; k < K; ++k) ev_reset += bad[k];
                 std::vector<int32_t> nl(2 * K), act(K, 0);
                 for (int q = 0; q < 2 * K; ++q) nl[q] = q / 2;
                 for (int k = 0; k < K; ++k) act[k] = bad[k] ? 1 : 0;
@@ -507,6 +638,7 @@ struct SamplerImpl {
                 act[k] = 2;
                 ++nsplit;
             }
+            ev_split += nsplit;
             if (nsplit) {
                 if ((rc = relabel(it, nl, act))) return rc;
                 std::vector<char> touched(cl.size(), 0);
@@ -600,7 +732,11 @@ struct SamplerImpl {
         for (int k = 0; k < K; ++k)
             if (cl[k].c.N > 0 || (outlier && (k == 0 || (k == 1 && K == 2)))) remap[k] = nk++;
         bool any_merge = false;
-        for (int k = 0; k < (int)merge_into.size(); ++k) any_merge |= merge_into[k] >= 0;
+        for (int k = 0; k < (int)merge_into.size(); ++k) {
+            any_merge |= merge_into[k] >= 0;
+            ev_merge += merge_into[k] >= 0;
+        }
+        ev_removed += K - nk;
         if (any_merge || nk != K) {
             std::vector<int32_t> nl(2 * K), act(K, 0);
             for (int k = 0; k < K; ++k) {
@@ -668,10 +804,12 @@ int dpmm_sampler_step(dpmm_sampler* S, int iter) {
     return S->s.step(iter);
 }
 
-int dpmm_sampler_num_clusters(dpmm_sampler* S) { return S ? (int)S->s.cl.size() : DPMM_EINVAL; }
+int dpmm_sampler_num_clusters(dpmm_sampler* S) { return S ? S->s.num_clusters() : DPMM_EINVAL; }
 
 int dpmm_sampler_weights(dpmm_sampler* S, double* w, int64_t* counts, int max_k) {
     if (!S) return DPMM_EINVAL;
+    int rc = S->s.pull_from_device();
+    if (rc) return rc;
     const int K = (int)S->s.cl.size();
     for (int k = 0; k < K && k < max_k; ++k) {
         if (w) w[k] = k < (int)S->s.weights.size() ? S->s.weights[k] : 0.0;
@@ -686,6 +824,8 @@ int dpmm_sampler_weights(dpmm_sampler* S, double* w, int64_t* counts, int max_k)
 // cluster split or merged in the step is its part's, as the reference keeps it).
 int dpmm_sampler_log_posterior(dpmm_sampler* S, double* lp) {
     if (!S || !lp) return DPMM_EINVAL;
+    int rc = S->s.pull_from_device();
+    if (rc) return rc;
     const dpmm::SamplerImpl& s = S->s;
     const double a = s.o.alpha;
     double v = std::lgamma(a) - std::lgamma((double)s.N + a);
@@ -740,6 +880,15 @@ int dpmm_sampler_pair_log(dpmm_sampler* S, int max_rows, float* feat, double* lo
         if (log_u) log_u[r] = S->s.log_u[r];
     }
     *n_rows = (int)S->s.log_h.size();
+    return DPMM_OK;
+}
+
+int dpmm_sampler_events(dpmm_sampler* S, int64_t* splits, int64_t* merges, int64_t* resets, int64_t* removed) {
+    if (!S) return DPMM_EINVAL;
+    if (splits) *splits = S->s.ev_split;
+    if (merges) *merges = S->s.ev_merge;
+    if (resets) *resets = S->s.ev_reset;
+    if (removed) *removed = S->s.ev_removed;
     return DPMM_OK;
 }
 

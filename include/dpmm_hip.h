@@ -293,6 +293,12 @@ typedef struct {
      * is the outlier cluster, with the prior set by dpmm_set_outlier_prior and mixture weight outlier_mod, and
      * the reference's index rules hold (local_clusters_actions.cpp:250, 295, 348, 364-370, 409-410). */
     double outlier_mod;
+    /* 0 (default): each iteration runs on the device from the parameter draw to the relabel passes (the
+     * decisions of reset_bad_clusters, check_and_split, check_and_merge and remove_empty_clusters,
+     * local_clusters_actions.cpp:244-321, 400-447, in one device kernel) with one synchronisation, in
+     * graph mode as one hipGraph; runs with an outlier cluster or a merge screener keep their decisions on
+     * the host.  != 0: decisions on the host between device launches.  Both take the same decisions. */
+    int host_decisions;
 } dpmm_options;
 
 /* init_model + init_first_clusters (dp_parallel_sampling.cpp:91-130).  Gaussian hyper:
@@ -317,6 +323,10 @@ int dpmm_sampler_host_ms(dpmm_sampler* s, double* host_ms);
  * step: lgamma(alpha) - lgamma(N + alpha) + sum over clusters with points of
  * log ML_k + log alpha + lgamma(N_k), the entry of ModelInfo::likelihood_history in verbose mode. */
 int dpmm_sampler_log_posterior(dpmm_sampler* s, double* log_posterior);
+/* Decisions since creation: accepted splits, merges, reset clusters (reset_bad_clusters), and clusters
+ * removed (merged away or emptied, remove_empty_clusters); the out_events of the decision step
+ * (local_clusters_actions.cpp:244-321, 400-447).  Any pointer may be NULL. */
+int dpmm_sampler_events(dpmm_sampler* s, int64_t* splits, int64_t* merges, int64_t* resets, int64_t* removed);
 int dpmm_sampler_destroy(dpmm_sampler* s);
 
 /* Merge screening for a Gaussian sampler (s not owned): from then on, iterations t with

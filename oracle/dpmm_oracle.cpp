@@ -26,7 +26,8 @@
  *           NIW draw's D Bartlett chi variates take the streams starting at n = 1024 (16 + i),
  *           so the D gammas are independent of each other's rejection counts (the NIW draw's
  *           normals, D (D - 1) / 2 of the Bartlett factor then D of mu, stay on n = 0, 1, ...)
- *     tag 7 left/right weights of a cluster, tag 8 mixture weights (c0 = 0),
+ *     tag 7 left/right weights of a cluster, tag 8 mixture weights (c0 = component index, one
+ *           gamma per stream),
  *     tag 9 split-test uniform (c0 = cluster), tag 10 merge-test uniform (c0 = i<<16 | j).
  * uniform(w) = (w + 0.5) 2^-32; normal = Box-Muller cos branch of two uniforms;
  * Gamma(a) = Marsaglia-Tsang (ACM TOMS 26(3), 2000), with Gamma(a+1) U^(1/a) for a < 1.
@@ -561,9 +562,14 @@ void dirichlet_mix(Model& M, uint32_t t) {   /* sample_clusters tail (local_clus
     std::vector<double> out(counts.size());
     if (counts.size() == 2 && counts[0] == 0.0) {   /* get_dirichlet_distribution special case (323-339) */
         out[0] = 0; out[1] = 1;
-    } else {
-        Stream rs(M.seed, 0, t, TAG_MIXW);
-        rs.dirichlet((int)counts.size(), counts.data(), out.data());
+    } else {   /* component i's gamma from stream (i, t, TAG_MIXW), the sum in index order */
+        double s = 0.0;
+        for (size_t i = 0; i < counts.size(); ++i) {
+            Stream rs(M.seed, (uint32_t)i, t, TAG_MIXW);
+            out[i] = rs.gamma(counts[i]);
+            s += out[i];
+        }
+        for (size_t i = 0; i < counts.size(); ++i) out[i] = out[i] / s;
     }
     /* weights scaled by 1 - outlier_mod, the outlier cluster's weight in front (362-370) */
     M.weights.clear();

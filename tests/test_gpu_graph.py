@@ -1,9 +1,11 @@
-"""Graph mode: the sweep segment of every iteration (label sweep, sub-label sweep, statistics)
-recorded into a hipGraph and replayed through one executable graph, updated in place.  Two samplers
-with the same data and seed run in lockstep in one process, one eager and one in graph mode, so
-every replay runs between the other sampler's eager launches and the host synchronisations of its
-own iteration; after every iteration both must hold the same clusters, weights and counts, and at
-the end the same labels and sub-labels, bit for bit."""
+"""Graph mode: every iteration recorded into a hipGraph and replayed through one executable graph,
+updated in place.  With the decisions on the device the graph holds the whole iteration (draw, sweeps,
+statistics, log MLs, decisions, relabel passes, the summary copy); with host decisions, the sweep
+segment (label sweep, sub-label sweep, statistics).  Two samplers with the same data and seed run in
+lockstep in one process, one eager and one in graph mode, so every replay runs between the other
+sampler's eager launches and the host synchronisations of its own iteration; after every iteration
+both must hold the same clusters, weights and counts, and at the end the same labels and sub-labels,
+bit for bit."""
 import numpy as np
 import pytest
 
@@ -20,14 +22,14 @@ def _gpu():
         pytest.fail("no GPU visible to the HIP library")
 
 
-def _lockstep(make_dev, hyper, alpha, iters, prec, prune=None):
+def _lockstep(make_dev, hyper, alpha, iters, prec, prune=None, host_decisions=False):
     runs = []
     for graph in (False, True):
         dev = make_dev()
         if prune is not None:
             dev.set_label_pruning(prune)
         dev.set_graph_mode(graph)
-        smp = dp.Sampler(dev, hyper, alpha, iters, seed=5, precision=prec)
+        smp = dp.Sampler(dev, hyper, alpha, iters, seed=5, precision=prec, host_decisions=host_decisions)
         runs.append((dev, smp))
     for t in range(1, iters + 1):
         for _, smp in runs:
@@ -44,7 +46,8 @@ def _lockstep(make_dev, hyper, alpha, iters, prec, prune=None):
     np.testing.assert_array_equal(l0, l1)
     np.testing.assert_array_equal(u0, u1)
     inst, upd = d1.graph_counts()
-    print(f"K={s1.num_clusters()} graphs instantiated {inst}, updated in place {upd}")
+    print(f"K={s1.num_clusters()} graphs instantiated {inst}, updated in place {upd}; events {s1.events()}")
+    assert s0.events() == s1.events()
     assert inst + upd == iters
     assert upd > 0
     for d, s in runs:
@@ -52,19 +55,22 @@ def _lockstep(make_dev, hyper, alpha, iters, prec, prune=None):
         d.close()
 
 
+@pytest.mark.parametrize("mode", ["device", "host"])
 @pytest.mark.parametrize("prec,prune", [("MIXED", True), ("MIXED", False), ("FP32", None)])
-def test_graph_mode_gaussian_matches_eager(prec, prune):
+def test_graph_mode_gaussian_matches_eager(prec, prune, mode):
     N, D, K = 150_000, 32, 12
     X, _, _ = gaussian_mixture(N, D, K, seed=8, a=1.0, ridge=0.5, weight=0.5, dtype=np.float32)
-    say(f"graph mode gaussian {prec} prune={prune}")
+    say(f"graph mode gaussian {prec} prune={prune} ({mode} decisions)")
     hyper = dp.niw_hyperparams(1.0, np.zeros(D), D + 3.0, np.eye(D))
     _lockstep(lambda: dp.DeviceSweep(X, dp.prior_type.Gaussian, seed=5), hyper, 10.0, 50,
-              getattr(dp.Precision, prec), prune)
+              getattr(dp.Precision, prec), prune, host_decisions=mode == "host")
 
 
-def test_graph_mode_multinomial_matches_eager():
+@pytest.mark.parametrize("mode", ["device", "host"])
+def test_graph_mode_multinomial_matches_eager(mode):
     N, D, K = 60_000, 200, 15
     X, _, _ = multinomial_mixture(N, D, K, seed=2)
-    say("graph mode multinomial")
+    say(f"graph mode multinomial ({mode} decisions)")
     hyper = dp.multinomial_hyper(np.ones(D))
-    _lockstep(lambda: dp.DeviceSweep(X, dp.prior_type.Multinomial, seed=5), hyper, 10.0, 40, dp.Precision.MIXED)
+    _lockstep(lambda: dp.DeviceSweep(X, dp.prior_type.Multinomial, seed=5), hyper, 10.0, 40, dp.Precision.MIXED,
+              host_decisions=mode == "host")

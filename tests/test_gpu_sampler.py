@@ -2,7 +2,8 @@
 
 Both run the same iteration (group_step, local_clusters_actions.cpp:460-490) from the same data
 and seed: the device on its FP64 path through the C ABI, the oracle in plain FP64 C++ on the same
-Philox schedule.  After every iteration the labels and sub-labels the two sweeps drew, the cluster
+Philox schedule.  The device sampler runs in both of its modes: every iteration on the device with
+the decisions in a device kernel, and the decisions on the host between device launches.  After every iteration the labels and sub-labels the two sweeps drew, the cluster
 count, the per-cluster counts, the mixture weights and the final labels must be identical.  A
 difference is allowed only where it starts: at a point whose winning margin in the oracle's sweep
 is below 1e-12 relative (FP64 rounding can decide such a tie either way); the comparison stops
@@ -35,7 +36,8 @@ CASES = {
 }
 
 
-def _lockstep(name, X, gt, seed, alpha, v0, iters=100, init_clusters=1, outlier=None, min_nmi=0.95):
+def _lockstep(name, X, gt, seed, alpha, v0, iters=100, init_clusters=1, outlier=None, min_nmi=0.95,
+              host_decisions=False):
     """Device (FP64) and oracle samplers side by side; see the module docstring."""
     D = X.shape[1]
     dev = dp.DeviceSweep(X, dp.prior_type.Gaussian, seed=seed)
@@ -46,7 +48,7 @@ def _lockstep(name, X, gt, seed, alpha, v0, iters=100, init_clusters=1, outlier=
         kw = dict(outlier_weight=w, outlier_params=dp.niw_hyperparams(ok0, om0, ov0, opsi0))
         okw = dict(outlier_weight=w, outlier_params=(ok0, om0, ov0, opsi0))
     smp = dp.Sampler(dev, hyper, alpha, iters, seed=seed, precision=dp.Precision.FP64, init_clusters=init_clusters,
-                     **kw)
+                     host_decisions=host_decisions, **kw)
     smp.trace_sweeps(True)
     om = orc.Model(X, prior=0, alpha=alpha, iters=iters, seed=seed, k0=1.0, v0=v0, init_clusters=init_clusters,
                    trace=True, **okw)
@@ -76,7 +78,7 @@ def _lockstep(name, X, gt, seed, alpha, v0, iters=100, init_clusters=1, outlier=
         np.testing.assert_array_equal(sub, osub2, err_msg=f"iteration {t}: sub-labels after the step")
         np.testing.assert_allclose(w, ow, rtol=1e-12, atol=0, err_msg=f"iteration {t}: mixture weights")
         khist.append(K)
-    print(f"{name}: identical for all {iters} iterations; K history {khist}")
+    print(f"{name}: identical for all {iters} iterations; K history {khist}; events {smp.events()}")
     lab = dev.get_labels()[0]
     if min_nmi is not None:
         assert dp._nmi(gt, lab) > min_nmi
@@ -86,12 +88,13 @@ def _lockstep(name, X, gt, seed, alpha, v0, iters=100, init_clusters=1, outlier=
     return khist, lab, w
 
 
+@pytest.mark.parametrize("mode", ["device", "host"])
 @pytest.mark.parametrize("case", sorted(CASES))
-def test_lockstep_fp64_against_oracle(case):
+def test_lockstep_fp64_against_oracle(case, mode):
     c = CASES[case]
     X, gt, _ = gaussian_mixture(c["N"], c["D"], c["K"], seed=c["data_seed"], dtype=np.float64)
-    say(f"lockstep {case}: N={c['N']} D={c['D']} FP64, 100 iterations")
-    _lockstep(case, X, gt, c["seed"], c["alpha"], c["v0"])
+    say(f"lockstep {case} ({mode} decisions): N={c['N']} D={c['D']} FP64, 100 iterations")
+    _lockstep(case, X, gt, c["seed"], c["alpha"], c["v0"], host_decisions=mode == "host")
 
 
 @pytest.mark.parametrize("init_clusters", [1, 3])
@@ -120,19 +123,20 @@ def test_lockstep_outlier_mode_against_oracle(init_clusters):
             assert khist[-1] == 4 and dp._nmi(gt, lab) > 0.95
 
 
-def test_cluster_cap_stops_splitting_and_run_continues():
+@pytest.mark.parametrize("mode", ["device", "host"])
+def test_cluster_cap_stops_splitting_and_run_continues(mode):
     """A context whose kernels take at most `cap` clusters (lowered with dpmm_limit_clusters): the
     sampler treats the cap as max_clusters (no splits from an iteration that starts at it) and
     accepts, within one iteration, only as many splits as fit, so K never exceeds the cap and the
     run finishes instead of failing in the next sweep."""
     D, cap = 8, 4
     X, gt, _ = gaussian_mixture(60_000, D, 8, seed=5, dtype=np.float32)
-    say(f"cluster cap {cap}")
+    say(f"cluster cap {cap} ({mode} decisions)")
     dev = dp.DeviceSweep(X, dp.prior_type.Gaussian, seed=4)
     dev.limit_clusters(cap)
     assert dev.max_clusters() == cap
     hyper = dp.niw_hyperparams(1.0, np.zeros(D), D + 3.0, np.eye(D))
-    smp = dp.Sampler(dev, hyper, 10.0, 60, seed=4, init_clusters=3)
+    smp = dp.Sampler(dev, hyper, 10.0, 60, seed=4, init_clusters=3, host_decisions=mode == "host")
     ks = []
     for t in range(1, 61):
         smp.step(t)

@@ -338,6 +338,9 @@ def main():
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp32", "fp64"])
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--host-decisions", action="store_true",
+                    help="reset / split / merge decisions on the host between device launches (default: on the "
+                         "device, one synchronisation per iteration)")
     ap.add_argument("--graph", action="store_true",
                     help="record each iteration's sweep segment into a hipGraph (no per-phase timings)")
     ap.add_argument("--seeds", type=int, default=0,
@@ -395,9 +398,11 @@ def main():
         del Y
     max_burn = max(400, args.burn_in)
     # the schedule's final / no-more-splits iterations lie beyond every iteration run here
-    iters = max_burn + args.warmup + args.steps + 10
+    iters = max_burn + args.warmup + args.steps + 20
     dev.set_graph_mode(args.graph)
-    smp = dp.Sampler(dev, hyper, cfg["alpha"], iters, seed=args.seed + 1, precision=prec)
+    smp = dp.Sampler(dev, hyper, cfg["alpha"], iters, seed=args.seed + 1, precision=prec,
+                     host_decisions=args.host_decisions)
+    decisions = "host" if args.host_decisions else "device"
     t_w = time.perf_counter()
     t, last_change, k_prev = 0, 0, smp.num_clusters()
     while t < max_burn and (t < args.burn_in or t - last_change < 20):
@@ -428,25 +433,32 @@ def main():
         kc = smp.num_clusters()
         ks.append(kc)
         smp.step(t)
-        a, b, c = dev.kernel_ms()
+        a, b, c = dev.kernel_ms()   # HIP events of the phases (the step has synchronised already)
         lab_ms.append(a); sub_ms.append(b); st_ms.append(c); host_ms.append(smp.host_ms())
-        if prec == dp.Precision.MIXED:
-            rechk.append(dev.last_recheck_count())
-            if not mult:
-                srechk.append(dev.last_sub_recheck_count())
-                evfrac.append(dev.last_label_evaluations())
         flops.append(2.0 * kc * D * (1 if mult else D) * N)
     dev.synchronize()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     ms_step = 1000 * elapsed / args.steps
     value = N * args.steps / elapsed
+    # the mixed sweeps' recheck and pruning counters, read after a few untimed steps (each read is a copy
+    # and a synchronisation of its own)
+    t = first + args.steps - 1
+    if prec == dp.Precision.MIXED:
+        for _ in range(5):
+            t += 1
+            smp.step(t)
+            rechk.append(dev.last_recheck_count())
+            if not mult:
+                srechk.append(dev.last_sub_recheck_count())
+                evfrac.append(dev.last_label_evaluations())
     if args.graph:   # phase events are not recorded inside a graph: the whole step only
         inst, upd = dev.graph_counts()
         print(json.dumps({"metric": "point*iters/sec", "value": value, "unit": "point*iters/s", "n_gpus": 1,
                           "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
                           "higher_is_better": True, "config": {"workload": args.config, "precision": args.precision,
-                                                               "graph_mode": True},
+                                                               "graph_mode": True, "decisions": decisions,
+                                                               "clusters_timed": int(round(float(np.mean(ks))))},
                           "graphs_instantiated": inst, "graphs_updated_in_place": upd,
                           "final_clusters": smp.num_clusters()}), flush=True)
         return
@@ -517,11 +529,15 @@ def main():
                                (f"{args.config}: Gaussian DPMM sampler iteration, N={N}, D={D}, "
                                 f"{cfg['K']} true clusters, alpha={cfg['alpha']}, NIW(k=1, m=0, v=D+3, psi=I)"),
                    "N": N, "D": D, "clusters_timed": k_timed, "clusters_after": K_final,
-                   "burn_in_iterations": burn_in, "precision": args.precision, "parallelism": "single GPU"},
+                   "burn_in_iterations": burn_in, "precision": args.precision, "decisions": decisions,
+                   "parallelism": "single GPU"},
         "roofline": roof,
         "roofline_by_phase": phase,
-        "breakdown_ms": {"label_sweep": avg_lab, "sub_sweep": avg_sub, "statistics": avg_st,
-                         "conjugate": float(np.mean(host_ms))},
+        # the rest of a step: parameter draw, log MLs, decisions, relabel passes, launch gaps and the
+        # synchronisation(s); with host decisions, `host_decisions` of it is host time between launches
+        "breakdown_ms": dict({"label_sweep": avg_lab, "sub_sweep": avg_sub, "statistics": avg_st,
+                              "rest": ms_step - avg_lab - avg_sub - avg_st},
+                             **({"host_decisions": float(np.mean(host_ms))} if args.host_decisions else {})),
         "final_nmi_vs_truth": nmi,
         "recheck_fraction": (float(np.mean(rechk)) / N) if rechk else None,
         "sub_recheck_fraction": (float(np.mean(srechk)) / N) if srechk else None,
@@ -530,7 +546,7 @@ def main():
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline and flip check (FP64 oracle)", file=sys.stderr, flush=True)
         out["cpu_baseline"], out["oracle_check"] = cpu_baseline_and_check(
-            dev, smp, X, N, mult, args.seed + 1, first + args.steps)
+            dev, smp, X, N, mult, args.seed + 1, t + 1)
     print(json.dumps(out), flush=True)
 
 

@@ -1820,8 +1820,10 @@ int dpmm_log_likelihoods(dpmm_ctx* c, int K, int which, int64_t first, int64_t n
     return DPMM_OK;
 }
 
-// Statistics kernels for K clusters (partition first if needed), nothing read back.
-static int enqueue_stats(dpmm_ctx* c, int K) {
+// Statistics kernels for K clusters (partition first if needed), nothing read back.  dev_mask (K ints
+// on the device, or null): after a relabel that only redrew the sub-labels of the clusters flagged
+// nonzero, the pass recomputes those clusters only when the chunk partials of the others are current.
+static int enqueue_stats(dpmm_ctx* c, int K, const int32_t* dev_mask = nullptr) {
     if (!c->partition_valid || c->Kpartition != K) {
         int rc = build_partition(c, K, 0);
         if (rc) return rc;
@@ -1837,10 +1839,13 @@ static int enqueue_stats(dpmm_ctx* c, int K) {
         rc = grow(c->dStats, c->statsCap, (size_t)2 * K * E);
         if (c->dStats != stats_before) c->Kstats = 0;   // nor statistics
         if (rc) return rc;
-        const bool partial = c->dirty_pending && c->partials_valid && c->partialsK == K && c->Kstats == K &&
-                             (int)c->dirty_host.size() == K && !t_capturing;
+        const bool partial_dev = dev_mask && c->partials_valid && c->partialsK == K && c->Kstats == K && !t_capturing;
+        const bool partial = partial_dev || (c->dirty_pending && c->partials_valid && c->partialsK == K &&
+                                             c->Kstats == K && (int)c->dirty_host.size() == K && !t_capturing);
         c->stats_mask = nullptr;
-        if (partial) {
+        if (partial_dev) {
+            c->stats_mask = dev_mask;
+        } else if (partial) {
             if ((rc = grow(c->dDirty, c->dirtyCap, (size_t)K))) return rc;
             HIPCHK(hipMemcpyAsync(c->dDirty, c->dirty_host.data(), (size_t)K * 4, hipMemcpyHostToDevice, c->stream));
             c->stats_mask = c->dDirty;
@@ -2767,14 +2772,22 @@ int step_iteration(dpmm_ctx* c, const StepArgs& args, int sweep_final, int full_
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(c->stream));
     StepSummary s = *c->hStepSum;
-    if (s.followup || s.relabel2) labels_rewritten(c);
+    if (s.followup && s.nsplit == 0) {
+        // resets only: the labels and the partition stand, the reset clusters' sub-labels changed
+        c->subs_perm_valid = false;
+        c->stats_cached = false;
+        c->dirty_pending = false;
+    } else if (s.followup || s.relabel2) {
+        labels_rewritten(c);
+    }
     if ((rc = step_error(s))) return rc;
     if (s.followup) {
-        // the statistics and log ML of the reset and split clusters, then the merges and removals
+        // the statistics and log ML of the reset and split clusters, then the merges and removals; with
+        // resets only, the pass recomputes the reset clusters (their relabel actions are the mask)
         const int Ks = s.K_split;
         if ((rc = grow(c->dStepJobs, c->stepJobsCap, (size_t)12 * Ks))) return rc;
         if ((rc = grow(c->dLM, c->lmCap, (size_t)6 * Ks))) return rc;
-        if ((rc = enqueue_stats(c, Ks))) return rc;
+        if ((rc = enqueue_stats(c, Ks, s.nsplit == 0 ? c->dStepTab + 2 * K : nullptr))) return rc;
         hipLaunchKernelGGL(k_step_touched_jobs, dim3(1), dim3(kStepThreads), 0, c->stream, Ks, a.E, c->dStepCl,
                            c->dStats, c->dStepCand, c->dStepJobs, c->dStepCounts);
         if ((rc = launch_log_marginal_dev(c, 3 * Ks, c->dStepJobs, c->dStepCounts))) return rc;

@@ -69,7 +69,7 @@ __device__ __forceinline__ void step_pair(int64_t p, int n, int& ra, int& rb) {
 }
 
 // Two-component Dirichlet on stream (c0, t, tag): the gammas in order, each divided by their sum.
-__device__ __forceinline__ void step_dirichlet2(uint32_t c0, uint32_t t, uint32_t tag, uint32_t k0, uint32_t k1,
+__device__ __noinline__ void step_dirichlet2(uint32_t c0, uint32_t t, uint32_t tag, uint32_t k0, uint32_t k1,
                                                 double a0, double a1, double* out) {
 #pragma clang fp contract(off)
     DevStream rs(c0, t, tag, k0, k1, 0);
@@ -92,7 +92,15 @@ __global__ void __launch_bounds__(kStepThreads) k_step_sample(StepArgs a, StepCl
     __shared__ double g[kStepMaxK + 1];
     __shared__ double s_sum;
     const int K = a.K, B = a.B;
-    for (int k = threadIdx.x; k < K; k += blockDim.x) {
+    // tasks 0 .. K-1: the clusters' sub-cluster weights and histories; K .. 2K: the mixture's gammas
+    for (int task = threadIdx.x; task <= 2 * K; task += blockDim.x) {
+        if (task >= K) {
+            const int i = task - K;
+            DevStream rs((uint32_t)i, a.t, TAG_MIXW, a.key0, a.key1, 0);
+            g[i] = rs.gamma(i < K ? (double)cl[i].c.N : a.alpha);
+            continue;
+        }
+        const int k = task;
         StepCluster& c = cl[k];
         step_dirichlet2((uint32_t)k, a.t, TAG_LRW, a.key0, a.key1, (double)c.l.N + a.alpha / 2,
                         (double)c.r.N + a.alpha / 2, c.lrw);
@@ -117,10 +125,6 @@ __global__ void __launch_bounds__(kStepThreads) k_step_sample(StepArgs a, StepCl
             for (int e = 0; e < 4; ++e) members[4 * (3 * k + p) + e] = ps[p]->mem[e];
         logw[K + 2 * k] = log(c.lrw[0]);
         logw[K + 2 * k + 1] = log(c.lrw[1]);
-    }
-    for (int i = threadIdx.x; i <= K; i += blockDim.x) {
-        DevStream rs((uint32_t)i, a.t, TAG_MIXW, a.key0, a.key1, 0);
-        g[i] = rs.gamma(i < K ? (double)cl[i].c.N : a.alpha);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -188,6 +192,50 @@ __global__ void __launch_bounds__(kStepThreads) k_step_jobs(StepArgs a, StepClus
     }
 }
 
+// split_cluster_local (local_clusters_actions.cpp:183-194): k keeps the left half, nk gets the right
+// half; each half is create_splittable_from_params of the old sub-cluster (shared_actions.cpp:57-70)
+// with sub-cluster weights ~ Dir(alpha / 2, alpha / 2), its parts recomputed from its new sub-labels.
+__device__ __noinline__ void step_split(const StepArgs& a, StepCluster* __restrict__ cl, int k, int nk) {
+    StepCluster& c = cl[k];
+    StepCluster& R = cl[nk];
+    const StepPart pl = c.l, pr = c.r;
+    R.c = R.l = R.r = pr;
+    c.c = c.l = c.r = pl;
+    R.splittable = c.splittable = 0;
+    R.hist_n = c.hist_n = 0;
+    R.hist_head = c.hist_head = 0;
+    R.touched = c.touched = 1;
+    step_dirichlet2((uint32_t)k, a.t, TAG_SPLIT_LRW, a.key0, a.key1, a.alpha / 2, a.alpha / 2, c.lrw);
+    step_dirichlet2((uint32_t)nk, a.t, TAG_SPLIT_LRW, a.key0, a.key1, a.alpha / 2, a.alpha / 2, R.lrw);
+}
+
+// merge_clusters + merge_clusters_to_splittable (local_clusters_actions.cpp:204-213, shared_actions.cpp:
+// 72-86): i becomes the merged cluster with log ML L, its sub-clusters the old i (a clone of the merged
+// part, as the reference leaves it) and j, sub-cluster weights ~ Dir(N_i + alpha / 2, N_j + alpha / 2);
+// j is emptied.
+__device__ __noinline__ void step_merge(const StepArgs& a, StepCluster* __restrict__ cl, int i, int j, double L) {
+    StepCluster& A = cl[i];
+    StepCluster& Bc = cl[j];
+    StepPart m;
+    m.N = A.c.N + Bc.c.N;
+    m.mem[0] = 2 * i;
+    m.mem[1] = 2 * i + 1;
+    m.mem[2] = 2 * j;
+    m.mem[3] = 2 * j + 1;
+    m.lml = L;
+    m.ld = 0.0;
+    step_dirichlet2((uint32_t)i, a.t, TAG_MERGE_LRW, a.key0, a.key1, (double)A.c.N + 0.5 * a.alpha,
+                    (double)Bc.c.N + 0.5 * a.alpha, A.lrw);
+    A.r = Bc.c;
+    A.c = m;
+    A.l = m;
+    A.splittable = 0;
+    A.hist_n = 0;
+    A.hist_head = 0;
+    Bc.c.N = 0;
+    Bc.splittable = 0;
+}
+
 // Decisions of the iteration over the jobs' log MLs lm (2 doubles per job: log ML, log det psi_n).
 // tab1: the reset + split relabel table over the K clusters (new label of (k, s) at 2k + s, action of
 // cluster k at 2K + k: 1 reset, 2 split); tab2: the merge + removal table over the K_split clusters
@@ -206,6 +254,7 @@ __global__ void __launch_bounds__(kStepThreads) k_step_decide(StepArgs a, StepCl
     __shared__ int ws[16];
     __shared__ int s_err, s_nreset, s_nmerge;
     __shared__ unsigned char used[kStepMaxK];
+    __shared__ double lgn[kStepMaxK], lgnh[kStepMaxK];
     const int K = a.K;
     if (threadIdx.x == 0) {
         s_err = 0;
@@ -265,20 +314,8 @@ __global__ void __launch_bounds__(kStepThreads) k_step_decide(StepArgs a, StepCl
             int tot;
             const int r = step_rank(acc, ws, tot);
             if (acc && nsplit + r < room) {
-                // split_cluster_local: k keeps the left half, K + rank gets the right half; each half is
-                // create_splittable_from_params of the old sub-cluster with sub-cluster weights
-                // ~ Dir(alpha / 2, alpha / 2); its parts are recomputed from its new sub-labels
                 const int nk = K + nsplit + r;
-                StepCluster& c = cl[k];
-                StepCluster L = {}, R = {};
-                L.c = L.l = L.r = c.l;
-                R.c = R.l = R.r = c.r;
-                L.touched = R.touched = 1;
-                step_dirichlet2((uint32_t)k, a.t, TAG_SPLIT_LRW, a.key0, a.key1, a.alpha / 2, a.alpha / 2, L.lrw);
-                step_dirichlet2((uint32_t)nk, a.t, TAG_SPLIT_LRW, a.key0, a.key1, a.alpha / 2, a.alpha / 2,
-                                R.lrw);
-                c = L;
-                cl[nk] = R;
+                step_split(a, cl, k, nk);
                 tab1[2 * k + 1] = nk;
                 tab1[2 * K + k] = 2;
             }
@@ -299,6 +336,13 @@ __global__ void __launch_bounds__(kStepThreads) k_step_decide(StepArgs a, StepCl
         const int64_t np = (int64_t)ncand * (ncand - 1) / 2;
         const double al = a.alpha;
         const double cst = -log(al) + lgamma(al) - 2 * lgamma(0.5 * al);
+        // per-cluster terms of the ratio, once (the candidates' counts are those before the splits)
+        for (int k = threadIdx.x; k < K; k += blockDim.x) {
+            const double Nk = (double)cl[k].c.N;
+            lgn[k] = cl[k].c.N > 0 ? lgamma(Nk) : 0.0;
+            lgnh[k] = lgamma(Nk + 0.5 * al);
+        }
+        __syncthreads();
         int nacc = 0;
         for (int64_t base = 0; base < np; base += blockDim.x) {
             const int64_t p = base + threadIdx.x;
@@ -311,12 +355,17 @@ __global__ void __launch_bounds__(kStepThreads) k_step_decide(StepArgs a, StepCl
                 const StepCluster& cj = cl[j];
                 const double L = lm[2 * (3 * (int64_t)K + p)];
                 if (isfinite(L) && ci.splittable && cj.splittable && ci.c.N > 0 && cj.c.N > 0) {
-                    const double Ni = (double)ci.c.N, Nj = (double)cj.c.N, Nm = Ni + Nj;
-                    const double lhr = cst + lgamma(Nm) - lgamma(Nm + al) + lgamma(Ni + 0.5 * al) - lgamma(Ni) -
-                                       lgamma(Nj) + lgamma(Nj + 0.5 * al) + L - ci.c.lml - cj.c.lml;
-                    const double u = u01_f64(
-                        philox4x32_10(((uint32_t)i << 16) | (uint32_t)j, 0, a.t, TAG_MERGE_U, a.key0, a.key1).v[0]);
-                    acc = lhr > log(u) || (a.fin && lhr > log(0.1));
+                    // lgamma(Nm) - lgamma(Nm + alpha) < 0 for Nm >= 2: without it the ratio is an upper
+                    // bound, and a pair far below log(2^-32) and log(0.1) is rejected whatever its uniform
+                    const double ub = cst + lgnh[i] - lgn[i] - lgn[j] + lgnh[j] + L - ci.c.lml - cj.c.lml;
+                    if (!(ub < -100.0)) {
+                        const double Nm = (double)ci.c.N + (double)cj.c.N;
+                        const double lhr = cst + lgamma(Nm) - lgamma(Nm + al) + lgnh[i] - lgn[i] - lgn[j] + lgnh[j] +
+                                           L - ci.c.lml - cj.c.lml;
+                        const double u = u01_f64(philox4x32_10(((uint32_t)i << 16) | (uint32_t)j, 0, a.t, TAG_MERGE_U,
+                                                               a.key0, a.key1).v[0]);
+                        acc = lhr > log(u) || (a.fin && lhr > log(0.1));
+                    }
                 }
             }
             int tot;
@@ -336,28 +385,7 @@ __global__ void __launch_bounds__(kStepThreads) k_step_decide(StepArgs a, StepCl
                 used[i] = used[j] = 1;
                 merge_into[j] = i;
                 ++nm;
-                // merge_clusters + merge_clusters_to_splittable: i becomes the merged cluster, its
-                // sub-clusters the old i (a clone of the merged part, as the reference leaves it) and j
-                StepCluster& A = cl[i];
-                StepCluster& Bc = cl[j];
-                StepPart m;
-                m.N = A.c.N + Bc.c.N;
-                m.mem[0] = 2 * i;
-                m.mem[1] = 2 * i + 1;
-                m.mem[2] = 2 * j;
-                m.mem[3] = 2 * j + 1;
-                m.lml = lm[2 * (3 * (int64_t)K + p)];
-                m.ld = 0.0;
-                step_dirichlet2((uint32_t)i, a.t, TAG_MERGE_LRW, a.key0, a.key1, (double)A.c.N + 0.5 * al,
-                                (double)Bc.c.N + 0.5 * al, A.lrw);
-                A.r = Bc.c;
-                A.c = m;
-                A.l = m;
-                A.splittable = 0;
-                A.hist_n = 0;
-                A.hist_head = 0;
-                Bc.c.N = 0;
-                Bc.splittable = 0;
+                step_merge(a, cl, i, j, lm[2 * (3 * (int64_t)K + p)]);
             }
             s_nmerge = nm;
         }

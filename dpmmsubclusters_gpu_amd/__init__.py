@@ -102,6 +102,8 @@ def load_library():
         "dpmm_last_recheck_count": [_P, _P],
         "dpmm_last_sub_recheck_count": [_P, _P],
         "dpmm_set_label_pruning": [_P, i],
+        "dpmm_set_statistics_refresh": [_P, i],
+        "dpmm_last_statistics_update": [_P, _P],
         "dpmm_last_label_evaluations": [_P, _P],
         "dpmm_synchronize": [_P],
         "dpmm_set_prior": [_P, dbl, _P, dbl, _P, _P],
@@ -394,6 +396,18 @@ class DeviceSweep:
         else:
             m = 1 if mode == "adaptive" else int(mode)
         _check(load_library().dpmm_set_label_pruning(self._ctx, m))
+
+    def set_statistics_refresh(self, every):
+        """Statistics passes per full recomputation; the passes between update the statistics from the
+        points whose sub-label moved (behind a pruned mixed label sweep that moved no label); <= 1: off."""
+        _check(load_library().dpmm_set_statistics_refresh(self._ctx, int(every)))
+
+    def last_statistics_update(self):
+        """The last statistics pass: 0 recomputed, 1 updated from the points whose sub-label moved, 2 updated
+        with the clusters a moved label touched recomputed."""
+        v = ctypes.c_int()
+        _check(load_library().dpmm_last_statistics_update(self._ctx, ctypes.byref(v)))
+        return v.value
 
     def last_label_evaluations(self):
         """Fraction of (wave, cluster) scorings the last pruned label sweep performed, -1 if none."""

@@ -166,8 +166,13 @@ __global__ void __launch_bounds__(kThreads) k_label_mult(const int32_t* __restri
 
 // ---- deterministic partition of points by label -----------------------------------------------
 // clamp != 0 files labels outside [0, K) under label 0 instead of leaving them out.
+// cond (null: always) on the partition kernels: they run only when *cond != 0, a device flag that the
+// pruned label sweep sets when a label moved; otherwise the partition of the labels before the sweep
+// stands (a stable sort of unchanged labels gives it again).
 __global__ void __launch_bounds__(kThreads) k_block_hist(const int32_t* __restrict__ labels, int64_t N, int K,
-                                                         int nblk, int clamp, int32_t* __restrict__ hist) {
+                                                         int nblk, int clamp, int32_t* __restrict__ hist,
+                                                         const int* __restrict__ cond) {
+    if (cond && !*cond) return;
     __shared__ int cnt[kMaxK];
     for (int k = threadIdx.x; k < K; k += kThreads) cnt[k] = 0;
     __syncthreads();
@@ -186,7 +191,8 @@ __global__ void __launch_bounds__(kThreads) k_block_hist(const int32_t* __restri
 
 // Exclusive scan of one label's row of block counts (one workgroup per label).
 __global__ void __launch_bounds__(kThreads) k_scan_rows(int32_t* __restrict__ hist, int nblk,
-                                                        int32_t* __restrict__ rowtot) {
+                                                        int32_t* __restrict__ rowtot, const int* __restrict__ cond) {
+    if (cond && !*cond) return;
     __shared__ int buf[kThreads];
     __shared__ int carry;
     int32_t* row = hist + (size_t)blockIdx.x * nblk;
@@ -214,8 +220,8 @@ __global__ void __launch_bounds__(kThreads) k_scan_rows(int32_t* __restrict__ hi
 
 // Segment starts per label and chunk starts (ceil(n_k / CH) chunks per label).
 __global__ void k_scan_labels(const int32_t* __restrict__ rowtot, int K, int CH, int32_t* __restrict__ label_start,
-                              int32_t* __restrict__ chunk_start) {
-    if (threadIdx.x != 0) return;
+                              int32_t* __restrict__ chunk_start, const int* __restrict__ cond) {
+    if (threadIdx.x != 0 || (cond && !*cond)) return;
     int s = 0, c = 0;
     for (int k = 0; k < K; ++k) {
         label_start[k] = s;
@@ -236,7 +242,8 @@ __global__ void k_scan_labels(const int32_t* __restrict__ rowtot, int K, int CH,
 __global__ void __launch_bounds__(kThreads) k_scatter(const int32_t* __restrict__ labels, int64_t N, int K, int nblk,
                                                       int clamp, const int32_t* __restrict__ hist,
                                                       const int32_t* __restrict__ label_start,
-                                                      int32_t* __restrict__ perm) {
+                                                      int32_t* __restrict__ perm, const int* __restrict__ cond) {
+    if (cond && !*cond) return;
     extern __shared__ __align__(16) unsigned char smem[];
     int* base = reinterpret_cast<int*>(smem);
     int* wc = base + K;   // [4][K]
@@ -498,6 +505,72 @@ __global__ void __launch_bounds__(kThreads) k_stats_reduce(const double* __restr
     }
     for (; c < c1; ++c) v += src[(size_t)(2 * c) * E];
     stats[(size_t)q * E + e] = v;
+}
+
+// The reduction of a pass in update mode (StatsGate): a cluster whose points kept their labels gets
+// d, the change of entry (k, 0), summed over its chunks in chunk order: entry (k, 0) gains d and (k, 1)
+// loses it.  With signed_slot the chunk partials of slot 0 hold d's terms (points moved into 0 counted
+// +, into 1 -); else slot s holds the sums of the points that moved into s and d = slot 0 - slot 1.  A
+// cluster with moved[k] != 0 gets the full sums of both slots as k_stats_reduce forms them.
+__global__ void __launch_bounds__(kThreads) k_stats_update(const double* __restrict__ partials,
+                                                           const int32_t* __restrict__ chunk_start, int K, int E,
+                                                           double* __restrict__ stats,
+                                                           const int32_t* __restrict__ moved, int signed_slot,
+                                                           const int32_t* __restrict__ dirty) {
+    const int k = blockIdx.x;
+    const int e = blockIdx.y * kThreads + threadIdx.x;
+    if (e >= E || (dirty && !dirty[k])) return;
+    const int c0 = chunk_start[k], c1 = chunk_start[k + 1];
+    const bool full = moved[k] != 0, two = full || !signed_slot;
+    double v0 = 0.0, v1 = 0.0;
+    const double* src = partials + e;
+    int c = c0;
+    for (; c + 8 <= c1; c += 8) {   // eight chunks' loads in flight, added in chunk order
+        double t0[8], t1[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            t0[u] = src[(size_t)(2 * (c + u)) * E];
+            t1[u] = two ? src[(size_t)(2 * (c + u) + 1) * E] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            v0 += t0[u];
+            v1 += t1[u];
+        }
+    }
+    for (; c < c1; ++c) {
+        v0 += src[(size_t)(2 * c) * E];
+        if (two) v1 += src[(size_t)(2 * c + 1) * E];
+    }
+    if (full) {
+        stats[(size_t)(2 * k) * E + e] = v0;
+        stats[(size_t)(2 * k + 1) * E + e] = v1;
+    } else {
+        const double d = signed_slot ? v0 : v0 - v1;
+        stats[(size_t)(2 * k) * E + e] += d;
+        stats[(size_t)(2 * k + 1) * E + e] -= d;
+    }
+}
+
+// After a relabel that redrew the sub-labels of the clusters flagged in `mask` (resets), the sub-labels
+// in partition order of those clusters' chunks: the old ones into prev, the new ones (by index) into cur.
+__global__ void __launch_bounds__(kThreads) k_refresh_subs_perm(int K, int CH, const int32_t* __restrict__ perm,
+                                                                const int32_t* __restrict__ label_start,
+                                                                const int32_t* __restrict__ chunk_start,
+                                                                const int32_t* __restrict__ subs,
+                                                                const int32_t* __restrict__ mask,
+                                                                int32_t* __restrict__ cur, int32_t* __restrict__ prev) {
+    int k, beg, end;
+    if (!chunk_of(blockIdx.x, K, chunk_start, label_start, CH, k, beg, end) || !mask[k]) return;
+    for (int p = beg + threadIdx.x; p < end; p += kThreads) {
+        prev[p] = cur[p];
+        cur[p] = subs[perm[p]];
+    }
+}
+
+__global__ void k_copy_int(const int32_t* __restrict__ src, int32_t* __restrict__ dst, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
 }
 
 __global__ void k_zero_int(int* p, int n) {
@@ -776,6 +849,26 @@ struct dpmm_ctx {
     int32_t* dPerm = nullptr;
     int32_t* dTable = nullptr;
     size_t tableCap = 0;
+    // incremental statistics (behind the pruned mixed label sweep): that sweep sets dFlags[0] when a label
+    // moved and dFlags[1 + k] for both clusters of every move; dSubsPermPrev keeps the previous pass's
+    // sub-labels in the order of its partition, whose segment starts dPrevStart keeps.  inc_base: dStats
+    // holds the statistics of the current partition with the sub-labels in dSubsPerm.
+    int* dFlags = nullptr;
+    size_t flagsCap = 0;
+    int32_t* dSubsPermPrev = nullptr;
+    size_t subsPermPrevCap = 0;
+    int32_t* dPrevStart = nullptr;
+    size_t prevStartCap = 0;
+    bool label_flag = false;   // the last label sweep set dFlags[0] and left the partition of the labels before it
+    bool inc_base = false;
+    bool inc_ready = false;    // this iteration's sub-label sweep moved the baseline sub-labels to dSubsPermPrev
+    int stats_refresh = 8;     // statistics passes per full pass (<= 1: every pass full)
+    int inc_passes = 0;        // passes since the last full one
+    StatsGate stats_gate{};    // of the statistics kernels being enqueued
+    bool last_inc = false;     // the last statistics pass enqueued the update from the moved points
+    // a relabel that redrew only the sub-labels of some clusters (resets) came after a pass that left
+    // inc_base: the next pass updates those clusters from their moved sub-labels
+    bool reset_update_ok = false;
     // statistics
     double* dPartials = nullptr;
     size_t partialsCap = 0;
@@ -975,21 +1068,22 @@ int chunks_upper(const dpmm_ctx* c, int K) { return (int)((c->N + c->CH - 1) / c
 
 int allow_lds(const void* kernel, size_t bytes);
 
-int build_partition(dpmm_ctx* c, int K, int clamp) {
+// cond (null: always): the partition kernels run only when *cond != 0 on the device (see k_block_hist).
+int build_partition(dpmm_ctx* c, int K, int clamp, const int* cond = nullptr) {
     int rc = ensure_partition(c, K);
     if (rc) return rc;
     c->partials_valid = false;   // the chunk layout may change
     c->CH = chunk_size(c);
     hipLaunchKernelGGL(k_block_hist, dim3(c->nblk), dim3(kThreads), 0, c->stream, c->dLabels, c->N, K, c->nblk, clamp,
-                       c->dHist);
-    hipLaunchKernelGGL(k_scan_rows, dim3(K), dim3(kThreads), 0, c->stream, c->dHist, c->nblk, c->dRowtot);
+                       c->dHist, cond);
+    hipLaunchKernelGGL(k_scan_rows, dim3(K), dim3(kThreads), 0, c->stream, c->dHist, c->nblk, c->dRowtot, cond);
     hipLaunchKernelGGL(k_scan_labels, dim3(1), dim3(64), 0, c->stream, c->dRowtot, K, c->CH, c->dLabelStart,
-                       c->dChunkStart);
+                       c->dChunkStart, cond);
     const size_t scatter_lds = (((size_t)5 * K * 4 + 15) & ~(size_t)15) + (size_t)4 * K * 8;
     int lrc = allow_lds(reinterpret_cast<const void*>(k_scatter), scatter_lds);
     if (lrc) return lrc;
     hipLaunchKernelGGL(k_scatter, dim3(c->nblk), dim3(kThreads), scatter_lds, c->stream, c->dLabels, c->N, K, c->nblk,
-                       clamp, c->dHist, c->dLabelStart, c->dPerm);
+                       clamp, c->dHist, c->dLabelStart, c->dPerm, cond);
     HIPCHK(hipGetLastError());
     c->partition_valid = true;
     c->Kpartition = K;
@@ -1283,7 +1377,7 @@ void run_stats_gauss(dpmm_ctx* c) {
     const int K = c->Kpartition;
     launch_stats_gauss<TX, DP>(c->stream, chunks_upper(c, K), reinterpret_cast<const TX*>(c->dX), c->D, K, c->CH,
                                c->dPerm, c->dLabelStart, c->dChunkStart, c->dSubs,
-                               c->subs_perm_valid ? c->dSubsPerm : nullptr, c->dPartials, c->stats_mask);
+                               c->subs_perm_valid ? c->dSubsPerm : nullptr, c->dPartials, c->stats_mask, c->stats_gate);
 }
 
 #define DPMM_DISPATCH_DP(DPV, FN, ...)            \
@@ -1324,7 +1418,7 @@ template <int DP> void lgl_f32(dpmm_ctx* c, uint32_t it, int f, int grid) {
 template <int DP> void lglp_f32(dpmm_ctx* c, uint32_t it, int f, int grid) {
     launch_label_gauss_listp<float, DP>(c->stream, grid, reinterpret_cast<const float*>(c->dX), c->N, c->D, c->K,
                                         reinterpret_cast<const float*>(c->dP), c->dPruneM, c->dPruneSC, c->key0(),
-                                        c->key1(), it, f, c->dRecheck, c->dRecheckCount, c->dLabels);
+                                        c->key1(), it, f, c->dRecheck, c->dRecheckCount, c->dLabels, c->dFlags);
 }
 
 template <int DP> void sgl_f32(dpmm_ctx* c, uint32_t it, int grid) {
@@ -1415,8 +1509,13 @@ int ensure_prune(dpmm_ctx* c) {
 int run_label_mixed_pruned(dpmm_ctx* c, uint32_t iter, int fin) {
     int rc = ensure_prune(c);
     if (rc) return rc;
-    if ((rc = build_partition(c, c->K, 1))) return rc;
+    // the partition the last sub-label sweep used is the one of the current labels unless a relabel
+    // came since (which clears partition_valid)
+    if (!(c->partition_valid && c->Kpartition == c->K) && (rc = build_partition(c, c->K, 1))) return rc;
     if ((rc = grow(c->dEvals, c->evalsCap, 2))) return rc;
+    if ((rc = grow(c->dFlags, c->flagsCap, (size_t)kMaxK + 1))) return rc;
+    // dFlags[0]: a label moved; dFlags[1 + k]: a label moved into or out of cluster k
+    hipLaunchKernelGGL(k_zero_int, dim3((c->K + 64) / 64), dim3(64), 0, c->stream, c->dFlags, c->K + 1);
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, c->dRecheckCount, 1);
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, reinterpret_cast<int*>(c->dEvals), 4);
     const int grid = chunks_upper(c, c->K) * kPruneSplit;
@@ -1425,7 +1524,7 @@ int run_label_mixed_pruned(dpmm_ctx* c, uint32_t iter, int fin) {
 #define DPMM_MXP(DPMV)                                                                                          \
     launch_label_gauss_mxp<DPMV>(c->stream, grid, X, c->D, c->K, c->CH, c->dPerm, c->dLabelStart, c->dChunkStart, \
                                  PB, c->dPruneM, c->dPruneSC, c->tau, c->key0(), c->key1(), iter, fin, c->dLabels, \
-                                 c->dRecheck, c->dRecheckCount, c->dEvals)
+                                 c->dRecheck, c->dRecheckCount, c->dEvals, c->dFlags)
     switch (c->DPM) {
         case 32: DPMM_MXP(32); break;
         case 64: DPMM_MXP(64); break;
@@ -1437,6 +1536,7 @@ int run_label_mixed_pruned(dpmm_ctx* c, uint32_t iter, int fin) {
     const int grid2 = (int)std::min<int64_t>(1024, (c->N + kThreads - 1) / kThreads);
     DPMM_DISPATCH_DP(c->DP, lglp_f32, c, iter, fin, grid2);   // the recheck pruned with the same bounds
     c->last_pruned_K = c->K;
+    c->label_flag = true;   // dFlags[0]: whether the partition of the labels before the sweep still stands
     return DPMM_OK;
 }
 
@@ -1531,7 +1631,8 @@ int dpmm_destroy(dpmm_ctx* c) {
                     (void*)c->dPrior1, (void*)c->dPsel, (void*)c->dCtl, (void*)c->dDrawsPrev, (void*)c->dSubsPerm,
                     (void*)c->dStepCl, (void*)c->dStepClTmp, (void*)c->dStepHist, (void*)c->dStepHistTmp,
                     (void*)c->dStepW, (void*)c->dStepMem, (void*)c->dStepCand, (void*)c->dStepCounts,
-                    (void*)c->dStepTab, (void*)c->dStepJobs, (void*)c->dStepAcc, (void*)c->dStepSum})
+                    (void*)c->dStepTab, (void*)c->dStepJobs, (void*)c->dStepAcc, (void*)c->dStepSum,
+                    (void*)c->dFlags, (void*)c->dSubsPermPrev, (void*)c->dPrevStart})
         if (p) (void)hipFree(p);
     if (c->hStepSum) (void)hipHostFree(c->hStepSum);
     if (c->hStats) (void)hipHostFree(c->hStats);
@@ -1598,6 +1699,8 @@ int dpmm_sample_initial_labels(dpmm_ctx* c, int K0) {
     c->partition_valid = false;
     c->subs_perm_valid = false;
     c->stats_cached = false;
+    c->inc_base = false;
+    c->reset_update_ok = false;
     c->partials_valid = false;
     return DPMM_OK;
 }
@@ -1609,6 +1712,8 @@ int dpmm_set_labels(dpmm_ctx* c, const int32_t* labels, const int32_t* subs) {
     c->partition_valid = false;
     c->subs_perm_valid = false;
     c->stats_cached = false;
+    c->inc_base = false;
+    c->reset_update_ok = false;
     c->partials_valid = false;
     return DPMM_OK;
 }
@@ -1705,6 +1810,7 @@ int dpmm_sample_cluster_labels(dpmm_ctx* c, uint32_t iter, int fin) {
     if (c->K <= 0) return fail(DPMM_ESTATE, "parameters not set");
     if (!t_capturing) (void)hipEventRecord(c->ev[0], c->stream);
     c->last_pruned_K = 0;
+    c->label_flag = false;
     if (c->prior == DPMM_PRIOR_GAUSSIAN && c->precision == DPMM_PREC_MIXED) {
         bool prune = c->prune_mode != 0 && c->K >= 2 && c->D % 4 == 0;   // 16-byte rows for the gathers
         // adaptive: small sweeps (N K DP^2 below 4e11 multiply-adds) stay unpruned, the fixed cost
@@ -1749,17 +1855,39 @@ int dpmm_sample_cluster_labels(dpmm_ctx* c, uint32_t iter, int fin) {
 int dpmm_sample_sub_labels(dpmm_ctx* c, uint32_t iter) {
     if (!c) return fail(DPMM_EINVAL, "null ctx");
     if (c->K <= 0) return fail(DPMM_ESTATE, "parameters not set");
-    if (!c->partition_valid || c->Kpartition != c->K) {
-        int rc = build_partition(c, c->K, 0);
-        if (rc) return rc;
+    int rc;
+    const bool mixed = c->prior == DPMM_PRIOR_GAUSSIAN && c->precision == DPMM_PREC_MIXED && c->spb_valid;
+    // incremental statistics: dStats holds the statistics of the partition the label sweep started from
+    // with the sub-labels in dSubsPerm; they become the baseline (dSubsPermPrev) of this pass, which updates
+    // them from the points whose sub-label moved when no label moved (decided on the device)
+    const bool baseline = mixed && c->label_flag && c->inc_base && c->subs_perm_valid && c->Kpartition == c->K &&
+                          c->Kstats == c->K && !c->dirty_pending && c->stats_refresh > 1 &&
+                          c->inc_passes + 1 < c->stats_refresh;
+    c->inc_ready = false;
+    c->inc_base = false;
+    if (baseline) {
+        if ((rc = grow(c->dSubsPermPrev, c->subsPermPrevCap, (size_t)c->N))) return rc;
+        if ((rc = grow(c->dSubsPerm, c->subsPermCap, (size_t)c->N))) return rc;
+        if ((rc = grow(c->dPrevStart, c->prevStartCap, (size_t)kMaxK + 1))) return rc;
+        std::swap(c->dSubsPerm, c->dSubsPermPrev);
+        std::swap(c->subsPermCap, c->subsPermPrevCap);
+        // the segment starts of the partition the baseline sub-labels are ordered by
+        hipLaunchKernelGGL(k_copy_int, dim3((c->K + 64) / 64), dim3(64), 0, c->stream, c->dLabelStart, c->dPrevStart,
+                           c->K + 1);
+        c->inc_ready = true;
+    }
+    if (c->label_flag && c->Kpartition == c->K) {
+        // the partition of the labels before the sweep, rebuilt on the device only if a label moved
+        if ((rc = build_partition(c, c->K, 0, c->dFlags))) return rc;
+    } else if (!c->partition_valid || c->Kpartition != c->K) {
+        if ((rc = build_partition(c, c->K, 0))) return rc;
     }
     if (!t_capturing) (void)hipEventRecord(c->ev[2], c->stream);
     c->stats_cached = false;
     c->partials_valid = false;
     c->subs_perm_valid = false;
-    if (c->prior == DPMM_PRIOR_GAUSSIAN && c->precision == DPMM_PREC_MIXED && c->spb_valid) {
-        int rc = run_sub_mixed(c, iter);
-        if (rc) return rc;
+    if (mixed) {
+        if ((rc = run_sub_mixed(c, iter))) return rc;
     } else if (c->prior == DPMM_PRIOR_GAUSSIAN) {
         if (c->params_f64) { DPMM_DISPATCH_DP64(c->DP, sg_f64, c, iter); }
         else { DPMM_DISPATCH_DP(c->DP, sg_f32, c, iter); }
@@ -1839,6 +1967,44 @@ static int enqueue_stats(dpmm_ctx* c, int K, const int32_t* dev_mask = nullptr) 
         rc = grow(c->dStats, c->statsCap, (size_t)2 * K * E);
         if (c->dStats != stats_before) c->Kstats = 0;   // nor statistics
         if (rc) return rc;
+        if (c->reset_update_ok && (dev_mask || c->dirty_pending) && c->Kstats == K && c->partition_valid &&
+            c->Kpartition == K && c->prior == DPMM_PRIOR_GAUSSIAN && !t_capturing &&
+            (dev_mask || (int)c->dirty_host.size() == K)) {
+            // resets only: the reset clusters' statistics updated from their points whose sub-label moved
+            const int32_t* mask = dev_mask;
+            if (!mask) {
+                if ((rc = grow(c->dDirty, c->dirtyCap, (size_t)K))) return rc;
+                HIPCHK(hipMemcpyAsync(c->dDirty, c->dirty_host.data(), (size_t)K * 4, hipMemcpyHostToDevice, c->stream));
+                mask = c->dDirty;
+            }
+            if ((rc = grow(c->dSubsPermPrev, c->subsPermPrevCap, (size_t)c->N))) return rc;
+            if ((rc = grow(c->dPrevStart, c->prevStartCap, (size_t)kMaxK + 1))) return rc;
+            if ((rc = grow(c->dFlags, c->flagsCap, (size_t)kMaxK + 1))) return rc;
+            hipLaunchKernelGGL(k_refresh_subs_perm, dim3(nch), dim3(kThreads), 0, c->stream, K, c->CH, c->dPerm,
+                               c->dLabelStart, c->dChunkStart, c->dSubs, mask, c->dSubsPerm, c->dSubsPermPrev);
+            hipLaunchKernelGGL(k_copy_int, dim3((K + 64) / 64), dim3(64), 0, c->stream, c->dLabelStart, c->dPrevStart,
+                               K + 1);
+            hipLaunchKernelGGL(k_zero_int, dim3((K + 63) / 64), dim3(64), 0, c->stream, c->dFlags + 1, K);
+            c->subs_perm_valid = true;
+            c->stats_mask = mask;
+            c->stats_gate = StatsGate{c->dSubsPermPrev, c->dFlags + 1, c->dPrevStart};
+            if (c->dtype == DPMM_DTYPE_F64) { DPMM_DISPATCH_DP64(c->DP, st_f64, c); }
+            else { DPMM_DISPATCH_DP(c->DP, st_f32, c); }
+            hipLaunchKernelGGL(k_stats_update, dim3(K, (unsigned)((E + kThreads - 1) / kThreads)), dim3(kThreads), 0,
+                               c->stream, c->dPartials, c->dChunkStart, K, (int)E, c->dStats, c->dFlags + 1,
+                               (c->DP % 16 == 0 && c->DP <= 64) ? 1 : 0, mask);
+            c->stats_gate = StatsGate{nullptr, nullptr, nullptr};
+            c->stats_mask = nullptr;
+            c->stats_cached = true;
+            c->partials_valid = false;
+            c->dirty_pending = false;
+            c->reset_update_ok = false;
+            c->inc_base = true;
+            c->last_inc = false;
+            HIPCHK(hipGetLastError());
+            return DPMM_OK;
+        }
+        c->reset_update_ok = false;
         const bool partial_dev = dev_mask && c->partials_valid && c->partialsK == K && c->Kstats == K && !t_capturing;
         const bool partial = partial_dev || (c->dirty_pending && c->partials_valid && c->partialsK == K &&
                                              c->Kstats == K && (int)c->dirty_host.size() == K && !t_capturing);
@@ -1852,6 +2018,12 @@ static int enqueue_stats(dpmm_ctx* c, int K, const int32_t* dev_mask = nullptr) 
         } else if (!t_capturing) {
             (void)hipEventRecord(c->ev[4], c->stream);   // the phase timing covers full passes only
         }
+        // update mode: the clusters whose points kept their labels (per the label sweep's flags, on the
+        // device) are updated from the points whose sub-label moved, the others recomputed
+        const bool inc = c->inc_ready && !partial && c->Kstats == K && c->subs_perm_valid &&
+                         c->prior == DPMM_PRIOR_GAUSSIAN;
+        const unsigned ey = (unsigned)((E + kThreads - 1) / kThreads);
+        c->stats_gate = inc ? StatsGate{c->dSubsPermPrev, c->dFlags + 1, c->dPrevStart} : StatsGate{nullptr, nullptr, nullptr};
         if (c->prior == DPMM_PRIOR_GAUSSIAN) {
             if (c->dtype == DPMM_DTYPE_F64) { DPMM_DISPATCH_DP64(c->DP, st_f64, c); }
             else { DPMM_DISPATCH_DP(c->DP, st_f32, c); }
@@ -1861,8 +2033,14 @@ static int enqueue_stats(dpmm_ctx* c, int K, const int32_t* dev_mask = nullptr) 
                                reinterpret_cast<const int32_t*>(c->dX), c->D, K, c->CH, c->dPerm, c->dLabelStart,
                                c->dChunkStart, c->dSubs, (int)E, c->dPartials, c->stats_mask);
         }
-        hipLaunchKernelGGL(k_stats_reduce, dim3(2 * K, (unsigned)((E + kThreads - 1) / kThreads)), dim3(kThreads), 0,
-                           c->stream, c->dPartials, c->dChunkStart, K, (int)E, c->dStats, c->stats_mask);
+        if (inc)   // the wave kernel (DP = 16 .. 64) writes the signed change in slot 0
+            hipLaunchKernelGGL(k_stats_update, dim3(K, ey), dim3(kThreads), 0, c->stream, c->dPartials, c->dChunkStart, K,
+                               (int)E, c->dStats, c->dFlags + 1, (c->DP % 16 == 0 && c->DP <= 64) ? 1 : 0,
+                               nullptr);
+        else
+            hipLaunchKernelGGL(k_stats_reduce, dim3(2 * K, ey), dim3(kThreads), 0, c->stream, c->dPartials,
+                               c->dChunkStart, K, (int)E, c->dStats, c->stats_mask);
+        c->stats_gate = StatsGate{nullptr, nullptr, nullptr};
         if (!partial && !t_capturing) {
             (void)hipEventRecord(c->ev[5], c->stream);
             c->ev_stats = true;
@@ -1870,9 +2048,14 @@ static int enqueue_stats(dpmm_ctx* c, int K, const int32_t* dev_mask = nullptr) 
         c->stats_mask = nullptr;
         c->stats_cached = true;
         c->Kstats = K;
-        c->partials_valid = true;
+        c->partials_valid = !inc;   // after an update from the moved points the partials are differences
         c->partialsK = K;
         c->dirty_pending = false;
+        c->inc_passes = inc ? c->inc_passes + 1 : 0;
+        c->last_inc = inc;
+        c->inc_ready = false;
+        // the baseline of the next pass: these statistics with the sub-labels in partition order
+        c->inc_base = !partial && c->subs_perm_valid && c->prior == DPMM_PRIOR_GAUSSIAN;
     }
     HIPCHK(hipGetLastError());
     return DPMM_OK;
@@ -1928,6 +2111,9 @@ static int ensure_sweep_buffers(dpmm_ctx* c) {
         if ((rc = grow(c->dPruneV, c->pruneVCap, (size_t)kMaxK * c->DP))) return rc;
         if ((rc = grow(c->dEvals, c->evalsCap, 2))) return rc;
         if ((rc = grow(c->dSubsPerm, c->subsPermCap, (size_t)c->N))) return rc;
+        if ((rc = grow(c->dSubsPermPrev, c->subsPermPrevCap, (size_t)c->N))) return rc;
+        if ((rc = grow(c->dFlags, c->flagsCap, (size_t)kMaxK + 1))) return rc;
+        if ((rc = grow(c->dPrevStart, c->prevStartCap, (size_t)kMaxK + 1))) return rc;
     }
     return DPMM_OK;
 }
@@ -1954,6 +2140,10 @@ static int run_captured(dpmm_ctx* c, const std::function<int()>& record) {
         c->prune_valid = false;
         c->evals_copy_due = false;
         c->last_pruned_K = 0;
+        c->label_flag = false;
+        c->inc_base = false;
+        c->inc_ready = false;
+        c->reset_update_ok = false;
     };
     if (rc || ec != hipSuccess) {
         if (g) (void)hipGraphDestroy(g);
@@ -2156,12 +2346,17 @@ int dpmm_relabel(dpmm_ctx* c, uint32_t iter, int K_old, const int32_t* new_label
                        c->dTable, K_old, c->key0(), c->key1(), iter);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));
+    bool resets_only = !labels_change;
+    for (int k = 0; k < K_old; ++k) resets_only = resets_only && (sub_action[k] == 0 || sub_action[k] == 1);
+    // resets after a pass that left the statistics of the partition with the sub-labels in partition order:
+    // the next pass updates the reset clusters from their moved sub-labels
+    c->reset_update_ok = resets_only && c->inc_base && c->subs_perm_valid && c->partition_valid &&
+                         c->Kpartition == K_old && c->Kstats == K_old;
+    c->inc_base = false;
     if (labels_change) c->partition_valid = false;
     c->subs_perm_valid = false;   // sub-labels were rewritten by index
     c->stats_cached = false;
-    bool resets_only = !labels_change;
-    for (int k = 0; k < K_old; ++k) resets_only = resets_only && (sub_action[k] == 0 || sub_action[k] == 1);
-    if (resets_only && c->partials_valid && c->partialsK == K_old) {
+    if (resets_only && ((c->partials_valid && c->partialsK == K_old) || c->reset_update_ok)) {
         if (!c->dirty_pending) c->dirty_host.assign(K_old, 0);
         for (int k = 0; k < K_old; ++k) c->dirty_host[k] |= sub_action[k] != 0 ? 1 : 0;
         c->dirty_pending = true;
@@ -2195,6 +2390,24 @@ int dpmm_last_recheck_count(dpmm_ctx* c, int64_t* n) {
     HIPCHK(hipStreamSynchronize(c->stream));
     HIPCHK(hipMemcpy(&v, c->dRecheckCount, sizeof(int), hipMemcpyDeviceToHost));
     *n = v;
+    return DPMM_OK;
+}
+
+int dpmm_set_statistics_refresh(dpmm_ctx* c, int every) {
+    if (!c) return fail(DPMM_EINVAL, "null ctx");
+    c->stats_refresh = every;
+    c->inc_passes = 0;
+    return DPMM_OK;
+}
+
+int dpmm_last_statistics_update(dpmm_ctx* c, int* moved_only) {
+    if (!c || !moved_only) return fail(DPMM_EINVAL, "bad args");
+    *moved_only = 0;
+    if (!c->last_inc) return DPMM_OK;
+    int flag = 1;
+    HIPCHK(hipMemcpyAsync(&flag, c->dFlags, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    *moved_only = flag == 0 ? 1 : 2;
     return DPMM_OK;
 }
 
@@ -2360,6 +2573,7 @@ int dpmm_set_statistics(dpmm_ctx* c, int K, const int64_t* Nsub, const double* s
     c->Kstats = K;
     c->stats_cached = false;
     c->partials_valid = false;   // not the statistics of the current labels
+    c->inc_base = false;
     return DPMM_OK;
 }
 
@@ -2597,6 +2811,8 @@ void labels_rewritten(dpmm_ctx* c) {
     c->stats_cached = false;
     c->partials_valid = false;
     c->dirty_pending = false;
+    c->inc_base = false;
+    c->reset_update_ok = false;
 }
 
 // k_log_marginal over the device-built job list jobs (4 statistics entries each), the grid sized for
@@ -2774,6 +2990,9 @@ int step_iteration(dpmm_ctx* c, const StepArgs& args, int sweep_final, int full_
     StepSummary s = *c->hStepSum;
     if (s.followup && s.nsplit == 0) {
         // resets only: the labels and the partition stand, the reset clusters' sub-labels changed
+        c->reset_update_ok = c->inc_base && c->subs_perm_valid && c->partition_valid && c->Kpartition == K &&
+                             c->Kstats == K;
+        c->inc_base = false;
         c->subs_perm_valid = false;
         c->stats_cached = false;
         c->dirty_pending = false;

@@ -100,7 +100,8 @@ __global__ void __launch_bounds__(kThreads) k_label_gauss_listp(const T* __restr
                                                                 uint32_t key1, uint32_t iter, int final_iter,
                                                                 const int32_t* __restrict__ list,
                                                                 const int* __restrict__ list_count,
-                                                                int32_t* __restrict__ labels) {
+                                                                int32_t* __restrict__ labels,
+                                                                int* __restrict__ changed) {
     constexpr bool kPair = paired_layout<T, DP>();
     constexpr int kPts = kPair ? kThreads / 2 : kThreads;
     constexpr int kCols = kPair ? DP / 2 : DP;
@@ -119,7 +120,8 @@ __global__ void __launch_bounds__(kThreads) k_label_gauss_listp(const T* __restr
             const int l = kPair ? pair_col(c, 0) + (par << 2) : c;
             x[c] = (valid && l < D) ? X[i * D + l] : T(0);
         }
-        int a = valid ? labels[i] : 0;
+        const int a0 = valid ? labels[i] : 0;   // the label before the sweep
+        int a = a0;
         if ((unsigned)a >= (unsigned)K) a = 0;
         const T* pa = P + (size_t)a * stride;
         T qa;
@@ -165,6 +167,16 @@ __global__ void __launch_bounds__(kThreads) k_label_gauss_listp(const T* __restr
             if (s > best || first) { best = s; bi = k; first = false; }
         }
         if (valid && par == 0) labels[i] = bi;
+        // a label that moved: the partition of the labels before the sweep no longer stands (changed[0]),
+        // and the statistics of both clusters are recomputed in full (changed[1 + k])
+        if (changed) {
+            const bool mv = valid && par == 0 && bi != a0;
+            if (mv) {
+                changed[1 + bi] = 1;
+                if ((unsigned)a0 < (unsigned)K) changed[1 + a0] = 1;
+            }
+            if (__ballot(mv) && (threadIdx.x & 63) == 0) changed[0] = 1;
+        }
     }
 }
 
@@ -229,7 +241,7 @@ __global__ void __launch_bounds__(kThreads) k_stats_gauss_mfma(const TX* __restr
                                                                const int32_t* __restrict__ subs,
                                                                const int32_t* __restrict__ subs_perm,
                                                                double* __restrict__ partials,
-                                                               const int32_t* __restrict__ dirty) {
+                                                               const int32_t* __restrict__ dirty, StatsGate sg) {
     typedef double f64x4 __attribute__((ext_vector_type(4)));
     constexpr int T = DP / 16;
     constexpr int NTILE = T * (T + 1) / 2;
@@ -247,6 +259,7 @@ __global__ void __launch_bounds__(kThreads) k_stats_gauss_mfma(const TX* __restr
     int k, beg, end;
     if (!chunk_of(blockIdx.x, K, chunk_start, label_start, CH, k, beg, end)) return;
     if (dirty && !dirty[k]) return;   // statistics of this cluster are kept from the last pass
+    const int shift = stats_delta_shift(sg, k, label_start);   // >= 0: moved points only
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int kp = lane >> 4, r = lane & 15;
     int us[TPW], ca[TPW], cb[TPW];
@@ -280,7 +293,11 @@ __global__ void __launch_bounds__(kThreads) k_stats_gauss_mfma(const TX* __restr
     int id1 = (tid < B && beg + tid < end) ? perm[beg + tid] : -1;
     int id2 = (tid < B && beg + B + tid < end) ? perm[beg + B + tid] : -1;
     // sub-labels in partition order when the sub-label sweep left them (coalesced), else gathered
-    auto sub_at = [&](int pos, int id) { return id < 0 ? -1 : (subs_perm ? subs_perm[pos] : subs[id]); };
+    auto sub_at = [&](int pos, int id) {
+        if (id < 0) return -1;
+        const int sv = subs_perm ? subs_perm[pos] : subs[id];
+        return shift >= 0 && sg.subs_prev[pos + shift] == sv ? -1 : sv;   // moved points only
+    };
     int sb1 = sub_at(beg + tid, id1);
     if (tid < B) idn[tid] = id1;
     __syncthreads();
@@ -390,7 +407,10 @@ __global__ void __launch_bounds__(kThreads) k_stats_gauss_mfma(const TX* __restr
 // owns 64-point batches b = wave, wave + 4, ... of the chunk, moves the batch's points of the
 // sub-cluster to the front with one lane permute (ds_permute of a stable partition), then gathers
 // 4 groups of 4 rows at a time straight from global memory into MFMA operands.  The four waves'
-// accumulators are summed through LDS in fixed order (waves 3, 2, 1, then 0).
+// accumulators are summed through LDS in fixed order (waves 3, 2, 1, then 0).  Each batch's indices
+// and sub-labels are loaded one batch ahead.  In update mode (StatsGate) a chunk of a cluster that kept
+// its points takes one pass over the points whose sub-label moved, the first operand signed (+ for a
+// point now in sub-cluster 0, - in 1): slot 0 then holds the change of (k, 0), which (k, 1) loses.
 template <typename TX, int DP>
 __global__ void __launch_bounds__(kThreads, sizeof(TX) == 4 ? 3 : 2) k_stats_gauss_wave(const TX* __restrict__ X, int D, int K, int CH,
                                                                const int32_t* __restrict__ perm,
@@ -399,7 +419,7 @@ __global__ void __launch_bounds__(kThreads, sizeof(TX) == 4 ? 3 : 2) k_stats_gau
                                                                const int32_t* __restrict__ subs,
                                                                const int32_t* __restrict__ subs_perm,
                                                                double* __restrict__ partials,
-                                                               const int32_t* __restrict__ dirty) {
+                                                               const int32_t* __restrict__ dirty, StatsGate sg) {
     typedef double f64x4 __attribute__((ext_vector_type(4)));
     typedef TX txv __attribute__((ext_vector_type(DP / 16 == 4 ? 4 : 2)));
     constexpr int T = DP / 16;
@@ -412,6 +432,7 @@ __global__ void __launch_bounds__(kThreads, sizeof(TX) == 4 ? 3 : 2) k_stats_gau
     int k, beg, end;
     if (!chunk_of(blockIdx.x, K, chunk_start, label_start, CH, k, beg, end)) return;
     if (dirty && !dirty[k]) return;   // statistics of this cluster are kept from the last pass
+    const int shift = stats_delta_shift(sg, k, label_start);   // >= 0: moved points only
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int kp = lane >> 4, r = lane & 15;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -419,7 +440,10 @@ __global__ void __launch_bounds__(kThreads, sizeof(TX) == 4 ? 3 : 2) k_stats_gau
     const int c0 = T * r;
     double* out = partials + (size_t)(2 * blockIdx.x) * E;
 
-    for (int s = 0; s < 2; ++s, out += E) {
+    // update mode (shift >= 0): one pass over the points whose sub-label moved, each signed (+ into
+    // sub-cluster 0, - into 1), giving the change of (k, 0) in slot 0; full mode: one pass per sub-cluster
+    const bool upd = shift >= 0 && subs_perm;   // update mode reads the sub-labels in partition order
+    for (int s = 0; s < (upd ? 1 : 2); ++s, out += E) {
         f64x4 acc[NT];
         double sm[T];
 #pragma unroll
@@ -427,27 +451,47 @@ __global__ void __launch_bounds__(kThreads, sizeof(TX) == 4 ? 3 : 2) k_stats_gau
 #pragma unroll
         for (int e = 0; e < T; ++e) sm[e] = 0.0;
         int64_t cnt = 0;
+        // a point's index, its sub-label and (update mode) its previous sub-label, loaded one batch ahead;
+        // sub-labels in partition order when the sub-label sweep left them (a coalesced read that does not
+        // wait for perm), else gathered by index
+        auto fetch = [&](int p, int& idx, int& sn, int& sp) {
+            const bool in = p < end;
+            idx = in ? perm[p] : 0;
+            sn = in ? (subs_perm ? subs_perm[p] : subs[idx]) : -1;
+            sp = (in && upd) ? sg.subs_prev[p + shift] : -1;
+        };
+        int idx_n, sn_n, sp_n;
+        fetch(beg + 64 * wv + lane, idx_n, sn_n, sp_n);
         for (int b0 = beg + 64 * wv; b0 < end; b0 += 4 * 64) {
             const int p = b0 + lane;
-            const int idx = p < end ? perm[p] : 0;
-            // sub-labels in partition order when the sub-label sweep left them (a coalesced read that
-            // does not wait for perm), else gathered by index
-            const bool mine = p < end && (subs_perm ? subs_perm[p] : subs[idx]) == s;
+            const int idx = idx_n, sn = sn_n, sp = sp_n;
+            fetch(p + 4 * 64, idx_n, sn_n, sp_n);
+            const bool mine = p < end && (upd ? sn != sp : sn == s);
             const uint64_t m = __ballot(mine);
             const int ns = __popcll(m);
-            // stable partition: the sub-cluster's points first (the rest are never used)
+            if (upd) {   // signed count: into 0 minus into 1
+                const int n1 = __popcll(__ballot(mine && sn == 1));
+                cnt += ns - 2 * n1;
+            } else {
+                cnt += ns;
+            }
+            // stable partition: the selected points first (the rest are never used); bit 31 of the value
+            // marks a point that moved into sub-cluster 1
             const int dst = mine ? __popcll(m & lt) : ns + __popcll(~m & lt);
-            const int cidx = __builtin_amdgcn_ds_permute(dst << 2, idx);
-            cnt += ns;
+            const int cval = __builtin_amdgcn_ds_permute(dst << 2, idx | ((upd && sn == 1) ? (int)0x80000000u : 0));
             for (int g = 0; g < ns; g += 16) {
                 TX xv[4][T];
+                double sgn[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {   // coordinates c0 .. c0 + T - 1 of 4 rows (zero past D)
-                    // a lane past the sub-cluster's last point reads a row of the batch that this pass
-                    // reads anyway when its group holds points (cached), else the batch's first row
                     const int q = g + 4 * u + kp;
-                    const TX* row = X + (int64_t)__shfl(cidx, q < ns ? q : (g + 4 * u < ns ? ns - 1 : 0)) * D + c0;
-                    if (vec && c0 + T <= D) {
+                    const int val = __shfl(cval, q < ns ? q : (g + 4 * u < ns ? ns - 1 : 0));
+                    sgn[u] = val < 0 ? -1.0 : 1.0;
+                    const TX* row = X + (int64_t)(val & 0x7fffffff) * D + c0;
+                    if (g + 4 * u >= ns) {   // a group past the selected points: not loaded, not used
+#pragma unroll
+                        for (int e = 0; e < T; ++e) xv[u][e] = TX(0);
+                    } else if (vec && c0 + T <= D) {
                         if constexpr (T == 2 || T == 4) {
                             const txv w = *reinterpret_cast<const txv*>(row);
 #pragma unroll
@@ -466,14 +510,14 @@ __global__ void __launch_bounds__(kThreads, sizeof(TX) == 4 ? 3 : 2) k_stats_gau
 #pragma unroll
                     for (int e = 0; e < T; ++e) {
                         v[e] = ok ? (double)xv[u][e] : 0.0;
-                        sm[e] += v[e];
+                        sm[e] += sgn[u] * v[e];
                     }
                     int t = 0;
 #pragma unroll
                     for (int e1 = 0; e1 < T; ++e1)
 #pragma unroll
                         for (int e2 = e1; e2 < T; ++e2, ++t)
-                            acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(v[e1], v[e2], acc[t], 0, 0, 0);
+                            acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(sgn[u] * v[e1], v[e2], acc[t], 0, 0, 0);
                 }
             }
         }
@@ -621,7 +665,7 @@ __global__ void __launch_bounds__(kThreads) k_stats_gauss(const TX* __restrict__
                                                           const int32_t* __restrict__ subs,
                                                           const int32_t* __restrict__ subs_perm,
                                                           double* __restrict__ partials,
-                                                          const int32_t* __restrict__ dirty) {
+                                                          const int32_t* __restrict__ dirty, StatsGate sg) {
     constexpr int NB = DP / 4;
     constexpr int NTRI = NB * (NB + 1) / 2;
     constexpr int R = (NTRI + kThreads - 1) / kThreads;
@@ -634,6 +678,7 @@ __global__ void __launch_bounds__(kThreads) k_stats_gauss(const TX* __restrict__
     int k, beg, end;
     if (!chunk_of(c, K, chunk_start, label_start, CH, k, beg, end)) return;
     if (dirty && !dirty[k]) return;   // statistics of this cluster are kept from the last pass
+    const int shift = stats_delta_shift(sg, k, label_start);   // >= 0: moved points only
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     int bi[R], bj[R];
 #pragma unroll
@@ -659,7 +704,10 @@ __global__ void __launch_bounds__(kThreads) k_stats_gauss(const TX* __restrict__
             const int p = b0 + lane;
             int idx = -1;
             bool f = false;
-            if (lane < BATCH && p < end) { idx = perm[p]; f = (subs_perm ? subs_perm[p] : subs[idx]) == s; }
+            if (lane < BATCH && p < end) {
+                idx = perm[p];
+                f = (subs_perm ? subs_perm[p] : subs[idx]) == s && !(shift >= 0 && sg.subs_prev[p + shift] == s);
+            }
             const uint64_t m = __ballot(f);
             if (f) idl[__popcll(m & lt)] = idx;
             if (lane == 0) nsel = __popcll(m);
@@ -729,9 +777,9 @@ void launch_label_gauss_list(hipStream_t st, int grid, const T* X, int64_t N, in
 template <typename T, int DP>
 void launch_label_gauss_listp(hipStream_t st, int grid, const T* X, int64_t N, int D, int K, const T* P,
                               const float* pm, const float* psc, uint32_t key0, uint32_t key1, uint32_t iter, int fin,
-                              const int32_t* list, const int* count, int32_t* labels) {
+                              const int32_t* list, const int* count, int32_t* labels, int* changed) {
     hipLaunchKernelGGL((k_label_gauss_listp<T, DP>), dim3(grid), dim3(kThreads), 0, st, X, N, D, K, P, pm, psc, key0,
-                       key1, iter, fin, list, count, labels);
+                       key1, iter, fin, list, count, labels, changed);
 }
 
 template <typename T, int DP>
@@ -762,16 +810,16 @@ void launch_score_gauss(hipStream_t st, int grid, const T* X, int D, int K, cons
 template <typename TX, int DP>
 void launch_stats_gauss(hipStream_t st, int nchunks, const TX* X, int D, int K, int CH, const int32_t* perm,
                         const int32_t* label_start, const int32_t* chunk_start, const int32_t* subs,
-                        const int32_t* subs_perm, double* partials, const int32_t* dirty) {
+                        const int32_t* subs_perm, double* partials, const int32_t* dirty, StatsGate sg) {
     if constexpr (DP % 16 == 0 && DP <= 64)
         hipLaunchKernelGGL((k_stats_gauss_wave<TX, DP>), dim3(nchunks), dim3(kThreads), 0, st, X, D, K, CH, perm,
-                           label_start, chunk_start, subs, subs_perm, partials, dirty);
+                           label_start, chunk_start, subs, subs_perm, partials, dirty, sg);
     else if constexpr (DP % 16 == 0)
         hipLaunchKernelGGL((k_stats_gauss_mfma<TX, DP>), dim3(nchunks), dim3(kThreads), 0, st, X, D, K, CH, perm,
-                           label_start, chunk_start, subs, subs_perm, partials, dirty);
+                           label_start, chunk_start, subs, subs_perm, partials, dirty, sg);
     else
         hipLaunchKernelGGL((k_stats_gauss<TX, DP>), dim3(2 * nchunks), dim3(kThreads), 0, st, X, D, K, CH, perm,
-                           label_start, chunk_start, subs, subs_perm, partials, dirty);
+                           label_start, chunk_start, subs, subs_perm, partials, dirty, sg);
 }
 
 #define DPMM_GAUSS_INSTANTIATE(T, DP)                                                                         \
@@ -785,7 +833,7 @@ void launch_stats_gauss(hipStream_t st, int nchunks, const TX* X, int D, int K, 
                                           uint32_t, int32_t*);                                                 \
     template void launch_stats_gauss<T, DP>(hipStream_t, int, const T*, int, int, int, const int32_t*,         \
                                             const int32_t*, const int32_t*, const int32_t*, const int32_t*,    \
-                                            double*, const int32_t*);                                          \
+                                            double*, const int32_t*, StatsGate);                               \
     template void launch_sub_gauss_list<T, DP>(hipStream_t, int, const T*, int, const T*, const int32_t*,      \
                                                const int32_t*, const int*, uint32_t, uint32_t, uint32_t,       \
                                                int32_t*, const int32_t*, int32_t*);                            \
@@ -793,7 +841,7 @@ void launch_stats_gauss(hipStream_t st, int nchunks, const TX* X, int D, int K, 
                                             const int32_t*, int64_t, int64_t, int, T*);                        \
     template void launch_label_gauss_listp<T, DP>(hipStream_t, int, const T*, int64_t, int, int, const T*,     \
                                                   const float*, const float*, uint32_t, uint32_t, uint32_t, int, \
-                                                  const int32_t*, const int*, int32_t*);
+                                                  const int32_t*, const int*, int32_t*, int*);
 
 #define DPMM_GAUSS_EXTERN(T, DP)                                                                              \
     extern template void launch_label_gauss<T, DP>(hipStream_t, int, const T*, int64_t, int, int, const T*,    \
@@ -806,7 +854,7 @@ void launch_stats_gauss(hipStream_t st, int nchunks, const TX* X, int D, int K, 
                                                  uint32_t, int32_t*);                                          \
     extern template void launch_stats_gauss<T, DP>(hipStream_t, int, const T*, int, int, int, const int32_t*,  \
                                                    const int32_t*, const int32_t*, const int32_t*,             \
-                                                   const int32_t*, double*, const int32_t*);                   \
+                                                   const int32_t*, double*, const int32_t*, StatsGate);        \
     extern template void launch_sub_gauss_list<T, DP>(hipStream_t, int, const T*, int, const T*,               \
                                                       const int32_t*, const int32_t*, const int*, uint32_t,    \
                                                       uint32_t, uint32_t, int32_t*, const int32_t*, int32_t*); \
@@ -815,6 +863,6 @@ void launch_stats_gauss(hipStream_t st, int nchunks, const TX* X, int D, int K, 
     extern template void launch_label_gauss_listp<T, DP>(hipStream_t, int, const T*, int64_t, int, int,        \
                                                          const T*, const float*, const float*, uint32_t,       \
                                                          uint32_t, uint32_t, int, const int32_t*, const int*,  \
-                                                         int32_t*);
+                                                         int32_t*, int*);
 
 }  // namespace dpmm

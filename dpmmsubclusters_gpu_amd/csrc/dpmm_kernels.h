@@ -211,6 +211,23 @@ __device__ __forceinline__ void load_coords4(const double* __restrict__ row, int
 // pruning bounds of the mixed label sweep.
 constexpr float kPruneNoise = 24.0f;
 
+// Update mode of a statistics pass (all null: every chunk summed in full).  With subs_prev, the
+// sub-labels of the previous pass in the order of its partition (segment of cluster k starting at
+// prev_start[k]), a chunk of a cluster whose points all kept their label (moved[k] == 0) counts a point
+// for sub-cluster s only when it moved into s: its segment holds the same points in the same order, at
+// offset prev_start[k] - label_start[k] in the previous partition, and the partials are the moved-in
+// sums from which the statistics are updated.  Chunks of clusters with moved[k] != 0 are summed in full.
+struct StatsGate {
+    const int32_t* subs_prev;
+    const int32_t* moved;
+    const int32_t* prev_start;
+};
+// The offset of cluster k's points in the previous partition, or -1 when its chunks are summed in full.
+__device__ __forceinline__ int stats_delta_shift(const StatsGate& g, int k, const int32_t* __restrict__ label_start) {
+    if (!g.subs_prev || g.moved[k]) return -1;
+    return g.prev_start[k] - label_start[k];
+}
+
 // Chunk c of the partition: the cluster k whose segment holds it and its point range [beg, end)
 // in perm.  False past the last chunk.
 __device__ __forceinline__ bool chunk_of(int c, int K, const int32_t* __restrict__ chunk_start,

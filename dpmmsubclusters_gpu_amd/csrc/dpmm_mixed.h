@@ -397,7 +397,8 @@ k_label_gauss_mxp(const float* __restrict__ X, int D, int K, int CH, const int32
                   const unsigned char* __restrict__ PB, const float* __restrict__ pm,
                   const float* __restrict__ psc, float tau, uint32_t key0, uint32_t key1, uint32_t iter,
                   int final_iter, int32_t* __restrict__ labels, int32_t* __restrict__ recheck,
-                  int* __restrict__ recheck_count, unsigned long long* __restrict__ evals) {
+                  int* __restrict__ recheck_count, unsigned long long* __restrict__ evals,
+                  int* __restrict__ changed) {
     constexpr int RB = DP / 32, KC = DP / 16;
     constexpr int SLOT = mx_slot_bytes(DP);
     // one 32-point block per wave: lane L scores point L % 32 on coordinate half L / 32
@@ -558,6 +559,16 @@ k_label_gauss_mxp(const float* __restrict__ X, int D, int K, int CH, const int32
         ++npass;
         const bool sure = sb - eb > U;
         if (own_valid && sure) labels[idx] = bi;
+        // a label that moved off the chunk's cluster: the partition of the labels before the sweep no longer
+        // stands (changed[0]), and the statistics of both clusters are recomputed in full (changed[1 + k])
+        if (changed) {
+            const bool mv = own_valid && sure && bi != a;
+            if (mv) {
+                changed[1 + a] = 1;
+                changed[1 + bi] = 1;
+            }
+            if (__ballot(mv) && lane == 0) changed[0] = 1;
+        }
         rl.push(own_valid && !sure, (int32_t)idx, recheck, recheck_count, lane, lt);
     }
     rl.flush(recheck, recheck_count, lane);
@@ -582,9 +593,10 @@ void launch_label_gauss_mxp(hipStream_t st, int grid, const float* X, int D, int
                             const int32_t* label_start, const int32_t* chunk_start, const unsigned char* PB,
                             const float* pm, const float* psc, float tau, uint32_t key0, uint32_t key1, uint32_t iter,
                             int fin, int32_t* labels, int32_t* recheck, int* recheck_count,
-                            unsigned long long* evals) {
+                            unsigned long long* evals, int* changed) {
     hipLaunchKernelGGL((k_label_gauss_mxp<DP>), dim3(grid), dim3(kThreads), 0, st, X, D, K, CH, perm, label_start,
-                       chunk_start, PB, pm, psc, tau, key0, key1, iter, fin, labels, recheck, recheck_count, evals);
+                       chunk_start, PB, pm, psc, tau, key0, key1, iter, fin, labels, recheck, recheck_count, evals,
+                       changed);
 }
 
 #define DPMM_MX_INSTANTIATE(DP)                                                                               \
@@ -596,7 +608,7 @@ void launch_label_gauss_mxp(hipStream_t st, int grid, const float* X, int D, int
     template void launch_label_gauss_mxp<DP>(hipStream_t, int, const float*, int, int, int, const int32_t*,   \
                                              const int32_t*, const int32_t*, const unsigned char*, const float*, \
                                              const float*, float, uint32_t, uint32_t, uint32_t, int, int32_t*,   \
-                                             int32_t*, int*, unsigned long long*);
+                                             int32_t*, int*, unsigned long long*, int*);
 #define DPMM_MX_EXTERN(DP)                                                                                    \
     extern template void launch_label_gauss_mx<DP>(hipStream_t, int64_t, int, int, const float*,               \
                                                    const unsigned char*, float, uint32_t, uint32_t, uint32_t,  \
@@ -609,6 +621,6 @@ void launch_label_gauss_mxp(hipStream_t st, int grid, const float* X, int D, int
                                                     const int32_t*, const int32_t*, const int32_t*,            \
                                                     const unsigned char*, const float*, const float*, float,   \
                                                     uint32_t, uint32_t, uint32_t, int, int32_t*, int32_t*, int*, \
-                                                    unsigned long long*);
+                                                    unsigned long long*, int*);
 
 }  // namespace dpmm

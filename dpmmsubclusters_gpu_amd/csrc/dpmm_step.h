@@ -421,6 +421,7 @@ __global__ void __launch_bounds__(kStepThreads) k_step_decide(StepArgs a, StepCl
         sum->followup = nsplit + s_nreset > 0;
         sum->relabel2 = s_nmerge > 0 || nk != Ks;
         sum->error = s_err;
+        sum->npairs = a.no_more_splits ? 0 : counts[1] * (counts[1] - 1) / 2;
     }
 }
 

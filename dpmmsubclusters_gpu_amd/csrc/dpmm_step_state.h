@@ -34,6 +34,7 @@ struct StepSummary {
     int32_t followup;   // resets or splits: their clusters' statistics come before the merges apply
     int32_t relabel2;   // merges or removals: labels and the cluster table are renumbered
     int32_t error;      // bit 0: a parameter draw failed (factor not SPD); bit 1: a log ML not finite
+    int32_t npairs;     // candidate pairs given the exact merge test
 };
 
 // Scalars of one iteration's kernels.

@@ -268,6 +268,7 @@ struct SamplerImpl {
         rc = step_iteration(ctx, a, (fin || o.hard_clustering) ? 1 : 0, o.full_mu_cov, o.mult_use_weight, trace_labels,
                             trace_subs, &sm);
         if (rc) return rc;
+        n_exact += sm.npairs;
         ev_split += sm.nsplit;
         ev_merge += sm.nmerge;
         ev_reset += sm.nreset;

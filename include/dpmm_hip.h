@@ -204,6 +204,17 @@ int dpmm_last_sub_recheck_count(dpmm_ctx* ctx, int64_t* n);
  * N K DP^2 >= 4e11 (DP: D padded) and following a sweep that still scored more than a quarter of
  * the (wave, cluster) pairs by four unpruned sweeps; 2: always. */
 int dpmm_set_label_pruning(dpmm_ctx* ctx, int mode);
+/* Statistics updated from the points that moved (no counterpart in the reference, which recomputes
+ * create_sufficient_statistics from every point each iteration, cudaKernel.cu:844-941).  After a pruned
+ * mixed label sweep that moved no label, the partition of the labels stands and the statistics of the
+ * iteration are the last ones plus the sums of the points whose sub-label moved: (k, 0) gains the points
+ * that moved into sub-cluster 0 and loses those that moved out, (k, 1) the reverse.  `every` statistics
+ * passes hold one full recomputation, which bounds the FP64 rounding the updates carry (default 8;
+ * <= 1: every pass recomputes).  Counts are exact either way. */
+int dpmm_set_statistics_refresh(dpmm_ctx* ctx, int every);
+/* The last statistics pass: 0 a full recomputation, 1 an update of every cluster from the points whose
+ * sub-label moved, 2 such an update with the clusters a moved label touched recomputed.  Synchronises. */
+int dpmm_last_statistics_update(dpmm_ctx* ctx, int* moved_only);
 /* Fraction of (wave, cluster) scorings the last pruned label sweep performed (-1 if the last
  * sweep was not pruned). */
 int dpmm_last_label_evaluations(dpmm_ctx* ctx, double* fraction);

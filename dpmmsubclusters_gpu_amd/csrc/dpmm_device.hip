@@ -618,8 +618,9 @@ __global__ void __launch_bounds__(kThreads) k_relabel(int32_t* __restrict__ labe
 // ||W||_2^2 = lambda_max(W^T W): a power iteration gives an estimate s2 from below, and t = f s2
 // (f = 1.05, 1.2, 1.5, 3 in turn) is certified by an FP64 Cholesky of t I - W^T W (its success bounds
 // lambda_max by t plus the factorisation's backward error, added); without a certificate the bound
-// ||W||_1 ||W||_inf is kept.  One workgroup per cluster k; LDS: W (D x D floats),
-// then one FP64 difference vector per wave, two power-iteration vectors and a packed D x D matrix.
+// ||W||_1 ||W||_inf is kept.  One workgroup per cluster k; LDS: W (D x D floats), then four FP64
+// difference vectors of max(D, 64) entries per wave, two power-iteration vectors and a packed D x D
+// matrix.
 __global__ void __launch_bounds__(kThreads) k_prune_prep(const float* __restrict__ P, int K, int D, int DP,
                                                          float* __restrict__ pm, float* __restrict__ psc,
                                                          double* __restrict__ pv, int kwarm) {
@@ -627,7 +628,7 @@ __global__ void __launch_bounds__(kThreads) k_prune_prep(const float* __restrict
     const int DS = D + 1;   // padded row stride: rows of W start in different LDS banks
     float* W = reinterpret_cast<float*>(smem);
     double* diff = reinterpret_cast<double*>(smem + (((size_t)D * DS * 4 + 15) & ~(size_t)15));
-    double* v = diff + 4 * D;
+    double* v = diff + 16 * (D > 64 ? D : 64);
     double* uu = v + D;
     double* M = uu + D;
     __shared__ double red[4], red_c[4], red_r[4];
@@ -637,19 +638,29 @@ __global__ void __launch_bounds__(kThreads) k_prune_prep(const float* __restrict
     const float* slot = P + (size_t)k * stride;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     // row j of W from the slot (its columns j & ~3 onwards are stored): thread t takes rows
-    // t / 64, t / 64 + 4, ..., lane l its columns l, l + 64; every load of a row is issued before
-    // the stores
-    for (int j = wv; j < D; j += 4) {
-        const float* src = slot + tri_row_offset(DP, j) - (j & ~3);
-        float w[2];
+    // t / 64, t / 64 + 4, ..., lane l its columns l, l + 64; the loads of 8 rows are issued before
+    // their stores
+    constexpr int kRowBatch = 8;
+    for (int j0 = wv; j0 < D; j0 += 4 * kRowBatch) {
+        float w[kRowBatch][2];
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
-            const int l = lane + 64 * u;
-            w[u] = (l < D && l >= j) ? src[l] : 0.f;
+        for (int i = 0; i < kRowBatch; ++i) {
+            const int j = j0 + 4 * i;
+            const float* src = slot + tri_row_offset(DP, j < D ? j : 0) - ((j < D ? j : 0) & ~3);
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const int l = lane + 64 * u;
+                w[i][u] = (j < D && l < D && l >= j) ? src[l] : 0.f;
+            }
         }
 #pragma unroll
-        for (int u = 0; u < 2; ++u)
-            if (lane + 64 * u < D) W[j * DS + lane + 64 * u] = w[u];
+        for (int i = 0; i < kRowBatch; ++i) {
+            const int j = j0 + 4 * i;
+            if (j >= D) break;
+#pragma unroll
+            for (int u = 0; u < 2; ++u)
+                if (lane + 64 * u < D) W[j * DS + lane + 64 * u] = w[i][u];
+        }
     }
     __syncthreads();
     // fallback bound ||W||_2^2 <= ||W||_1 ||W||_inf, and ||W||_F^2
@@ -712,9 +723,8 @@ __global__ void __launch_bounds__(kThreads) k_prune_prep(const float* __restrict
         __syncthreads();
     }
     if (row < D && part == 0) pv[(size_t)k * DP + row] = v[row];
-    const double fac[4] = {1.05, 1.2, 1.5, 3.0};
     for (int tr = 0; tr < 4; ++tr) {   // uniform: s2 and sig2 are the same in every thread
-        const double tt = s2 * fac[tr];
+        const double tt = s2 * (tr == 0 ? 1.05 : tr == 1 ? 1.2 : tr == 2 ? 1.5 : 3.0);
         if (!(tt > 0.0) || tt >= sig2) break;
         const int T = D * (D + 1) / 2;
         for (int e = threadIdx.x; e < T; e += kThreads) {
@@ -738,14 +748,78 @@ __global__ void __launch_bounds__(kThreads) k_prune_prep(const float* __restrict
         psc[2 * k + 1] = slot[gauss_c_offset(DP)];
     }
     const float* muk = slot + gauss_mu_offset(DP);
-    double* dv = diff + (size_t)wv * D;
-    // anchors a = wave, wave + 4, ...: each wave works on its own difference vector with wave-level
-    // ordering only, and loads the next anchor's mean while it finishes the current one
     auto sync_wave = []() {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     };
+    auto put_bound = [&](int a, double sy, double se) {
+        const double m = sqrt(sy) * (1.0 - 1e-12) - gam * F * sqrt(se) * (1.0 + 1e-12);
+        pm[(size_t)a * K + k] = a == k ? 0.f : __double2float_rd(fmax(m, 0.0));
+    };
+    if (D <= 64) {
+        // lane j keeps row j of W (zero left of the diagonal and past column D, so the full sum over
+        // 64 columns is the triangular one) in registers; each wave scores 4 anchors at a time (a,
+        // a + 4, a + 8, a + 12) from 4 difference vectors of 64 entries (zero past D) in its own LDS,
+        // the loads of their means issued together, and reads them 8 columns at a time
+        float wr[64];
+#pragma unroll
+        for (int l = 0; l < 64; ++l) wr[l] = (lane < D && l < D) ? W[lane * DS + l] : 0.f;
+        const float mkl = lane < D ? muk[lane] : 0.f;
+        const int mu_off = gauss_mu_offset(DP);
+        double* dv = diff + (size_t)wv * 4 * 64;
+        for (int a0 = wv; a0 < K; a0 += 16) {
+            float ma[4];
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                const int a = a0 + 4 * g;
+                ma[g] = (a < K && lane < D) ? P[(size_t)a * stride + mu_off + lane] : 0.f;
+            }
+            double se[4];
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                const double d = lane < D ? (double)ma[g] - (double)mkl : 0.0;
+                dv[g * 64 + lane] = d;
+                se[g] = d * d;
+            }
+            sync_wave();
+            double y[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int l0 = 0; l0 < 64; l0 += 8) {
+                double dd[4][8];
+#pragma unroll
+                for (int g = 0; g < 4; ++g)
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) dd[g][i] = dv[g * 64 + l0 + i];
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const double wl = (double)wr[l0 + i];
+#pragma unroll
+                    for (int g = 0; g < 4; ++g) y[g] = fma(wl, dd[g][i], y[g]);
+                }
+            }
+            double sy[4];
+#pragma unroll
+            for (int g = 0; g < 4; ++g) sy[g] = y[g] * y[g];
+            for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+                for (int g = 0; g < 4; ++g) {
+                    sy[g] += __shfl_xor(sy[g], o);
+                    se[g] += __shfl_xor(se[g], o);
+                }
+            }
+            if (lane == 0) {
+#pragma unroll
+                for (int g = 0; g < 4; ++g)
+                    if (a0 + 4 * g < K) put_bound(a0 + 4 * g, sy[g], se[g]);
+            }
+            sync_wave();   // dv is read before the next anchors' differences overwrite it
+        }
+        return;
+    }
+    double* dv = diff + (size_t)wv * D;
+    // anchors a = wave, wave + 4, ...: each wave works on its own difference vector with wave-level
+    // ordering only, and loads the next anchor's mean while it finishes the current one
     float mk[2], ma[2];
 #pragma unroll
     for (int u = 0; u < 2; ++u) {
@@ -776,10 +850,7 @@ __global__ void __launch_bounds__(kThreads) k_prune_prep(const float* __restrict
             sy += __shfl_xor(sy, o);
             se += __shfl_xor(se, o);
         }
-        if (lane == 0) {
-            const double m = sqrt(sy) * (1.0 - 1e-12) - gam * F * sqrt(se) * (1.0 + 1e-12);
-            pm[(size_t)a * K + k] = a == k ? 0.f : __double2float_rd(fmax(m, 0.0));
-        }
+        if (lane == 0) put_bound(a, sy, se);
         sync_wave();   // dv is read before the next anchor's difference overwrites it
     }
 }
@@ -1493,7 +1564,8 @@ int ensure_prune(dpmm_ctx* c) {
     if (rc) return rc;
     if ((rc = grow(c->dPruneSC, c->pruneSCCap, (size_t)2 * K))) return rc;
     if ((rc = grow(c->dPruneV, c->pruneVCap, (size_t)kMaxK * c->DP))) return rc;   // once, at full size
-    const size_t lds = (((size_t)D * (D + 1) * 4 + 15) & ~(size_t)15) + ((size_t)6 * D + (size_t)D * (D + 1) / 2) * 8;
+    const size_t lds = (((size_t)D * (D + 1) * 4 + 15) & ~(size_t)15) +
+                       ((size_t)16 * std::max(D, 64) + 2 * (size_t)D + (size_t)D * (D + 1) / 2) * 8;
     if ((rc = allow_lds(reinterpret_cast<const void*>(k_prune_prep), lds))) return rc;
     hipLaunchKernelGGL(k_prune_prep, dim3(K), dim3(kThreads), lds, c->stream, reinterpret_cast<const float*>(c->dP),
                        K, D, c->DP, c->dPruneM, c->dPruneSC, c->dPruneV, std::min(c->prune_v_k, K));

@@ -12,10 +12,11 @@ constexpr int kThreads = 256;
 // Packed upper-triangular whitening factor W = R^T (R the lower Cholesky factor of the
 // precision matrix), row j stored from column (j & ~3) to DP-1 so every row starts on a
 // 16-byte boundary: entries below the diagonal inside that first group are zero.
+// Offset of row j: sum over rows r < j of DP - (r & ~3); the 4-row groups g < j / 4 drop 16 g
+// columns in all and the j % 4 rows of the last group 4 (j / 4) each.
 __host__ __device__ constexpr int tri_row_offset(int DP, int j) {
-    int off = 0;
-    for (int r = 0; r < j; ++r) off += DP - (r & ~3);
-    return off;
+    const int q = j >> 2, rem = j & 3;
+    return j * DP - 8 * q * (q - 1) - 4 * rem * q;
 }
 __host__ __device__ constexpr int tri_size(int DP) { return tri_row_offset(DP, DP); }
 // Gaussian parameter slot: [W packed | mu (DP) | c | pad] in elements of the compute type.

@@ -34,12 +34,6 @@ __device__ __forceinline__ void tri_ij(int e, int& i, int& j) {
     j = e - r * (r + 1) / 2;
 }
 
-// Closed form of tri_row_offset(DP, j) (row j of the sweep's packed W starts at column j & ~3).
-__device__ __forceinline__ int tri_row_offset_rt(int DP, int j) {
-    const int q = j >> 2, rem = j & 3;
-    return j * DP - 8 * q * (q - 1) - 4 * rem * q;
-}
-
 struct PriorArgs {
     double k0, v0, logdet_psi0, lmvg0;   // lmvg0 = log Gamma_D(v0 / 2)
     double alpha0_sum, lg_alpha0_sum;    // multinomial: sum alpha0, lgamma(sum alpha0)
@@ -109,9 +103,18 @@ __device__ __forceinline__ double stream_normal_at(uint32_t c0, uint32_t t, uint
 __device__ void niw_posterior(const double* __restrict__ stats, size_t E, int DP, int D, const int* mem,
                               const PriorArgs& h, double scale_by_nu, double* P, double* m, double* scal) {
     if (threadIdx.x == 0) {
+        // the members' counts loaded together (an unused member reads entry 0, which exists when any
+        // member does, and is not added)
+        const bool any = mem[0] >= 0 || mem[1] >= 0 || mem[2] >= 0 || mem[3] >= 0;
+        double c[4] = {0.0, 0.0, 0.0, 0.0};
+        if (any) {
+#pragma unroll
+            for (int t = 0; t < 4; ++t) c[t] = stats[(size_t)(mem[t] >= 0 ? mem[t] : 0) * E];
+        }
         double n = 0.0;
+#pragma unroll
         for (int t = 0; t < 4; ++t)
-            if (mem[t] >= 0) n += stats[(size_t)mem[t] * E];
+            if (mem[t] >= 0) n += c[t];
         scal[0] = n;
         scal[1] = n == 0.0 ? h.k0 : h.k0 + n;
         scal[2] = n == 0.0 ? h.v0 : h.v0 + n;
@@ -130,22 +133,51 @@ __device__ void niw_posterior(const double* __restrict__ stats, size_t E, int DP
         __syncthreads();
         return;
     }
+    // every member's value is loaded before any is summed (an unused member reads entry 0 and is not
+    // added), so the loads of a thread are in flight together rather than one after another
+    const size_t base[4] = {(size_t)(mem[0] >= 0 ? mem[0] : 0) * E, (size_t)(mem[1] >= 0 ? mem[1] : 0) * E,
+                            (size_t)(mem[2] >= 0 ? mem[2] : 0) * E, (size_t)(mem[3] >= 0 ? mem[3] : 0) * E};
     for (int d = threadIdx.x; d < D; d += blockDim.x) {
+        double x[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) x[t] = stats[base[t] + 1 + d];
         double s = 0.0;
+#pragma unroll
         for (int t = 0; t < 4; ++t)
-            if (mem[t] >= 0) s += stats[(size_t)mem[t] * E + 1 + d];
+            if (mem[t] >= 0) s += x[t];
         m[d] = (h.k0 * h.m0[d] + s) / kn;
     }
     __syncthreads();
     const double iv = 1.0 / vn;
-    for (int e = threadIdx.x; e < T; e += blockDim.x) {
-        int a, b;
-        tri_ij(e, a, b);
-        double s = 0.0;
-        for (int t = 0; t < 4; ++t)
-            if (mem[t] >= 0) s += stats[(size_t)mem[t] * E + 1 + DP + (size_t)b * DP + a];
-        const double p = (h.v0 * h.psi0[(size_t)b * D + a] + h.k0 * h.m0[b] * h.m0[a] + s - kn * m[b] * m[a]) * iv;
-        P[e] = scale_by_nu != 0.0 ? p * vn : p;
+    // entry (a, b), a >= b, reads element b * DP + a of the scatter sums: wave w takes rows
+    // b = w, w + W, ... (W waves), lane l the columns a = b + l, b + l + 64, ..., so a wave's loads are
+    // contiguous; the loads of 8 rows are issued before any is summed
+    const int lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+    constexpr int kRows = 8;
+    for (int b0 = threadIdx.x >> 6; b0 < D; b0 += kRows * nw) {
+        for (int a0 = 0; a0 < D; a0 += 64) {
+            double x[kRows][4], ps[kRows];
+#pragma unroll
+            for (int u = 0; u < kRows; ++u) {
+                const int b = b0 + u * nw, a = b + a0 + lane;
+                const bool in = b < D && a < D;
+                const size_t off = in ? (size_t)b * DP + a : 0;
+#pragma unroll
+                for (int t = 0; t < 4; ++t) x[u][t] = stats[base[t] + 1 + DP + off];
+                ps[u] = h.psi0[in ? (size_t)b * D + a : 0];
+            }
+#pragma unroll
+            for (int u = 0; u < kRows; ++u) {
+                const int b = b0 + u * nw, a = b + a0 + lane;
+                if (b >= D || a >= D) continue;
+                double s = 0.0;
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+                    if (mem[t] >= 0) s += x[u][t];
+                const double p = (h.v0 * ps[u] + h.k0 * h.m0[b] * h.m0[a] + s - kn * m[b] * m[a]) * iv;
+                P[tri(a, b)] = scale_by_nu != 0.0 ? p * vn : p;
+            }
+        }
     }
     __syncthreads();
 }
@@ -154,12 +186,181 @@ __device__ void niw_posterior(const double* __restrict__ stats, size_t E, int DP
 // owns row t / LN and, within it, the columns congruent to t % LN modulo LN.
 __device__ __forceinline__ int row_lanes(int D) { return D <= 64 ? 4 : 2; }
 
+// Lane `lane` (a constant or wave-uniform index) of x, read by every lane.
+__device__ __forceinline__ double readlane_f64(double x, int lane) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, x);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, lane);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), lane);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// ---- one-wave forms of the triangular factorisations (D <= 64) ----------------------------------
+// The first wave does the work with one row per lane held in registers; values of other rows are
+// read across lanes (v_readlane into scalar registers), so there is no LDS traffic and no barrier
+// inside.  Each matrix-index loop is unrolled so every register index is static.  Each entry sees
+// the same operations in the same order as in the block-wide LDS forms below, so both give the same
+// bits.
+
+// Cholesky: lane r holds row r.  Step k: a lane r > k subtracts (a_rk / a_kk) a_jk from its entries
+// k < j <= r, a_jk read from lane j.  The same subtraction applied by the other lanes only reaches
+// entries above their diagonal (columns j > r, never read) or subtracts an exact zero (lanes r >= D
+// hold zeros), so it needs no mask.  Lane k keeps the pivot a_kk; the columns are scaled by
+// 1 / sqrt(a_kk) at the end (the LDS form's lazy scaling, the same product).
+__device__ bool chol_packed_wave(double* P, int D) {
+    __shared__ int ok_w;
+    if (threadIdx.x < 64) {
+        const int r = threadIdx.x;
+        const int rb = r * (r + 1) / 2;
+        const bool own = r < D;
+        double v[64];
+#pragma unroll
+        for (int j = 0; j < 64; ++j) v[j] = (own && j <= r) ? P[rb + j] : 0.0;
+        bool ok = true;
+        double mypiv = 1.0;
+#pragma unroll
+        for (int k = 0; k < 64; ++k) {
+            if (k < D && ok) {   // uniform
+                const double akk = readlane_f64(v[k], k);
+                ok = akk > 0.0 && isfinite(akk);   // uniform: every lane read the same pivot
+                if (ok) {
+                    if (r == k) mypiv = akk;
+                    const double f = v[k] / akk;
+#pragma unroll
+                    for (int j = k + 1; j < 64; ++j) v[j] -= f * readlane_f64(v[k], j);
+                }
+            }
+        }
+        if (ok) {
+            const double psq = sqrt(mypiv);
+            const double pinv = 1.0 / psq;
+#pragma unroll
+            for (int j = 0; j < 64; ++j) {
+                const double pj = readlane_f64(pinv, j);
+                const double sj = readlane_f64(psq, j);
+                if (own && j <= r) P[rb + j] = j == r ? sj : v[j] * pj;
+            }
+        }
+        if (r == 0) ok_w = ok;
+    }
+    __syncthreads();
+    const bool res = ok_w != 0;
+    __syncthreads();   // ok_w is read before a later call writes it
+    return res;
+}
+
+// X = L^-1 by forward substitution: lane r holds row r of X.  Step k: a lane r > k subtracts
+// (L_rk / L_kk) X_kj from its entries j <= k, X_kj read from lane k (unscaled); every row is scaled
+// by 1 / L_rr at the end.  Lanes r <= k take a zero factor (an exact no-op).  L is not modified.
+__device__ void lower_inverse_wave(const double* L, double* X, int D) {
+    if (threadIdx.x < 64) {
+        const int r = threadIdx.x;
+        const int rb = r * (r + 1) / 2;
+        double x[64];
+#pragma unroll
+        for (int j = 0; j < 64; ++j) x[j] = j == r ? 1.0 : 0.0;
+#pragma unroll
+        for (int k = 0; k < 64; ++k) {
+            if (k < D) {   // uniform
+                const double f = (r > k && r < D) ? L[rb + (r > k ? k : 0)] / L[k * (k + 1) / 2 + k] : 0.0;
+#pragma unroll
+                for (int j = 0; j <= k; ++j) x[j] -= f * readlane_f64(x[j], k);
+            }
+        }
+        if (r < D) {
+            const double inv = 1.0 / L[rb + r];
+#pragma unroll
+            for (int j = 0; j < 64; ++j)
+                if (j <= r) X[rb + j] = x[j] * inv;
+        }
+    }
+    __syncthreads();
+}
+
+// ---- FP64 MFMA products of packed lower-triangular D x D matrices (D <= 64) ---------------------
+// v_mfma_f64_16x16x4f64 with lane l = 16 kp + q: the first operand is element (q, kp) of a 16 x 4
+// block, the second element (kp, q) of a 4 x 16 block, and accumulator v element (4 v + kp, q) of the
+// 16 x 16 product.  Entries outside the triangle or past D read as zero.  The sums run in the
+// hardware's order, not the LDS forms' (the results agree to rounding).
+typedef double f64x4_t __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ double packed_at(const double* M, int i, int j, int D) {
+    return (j <= i && i < D) ? M[i * (i + 1) / 2 + j] : 0.0;
+}
+
+// R = B A, written over A.  The ten 16 x 16 tiles (I >= J) of the lower triangle, tile (I, J)
+// summing over k in [16 J, 16 I + 16), are shared among the waves with their MFMA counts balanced.
+__device__ void lower_product_mfma(const double* B, double* A, int D) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int kp = lane >> 4, q = lane & 15;
+    // tiles as 4 I + J, 4 bits each, in the order dealt to waves w, w + 4, w + 8:
+    // (3,0) (3,1) (2,0) (2,1) (0,0) (1,0) (3,2) (1,1) (2,2) (3,3)
+    constexpr uint64_t kTiles = 12ull | 13ull << 4 | 8ull << 8 | 9ull << 12 | 0ull << 16 | 4ull << 20 | 14ull << 24 |
+                                5ull << 28 | 10ull << 32 | 15ull << 36;
+    f64x4_t acc[3];
+#pragma unroll
+    for (int u = 0; u < 3; ++u) {
+        const int t = wv + 4 * u;
+        const int tile = (int)((kTiles >> (4 * (t < 10 ? t : 0))) & 15);
+        const int I = tile >> 2, J = tile & 3;
+        f64x4_t c = {0.0, 0.0, 0.0, 0.0};
+        if (t < 10 && 16 * I < D) {
+            for (int s = 4 * J; s < 4 * I + 4; ++s) {
+                const double x = packed_at(B, 16 * I + q, 4 * s + kp, D);
+                const double y = packed_at(A, 4 * s + kp, 16 * J + q, D);
+                c = __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, c, 0, 0, 0);
+            }
+        }
+        acc[u] = c;
+    }
+    __syncthreads();   // every wave has read A
+#pragma unroll
+    for (int u = 0; u < 3; ++u) {
+        const int t = wv + 4 * u;
+        if (t >= 10) continue;
+        const int tile = (int)((kTiles >> (4 * t)) & 15);
+        const int I = tile >> 2, J = tile & 3;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int i = 16 * I + 4 * v + kp, j = 16 * J + q;
+            if (j <= i && i < D) A[i * (i + 1) / 2 + j] = acc[u][v];
+        }
+    }
+    __syncthreads();
+}
+
+// G = J (R R^T) J: tile (I, J) of S = R R^T (I >= J) sums over k < 16 J + 16; entry (i, j) of S,
+// i >= j, goes to G's entry (D-1-j, D-1-i).
+__device__ void gram_rev_mfma(const double* R, double* G, int D) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int kp = lane >> 4, q = lane & 15;
+    // (3,3) (2,2) (3,2) (1,1) (2,1) (3,1) (0,0) (1,0) (2,0) (3,0), 4 bits each as 4 I + J
+    constexpr uint64_t kTiles = 15ull | 10ull << 4 | 14ull << 8 | 5ull << 12 | 9ull << 16 | 13ull << 20 | 0ull << 24 |
+                                4ull << 28 | 8ull << 32 | 12ull << 36;
+    for (int t = wv; t < 10; t += 4) {
+        const int tile = (int)((kTiles >> (4 * t)) & 15);
+        const int I = tile >> 2, J = tile & 3;
+        if (16 * I >= D) continue;
+        f64x4_t c = {0.0, 0.0, 0.0, 0.0};
+        for (int s = 0; s < 4 * J + 4; ++s) {
+            const double x = packed_at(R, 16 * I + q, 4 * s + kp, D);
+            const double y = packed_at(R, 16 * J + q, 4 * s + kp, D);
+            c = __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, c, 0, 0, 0);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int i = 16 * I + 4 * v + kp, j = 16 * J + q;
+            if (j <= i && i < D) G[tri(D - 1 - j, D - 1 - i)] = c[v];
+        }
+    }
+    __syncthreads();
+}
+
 // In-place Cholesky of packed lower P, right-looking with a lazily scaled pivot column: step k
 // subtracts (a_ik / a_kk) a_jk from the trailing entries (k < j <= i) with column k as the
 // earlier steps left it, while the owners of column k - 1, which nothing reads any more, scale it
 // by 1 / sqrt(a_{k-1,k-1}).  One barrier per column.  Uniform result: false when a pivot is not
-// positive and finite (every thread reads the same pivot).
+// positive and finite (every thread reads the same pivot).  D <= 64 takes the one-wave form.
 __device__ bool chol_packed(double* P, int D, double* /*unused*/) {
+    if (D <= 64) return chol_packed_wave(P, D);
     const int LN = row_lanes(D), r = threadIdx.x / LN, q = threadIdx.x % LN;
     const int rb = r * (r + 1) / 2;
     const bool own = r < D;
@@ -188,6 +389,7 @@ __device__ bool chol_packed(double* P, int D, double* /*unused*/) {
 // (L_ik / L_kk) X_kj (j <= k) from every row i > k, row k as the earlier steps left it, while
 // row k - 1, which nothing reads any more, is scaled by 1 / L_{k-1,k-1}.  One barrier per row.
 __device__ void lower_inverse_packed(const double* L, double* X, int D) {
+    if (D <= 64) return lower_inverse_wave(L, X, D);
     const int LN = row_lanes(D), r = threadIdx.x / LN, q = threadIdx.x % LN;
     const int rb = r * (r + 1) / 2;
     const bool own = r < D;
@@ -216,6 +418,7 @@ __device__ void lower_inverse_packed(const double* L, double* X, int D) {
 // B A for packed lower B and A (D <= 128), written over A: every thread forms its entries
 // (k from j to i, in order) in registers, then all are stored after a barrier.
 __device__ void lower_product_packed(const double* B, double* A, int D) {
+    if (D <= 64) return lower_product_mfma(B, A, D);
     constexpr int kMaxE = (128 * 129 / 2 + kThreads - 1) / kThreads;
     const int T = D * (D + 1) / 2;
     double acc[kMaxE];
@@ -274,6 +477,7 @@ __device__ void reverse_packed(double* X, int D) {
 // G = J (R R^T) J for packed lower R: entry (i, j) is sum_{k <= a} R[a][k] R[b][k] with
 // a = D-1-i <= b = D-1-j (k in increasing order).
 __device__ void gram_rev_packed(const double* R, double* G, int D) {
+    if (D <= 64) return gram_rev_mfma(R, G, D);
     const int T = D * (D + 1) / 2;
     for (int e = threadIdx.x; e < T; e += blockDim.x) {
         int i, j;
@@ -538,7 +742,7 @@ __global__ void __launch_bounds__(kThreads) k_pack_gauss(const double* __restric
         int l, j;
         tri_ij(e, l, j);
         const double v = src[D + e];
-        dst[tri_row_offset_rt(DP, j) - (j & ~3) + l] = (T)v;
+        dst[tri_row_offset(DP, j) - (j & ~3) + l] = (T)v;
         if (tiles) {
             const float w = (float)v;
             const uint16_t hb = bf16_bits_rne(w);

@@ -102,6 +102,7 @@ def load_library():
         "dpmm_last_recheck_count": [_P, _P],
         "dpmm_last_sub_recheck_count": [_P, _P],
         "dpmm_set_label_pruning": [_P, i],
+        "dpmm_set_fused_sub_labels": [_P, i],
         "dpmm_set_statistics_refresh": [_P, i],
         "dpmm_last_statistics_update": [_P, _P],
         "dpmm_last_label_evaluations": [_P, _P],
@@ -396,6 +397,10 @@ class DeviceSweep:
         else:
             m = 1 if mode == "adaptive" else int(mode)
         _check(load_library().dpmm_set_label_pruning(self._ctx, m))
+
+    def set_fused_sub_labels(self, enable):
+        """Sub-label draw fused into the pruned mixed label sweep (default on); False: a separate sweep."""
+        _check(load_library().dpmm_set_fused_sub_labels(self._ctx, 1 if enable else 0))
 
     def set_statistics_refresh(self, every):
         """Statistics passes per full recomputation; the passes between update the statistics from the

@@ -568,6 +568,28 @@ __global__ void __launch_bounds__(kThreads) k_refresh_subs_perm(int K, int CH, c
     }
 }
 
+// Sub-labels in the order of the current partition from those the fused label + sub-label sweep left
+// in the order of the partition it swept (segment starts old_start): a cluster no label moved into or
+// out of (moved[k] == 0) holds the same points in the same order there, at offset old_start[k] -
+// label_start[k]; the points of the other clusters are gathered by index.
+__global__ void __launch_bounds__(kThreads) k_subs_to_partition(int K, int CH, const int32_t* __restrict__ perm,
+                                                                const int32_t* __restrict__ label_start,
+                                                                const int32_t* __restrict__ chunk_start,
+                                                                const int32_t* __restrict__ old_start,
+                                                                const int32_t* __restrict__ moved,
+                                                                const int32_t* __restrict__ subs_old,
+                                                                const int32_t* __restrict__ subs,
+                                                                int32_t* __restrict__ cur) {
+    int k, beg, end;
+    if (!chunk_of(blockIdx.x, K, chunk_start, label_start, CH, k, beg, end)) return;
+    if (moved[k]) {
+        for (int p = beg + threadIdx.x; p < end; p += kThreads) cur[p] = subs[perm[p]];
+    } else {
+        const int shift = old_start[k] - label_start[k];
+        for (int p = beg + threadIdx.x; p < end; p += kThreads) cur[p] = subs_old[p + shift];
+    }
+}
+
 __global__ void k_copy_int(const int32_t* __restrict__ src, int32_t* __restrict__ dst, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) dst[i] = src[i];
@@ -958,6 +980,15 @@ struct dpmm_ctx {
     bool spb_valid = false;
     int32_t* dRecheck = nullptr;
     int* dRecheckCount = nullptr;
+    // sub-label screen fused into the pruned label sweep (DPM <= 64): the sub-labels it leaves in the
+    // order of the partition it swept, and its list of positions left to the FP32 draw
+    int fuse_mode = 1;            // 0: separate sub-label sweep
+    bool fused_subs = false;      // the last label sweep drew the sub-labels as well, for iteration fused_iter
+    uint32_t fused_iter = 0;
+    int32_t* dSubsOld = nullptr;
+    size_t subsOldCap = 0;
+    int32_t* dSubList = nullptr;
+    size_t subListCap = 0;
     float tau = 1.0f;   // scale of the mixed label screens' error bound (1: the derived bound)
     // exact pruning of the mixed Gaussian label sweep: bounds of the current parameters
     int prune_mode = 1;           // 0 off, 1 adaptive, 2 always
@@ -1285,6 +1316,14 @@ void pack_mx_slot(int D, int DPM, const double* mu, const double* R, double cst,
     float* m = reinterpret_cast<float*>(dst + mx_tiles_bytes(DPM));
     for (int l = 0; l < D; ++l) m[l] = (float)mu[l];
     m[DPM] = (float)cst;
+    // F >= ||W||_F^2 of the FP32 W (rounded up)
+    double F = 0.0;
+    for (int j = 0; j < D; ++j)
+        for (int l = j; l < D; ++l) {
+            const double w = (double)(float)R[(size_t)l * D + j];
+            F += w * w;
+        }
+    m[DPM + 1] = (float)(F * (1.0 + 0x1p-20));
 }
 
 // Recheck list of the mixed sweeps.
@@ -1334,7 +1373,8 @@ int pack_mult_mixed(dpmm_ctx* c, int K, const double* src, size_t row_stride, co
 
 // Split-BF16 slot of one sub-cluster for the sub-label screen: hi and lo tiles of W = R^T (of the
 // FP32 value the FP32 sweep uses), mu, score constant.
-void pack_mx2_slot(int D, int DPM, const double* mu, const double* R, double cst, unsigned char* dst) {
+void pack_mx2_slot(int D, int DPM, const double* mu, const double* R, double cst, const double* parent_mu,
+                   unsigned char* dst) {
     std::memset(dst, 0, mx2_slot_bytes(DPM));
     uint16_t* hi = reinterpret_cast<uint16_t*>(dst);
     uint16_t* lo = reinterpret_cast<uint16_t*>(dst + mx_tiles_bytes(DPM));
@@ -1351,6 +1391,20 @@ void pack_mx2_slot(int D, int DPM, const double* mu, const double* R, double cst
     float* m = reinterpret_cast<float*>(dst + 2 * mx_tiles_bytes(DPM));
     for (int l = 0; l < D; ++l) m[l] = (float)mu[l];
     m[DPM] = (float)cst;
+    // the screen relative to the parent cluster's mean: -c and g (see mx2_slot_bytes)
+    float* rel = reinterpret_cast<float*>(dst + mx2_rel_offset(DPM));
+    const double kappa = (double)sub_rel_eps_g(DPM) / (double)sub_rel_eps_a(DPM);
+    for (int j = 0; j < D; ++j) {
+        double c = 0.0, b = 0.0;
+        for (int l = j; l < D; ++l) {
+            const double w = (double)(float)R[(size_t)l * D + j];
+            const double dl = (double)(float)mu[l] - (double)(float)parent_mu[l];
+            c += w * dl;
+            b += std::fabs(w) * std::fabs(dl);
+        }
+        rel[j] = (float)(-c);
+        rel[DPM + j] = (float)(kappa * (b + std::fabs(c)) * (1.0 + 0x1p-20) + 0x1p-120);
+    }
 }
 
 int set_gauss(dpmm_ctx* c, int K, const double* mu, const double* mat, const double* logdet, const double* weights,
@@ -1390,6 +1444,7 @@ int set_gauss(dpmm_ctx* c, int K, const double* mu, const double* mat, const dou
     const bool mixed = c->precision == DPMM_PREC_MIXED;
     const int mslot = mx_slot_bytes(c->DPM);
     c->spb_valid = false;
+    c->fused_subs = false;
     if (mixed) {
         c->hPB.assign((size_t)K * mslot, 0);
         c->hSPB.assign((size_t)2 * K * mx2_slot_bytes(c->DPM), 0);
@@ -1414,7 +1469,7 @@ int set_gauss(dpmm_ctx* c, int K, const double* mu, const double* mat, const dou
             return fail(DPMM_ENOTSPD, "sub-cluster precision matrix not SPD");
         if (mixed)
             pack_mx2_slot(D, c->DPM, smu + (size_t)q * D, R.data(), -0.5 * slogdet[q] + std::log(lrw[q]),
-                          c->hSPB.data() + (size_t)q * mx2_slot_bytes(c->DPM));
+                          mu + (size_t)(q / 2) * D, c->hSPB.data() + (size_t)q * mx2_slot_bytes(c->DPM));
     }
     if (mixed) {
         HIPCHK(hipMemcpyAsync(c->dSPB, c->hSPB.data(), c->hSPB.size(), hipMemcpyHostToDevice, c->stream));
@@ -1497,6 +1552,13 @@ template <int DP> void sgl_f32(dpmm_ctx* c, uint32_t it, int grid) {
                                      reinterpret_cast<const float*>(c->dSP), c->dLabels, c->dRecheck,
                                      c->dRecheckCount + 1, c->key0(), c->key1(), it, c->dSubs, c->dPerm,
                                      c->dSubsPerm);
+}
+// the list of the fused sweep: positions in the partition it swept, sub-labels into dSubsOld
+template <int DP> void sglf_f32(dpmm_ctx* c, uint32_t it, int grid) {
+    launch_sub_gauss_list<float, DP>(c->stream, grid, reinterpret_cast<const float*>(c->dX), c->D,
+                                     reinterpret_cast<const float*>(c->dSP), c->dLabels, c->dSubList,
+                                     c->dRecheckCount + 1, c->key0(), c->key1(), it, c->dSubs, c->dPerm,
+                                     c->dSubsOld);
 }
 template <int DPM> void sgm(dpmm_ctx* c, uint32_t it) {
     launch_sub_gauss_mx<DPM>(c->stream, chunks_upper(c, c->K), reinterpret_cast<const float*>(c->dX), c->D, c->K,
@@ -1582,21 +1644,39 @@ int run_label_mixed_pruned(dpmm_ctx* c, uint32_t iter, int fin) {
     int rc = ensure_prune(c);
     if (rc) return rc;
     // the partition the last sub-label sweep used is the one of the current labels unless a relabel
-    // came since (which clears partition_valid)
-    if (!(c->partition_valid && c->Kpartition == c->K) && (rc = build_partition(c, c->K, 1))) return rc;
+    // came since (which clears partition_valid); that partition files every point under its label
+    const bool reuse = c->partition_valid && c->Kpartition == c->K;
+    if (!reuse && (rc = build_partition(c, c->K, 1))) return rc;
     if ((rc = grow(c->dEvals, c->evalsCap, 2))) return rc;
     if ((rc = grow(c->dFlags, c->flagsCap, (size_t)kMaxK + 1))) return rc;
+    // the sub-label screen rides along (its slots hold this iteration's sub-cluster parameters)
+    const bool fuse = c->fuse_mode != 0 && c->DPM <= 64 && c->spb_valid;
+    if (fuse) {
+        if ((rc = grow(c->dSubsOld, c->subsOldCap, (size_t)c->N))) return rc;
+        if ((rc = grow(c->dSubList, c->subListCap, (size_t)c->N))) return rc;
+    }
     // dFlags[0]: a label moved; dFlags[1 + k]: a label moved into or out of cluster k
     hipLaunchKernelGGL(k_zero_int, dim3((c->K + 64) / 64), dim3(64), 0, c->stream, c->dFlags, c->K + 1);
-    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, c->dRecheckCount, 1);
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, c->dRecheckCount, 2);
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, reinterpret_cast<int*>(c->dEvals), 4);
     const int grid = chunks_upper(c, c->K) * kPruneSplit;
     const float* X = reinterpret_cast<const float*>(c->dX);
     const unsigned char* PB = reinterpret_cast<const unsigned char*>(c->dPB);
+    SubFuse sf{};
+    sf.skip_same = reuse ? 1 : 0;
+    if (fuse) {
+        sf.SPB = reinterpret_cast<const unsigned char*>(c->dSPB);
+        // the sub-labels of this partition, when the last pass left them in its order
+        sf.subs_prev = c->subs_perm_valid ? c->dSubsPerm : nullptr;
+        sf.subs = c->dSubs;
+        sf.subs_old = c->dSubsOld;
+        sf.sub_list = c->dSubList;
+        sf.sub_count = c->dRecheckCount + 1;
+    }
 #define DPMM_MXP(DPMV)                                                                                          \
     launch_label_gauss_mxp<DPMV>(c->stream, grid, X, c->D, c->K, c->CH, c->dPerm, c->dLabelStart, c->dChunkStart, \
                                  PB, c->dPruneM, c->dPruneSC, c->tau, c->key0(), c->key1(), iter, fin, c->dLabels, \
-                                 c->dRecheck, c->dRecheckCount, c->dEvals, c->dFlags)
+                                 c->dRecheck, c->dRecheckCount, c->dEvals, c->dFlags, sf)
     switch (c->DPM) {
         case 32: DPMM_MXP(32); break;
         case 64: DPMM_MXP(64); break;
@@ -1607,6 +1687,10 @@ int run_label_mixed_pruned(dpmm_ctx* c, uint32_t iter, int fin) {
 #undef DPMM_MXP
     const int grid2 = (int)std::min<int64_t>(1024, (c->N + kThreads - 1) / kThreads);
     DPMM_DISPATCH_DP(c->DP, lglp_f32, c, iter, fin, grid2);   // the recheck pruned with the same bounds
+    // fused: the FP32 sub-label draw of the listed positions, by their final labels (after the recheck)
+    if (fuse) DPMM_DISPATCH_DP(c->DP, sglf_f32, c, iter, grid2);
+    c->fused_subs = fuse;
+    c->fused_iter = iter;
     c->last_pruned_K = c->K;
     c->label_flag = true;   // dFlags[0]: whether the partition of the labels before the sweep still stands
     return DPMM_OK;
@@ -1704,7 +1788,8 @@ int dpmm_destroy(dpmm_ctx* c) {
                     (void*)c->dStepCl, (void*)c->dStepClTmp, (void*)c->dStepHist, (void*)c->dStepHistTmp,
                     (void*)c->dStepW, (void*)c->dStepMem, (void*)c->dStepCand, (void*)c->dStepCounts,
                     (void*)c->dStepTab, (void*)c->dStepJobs, (void*)c->dStepAcc, (void*)c->dStepSum,
-                    (void*)c->dFlags, (void*)c->dSubsPermPrev, (void*)c->dPrevStart})
+                    (void*)c->dFlags, (void*)c->dSubsPermPrev, (void*)c->dPrevStart, (void*)c->dSubsOld,
+                    (void*)c->dSubList})
         if (p) (void)hipFree(p);
     if (c->hStepSum) (void)hipHostFree(c->hStepSum);
     if (c->hStats) (void)hipHostFree(c->hStats);
@@ -1769,6 +1854,7 @@ int dpmm_sample_initial_labels(dpmm_ctx* c, int K0) {
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));
     c->partition_valid = false;
+    c->fused_subs = false;
     c->subs_perm_valid = false;
     c->stats_cached = false;
     c->inc_base = false;
@@ -1782,6 +1868,7 @@ int dpmm_set_labels(dpmm_ctx* c, const int32_t* labels, const int32_t* subs) {
     if (labels) HIPCHK(hipMemcpy(c->dLabels, labels, c->N * sizeof(int32_t), hipMemcpyHostToDevice));
     if (subs) HIPCHK(hipMemcpy(c->dSubs, subs, c->N * sizeof(int32_t), hipMemcpyHostToDevice));
     c->partition_valid = false;
+    c->fused_subs = false;
     c->subs_perm_valid = false;
     c->stats_cached = false;
     c->inc_base = false;
@@ -1883,6 +1970,7 @@ int dpmm_sample_cluster_labels(dpmm_ctx* c, uint32_t iter, int fin) {
     if (!t_capturing) (void)hipEventRecord(c->ev[0], c->stream);
     c->last_pruned_K = 0;
     c->label_flag = false;
+    c->fused_subs = false;
     if (c->prior == DPMM_PRIOR_GAUSSIAN && c->precision == DPMM_PREC_MIXED) {
         bool prune = c->prune_mode != 0 && c->K >= 2 && c->D % 4 == 0;   // 16-byte rows for the gathers
         // adaptive: small sweeps (N K DP^2 below 4e11 multiply-adds) stay unpruned, the fixed cost
@@ -1943,10 +2031,15 @@ int dpmm_sample_sub_labels(dpmm_ctx* c, uint32_t iter) {
         if ((rc = grow(c->dPrevStart, c->prevStartCap, (size_t)kMaxK + 1))) return rc;
         std::swap(c->dSubsPerm, c->dSubsPermPrev);
         std::swap(c->subsPermCap, c->subsPermPrevCap);
-        // the segment starts of the partition the baseline sub-labels are ordered by
+        c->inc_ready = true;
+    }
+    // fused: the sub-labels were drawn by the label sweep, in the order of the partition it swept
+    const bool fused = mixed && c->fused_subs && c->fused_iter == iter && c->label_flag && c->Kpartition == c->K;
+    if (baseline || fused) {
+        if ((rc = grow(c->dPrevStart, c->prevStartCap, (size_t)kMaxK + 1))) return rc;
+        // the segment starts of the partition swept by the label sweep (that of the baseline sub-labels)
         hipLaunchKernelGGL(k_copy_int, dim3((c->K + 64) / 64), dim3(64), 0, c->stream, c->dLabelStart, c->dPrevStart,
                            c->K + 1);
-        c->inc_ready = true;
     }
     if (c->label_flag && c->Kpartition == c->K) {
         // the partition of the labels before the sweep, rebuilt on the device only if a label moved
@@ -1958,7 +2051,13 @@ int dpmm_sample_sub_labels(dpmm_ctx* c, uint32_t iter) {
     c->stats_cached = false;
     c->partials_valid = false;
     c->subs_perm_valid = false;
-    if (mixed) {
+    if (fused) {
+        if ((rc = grow(c->dSubsPerm, c->subsPermCap, (size_t)c->N))) return rc;
+        hipLaunchKernelGGL(k_subs_to_partition, dim3(chunks_upper(c, c->K)), dim3(kThreads), 0, c->stream, c->K, c->CH,
+                           c->dPerm, c->dLabelStart, c->dChunkStart, c->dPrevStart, c->dFlags + 1, c->dSubsOld,
+                           c->dSubs, c->dSubsPerm);
+        c->subs_perm_valid = true;
+    } else if (mixed) {
         if ((rc = run_sub_mixed(c, iter))) return rc;
     } else if (c->prior == DPMM_PRIOR_GAUSSIAN) {
         if (c->params_f64) { DPMM_DISPATCH_DP64(c->DP, sg_f64, c, iter); }
@@ -2186,6 +2285,10 @@ static int ensure_sweep_buffers(dpmm_ctx* c) {
         if ((rc = grow(c->dSubsPermPrev, c->subsPermPrevCap, (size_t)c->N))) return rc;
         if ((rc = grow(c->dFlags, c->flagsCap, (size_t)kMaxK + 1))) return rc;
         if ((rc = grow(c->dPrevStart, c->prevStartCap, (size_t)kMaxK + 1))) return rc;
+        if (c->fuse_mode != 0 && c->DPM <= 64) {
+            if ((rc = grow(c->dSubsOld, c->subsOldCap, (size_t)c->N))) return rc;
+            if ((rc = grow(c->dSubList, c->subListCap, (size_t)c->N))) return rc;
+        }
     }
     return DPMM_OK;
 }
@@ -2213,6 +2316,7 @@ static int run_captured(dpmm_ctx* c, const std::function<int()>& record) {
         c->evals_copy_due = false;
         c->last_pruned_K = 0;
         c->label_flag = false;
+        c->fused_subs = false;
         c->inc_base = false;
         c->inc_ready = false;
         c->reset_update_ok = false;
@@ -2425,7 +2529,10 @@ int dpmm_relabel(dpmm_ctx* c, uint32_t iter, int K_old, const int32_t* new_label
     c->reset_update_ok = resets_only && c->inc_base && c->subs_perm_valid && c->partition_valid &&
                          c->Kpartition == K_old && c->Kstats == K_old;
     c->inc_base = false;
-    if (labels_change) c->partition_valid = false;
+    if (labels_change) {
+        c->partition_valid = false;
+        c->fused_subs = false;
+    }
     c->subs_perm_valid = false;   // sub-labels were rewritten by index
     c->stats_cached = false;
     if (resets_only && ((c->partials_valid && c->partialsK == K_old) || c->reset_update_ok)) {
@@ -2487,6 +2594,12 @@ int dpmm_set_label_pruning(dpmm_ctx* c, int mode) {
     if (!c || mode < 0 || mode > 2) return fail(DPMM_EINVAL, "pruning mode is 0, 1 or 2");
     c->prune_mode = mode;
     c->prune_skip = 0;
+    return DPMM_OK;
+}
+
+int dpmm_set_fused_sub_labels(dpmm_ctx* c, int enable) {
+    if (!c) return fail(DPMM_EINVAL, "null ctx");
+    c->fuse_mode = enable ? 1 : 0;
     return DPMM_OK;
 }
 
@@ -2725,6 +2838,7 @@ static int enqueue_draws(dpmm_ctx* c, uint32_t iter, int K, const int32_t* dmem,
         unsigned char* pb = nullptr;
         unsigned char* spb = nullptr;
         c->spb_valid = false;
+        c->fused_subs = false;
         if (c->precision == DPMM_PREC_MIXED) {
             if ((rc = ensure_mixed_buffers(c, K))) return rc;
             pb = reinterpret_cast<unsigned char*>(c->dPB);
@@ -2879,6 +2993,7 @@ int grid_relabel(const dpmm_ctx* c) { return (int)std::min<int64_t>((c->N + kThr
 // Every cache derived from the labels or sub-labels, after a relabel pass on the device.
 void labels_rewritten(dpmm_ctx* c) {
     c->partition_valid = false;
+    c->fused_subs = false;
     c->subs_perm_valid = false;
     c->stats_cached = false;
     c->partials_valid = false;

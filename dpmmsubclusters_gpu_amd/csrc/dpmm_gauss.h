@@ -168,12 +168,13 @@ __global__ void __launch_bounds__(kThreads) k_label_gauss_listp(const T* __restr
         }
         if (valid && par == 0) labels[i] = bi;
         // a label that moved: the partition of the labels before the sweep no longer stands (changed[0]),
-        // and the statistics of both clusters are recomputed in full (changed[1 + k])
+        // and the statistics of both clusters are recomputed in full (changed[1 + k]); the cluster it
+        // left is the one the swept partition filed it under (a, 0 for a label outside [0, K))
         if (changed) {
             const bool mv = valid && par == 0 && bi != a0;
             if (mv) {
                 changed[1 + bi] = 1;
-                if ((unsigned)a0 < (unsigned)K) changed[1 + a0] = 1;
+                changed[1 + a] = 1;
             }
             if (__ballot(mv) && (threadIdx.x & 63) == 0) changed[0] = 1;
         }

@@ -92,8 +92,19 @@ struct WaveList {
 // Slot of one cluster for the screen: [tiles RB x KC x 64 lanes x 8 bf16 | mu (DP f32) | c | pad].
 __host__ __device__ constexpr int mx_tiles_bytes(int DP) { return (DP / 32) * (DP / 16) * 64 * 16; }
 __host__ __device__ constexpr int mx_slot_bytes(int DP) { return (mx_tiles_bytes(DP) + DP * 4 + 4 + 15) & ~15; }
-// Sub-cluster slot of the sub-label screen: W split as hi + lo tiles, then mu (DP f32), c, pad.
-__host__ __device__ constexpr int mx2_slot_bytes(int DP) { return (2 * mx_tiles_bytes(DP) + DP * 4 + 4 + 15) & ~15; }
+// A cluster slot also holds, after c, F >= ||W||_F^2 of the FP32 W (the fused sweep's anchor bound).
+// Sub-cluster slot of the sub-label screen: W split as hi + lo tiles, then mu (DP f32), c, pad, and for
+// the screen relative to the parent cluster's mean mu_a (the fused sweep): -c_j (DP f32) with
+// c = W (mu - mu_a) from the FP32 slot values, and g_j (DP f32) = kappa (sum_l |W_jl| |mu_l - mu_a,l| + |c_j|)
+// rounded up, kappa = sub_rel_eps_g / sub_rel_eps_a.
+__host__ __device__ constexpr int mx2_slot_bytes(int DP) { return 2 * mx_tiles_bytes(DP) + DP * 4 + 16 + 2 * DP * 4; }
+__host__ __device__ constexpr int mx2_rel_offset(int DP) { return 2 * mx_tiles_bytes(DP) + DP * 4 + 16; }
+// Per-row error factors of the relative sub-cluster screen (see sub_screen_rel): |y_screen,j - y_fp32,j| <=
+// eps_a a_j + eps_g g_j with a_j = sum_l |W_jl| |x_l - mu_a,l| from BF16 magnitudes (inflated by 2^-6).
+__host__ __device__ constexpr float sub_rel_eps_a(int DP) {
+    return (3.0f * 0x1p-18f + (5.0f * (float)DP + 24.0f) * 0x1p-24f) * (1.0f + 0x1p-6f);
+}
+__host__ __device__ constexpr float sub_rel_eps_g(int DP) { return (3.0f * (float)DP + 16.0f) * 0x1p-24f * (1.0f + 0x1p-6f); }
 // Lane-order index of W[j][l] inside the tile array (row j, column l).
 __host__ __device__ constexpr int mx_tile_index(int DP, int j, int l) {
     // tile (rb, kc); lane = (j % 32) + 32 * ((l % 16) / 8); element e = l % 8
@@ -242,6 +253,83 @@ void launch_label_gauss_mx(hipStream_t st, int64_t N, int D, int K, const float*
                        iter, fin, labels, recheck, recheck_count);
 }
 
+// Split-BF16 screen of the two sub-clusters whose slots start at sp (mx2 slots, one after the other)
+// for the 32 points of a wave (lane L: point L % 32, coordinates 16 kc + 8 (L / 32) .. + 7 in x):
+// returns s_0 - s_1 of the lane's point and sets err, the bound on its distance from the FP32 sweep's
+// S_0 - S_1 (see k_sub_gauss_mx).  Every lane of the wave calls it.
+template <int DP>
+__device__ __forceinline__ float sub_screen(const float (&x)[DP / 16][8], const unsigned char* sp, int lane,
+                                            float& err) {
+    typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+    constexpr int RB = DP / 32, KC = DP / 16;
+    constexpr int SLOT = mx2_slot_bytes(DP), TB = mx_tiles_bytes(DP);
+    const int h = lane >> 5;
+    // |y_screen - y_fp32| <= eps a_j: split residuals (3 x 2^-18 of |W||d|) and the FP32 sums of the
+    // screen (4 D terms) and of the sweep (D terms), with a_j from BF16 magnitudes inflated by 2^-6
+    const float eps = (3.0f * 0x1p-18f + (5.0f * (float)DP + 16.0f) * 0x1p-24f) * (1.0f + 0x1p-6f);
+    const float heps = 0.5f * eps;
+    float s0 = 0.f, s1 = 0.f, dq0 = 0.f, dq1 = 0.f;
+#pragma unroll 1
+    for (int hh = 0; hh < 2; ++hh) {
+        const unsigned char* slot = sp + hh * SLOT;
+        const bf16x8* thi = reinterpret_cast<const bf16x8*>(slot);
+        const bf16x8* tlo = reinterpret_cast<const bf16x8*>(slot + TB);
+        const float* mu = reinterpret_cast<const float*>(slot + 2 * TB);
+        // d = x - mu split as d_hi + d_lo: one packed conversion per coordinate pair gives both
+        // BF16 highs, read back as FP32 by shifting / masking the packed word
+        u32x4 dh[KC], dl[KC];
+#pragma unroll
+        for (int kc = 0; kc < KC; ++kc)
+#pragma unroll
+            for (int e = 0; e < 8; e += 2) {
+                typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+                const float d0 = x[kc][e] - mu[16 * kc + 8 * h + e];
+                const float d1 = x[kc][e + 1] - mu[16 * kc + 8 * h + e + 1];
+                bf16x2 hp;
+                hp[0] = (__bf16)d0;
+                hp[1] = (__bf16)d1;
+                const uint32_t hb = __builtin_bit_cast(uint32_t, hp);
+                bf16x2 lp;
+                lp[0] = (__bf16)(d0 - __uint_as_float(hb << 16));
+                lp[1] = (__bf16)(d1 - __uint_as_float(hb & 0xffff0000u));
+                dh[kc][e / 2] = hb;
+                dl[kc][e / 2] = __builtin_bit_cast(uint32_t, lp);
+            }
+        // qq = sum y_j^2; ee = sum a_j (|y_j| + eps a_j / 2), so that the bound
+        // sum (2 |y_j| + eps a_j) eps a_j is 2 eps ee
+        float qq = 0.f, ee = 0.f;
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb) {
+            f32x16 y = {}, a = {};
+#pragma unroll
+            for (int kc = 2 * rb; kc < KC; ++kc) {
+                const bf16x8 wh = thi[(rb * KC + kc) * 64 + lane];
+                const bf16x8 wl = tlo[(rb * KC + kc) * 64 + lane];
+                const bf16x8 bh = __builtin_bit_cast(bf16x8, dh[kc]), bl = __builtin_bit_cast(bf16x8, dl[kc]);
+                const u32x4 wabs = __builtin_bit_cast(u32x4, wh) & 0x7fff7fffu;
+                const u32x4 dabs = dh[kc] & 0x7fff7fffu;
+                y = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wh, bh, y, 0, 0, 0);
+                y = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wh, bl, y, 0, 0, 0);
+                y = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wl, bh, y, 0, 0, 0);
+                a = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, wabs),
+                                                            __builtin_bit_cast(bf16x8, dabs), a, 0, 0, 0);
+            }
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                qq = fmaf(y[v], y[v], qq);
+                ee = fmaf(a[v], fmaf(heps, a[v], fabsf(y[v])), ee);
+            }
+        }
+        qq += __shfl_xor(qq, 32);
+        ee = 2.0f * eps * (ee + __shfl_xor(ee, 32));
+        const float sv = mu[DP] - 0.5f * qq;
+        const float dv = ee + (2.0f * (float)DP + 4.0f) * 0x1p-24f * qq;   // the two sums of squares
+        if (hh == 0) { s0 = sv; dq0 = dv; } else { s1 = sv; dq1 = dv; }
+    }
+    err = 0.5f * (dq0 + dq1) * (1.0f + 0x1p-10f) + 0x1p-21f * (fabsf(s0) + fabsf(s1)) + 1e-6f;
+    return s0 - s1;
+}
+
 // Sub-label screen of one chunk (the points of cluster k): both sub-clusters' split-BF16 slots in
 // LDS, 32 gathered points per wave.  With d = x - mu_h in FP32 split as d_hi + d_lo and W = W_hi + W_lo,
 // y = W_hi d_hi + W_hi d_lo + W_lo d_hi on MFMA (FP32 accumulation), and a fourth MFMA gives
@@ -260,9 +348,8 @@ __global__ void __launch_bounds__(kThreads) k_sub_gauss_mx(const float* __restri
                                                            int32_t* __restrict__ subs, int32_t* __restrict__ subs_perm,
                                                            int32_t* __restrict__ recheck,
                                                            int* __restrict__ recheck_count) {
-    typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-    constexpr int RB = DP / 32, KC = DP / 16;
-    constexpr int SLOT = mx2_slot_bytes(DP), TB = mx_tiles_bytes(DP);
+    constexpr int KC = DP / 16;
+    constexpr int SLOT = mx2_slot_bytes(DP);
     __shared__ __align__(16) unsigned char sp[2 * SLOT];
     int k, beg, end;
     if (!chunk_of(blockIdx.x, K, chunk_start, label_start, CH, k, beg, end)) return;
@@ -273,10 +360,6 @@ __global__ void __launch_bounds__(kThreads) k_sub_gauss_mx(const float* __restri
     }
     __syncthreads();
     (void)tau;
-    // |y_screen - y_fp32| <= eps a_j: split residuals (3 x 2^-18 of |W||d|) and the FP32 sums of the
-    // screen (4 D terms) and of the sweep (D terms), with a_j from BF16 magnitudes inflated by 2^-6
-    const float eps = (3.0f * 0x1p-18f + (5.0f * (float)DP + 16.0f) * 0x1p-24f) * (1.0f + 0x1p-6f);
-    const float heps = 0.5f * eps;
     const int lane = threadIdx.x & 63, h = lane >> 5, pl = lane & 31, wv = threadIdx.x >> 6;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     __shared__ int32_t wlist[4][64];
@@ -289,66 +372,8 @@ __global__ void __launch_bounds__(kThreads) k_sub_gauss_mx(const float* __restri
         const float* row = X + (int64_t)idx * D;   // idx = 0 for an empty lane
 #pragma unroll
         for (int kc = 0; kc < KC; ++kc) load_coords8(row, 16 * kc + 8 * h, D, valid, x[kc]);
-        float s0 = 0.f, s1 = 0.f, dq0 = 0.f, dq1 = 0.f;
-#pragma unroll 1
-        for (int hh = 0; hh < 2; ++hh) {
-            const unsigned char* slot = sp + hh * SLOT;
-            const bf16x8* thi = reinterpret_cast<const bf16x8*>(slot);
-            const bf16x8* tlo = reinterpret_cast<const bf16x8*>(slot + TB);
-            const float* mu = reinterpret_cast<const float*>(slot + 2 * TB);
-            // d = x - mu split as d_hi + d_lo: one packed conversion per coordinate pair gives both
-            // BF16 highs, read back as FP32 by shifting / masking the packed word
-            u32x4 dh[KC], dl[KC];
-#pragma unroll
-            for (int kc = 0; kc < KC; ++kc)
-#pragma unroll
-                for (int e = 0; e < 8; e += 2) {
-                    typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
-                    const float d0 = x[kc][e] - mu[16 * kc + 8 * h + e];
-                    const float d1 = x[kc][e + 1] - mu[16 * kc + 8 * h + e + 1];
-                    bf16x2 hp;
-                    hp[0] = (__bf16)d0;
-                    hp[1] = (__bf16)d1;
-                    const uint32_t hb = __builtin_bit_cast(uint32_t, hp);
-                    bf16x2 lp;
-                    lp[0] = (__bf16)(d0 - __uint_as_float(hb << 16));
-                    lp[1] = (__bf16)(d1 - __uint_as_float(hb & 0xffff0000u));
-                    dh[kc][e / 2] = hb;
-                    dl[kc][e / 2] = __builtin_bit_cast(uint32_t, lp);
-                }
-            // qq = sum y_j^2; ee = sum a_j (|y_j| + eps a_j / 2), so that the bound
-            // sum (2 |y_j| + eps a_j) eps a_j is 2 eps ee
-            float qq = 0.f, ee = 0.f;
-#pragma unroll
-            for (int rb = 0; rb < RB; ++rb) {
-                f32x16 y = {}, a = {};
-#pragma unroll
-                for (int kc = 2 * rb; kc < KC; ++kc) {
-                    const bf16x8 wh = thi[(rb * KC + kc) * 64 + lane];
-                    const bf16x8 wl = tlo[(rb * KC + kc) * 64 + lane];
-                    const bf16x8 bh = __builtin_bit_cast(bf16x8, dh[kc]), bl = __builtin_bit_cast(bf16x8, dl[kc]);
-                    const u32x4 wabs = __builtin_bit_cast(u32x4, wh) & 0x7fff7fffu;
-                    const u32x4 dabs = dh[kc] & 0x7fff7fffu;
-                    y = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wh, bh, y, 0, 0, 0);
-                    y = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wh, bl, y, 0, 0, 0);
-                    y = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wl, bh, y, 0, 0, 0);
-                    a = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, wabs),
-                                                                __builtin_bit_cast(bf16x8, dabs), a, 0, 0, 0);
-                }
-#pragma unroll
-                for (int v = 0; v < 16; ++v) {
-                    qq = fmaf(y[v], y[v], qq);
-                    ee = fmaf(a[v], fmaf(heps, a[v], fabsf(y[v])), ee);
-                }
-            }
-            qq += __shfl_xor(qq, 32);
-            ee = 2.0f * eps * (ee + __shfl_xor(ee, 32));
-            const float sv = mu[DP] - 0.5f * qq;
-            const float dv = ee + (2.0f * (float)DP + 4.0f) * 0x1p-24f * qq;   // the two sums of squares
-            if (hh == 0) { s0 = sv; dq0 = dv; } else { s1 = sv; dq1 = dv; }
-        }
-        const float d = s0 - s1;
-        const float err = 0.5f * (dq0 + dq1) * (1.0f + 0x1p-10f) + 0x1p-21f * (fabsf(s0) + fabsf(s1)) + 1e-6f;
+        float err;
+        const float d = sub_screen<DP>(x, sp, lane, err);
         bool need = false;
         if (valid && h == 0) {
             const float lg = logit_u32<float>(philox4x32_10((uint32_t)idx, 0, iter, TAG_SUB, key0, key1).v[0]);
@@ -387,10 +412,139 @@ void launch_sub_gauss_mx(hipStream_t st, int grid, const float* X, int D, int K,
 // point can take label k and the cluster is skipped (ub_k joins the runner-up bound U); the other
 // clusters are screened as in k_label_gauss_mx, so the decided labels are the FP32 sweep's.
 
+// Screens of the fused sweep for the 32 points of a wave (lane L: point L % 32, coordinates
+// 16 kc + 8 (L / 32) .. + 7 in x) against the anchor a (slot sa) and its two sub-clusters (slots ssub),
+// from one hi / lo BF16 split of p = fl(x - mu_a):
+//   anchor: y = bf16(W_a) p_hi on MFMA, the label screen's y (k_label_gauss_mx rounds p the same way);
+//     its bound takes A = F_a ||p||^2 >= sum_j (sum_l |W_jl| |p_l|)^2 (Cauchy-Schwarz per row) in place
+//     of the magnitude MFMA;
+//   sub-cluster h: W_h (x - mu_h) = W_h p - c_h, c_h = W_h (mu_h - mu_a), as
+//     y = -c_h + W_hi p_hi + W_hi p_lo + W_lo p_hi (FP32 accumulation started from -c_h), with
+//     a'_j = g_j + sum_l |W_hi,jl| |p_hi,l| (started from g_j).  Against the FP32 sweep's
+//     y_j = sum_l W_jl fl(x_l - mu_h,l) the screen is off by at most eps_a a_j + eps_g g'_j, where the
+//     split residuals and the FP32 sums of both sides give eps_a and the FP32 roundings of p, of c and
+//     of the sweep's residual (|x - mu_h| <= |p| + |mu_h - mu_a|) give eps_g; g_j already carries
+//     eps_g / eps_a, so e_j <= eps_a a'_j and the bound on q_h follows as in sub_screen.
+// Returns q_a; sets d2 = ||p||^2 (FP32), Aa, and sd = s_0 - s_1 with its bound sderr.
+template <int DP>
+__device__ __forceinline__ float fused_screens(const float (&x)[DP / 16][8], const unsigned char* sa,
+                                               const unsigned char* ssub, int lane, float& d2, float& Aa,
+                                               float& sd, float& sderr) {
+    typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+    typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+    constexpr int RB = DP / 32, KC = DP / 16, TB = mx_tiles_bytes(DP), SLOT2 = mx2_slot_bytes(DP);
+    const int h = lane >> 5;
+    const float4* mu4 = reinterpret_cast<const float4*>(sa + TB);
+    u32x4 ph[KC], pl[KC];
+    float dd = 0.f;
+#pragma unroll
+    for (int kc = 0; kc < KC; ++kc) {
+        const float4 ma = mu4[4 * kc + 2 * h], mb = mu4[4 * kc + 2 * h + 1];
+        const float mv[8] = {ma.x, ma.y, ma.z, ma.w, mb.x, mb.y, mb.z, mb.w};
+#pragma unroll
+        for (int e = 0; e < 8; e += 2) {
+            const float d0 = x[kc][e] - mv[e];
+            const float d1 = x[kc][e + 1] - mv[e + 1];
+            dd = fmaf(d0, d0, dd);
+            dd = fmaf(d1, d1, dd);
+            bf16x2 hp;
+            hp[0] = (__bf16)d0;
+            hp[1] = (__bf16)d1;
+            const uint32_t hb = __builtin_bit_cast(uint32_t, hp);
+            bf16x2 lp;
+            lp[0] = (__bf16)(d0 - __uint_as_float(hb << 16));
+            lp[1] = (__bf16)(d1 - __uint_as_float(hb & 0xffff0000u));
+            ph[kc][e / 2] = hb;
+            pl[kc][e / 2] = __builtin_bit_cast(uint32_t, lp);
+        }
+    }
+    // anchor
+    const bf16x8* ta = reinterpret_cast<const bf16x8*>(sa);
+    float q = 0.f;
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) {
+        f32x16 acc = {};
+#pragma unroll
+        for (int kc = 2 * rb; kc < KC; ++kc)
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ta[(rb * KC + kc) * 64 + lane],
+                                                          __builtin_bit_cast(bf16x8, ph[kc]), acc, 0, 0, 0);
+#pragma unroll
+        for (int v = 0; v < 16; ++v) q = fmaf(acc[v], acc[v], q);
+    }
+    d2 = dd + __shfl_xor(dd, 32);
+    Aa = reinterpret_cast<const float*>(sa + TB)[DP + 1] * d2 * (1.0f + ((float)DP + 4.0f) * 0x1p-24f);
+    // the anchor's sub-clusters
+    constexpr float eps = sub_rel_eps_a(DP);
+    const float heps = 0.5f * eps;
+    float s0 = 0.f, s1 = 0.f, dq0 = 0.f, dq1 = 0.f;
+#pragma unroll 1
+    for (int hh = 0; hh < 2; ++hh) {
+        const unsigned char* slot = ssub + hh * SLOT2;
+        const bf16x8* thi = reinterpret_cast<const bf16x8*>(slot);
+        const bf16x8* tlo = reinterpret_cast<const bf16x8*>(slot + TB);
+        const float* rel = reinterpret_cast<const float*>(slot + mx2_rel_offset(DP));   // -c (DP), then g (DP)
+        float qq = 0.f, ee = 0.f;
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb) {
+            // accumulators start from -c_j and g_j of this lane's 16 output rows 32 rb + 8 (v / 4) + 4 h + v % 4
+            f32x16 y, a;
+#pragma unroll
+            for (int g4 = 0; g4 < 4; ++g4) {
+                const float4 cv = *reinterpret_cast<const float4*>(rel + 32 * rb + 8 * g4 + 4 * h);
+                const float4 gv = *reinterpret_cast<const float4*>(rel + DP + 32 * rb + 8 * g4 + 4 * h);
+                y[4 * g4] = cv.x; y[4 * g4 + 1] = cv.y; y[4 * g4 + 2] = cv.z; y[4 * g4 + 3] = cv.w;
+                a[4 * g4] = gv.x; a[4 * g4 + 1] = gv.y; a[4 * g4 + 2] = gv.z; a[4 * g4 + 3] = gv.w;
+            }
+#pragma unroll
+            for (int kc = 2 * rb; kc < KC; ++kc) {
+                const bf16x8 wh = thi[(rb * KC + kc) * 64 + lane];
+                const bf16x8 wl = tlo[(rb * KC + kc) * 64 + lane];
+                const bf16x8 bh = __builtin_bit_cast(bf16x8, ph[kc]), bl = __builtin_bit_cast(bf16x8, pl[kc]);
+                const u32x4 wabs = __builtin_bit_cast(u32x4, wh) & 0x7fff7fffu;
+                const u32x4 pabs = ph[kc] & 0x7fff7fffu;
+                y = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wh, bh, y, 0, 0, 0);
+                y = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wh, bl, y, 0, 0, 0);
+                y = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wl, bh, y, 0, 0, 0);
+                a = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, wabs),
+                                                            __builtin_bit_cast(bf16x8, pabs), a, 0, 0, 0);
+            }
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                qq = fmaf(y[v], y[v], qq);
+                ee = fmaf(a[v], fmaf(heps, a[v], fabsf(y[v])), ee);
+            }
+        }
+        qq += __shfl_xor(qq, 32);
+        ee = 2.0f * eps * (ee + __shfl_xor(ee, 32));
+        const float sv = reinterpret_cast<const float*>(slot + 2 * TB)[DP] - 0.5f * qq;
+        const float dv = ee + (2.0f * (float)DP + 4.0f) * 0x1p-24f * qq;   // the two sums of squares
+        if (hh == 0) { s0 = sv; dq0 = dv; } else { s1 = sv; dq1 = dv; }
+    }
+    sderr = 0.5f * (dq0 + dq1) * (1.0f + 0x1p-10f) + 0x1p-21f * (fabsf(s0) + fabsf(s1)) + 1e-6f;
+    sd = s0 - s1;
+    return q + __shfl_xor(q, 32);
+}
+
+// The sub-label screen fused into the pruned sweep.  A point whose label stays its chunk's cluster
+// (the anchor) is screened against the anchor's two sub-clusters in the same pass, with the
+// arithmetic and bound of k_sub_gauss_mx; the points whose label moved or went to the recheck, and
+// those whose sub-label the screen leaves undecided, are listed (positions in the swept partition)
+// for the FP32 draw after the label recheck.  skip_same (both forms): every label of a chunk equals
+// its anchor before the sweep, so a label that stays is not written again.
+struct SubFuse {
+    const unsigned char* SPB;    // 2K sub-cluster screen slots (split-BF16), null: no fused screen
+    const int32_t* subs_prev;    // sub-labels before the sweep in the swept partition's order, or null
+    int32_t* subs;               // sub-labels by point index, written where they change
+    int32_t* subs_old;           // sub-labels in the swept partition's order
+    int32_t* sub_list;           // positions left to the FP32 sub-label draw
+    int* sub_count;
+    int skip_same;
+};
+
 // Workgroups per partition chunk in the pruned sweep (each takes an equal share of the chunk).
 constexpr int kPruneSplit = 4;
 
-template <int DP>
+template <int DP, bool FUSE>
 __global__ void __launch_bounds__(kThreads, DP <= 64 ? 3 : 2)
 k_label_gauss_mxp(const float* __restrict__ X, int D, int K, int CH, const int32_t* __restrict__ perm,
                   const int32_t* __restrict__ label_start, const int32_t* __restrict__ chunk_start,
@@ -398,7 +552,8 @@ k_label_gauss_mxp(const float* __restrict__ X, int D, int K, int CH, const int32
                   const float* __restrict__ psc, float tau, uint32_t key0, uint32_t key1, uint32_t iter,
                   int final_iter, int32_t* __restrict__ labels, int32_t* __restrict__ recheck,
                   int* __restrict__ recheck_count, unsigned long long* __restrict__ evals,
-                  int* __restrict__ changed) {
+                  int* __restrict__ changed, SubFuse sf) {
+    static_assert(!FUSE || DP <= 64, "the fused sub-label screen keeps three slots in LDS");
     constexpr int RB = DP / 32, KC = DP / 16;
     constexpr int SLOT = mx_slot_bytes(DP);
     // one 32-point block per wave: lane L scores point L % 32 on coordinate half L / 32
@@ -413,10 +568,18 @@ k_label_gauss_mxp(const float* __restrict__ X, int D, int K, int CH, const int32
     // the anchor's slot, scored by every pass of the workgroup, is staged in LDS once; the other
     // clusters' slots are read from global memory by the few passes that score them
     __shared__ __align__(16) unsigned char sa[SLOT];
+    // fused: the anchor's two sub-cluster slots as well (the sub-label screen of the points that stay)
+    constexpr int SLOT2 = mx2_slot_bytes(DP);
+    __shared__ __align__(16) unsigned char ssub[FUSE ? 2 * SLOT2 : 16];
     {
         const uint4* src = reinterpret_cast<const uint4*>(PB + (size_t)a * SLOT);
         uint4* dst = reinterpret_cast<uint4*>(sa);
         for (int e = threadIdx.x; e < SLOT / 16; e += kThreads) dst[e] = src[e];
+        if constexpr (FUSE) {
+            const uint4* s2 = reinterpret_cast<const uint4*>(sf.SPB + (size_t)(2 * a) * SLOT2);
+            uint4* d2 = reinterpret_cast<uint4*>(ssub);
+            for (int e = threadIdx.x; e < 2 * SLOT2 / 16; e += kThreads) d2[e] = s2[e];
+        }
     }
     __syncthreads();
     // the anchor's bound inputs of clusters 0..63, one per lane (the same for every pass)
@@ -425,6 +588,8 @@ k_label_gauss_mxp(const float* __restrict__ X, int D, int K, int CH, const int32
     unsigned long long nev = 0, npass = 0;
     __shared__ int32_t wlist[4][64];
     WaveList rl{wlist[wv], 0};
+    __shared__ int32_t wlist2[FUSE ? 4 : 1][64];
+    WaveList sl{wlist2[FUSE ? wv : 0], 0};
     int idx_next = (beg + wv * 32 + pl < end) ? perm[beg + wv * 32 + pl] : 0;
     for (int p0 = beg + wv * 32; p0 < end; p0 += 4 * 32) {   // wave-uniform trip count
         const int pos = p0 + pl;
@@ -447,17 +612,8 @@ k_label_gauss_mxp(const float* __restrict__ X, int D, int K, int CH, const int32
         float xh[kHoldX ? KC : 1][8];
         if constexpr (kHoldX) load_x((gfloat*)X, xh);
         // screened q of the cluster whose slot is `slot` (the anchor's in LDS, or one in global
-        // memory) for this lane's point, sum (x - mu_k)^2 and the bound's sum ee
-        auto score = [&](const unsigned char* slot, float& d2, float& eo) -> float {
-            float xr[kHoldX ? 1 : KC][8];
-            if constexpr (!kHoldX) {
-                gfloat* Xs = (gfloat*)X;
-                asm volatile("" : "+s"(Xs));   // a load per scoring, not one hoisted out of the cluster loop
-                load_x(Xs, xr);
-            }
-            auto& x = [&]() -> float(&)[KC][8] {
-                if constexpr (kHoldX) return xh; else return xr;
-            }();
+        // memory) for this lane's point with coordinates x, sum (x - mu_k)^2 and the bound's sum ee
+        auto score_x = [&](const unsigned char* slot, const float (&x)[KC][8], float& d2, float& eo) -> float {
             const bf16x8* tiles = reinterpret_cast<const bf16x8*>(slot);
             const float4* mu4 = reinterpret_cast<const float4*>(slot + mx_tiles_bytes(DP));
             bf16x8 Bv[KC];
@@ -497,6 +653,17 @@ k_label_gauss_mxp(const float* __restrict__ X, int D, int K, int CH, const int32
             eo = ee + __shfl_xor(ee, 32);
             return q + __shfl_xor(q, 32);
         };
+        auto score = [&](const unsigned char* slot, float& d2, float& eo) -> float {
+            if constexpr (kHoldX) {
+                return score_x(slot, xh, d2, eo);
+            } else {
+                float xr[KC][8];
+                gfloat* Xs = (gfloat*)X;
+                asm volatile("" : "+s"(Xs));   // a load per scoring, not one hoisted out of the cluster loop
+                load_x(Xs, xr);
+                return score_x(slot, xr, d2, eo);
+            }
+        };
         auto noisy = [&](const unsigned char* slot, int k, float q) -> float {
             const float c = reinterpret_cast<const float*>(slot + mx_tiles_bytes(DP))[DP];
             float s = c - 0.5f * q;
@@ -508,10 +675,38 @@ k_label_gauss_mxp(const float* __restrict__ X, int D, int K, int CH, const int32
         };
         // the anchor, then the wave bounds: r >= ||x - mu_a|| over the points (the FP32 rounding of
         // the distance covered) and T = the smallest screened lower bound s_a - e_a
-        float d2 = 0.f, ea = 0.f;
-        const float qa = score(sa, d2, ea);
+        float d2 = 0.f, ea = 0.f, qa;
+        // fused: the screen of the anchor's sub-clusters for every point, on the coordinates the
+        // anchor's scoring read (the few points whose label moves ignore it)
+        float sd = 0.f, sderr = 0.f;
+        if constexpr (FUSE && !kHoldX) {
+            float xa[KC][8];
+            load_x((gfloat*)X, xa);
+            qa = fused_screens<DP>(xa, sa, ssub, lane, d2, ea, sd, sderr);
+        } else if constexpr (FUSE) {
+            qa = fused_screens<DP>(xh, sa, ssub, lane, d2, ea, sd, sderr);
+        } else {
+            qa = score(sa, d2, ea);
+        }
         const float* ca = reinterpret_cast<const float*>(sa + mx_tiles_bytes(DP)) + DP;
-        float sb = noisy(sa, a, qa), eb = label_screen_err<DP>(tau, qa, ea, *ca), U = neg_inf<float>();
+        float sb = *ca - 0.5f * qa;
+        // the anchor's label noise and, fused, the sub-label draw's word come from one Philox
+        // evaluation per lane: half 0 draws the former, half 1 (the same points) the latter
+        uint32_t sub_word = 0;
+        if (FUSE || !final_iter) {
+            const bool sub_half = FUSE && h;
+            const Ctr4 w = philox4x32_10((uint32_t)idx, sub_half ? 0u : (uint32_t)(a >> 2), iter,
+                                         sub_half ? TAG_SUB : TAG_LABEL, key0, key1);
+            if constexpr (FUSE) {
+                const uint32_t mine = h ? w.v[0] : sel4(w, a & 3);
+                const uint32_t other = __shfl_xor(mine, 32);
+                if (!final_iter) sb += gumbel_screen(h ? other : mine);
+                sub_word = h ? mine : other;
+            } else {
+                sb += gumbel_screen(sel4(w, a & 3));
+            }
+        }
+        float eb = label_screen_err<DP>(tau, qa, ea, *ca), U = neg_inf<float>();
         int bi = a;
         float rr = own_valid ? sqrtf(d2) : 0.f;
         float tt = own_valid ? sb - eb : __builtin_huge_valf();
@@ -558,7 +753,26 @@ k_label_gauss_mxp(const float* __restrict__ X, int D, int K, int CH, const int32
         nev += ne;
         ++npass;
         const bool sure = sb - eb > U;
-        if (own_valid && sure) labels[idx] = bi;
+        // with skip_same every label in the chunk is the anchor before the sweep: a label that stays is not
+        // written again
+        if (own_valid && sure && !(sf.skip_same && bi == a)) labels[idx] = bi;
+        if constexpr (FUSE) {
+            // sub-labels of the points whose label stays the anchor, from its sub-clusters' screen; the others
+            // (labels moved or left to the recheck, undecided sub-labels) go to the sub list
+            const bool stay = own_valid && sure && bi == a;
+            bool need_sub = own_valid && !stay;
+            if (stay) {
+                const float lg = logit_u32<float>(sub_word);
+                if (fabsf(lg - sd) > sderr) {
+                    const int v = lg < sd ? 0 : 1;
+                    sf.subs_old[pos] = v;
+                    if (!sf.subs_prev || sf.subs_prev[pos] != v) sf.subs[idx] = v;
+                } else {
+                    need_sub = true;
+                }
+            }
+            sl.push(need_sub, pos, sf.sub_list, sf.sub_count, lane, lt);
+        }
         // a label that moved off the chunk's cluster: the partition of the labels before the sweep no longer
         // stands (changed[0]), and the statistics of both clusters are recomputed in full (changed[1 + k])
         if (changed) {
@@ -572,6 +786,7 @@ k_label_gauss_mxp(const float* __restrict__ X, int D, int K, int CH, const int32
         rl.push(own_valid && !sure, (int32_t)idx, recheck, recheck_count, lane, lt);
     }
     rl.flush(recheck, recheck_count, lane);
+    if constexpr (FUSE) sl.flush(sf.sub_list, sf.sub_count, lane);
     __shared__ unsigned long long cnt[2][4];   // one pair of counter atomics per workgroup
     if (lane == 0) {
         cnt[0][wv] = nev;
@@ -588,15 +803,24 @@ k_label_gauss_mxp(const float* __restrict__ X, int D, int K, int CH, const int32
     }
 }
 
+// sf.SPB non-null (DP <= 64 only): the sub-label screen fused into the sweep.
 template <int DP>
 void launch_label_gauss_mxp(hipStream_t st, int grid, const float* X, int D, int K, int CH, const int32_t* perm,
                             const int32_t* label_start, const int32_t* chunk_start, const unsigned char* PB,
                             const float* pm, const float* psc, float tau, uint32_t key0, uint32_t key1, uint32_t iter,
                             int fin, int32_t* labels, int32_t* recheck, int* recheck_count,
-                            unsigned long long* evals, int* changed) {
-    hipLaunchKernelGGL((k_label_gauss_mxp<DP>), dim3(grid), dim3(kThreads), 0, st, X, D, K, CH, perm, label_start,
-                       chunk_start, PB, pm, psc, tau, key0, key1, iter, fin, labels, recheck, recheck_count, evals,
-                       changed);
+                            unsigned long long* evals, int* changed, SubFuse sf) {
+    if constexpr (DP <= 64) {
+        if (sf.SPB) {
+            hipLaunchKernelGGL((k_label_gauss_mxp<DP, true>), dim3(grid), dim3(kThreads), 0, st, X, D, K, CH, perm,
+                               label_start, chunk_start, PB, pm, psc, tau, key0, key1, iter, fin, labels, recheck,
+                               recheck_count, evals, changed, sf);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_label_gauss_mxp<DP, false>), dim3(grid), dim3(kThreads), 0, st, X, D, K, CH, perm,
+                       label_start, chunk_start, PB, pm, psc, tau, key0, key1, iter, fin, labels, recheck,
+                       recheck_count, evals, changed, sf);
 }
 
 #define DPMM_MX_INSTANTIATE(DP)                                                                               \
@@ -608,7 +832,7 @@ void launch_label_gauss_mxp(hipStream_t st, int grid, const float* X, int D, int
     template void launch_label_gauss_mxp<DP>(hipStream_t, int, const float*, int, int, int, const int32_t*,   \
                                              const int32_t*, const int32_t*, const unsigned char*, const float*, \
                                              const float*, float, uint32_t, uint32_t, uint32_t, int, int32_t*,   \
-                                             int32_t*, int*, unsigned long long*, int*);
+                                             int32_t*, int*, unsigned long long*, int*, SubFuse);
 #define DPMM_MX_EXTERN(DP)                                                                                    \
     extern template void launch_label_gauss_mx<DP>(hipStream_t, int64_t, int, int, const float*,               \
                                                    const unsigned char*, float, uint32_t, uint32_t, uint32_t,  \
@@ -621,6 +845,6 @@ void launch_label_gauss_mxp(hipStream_t st, int grid, const float* X, int D, int
                                                     const int32_t*, const int32_t*, const int32_t*,            \
                                                     const unsigned char*, const float*, const float*, float,   \
                                                     uint32_t, uint32_t, uint32_t, int, int32_t*, int32_t*, int*, \
-                                                    unsigned long long*, int*);
+                                                    unsigned long long*, int*, SubFuse);
 
 }  // namespace dpmm

@@ -758,6 +758,39 @@ __global__ void __launch_bounds__(kThreads) k_pack_gauss(const double* __restric
         dst[gauss_c_offset(DP)] = (T)cst;
         if (tiles) reinterpret_cast<float*>(mslot_base + mu_off)[DPM] = (float)cst;
     }
+    if (!tiles) return;
+    // W[j][l] of the FP32 value, row j by thread j
+    auto wf = [&](int j, int l) -> double { return (double)(float)src[D + l * (l + 1) / 2 + j]; };
+    if (r == 0) {
+        // F >= ||W||_F^2 (rounded up), summed in a fixed order
+        __shared__ double part[kThreads];
+        double f = 0.0;
+        for (int j = threadIdx.x; j < D; j += blockDim.x)
+            for (int l = j; l < D; ++l) f += wf(j, l) * wf(j, l);
+        part[threadIdx.x] = f;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double F = 0.0;
+            for (int t = 0; t < (int)blockDim.x; ++t) F += part[t];
+            reinterpret_cast<float*>(mslot_base + mu_off)[DPM + 1] = (float)(F * (1.0 + 0x1p-20));
+        }
+    } else {
+        // the screen relative to the parent cluster's mean (draws of part 3k): -c and g
+        const double* par = draws + (size_t)(3 * k) * draw_stride;
+        float* rel = reinterpret_cast<float*>(mslot_base + mx2_rel_offset(DPM));
+        const double kappa = (double)sub_rel_eps_g(DPM) / (double)sub_rel_eps_a(DPM);
+        for (int j = threadIdx.x; j < D; j += blockDim.x) {
+            double c = 0.0, b = 0.0;
+            for (int l = j; l < D; ++l) {
+                const double w = wf(j, l);
+                const double dl = (double)(float)src[l] - (double)(float)par[l];
+                c += w * dl;
+                b += fabs(w) * fabs(dl);
+            }
+            rel[j] = (float)(-c);
+            rel[DPM + j] = (float)(kappa * (b + fabs(c)) * (1.0 + 0x1p-20) + 0x1p-120);
+        }
+    }
 }
 
 // Multinomial slots: [log theta (D) | log weight | pad] in FP32; clusters carry log pi_k only when

@@ -215,6 +215,15 @@ int dpmm_set_statistics_refresh(dpmm_ctx* ctx, int every);
 /* The last statistics pass: 0 a full recomputation, 1 an update of every cluster from the points whose
  * sub-label moved, 2 such an update with the clusters a moved label touched recomputed.  Synchronises. */
 int dpmm_last_statistics_update(dpmm_ctx* ctx, int* moved_only);
+/* Sub-label draw fused into the pruned mixed label sweep (on by default, padded D <= 64; no counterpart
+ * in the reference, whose sub-label sweep is a pass of its own, cudaKernel.cu:1299-1398).  A point
+ * whose label stays the cluster it was filed under is screened against that cluster's two
+ * sub-clusters in the same pass over its coordinates, with the split-BF16 screen and error bound of
+ * the separate sweep; points whose label moved and undecided ones are drawn in FP32 afterwards by
+ * their final labels.  Sub-labels are identical with fusion on or off (same Philox draws); the next
+ * dpmm_sample_sub_labels of the same iteration only orders them for the statistics pass.
+ * enable 0: the separate sub-label sweep. */
+int dpmm_set_fused_sub_labels(dpmm_ctx* ctx, int enable);
 /* Fraction of (wave, cluster) scorings the last pruned label sweep performed (-1 if the last
  * sweep was not pruned). */
 int dpmm_last_label_evaluations(dpmm_ctx* ctx, double* fraction);

@@ -348,6 +348,8 @@ def main():
     ap.add_argument("--paths", default="mixed,fp32,fp64")
     ap.add_argument("--n", type=int, default=0, help="override the configuration's N (a reduced run, not its bench line)")
     ap.add_argument("--iters", type=int, default=0, help="override the screener workload's iteration count")
+    ap.add_argument("--sweep-workgroups", type=int, default=0,
+                    help="workgroups per partition chunk of the pruned label sweep (0: the library default)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="copy X once (N*D*4 bytes read + written) before the run: a plain copy kernel of known "
                          "size that calibrates the HBM byte counters of a profiler")
@@ -400,6 +402,8 @@ def main():
     # the schedule's final / no-more-splits iterations lie beyond every iteration run here
     iters = max_burn + args.warmup + args.steps + 20
     dev.set_graph_mode(args.graph)
+    if args.sweep_workgroups:
+        dev.set_sweep_workgroups(args.sweep_workgroups)
     smp = dp.Sampler(dev, hyper, cfg["alpha"], iters, seed=args.seed + 1, precision=prec,
                      host_decisions=args.host_decisions)
     decisions = "host" if args.host_decisions else "device"

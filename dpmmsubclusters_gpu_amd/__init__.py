@@ -103,6 +103,8 @@ def load_library():
         "dpmm_last_sub_recheck_count": [_P, _P],
         "dpmm_set_label_pruning": [_P, i],
         "dpmm_set_fused_sub_labels": [_P, i],
+        "dpmm_set_sweep_workgroups": [_P, i],
+        "dpmm_set_label_refinement": [_P, i],
         "dpmm_set_statistics_refresh": [_P, i],
         "dpmm_last_statistics_update": [_P, _P],
         "dpmm_last_label_evaluations": [_P, _P],
@@ -401,6 +403,14 @@ class DeviceSweep:
     def set_fused_sub_labels(self, enable):
         """Sub-label draw fused into the pruned mixed label sweep (default on); False: a separate sweep."""
         _check(load_library().dpmm_set_fused_sub_labels(self._ctx, 1 if enable else 0))
+
+    def set_label_refinement(self, enable):
+        """Split-BF16 refinement of the pruned sweep's undecided waves (default on)."""
+        _check(load_library().dpmm_set_label_refinement(self._ctx, 1 if enable else 0))
+
+    def set_sweep_workgroups(self, per_chunk):
+        """Workgroups per partition chunk of the pruned mixed label sweep (0: the default)."""
+        _check(load_library().dpmm_set_sweep_workgroups(self._ctx, int(per_chunk)))
 
     def set_statistics_refresh(self, every):
         """Statistics passes per full recomputation; the passes between update the statistics from the

@@ -975,6 +975,11 @@ struct dpmm_ctx {
     size_t pbCap = 0;
     std::vector<unsigned char> hPB;
     void* dSPB = nullptr;   // the 2K sub-clusters' BF16 slots
+    // lo tiles of the K clusters' W (W - bf16(W) in BF16, the slot's tile order): the pruned sweep's
+    // split-precision refinement of the points its BF16 screen leaves undecided
+    void* dPBlo = nullptr;
+    size_t pbloCap = 0;
+    std::vector<unsigned char> hPBlo;
     size_t spbCap = 0;
     std::vector<unsigned char> hSPB;
     bool spb_valid = false;
@@ -983,6 +988,9 @@ struct dpmm_ctx {
     // sub-label screen fused into the pruned label sweep (DPM <= 64): the sub-labels it leaves in the
     // order of the partition it swept, and its list of positions left to the FP32 draw
     int fuse_mode = 1;            // 0: separate sub-label sweep
+    int prune_split_fused = 2;    // workgroups per partition chunk of the fused sweep
+    int prune_split = 0;          // dpmm_set_sweep_workgroups (0: the defaults)
+    int refine_mode = 1;          // split-BF16 refinement of undecided waves in the pruned sweep
     bool fused_subs = false;      // the last label sweep drew the sub-labels as well, for iteration fused_iter
     uint32_t fused_iter = 0;
     int32_t* dSubsOld = nullptr;
@@ -1308,11 +1316,22 @@ uint16_t f32_to_bf16_rne(float f) {
 }
 
 // BF16 screen slot of one cluster: W = R^T tiles in MFMA lane order, mu, score constant.
-void pack_mx_slot(int D, int DPM, const double* mu, const double* R, double cst, unsigned char* dst) {
+void pack_mx_slot(int D, int DPM, const double* mu, const double* R, double cst, unsigned char* dst,
+                  unsigned char* lo_dst) {
     std::memset(dst, 0, mx_slot_bytes(DPM));
+    std::memset(lo_dst, 0, mx_tiles_bytes(DPM));
     uint16_t* tiles = reinterpret_cast<uint16_t*>(dst);
+    uint16_t* lo = reinterpret_cast<uint16_t*>(lo_dst);
     for (int j = 0; j < D; ++j)
-        for (int l = j; l < D; ++l) tiles[mx_tile_index(DPM, j, l)] = f32_to_bf16_rne((float)R[(size_t)l * D + j]);
+        for (int l = j; l < D; ++l) {
+            const float w = (float)R[(size_t)l * D + j];
+            const uint16_t hb = f32_to_bf16_rne(w);
+            uint32_t hu = (uint32_t)hb << 16;
+            float hf;
+            std::memcpy(&hf, &hu, 4);
+            tiles[mx_tile_index(DPM, j, l)] = hb;
+            lo[mx_tile_index(DPM, j, l)] = f32_to_bf16_rne(w - hf);
+        }
     float* m = reinterpret_cast<float*>(dst + mx_tiles_bytes(DPM));
     for (int l = 0; l < D; ++l) m[l] = (float)mu[l];
     m[DPM] = (float)cst;
@@ -1350,6 +1369,10 @@ int ensure_mixed_buffers(dpmm_ctx* c, int K) {
     unsigned char* spb = reinterpret_cast<unsigned char*>(c->dSPB);
     rc = grow(spb, c->spbCap, (size_t)2 * K * mx2_slot_bytes(c->DPM));
     c->dSPB = spb;
+    if (rc) return rc;
+    unsigned char* plo = reinterpret_cast<unsigned char*>(c->dPBlo);
+    rc = grow(plo, c->pbloCap, (size_t)K * mx_tiles_bytes(c->DPM));
+    c->dPBlo = plo;
     if (rc) return rc;
     return ensure_recheck(c);
 }
@@ -1447,6 +1470,7 @@ int set_gauss(dpmm_ctx* c, int K, const double* mu, const double* mat, const dou
     c->fused_subs = false;
     if (mixed) {
         c->hPB.assign((size_t)K * mslot, 0);
+        c->hPBlo.assign((size_t)K * mx_tiles_bytes(c->DPM), 0);
         c->hSPB.assign((size_t)2 * K * mx2_slot_bytes(c->DPM), 0);
     }
     for (int k = 0; k < K; ++k) {
@@ -1456,12 +1480,13 @@ int set_gauss(dpmm_ctx* c, int K, const double* mu, const double* mat, const dou
             return fail(DPMM_ENOTSPD, "cluster precision matrix not SPD");
         if (mixed)
             pack_mx_slot(D, c->DPM, mu + (size_t)k * D, R.data(), -0.5 * logdet[k] + std::log(weights[k]),
-                         c->hPB.data() + (size_t)k * mslot);
+                         c->hPB.data() + (size_t)k * mslot, c->hPBlo.data() + (size_t)k * mx_tiles_bytes(c->DPM));
     }
     if (mixed) {
         int rc2 = ensure_mixed_buffers(c, K);
         if (rc2) return rc2;
         HIPCHK(hipMemcpyAsync(c->dPB, c->hPB.data(), c->hPB.size(), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(c->dPBlo, c->hPBlo.data(), c->hPBlo.size(), hipMemcpyHostToDevice, c->stream));
     }
     for (int q = 0; q < 2 * K; ++q) {
         if (!one(smu + (size_t)q * D, smat + (size_t)q * D * D, slogdet[q], std::log(lrw[q]),
@@ -1548,14 +1573,14 @@ template <int DP> void lglp_f32(dpmm_ctx* c, uint32_t it, int f, int grid) {
 }
 
 template <int DP> void sgl_f32(dpmm_ctx* c, uint32_t it, int grid) {
-    launch_sub_gauss_list<float, DP>(c->stream, grid, reinterpret_cast<const float*>(c->dX), c->D,
+    launch_sub_gauss_list<float, DP>(c->stream, grid, reinterpret_cast<const float*>(c->dX), c->D, c->K,
                                      reinterpret_cast<const float*>(c->dSP), c->dLabels, c->dRecheck,
                                      c->dRecheckCount + 1, c->key0(), c->key1(), it, c->dSubs, c->dPerm,
                                      c->dSubsPerm);
 }
 // the list of the fused sweep: positions in the partition it swept, sub-labels into dSubsOld
 template <int DP> void sglf_f32(dpmm_ctx* c, uint32_t it, int grid) {
-    launch_sub_gauss_list<float, DP>(c->stream, grid, reinterpret_cast<const float*>(c->dX), c->D,
+    launch_sub_gauss_list<float, DP>(c->stream, grid, reinterpret_cast<const float*>(c->dX), c->D, c->K,
                                      reinterpret_cast<const float*>(c->dSP), c->dLabels, c->dSubList,
                                      c->dRecheckCount + 1, c->key0(), c->key1(), it, c->dSubs, c->dPerm,
                                      c->dSubsOld);
@@ -1659,11 +1684,13 @@ int run_label_mixed_pruned(dpmm_ctx* c, uint32_t iter, int fin) {
     hipLaunchKernelGGL(k_zero_int, dim3((c->K + 64) / 64), dim3(64), 0, c->stream, c->dFlags, c->K + 1);
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, c->dRecheckCount, 2);
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, reinterpret_cast<int*>(c->dEvals), 4);
-    const int grid = chunks_upper(c, c->K) * kPruneSplit;
     const float* X = reinterpret_cast<const float*>(c->dX);
     const unsigned char* PB = reinterpret_cast<const unsigned char*>(c->dPB);
-    SubFuse sf{};
+    MxpExtra sf{};
     sf.skip_same = reuse ? 1 : 0;
+    sf.split = c->prune_split > 0 ? c->prune_split : fuse ? c->prune_split_fused : kPruneSplit;
+    sf.PBlo = c->refine_mode && c->spb_valid ? reinterpret_cast<const unsigned char*>(c->dPBlo) : nullptr;
+    const int grid = chunks_upper(c, c->K) * sf.split;
     if (fuse) {
         sf.SPB = reinterpret_cast<const unsigned char*>(c->dSPB);
         // the sub-labels of this partition, when the last pass left them in its order
@@ -1789,7 +1816,7 @@ int dpmm_destroy(dpmm_ctx* c) {
                     (void*)c->dStepW, (void*)c->dStepMem, (void*)c->dStepCand, (void*)c->dStepCounts,
                     (void*)c->dStepTab, (void*)c->dStepJobs, (void*)c->dStepAcc, (void*)c->dStepSum,
                     (void*)c->dFlags, (void*)c->dSubsPermPrev, (void*)c->dPrevStart, (void*)c->dSubsOld,
-                    (void*)c->dSubList})
+                    (void*)c->dSubList, c->dPBlo})
         if (p) (void)hipFree(p);
     if (c->hStepSum) (void)hipHostFree(c->hStepSum);
     if (c->hStats) (void)hipHostFree(c->hStats);
@@ -2597,6 +2624,18 @@ int dpmm_set_label_pruning(dpmm_ctx* c, int mode) {
     return DPMM_OK;
 }
 
+int dpmm_set_label_refinement(dpmm_ctx* c, int enable) {
+    if (!c) return fail(DPMM_EINVAL, "null ctx");
+    c->refine_mode = enable ? 1 : 0;
+    return DPMM_OK;
+}
+
+int dpmm_set_sweep_workgroups(dpmm_ctx* c, int per_chunk) {
+    if (!c || per_chunk < 0 || per_chunk > 64) return fail(DPMM_EINVAL, "workgroups per chunk is 0 .. 64");
+    c->prune_split = per_chunk;
+    return DPMM_OK;
+}
+
 int dpmm_set_fused_sub_labels(dpmm_ctx* c, int enable) {
     if (!c) return fail(DPMM_EINVAL, "null ctx");
     c->fuse_mode = enable ? 1 : 0;
@@ -2837,22 +2876,24 @@ static int enqueue_draws(dpmm_ctx* c, uint32_t iter, int K, const int32_t* dmem,
         if ((rc = ensure_param_buffers(c, K, (size_t)stride * esz))) return rc;
         unsigned char* pb = nullptr;
         unsigned char* spb = nullptr;
+        unsigned char* plo = nullptr;
         c->spb_valid = false;
         c->fused_subs = false;
         if (c->precision == DPMM_PREC_MIXED) {
             if ((rc = ensure_mixed_buffers(c, K))) return rc;
             pb = reinterpret_cast<unsigned char*>(c->dPB);
             spb = reinterpret_cast<unsigned char*>(c->dSPB);
+            plo = reinterpret_cast<unsigned char*>(c->dPBlo);
             c->spb_valid = true;
         }
         if (f64)
             hipLaunchKernelGGL(k_pack_gauss<double>, dim3(np), dim3(kThreads), 0, c->stream, c->dDraws, dstride, D,
                                c->DP, c->DPM, c->dLogW, K, reinterpret_cast<double*>(c->dP),
-                               reinterpret_cast<double*>(c->dSP), pb, spb);
+                               reinterpret_cast<double*>(c->dSP), pb, spb, plo);
         else
             hipLaunchKernelGGL(k_pack_gauss<float>, dim3(np), dim3(kThreads), 0, c->stream, c->dDraws, dstride, D,
                                c->DP, c->DPM, c->dLogW, K, reinterpret_cast<float*>(c->dP),
-                               reinterpret_cast<float*>(c->dSP), pb, spb);
+                               reinterpret_cast<float*>(c->dSP), pb, spb, plo);
     } else {
         const int stride = (D + 1 + 3) & ~3;
         c->params_f64 = false;

@@ -569,7 +569,7 @@ __global__ void __launch_bounds__(kThreads, sizeof(TX) == 4 ? 3 : 2) k_stats_gau
 // the arithmetic of k_sub_gauss: each point's two sub-cluster slots (of its label) are read from
 // global memory; workgroups stride over the list, whose length is read from device memory.
 template <typename T, int DP>
-__global__ void __launch_bounds__(kThreads) k_sub_gauss_list(const T* __restrict__ X, int D,
+__global__ void __launch_bounds__(kThreads) k_sub_gauss_list(const T* __restrict__ X, int D, int K,
                                                              const T* __restrict__ SP,
                                                              const int32_t* __restrict__ labels,
                                                              const int32_t* __restrict__ list,
@@ -591,12 +591,14 @@ __global__ void __launch_bounds__(kThreads) k_sub_gauss_list(const T* __restrict
         // with perm, list entries are positions in the partition (the sub-label is also written there)
         const int ent = valid ? list[pos] : 0;
         const int idx = perm ? perm[ent] : ent;
-        const int k = valid ? labels[idx] : 0;
+        const int k0 = valid ? labels[idx] : 0;
+        const bool ok = valid && (unsigned)k0 < (unsigned)K;   // labels outside [0, K) are left alone
+        const int k = ok ? k0 : 0;
         T x[kCols];
 #pragma unroll
         for (int c = 0; c < kCols; ++c) {
             const int l = kPair ? pair_col(c, 0) + (par << 2) : c;
-            x[c] = (valid && l < D) ? X[(int64_t)idx * D + l] : T(0);
+            x[c] = (ok && l < D) ? X[(int64_t)idx * D + l] : T(0);
         }
         T sc[2];
 #pragma unroll 1
@@ -607,7 +609,7 @@ __global__ void __launch_bounds__(kThreads) k_sub_gauss_list(const T* __restrict
             else q = whitened_sq_regs<T, DP>(pp, x);
             sc[h] = pp[gauss_c_offset(DP)] - T(0.5) * q;
         }
-        if (valid && par == 0) {
+        if (ok && par == 0) {
             const uint32_t w = philox4x32_10((uint32_t)idx, 0, iter, TAG_SUB, key0, key1).v[0];
             const int v = (logit_u32<T>(w) < sc[0] - sc[1]) ? 0 : 1;
             subs[idx] = v;
@@ -784,11 +786,11 @@ void launch_label_gauss_listp(hipStream_t st, int grid, const T* X, int64_t N, i
 }
 
 template <typename T, int DP>
-void launch_sub_gauss_list(hipStream_t st, int grid, const T* X, int D, const T* SP, const int32_t* labels,
+void launch_sub_gauss_list(hipStream_t st, int grid, const T* X, int D, int K, const T* SP, const int32_t* labels,
                            const int32_t* list, const int* count, uint32_t key0, uint32_t key1, uint32_t iter,
                            int32_t* subs, const int32_t* perm, int32_t* subs_perm) {
-    hipLaunchKernelGGL((k_sub_gauss_list<T, DP>), dim3(grid), dim3(kThreads), 0, st, X, D, SP, labels, list, count,
-                       key0, key1, iter, subs, perm, subs_perm);
+    hipLaunchKernelGGL((k_sub_gauss_list<T, DP>), dim3(grid), dim3(kThreads), 0, st, X, D, K, SP, labels, list,
+                       count, key0, key1, iter, subs, perm, subs_perm);
 }
 
 template <typename T, int DP>
@@ -835,7 +837,7 @@ void launch_stats_gauss(hipStream_t st, int nchunks, const TX* X, int D, int K, 
     template void launch_stats_gauss<T, DP>(hipStream_t, int, const T*, int, int, int, const int32_t*,         \
                                             const int32_t*, const int32_t*, const int32_t*, const int32_t*,    \
                                             double*, const int32_t*, StatsGate);                               \
-    template void launch_sub_gauss_list<T, DP>(hipStream_t, int, const T*, int, const T*, const int32_t*,      \
+    template void launch_sub_gauss_list<T, DP>(hipStream_t, int, const T*, int, int, const T*, const int32_t*, \
                                                const int32_t*, const int*, uint32_t, uint32_t, uint32_t,       \
                                                int32_t*, const int32_t*, int32_t*);                            \
     template void launch_score_gauss<T, DP>(hipStream_t, int, const T*, int, int, const T*, const T*,          \
@@ -856,7 +858,7 @@ void launch_stats_gauss(hipStream_t st, int nchunks, const TX* X, int D, int K, 
     extern template void launch_stats_gauss<T, DP>(hipStream_t, int, const T*, int, int, int, const int32_t*,  \
                                                    const int32_t*, const int32_t*, const int32_t*,             \
                                                    const int32_t*, double*, const int32_t*, StatsGate);        \
-    extern template void launch_sub_gauss_list<T, DP>(hipStream_t, int, const T*, int, const T*,               \
+    extern template void launch_sub_gauss_list<T, DP>(hipStream_t, int, const T*, int, int, const T*,          \
                                                       const int32_t*, const int32_t*, const int*, uint32_t,    \
                                                       uint32_t, uint32_t, int32_t*, const int32_t*, int32_t*); \
     extern template void launch_score_gauss<T, DP>(hipStream_t, int, const T*, int, int, const T*, const T*,   \

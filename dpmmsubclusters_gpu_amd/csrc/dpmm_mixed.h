@@ -253,77 +253,89 @@ void launch_label_gauss_mx(hipStream_t st, int64_t N, int D, int K, const float*
                        iter, fin, labels, recheck, recheck_count);
 }
 
-// Split-BF16 screen of the two sub-clusters whose slots start at sp (mx2 slots, one after the other)
-// for the 32 points of a wave (lane L: point L % 32, coordinates 16 kc + 8 (L / 32) .. + 7 in x):
-// returns s_0 - s_1 of the lane's point and sets err, the bound on its distance from the FP32 sweep's
-// S_0 - S_1 (see k_sub_gauss_mx).  Every lane of the wave calls it.
+// Split-BF16 score of one cluster for the 32 points of a wave (lane L: point L % 32, coordinates
+// 16 kc + 8 (L / 32) .. + 7 in x): with d = x - mu in FP32 split as d_hi + d_lo and W = W_hi + W_lo,
+// y = W_hi d_hi + W_hi d_lo + W_lo d_hi on MFMA (FP32 accumulation), and a fourth MFMA gives
+// a_j = sum_l |W_hi,jl| |d_hi,l|.  Each y_j is then within eps a_j of the FP32 sweep's (eps covers the
+// split residuals and both FP32 accumulations), so q is within sum_j (2 |y_j| + eps a_j) eps a_j of the
+// FP32 sweep's q.  Returns s = c - q / 2 and sets dq, the bound on |q - q_fp32| (both sums of squares
+// included).  Every lane of the wave calls it.
 template <int DP>
-__device__ __forceinline__ float sub_screen(const float (&x)[DP / 16][8], const unsigned char* sp, int lane,
-                                            float& err) {
+__device__ __forceinline__ float split_score(const float (&x)[DP / 16][8], const bf16x8* thi, const bf16x8* tlo,
+                                             const float* mu, float c, int lane, float& dq) {
     typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+    typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
     constexpr int RB = DP / 32, KC = DP / 16;
-    constexpr int SLOT = mx2_slot_bytes(DP), TB = mx_tiles_bytes(DP);
     const int h = lane >> 5;
     // |y_screen - y_fp32| <= eps a_j: split residuals (3 x 2^-18 of |W||d|) and the FP32 sums of the
     // screen (4 D terms) and of the sweep (D terms), with a_j from BF16 magnitudes inflated by 2^-6
     const float eps = (3.0f * 0x1p-18f + (5.0f * (float)DP + 16.0f) * 0x1p-24f) * (1.0f + 0x1p-6f);
     const float heps = 0.5f * eps;
+    // d = x - mu split as d_hi + d_lo: one packed conversion per coordinate pair gives both BF16 highs,
+    // read back as FP32 by shifting / masking the packed word
+    u32x4 dh[KC], dl[KC];
+#pragma unroll
+    for (int kc = 0; kc < KC; ++kc)
+#pragma unroll
+        for (int e = 0; e < 8; e += 2) {
+            const float d0 = x[kc][e] - mu[16 * kc + 8 * h + e];
+            const float d1 = x[kc][e + 1] - mu[16 * kc + 8 * h + e + 1];
+            bf16x2 hp;
+            hp[0] = (__bf16)d0;
+            hp[1] = (__bf16)d1;
+            const uint32_t hb = __builtin_bit_cast(uint32_t, hp);
+            bf16x2 lp;
+            lp[0] = (__bf16)(d0 - __uint_as_float(hb << 16));
+            lp[1] = (__bf16)(d1 - __uint_as_float(hb & 0xffff0000u));
+            dh[kc][e / 2] = hb;
+            dl[kc][e / 2] = __builtin_bit_cast(uint32_t, lp);
+        }
+    // qq = sum y_j^2; ee = sum a_j (|y_j| + eps a_j / 2), so that the bound
+    // sum (2 |y_j| + eps a_j) eps a_j is 2 eps ee
+    float qq = 0.f, ee = 0.f;
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) {
+        f32x16 y = {}, a = {};
+#pragma unroll
+        for (int kc = 2 * rb; kc < KC; ++kc) {
+            const bf16x8 wh = thi[(rb * KC + kc) * 64 + lane];
+            const bf16x8 wl = tlo[(rb * KC + kc) * 64 + lane];
+            const bf16x8 bh = __builtin_bit_cast(bf16x8, dh[kc]), bl = __builtin_bit_cast(bf16x8, dl[kc]);
+            const u32x4 wabs = __builtin_bit_cast(u32x4, wh) & 0x7fff7fffu;
+            const u32x4 dabs = dh[kc] & 0x7fff7fffu;
+            y = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wh, bh, y, 0, 0, 0);
+            y = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wh, bl, y, 0, 0, 0);
+            y = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wl, bh, y, 0, 0, 0);
+            a = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, wabs),
+                                                        __builtin_bit_cast(bf16x8, dabs), a, 0, 0, 0);
+        }
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            qq = fmaf(y[v], y[v], qq);
+            ee = fmaf(a[v], fmaf(heps, a[v], fabsf(y[v])), ee);
+        }
+    }
+    qq += __shfl_xor(qq, 32);
+    ee = 2.0f * eps * (ee + __shfl_xor(ee, 32));
+    dq = ee + (2.0f * (float)DP + 4.0f) * 0x1p-24f * qq;   // the two sums of squares
+    return c - 0.5f * qq;
+}
+
+// Split-BF16 screen of the two sub-clusters whose slots start at sp (mx2 slots, one after the other):
+// returns s_0 - s_1 of the lane's point and sets err, the bound on its distance from the FP32 sweep's
+// S_0 - S_1 (see k_sub_gauss_mx).  Every lane of the wave calls it.
+template <int DP>
+__device__ __forceinline__ float sub_screen(const float (&x)[DP / 16][8], const unsigned char* sp, int lane,
+                                            float& err) {
+    constexpr int SLOT = mx2_slot_bytes(DP), TB = mx_tiles_bytes(DP);
     float s0 = 0.f, s1 = 0.f, dq0 = 0.f, dq1 = 0.f;
 #pragma unroll 1
     for (int hh = 0; hh < 2; ++hh) {
         const unsigned char* slot = sp + hh * SLOT;
-        const bf16x8* thi = reinterpret_cast<const bf16x8*>(slot);
-        const bf16x8* tlo = reinterpret_cast<const bf16x8*>(slot + TB);
         const float* mu = reinterpret_cast<const float*>(slot + 2 * TB);
-        // d = x - mu split as d_hi + d_lo: one packed conversion per coordinate pair gives both
-        // BF16 highs, read back as FP32 by shifting / masking the packed word
-        u32x4 dh[KC], dl[KC];
-#pragma unroll
-        for (int kc = 0; kc < KC; ++kc)
-#pragma unroll
-            for (int e = 0; e < 8; e += 2) {
-                typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
-                const float d0 = x[kc][e] - mu[16 * kc + 8 * h + e];
-                const float d1 = x[kc][e + 1] - mu[16 * kc + 8 * h + e + 1];
-                bf16x2 hp;
-                hp[0] = (__bf16)d0;
-                hp[1] = (__bf16)d1;
-                const uint32_t hb = __builtin_bit_cast(uint32_t, hp);
-                bf16x2 lp;
-                lp[0] = (__bf16)(d0 - __uint_as_float(hb << 16));
-                lp[1] = (__bf16)(d1 - __uint_as_float(hb & 0xffff0000u));
-                dh[kc][e / 2] = hb;
-                dl[kc][e / 2] = __builtin_bit_cast(uint32_t, lp);
-            }
-        // qq = sum y_j^2; ee = sum a_j (|y_j| + eps a_j / 2), so that the bound
-        // sum (2 |y_j| + eps a_j) eps a_j is 2 eps ee
-        float qq = 0.f, ee = 0.f;
-#pragma unroll
-        for (int rb = 0; rb < RB; ++rb) {
-            f32x16 y = {}, a = {};
-#pragma unroll
-            for (int kc = 2 * rb; kc < KC; ++kc) {
-                const bf16x8 wh = thi[(rb * KC + kc) * 64 + lane];
-                const bf16x8 wl = tlo[(rb * KC + kc) * 64 + lane];
-                const bf16x8 bh = __builtin_bit_cast(bf16x8, dh[kc]), bl = __builtin_bit_cast(bf16x8, dl[kc]);
-                const u32x4 wabs = __builtin_bit_cast(u32x4, wh) & 0x7fff7fffu;
-                const u32x4 dabs = dh[kc] & 0x7fff7fffu;
-                y = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wh, bh, y, 0, 0, 0);
-                y = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wh, bl, y, 0, 0, 0);
-                y = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wl, bh, y, 0, 0, 0);
-                a = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, wabs),
-                                                            __builtin_bit_cast(bf16x8, dabs), a, 0, 0, 0);
-            }
-#pragma unroll
-            for (int v = 0; v < 16; ++v) {
-                qq = fmaf(y[v], y[v], qq);
-                ee = fmaf(a[v], fmaf(heps, a[v], fabsf(y[v])), ee);
-            }
-        }
-        qq += __shfl_xor(qq, 32);
-        ee = 2.0f * eps * (ee + __shfl_xor(ee, 32));
-        const float sv = mu[DP] - 0.5f * qq;
-        const float dv = ee + (2.0f * (float)DP + 4.0f) * 0x1p-24f * qq;   // the two sums of squares
+        float dv;
+        const float sv = split_score<DP>(x, reinterpret_cast<const bf16x8*>(slot),
+                                         reinterpret_cast<const bf16x8*>(slot + TB), mu, mu[DP], lane, dv);
         if (hh == 0) { s0 = sv; dq0 = dv; } else { s1 = sv; dq1 = dv; }
     }
     err = 0.5f * (dq0 + dq1) * (1.0f + 0x1p-10f) + 0x1p-21f * (fabsf(s0) + fabsf(s1)) + 1e-6f;
@@ -331,13 +343,9 @@ __device__ __forceinline__ float sub_screen(const float (&x)[DP / 16][8], const 
 }
 
 // Sub-label screen of one chunk (the points of cluster k): both sub-clusters' split-BF16 slots in
-// LDS, 32 gathered points per wave.  With d = x - mu_h in FP32 split as d_hi + d_lo and W = W_hi + W_lo,
-// y = W_hi d_hi + W_hi d_lo + W_lo d_hi on MFMA (FP32 accumulation), and a fourth MFMA gives
-// a_j = sum_l |W_hi,jl| |d_hi,l|.  Each y_j is then within eps a_j of the FP32 sweep's (eps covers the
-// split residuals and both FP32 accumulations), so q_h is within sum_j (2 |y_j| + eps a_j) eps a_j of
-// the FP32 sweep's q_h.  The exact draw sets sub-label 0 when logit(u) < S_0 - S_1 (k_sub_gauss); a
-// point whose threshold is farther than that bound from s_0 - s_1 gets its sub-label here, the others
-// go to the recheck list.
+// LDS, 32 gathered points per wave, scored by split_score.  The exact draw sets sub-label 0 when
+// logit(u) < S_0 - S_1 (k_sub_gauss); a point whose threshold is farther than the bound from s_0 - s_1
+// gets its sub-label here, the others go to the recheck list.
 template <int DP>
 __global__ void __launch_bounds__(kThreads) k_sub_gauss_mx(const float* __restrict__ X, int D, int K, int CH,
                                                            const int32_t* __restrict__ perm,
@@ -426,17 +434,16 @@ void launch_sub_gauss_mx(hipStream_t st, int grid, const float* X, int D, int K,
 //     of the sweep's residual (|x - mu_h| <= |p| + |mu_h - mu_a|) give eps_g; g_j already carries
 //     eps_g / eps_a, so e_j <= eps_a a'_j and the bound on q_h follows as in sub_screen.
 // Returns q_a; sets d2 = ||p||^2 (FP32), Aa, and sd = s_0 - s_1 with its bound sderr.
+typedef unsigned int mx_u32x4 __attribute__((ext_vector_type(4)));
+// The split p = p_hi + p_lo (BF16 pairs packed in 32-bit words) and dd = sum p^2 of this lane's coordinates.
 template <int DP>
-__device__ __forceinline__ float fused_screens(const float (&x)[DP / 16][8], const unsigned char* sa,
-                                               const unsigned char* ssub, int lane, float& d2, float& Aa,
-                                               float& sd, float& sderr) {
-    typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void split_residual(const float (&x)[DP / 16][8], const unsigned char* sa, int lane,
+                                               mx_u32x4 (&ph)[DP / 16], mx_u32x4 (&pl)[DP / 16], float& dd) {
     typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
-    constexpr int RB = DP / 32, KC = DP / 16, TB = mx_tiles_bytes(DP), SLOT2 = mx2_slot_bytes(DP);
+    constexpr int KC = DP / 16, TB = mx_tiles_bytes(DP);
     const int h = lane >> 5;
     const float4* mu4 = reinterpret_cast<const float4*>(sa + TB);
-    u32x4 ph[KC], pl[KC];
-    float dd = 0.f;
+    dd = 0.f;
 #pragma unroll
     for (int kc = 0; kc < KC; ++kc) {
         const float4 ma = mu4[4 * kc + 2 * h], mb = mu4[4 * kc + 2 * h + 1];
@@ -458,6 +465,15 @@ __device__ __forceinline__ float fused_screens(const float (&x)[DP / 16][8], con
             pl[kc][e / 2] = __builtin_bit_cast(uint32_t, lp);
         }
     }
+}
+
+template <int DP>
+__device__ __forceinline__ float fused_screens_split(const mx_u32x4 (&ph)[DP / 16], const mx_u32x4 (&pl)[DP / 16],
+                                                     float dd, const unsigned char* sa, const unsigned char* ssub,
+                                                     int lane, float& d2, float& Aa, float& sd, float& sderr) {
+    typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+    constexpr int RB = DP / 32, KC = DP / 16, TB = mx_tiles_bytes(DP), SLOT2 = mx2_slot_bytes(DP);
+    const int h = lane >> 5;
     // anchor
     const bf16x8* ta = reinterpret_cast<const bf16x8*>(sa);
     float q = 0.f;
@@ -525,13 +541,80 @@ __device__ __forceinline__ float fused_screens(const float (&x)[DP / 16][8], con
     return q + __shfl_xor(q, 32);
 }
 
+template <int DP>
+__device__ __forceinline__ float fused_screens(const float (&x)[DP / 16][8], const unsigned char* sa,
+                                               const unsigned char* ssub, int lane, float& d2, float& Aa,
+                                               float& sd, float& sderr) {
+    mx_u32x4 ph[DP / 16], pl[DP / 16];
+    float dd;
+    split_residual<DP>(x, sa, lane, ph, pl, dd);
+    return fused_screens_split<DP>(ph, pl, dd, sa, ssub, lane, d2, Aa, sd, sderr);
+}
+
+// split_score with each 8-coordinate chunk of the point read where it is used (from cache, row: the
+// point's row, ok: a real point) and split on the fly, so that neither the coordinates nor their
+// split stay in registers: the same arithmetic in the same order, for the rare refinement path.
+template <int DP>
+__device__ __forceinline__ float split_score_lean(gfloat* row, int D, bool ok, const bf16x8* thi, const bf16x8* tlo,
+                                                  const float* mu, float c, int lane, float& dq) {
+    typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+    typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+    constexpr int RB = DP / 32, KC = DP / 16;
+    const int h = lane >> 5;
+    const float eps = (3.0f * 0x1p-18f + (5.0f * (float)DP + 16.0f) * 0x1p-24f) * (1.0f + 0x1p-6f);
+    const float heps = 0.5f * eps;
+    float qq = 0.f, ee = 0.f;
+#pragma unroll 1
+    for (int rb = 0; rb < RB; ++rb) {
+        f32x16 y = {}, a = {};
+#pragma unroll 1
+        for (int kc = 2 * rb; kc < KC; ++kc) {
+            float xv[8];
+            load_coords8_aligned(row, 16 * kc + 8 * h, D, ok, xv);
+            u32x4 dh, dl;
+#pragma unroll
+            for (int e = 0; e < 8; e += 2) {
+                const float d0 = xv[e] - mu[16 * kc + 8 * h + e];
+                const float d1 = xv[e + 1] - mu[16 * kc + 8 * h + e + 1];
+                bf16x2 hp;
+                hp[0] = (__bf16)d0;
+                hp[1] = (__bf16)d1;
+                const uint32_t hb = __builtin_bit_cast(uint32_t, hp);
+                bf16x2 lp;
+                lp[0] = (__bf16)(d0 - __uint_as_float(hb << 16));
+                lp[1] = (__bf16)(d1 - __uint_as_float(hb & 0xffff0000u));
+                dh[e / 2] = hb;
+                dl[e / 2] = __builtin_bit_cast(uint32_t, lp);
+            }
+            const bf16x8 wh = thi[(rb * KC + kc) * 64 + lane];
+            const bf16x8 wl = tlo[(rb * KC + kc) * 64 + lane];
+            const u32x4 wabs = __builtin_bit_cast(u32x4, wh) & 0x7fff7fffu;
+            y = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wh, __builtin_bit_cast(bf16x8, dh), y, 0, 0, 0);
+            y = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wh, __builtin_bit_cast(bf16x8, dl), y, 0, 0, 0);
+            y = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wl, __builtin_bit_cast(bf16x8, dh), y, 0, 0, 0);
+            a = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, wabs),
+                                                        __builtin_bit_cast(bf16x8, dh & 0x7fff7fffu), a, 0, 0, 0);
+        }
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            qq = fmaf(y[v], y[v], qq);
+            ee = fmaf(a[v], fmaf(heps, a[v], fabsf(y[v])), ee);
+        }
+    }
+    qq += __shfl_xor(qq, 32);
+    ee = 2.0f * eps * (ee + __shfl_xor(ee, 32));
+    dq = ee + (2.0f * (float)DP + 4.0f) * 0x1p-24f * qq;
+    return c - 0.5f * qq;
+}
+
 // The sub-label screen fused into the pruned sweep.  A point whose label stays its chunk's cluster
 // (the anchor) is screened against the anchor's two sub-clusters in the same pass, with the
 // arithmetic and bound of k_sub_gauss_mx; the points whose label moved or went to the recheck, and
 // those whose sub-label the screen leaves undecided, are listed (positions in the swept partition)
 // for the FP32 draw after the label recheck.  skip_same (both forms): every label of a chunk equals
-// its anchor before the sweep, so a label that stays is not written again.
-struct SubFuse {
+// its anchor before the sweep, so a label that stays is not written again.  (Extra arguments of
+// k_label_gauss_mxp.)
+struct MxpExtra {
     const unsigned char* SPB;    // 2K sub-cluster screen slots (split-BF16), null: no fused screen
     const int32_t* subs_prev;    // sub-labels before the sweep in the swept partition's order, or null
     int32_t* subs;               // sub-labels by point index, written where they change
@@ -539,9 +622,12 @@ struct SubFuse {
     int32_t* sub_list;           // positions left to the FP32 sub-label draw
     int* sub_count;
     int skip_same;
+    int split;                   // workgroups per partition chunk (the grid is chunks x split)
+    const unsigned char* PBlo;   // the clusters' lo tiles (refinement of undecided waves), or null
 };
 
-// Workgroups per partition chunk in the pruned sweep (each takes an equal share of the chunk).
+// Workgroups per partition chunk in the pruned sweep (each takes an equal share of the chunk): more
+// workgroups shorten the tail, fewer stage the chunk's slots into LDS fewer times.
 constexpr int kPruneSplit = 4;
 
 template <int DP, bool FUSE>
@@ -552,17 +638,17 @@ k_label_gauss_mxp(const float* __restrict__ X, int D, int K, int CH, const int32
                   const float* __restrict__ psc, float tau, uint32_t key0, uint32_t key1, uint32_t iter,
                   int final_iter, int32_t* __restrict__ labels, int32_t* __restrict__ recheck,
                   int* __restrict__ recheck_count, unsigned long long* __restrict__ evals,
-                  int* __restrict__ changed, SubFuse sf) {
+                  int* __restrict__ changed, MxpExtra sf) {
     static_assert(!FUSE || DP <= 64, "the fused sub-label screen keeps three slots in LDS");
     constexpr int RB = DP / 32, KC = DP / 16;
     constexpr int SLOT = mx_slot_bytes(DP);
     // one 32-point block per wave: lane L scores point L % 32 on coordinate half L / 32
     const int lane = threadIdx.x & 63, h = lane >> 5, pl = lane & 31, wv = threadIdx.x >> 6;
     int a, cbeg, cend;
-    if (!chunk_of(blockIdx.x / kPruneSplit, K, chunk_start, label_start, CH, a, cbeg, cend)) return;
-    const int part = blockIdx.x % kPruneSplit, len = cend - cbeg;
-    const int beg = cbeg + (int)((int64_t)len * part / kPruneSplit);
-    const int end = cbeg + (int)((int64_t)len * (part + 1) / kPruneSplit);
+    if (!chunk_of(blockIdx.x / sf.split, K, chunk_start, label_start, CH, a, cbeg, cend)) return;
+    const int part = blockIdx.x % sf.split, len = cend - cbeg;
+    const int beg = cbeg + (int)((int64_t)len * part / sf.split);
+    const int end = cbeg + (int)((int64_t)len * (part + 1) / sf.split);
     const float* mrow = pm + (size_t)a * K;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     // the anchor's slot, scored by every pass of the workgroup, is staged in LDS once; the other
@@ -590,6 +676,12 @@ k_label_gauss_mxp(const float* __restrict__ X, int D, int K, int CH, const int32
     WaveList rl{wlist[wv], 0};
     __shared__ int32_t wlist2[FUSE ? 4 : 1][64];
     WaveList sl{wlist2[FUSE ? wv : 0], 0};
+    // the coordinates 16 kc + 8 h .. + 7 of point `pt` (row 0 stands in for an empty lane)
+    auto load_row = [&](gfloat* Xs, int pt, bool ok, float (&x)[KC][8]) {
+        gfloat* row = Xs + (int64_t)pt * D;
+#pragma unroll
+        for (int kc = 0; kc < KC; ++kc) load_coords8_aligned(row, 16 * kc + 8 * h, D, ok, x[kc]);   // D % 4 == 0
+    };
     int idx_next = (beg + wv * 32 + pl < end) ? perm[beg + wv * 32 + pl] : 0;
     for (int p0 = beg + wv * 32; p0 < end; p0 += 4 * 32) {   // wave-uniform trip count
         const int pos = p0 + pl;
@@ -599,12 +691,7 @@ k_label_gauss_mxp(const float* __restrict__ X, int D, int K, int CH, const int32
         // branch-free load, so that waiting for it is left to its first use in the next pass)
         idx_next = perm[pos + 128 < end ? pos + 128 : end - 1];
         const bool own_valid = val && h == 0;
-        // the point's coordinates 16 kc + 8 h .. + 7 (row 0 stands in for an empty lane)
-        auto load_x = [&](gfloat* Xs, float (&x)[KC][8]) {
-            gfloat* row = Xs + (int64_t)idx * D;
-#pragma unroll
-            for (int kc = 0; kc < KC; ++kc) load_coords8_aligned(row, 16 * kc + 8 * h, D, val, x[kc]);   // D % 4 == 0
-        };
+        auto load_x = [&](gfloat* Xs, float (&x)[KC][8]) { load_row(Xs, idx, val, x); };
         // up to 32 dimensions the coordinates stay in registers for the whole pass; wider points are
         // read again by each scoring (from cache after the anchor's), which keeps the registers for
         // the MFMA operands
@@ -707,6 +794,7 @@ k_label_gauss_mxp(const float* __restrict__ X, int D, int K, int CH, const int32
             }
         }
         float eb = label_screen_err<DP>(tau, qa, ea, *ca), U = neg_inf<float>();
+        float Up = neg_inf<float>();   // the largest bound of the pruned clusters
         int bi = a;
         float rr = own_valid ? sqrtf(d2) : 0.f;
         float tt = own_valid ? sb - eb : __builtin_huge_valf();
@@ -731,6 +819,7 @@ k_label_gauss_mxp(const float* __restrict__ X, int D, int K, int CH, const int32
             float upr = keep ? neg_inf<float>() : ub;   // bound of the skipped clusters
             for (int o = 32; o > 0; o >>= 1) upr = fmaxf(upr, __shfl_xor(upr, o));
             U = fmaxf(U, upr);
+            Up = fmaxf(Up, upr);
             uint64_t todo = __ballot(keep);
             while (todo) {
                 const int k = k0 + __ffsll((unsigned long long)todo) - 1;
@@ -752,7 +841,57 @@ k_label_gauss_mxp(const float* __restrict__ X, int D, int K, int CH, const int32
         }
         nev += ne;
         ++npass;
-        const bool sure = sb - eb > U;
+        bool sure = sb - eb > U;
+        // a wave with undecided points is scored again in split BF16 (refine_wave, bounds some 2^-9 of the
+        // BF16 screen's); a point whose best refined lower bound beats every other refined upper bound
+        // and the pruned clusters' bound Up takes that label (the FP32 sweep's: its score is the strict
+        // maximum).  Siblings of a split or an over-split start leave most points within the BF16
+        // bound of a tie.
+        if (sf.PBlo && __ballot(own_valid && !sure)) {
+            gfloat* Xs = (gfloat*)X;
+            asm volatile("" : "+s"(Xs));   // coordinates read again (from cache), not kept from the screen
+            gfloat* row = Xs + (int64_t)idx * D;
+            float rs = neg_inf<float>(), re = 0.f, rU = Up;
+            int rbi = a;
+            auto refine = [&](int k, const unsigned char* slot) {
+                const float* mu = reinterpret_cast<const float*>(slot + mx_tiles_bytes(DP));
+                float dq;
+                float sk = split_score_lean<DP>(row, D, val, reinterpret_cast<const bf16x8*>(slot),
+                                                reinterpret_cast<const bf16x8*>(sf.PBlo + (size_t)k * mx_tiles_bytes(DP)),
+                                                mu, mu[DP], lane, dq);
+                const float e = 0.5f * dq * (1.0f + 0x1p-10f) + 0x1p-21f * (fabsf(sk) + 24.0f) + kNoiseSlack;
+                if (!final_iter) {
+                    const Ctr4 w = philox4x32_10((uint32_t)idx, (uint32_t)(k >> 2), iter, TAG_LABEL, key0, key1);
+                    sk += gumbel_screen(sel4(w, k & 3));
+                }
+                if (sk > rs) {
+                    rU = fmaxf(rU, rs + re);
+                    rs = sk; re = e; rbi = k;
+                } else {
+                    rU = fmaxf(rU, sk + e);
+                }
+            };
+            refine(a, sa);
+            for (int k0 = 0; k0 < K; k0 += 64) {   // the kept clusters again (the same test, in index order)
+                const int kk = k0 + lane;
+                bool keep = false;
+                if (kk < K && kk != a) {
+                    const float ck = psc[2 * kk + 1];
+                    const float lb = fmaxf(0.f, mrow[kk] - psc[2 * kk] * rad);
+                    keep = !(ck - 0.49f * lb * lb + 1e-6f * fabsf(ck) + kPruneNoise < T);
+                }
+                uint64_t todo = __ballot(keep);
+                while (todo) {
+                    const int k = k0 + __ffsll((unsigned long long)todo) - 1;
+                    todo &= todo - 1;
+                    refine(k, PB + (size_t)k * SLOT);
+                }
+            }
+            if (!sure && rs - re > rU) {
+                sure = true;
+                bi = rbi;
+            }
+        }
         // with skip_same every label in the chunk is the anchor before the sweep: a label that stays is not
         // written again
         if (own_valid && sure && !(sf.skip_same && bi == a)) labels[idx] = bi;
@@ -809,7 +948,7 @@ void launch_label_gauss_mxp(hipStream_t st, int grid, const float* X, int D, int
                             const int32_t* label_start, const int32_t* chunk_start, const unsigned char* PB,
                             const float* pm, const float* psc, float tau, uint32_t key0, uint32_t key1, uint32_t iter,
                             int fin, int32_t* labels, int32_t* recheck, int* recheck_count,
-                            unsigned long long* evals, int* changed, SubFuse sf) {
+                            unsigned long long* evals, int* changed, MxpExtra sf) {
     if constexpr (DP <= 64) {
         if (sf.SPB) {
             hipLaunchKernelGGL((k_label_gauss_mxp<DP, true>), dim3(grid), dim3(kThreads), 0, st, X, D, K, CH, perm,
@@ -832,7 +971,7 @@ void launch_label_gauss_mxp(hipStream_t st, int grid, const float* X, int D, int
     template void launch_label_gauss_mxp<DP>(hipStream_t, int, const float*, int, int, int, const int32_t*,   \
                                              const int32_t*, const int32_t*, const unsigned char*, const float*, \
                                              const float*, float, uint32_t, uint32_t, uint32_t, int, int32_t*,   \
-                                             int32_t*, int*, unsigned long long*, int*, SubFuse);
+                                             int32_t*, int*, unsigned long long*, int*, MxpExtra);
 #define DPMM_MX_EXTERN(DP)                                                                                    \
     extern template void launch_label_gauss_mx<DP>(hipStream_t, int64_t, int, int, const float*,               \
                                                    const unsigned char*, float, uint32_t, uint32_t, uint32_t,  \
@@ -845,6 +984,6 @@ void launch_label_gauss_mxp(hipStream_t st, int grid, const float* X, int D, int
                                                     const int32_t*, const int32_t*, const int32_t*,            \
                                                     const unsigned char*, const float*, const float*, float,   \
                                                     uint32_t, uint32_t, uint32_t, int, int32_t*, int32_t*, int*, \
-                                                    unsigned long long*, int*, SubFuse);
+                                                    unsigned long long*, int*, MxpExtra);
 
 }  // namespace dpmm

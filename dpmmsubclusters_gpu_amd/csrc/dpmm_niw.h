@@ -713,7 +713,8 @@ __global__ void __launch_bounds__(kThreads) k_pack_gauss(const double* __restric
                                                          int DP, int DPM, const double* __restrict__ logw, int K,
                                                          T* __restrict__ P, T* __restrict__ SP,
                                                          unsigned char* __restrict__ PB,
-                                                         unsigned char* __restrict__ SPB) {
+                                                         unsigned char* __restrict__ SPB,
+                                                         unsigned char* __restrict__ PBlo) {
     const int p = blockIdx.x, k = p / 3, r = p - 3 * k;
     const double* src = draws + (size_t)p * draw_stride;
     const int stride = gauss_slot_stride(DP);
@@ -722,8 +723,8 @@ __global__ void __launch_bounds__(kThreads) k_pack_gauss(const double* __restric
     const int TR = D * (D + 1) / 2;
     const double cst = -0.5 * src[D + TR] + lw;
     for (int e = threadIdx.x; e < stride; e += blockDim.x) dst[e] = T(0);
-    // BF16 slots: a cluster's screen slot (tiles | mu | c) or a sub-cluster's split slot
-    // (hi tiles | lo tiles | mu | c)
+    // BF16 slots: a cluster's screen slot (tiles | mu | c | F) with its lo tiles in PBlo, or a
+    // sub-cluster's split slot (hi tiles | lo tiles | mu | c | -c_rel | g_rel)
     uint16_t* tiles = nullptr;
     uint16_t* lo_tiles = nullptr;
     unsigned char* mslot_base = nullptr;
@@ -734,7 +735,10 @@ __global__ void __launch_bounds__(kThreads) k_pack_gauss(const double* __restric
     if (mslot_base) {
         tiles = reinterpret_cast<uint16_t*>(mslot_base);
         if (r > 0) lo_tiles = reinterpret_cast<uint16_t*>(mslot_base + mx_tiles_bytes(DPM));
+        else if (PBlo) lo_tiles = reinterpret_cast<uint16_t*>(PBlo + (size_t)k * mx_tiles_bytes(DPM));
         for (int e = threadIdx.x; e < mslot / 2; e += blockDim.x) tiles[e] = 0;
+        if (r == 0 && lo_tiles)
+            for (int e = threadIdx.x; e < mx_tiles_bytes(DPM) / 2; e += blockDim.x) lo_tiles[e] = 0;
     }
     __syncthreads();
     // W = R^T: W[j][l] = R[l][j] for l >= j

@@ -224,6 +224,18 @@ int dpmm_last_statistics_update(dpmm_ctx* ctx, int* moved_only);
  * dpmm_sample_sub_labels of the same iteration only orders them for the statistics pass.
  * enable 0: the separate sub-label sweep. */
 int dpmm_set_fused_sub_labels(dpmm_ctx* ctx, int enable);
+/* Split-BF16 refinement in the pruned mixed label sweep (on by default; no counterpart in the
+ * reference).  A wave whose BF16 screen leaves points undecided scores the cluster they were filed
+ * under and every cluster the prune test keeps again with W and x - mu split into BF16 highs and lows
+ * (three MFMAs) and a bound some 2^-9 of the screen's; a point takes the best cluster when its lower
+ * bound beats every other upper bound, which makes it the FP32 sweep's label.  Near-ties between
+ * sibling clusters (after a split, or an over-split start) then rarely reach the FP32 recheck.
+ * enable 0: every undecided point goes to the FP32 recheck. */
+int dpmm_set_label_refinement(dpmm_ctx* ctx, int enable);
+/* Workgroups per partition chunk of the pruned mixed label sweep (a tuning knob, no counterpart in
+ * the reference): each takes an equal share of the chunk's points and stages the chunk's cluster slots
+ * in LDS once.  0 (default): 2 with the fused sub-label draw, else 4; at most 64. */
+int dpmm_set_sweep_workgroups(dpmm_ctx* ctx, int per_chunk);
 /* Fraction of (wave, cluster) scorings the last pruned label sweep performed (-1 if the last
  * sweep was not pruned). */
 int dpmm_last_label_evaluations(dpmm_ctx* ctx, double* fraction);

@@ -295,3 +295,47 @@ def test_mixed_sublabels_equal_fp32_and_oracle(D, K, close):
     np.testing.assert_array_equal(out[dp.Precision.MIXED], out[dp.Precision.FP32])
     osub, margin = orc.sublabel_sweep(0, X, labels, smu, slam, sld, lr_weights=lrw, seed=13, it=6)
     assert not ((out[dp.Precision.MIXED] != osub) & (margin >= 1e-6)).any()
+
+
+@pytest.mark.parametrize("D,K", [(64, 6), (128, 5), (32, 8)])
+def test_refined_sibling_ties_equal_fp32(D, K):
+    """Sibling clusters (each cluster of the mixture present twice, with means 0.05 apart and the same
+    covariance), the situation of a split or an over-split start: the BF16 screen leaves most points
+    within its bound of a tie.  The pruned sweep's split-BF16 refinement must decide them as the FP32
+    sweep does (labels identical, hence the oracle's wherever its margin exceeds 1e-6), and leave far
+    fewer points to the FP32 recheck than with the refinement off."""
+    rng = np.random.default_rng(D * 11 + K)
+    N = 40_000
+    X, gt, _ = gaussian_mixture(N, D, K, seed=D + K, dtype=np.float32)
+    mu, lam, ld, w, smu, slam, sld, lrw = _fitted_params(X, gt, K, rng)
+    mu2 = np.repeat(mu, 2, axis=0)
+    mu2[1::2] += 0.05 * rng.normal(size=(K, D)) / np.sqrt(D)
+    lam2, ld2, w2 = np.repeat(lam, 2, axis=0), np.repeat(ld, 2), np.repeat(w, 2) / 2
+    K2 = 2 * K
+    params = (mu2, lam2, ld2, w2, np.repeat(mu2, 2, axis=0), np.repeat(lam2, 2, axis=0), np.repeat(ld2, 2),
+              np.full(2 * K2, 0.5))
+    init = (2 * gt + rng.integers(0, 2, N)).astype(np.int32)
+    say(f"refined sibling ties D={D} K={K2}")
+    dev = dp.DeviceSweep(X, dp.prior_type.Gaussian, seed=12)
+    dev.set_gaussian_params(*params)
+    dev.set_labels(init, None)
+    dev.sample_cluster_labels(5)
+    ref, _ = dev.get_labels()
+    counts = {}
+    for refine in (True, False):
+        dev.set_precision(dp.Precision.MIXED)
+        dev.set_label_pruning(True)
+        dev.set_label_refinement(refine)
+        dev.set_gaussian_params(*params)
+        dev.set_labels(init, None)
+        dev.sample_cluster_labels(5)
+        mx, _ = dev.get_labels()
+        np.testing.assert_array_equal(mx, ref, err_msg=f"refinement {refine}")
+        counts[refine] = dev.last_recheck_count()
+        dev.set_precision(dp.Precision.FP32)
+    print(f"D={D} K={K2}: FP32 rechecks {counts[True]} with the refinement, {counts[False]} without, of {N}")
+    assert counts[False] > N // 10, "the siblings should defeat the BF16 screen"
+    assert counts[True] * 10 < counts[False]
+    olab, margin = orc.label_sweep(0, X, params[0], params[1], params[2], weights=params[3], seed=12, it=5)
+    assert not ((ref != olab) & (margin >= 1e-6)).any()
+    dev.close()

@@ -128,6 +128,8 @@ def load_library():
         "dpmm_screener_forward": [_P, i, _P, _P],
         "dpmm_sampler_set_screener": [_P, _P, i, ctypes.c_float, i],
         "dpmm_sampler_screen_stats": [_P, _P, _P, _P, _P],
+        "dpmm_sampler_set_merge_bound": [_P, i],
+        "dpmm_sampler_merge_bound_rejects": [_P, _P],
         "dpmm_sampler_events": [_P, _P, _P, _P, _P],
         "dpmm_sampler_pair_log": [_P, i, _P, _P, _P, _P],
         "dpmm_sampler_create": [_P, _P, dbl, _P, dbl, _P, _P, _P],
@@ -576,6 +578,16 @@ class Sampler:
         h, u = np.empty(n.value), np.empty(n.value)
         _check(load_library().dpmm_sampler_pair_log(self._s, n.value, _ptr(f), _ptr(h), _ptr(u), ctypes.byref(n)))
         return f, h, u
+
+    def set_merge_bound(self, enable):
+        """Reject hopeless merge pairs by a bound before their log ML (device iterations, NIW; default on)."""
+        _check(load_library().dpmm_sampler_set_merge_bound(self._s, 1 if enable else 0))
+
+    def merge_bound_rejects(self):
+        """Merge pairs rejected by the bound, without their log ML, since creation."""
+        v = ctypes.c_int64()
+        _check(load_library().dpmm_sampler_merge_bound_rejects(self._s, ctypes.byref(v)))
+        return v.value
 
     def events(self):
         """Decisions since creation: {"splits", "merges", "resets", "removed"}."""

@@ -1089,7 +1089,11 @@ struct dpmm_ctx {
     double* dStepW = nullptr;         // mixture weights of the last draw
     int32_t* dStepMem = nullptr;      // statistics entries of the draw's 3K parts
     int32_t* dStepCand = nullptr;     // merge candidates, then the touched clusters
-    int32_t* dStepCounts = nullptr;   // jobs, candidates
+    int32_t* dStepCounts = nullptr;   // jobs, candidates, pairs the merge bound kept, pairs it rejected
+    int32_t* dStepMap = nullptr;      // log-ML row of each pair job the merge bound kept
+    size_t stepMapCap = 0;
+    double* dStepBnd = nullptr;       // the merge bound's per-cluster means (K x D) and scalars (K x 6)
+    size_t stepBndCap = 0;
     int32_t* dStepTab = nullptr;      // relabel tables: resets + splits (3 kMaxK), merges + removals (3 kMaxK), aux (2 kMaxK)
     int32_t* dStepJobs = nullptr;
     size_t stepJobsCap = 0;
@@ -1816,7 +1820,7 @@ int dpmm_destroy(dpmm_ctx* c) {
                     (void*)c->dStepW, (void*)c->dStepMem, (void*)c->dStepCand, (void*)c->dStepCounts,
                     (void*)c->dStepTab, (void*)c->dStepJobs, (void*)c->dStepAcc, (void*)c->dStepSum,
                     (void*)c->dFlags, (void*)c->dSubsPermPrev, (void*)c->dPrevStart, (void*)c->dSubsOld,
-                    (void*)c->dSubList, c->dPBlo})
+                    (void*)c->dSubList, c->dPBlo, (void*)c->dStepMap, (void*)c->dStepBnd})
         if (p) (void)hipFree(p);
     if (c->hStepSum) (void)hipHostFree(c->hStepSum);
     if (c->hStats) (void)hipHostFree(c->hStats);
@@ -2688,6 +2692,9 @@ static int make_prior(dpmm_ctx* c, double k0, const double* m0, double v0, const
         double lg = (double)D * (D - 1) / 4.0 * std::log(kPiF64);
         for (int d = 1; d <= D; ++d) lg += std::lgamma(v0 / 2.0 + (1 - d) / 2.0);
         h.lmvg0 = lg;
+        double tp = 0.0;
+        for (int d = 0; d < D; ++d) tp += psi0[(size_t)d * D + d];
+        h.trpsi0 = v0 * tp;
         host.assign(m0, m0 + D);
         host.insert(host.end(), psi0, psi0 + (size_t)D * D);
     } else {
@@ -2835,7 +2842,7 @@ int dpmm_log_marginal_batch_ex(dpmm_ctx* c, int n, const int32_t* members, const
     if (rc) return rc;
     hipLaunchKernelGGL(k_log_marginal, dim3(n), dim3(kThreads), lds, c->stream, c->dStats, (int)c->statEntries(),
                        c->DP, D, c->prior, c->dMembers, dsel, c->prior_args, dsel ? c->prior_args1 : c->prior_args,
-                       c->dLM, nullptr);
+                       c->dLM, nullptr, nullptr);
     HIPCHK(hipGetLastError());
     if ((rc = pin_reserve(c->pinOut, c->pinOutCap, (size_t)2 * n * sizeof(double)))) return rc;
     const double* out = static_cast<const double*>(c->pinOut);
@@ -3044,15 +3051,17 @@ void labels_rewritten(dpmm_ctx* c) {
 }
 
 // k_log_marginal over the device-built job list jobs (4 statistics entries each), the grid sized for
-// `grid` jobs and the count read from njobs; results (log ML, log det psi_n) at c->dLM.
-int launch_log_marginal_dev(dpmm_ctx* c, int grid, const int32_t* jobs, const int32_t* njobs) {
+// `grid` jobs and the count read from njobs; results (log ML, log det psi_n) at c->dLM, in row
+// out_map[j] of job j when out_map is given.
+int launch_log_marginal_dev(dpmm_ctx* c, int grid, const int32_t* jobs, const int32_t* njobs,
+                            const int32_t* out_map = nullptr) {
     const int D = c->D;
     const size_t lds = c->prior == DPMM_PRIOR_GAUSSIAN ? lds_bytes((size_t)D * (D + 1) / 2 + D) : lds_bytes(0);
     int rc = allow_lds(reinterpret_cast<const void*>(k_log_marginal), lds);
     if (rc) return rc;
     if (grid <= 0) return DPMM_OK;
     hipLaunchKernelGGL(k_log_marginal, dim3(grid), dim3(kThreads), lds, c->stream, c->dStats, (int)c->statEntries(),
-                       c->DP, D, c->prior, jobs, nullptr, c->prior_args, c->prior_args, c->dLM, njobs);
+                       c->DP, D, c->prior, jobs, nullptr, c->prior_args, c->prior_args, c->dLM, njobs, out_map);
     HIPCHK(hipGetLastError());
     return DPMM_OK;
 }
@@ -3078,9 +3087,12 @@ int ensure_draw_buffers(dpmm_ctx* c, int K) {
 }
 
 // One iteration from the parameter draw to the summary copy, enqueued without a synchronisation.
-int enqueue_step(dpmm_ctx* c, const StepArgs& a, int sweep_final, int full_mu_cov, int use_weight, int grid_jobs,
+int enqueue_step(dpmm_ctx* c, const StepArgs& a0, int sweep_final, int full_mu_cov, int use_weight, int grid_jobs,
                  int32_t* trace_labels, int32_t* trace_subs) {
-    const int K = a.K;
+    const int K = a0.K;
+    // the merge bound applies when there are candidate pairs to test
+    StepArgs a = a0;
+    a.merge_bound = a0.merge_bound && grid_jobs > 3 * K ? 1 : 0;
     hipLaunchKernelGGL(k_step_sample, dim3(1), dim3(kStepThreads), 0, c->stream, a, c->dStepCl, c->dStepHist,
                        c->dStepW, c->dStepMem, c->dLogW);
     HIPCHK(hipGetLastError());
@@ -3095,9 +3107,40 @@ int enqueue_step(dpmm_ctx* c, const StepArgs& a, int sweep_final, int full_mu_co
     if (trace_labels)
         HIPCHK(hipMemcpyAsync(trace_labels, c->dLabels, (size_t)c->N * 4, hipMemcpyDeviceToHost, c->stream));
     if (trace_subs) HIPCHK(hipMemcpyAsync(trace_subs, c->dSubs, (size_t)c->N * 4, hipMemcpyDeviceToHost, c->stream));
+    const int npairs_max = grid_jobs - 3 * K;
+    const bool bound = a.merge_bound != 0;
+    // with the merge bound: the pairs' output map, then the map and prior selectors of the 4K jobs of
+    // the first launch; W_k's results in log-ML rows wrow + k, past the pairs'
+    int32_t* pmap = c->dStepMap;
+    int32_t* omap = pmap + npairs_max;
+    int32_t* psel = omap + 4 * K;
+    const int wrow = grid_jobs;
     hipLaunchKernelGGL(k_step_jobs, dim3(1), dim3(kStepThreads), 0, c->stream, a, c->dStepCl, c->dStats,
-                       c->dStepCand, c->dStepJobs, c->dStepCounts);
-    if ((rc = launch_log_marginal_dev(c, grid_jobs, c->dStepJobs, c->dStepCounts))) return rc;
+                       c->dStepCand, c->dStepJobs, c->dStepCounts, bound ? psel : nullptr, bound ? omap : nullptr,
+                       wrow);
+    if (bound) {
+        // the parts' log MLs with log det W_k of every cluster in one launch, the merge bound of every
+        // candidate pair, then the log MLs of the pairs it keeps
+        PriorArgs hw = c->prior_args;
+        hw.k0 = 0.0;
+        hw.v0 = 0.5 * c->prior_args.v0;
+        const size_t lds = lds_bytes((size_t)c->D * (c->D + 1) / 2 + c->D);
+        if ((rc = allow_lds(reinterpret_cast<const void*>(k_log_marginal), lds))) return rc;
+        hipLaunchKernelGGL(k_log_marginal, dim3(4 * K), dim3(kThreads), lds, c->stream, c->dStats,
+                           (int)c->statEntries(), c->DP, c->D, c->prior, c->dStepJobs, psel, c->prior_args, hw, c->dLM,
+                           nullptr, omap);
+        double* vec = c->dStepBnd;
+        double* sc = vec + (size_t)c->D * K;
+        hipLaunchKernelGGL(k_step_bound_prep, dim3(K), dim3(64), 0, c->stream, a, c->dStats, c->dLM,
+                           c->dLM + (size_t)2 * wrow, c->prior_args, c->D, c->DP, vec, sc);
+        int32_t* pjobs = c->dStepJobs + (size_t)4 * 4 * K;
+        hipLaunchKernelGGL(k_step_pair_bound, dim3((npairs_max + kThreads / 64 - 1) / (kThreads / 64)), dim3(kThreads),
+                           0, c->stream, a, c->dStepCand, c->dStepCounts, vec, sc, c->prior_args, c->D,
+                           c->prior_args.trpsi0, pjobs, pmap, c->dLM);
+        if ((rc = launch_log_marginal_dev(c, npairs_max, pjobs, c->dStepCounts + 2, pmap))) return rc;
+    } else if ((rc = launch_log_marginal_dev(c, grid_jobs, c->dStepJobs, c->dStepCounts))) {
+        return rc;
+    }
     int32_t* tab1 = c->dStepTab;
     int32_t* tab2 = tab1 + 3 * kMaxK;
     int32_t* aux = tab2 + 3 * kMaxK;
@@ -3198,6 +3241,12 @@ int step_iteration(dpmm_ctx* c, const StepArgs& args, int sweep_final, int full_
     const int64_t maxjobs = 3 * (int64_t)K + maxpairs;
     int rc;
     if ((rc = grow(c->dStepJobs, c->stepJobsCap, (size_t)4 * maxjobs))) return rc;
+    if (a.merge_bound && maxpairs > 0) {
+        if ((rc = grow(c->dStepJobs, c->stepJobsCap, (size_t)4 * (maxjobs + K)))) return rc;
+        if ((rc = grow(c->dStepMap, c->stepMapCap, (size_t)maxpairs + 8 * K))) return rc;
+        if ((rc = grow(c->dStepBnd, c->stepBndCap, (size_t)K * (c->D + 6)))) return rc;
+        if ((rc = grow(c->dLM, c->lmCap, (size_t)2 * (maxjobs + K)))) return rc;
+    }
     if ((rc = grow(c->dStepAcc, c->stepAccCap, (size_t)std::max<int64_t>(maxpairs, 1)))) return rc;
     if ((rc = grow(c->dLM, c->lmCap, (size_t)2 * maxjobs))) return rc;
     if ((rc = grow(c->dLogW, c->logwCap, (size_t)3 * K))) return rc;

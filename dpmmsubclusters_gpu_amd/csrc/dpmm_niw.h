@@ -36,6 +36,7 @@ __device__ __forceinline__ void tri_ij(int e, int& i, int& j) {
 
 struct PriorArgs {
     double k0, v0, logdet_psi0, lmvg0;   // lmvg0 = log Gamma_D(v0 / 2)
+    double trpsi0;                       // v0 tr psi0
     double alpha0_sum, lg_alpha0_sum;    // multinomial: sum alpha0, lgamma(sum alpha0)
     const double* m0;                    // D
     const double* psi0;                  // D x D row-major (the reference's psi = Psi / v0)
@@ -493,7 +494,8 @@ __device__ void gram_rev_packed(const double* R, double* G, int D) {
 
 // ---- kernels -------------------------------------------------------------------------------------
 
-// out[2 j] = log marginal likelihood, out[2 j + 1] = log det psi_n (NaN when psi_n is not SPD).
+// out[2 j] = log marginal likelihood, out[2 j + 1] = log det psi_n (NaN when psi_n is not SPD); with
+// out_map, row out_map[j] instead of j.
 // Two priors: h (the model's) and h1 (the outlier cluster's); psel[j] (NULL: 0) picks, bit 0, the prior
 // the posterior is formed from and, bit 1, the prior of the log marginal likelihood's prior terms (the
 // reference's log_marginal_likelihood(prior, posterior, stats) takes them separately, niw.cpp:122-138,
@@ -503,7 +505,8 @@ __global__ void __launch_bounds__(kThreads, 4) k_log_marginal(const double* __re
                                                            int prior, const int32_t* __restrict__ members,
                                                            const int32_t* __restrict__ psel, PriorArgs h,
                                                            PriorArgs h1, double* __restrict__ out,
-                                                           const int32_t* __restrict__ njobs) {
+                                                           const int32_t* __restrict__ njobs,
+                                                           const int32_t* __restrict__ out_map) {
     extern __shared__ double lds[];
     __shared__ double red[8];
     __shared__ double scal[4];
@@ -561,8 +564,9 @@ __global__ void __launch_bounds__(kThreads, 4) k_log_marginal(const double* __re
                 (hm.v0 / 2.0) * (D * log(hm.v0) + hm.logdet_psi0) - (vn / 2.0) * (D * log(vn) + ld) +
                 (D / 2.0) * log(hm.k0 / kn);
         }
-        out[2 * job] = L;
-        out[2 * job + 1] = ld;
+        const size_t o = out_map ? (size_t)out_map[job] : (size_t)job;   // out_map: the job's row of out
+        out[2 * o] = L;
+        out[2 * o + 1] = ld;
     }
 }
 

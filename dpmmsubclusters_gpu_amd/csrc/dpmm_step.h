@@ -145,11 +145,14 @@ __global__ void __launch_bounds__(kStepThreads) k_step_sample(StepArgs a, StepCl
 // The parts of clusters 0 .. K-1 from the statistics pass (entry q = 2k + s holds sub-cluster s of
 // cluster k; its first double is the count), the merge candidates (splittable, non-empty) and the
 // log-ML jobs: 3k + {0, 1, 2} the parts of cluster k, then 3K + p the candidate pairs in
-// lexicographic order.  counts[0] = jobs, counts[1] = candidates.
+// lexicographic order.  counts[0] = jobs, counts[1] = candidates.  With the merge bound the pairs are
+// listed later (k_step_pair_bound) and jobs 3K + k are the bound's W_k, with their prior selectors
+// (psel) and output rows (omap).
 __global__ void __launch_bounds__(kStepThreads) k_step_jobs(StepArgs a, StepCluster* __restrict__ cl,
                                                            const double* __restrict__ stats,
                                                            int32_t* __restrict__ cand, int32_t* __restrict__ jobs,
-                                                           int32_t* __restrict__ counts) {
+                                                           int32_t* __restrict__ counts, int32_t* __restrict__ psel,
+                                                           int32_t* __restrict__ omap, int wrow) {
     __shared__ int ws[16];
     const int K = a.K;
     int ncand = 0;
@@ -166,6 +169,21 @@ __global__ void __launch_bounds__(kStepThreads) k_step_jobs(StepArgs a, StepClus
             const StepPart* ps[3] = {&c.c, &c.l, &c.r};
             for (int p = 0; p < 3; ++p)
                 for (int e = 0; e < 4; ++e) jobs[4 * (3 * k + p) + e] = ps[p]->mem[e];
+            if (a.merge_bound) {
+                // the merge bound's W_k (job 3K + k: both sub-clusters under the prior h1 of the launch,
+                // result in row wrow + k) beside the parts (rows 3k + p, prior h)
+                int32_t* w = jobs + 4 * (3 * K + k);
+                w[0] = 2 * k;
+                w[1] = 2 * k + 1;
+                w[2] = -1;
+                w[3] = -1;
+                for (int p = 0; p < 3; ++p) {
+                    psel[3 * k + p] = 0;
+                    omap[3 * k + p] = 3 * k + p;
+                }
+                psel[3 * K + k] = 3;
+                omap[3 * K + k] = wrow + k;
+            }
             f = !a.no_more_splits && c.splittable && Nl + Nr > 0;
         }
         int tot;
@@ -174,7 +192,8 @@ __global__ void __launch_bounds__(kStepThreads) k_step_jobs(StepArgs a, StepClus
         ncand += tot;
     }
     __syncthreads();
-    for (int ra = 0; ra + 1 < ncand; ++ra) {
+    // with the merge bound, k_step_pair_bound lists the pairs that need their log ML
+    for (int ra = 0; ra + 1 < ncand && !a.merge_bound; ++ra) {
         const int64_t row0 = (int64_t)ra * ncand - (int64_t)ra * (ra + 1) / 2;
         const int i = cand[ra];
         for (int rb = ra + 1 + (int)threadIdx.x; rb < ncand; rb += blockDim.x) {
@@ -189,6 +208,118 @@ __global__ void __launch_bounds__(kStepThreads) k_step_jobs(StepArgs a, StepClus
     if (threadIdx.x == 0) {
         counts[0] = 3 * K + (int)((int64_t)ncand * (ncand - 1) / 2);
         counts[1] = ncand;
+        counts[2] = 0;   // pairs listed by the merge bound
+        counts[3] = 0;   // pairs it rejected
+    }
+}
+
+// The merge bound.  For the NIW posterior, Psi_n = v0 psi0 + S + P (the unnormalised scatter, v_n
+// times the psi the reference keeps), where S is the scatter of the points about their mean xbar and
+// P = (k0 N / kappa_n) (xbar - m0)(xbar - m0)^T.  Merging clusters i and j gives
+//   Psi_m = W_i + W_j + c d d^T + P_m,   W_k = v0 psi0 / 2 + S_k,  d = xbar_i - xbar_j,
+//   c = N_i N_j / (N_i + N_j),
+// with P_m positive semidefinite, so by Minkowski's determinant inequality and the rank-one update
+// (log det (A + c d d^T) = log det A + log(1 + c d^T A^-1 d) >= log det A + log(1 + c |d|^2 / tr A)),
+//   log det Psi_m >= D log(det W_i^(1/D) + det W_j^(1/D)) + log(1 + c |d|^2 / tr(W_i + W_j)).
+// The log ML (niw.cpp:122-138) decreases in log det Psi_m, so its value at the bound bounds the merged
+// log ML from above, and with it the merge's Hastings ratio (should_merge, shared_actions.cpp:88-111).
+// A pair whose bounded ratio is below -100 is rejected whatever its uniform (the test accepts above
+// log u >= log 2^-33, or log 0.1 in the final iterations), which is what its exact test would decide,
+// so its log ML is not computed.  log det W_k comes from one factorisation per cluster (the log-ML
+// kernel under a prior with k0 = 0 and v0 / 2, whose psi_n is W_k / v_n).
+//
+// Per cluster k (one wave each): the data mean (vec, D doubles) and, in sc (6 doubles): an upper bound
+// of tr S_k (the cancellation of sum x^2 - N |xbar|^2 covered), log det W_k, N_k, the cluster's log ML
+// and lgamma(N_k + alpha / 2) - lgamma(N_k).
+__global__ void __launch_bounds__(64) k_step_bound_prep(StepArgs a, const double* __restrict__ stats,
+                                                        const double* __restrict__ lm,
+                                                        const double* __restrict__ lw, PriorArgs h, int D, int DP,
+                                                        double* __restrict__ vec, double* __restrict__ sc) {
+    const int k = blockIdx.x, lane = threadIdx.x;
+    const double* e0 = stats + (size_t)(2 * k) * a.E;
+    const double* e1 = stats + (size_t)(2 * k + 1) * a.E;
+    const double N = e0[0] + e1[0];
+    double m2 = 0.0, trq = 0.0;
+    for (int d = lane; d < D; d += 64) {
+        const double xb = N > 0.0 ? (e0[1 + d] + e1[1 + d]) / N : 0.0;
+        vec[(size_t)k * D + d] = xb;
+        m2 += xb * xb;
+        trq += e0[1 + DP + (size_t)d * DP + d] + e1[1 + DP + (size_t)d * DP + d];
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        m2 += __shfl_xor(m2, o);
+        trq += __shfl_xor(trq, o);
+    }
+    if (lane == 0) {
+        sc[6 * k] = (trq - N * m2) + 1e-12 * (trq + N * m2);
+        sc[6 * k + 1] = (double)D * log(0.5 * h.v0 + N) + lw[2 * k + 1];
+        sc[6 * k + 2] = N;
+        sc[6 * k + 3] = lm[2 * (3 * k)];
+        sc[6 * k + 4] = N > 0.0 ? lgamma(N + 0.5 * a.alpha) - lgamma(N) : 0.0;   // the ratio's terms of k
+    }
+}
+
+// One wave per candidate pair p (lexicographic over the candidates): the merge bound, then either the
+// pair's log-ML job listed (jobs row q, out_map[q] = 3K + p) or its log ML set to -inf.  trpsi0 =
+// v0 tr psi0.
+__global__ void __launch_bounds__(kThreads) k_step_pair_bound(StepArgs a, const int32_t* __restrict__ cand,
+                                                              int32_t* __restrict__ counts,
+                                                              const double* __restrict__ vec,
+                                                              const double* __restrict__ sc, PriorArgs h, int D,
+                                                              double trpsi0, int32_t* __restrict__ jobs,
+                                                              int32_t* __restrict__ out_map, double* __restrict__ lm) {
+    const int lane = threadIdx.x & 63;
+    const int64_t p = (int64_t)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+    const int K = a.K, ncand = counts[1];
+    const int64_t np = (int64_t)ncand * (ncand - 1) / 2;
+    if (p >= np) return;
+    int ra, rb;
+    step_pair(p, ncand, ra, rb);
+    const int i = cand[ra], j = cand[rb];
+    const double* vi = vec + (size_t)i * D;
+    const double* vj = vec + (size_t)j * D;
+    const double Ni = sc[6 * i + 2], Nj = sc[6 * j + 2], Nm = Ni + Nj, vm = h.v0 + Nm;
+    double d2 = 0.0, lg = 0.0;
+    for (int d = lane; d < D; d += 64) {
+        const double t = vi[d] - vj[d];
+        d2 += t * t;
+        // log Gamma_D(v_m / 2) from above, term by term: lgamma(x) < (x - 1/2) log x - x + log(2 pi) / 2 +
+        // 1 / (12 x) for x > 0 (Stirling's series with Robbins' remainder bound), one log instead of lgamma
+        const double x = vm / 2.0 + (double)(1 - (d + 1)) / 2.0;
+        lg += (x - 0.5) * log(x) - x + 0.91893853320467274 + 1.0 / (12.0 * x);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        d2 += __shfl_xor(d2, o);
+        lg += __shfl_xor(lg, o);
+    }
+    if (lane != 0) return;
+    const double tri = sc[6 * i], trj = sc[6 * j], lwi = sc[6 * i + 1], lwj = sc[6 * j + 1];
+    const double lmi = sc[6 * i + 3], lmj = sc[6 * j + 3];
+    const double trA = trpsi0 + tri + trj;
+    bool reject = false;
+    if (Ni > 0.0 && Nj > 0.0 && trA > 0.0 && isfinite(lwi) && isfinite(lwj) && isfinite(lmi) && isfinite(lmj)) {
+        const double hi = fmax(lwi, lwj) / D, lo = fmin(lwi, lwj) / D;
+        const double lb = (double)D * (hi + log1p(exp(lo - hi))) + log1p(Ni * Nj / Nm * d2 / trA);
+        const double Lub = -Nm * (double)D * 0.5 * log(kPiF64) + lg + (double)D * (D - 1) / 4.0 * log(kPiF64) -
+                           h.lmvg0 + (h.v0 / 2.0) * (D * log(h.v0) + h.logdet_psi0) - (vm / 2.0) * lb +
+                           (D / 2.0) * log(h.k0 / (h.k0 + Nm));
+        const double al = a.alpha;
+        const double ub = -log(al) + lgamma(al) - 2 * lgamma(0.5 * al) + lgamma(Nm) - lgamma(Nm + al) +
+                          sc[6 * i + 4] + sc[6 * j + 4] + Lub - lmi - lmj;
+        reject = ub < -100.0;
+    }
+    if (reject) {
+        lm[2 * (3 * (int64_t)K + p)] = -__builtin_huge_val();
+        lm[2 * (3 * (int64_t)K + p) + 1] = __builtin_nan("");
+        atomicAdd(&counts[3], 1);
+    } else {
+        const int q = atomicAdd(&counts[2], 1);
+        int32_t* m = jobs + 4 * (int64_t)q;
+        m[0] = 2 * i;
+        m[1] = 2 * i + 1;
+        m[2] = 2 * j;
+        m[3] = 2 * j + 1;
+        out_map[q] = 3 * K + (int32_t)p;
     }
 }
 
@@ -421,7 +552,8 @@ __global__ void __launch_bounds__(kStepThreads) k_step_decide(StepArgs a, StepCl
         sum->followup = nsplit + s_nreset > 0;
         sum->relabel2 = s_nmerge > 0 || nk != Ks;
         sum->error = s_err;
-        sum->npairs = a.no_more_splits ? 0 : counts[1] * (counts[1] - 1) / 2;
+        sum->npairs = a.no_more_splits ? 0 : a.merge_bound ? counts[2] : counts[1] * (counts[1] - 1) / 2;
+        sum->nbound = a.no_more_splits || !a.merge_bound ? 0 : counts[3];
     }
 }
 

@@ -35,6 +35,7 @@ struct StepSummary {
     int32_t relabel2;   // merges or removals: labels and the cluster table are renumbered
     int32_t error;      // bit 0: a parameter draw failed (factor not SPD); bit 1: a log ML not finite
     int32_t npairs;     // candidate pairs given the exact merge test
+    int32_t nbound;     // candidate pairs rejected by the merge bound before their log ML
 };
 
 // Scalars of one iteration's kernels.
@@ -46,6 +47,7 @@ struct StepArgs {
     int kcap;            // largest cluster count the device kernels take
     int B;               // burn-out length
     int E;               // doubles per statistics entry
+    int merge_bound;     // NIW: reject hopeless merge pairs by a bound before their log ML
     double alpha;
 };
 

@@ -158,6 +158,9 @@ struct SamplerImpl {
     std::vector<float> log_feat;
     std::vector<double> log_h, log_u;
     int64_t n_exact = 0, n_skipped = 0, n_missed = 0, n_full = 0;
+    // merge bound of the device iteration (NIW): pairs rejected without their log ML
+    bool merge_bound = true;
+    int64_t n_bound = 0;
     // totals since creation: splits, merges, resets, clusters removed (merged away or emptied)
     int64_t ev_split = 0, ev_merge = 0, ev_reset = 0, ev_removed = 0;
     bool counts_fresh = false;   // Nsub holds the counts of the current labels (from the sweep call)
@@ -263,12 +266,14 @@ struct SamplerImpl {
         a.no_more_splits = no_more_splits ? 1 : 0;
         a.kcap = kcap;
         a.alpha = o.alpha;
+        a.merge_bound = merge_bound && prior == DPMM_PRIOR_GAUSSIAN ? 1 : 0;
         StepSummary sm{};
         host_ms = 0;
         rc = step_iteration(ctx, a, (fin || o.hard_clustering) ? 1 : 0, o.full_mu_cov, o.mult_use_weight, trace_labels,
                             trace_subs, &sm);
         if (rc) return rc;
         n_exact += sm.npairs;
+        n_bound += sm.nbound;
         ev_split += sm.nsplit;
         ev_merge += sm.nmerge;
         ev_reset += sm.nreset;
@@ -868,6 +873,18 @@ int dpmm_sampler_screen_stats(dpmm_sampler* S, int64_t* exact, int64_t* skipped,
     if (skipped) *skipped = S->s.n_skipped;
     if (missed) *missed = S->s.n_missed;
     if (full) *full = S->s.n_full;
+    return DPMM_OK;
+}
+
+int dpmm_sampler_set_merge_bound(dpmm_sampler* S, int enable) {
+    if (!S) return DPMM_EINVAL;
+    S->s.merge_bound = enable != 0;
+    return DPMM_OK;
+}
+
+int dpmm_sampler_merge_bound_rejects(dpmm_sampler* S, int64_t* rejected) {
+    if (!S || !rejected) return DPMM_EINVAL;
+    *rejected = S->s.n_bound;
     return DPMM_OK;
 }
 

@@ -372,6 +372,17 @@ int dpmm_sampler_set_screener(dpmm_sampler* sampler, dpmm_screener* s, int full_
  * screener scored below threshold on full-scan iterations, full scans. */
 int dpmm_sampler_screen_stats(dpmm_sampler* sampler, int64_t* exact_tests, int64_t* skipped, int64_t* missed,
                               int64_t* full_scans);
+/* Merge bound of the device-resident iteration (NIW prior; on by default; no counterpart in the
+ * reference, which computes the log marginal likelihood of every candidate pair's union,
+ * local_clusters_actions.cpp:287-321).  Merging i and j gives Psi_m = Psi_i + S_j + c r r^T exactly
+ * (r = mu_i - xbar_j, c = kappa_i N_j / (kappa_i + N_j)), so log det Psi_m >= log det Psi_i +
+ * log(1 + c |r|^2 / tr Psi_i), and the larger of that bound from i and from j bounds the merged log ML,
+ * and the Hastings ratio, from above; a pair whose bounded ratio is below -100 is rejected without its
+ * log ML, as its exact test would decide whatever its uniform.  Decisions are identical with the bound
+ * on or off.  dpmm_sampler_merge_bound_rejects: pairs rejected so since creation (exact tests: the
+ * other candidates, counted by dpmm_sampler_screen_stats). */
+int dpmm_sampler_set_merge_bound(dpmm_sampler* sampler, int enable);
+int dpmm_sampler_merge_bound_rejects(dpmm_sampler* sampler, int64_t* rejected);
 /* The last step's pair log: up to max_rows rows of features (16 floats), log H and log U. */
 int dpmm_sampler_pair_log(dpmm_sampler* sampler, int max_rows, float* features, double* log_h, double* log_u,
                           int* n_rows);

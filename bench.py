@@ -348,6 +348,8 @@ def main():
     ap.add_argument("--paths", default="mixed,fp32,fp64")
     ap.add_argument("--n", type=int, default=0, help="override the configuration's N (a reduced run, not its bench line)")
     ap.add_argument("--iters", type=int, default=0, help="override the screener workload's iteration count")
+    ap.add_argument("--no-fused", action="store_true",
+                    help="separate sub-label sweep instead of the draw fused into the pruned label sweep")
     ap.add_argument("--sweep-workgroups", type=int, default=0,
                     help="workgroups per partition chunk of the pruned label sweep (0: the library default)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -404,6 +406,8 @@ def main():
     dev.set_graph_mode(args.graph)
     if args.sweep_workgroups:
         dev.set_sweep_workgroups(args.sweep_workgroups)
+    if args.no_fused:
+        dev.set_fused_sub_labels(False)
     smp = dp.Sampler(dev, hyper, cfg["alpha"], iters, seed=args.seed + 1, precision=prec,
                      host_decisions=args.host_decisions)
     decisions = "host" if args.host_decisions else "device"
@@ -432,6 +436,7 @@ def main():
         sys.exit(3)
     first = t + 1
     lab_ms, sub_ms, st_ms, host_ms, flops, rechk, srechk, evfrac, ks = [], [], [], [], [], [], [], [], []
+    main_ms, st_mode = [], []
     t0 = time.perf_counter()
     for t in range(first, first + args.steps):
         kc = smp.num_clusters()
@@ -439,6 +444,8 @@ def main():
         smp.step(t)
         a, b, c = dev.kernel_ms()   # HIP events of the phases (the step has synchronised already)
         lab_ms.append(a); sub_ms.append(b); st_ms.append(c); host_ms.append(smp.host_ms())
+        main_ms.append(dev.sweep_kernel_ms())   # the pruned sweep's main kernel alone
+        st_mode.append(dev.last_statistics_update() if not mult else 0)
         flops.append(2.0 * kc * D * (1 if mult else D) * N)
     dev.synchronize()
     torch.cuda.synchronize()
@@ -487,7 +494,18 @@ def main():
                                   "4 D + 8 bytes per document (gathered counts, index, sub-label)")
     else:
         esz = 8.0 if prec == dp.Precision.FP64 else 4.0
-        if pruned:   # exact pruning leaves about one cluster scored per point: a gather of the points
+        dpad = 32 * ((D + 31) // 32)
+        fused = pruned and dpad <= 64 and not args.no_fused
+        if fused:   # the sub-label draw rides in the pruned sweep: one gathered read of each point
+            avg_main = float(np.mean(main_ms))
+            phase["label_sub_sweep"] = hbm(
+                "k_label_gauss_mxp (fused sub-labels)", N * (esz * D + 12.0), avg_main,
+                "D coordinates + index + previous and new sub-label per point, rows gathered in partition order "
+                f"({float(np.mean(evfrac)):.3f} of the (wave, cluster) scorings done); this kernel's own HIP events")
+            phase["label_sub_sweep"]["phase_ms"] = avg_lab
+            phase["label_sub_sweep"]["phase_kernels"] = ("k_prune_prep + partition + k_label_gauss_mxp + "
+                                                         "k_label_gauss_listp + k_sub_gauss_list")
+        elif pruned:   # exact pruning leaves about one cluster scored per point: a gather of the points
             phase["label_sweep"] = hbm("k_prune_prep + partition + k_label_gauss_mxp + k_label_gauss_list",
                                        N * (esz * D + 12.0), avg_lab,
                                        "D coordinates + index + label read + label write per point "
@@ -500,21 +518,30 @@ def main():
                 "achieved": float(np.mean(flops)) / (avg_lab * 1e-3) / 1e12, "peak": peak, "unit": "TFLOP/s",
                 "traffic": None,
                 "algorithmic": "2 K D^2 flops per point (dense whitening of K clusters)"}
-        phase["sub_sweep"] = hbm("k_sub_gauss_mx + k_sub_gauss_list" if prec == dp.Precision.MIXED else
-                                 "k_sub_gauss", N * (esz * D + 12.0), avg_sub,
-                                 "D coordinates + index + label + sub-label write per point (gathered)")
+        if not fused:
+            phase["sub_sweep"] = hbm("k_sub_gauss_mx + k_sub_gauss_list" if prec == dp.Precision.MIXED else
+                                     "k_sub_gauss", N * (esz * D + 12.0), avg_sub,
+                                     "D coordinates + index + label + sub-label write per point (gathered)")
+        # the statistics' FP64 rate from the full recomputations; the passes between update the statistics
+        # from the points whose sub-label moved and are reported by their time only
+        full = [m for m, u in zip(st_ms, st_mode) if u == 0]
+        upd = [m for m, u in zip(st_ms, st_mode) if u != 0]
+        t_full = float(np.mean(full)) if full else float("nan")
         phase["statistics"] = {
             "bound": "mfma", "kernel": ("k_stats_gauss" if D <= 8 else "k_stats_gauss_wave" if D <= 64 else
                                         "k_stats_gauss_mfma") + " + k_stats_reduce",
-            "achieved": N * (D * (D + 1.0) + D) / (avg_st * 1e-3) / 1e12, "peak": PEAK["fp64"], "unit": "TFLOP/s",
-            "traffic": None,
-            "algorithmic": "D (D + 1) + D flops per point (upper triangle of x x^T and the sum, FP64)"}
+            "achieved": N * (D * (D + 1.0) + D) / (t_full * 1e-3) / 1e12 if full else None, "peak": PEAK["fp64"],
+            "unit": "TFLOP/s", "traffic": None,
+            "algorithmic": "D (D + 1) + D flops per point (upper triangle of x x^T and the sum, FP64), full "
+                           "recomputations only",
+            "full_passes": len(full), "full_pass_ms": t_full if full else None,
+            "update_passes": len(upd), "update_pass_ms": float(np.mean(upd)) if upd else None}
     k_timed = int(round(float(np.mean(ks))))
     for name, r in phase.items():
-        r["frac"] = r["achieved"] / r["peak"]
+        r["frac"] = r["achieved"] / r["peak"] if r["achieved"] is not None else None
         r["traffic"] = measured_traffic(args.config, args.precision, k_timed, name)
-    dominant = max(("label_sweep", "sub_sweep", "statistics"),
-                   key=lambda k: {"label_sweep": avg_lab, "sub_sweep": avg_sub, "statistics": avg_st}[k])
+    phase_ms = {"label_sweep": avg_lab, "label_sub_sweep": avg_lab, "sub_sweep": avg_sub, "statistics": avg_st}
+    dominant = max(phase, key=lambda k: phase_ms[k])
     roof = dict(phase[dominant], phase=dominant)
     K_final = smp.num_clusters()
     from dpmmsubclusters_gpu_amd import _nmi
@@ -539,8 +566,9 @@ def main():
         "roofline_by_phase": phase,
         # the rest of a step: parameter draw, log MLs, decisions, relabel passes, launch gaps and the
         # synchronisation(s); with host decisions, `host_decisions` of it is host time between launches
-        "breakdown_ms": dict({"label_sweep": avg_lab, "sub_sweep": avg_sub, "statistics": avg_st,
-                              "rest": ms_step - avg_lab - avg_sub - avg_st},
+        "breakdown_ms": dict({("label_sub_sweep" if "label_sub_sweep" in phase else "label_sweep"): avg_lab,
+                              ("sub_label_order" if "label_sub_sweep" in phase else "sub_sweep"): avg_sub,
+                              "statistics": avg_st, "rest": ms_step - avg_lab - avg_sub - avg_st},
                              **({"host_decisions": float(np.mean(host_ms))} if args.host_decisions else {})),
         "final_nmi_vs_truth": nmi,
         "recheck_fraction": (float(np.mean(rechk)) / N) if rechk else None,

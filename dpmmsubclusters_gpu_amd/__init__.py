@@ -98,6 +98,7 @@ def load_library():
         "dpmm_sufficient_statistics": [_P, i, _P, _P, _P],
         "dpmm_relabel": [_P, u32, i, _P, _P],
         "dpmm_last_kernel_ms": [_P, _P, _P, _P],
+        "dpmm_last_sweep_kernel_ms": [_P, _P],
         "dpmm_set_mixed_tolerance": [_P, ctypes.c_float],
         "dpmm_last_recheck_count": [_P, _P],
         "dpmm_last_sub_recheck_count": [_P, _P],
@@ -441,6 +442,12 @@ class DeviceSweep:
         a, b, c = ctypes.c_float(), ctypes.c_float(), ctypes.c_float()
         _check(load_library().dpmm_last_kernel_ms(self._ctx, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
         return a.value, b.value, c.value
+
+    def sweep_kernel_ms(self):
+        """The last pruned mixed label sweep's main kernel alone (ms, HIP events), 0 if none."""
+        a = ctypes.c_float()
+        _check(load_library().dpmm_last_sweep_kernel_ms(self._ctx, ctypes.byref(a)))
+        return a.value
 
     def synchronize(self):
         _check(load_library().dpmm_synchronize(self._ctx))

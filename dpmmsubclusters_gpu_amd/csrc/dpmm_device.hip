@@ -1104,9 +1104,9 @@ struct dpmm_ctx {
     int drawStride() const {
         return prior == DPMM_PRIOR_GAUSSIAN ? ((D + D * (D + 1) / 2 + 1) + 1) & ~1 : (D + 1) & ~1;
     }
-    hipEvent_t ev[6] = {};
+    hipEvent_t ev[8] = {};   // phases: label sweep 0-1, sub-labels 2-3, statistics 4-5; the sweep's main kernel 6-7
     float ms_label = 0, ms_sub = 0, ms_stats = 0;
-    bool ev_label = false, ev_sub = false, ev_stats = false;
+    bool ev_label = false, ev_sub = false, ev_stats = false, ev_main = false;
     uint32_t key0() const { return (uint32_t)seed; }
     uint32_t key1() const { return (uint32_t)(seed >> 32); }
     size_t statEntries() const { return prior == DPMM_PRIOR_GAUSSIAN ? 1 + DP + (size_t)DP * DP : 1 + (size_t)D; }
@@ -1708,6 +1708,7 @@ int run_label_mixed_pruned(dpmm_ctx* c, uint32_t iter, int fin) {
     launch_label_gauss_mxp<DPMV>(c->stream, grid, X, c->D, c->K, c->CH, c->dPerm, c->dLabelStart, c->dChunkStart, \
                                  PB, c->dPruneM, c->dPruneSC, c->tau, c->key0(), c->key1(), iter, fin, c->dLabels, \
                                  c->dRecheck, c->dRecheckCount, c->dEvals, c->dFlags, sf)
+    if (!t_capturing) (void)hipEventRecord(c->ev[6], c->stream);
     switch (c->DPM) {
         case 32: DPMM_MXP(32); break;
         case 64: DPMM_MXP(64); break;
@@ -1716,6 +1717,8 @@ int run_label_mixed_pruned(dpmm_ctx* c, uint32_t iter, int fin) {
         default: return fail(DPMM_EINVAL, "mixed sweep needs D <= 128");
     }
 #undef DPMM_MXP
+    if (!t_capturing) (void)hipEventRecord(c->ev[7], c->stream);
+    c->ev_main = !t_capturing;
     const int grid2 = (int)std::min<int64_t>(1024, (c->N + kThreads - 1) / kThreads);
     DPMM_DISPATCH_DP(c->DP, lglp_f32, c, iter, fin, grid2);   // the recheck pruned with the same bounds
     // fused: the FP32 sub-label draw of the listed positions, by their final labels (after the recheck)
@@ -2002,6 +2005,7 @@ int dpmm_sample_cluster_labels(dpmm_ctx* c, uint32_t iter, int fin) {
     c->last_pruned_K = 0;
     c->label_flag = false;
     c->fused_subs = false;
+    c->ev_main = false;
     if (c->prior == DPMM_PRIOR_GAUSSIAN && c->precision == DPMM_PREC_MIXED) {
         bool prune = c->prune_mode != 0 && c->K >= 2 && c->D % 4 == 0;   // 16-byte rows for the gathers
         // adaptive: small sweeps (N K DP^2 below 4e11 multiply-adds) stay unpruned, the fixed cost
@@ -2583,6 +2587,14 @@ int dpmm_last_kernel_ms(dpmm_ctx* c, float* a, float* b, float* s) {
     if (a) { *a = 0; if (c->ev_label) (void)hipEventElapsedTime(a, c->ev[0], c->ev[1]); }
     if (b) { *b = 0; if (c->ev_sub) (void)hipEventElapsedTime(b, c->ev[2], c->ev[3]); }
     if (s) { *s = 0; if (c->ev_stats) (void)hipEventElapsedTime(s, c->ev[4], c->ev[5]); }
+    return DPMM_OK;
+}
+
+int dpmm_last_sweep_kernel_ms(dpmm_ctx* c, float* ms) {
+    if (!c || !ms) return fail(DPMM_EINVAL, "bad args");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    *ms = 0;
+    if (c->ev_main) (void)hipEventElapsedTime(ms, c->ev[6], c->ev[7]);
     return DPMM_OK;
 }
 

@@ -489,6 +489,9 @@ __device__ __forceinline__ float fused_screens_split(const mx_u32x4 (&ph)[DP / 1
     }
     d2 = dd + __shfl_xor(dd, 32);
     Aa = reinterpret_cast<const float*>(sa + TB)[DP + 1] * d2 * (1.0f + ((float)DP + 4.0f) * 0x1p-24f);
+    // the anchor's screen is finished here, before the sub-clusters' loop (q materialised): sunk past
+    // it, its operands would be live through the loop and spill
+    asm volatile("" : "+v"(q));
     // the anchor's sub-clusters
     constexpr float eps = sub_rel_eps_a(DP);
     const float heps = 0.5f * eps;
@@ -669,8 +672,15 @@ k_label_gauss_mxp(const float* __restrict__ X, int D, int K, int CH, const int32
     }
     __syncthreads();
     // the anchor's bound inputs of clusters 0..63, one per lane (the same for every pass)
-    const bool in0 = lane < K && lane != a;
-    const float sg0 = in0 ? psc[2 * lane] : 0.f, ck0 = in0 ? psc[2 * lane + 1] : 0.f, m0 = in0 ? mrow[lane] : 0.f;
+    // (in LDS, not held in registers through the passes)
+    __shared__ float bnd0[3][64];
+    if (threadIdx.x < 64) {
+        const bool in0 = lane < K && lane != a;
+        bnd0[0][lane] = in0 ? psc[2 * lane] : 0.f;
+        bnd0[1][lane] = in0 ? psc[2 * lane + 1] : 0.f;
+        bnd0[2][lane] = in0 ? mrow[lane] : 0.f;
+    }
+    __syncthreads();
     unsigned long long nev = 0, npass = 0;
     __shared__ int32_t wlist[4][64];
     WaveList rl{wlist[wv], 0};
@@ -811,8 +821,8 @@ k_label_gauss_mxp(const float* __restrict__ X, int D, int K, int CH, const int32
             float ub = neg_inf<float>();
             bool keep = false;
             if (kk < K && kk != a) {
-                const float sg = k0 == 0 ? sg0 : psc[2 * kk], ck = k0 == 0 ? ck0 : psc[2 * kk + 1];
-                const float lb = fmaxf(0.f, (k0 == 0 ? m0 : mrow[kk]) - sg * rad);
+                const float sg = k0 == 0 ? bnd0[0][lane] : psc[2 * kk], ck = k0 == 0 ? bnd0[1][lane] : psc[2 * kk + 1];
+                const float lb = fmaxf(0.f, (k0 == 0 ? bnd0[2][lane] : mrow[kk]) - sg * rad);
                 ub = ck - 0.49f * lb * lb + 1e-6f * fabsf(ck) + kPruneNoise;
                 keep = !(ub < T);
             }

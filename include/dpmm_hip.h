@@ -261,6 +261,10 @@ int dpmm_label_contingency_static(const int32_t* d_labels, const int32_t* d_clas
 
 /* Kernel timings of the last calls (ms, HIP events on the library's stream). */
 int dpmm_last_kernel_ms(dpmm_ctx* ctx, float* label_sweep_ms, float* sub_sweep_ms, float* stats_ms);
+/* Duration of the last pruned mixed label sweep's main kernel alone (the BF16 screen, with the fused
+ * sub-label draw when on; ms, HIP events around its launch; 0 when the last sweep was not pruned or ran
+ * inside a graph capture).  No counterpart in the reference. */
+int dpmm_last_sweep_kernel_ms(dpmm_ctx* ctx, float* ms);
 int dpmm_synchronize(dpmm_ctx* ctx);
 
 /* ------------------------------------------------------------------ merge screener ----- */

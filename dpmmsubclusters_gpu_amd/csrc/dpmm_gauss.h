@@ -260,7 +260,7 @@ __global__ void __launch_bounds__(kThreads) k_stats_gauss_mfma(const TX* __restr
     int k, beg, end;
     if (!chunk_of(blockIdx.x, K, chunk_start, label_start, CH, k, beg, end)) return;
     if (dirty && !dirty[k]) return;   // statistics of this cluster are kept from the last pass
-    const int shift = stats_delta_shift(sg, k, label_start);   // >= 0: moved points only
+    const int shift = stats_delta_shift(sg, k, label_start);   // kFullPass: every point
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int kp = lane >> 4, r = lane & 15;
     int us[TPW], ca[TPW], cb[TPW];
@@ -297,7 +297,7 @@ __global__ void __launch_bounds__(kThreads) k_stats_gauss_mfma(const TX* __restr
     auto sub_at = [&](int pos, int id) {
         if (id < 0) return -1;
         const int sv = subs_perm ? subs_perm[pos] : subs[id];
-        return shift >= 0 && sg.subs_prev[pos + shift] == sv ? -1 : sv;   // moved points only
+        return shift != kFullPass && sg.subs_prev[pos + shift] == sv ? -1 : sv;   // moved points only
     };
     int sb1 = sub_at(beg + tid, id1);
     if (tid < B) idn[tid] = id1;
@@ -433,7 +433,7 @@ __global__ void __launch_bounds__(kThreads, sizeof(TX) == 4 ? 3 : 2) k_stats_gau
     int k, beg, end;
     if (!chunk_of(blockIdx.x, K, chunk_start, label_start, CH, k, beg, end)) return;
     if (dirty && !dirty[k]) return;   // statistics of this cluster are kept from the last pass
-    const int shift = stats_delta_shift(sg, k, label_start);   // >= 0: moved points only
+    const int shift = stats_delta_shift(sg, k, label_start);   // kFullPass: every point
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int kp = lane >> 4, r = lane & 15;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -441,9 +441,9 @@ __global__ void __launch_bounds__(kThreads, sizeof(TX) == 4 ? 3 : 2) k_stats_gau
     const int c0 = T * r;
     double* out = partials + (size_t)(2 * blockIdx.x) * E;
 
-    // update mode (shift >= 0): one pass over the points whose sub-label moved, each signed (+ into
+    // update mode (shift != kFullPass): one pass over the points whose sub-label moved, each signed (+ into
     // sub-cluster 0, - into 1), giving the change of (k, 0) in slot 0; full mode: one pass per sub-cluster
-    const bool upd = shift >= 0 && subs_perm;   // update mode reads the sub-labels in partition order
+    const bool upd = shift != kFullPass && subs_perm;   // update mode reads the sub-labels in partition order
     for (int s = 0; s < (upd ? 1 : 2); ++s, out += E) {
         f64x4 acc[NT];
         double sm[T];
@@ -681,7 +681,7 @@ __global__ void __launch_bounds__(kThreads) k_stats_gauss(const TX* __restrict__
     int k, beg, end;
     if (!chunk_of(c, K, chunk_start, label_start, CH, k, beg, end)) return;
     if (dirty && !dirty[k]) return;   // statistics of this cluster are kept from the last pass
-    const int shift = stats_delta_shift(sg, k, label_start);   // >= 0: moved points only
+    const int shift = stats_delta_shift(sg, k, label_start);   // kFullPass: every point
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     int bi[R], bj[R];
 #pragma unroll
@@ -709,7 +709,7 @@ __global__ void __launch_bounds__(kThreads) k_stats_gauss(const TX* __restrict__
             bool f = false;
             if (lane < BATCH && p < end) {
                 idx = perm[p];
-                f = (subs_perm ? subs_perm[p] : subs[idx]) == s && !(shift >= 0 && sg.subs_prev[p + shift] == s);
+                f = (subs_perm ? subs_perm[p] : subs[idx]) == s && !(shift != kFullPass && sg.subs_prev[p + shift] == s);
             }
             const uint64_t m = __ballot(f);
             if (f) idl[__popcll(m & lt)] = idx;

@@ -223,9 +223,11 @@ struct StatsGate {
     const int32_t* moved;
     const int32_t* prev_start;
 };
-// The offset of cluster k's points in the previous partition, or -1 when its chunks are summed in full.
+// The offset of cluster k's points in the previous partition (negative when its segment moved up), or
+// kFullPass when its chunks are summed in full.
+constexpr int kFullPass = INT32_MIN;
 __device__ __forceinline__ int stats_delta_shift(const StatsGate& g, int k, const int32_t* __restrict__ label_start) {
-    if (!g.subs_prev || g.moved[k]) return -1;
+    if (!g.subs_prev || g.moved[k]) return kFullPass;
     return g.prev_start[k] - label_start[k];
 }
 

@@ -101,3 +101,63 @@ def test_updated_statistics_equal_recomputed(case):
     for d, s in runs:
         s.close()
         d.close()
+
+
+def test_update_of_a_cluster_whose_segment_moved_later():
+    """Clusters 0 and 2 overlap, so every noisy sweep moves labels between them; cluster 1 lies far away
+    and keeps its points.  When cluster 0 gains points, cluster 1's segment of the new partition starts
+    later than in the partition swept (its offset into the previous partition is negative), and its
+    statistics are still updated from its moved sub-labels.  Fixed parameters, the same seed for the
+    updating and the recomputing context: counts equal, sums and scatter within 1e-12."""
+    N, D = 60_000, 32
+    rng = np.random.default_rng(17)
+    means = np.zeros((3, D))
+    means[1, :] = 40.0
+    means[2, 0] = 0.3   # overlaps cluster 0
+    gt = rng.integers(0, 3, N)
+    X = (means[gt] + rng.normal(size=(N, D))).astype(np.float32)
+    K = 3
+    mu = np.stack([X[gt == k].mean(0) for k in range(K)]).astype(np.float64)
+    lam = np.stack([np.linalg.inv(np.cov(X[gt == k].T.astype(np.float64))) for k in range(K)])
+    ld = np.array([-np.linalg.slogdet(l)[1] for l in lam])
+    w = np.bincount(gt, minlength=K) / N
+    smu = np.empty((2 * K, D))
+    for k in range(K):
+        v = rng.normal(size=D)
+        v *= 0.5 / np.linalg.norm(v)
+        smu[2 * k], smu[2 * k + 1] = mu[k] + v, mu[k] - v
+    params = (mu, lam, ld, w, smu, np.repeat(lam, 2, axis=0), np.repeat(ld, 2), np.full(2 * K, 0.5))
+    devs = []
+    for every in (8, 1):
+        dev = dp.DeviceSweep(X, dp.prior_type.Gaussian, seed=9)
+        dev.set_precision(dp.Precision.MIXED)
+        dev.set_label_pruning(True)
+        dev.set_statistics_refresh(every)
+        dev.set_gaussian_params(*params)
+        dev.set_labels(gt.astype(np.int32), None)
+        devs.append(dev)
+    du, df = devs
+    say("updated statistics: a cluster whose segment moved later")
+    later = 0
+    prev0 = None
+    for t in range(1, 9):
+        for d in devs:
+            d.sample_cluster_labels(t)
+            d.sample_sub_labels(t)
+        Nu, su_, Su = du.sufficient_statistics(K)
+        Nf, sf_, Sf = df.sufficient_statistics(K)
+        lu, uu = du.get_labels()
+        lf, uf = df.get_labels()
+        np.testing.assert_array_equal(lu, lf, err_msg=f"iteration {t}: labels")
+        np.testing.assert_array_equal(uu, uf, err_msg=f"iteration {t}: sub-labels")
+        np.testing.assert_array_equal(Nu, Nf, err_msg=f"iteration {t}: sub-cluster counts")
+        _close(su_, sf_, f"iteration {t}: sums")
+        _close(Su, Sf, f"iteration {t}: scatter")
+        n0 = int((lu == 0).sum())
+        if prev0 is not None and n0 > prev0 and du.last_statistics_update() == 2:
+            later += 1   # cluster 1 was updated with a negative offset
+        prev0 = n0
+    print(f"passes that updated cluster 1 after cluster 0 grew: {later}")
+    assert later >= 1
+    for d in devs:
+        d.close()

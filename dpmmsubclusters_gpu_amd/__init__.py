@@ -100,6 +100,7 @@ def load_library():
         "dpmm_last_kernel_ms": [_P, _P, _P, _P],
         "dpmm_last_sweep_kernel_ms": [_P, _P],
         "dpmm_set_mixed_tolerance": [_P, ctypes.c_float],
+        "dpmm_screen_error_factors": [ctypes.c_int, ctypes.POINTER(ctypes.c_float)],
         "dpmm_last_recheck_count": [_P, _P],
         "dpmm_last_sub_recheck_count": [_P, _P],
         "dpmm_set_label_pruning": [_P, i],
@@ -200,6 +201,14 @@ class ModelInfo:
         self.nmi_score_history = []
         self.weights = None
         self.cluster_counts = None
+
+
+def screen_error_factors(DP):
+    """Per-row error factors of the mixed screens at padded width DP: the single-product label screen's,
+    the split screens', and the relative sub-cluster screen's eps_a and eps_g (needs no GPU)."""
+    out = (ctypes.c_float * 4)()
+    _check(load_library().dpmm_screen_error_factors(int(DP), out))
+    return [float(v) for v in out]
 
 
 def _nmi(a, b):

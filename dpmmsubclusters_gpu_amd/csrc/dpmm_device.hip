@@ -2604,6 +2604,15 @@ int dpmm_set_mixed_tolerance(dpmm_ctx* c, float tau) {
     return DPMM_OK;
 }
 
+int dpmm_screen_error_factors(int DP, float* out) {
+    if (!out || (DP != 32 && DP != 64 && DP != 96 && DP != 128)) return fail(DPMM_EINVAL, "DP must be 32, 64, 96 or 128");
+    out[0] = label_screen_eps_of(DP);
+    out[1] = split_screen_eps(DP);
+    out[2] = sub_rel_eps_a(DP);
+    out[3] = sub_rel_eps_g(DP);
+    return DPMM_OK;
+}
+
 int dpmm_last_recheck_count(dpmm_ctx* c, int64_t* n) {
     if (!c || !n) return fail(DPMM_EINVAL, "bad args");
     *n = -1;

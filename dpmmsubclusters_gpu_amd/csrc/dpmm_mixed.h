@@ -10,9 +10,9 @@
 // L ^ 32, so q = sum of squares is one in-lane sum plus one cross-half add.
 //
 // Error bound against the FP32 sweep (k_label_gauss, which forms y_j = sum_l W_jl d_l by FMAs in
-// FP32 and q = sum_j y_j^2 the same way, with the same FP32 W, mu and c): BF16 rounding of W and of
-// d = x - mu is relative 2^-9 each, so every product W_jl d_l is off by at most (2^-8 + 2^-18)
-// |W_jl| |d_l|, and the screen's and the sweep's FP32 sums add at most (2 DP + 16) 2^-24 of
+// FP32 and q = sum_j y_j^2 the same way, with the same FP32 W, mu and c): BF16 rounding (8 significant
+// bits, to nearest) of W and of d = x - mu is relative 2^-8 each, so every product W_jl d_l is off by at
+// most (2^-7 + 2^-16) |W_jl| |d_l|, and the screen's and the sweep's FP32 sums add at most (2 DP + 16) 2^-24 of
 // a_j = sum_l |W_jl| |d_l|.  A second MFMA per tile, on the magnitudes |bf16(W)| |bf16(d)|, gives a_j
 // (inflated by 2^-6 for its own rounding), so |y_screen,j - y_fp32,j| <= eps a_j and, with
 // A = sum_j a_j^2 and sum_j |y_j| a_j <= sqrt(q A) (Cauchy-Schwarz),
@@ -99,11 +99,24 @@ __host__ __device__ constexpr int mx_slot_bytes(int DP) { return (mx_tiles_bytes
 // rounded up, kappa = sub_rel_eps_g / sub_rel_eps_a.
 __host__ __device__ constexpr int mx2_slot_bytes(int DP) { return 2 * mx_tiles_bytes(DP) + DP * 4 + 16 + 2 * DP * 4; }
 __host__ __device__ constexpr int mx2_rel_offset(int DP) { return 2 * mx_tiles_bytes(DP) + DP * 4 + 16; }
-// Per-row error factors of the relative sub-cluster screen (see sub_screen_rel): |y_screen,j - y_fp32,j| <=
-// eps_a a_j + eps_g g_j with a_j = sum_l |W_jl| |x_l - mu_a,l| from BF16 magnitudes (inflated by 2^-6).
-__host__ __device__ constexpr float sub_rel_eps_a(int DP) {
-    return (3.0f * 0x1p-18f + (5.0f * (float)DP + 24.0f) * 0x1p-24f) * (1.0f + 0x1p-6f);
+// BF16 has 8 significant bits: rounding to nearest is relative 2^-8, |v - bf16(v)| <= 2^-8 |v|.  An FP32
+// v split as v_hi = bf16(v), v_lo = bf16(v - v_hi) has |v_lo| <= 2^-8 (1 + 2^-8) |v| and leaves
+// |v - v_hi - v_lo| <= 2^-17 |v| (v - v_hi is exact, below half an ulp of v_hi, with at most 16 significant
+// bits).  Per-row factor of the split screens, |y_screen,j - y_fp32,j| <= eps a_j with a_j = sum_l |W_jl| |d_l|
+// from the BF16 magnitudes |W_hi| |d_hi| inflated by 2^-6 (which covers |W| <= |W_hi| / (1 - 2^-8), the same
+// for d, and the magnitude MFMA's rounding): the three products W_hi d_hi + W_hi d_lo + W_lo d_hi leave
+// W_lo d_lo + (W_hi + W_lo) delta_d + delta_W d, at most (2^-16 (1 + 2^-8)^2 + 2^-17 (2 + 2^-7)) |W| |d|
+// < (2^-15 + 2^-22) |W| |d| (attained to within 2^-6); the screen's FP32 sum of 3 DP products (their
+// magnitudes sum to at most (1 + 2^-7) |W| |d|), the sweep's sum of DP and the FP32 roundings of the
+// residuals add at most (4.03 DP + 2) 2^-24 of a_j.  (A fourth product, W_lo d_lo, would halve the first
+// term at 12 more MFMAs per 32 points of the fused sweep, which cost more than the extra FP32 draws.)
+__host__ __device__ constexpr float split_screen_eps(int DP) {
+    return (0x1p-15f + 0x1p-22f + (5.0f * (float)DP + 24.0f) * 0x1p-24f) * (1.0f + 0x1p-6f);
 }
+// Per-row error factors of the relative sub-cluster screen (fused_screens_split): |y_screen,j - y_fp32,j| <=
+// eps_a a_j + eps_g g_j, eps_a as above; eps_g covers the screen's FP32 sum started from -c_j (3 DP + 1
+// additions), the rounding of c_j and the sweep's residual and sum over |W| |mu_h - mu_a|.
+__host__ __device__ constexpr float sub_rel_eps_a(int DP) { return split_screen_eps(DP); }
 __host__ __device__ constexpr float sub_rel_eps_g(int DP) { return (3.0f * (float)DP + 16.0f) * 0x1p-24f * (1.0f + 0x1p-6f); }
 // Lane-order index of W[j][l] inside the tile array (row j, column l).
 __host__ __device__ constexpr int mx_tile_index(int DP, int j, int l) {
@@ -116,9 +129,10 @@ __host__ __device__ constexpr int mx_tile_index(int DP, int j, int l) {
 constexpr float kNoiseSlack = 1e-3f;
 
 // eps of the label screen's per-row bound |y_screen,j - y_fp32,j| <= eps a_j (see the header).
-template <int DP> __device__ __forceinline__ constexpr float label_screen_eps() {
-    return (0x1p-8f + 0x1p-18f + (2.0f * (float)DP + 16.0f) * 0x1p-24f) * (1.0f + 0x1p-6f);
+__host__ __device__ constexpr float label_screen_eps_of(int DP) {
+    return (0x1p-7f + 0x1p-16f + (2.0f * (float)DP + 16.0f) * 0x1p-24f) * (1.0f + 0x1p-6f);
 }
+template <int DP> __device__ __forceinline__ constexpr float label_screen_eps() { return label_screen_eps_of(DP); }
 // e_k from q = sum y_j^2, A = sum a_j^2 and the score constant c (sqrt rounding covered by 2^-10).
 template <int DP> __device__ __forceinline__ float label_screen_err(float tau, float q, float A, float c) {
     constexpr float eps = label_screen_eps<DP>();
@@ -267,9 +281,8 @@ __device__ __forceinline__ float split_score(const float (&x)[DP / 16][8], const
     typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
     constexpr int RB = DP / 32, KC = DP / 16;
     const int h = lane >> 5;
-    // |y_screen - y_fp32| <= eps a_j: split residuals (3 x 2^-18 of |W||d|) and the FP32 sums of the
-    // screen (4 D terms) and of the sweep (D terms), with a_j from BF16 magnitudes inflated by 2^-6
-    const float eps = (3.0f * 0x1p-18f + (5.0f * (float)DP + 16.0f) * 0x1p-24f) * (1.0f + 0x1p-6f);
+    // |y_screen - y_fp32| <= eps a_j (split_screen_eps)
+    constexpr float eps = split_screen_eps(DP);
     const float heps = 0.5f * eps;
     // d = x - mu split as d_hi + d_lo: one packed conversion per coordinate pair gives both BF16 highs,
     // read back as FP32 by shifting / masking the packed word
@@ -564,7 +577,7 @@ __device__ __forceinline__ float split_score_lean(gfloat* row, int D, bool ok, c
     typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
     constexpr int RB = DP / 32, KC = DP / 16;
     const int h = lane >> 5;
-    const float eps = (3.0f * 0x1p-18f + (5.0f * (float)DP + 16.0f) * 0x1p-24f) * (1.0f + 0x1p-6f);
+    constexpr float eps = split_screen_eps(DP);
     const float heps = 0.5f * eps;
     float qq = 0.f, ee = 0.f;
 #pragma unroll 1
@@ -852,7 +865,7 @@ k_label_gauss_mxp(const float* __restrict__ X, int D, int K, int CH, const int32
         nev += ne;
         ++npass;
         bool sure = sb - eb > U;
-        // a wave with undecided points is scored again in split BF16 (refine_wave, bounds some 2^-9 of the
+        // a wave with undecided points is scored again in split BF16 (split_score_lean, bounds some 2^-7 of the
         // BF16 screen's); a point whose best refined lower bound beats every other refined upper bound
         // and the pruned clusters' bound Up takes that label (the FP32 sweep's: its score is the strict
         // maximum).  Siblings of a split or an over-split start leave most points within the BF16

@@ -188,6 +188,12 @@ int dpmm_get_drawn_params(dpmm_ctx* ctx, double* mu, double* chol_inv_sigma, dou
  * scales the bound: 1 (default) the derived bound, 0 keeps every screened label (a diagnostic of the
  * raw BF16 screen), up to 16 widens it.  No counterpart in the reference (whose sweep is FP64). */
 int dpmm_set_mixed_tolerance(dpmm_ctx* ctx, float tau);
+/* The per-row error factors of the mixed screens for padded width DP (32, 64, 96 or 128), in out[0..3]:
+ * the single-product label screen's, the split screens' (four hi / lo products), and the relative
+ * sub-cluster screen's eps_a and eps_g.  Each screen keeps |y_screen,j - y_fp32,j| within its factor
+ * times sum_l |W_jl| |d_l| (eps_g: times sum_l |W_jl| |mu_h,l - mu_a,l| + |c_j|).  Needs no GPU.  No
+ * counterpart in the reference (its sweeps have no reduced-precision screen). */
+int dpmm_screen_error_factors(int DP, float* out);
 /* Number of points the last mixed label sweep re-scored in FP32 (-1 if none ran). */
 int dpmm_last_recheck_count(dpmm_ctx* ctx, int64_t* n);
 /* Number of points whose sub-label the last mixed sub-label sweep drew in FP32 (-1 if none ran).
@@ -227,7 +233,7 @@ int dpmm_set_fused_sub_labels(dpmm_ctx* ctx, int enable);
 /* Split-BF16 refinement in the pruned mixed label sweep (on by default; no counterpart in the
  * reference).  A wave whose BF16 screen leaves points undecided scores the cluster they were filed
  * under and every cluster the prune test keeps again with W and x - mu split into BF16 highs and lows
- * (three MFMAs) and a bound some 2^-9 of the screen's; a point takes the best cluster when its lower
+ * (three MFMAs) and a bound some 2^-7 of the screen's; a point takes the best cluster when its lower
  * bound beats every other upper bound, which makes it the FP32 sweep's label.  Near-ties between
  * sibling clusters (after a split, or an over-split start) then rarely reach the FP32 recheck.
  * enable 0: every undecided point goes to the FP32 recheck. */

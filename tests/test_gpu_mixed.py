@@ -335,7 +335,9 @@ def test_refined_sibling_ties_equal_fp32(D, K):
         dev.set_precision(dp.Precision.FP32)
     print(f"D={D} K={K2}: FP32 rechecks {counts[True]} with the refinement, {counts[False]} without, of {N}")
     assert counts[False] > N // 10, "the siblings should defeat the BF16 screen"
-    assert counts[True] * 10 < counts[False]
+    # at D = 128 the FP32 sums' share of the split bound, (5 DP + 24) 2^-24, exceeds the split's own
+    # 2^-15, so fewer ties resolve there (about 10 % of the points stay undecided)
+    assert counts[True] * (10 if D <= 64 else 5) < counts[False]
     olab, margin = orc.label_sweep(0, X, params[0], params[1], params[2], weights=params[3], seed=12, it=5)
     assert not ((ref != olab) & (margin >= 1e-6)).any()
     dev.close()

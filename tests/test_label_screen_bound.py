@@ -2,14 +2,16 @@
 emulation of its arithmetic.  The screen forms y = bf16(W) bf16(x - mu) (products of BF16 values are
 exact in FP32; here they are summed in FP64, within the bound's accumulation term of the MFMA's FP32
 sum) and takes q = |y|^2; the FP32 sweep's q is |W (x - mu)|^2 with W and mu in FP32 (here in FP64 from
-those FP32 values).  With a_j = sum_l |bf16(W_jl)| |bf16(d_l)|, A = sum_j a_j^2 and
-eps = (2^-8 + 2^-18 + (2 DP + 16) 2^-24)(1 + 2^-6), half the difference of the two q must stay below
+those FP32 values).  With a_j = sum_l |bf16(W_jl)| |bf16(d_l)|, A = sum_j a_j^2 and the library's
+factor eps = (2^-7 + 2^-16 + (2 DP + 16) 2^-24)(1 + 2^-6) (dpmm_screen_error_factors), half the
+difference of the two q must stay below
 eps (sqrt(q A) + eps A / 2), whatever the covariance.  The heuristic the screen used before, q / 32,
 fails on strongly correlated, ill-conditioned covariances; the case is kept as the reason for the
 derived bound."""
 import numpy as np
 import pytest
 
+import dpmmsubclusters_gpu_amd as dp
 from helpers import adversarial_mixture
 
 
@@ -22,7 +24,7 @@ def bf16(a):
 
 
 def screen_errors(X, mu, lam, D):
-    eps = (2.0 ** -8 + 2.0 ** -18 + (2 * D + 16) * 2.0 ** -24) * (1 + 2.0 ** -6)
+    eps = dp.screen_error_factors(D)[0]
     W32 = np.linalg.cholesky(lam).T.astype(np.float32)   # W = R^T, invSigma = R R^T
     Wb = bf16(W32).astype(np.float64)
     mu32 = mu.astype(np.float32)

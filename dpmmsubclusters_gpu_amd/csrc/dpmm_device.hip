@@ -946,7 +946,7 @@ struct dpmm_ctx {
     // moved and dFlags[1 + k] for both clusters of every move; dSubsPermPrev keeps the previous pass's
     // sub-labels in the order of its partition, whose segment starts dPrevStart keeps.  inc_base: dStats
     // holds the statistics of the current partition with the sub-labels in dSubsPerm.
-    int* dFlags = nullptr;
+    int* dFlags = nullptr;     // then, from dFlags + 1 + K, the fused sweep's 2K sub-label list lengths
     size_t flagsCap = 0;
     int32_t* dSubsPermPrev = nullptr;
     size_t subsPermPrevCap = 0;
@@ -1582,12 +1582,14 @@ template <int DP> void sgl_f32(dpmm_ctx* c, uint32_t it, int grid) {
                                      c->dRecheckCount + 1, c->key0(), c->key1(), it, c->dSubs, c->dPerm,
                                      c->dSubsPerm);
 }
-// the list of the fused sweep: positions in the partition it swept, sub-labels into dSubsOld
-template <int DP> void sglf_f32(dpmm_ctx* c, uint32_t it, int grid) {
-    launch_sub_gauss_list<float, DP>(c->stream, grid, reinterpret_cast<const float*>(c->dX), c->D, c->K,
-                                     reinterpret_cast<const float*>(c->dSP), c->dLabels, c->dSubList,
-                                     c->dRecheckCount + 1, c->key0(), c->key1(), it, c->dSubs, c->dPerm,
-                                     c->dSubsOld);
+// the lists of the fused sweep (per workgroup of its grid): positions in the partition it swept,
+// sub-labels into dSubsOld
+template <int DP> void sgsf_f32(dpmm_ctx* c, uint32_t it) {
+    static_assert(2 * kMaxK <= kSegMaxLists, "list counts of the fused sweep");
+    launch_sub_gauss_seg<float, DP>(c->stream, 1024, reinterpret_cast<const float*>(c->dX), c->D, c->K, c->dPerm,
+                                    c->dLabelStart, reinterpret_cast<const float*>(c->dSP), c->dLabels,
+                                    c->dSubList, c->dFlags + 1 + c->K, c->key0(), c->key1(), it, c->dSubs,
+                                    c->dSubsOld, c->dRecheckCount + 1);
 }
 template <int DPM> void sgm(dpmm_ctx* c, uint32_t it) {
     launch_sub_gauss_mx<DPM>(c->stream, chunks_upper(c, c->K), reinterpret_cast<const float*>(c->dX), c->D, c->K,
@@ -1666,6 +1668,11 @@ int ensure_prune(dpmm_ctx* c) {
     return DPMM_OK;
 }
 
+// Workgroups per partition chunk of the pruned mixed sweep.
+static int sweep_split(const dpmm_ctx* c, bool fuse) {
+    return c->prune_split > 0 ? c->prune_split : fuse ? c->prune_split_fused : kPruneSplit;
+}
+
 // Mixed sweep with exact pruning: partition by the labels before the sweep (labels outside [0, K)
 // filed under 0), the pruned BF16 screen over the partition's chunks, then the FP32 sweep over the
 // points it could not decide.
@@ -1677,22 +1684,24 @@ int run_label_mixed_pruned(dpmm_ctx* c, uint32_t iter, int fin) {
     const bool reuse = c->partition_valid && c->Kpartition == c->K;
     if (!reuse && (rc = build_partition(c, c->K, 1))) return rc;
     if ((rc = grow(c->dEvals, c->evalsCap, 2))) return rc;
-    if ((rc = grow(c->dFlags, c->flagsCap, (size_t)kMaxK + 1))) return rc;
+    if ((rc = grow(c->dFlags, c->flagsCap, (size_t)3 * kMaxK + 1))) return rc;
     // the sub-label screen rides along (its slots hold this iteration's sub-cluster parameters)
     const bool fuse = c->fuse_mode != 0 && c->DPM <= 64 && c->spb_valid;
     if (fuse) {
         if ((rc = grow(c->dSubsOld, c->subsOldCap, (size_t)c->N))) return rc;
         if ((rc = grow(c->dSubList, c->subListCap, (size_t)c->N))) return rc;
     }
-    // dFlags[0]: a label moved; dFlags[1 + k]: a label moved into or out of cluster k
-    hipLaunchKernelGGL(k_zero_int, dim3((c->K + 64) / 64), dim3(64), 0, c->stream, c->dFlags, c->K + 1);
+    // dFlags[0]: a label moved; dFlags[1 + k]: a label moved into or out of cluster k; fused, then the
+    // lengths of the 2K sub-label lists
+    const int nflags = fuse ? 3 * c->K + 1 : c->K + 1;
+    hipLaunchKernelGGL(k_zero_int, dim3((nflags + 63) / 64), dim3(64), 0, c->stream, c->dFlags, nflags);
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, c->dRecheckCount, 2);
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, reinterpret_cast<int*>(c->dEvals), 4);
     const float* X = reinterpret_cast<const float*>(c->dX);
     const unsigned char* PB = reinterpret_cast<const unsigned char*>(c->dPB);
     MxpExtra sf{};
     sf.skip_same = reuse ? 1 : 0;
-    sf.split = c->prune_split > 0 ? c->prune_split : fuse ? c->prune_split_fused : kPruneSplit;
+    sf.split = sweep_split(c, fuse);
     sf.PBlo = c->refine_mode && c->spb_valid ? reinterpret_cast<const unsigned char*>(c->dPBlo) : nullptr;
     const int grid = chunks_upper(c, c->K) * sf.split;
     if (fuse) {
@@ -1702,6 +1711,7 @@ int run_label_mixed_pruned(dpmm_ctx* c, uint32_t iter, int fin) {
         sf.subs = c->dSubs;
         sf.subs_old = c->dSubsOld;
         sf.sub_list = c->dSubList;
+        sf.seg = c->dFlags + 1 + c->K;
         sf.sub_count = c->dRecheckCount + 1;
     }
 #define DPMM_MXP(DPMV)                                                                                          \
@@ -1722,7 +1732,7 @@ int run_label_mixed_pruned(dpmm_ctx* c, uint32_t iter, int fin) {
     const int grid2 = (int)std::min<int64_t>(1024, (c->N + kThreads - 1) / kThreads);
     DPMM_DISPATCH_DP(c->DP, lglp_f32, c, iter, fin, grid2);   // the recheck pruned with the same bounds
     // fused: the FP32 sub-label draw of the listed positions, by their final labels (after the recheck)
-    if (fuse) DPMM_DISPATCH_DP(c->DP, sglf_f32, c, iter, grid2);
+    if (fuse) DPMM_DISPATCH_DP(c->DP, sgsf_f32, c, iter);
     c->fused_subs = fuse;
     c->fused_iter = iter;
     c->last_pruned_K = c->K;
@@ -2185,7 +2195,7 @@ static int enqueue_stats(dpmm_ctx* c, int K, const int32_t* dev_mask = nullptr) 
             }
             if ((rc = grow(c->dSubsPermPrev, c->subsPermPrevCap, (size_t)c->N))) return rc;
             if ((rc = grow(c->dPrevStart, c->prevStartCap, (size_t)kMaxK + 1))) return rc;
-            if ((rc = grow(c->dFlags, c->flagsCap, (size_t)kMaxK + 1))) return rc;
+            if ((rc = grow(c->dFlags, c->flagsCap, (size_t)3 * kMaxK + 1))) return rc;
             hipLaunchKernelGGL(k_refresh_subs_perm, dim3(nch), dim3(kThreads), 0, c->stream, K, c->CH, c->dPerm,
                                c->dLabelStart, c->dChunkStart, c->dSubs, mask, c->dSubsPerm, c->dSubsPermPrev);
             hipLaunchKernelGGL(k_copy_int, dim3((K + 64) / 64), dim3(64), 0, c->stream, c->dLabelStart, c->dPrevStart,
@@ -2318,7 +2328,7 @@ static int ensure_sweep_buffers(dpmm_ctx* c) {
         if ((rc = grow(c->dEvals, c->evalsCap, 2))) return rc;
         if ((rc = grow(c->dSubsPerm, c->subsPermCap, (size_t)c->N))) return rc;
         if ((rc = grow(c->dSubsPermPrev, c->subsPermPrevCap, (size_t)c->N))) return rc;
-        if ((rc = grow(c->dFlags, c->flagsCap, (size_t)kMaxK + 1))) return rc;
+        if ((rc = grow(c->dFlags, c->flagsCap, (size_t)3 * kMaxK + 1))) return rc;
         if ((rc = grow(c->dPrevStart, c->prevStartCap, (size_t)kMaxK + 1))) return rc;
         if (c->fuse_mode != 0 && c->DPM <= 64) {
             if ((rc = grow(c->dSubsOld, c->subsOldCap, (size_t)c->N))) return rc;

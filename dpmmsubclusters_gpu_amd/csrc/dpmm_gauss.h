@@ -618,6 +618,140 @@ __global__ void __launch_bounds__(kThreads) k_sub_gauss_list(const T* __restrict
     }
 }
 
+// FP32 sub-label draw of the points the fused label + sub-label sweep (k_label_gauss_mxp) left, listed
+// per cluster inside the cluster's segment [label_start[a], label_start[a + 1]) of the swept partition:
+// seg[a] points whose label stayed a from the segment's start up, seg[K + a] others (label moved, or
+// decided by the FP32 label recheck) from its end down.  The 2K lists are cut into pieces of kThreads
+// points (one per thread) and each workgroup takes a contiguous range of the pieces, so a piece of a
+// first list shares a's two sub-cluster slots, staged in LDS once per run of pieces of one cluster;
+// a lane draws its point with the arithmetic of k_sub_gauss (whitened_sq_lane), for the second lists
+// from its own label's slots in global memory.  Sub-labels go to subs (by index) and subs_perm (by
+// position); labels outside [0, K) are left alone.  *total: how many points were listed.
+constexpr int kSegMaxLists = 4096;   // 2 K
+template <typename T, int DP>
+__global__ void __launch_bounds__(kThreads) k_sub_gauss_seg(const T* __restrict__ X, int D, int K,
+                                                            const int32_t* __restrict__ perm,
+                                                            const int32_t* __restrict__ label_start,
+                                                            const T* __restrict__ SP,
+                                                            const int32_t* __restrict__ labels,
+                                                            const int32_t* __restrict__ list,
+                                                            const int32_t* __restrict__ seg, uint32_t key0,
+                                                            uint32_t key1, uint32_t iter, int32_t* __restrict__ subs,
+                                                            int32_t* __restrict__ subs_perm, int* __restrict__ total) {
+    constexpr int stride = gauss_slot_stride(DP);
+    constexpr bool kLds = sizeof(T) == 4 && DP <= 64;   // the fused sweep's widths
+    __shared__ __align__(16) T ws[kLds ? 2 * stride : 4];
+    __shared__ int P[kSegMaxLists + 1];   // exclusive prefix of the lists' piece counts
+    __shared__ int wsum[2][kThreads / 64];
+    const int nl = 2 * K, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    // pieces per list, scanned over the workgroup: thread t owns lists [t m, t m + m)
+    const int m = (nl + kThreads - 1) / kThreads;
+    int pcs = 0, ents = 0;
+    for (int u = 0; u < m; ++u) {
+        const int t = tid * m + u;
+        const int c = t < nl ? seg[t] : 0;
+        pcs += (c + kThreads - 1) / kThreads;
+        ents += c;
+    }
+    int inc = pcs, tot = ents;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(inc, o);
+        if (lane >= o) inc += v;
+        tot += __shfl_xor(tot, o);
+    }
+    if (lane == 63) wsum[0][wv] = inc;
+    if (lane == 0) wsum[1][wv] = tot;
+    __syncthreads();
+    int off = 0;
+    for (int w = 0; w < wv; ++w) off += wsum[0][w];
+    int run = off + inc - pcs;
+    for (int u = 0; u < m; ++u) {
+        const int t = tid * m + u;
+        if (t < nl) {
+            P[t] = run;
+            const int c = seg[t];
+            run += (c + kThreads - 1) / kThreads;
+        }
+    }
+    if (tid == kThreads - 1) P[nl] = run;
+    __syncthreads();
+    if (blockIdx.x == 0 && tid == 0) {
+        int e = 0;
+        for (int w = 0; w < kThreads / 64; ++w) e += wsum[1][w];
+        *total = e;
+    }
+    const int npieces = P[nl];
+    const int q0 = (int)((int64_t)npieces * blockIdx.x / gridDim.x);
+    const int q1 = (int)((int64_t)npieces * (blockIdx.x + 1) / gridDim.x);
+    int staged = -1;
+    for (int q = q0; q < q1; ++q) {   // uniform over the workgroup
+        int lo = 0, hi = nl;          // the list holding piece q: the last t with P[t] <= q
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (P[mid] <= q) lo = mid;
+            else hi = mid;
+        }
+        const bool stays = lo < K;
+        const int a = stays ? lo : lo - K;
+        const int i = (q - P[lo]) * kThreads + tid;
+        const bool in = i < seg[lo];
+        if constexpr (kLds) {
+            if (stays && staged != a) {
+                __syncthreads();   // the last piece's readers are done
+                const float4* src = reinterpret_cast<const float4*>(SP + (size_t)(2 * a) * stride);
+                float4* dst = reinterpret_cast<float4*>(ws);
+                static_assert(stride % 4 == 0, "16-byte slot rows");
+                for (int e = tid; e < 2 * stride / 4; e += kThreads) dst[e] = src[e];
+                __syncthreads();
+                staged = a;
+            }
+        }
+        const int ent = in ? (stays ? list[label_start[a] + i] : list[label_start[a + 1] - 1 - i]) : 0;
+        const int idx = in ? perm[ent] : 0;
+        int k = a;
+        bool ok = in;
+        if (!stays) {
+            const int k0 = in ? labels[idx] : 0;
+            ok = in && (unsigned)k0 < (unsigned)K;   // labels outside [0, K) are left alone
+            k = ok ? k0 : 0;
+        }
+        T sc[2];
+#pragma unroll 1
+        for (int h = 0; h < 2; ++h) {
+            // the coordinates are read for each sub-cluster (the second time from cache): kept across
+            // both, they would hold 64 more registers
+            T x[DP];
+            if constexpr (sizeof(T) == 4) {
+#pragma unroll
+                for (int l = 0; l < DP; l += 4) {
+                    const bool c = ok && l < D;   // D % 4 == 0 (the fused sweep's condition): 16-byte rows
+                    const float4 v = *reinterpret_cast<const float4*>(X + (int64_t)(c ? idx : 0) * D + (c ? l : 0));
+                    x[l] = c ? v.x : T(0);
+                    x[l + 1] = c ? v.y : T(0);
+                    x[l + 2] = c ? v.z : T(0);
+                    x[l + 3] = c ? v.w : T(0);
+                }
+            } else {
+#pragma unroll
+                for (int l = 0; l < DP; ++l) x[l] = (ok && l < D) ? X[(int64_t)idx * D + l] : T(0);
+            }
+            if (kLds && stays) {
+                const T* pp = ws + h * stride;
+                sc[h] = pp[gauss_c_offset(DP)] - T(0.5) * whitened_sq_lane<T, DP>(pp, x);
+            } else {
+                const T* pp = SP + (size_t)(2 * k + h) * stride;
+                sc[h] = pp[gauss_c_offset(DP)] - T(0.5) * whitened_sq_lane<T, DP>(pp, x);
+            }
+        }
+        if (ok) {
+            const uint32_t w = philox4x32_10((uint32_t)idx, 0, iter, TAG_SUB, key0, key1).v[0];
+            const int v = (logit_u32<T>(w) < sc[0] - sc[1]) ? 0 : 1;
+            subs[idx] = v;
+            subs_perm[ent] = v;
+        }
+    }
+}
+
 // Noise-free scores of points [first, first + n) with the sweeps' arithmetic (the FP32 / FP64
 // expression c - q / 2 of k_label_gauss and k_sub_gauss): which = 0 writes the n x K matrix of
 // cluster scores (log weight + log N(x | mu_k, Sigma_k)), which = 1 the n x 2 scores of the two
@@ -794,6 +928,15 @@ void launch_sub_gauss_list(hipStream_t st, int grid, const T* X, int D, int K, c
 }
 
 template <typename T, int DP>
+void launch_sub_gauss_seg(hipStream_t st, int grid, const T* X, int D, int K, const int32_t* perm,
+                          const int32_t* label_start, const T* SP, const int32_t* labels, const int32_t* list,
+                          const int32_t* seg, uint32_t key0, uint32_t key1, uint32_t iter, int32_t* subs,
+                          int32_t* subs_perm, int* total) {
+    hipLaunchKernelGGL((k_sub_gauss_seg<T, DP>), dim3(grid), dim3(kThreads), 0, st, X, D, K, perm, label_start, SP,
+                       labels, list, seg, key0, key1, iter, subs, subs_perm, total);
+}
+
+template <typename T, int DP>
 void launch_sub_gauss(hipStream_t st, int grid, const T* X, int D, int K, int CH, const int32_t* perm,
                       const int32_t* label_start, const int32_t* chunk_start, const T* SP, uint32_t key0,
                       uint32_t key1, uint32_t iter, int32_t* subs) {
@@ -842,6 +985,10 @@ void launch_stats_gauss(hipStream_t st, int nchunks, const TX* X, int D, int K, 
                                                int32_t*, const int32_t*, int32_t*);                            \
     template void launch_score_gauss<T, DP>(hipStream_t, int, const T*, int, int, const T*, const T*,          \
                                             const int32_t*, int64_t, int64_t, int, T*);                        \
+    template void launch_sub_gauss_seg<T, DP>(hipStream_t, int, const T*, int, int, const int32_t*,            \
+                                              const int32_t*, const T*, const int32_t*, const int32_t*,       \
+                                              const int32_t*, uint32_t, uint32_t, uint32_t, int32_t*,         \
+                                              int32_t*, int*);                                                 \
     template void launch_label_gauss_listp<T, DP>(hipStream_t, int, const T*, int64_t, int, int, const T*,     \
                                                   const float*, const float*, uint32_t, uint32_t, uint32_t, int, \
                                                   const int32_t*, const int*, int32_t*, int*);
@@ -863,6 +1010,10 @@ void launch_stats_gauss(hipStream_t st, int nchunks, const TX* X, int D, int K, 
                                                       uint32_t, uint32_t, int32_t*, const int32_t*, int32_t*); \
     extern template void launch_score_gauss<T, DP>(hipStream_t, int, const T*, int, int, const T*, const T*,   \
                                                    const int32_t*, int64_t, int64_t, int, T*);                 \
+    extern template void launch_sub_gauss_seg<T, DP>(hipStream_t, int, const T*, int, int, const int32_t*,    \
+                                                     const int32_t*, const T*, const int32_t*, const int32_t*, \
+                                                     const int32_t*, uint32_t, uint32_t, uint32_t, int32_t*,   \
+                                                     int32_t*, int*);                                          \
     extern template void launch_label_gauss_listp<T, DP>(hipStream_t, int, const T*, int64_t, int, int,        \
                                                          const T*, const float*, const float*, uint32_t,       \
                                                          uint32_t, uint32_t, int, const int32_t*, const int*,  \

@@ -136,6 +136,92 @@ __device__ __forceinline__ T whitened_sq_pair(const T* __restrict__ ps, const T 
     return q;
 }
 
+// whitened_sq_regs's arithmetic for a slot `ps` that is the same across the wave (its rows become scalar
+// loads, SGPR operands of the FMAs): one row per trip of a loop that is not unrolled, so that only that
+// row's loads are in flight; row j's terms run over columns (j & ~3) .. DP - 1 in order.  The rows of a
+// 4-block share their first column, which keeps the column loop's bounds constant.
+template <typename T, int DP>
+__device__ __forceinline__ T whitened_sq_uniform(const T* __restrict__ ps, const T (&x)[DP]) {
+    T d[DP];
+#pragma unroll
+    for (int l = 0; l < DP; ++l) d[l] = x[l] - ps[gauss_mu_offset(DP) + l];
+    T q = 0;
+#pragma unroll
+    for (int l0 = 0; l0 < DP; l0 += 4) {   // the four rows j = l0 .. l0 + 3 share their first column
+#pragma unroll 1
+        for (int j = l0; j < l0 + 4; ++j) {
+            const T* row = ps + tri_row_offset(DP, j) - l0;
+            T y = 0;
+#pragma unroll
+            for (int l = l0; l < DP; ++l) y = fma(row[l], d[l], y);
+            q = fma(y, y, q);
+        }
+    }
+    return q;
+}
+
+// One lane's q with the arithmetic of the FP32 sweep's layout for DP: whitened_sq_regs's, or for a paired
+// layout both halves of whitened_sq_pair formed in this lane, each in the order its lane of the pair
+// uses, and added as the pair adds them (the same value in either lane).  x: the point's DP
+// coordinates, zero past D.
+template <typename T, int DP>
+__device__ __forceinline__ T whitened_sq_lane(const T* __restrict__ ps, const T (&x)[DP]) {
+    if constexpr (!paired_layout<T, DP>()) {
+        // whitened_sq_regs's arithmetic, one row's loads at a time (hoisted over the whole triangle
+        // they would take every register)
+        T d[DP];
+#pragma unroll
+        for (int l = 0; l < DP; ++l) d[l] = x[l] - ps[gauss_mu_offset(DP) + l];
+        T q = 0;
+#pragma unroll
+        for (int j = 0; j < DP; ++j) {
+            const int l0 = j & ~3;
+            const T* row = ps + tri_row_offset(DP, j) - l0;
+            T y = 0;
+#pragma unroll
+            for (int l = l0; l < DP; ++l) y = fma(row[l], d[l], y);
+            q = fma(y, y, q);
+            asm volatile("" ::: "memory");
+        }
+        return q;
+    } else {
+        constexpr int HC = DP / 2;
+        const T* mu = ps + gauss_mu_offset(DP);
+        T d[2][HC];
+#pragma unroll
+        for (int par = 0; par < 2; ++par)
+#pragma unroll
+            for (int c = 0; c < HC; ++c) d[par][c] = x[pair_col(c, 0) + (par << 2)] - mu[pair_col(c, 0) + (par << 2)];
+        T q = 0;
+#pragma unroll
+        for (int j = 0; j < DP; ++j) {
+            const int jb = j >> 2;
+            T yh[2];
+#pragma unroll
+            for (int par = 0; par < 2; ++par) {
+                const T* row = ps + tri_row_offset(DP, j) - (j & ~3) + (par << 2);
+                const T keep = par ? T(1) : T(0);
+                T y = 0;
+#pragma unroll
+                for (int b = 0; b < HC / 4; ++b) {
+                    if (2 * b + 1 < jb) continue;
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        T dv = d[par][4 * b + u];
+                        if (2 * b + 1 == jb) dv *= keep;
+                        y = fma(row[(b << 3) + u], dv, y);
+                    }
+                }
+                yh[par] = y;
+            }
+            const T y = yh[0] + yh[1];
+            q = fma(y, y, q);
+            asm volatile("" ::: "memory");   // one row's loads at a time, as above
+        }
+        return q;
+    }
+}
+
 template <typename T, int DP>
 __device__ __forceinline__ T whitened_sq(const T* __restrict__ ps, const T (&x)[DP]) {
     return whitened_sq_regs<T, DP>(ps, x);

@@ -69,21 +69,23 @@ struct WaveList {
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
-    __device__ __forceinline__ void flush(int32_t* list, int* count, int lane) {
+    // entry i of the list goes to list[dir * i] (dir -1: a list that grows down from `list`)
+    __device__ __forceinline__ void flush(int32_t* list, int* count, int lane, int dir = 1) {
         if (n == 0) return;
         sync_wave();   // the lanes' appends are visible to every lane
         int pos0 = 0;
         if (lane == 0) pos0 = atomicAdd(count, n);
         pos0 = __shfl(pos0, 0);
-        if (lane < n) list[pos0 + lane] = buf[lane];
+        if (lane < n) list[dir * (pos0 + lane)] = buf[lane];
         sync_wave();   // read before the next appends overwrite
         n = 0;
     }
-    __device__ __forceinline__ void push(bool need, int32_t v, int32_t* list, int* count, int lane, uint64_t lt) {
+    __device__ __forceinline__ void push(bool need, int32_t v, int32_t* list, int* count, int lane, uint64_t lt,
+                                         int dir = 1) {
         const uint64_t m = __ballot(need);
         if (!m) return;
         const int c = __popcll(m);
-        if (n + c > 64) flush(list, count, lane);
+        if (n + c > 64) flush(list, count, lane, dir);
         if (need) buf[n + __popcll(m & lt)] = v;
         n += c;
     }
@@ -625,9 +627,11 @@ __device__ __forceinline__ float split_score_lean(gfloat* row, int D, bool ok, c
 
 // The sub-label screen fused into the pruned sweep.  A point whose label stays its chunk's cluster
 // (the anchor) is screened against the anchor's two sub-clusters in the same pass, with the
-// arithmetic and bound of k_sub_gauss_mx; the points whose label moved or went to the recheck, and
-// those whose sub-label the screen leaves undecided, are listed (positions in the swept partition)
-// for the FP32 draw after the label recheck.  skip_same (both forms): every label of a chunk equals
+// arithmetic and bound of k_sub_gauss_mx; the points whose sub-label the screen leaves undecided, and
+// those whose label moved or went to the recheck, are listed (positions in the swept partition) for
+// the FP32 draw after the label recheck (k_sub_gauss_seg): they are listed inside the anchor's segment
+// of the swept partition, the first kind from its start up (count seg[a]), the second from its end
+// down (count seg[K + a]).  skip_same (both forms): every label of a chunk equals
 // its anchor before the sweep, so a label that stays is not written again.  (Extra arguments of
 // k_label_gauss_mxp.)
 struct MxpExtra {
@@ -635,8 +639,9 @@ struct MxpExtra {
     const int32_t* subs_prev;    // sub-labels before the sweep in the swept partition's order, or null
     int32_t* subs;               // sub-labels by point index, written where they change
     int32_t* subs_old;           // sub-labels in the swept partition's order
-    int32_t* sub_list;           // positions left to the FP32 sub-label draw
-    int* sub_count;
+    int32_t* sub_list;           // positions left to the FP32 sub-label draw (N entries)
+    int* sub_count;              // how many (written by k_sub_gauss_seg)
+    int32_t* seg;                // 2K list lengths: per cluster, listed points whose label stayed, then the others
     int skip_same;
     int split;                   // workgroups per partition chunk (the grid is chunks x split)
     const unsigned char* PBlo;   // the clusters' lo tiles (refinement of undecided waves), or null
@@ -698,7 +703,10 @@ k_label_gauss_mxp(const float* __restrict__ X, int D, int K, int CH, const int32
     __shared__ int32_t wlist[4][64];
     WaveList rl{wlist[wv], 0};
     __shared__ int32_t wlist2[FUSE ? 4 : 1][64];
-    WaveList sl{wlist2[FUSE ? wv : 0], 0};
+    WaveList sl{wlist2[FUSE ? wv : 0], 0};       // fused: undecided sub-labels of points that stay
+    __shared__ int32_t wlist3[FUSE ? 4 : 1][64];
+    WaveList ml{wlist3[FUSE ? wv : 0], 0};       // fused: points whose label moved or is rechecked
+
     // the coordinates 16 kc + 8 h .. + 7 of point `pt` (row 0 stands in for an empty lane)
     auto load_row = [&](gfloat* Xs, int pt, bool ok, float (&x)[KC][8]) {
         gfloat* row = Xs + (int64_t)pt * D;
@@ -922,7 +930,7 @@ k_label_gauss_mxp(const float* __restrict__ X, int D, int K, int CH, const int32
             // sub-labels of the points whose label stays the anchor, from its sub-clusters' screen; the others
             // (labels moved or left to the recheck, undecided sub-labels) go to the sub list
             const bool stay = own_valid && sure && bi == a;
-            bool need_sub = own_valid && !stay;
+            bool undecided = false;
             if (stay) {
                 const float lg = logit_u32<float>(sub_word);
                 if (fabsf(lg - sd) > sderr) {
@@ -930,10 +938,11 @@ k_label_gauss_mxp(const float* __restrict__ X, int D, int K, int CH, const int32
                     sf.subs_old[pos] = v;
                     if (!sf.subs_prev || sf.subs_prev[pos] != v) sf.subs[idx] = v;
                 } else {
-                    need_sub = true;
+                    undecided = true;
                 }
             }
-            sl.push(need_sub, pos, sf.sub_list, sf.sub_count, lane, lt);
+            sl.push(undecided, pos, sf.sub_list + label_start[a], sf.seg + a, lane, lt);
+            ml.push(own_valid && !stay, pos, sf.sub_list + label_start[a + 1] - 1, sf.seg + K + a, lane, lt, -1);
         }
         // a label that moved off the chunk's cluster: the partition of the labels before the sweep no longer
         // stands (changed[0]), and the statistics of both clusters are recomputed in full (changed[1 + k])
@@ -948,7 +957,10 @@ k_label_gauss_mxp(const float* __restrict__ X, int D, int K, int CH, const int32
         rl.push(own_valid && !sure, (int32_t)idx, recheck, recheck_count, lane, lt);
     }
     rl.flush(recheck, recheck_count, lane);
-    if constexpr (FUSE) sl.flush(sf.sub_list, sf.sub_count, lane);
+    if constexpr (FUSE) {
+        sl.flush(sf.sub_list + label_start[a], sf.seg + a, lane);
+        ml.flush(sf.sub_list + label_start[a + 1] - 1, sf.seg + K + a, lane, -1);
+    }
     __shared__ unsigned long long cnt[2][4];   // one pair of counter atomics per workgroup
     if (lane == 0) {
         cnt[0][wv] = nev;

@@ -1421,6 +1421,7 @@ void pack_mx2_slot(int D, int DPM, const double* mu, const double* R, double cst
     // the screen relative to the parent cluster's mean: -c and g (see mx2_slot_bytes)
     float* rel = reinterpret_cast<float*>(dst + mx2_rel_offset(DPM));
     const double kappa = (double)sub_rel_eps_g(DPM) / (double)sub_rel_eps_a(DPM);
+    double G = 0.0, H = 0.0;
     for (int j = 0; j < D; ++j) {
         double c = 0.0, b = 0.0;
         for (int l = j; l < D; ++l) {
@@ -1428,10 +1429,19 @@ void pack_mx2_slot(int D, int DPM, const double* mu, const double* R, double cst
             const double dl = (double)(float)mu[l] - (double)(float)parent_mu[l];
             c += w * dl;
             b += std::fabs(w) * std::fabs(dl);
+            const uint32_t hb = (uint32_t)f32_to_bf16_rne((float)w) << 16;
+            float hf;
+            std::memcpy(&hf, &hb, 4);
+            H += (double)hf * (double)hf;
         }
         rel[j] = (float)(-c);
-        rel[DPM + j] = (float)(kappa * (b + std::fabs(c)) * (1.0 + 0x1p-20) + 0x1p-120);
+        const float g = (float)(kappa * (b + std::fabs(c)) * (1.0 + 0x1p-20) + 0x1p-120);
+        rel[DPM + j] = g;
+        G += (double)g * (double)g;
     }
+    // ||g|| and ||W_hi||_F, rounded up
+    m[DPM + 1] = (float)(std::sqrt(G) * (1.0 + 0x1p-20));
+    m[DPM + 2] = (float)(std::sqrt(H) * (1.0 + 0x1p-20));
 }
 
 int set_gauss(dpmm_ctx* c, int K, const double* mu, const double* mat, const double* logdet, const double* weights,

@@ -115,6 +115,8 @@ __host__ __device__ constexpr int mx2_rel_offset(int DP) { return 2 * mx_tiles_b
 __host__ __device__ constexpr float split_screen_eps(int DP) {
     return (0x1p-15f + 0x1p-22f + (5.0f * (float)DP + 24.0f) * 0x1p-24f) * (1.0f + 0x1p-6f);
 }
+// A sub-cluster slot also holds, after c, gn >= ||g|| and fh >= ||W_hi||_F (the fused sweep's bound on
+// sum_j a'_j^2, see fused_screens_split).
 // Per-row error factors of the relative sub-cluster screen (fused_screens_split): |y_screen,j - y_fp32,j| <=
 // eps_a a_j + eps_g g_j, eps_a as above; eps_g covers the screen's FP32 sum started from -c_j (3 DP + 1
 // additions), the rounding of c_j and the sweep's residual and sum over |W| |mu_h - mu_a|.
@@ -447,7 +449,9 @@ void launch_sub_gauss_mx(hipStream_t st, int grid, const float* X, int D, int K,
 //     y_j = sum_l W_jl fl(x_l - mu_h,l) the screen is off by at most eps_a a_j + eps_g g'_j, where the
 //     split residuals and the FP32 sums of both sides give eps_a and the FP32 roundings of p, of c and
 //     of the sweep's residual (|x - mu_h| <= |p| + |mu_h - mu_a|) give eps_g; g_j already carries
-//     eps_g / eps_a, so e_j <= eps_a a'_j and the bound on q_h follows as in sub_screen.
+//     eps_g / eps_a, so e_j <= eps_a a'_j and the bound on q_h follows as in sub_screen, with its
+//     second-order part eps_a^2 sum_j a'_j^2 / 2 taken per point: ||a'|| <= ||g|| + ||W_hi||_F ||p_hi||
+//     (triangle inequality, Cauchy-Schwarz per row) and ||p_hi|| <= (1 + 2^-8) ||p||.
 // Returns q_a; sets d2 = ||p||^2 (FP32), Aa, and sd = s_0 - s_1 with its bound sderr.
 typedef unsigned int mx_u32x4 __attribute__((ext_vector_type(4)));
 // The split p = p_hi + p_lo (BF16 pairs packed in 32-bit words) and dd = sum p^2 of this lane's coordinates.
@@ -545,12 +549,16 @@ __device__ __forceinline__ float fused_screens_split(const mx_u32x4 (&ph)[DP / 1
 #pragma unroll
             for (int v = 0; v < 16; ++v) {
                 qq = fmaf(y[v], y[v], qq);
-                ee = fmaf(a[v], fmaf(heps, a[v], fabsf(y[v])), ee);
+                ee = fmaf(a[v], fabsf(y[v]), ee);
             }
         }
         qq += __shfl_xor(qq, 32);
-        ee = 2.0f * eps * (ee + __shfl_xor(ee, 32));
-        const float sv = reinterpret_cast<const float*>(slot + 2 * TB)[DP] - 0.5f * qq;
+        // sum_j a'_j^2 <= (||g|| + ||W_hi||_F ||p_hi||)^2, ||p||^2 <= d2 (1 + (DP + 4) 2^-24); 2^-10 covers
+        // this line's rounding and the MFMA's of a'
+        const float* msub = reinterpret_cast<const float*>(slot + 2 * TB);
+        const float an = msub[DP + 1] + msub[DP + 2] * sqrtf(d2 * (1.0f + ((float)DP + 4.0f) * 0x1p-24f)) * (1.0f + 0x1p-7f);
+        ee = 2.0f * eps * (ee + __shfl_xor(ee, 32) + heps * an * an * (1.0f + 0x1p-10f));
+        const float sv = msub[DP] - 0.5f * qq;
         const float dv = ee + (2.0f * (float)DP + 4.0f) * 0x1p-24f * qq;   // the two sums of squares
         if (hh == 0) { s0 = sv; dq0 = dv; } else { s1 = sv; dq1 = dv; }
     }

@@ -787,6 +787,8 @@ __global__ void __launch_bounds__(kThreads) k_pack_gauss(const double* __restric
         const double* par = draws + (size_t)(3 * k) * draw_stride;
         float* rel = reinterpret_cast<float*>(mslot_base + mx2_rel_offset(DPM));
         const double kappa = (double)sub_rel_eps_g(DPM) / (double)sub_rel_eps_a(DPM);
+        __shared__ double part[2][kThreads];
+        double g2 = 0.0, h2 = 0.0;
         for (int j = threadIdx.x; j < D; j += blockDim.x) {
             double c = 0.0, b = 0.0;
             for (int l = j; l < D; ++l) {
@@ -794,9 +796,26 @@ __global__ void __launch_bounds__(kThreads) k_pack_gauss(const double* __restric
                 const double dl = (double)(float)src[l] - (double)(float)par[l];
                 c += w * dl;
                 b += fabs(w) * fabs(dl);
+                const double wh = (double)__uint_as_float((uint32_t)bf16_bits_rne((float)w) << 16);
+                h2 += wh * wh;
             }
             rel[j] = (float)(-c);
-            rel[DPM + j] = (float)(kappa * (b + fabs(c)) * (1.0 + 0x1p-20) + 0x1p-120);
+            const float g = (float)(kappa * (b + fabs(c)) * (1.0 + 0x1p-20) + 0x1p-120);
+            rel[DPM + j] = g;
+            g2 += (double)g * (double)g;
+        }
+        part[0][threadIdx.x] = g2;
+        part[1][threadIdx.x] = h2;
+        __syncthreads();
+        if (threadIdx.x == 0) {   // ||g|| and ||W_hi||_F, rounded up, in a fixed order
+            double G = 0.0, H = 0.0;
+            for (int t = 0; t < (int)blockDim.x; ++t) {
+                G += part[0][t];
+                H += part[1][t];
+            }
+            float* m = reinterpret_cast<float*>(mslot_base + mu_off);
+            m[DPM + 1] = (float)(sqrt(G) * (1.0 + 0x1p-20));
+            m[DPM + 2] = (float)(sqrt(H) * (1.0 + 0x1p-20));
         }
     }
 }

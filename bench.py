@@ -445,24 +445,30 @@ def main():
         a, b, c = dev.kernel_ms()   # HIP events of the phases (the step has synchronised already)
         lab_ms.append(a); sub_ms.append(b); st_ms.append(c); host_ms.append(smp.host_ms())
         main_ms.append(dev.sweep_kernel_ms())   # the pruned sweep's main kernel alone
-        st_mode.append(dev.last_statistics_update() if not mult else 0)
         flops.append(2.0 * kc * D * (1 if mult else D) * N)
     dev.synchronize()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     ms_step = 1000 * elapsed / args.steps
     value = N * args.steps / elapsed
-    # the mixed sweeps' recheck and pruning counters, read after a few untimed steps (each read is a copy
-    # and a synchronisation of its own)
+    # after the timed steps, untimed ones for what needs a copy and a synchronisation of its own: the
+    # mixed sweeps' recheck and pruning counters, and which statistics passes recomputed every point
+    # (the statistics' FP64 rate is taken from those); the timed steps' statistics times stand otherwise
     t = first + args.steps - 1
-    if prec == dp.Precision.MIXED:
-        for _ in range(5):
-            t += 1
-            smp.step(t)
+    st_ms_t, st_ms = st_ms, []
+    for u in range(10 if not mult else 5):
+        t += 1
+        smp.step(t)
+        if not mult:
+            st_ms.append(dev.kernel_ms()[2])
+            st_mode.append(dev.last_statistics_update())
+        if prec == dp.Precision.MIXED and u < 5:
             rechk.append(dev.last_recheck_count())
             if not mult:
                 srechk.append(dev.last_sub_recheck_count())
                 evfrac.append(dev.last_label_evaluations())
+    if mult or not st_ms:
+        st_ms, st_mode = st_ms_t, [0] * len(st_ms_t)
     if args.graph:   # phase events are not recorded inside a graph: the whole step only
         inst, upd = dev.graph_counts()
         print(json.dumps({"metric": "point*iters/sec", "value": value, "unit": "point*iters/s", "n_gpus": 1,
@@ -474,7 +480,7 @@ def main():
                           "final_clusters": smp.num_clusters()}), flush=True)
         return
     avg_lab = float(np.mean(lab_ms))
-    avg_sub, avg_st = float(np.mean(sub_ms)), float(np.mean(st_ms))
+    avg_sub, avg_st = float(np.mean(sub_ms)), float(np.mean(st_ms_t))   # the timed steps
     pruned = bool(evfrac) and float(np.mean(evfrac)) >= 0.0
 
     def hbm(kernel, bytes_launch, ms, what):

@@ -430,8 +430,9 @@ class DeviceSweep:
         _check(load_library().dpmm_set_statistics_refresh(self._ctx, int(every)))
 
     def last_statistics_update(self):
-        """The last statistics pass: 0 recomputed, 1 updated from the points whose sub-label moved, 2 updated
-        with the clusters a moved label touched recomputed."""
+        """The last statistics pass of a sweep (the one kernel_ms times, not a resets-only pass): 0 recomputed,
+        1 updated from the points whose sub-label moved, 2 updated with the clusters a moved label touched
+        recomputed."""
         v = ctypes.c_int()
         _check(load_library().dpmm_last_statistics_update(self._ctx, ctypes.byref(v)))
         return v.value

@@ -959,6 +959,7 @@ struct dpmm_ctx {
     int inc_passes = 0;        // passes since the last full one
     StatsGate stats_gate{};    // of the statistics kernels being enqueued
     bool last_inc = false;     // the last statistics pass enqueued the update from the moved points
+    bool timed_inc = false;    // so did the last pass the statistics phase events time (not a resets-only pass)
     // a relabel that redrew only the sub-labels of some clusters (resets) came after a pass that left
     // inc_base: the next pass updates those clusters from their moved sub-labels
     bool reset_update_ok = false;
@@ -2270,6 +2271,7 @@ static int enqueue_stats(dpmm_ctx* c, int K, const int32_t* dev_mask = nullptr) 
         if (!partial && !t_capturing) {
             (void)hipEventRecord(c->ev[5], c->stream);
             c->ev_stats = true;
+            c->timed_inc = inc;
         }
         c->stats_mask = nullptr;
         c->stats_cached = true;
@@ -2654,7 +2656,7 @@ int dpmm_set_statistics_refresh(dpmm_ctx* c, int every) {
 int dpmm_last_statistics_update(dpmm_ctx* c, int* moved_only) {
     if (!c || !moved_only) return fail(DPMM_EINVAL, "bad args");
     *moved_only = 0;
-    if (!c->last_inc) return DPMM_OK;
+    if (!c->timed_inc) return DPMM_OK;
     int flag = 1;
     HIPCHK(hipMemcpyAsync(&flag, c->dFlags, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));

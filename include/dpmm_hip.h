@@ -218,8 +218,10 @@ int dpmm_set_label_pruning(dpmm_ctx* ctx, int mode);
  * passes hold one full recomputation, which bounds the FP64 rounding the updates carry (default 8;
  * <= 1: every pass recomputes).  Counts are exact either way. */
 int dpmm_set_statistics_refresh(dpmm_ctx* ctx, int every);
-/* The last statistics pass: 0 a full recomputation, 1 an update of every cluster from the points whose
- * sub-label moved, 2 such an update with the clusters a moved label touched recomputed.  Synchronises. */
+/* The last statistics pass of a sweep (the one dpmm_last_kernel_ms times; a pass that only updates the
+ * clusters reset by the decisions is not counted): 0 a full recomputation, 1 an update of every cluster
+ * from the points whose sub-label moved, 2 such an update with the clusters a moved label touched
+ * recomputed.  Synchronises. */
 int dpmm_last_statistics_update(dpmm_ctx* ctx, int* moved_only);
 /* Sub-label draw fused into the pruned mixed label sweep (on by default, padded D <= 64; no counterpart
  * in the reference, whose sub-label sweep is a pass of its own, cudaKernel.cu:1299-1398).  A point

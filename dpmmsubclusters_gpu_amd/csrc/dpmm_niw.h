@@ -183,9 +183,12 @@ __device__ void niw_posterior(const double* __restrict__ stats, size_t E, int DP
     __syncthreads();
 }
 
-// Row-parallel layout of the triangular routines below (blockDim = kThreads, D <= 128): thread t
-// owns row t / LN and, within it, the columns congruent to t % LN modulo LN.
-__device__ __forceinline__ int row_lanes(int D) { return D <= 64 ? 4 : 2; }
+// ---- register forms of the triangular factorisations (D <= 64, blockDim = kThreads = 4 waves) ----
+// Lane r of every wave holds row r; wave w holds the columns j = 4 s + w (s < 16) in registers, so
+// each wave does a quarter of every step's updates.  Every matrix-index loop is unrolled so every
+// register index is static.  Each entry sees the same operations in the same order as in the
+// block-wide LDS forms below, so both give the same bits.
+static_assert(kThreads == 256, "the register forms assume four waves per workgroup");
 
 // Lane `lane` (a constant or wave-uniform index) of x, read by every lane.
 __device__ __forceinline__ double readlane_f64(double x, int lane) {
@@ -195,87 +198,99 @@ __device__ __forceinline__ double readlane_f64(double x, int lane) {
     return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
-// ---- one-wave forms of the triangular factorisations (D <= 64) ----------------------------------
-// The first wave does the work with one row per lane held in registers; values of other rows are
-// read across lanes (v_readlane into scalar registers), so there is no LDS traffic and no barrier
-// inside.  Each matrix-index loop is unrolled so every register index is static.  Each entry sees
-// the same operations in the same order as in the block-wide LDS forms below, so both give the same
-// bits.
-
-// Cholesky: lane r holds row r.  Step k: a lane r > k subtracts (a_rk / a_kk) a_jk from its entries
-// k < j <= r, a_jk read from lane j.  The same subtraction applied by the other lanes only reaches
-// entries above their diagonal (columns j > r, never read) or subtracts an exact zero (lanes r >= D
-// hold zeros), so it needs no mask.  Lane k keeps the pivot a_kk; the columns are scaled by
-// 1 / sqrt(a_kk) at the end (the LDS form's lazy scaling, the same product).
+// Cholesky.  Step k: the wave holding column k publishes it (as the earlier steps left it) in one of
+// two LDS vectors, one barrier, then every lane r forms f = a_rk / a_kk and subtracts f a_jk from
+// its entries k < j of its own columns.  The same subtraction reaches entries above the diagonal
+// (j > r, never read) or subtracts an exact zero (rows r >= D hold zeros), so it needs no mask.
+// The step after next overwrites the vector a step read, after the barrier between them.  Every
+// lane of wave k mod 4 keeps a_kk; the columns are scaled by 1 / sqrt(a_kk) at the end (the LDS
+// form's lazy scaling, the same product).
 __device__ bool chol_packed_wave(double* P, int D) {
     __shared__ int ok_w;
-    if (threadIdx.x < 64) {
-        const int r = threadIdx.x;
-        const int rb = r * (r + 1) / 2;
-        const bool own = r < D;
-        double v[64];
+    __shared__ double colk[2][64];
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), r = threadIdx.x & 63;
+    const int rb = r * (r + 1) / 2;
+    const bool own = r < D;
+    double v[16], piv[16];
 #pragma unroll
-        for (int j = 0; j < 64; ++j) v[j] = (own && j <= r) ? P[rb + j] : 0.0;
-        bool ok = true;
-        double mypiv = 1.0;
-#pragma unroll
-        for (int k = 0; k < 64; ++k) {
-            if (k < D && ok) {   // uniform
-                const double akk = readlane_f64(v[k], k);
-                ok = akk > 0.0 && isfinite(akk);   // uniform: every lane read the same pivot
-                if (ok) {
-                    if (r == k) mypiv = akk;
-                    const double f = v[k] / akk;
-#pragma unroll
-                    for (int j = k + 1; j < 64; ++j) v[j] -= f * readlane_f64(v[k], j);
-                }
-            }
-        }
-        if (ok) {
-            const double psq = sqrt(mypiv);
-            const double pinv = 1.0 / psq;
-#pragma unroll
-            for (int j = 0; j < 64; ++j) {
-                const double pj = readlane_f64(pinv, j);
-                const double sj = readlane_f64(psq, j);
-                if (own && j <= r) P[rb + j] = j == r ? sj : v[j] * pj;
-            }
-        }
-        if (r == 0) ok_w = ok;
+    for (int s = 0; s < 16; ++s) {
+        const int j = 4 * s + w;
+        v[s] = (own && j <= r) ? P[rb + j] : 0.0;
+        piv[s] = 1.0;
     }
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < 64; ++k) {
+        if (k < D && ok) {   // uniform in the workgroup
+            double* c = colk[k & 1];
+            const bool owner = w == (k & 3);
+            if (owner) c[r] = v[k >> 2];
+            __syncthreads();
+            // every read of the step is issued before any is used: the pivot, a_rk and a_jk of the
+            // columns j > k this wave holds (slots above k / 4, and slot k / 4 in the waves w > k % 4)
+            const double akk = c[k];
+            const double ark = c[r];
+            double cj[16];
+#pragma unroll
+            for (int s = k >> 2; s < 16; ++s) cj[s] = c[4 * s + w];
+            ok = akk > 0.0 && isfinite(akk);   // uniform: every thread read the same pivot
+            if (owner) piv[k >> 2] = akk;
+            const double f = ark / akk;   // garbage after a failed pivot, and then never written out
+#pragma unroll
+            for (int s = (k >> 2) + 1; s < 16; ++s) v[s] -= f * cj[s];
+            if (w > (k & 3)) v[k >> 2] -= f * cj[k >> 2];
+        }
+    }
+    if (ok) {
+#pragma unroll
+        for (int s = 0; s < 16; ++s) {
+            const int j = 4 * s + w;
+            const double psq = sqrt(piv[s]);
+            const double pinv = 1.0 / psq;
+            if (own && j <= r) P[rb + j] = j == r ? psq : v[s] * pinv;
+        }
+    }
+    if (threadIdx.x == 0) ok_w = ok;
     __syncthreads();
     const bool res = ok_w != 0;
     __syncthreads();   // ok_w is read before a later call writes it
     return res;
 }
 
-// X = L^-1 by forward substitution: lane r holds row r of X.  Step k: a lane r > k subtracts
-// (L_rk / L_kk) X_kj from its entries j <= k, X_kj read from lane k (unscaled); every row is scaled
-// by 1 / L_rr at the end.  Lanes r <= k take a zero factor (an exact no-op).  L is not modified.
+// X = L^-1 by forward substitution: step k subtracts (L_rk / L_kk) X_kj from the entries X_rj,
+// j <= k, of every row r > k (X unscaled); every row is scaled by 1 / L_rr at the end.  Row k of a
+// wave's columns is read from its lane k (v_readlane), so the waves never wait for one another and
+// the only barrier is the final one.  Lanes r <= k take a zero factor (an exact no-op).  L is not
+// modified.
 __device__ void lower_inverse_wave(const double* L, double* X, int D) {
-    if (threadIdx.x < 64) {
-        const int r = threadIdx.x;
-        const int rb = r * (r + 1) / 2;
-        double x[64];
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), r = threadIdx.x & 63;
+    const int rb = r * (r + 1) / 2;
+    double x[16];
 #pragma unroll
-        for (int j = 0; j < 64; ++j) x[j] = j == r ? 1.0 : 0.0;
+    for (int s = 0; s < 16; ++s) x[s] = 4 * s + w == r ? 1.0 : 0.0;
 #pragma unroll
-        for (int k = 0; k < 64; ++k) {
-            if (k < D) {   // uniform
-                const double f = (r > k && r < D) ? L[rb + (r > k ? k : 0)] / L[k * (k + 1) / 2 + k] : 0.0;
+    for (int k = 0; k < 64; ++k) {
+        if (k < D) {   // uniform
+            const double f = (r > k && r < D) ? L[rb + (r > k ? k : 0)] / L[k * (k + 1) / 2 + k] : 0.0;
 #pragma unroll
-                for (int j = 0; j <= k; ++j) x[j] -= f * readlane_f64(x[j], k);
-            }
+            for (int s = 0; s < (k >> 2); ++s) x[s] -= f * readlane_f64(x[s], k);   // columns j < k - 3
+            if (w <= (k & 3)) x[k >> 2] -= f * readlane_f64(x[k >> 2], k);
         }
-        if (r < D) {
-            const double inv = 1.0 / L[rb + r];
+    }
+    if (r < D) {
+        const double inv = 1.0 / L[rb + r];
 #pragma unroll
-            for (int j = 0; j < 64; ++j)
-                if (j <= r) X[rb + j] = x[j] * inv;
+        for (int s = 0; s < 16; ++s) {
+            const int j = 4 * s + w;
+            if (j <= r) X[rb + j] = x[s] * inv;
         }
     }
     __syncthreads();
 }
+
+// Row-parallel layout of the triangular routines below (blockDim = kThreads, D <= 128): thread t
+// owns row t / LN and, within it, the columns congruent to t % LN modulo LN.
+__device__ __forceinline__ int row_lanes(int D) { return D <= 64 ? 4 : 2; }
 
 // ---- FP64 MFMA products of packed lower-triangular D x D matrices (D <= 64) ---------------------
 // v_mfma_f64_16x16x4f64 with lane l = 16 kp + q: the first operand is element (q, kp) of a 16 x 4
@@ -359,7 +374,7 @@ __device__ void gram_rev_mfma(const double* R, double* G, int D) {
 // subtracts (a_ik / a_kk) a_jk from the trailing entries (k < j <= i) with column k as the
 // earlier steps left it, while the owners of column k - 1, which nothing reads any more, scale it
 // by 1 / sqrt(a_{k-1,k-1}).  One barrier per column.  Uniform result: false when a pivot is not
-// positive and finite (every thread reads the same pivot).  D <= 64 takes the one-wave form.
+// positive and finite (every thread reads the same pivot).  D <= 64 takes the register form.
 __device__ bool chol_packed(double* P, int D, double* /*unused*/) {
     if (D <= 64) return chol_packed_wave(P, D);
     const int LN = row_lanes(D), r = threadIdx.x / LN, q = threadIdx.x % LN;

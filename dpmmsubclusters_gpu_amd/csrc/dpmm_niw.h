@@ -184,10 +184,10 @@ __device__ void niw_posterior(const double* __restrict__ stats, size_t E, int DP
 }
 
 // ---- register forms of the triangular factorisations (D <= 64, blockDim = kThreads = 4 waves) ----
-// Lane r of every wave holds row r; wave w holds the columns j = 4 s + w (s < 16) in registers, so
-// each wave does a quarter of every step's updates.  Every matrix-index loop is unrolled so every
-// register index is static.  Each entry sees the same operations in the same order as in the
-// block-wide LDS forms below, so both give the same bits.
+// Lane r of every wave holds row r of the matrix; each wave holds a quarter of the columns in its
+// registers (16 slots per lane) and does that quarter of every step's updates.  Every matrix-index
+// loop is unrolled so every register index is static.  A step's cost is the latency of its chain
+// (pivot, quotient, update of the next column), not its arithmetic.
 static_assert(kThreads == 256, "the register forms assume four waves per workgroup");
 
 // Lane `lane` (a constant or wave-uniform index) of x, read by every lane.
@@ -198,53 +198,74 @@ __device__ __forceinline__ double readlane_f64(double x, int lane) {
     return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
-// Cholesky.  Step k: the wave holding column k publishes it (as the earlier steps left it) in one of
-// two LDS vectors, one barrier, then every lane r forms f = a_rk / a_kk and subtracts f a_jk from
-// its entries k < j of its own columns.  The same subtraction reaches entries above the diagonal
-// (j > r, never read) or subtracts an exact zero (rows r >= D hold zeros), so it needs no mask.
-// The step after next overwrites the vector a step read, after the barrier between them.  Every
-// lane of wave k mod 4 keeps a_kk; the columns are scaled by 1 / sqrt(a_kk) at the end (the LDS
-// form's lazy scaling, the same product).
+// Cholesky in panels of four columns, panel b held by wave b % 4 (wave w holds the columns
+// j = 16 (s / 4) + 4 w + s % 4 in its slots s < 16).  The panel's wave runs its four steps on the
+// panel's own columns in-wave (pivot and a_jk read across lanes with v_readlane), publishes the four
+// columns as each step left them in one of two LDS panels, one barrier, then every wave applies the
+// four steps to its columns right of the panel: lane r forms f = a_rk / a_kk and subtracts f a_jk.
+// Every entry sees the steps in order k = 0, 1, ... with the same operands and operations as in the
+// block-wide LDS form below, so both give the same bits.
+// The subtraction also reaches entries above the diagonal (j > r, never read) or subtracts an exact
+// zero (rows r >= D hold zeros), so it needs no mask.  The panel after next overwrites the LDS panel
+// a step read, after the barrier between them.  The panel's wave keeps a_kk; the columns are scaled
+// by 1 / sqrt(a_kk) at the end (the LDS form's lazy scaling, the same product).
 __device__ bool chol_packed_wave(double* P, int D) {
     __shared__ int ok_w;
-    __shared__ double colk[2][64];
+    __shared__ double colp[2][4][64];
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), r = threadIdx.x & 63;
     const int rb = r * (r + 1) / 2;
     const bool own = r < D;
     double v[16], piv[16];
 #pragma unroll
     for (int s = 0; s < 16; ++s) {
-        const int j = 4 * s + w;
+        const int j = 16 * (s >> 2) + 4 * w + (s & 3);
         v[s] = (own && j <= r) ? P[rb + j] : 0.0;
         piv[s] = 1.0;
     }
     bool ok = true;
 #pragma unroll
-    for (int k = 0; k < 64; ++k) {
-        if (k < D && ok) {   // uniform in the workgroup
-            double* c = colk[k & 1];
-            const bool owner = w == (k & 3);
-            if (owner) c[r] = v[k >> 2];
+    for (int b = 0; b < 16; ++b) {
+        if (4 * b < D && ok) {   // uniform in the workgroup
+            const int q = b >> 2;
+            double(*c)[64] = colp[b & 1];
+            if (w == (b & 3)) {
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const int k = 4 * b + t;
+                    if (k < D && ok) {   // uniform
+                        const double akk = readlane_f64(v[4 * q + t], k);
+                        ok = akk > 0.0 && isfinite(akk);
+                        piv[4 * q + t] = akk;
+                        const double f = v[4 * q + t] / akk;
+#pragma unroll
+                        for (int u = t + 1; u < 4; ++u) v[4 * q + u] -= f * readlane_f64(v[4 * q + t], 4 * b + u);
+                    }
+                    c[t][r] = v[4 * q + t];
+                }
+            }
             __syncthreads();
-            // every read of the step is issued before any is used: the pivot, a_rk and a_jk of the
-            // columns j > k this wave holds (slots above k / 4, and slot k / 4 in the waves w > k % 4)
-            const double akk = c[k];
-            const double ark = c[r];
-            double cj[16];
 #pragma unroll
-            for (int s = k >> 2; s < 16; ++s) cj[s] = c[4 * s + w];
-            ok = akk > 0.0 && isfinite(akk);   // uniform: every thread read the same pivot
-            if (owner) piv[k >> 2] = akk;
-            const double f = ark / akk;   // garbage after a failed pivot, and then never written out
+            for (int t = 0; t < 4; ++t) {
+                const int k = 4 * b + t;
+                if (k < D) {   // uniform
+                    const double akk = c[t][k];
+                    const double ark = c[t][r];
+                    ok = ok && akk > 0.0 && isfinite(akk);   // every thread reads the same pivots
+                    const double f = ark / akk;   // garbage after a failed pivot, never written out
 #pragma unroll
-            for (int s = (k >> 2) + 1; s < 16; ++s) v[s] -= f * cj[s];
-            if (w > (k & 3)) v[k >> 2] -= f * cj[k >> 2];
+                    for (int s = 4 * (q + 1); s < 16; ++s) v[s] -= f * c[t][16 * (s >> 2) + 4 * w + (s & 3)];
+                    if (w > (b & 3)) {   // this wave's panel 4 q + w lies right of panel b too
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) v[4 * q + u] -= f * c[t][16 * q + 4 * w + u];
+                    }
+                }
+            }
         }
     }
     if (ok) {
 #pragma unroll
         for (int s = 0; s < 16; ++s) {
-            const int j = 4 * s + w;
+            const int j = 16 * (s >> 2) + 4 * w + (s & 3);
             const double psq = sqrt(piv[s]);
             const double pinv = 1.0 / psq;
             if (own && j <= r) P[rb + j] = j == r ? psq : v[s] * pinv;
@@ -258,10 +279,10 @@ __device__ bool chol_packed_wave(double* P, int D) {
 }
 
 // X = L^-1 by forward substitution: step k subtracts (L_rk / L_kk) X_kj from the entries X_rj,
-// j <= k, of every row r > k (X unscaled); every row is scaled by 1 / L_rr at the end.  Row k of a
-// wave's columns is read from its lane k (v_readlane), so the waves never wait for one another and
-// the only barrier is the final one.  Lanes r <= k take a zero factor (an exact no-op).  L is not
-// modified.
+// j <= k, of every row r > k (X unscaled); every row is scaled by 1 / L_rr at the end.  Wave w holds
+// the columns j = 4 s + w in its slots s; row k of a wave's columns is read from its lane k
+// (v_readlane), so the waves never wait for one another and the only barrier is the final one.
+// Lanes r <= k take a zero factor (an exact no-op).  L is not modified.
 __device__ void lower_inverse_wave(const double* L, double* X, int D) {
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), r = threadIdx.x & 63;
     const int rb = r * (r + 1) / 2;

@@ -28,7 +28,10 @@ DEVICE_SOURCES = ["dpmm_device.hip", "inst_gauss_small.hip", "inst_gauss_64.hip"
 HOST_SOURCES = ["sampler.cpp"]
 # The BF16 screens read every MFMA result back into VALU arithmetic: the VGPR form of the MFMA
 # instructions saves one v_accvgpr_read per result element.
-SOURCE_FLAGS = {"inst_mixed.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+# dpmm_device.hip: the register forms of the FP64 triangular factorisations (D <= 128) are fully
+# unrolled loops over register arrays; above LLVM's default unroll budget the arrays stay in memory
+SOURCE_FLAGS = {"inst_mixed.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
+                "dpmm_device.hip": ["-mllvm", "-pragma-unroll-threshold=200000"]}
 
 
 def _run(cmd):

@@ -183,11 +183,13 @@ __device__ void niw_posterior(const double* __restrict__ stats, size_t E, int DP
     __syncthreads();
 }
 
-// ---- register forms of the triangular factorisations (D <= 64, blockDim = kThreads = 4 waves) ----
-// Lane r of every wave holds row r of the matrix; each wave holds a quarter of the columns in its
-// registers (16 slots per lane) and does that quarter of every step's updates.  Every matrix-index
-// loop is unrolled so every register index is static.  A step's cost is the latency of its chain
-// (pivot, quotient, update of the next column), not its arithmetic.
+// ---- register forms of the triangular factorisations (D <= 64 RPL, blockDim = kThreads = 4 waves) --
+// Lane r of every wave holds the rows r + 64 h (h < RPL) of the matrix; each wave holds a quarter of
+// the columns in its registers (16 RPL slots per row) and does that quarter of every step's updates.
+// Every matrix-index loop is unrolled so every register index is static; entries above the diagonal
+// that a static index bound shows are never read (row block 0 against columns >= 64) are skipped.
+// A step's cost is the latency of its chain (pivot, quotient, update of the next column), not its
+// arithmetic.
 static_assert(kThreads == 256, "the register forms assume four waves per workgroup");
 
 // Lane `lane` (a constant or wave-uniform index) of x, read by every lane.
@@ -198,65 +200,89 @@ __device__ __forceinline__ double readlane_f64(double x, int lane) {
     return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
+// LDS of the register Cholesky, shared by its instantiations: two panels of four 128-entry columns,
+// then the 128 pivots.
+__device__ __forceinline__ double* chol_wave_lds() {
+    __shared__ double buf[2 * 4 * 128 + 128];
+    return buf;
+}
+
 // Cholesky in panels of four columns, panel b held by wave b % 4 (wave w holds the columns
-// j = 16 (s / 4) + 4 w + s % 4 in its slots s < 16).  The panel's wave runs its four steps on the
+// j = 16 (s / 4) + 4 w + s % 4 in its slots s).  The panel's wave runs its four steps on the
 // panel's own columns in-wave (pivot and a_jk read across lanes with v_readlane), publishes the four
 // columns as each step left them in one of two LDS panels, one barrier, then every wave applies the
-// four steps to its columns right of the panel: lane r forms f = a_rk / a_kk and subtracts f a_jk.
+// four steps to its columns right of the panel: row i forms f = a_ik / a_kk and subtracts f a_jk.
 // Every entry sees the steps in order k = 0, 1, ... with the same operands and operations as in the
-// block-wide LDS form below, so both give the same bits.
-// The subtraction also reaches entries above the diagonal (j > r, never read) or subtracts an exact
-// zero (rows r >= D hold zeros), so it needs no mask.  The panel after next overwrites the LDS panel
-// a step read, after the barrier between them.  The panel's wave keeps a_kk; the columns are scaled
-// by 1 / sqrt(a_kk) at the end (the LDS form's lazy scaling, the same product).
+// block-wide LDS form below, so both give the same bits.  The subtraction also reaches entries above
+// the diagonal (never read) or subtracts an exact zero (rows past D hold zeros), so it needs no mask.
+// The panel after next overwrites the LDS panel a step read, after the barrier between them.  The
+// columns are scaled by 1 / sqrt(a_kk) at the end (the LDS form's lazy scaling, the same product).
+template <int RPL>
 __device__ bool chol_packed_wave(double* P, int D) {
+    constexpr int NS = 16 * RPL, NC = 64 * RPL;
     __shared__ int ok_w;
-    __shared__ double colp[2][4][64];
+    double* lds = chol_wave_lds();
+    double* piv = lds + 2 * 4 * 128;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), r = threadIdx.x & 63;
-    const int rb = r * (r + 1) / 2;
-    const bool own = r < D;
-    double v[16], piv[16];
+    double v[RPL][NS];
 #pragma unroll
-    for (int s = 0; s < 16; ++s) {
-        const int j = 16 * (s >> 2) + 4 * w + (s & 3);
-        v[s] = (own && j <= r) ? P[rb + j] : 0.0;
-        piv[s] = 1.0;
+    for (int h = 0; h < RPL; ++h) {
+        const int i = r + 64 * h;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            const int j = 16 * (s >> 2) + 4 * w + (s & 3);
+            v[h][s] = (i < D && j <= i) ? P[i * (i + 1) / 2 + j] : 0.0;
+        }
     }
     bool ok = true;
 #pragma unroll
-    for (int b = 0; b < 16; ++b) {
+    for (int b = 0; b < NS; ++b) {
         if (4 * b < D && ok) {   // uniform in the workgroup
             const int q = b >> 2;
-            double(*c)[64] = colp[b & 1];
+            double* c = lds + (b & 1) * 4 * 128;   // column t of the panel at c + 128 t
             if (w == (b & 3)) {
 #pragma unroll
                 for (int t = 0; t < 4; ++t) {
                     const int k = 4 * b + t;
                     if (k < D && ok) {   // uniform
-                        const double akk = readlane_f64(v[4 * q + t], k);
+                        const double akk = readlane_f64(v[k >> 6][4 * q + t], k & 63);
                         ok = akk > 0.0 && isfinite(akk);
-                        piv[4 * q + t] = akk;
-                        const double f = v[4 * q + t] / akk;
+                        if (r == 0) piv[k] = akk;
 #pragma unroll
-                        for (int u = t + 1; u < 4; ++u) v[4 * q + u] -= f * readlane_f64(v[4 * q + t], 4 * b + u);
+                        for (int h = 0; h < RPL; ++h) {
+                            if (64 * h + 63 <= k) continue;   // rows above the pivot
+                            const double f = v[h][4 * q + t] / akk;
+#pragma unroll
+                            for (int u = t + 1; u < 4; ++u)
+                                v[h][4 * q + u] -= f * readlane_f64(v[(4 * b + u) >> 6][4 * q + t], (4 * b + u) & 63);
+                        }
                     }
-                    c[t][r] = v[4 * q + t];
                 }
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+#pragma unroll
+                    for (int h = 0; h < RPL; ++h) c[128 * t + r + 64 * h] = v[h][4 * q + t];
             }
             __syncthreads();
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
                 const int k = 4 * b + t;
                 if (k < D) {   // uniform
-                    const double akk = c[t][k];
-                    const double ark = c[t][r];
+                    const double akk = c[128 * t + k];
                     ok = ok && akk > 0.0 && isfinite(akk);   // every thread reads the same pivots
-                    const double f = ark / akk;   // garbage after a failed pivot, never written out
 #pragma unroll
-                    for (int s = 4 * (q + 1); s < 16; ++s) v[s] -= f * c[t][16 * (s >> 2) + 4 * w + (s & 3)];
-                    if (w > (b & 3)) {   // this wave's panel 4 q + w lies right of panel b too
+                    for (int h = 0; h < RPL; ++h) {
+                        if (64 * h + 63 <= k) continue;   // rows above the pivot: nothing below the diagonal
+                        const double f = c[128 * t + r + 64 * h] / akk;   // garbage after a failed pivot
 #pragma unroll
-                        for (int u = 0; u < 4; ++u) v[4 * q + u] -= f * c[t][16 * q + 4 * w + u];
+                        for (int s = 4 * (q + 1); s < NS; ++s) {
+                            if (64 * h + 63 < 16 * (s >> 2)) continue;   // above the diagonal
+                            v[h][s] -= f * c[128 * t + 16 * (s >> 2) + 4 * w + (s & 3)];
+                        }
+                        if (w > (b & 3) && 64 * h + 63 >= 16 * q) {   // this wave's panel 4 q + w is right of b
+#pragma unroll
+                            for (int u = 0; u < 4; ++u) v[h][4 * q + u] -= f * c[128 * t + 16 * q + 4 * w + u];
+                        }
                     }
                 }
             }
@@ -264,46 +290,66 @@ __device__ bool chol_packed_wave(double* P, int D) {
     }
     if (ok) {
 #pragma unroll
-        for (int s = 0; s < 16; ++s) {
-            const int j = 16 * (s >> 2) + 4 * w + (s & 3);
-            const double psq = sqrt(piv[s]);
-            const double pinv = 1.0 / psq;
-            if (own && j <= r) P[rb + j] = j == r ? psq : v[s] * pinv;
+        for (int h = 0; h < RPL; ++h) {
+            const int i = r + 64 * h;
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                const int j = 16 * (s >> 2) + 4 * w + (s & 3);
+                if (64 * h + 63 < 16 * (s >> 2)) continue;
+                if (i < D && j <= i) {
+                    const double psq = sqrt(piv[j]);
+                    P[i * (i + 1) / 2 + j] = j == i ? psq : v[h][s] * (1.0 / psq);
+                }
+            }
         }
     }
+    static_assert(NC <= 128, "the LDS panels hold 128-entry columns");
     if (threadIdx.x == 0) ok_w = ok;
     __syncthreads();
     const bool res = ok_w != 0;
-    __syncthreads();   // ok_w is read before a later call writes it
+    __syncthreads();   // ok_w and the pivots are read before a later call writes them
     return res;
 }
 
-// X = L^-1 by forward substitution: step k subtracts (L_rk / L_kk) X_kj from the entries X_rj,
-// j <= k, of every row r > k (X unscaled); every row is scaled by 1 / L_rr at the end.  Wave w holds
-// the columns j = 4 s + w in its slots s; row k of a wave's columns is read from its lane k
-// (v_readlane), so the waves never wait for one another and the only barrier is the final one.
-// Lanes r <= k take a zero factor (an exact no-op).  L is not modified.
+// X = L^-1 by forward substitution: step k subtracts (L_ik / L_kk) X_kj from the entries X_ij,
+// j <= k, of every row i > k (X unscaled); every row is scaled by 1 / L_ii at the end.  Wave w holds
+// the columns j = 4 s + w in its slots s; row k of a wave's columns is read from the lane holding it
+// (v_readlane), so the waves never wait for one another and the only barrier is the final one.  Rows
+// i <= k take a zero factor (an exact no-op).  L is not modified.
+template <int RPL>
 __device__ void lower_inverse_wave(const double* L, double* X, int D) {
+    constexpr int NS = 16 * RPL;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), r = threadIdx.x & 63;
-    const int rb = r * (r + 1) / 2;
-    double x[16];
+    double x[RPL][NS];
 #pragma unroll
-    for (int s = 0; s < 16; ++s) x[s] = 4 * s + w == r ? 1.0 : 0.0;
+    for (int h = 0; h < RPL; ++h)
 #pragma unroll
-    for (int k = 0; k < 64; ++k) {
+        for (int s = 0; s < NS; ++s) x[h][s] = 4 * s + w == r + 64 * h ? 1.0 : 0.0;
+#pragma unroll
+    for (int k = 0; k < 64 * RPL; ++k) {
         if (k < D) {   // uniform
-            const double f = (r > k && r < D) ? L[rb + (r > k ? k : 0)] / L[k * (k + 1) / 2 + k] : 0.0;
+            const double lkk = L[k * (k + 1) / 2 + k];
 #pragma unroll
-            for (int s = 0; s < (k >> 2); ++s) x[s] -= f * readlane_f64(x[s], k);   // columns j < k - 3
-            if (w <= (k & 3)) x[k >> 2] -= f * readlane_f64(x[k >> 2], k);
+            for (int h = 0; h < RPL; ++h) {
+                if (64 * h + 63 <= k) continue;   // every row of the block is at or above k
+                const int i = r + 64 * h;
+                const double f = (i > k && i < D) ? L[i * (i + 1) / 2 + (i > k ? k : 0)] / lkk : 0.0;
+#pragma unroll
+                for (int s = 0; s < (k >> 2); ++s) x[h][s] -= f * readlane_f64(x[k >> 6][s], k & 63);
+                if (w <= (k & 3)) x[h][k >> 2] -= f * readlane_f64(x[k >> 6][k >> 2], k & 63);
+            }
         }
     }
-    if (r < D) {
-        const double inv = 1.0 / L[rb + r];
 #pragma unroll
-        for (int s = 0; s < 16; ++s) {
-            const int j = 4 * s + w;
-            if (j <= r) X[rb + j] = x[s] * inv;
+    for (int h = 0; h < RPL; ++h) {
+        const int i = r + 64 * h;
+        if (i < D) {
+            const double inv = 1.0 / L[i * (i + 1) / 2 + i];
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                const int j = 4 * s + w;
+                if (j <= i) X[i * (i + 1) / 2 + j] = x[h][s] * inv;
+            }
         }
     }
     __syncthreads();
@@ -395,9 +441,10 @@ __device__ void gram_rev_mfma(const double* R, double* G, int D) {
 // subtracts (a_ik / a_kk) a_jk from the trailing entries (k < j <= i) with column k as the
 // earlier steps left it, while the owners of column k - 1, which nothing reads any more, scale it
 // by 1 / sqrt(a_{k-1,k-1}).  One barrier per column.  Uniform result: false when a pivot is not
-// positive and finite (every thread reads the same pivot).  D <= 64 takes the register form.
+// positive and finite (every thread reads the same pivot).  D <= 128 takes the register form.
 __device__ bool chol_packed(double* P, int D, double* /*unused*/) {
-    if (D <= 64) return chol_packed_wave(P, D);
+    if (D <= 64) return chol_packed_wave<1>(P, D);
+    if (D <= 128) return chol_packed_wave<2>(P, D);
     const int LN = row_lanes(D), r = threadIdx.x / LN, q = threadIdx.x % LN;
     const int rb = r * (r + 1) / 2;
     const bool own = r < D;
@@ -426,7 +473,8 @@ __device__ bool chol_packed(double* P, int D, double* /*unused*/) {
 // (L_ik / L_kk) X_kj (j <= k) from every row i > k, row k as the earlier steps left it, while
 // row k - 1, which nothing reads any more, is scaled by 1 / L_{k-1,k-1}.  One barrier per row.
 __device__ void lower_inverse_packed(const double* L, double* X, int D) {
-    if (D <= 64) return lower_inverse_wave(L, X, D);
+    if (D <= 64) return lower_inverse_wave<1>(L, X, D);
+    if (D <= 128) return lower_inverse_wave<2>(L, X, D);
     const int LN = row_lanes(D), r = threadIdx.x / LN, q = threadIdx.x % LN;
     const int rb = r * (r + 1) / 2;
     const bool own = r < D;
@@ -537,7 +585,7 @@ __device__ void gram_rev_packed(const double* R, double* G, int D) {
 // reference's log_marginal_likelihood(prior, posterior, stats) takes them separately, niw.cpp:122-138,
 // and its split test forms sub-cluster posteriors from the cluster's prior,
 // local_clusters_actions.cpp:221-228).
-__global__ void __launch_bounds__(kThreads, 4) k_log_marginal(const double* __restrict__ stats, int E, int DP, int D,
+__global__ void __launch_bounds__(kThreads) k_log_marginal(const double* __restrict__ stats, int E, int DP, int D,
                                                            int prior, const int32_t* __restrict__ members,
                                                            const int32_t* __restrict__ psel, PriorArgs h,
                                                            PriorArgs h1, double* __restrict__ out,

@@ -359,7 +359,7 @@ __device__ void lower_inverse_wave(const double* L, double* X, int D) {
 // owns row t / LN and, within it, the columns congruent to t % LN modulo LN.
 __device__ __forceinline__ int row_lanes(int D) { return D <= 64 ? 4 : 2; }
 
-// ---- FP64 MFMA products of packed lower-triangular D x D matrices (D <= 64) ---------------------
+// ---- FP64 MFMA products of packed lower-triangular D x D matrices (D <= 128) --------------------
 // v_mfma_f64_16x16x4f64 with lane l = 16 kp + q: the first operand is element (q, kp) of a 16 x 4
 // block, the second element (kp, q) of a 4 x 16 block, and accumulator v element (4 v + kp, q) of the
 // 16 x 16 product.  Entries outside the triangle or past D read as zero.  The sums run in the
@@ -421,6 +421,76 @@ __device__ void gram_rev_mfma(const double* R, double* G, int D) {
     for (int t = wv; t < 10; t += 4) {
         const int tile = (int)((kTiles >> (4 * t)) & 15);
         const int I = tile >> 2, J = tile & 3;
+        if (16 * I >= D) continue;
+        f64x4_t c = {0.0, 0.0, 0.0, 0.0};
+        for (int s = 0; s < 4 * J + 4; ++s) {
+            const double x = packed_at(R, 16 * I + q, 4 * s + kp, D);
+            const double y = packed_at(R, 16 * J + q, 4 * s + kp, D);
+            c = __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, c, 0, 0, 0);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int i = 16 * I + 4 * v + kp, j = 16 * J + q;
+            if (j <= i && i < D) G[tri(D - 1 - j, D - 1 - i)] = c[v];
+        }
+    }
+    __syncthreads();
+}
+
+// Tile t of the lower 16 x 16 block triangle of an 8 x 8 block matrix (64 < D <= 128), dealt by
+// decreasing block distance I - J (the MFMA count of a product tile grows with it), so the waves
+// taking t = w, w + 4, ... get similar shares.
+__device__ __forceinline__ void lower_tile8(int t, int& I, int& J) {
+    int d = 7;
+    while (t >= 8 - d) {
+        t -= 8 - d;
+        --d;
+    }
+    I = t + d;
+    J = t;
+}
+
+// R = B A written over A for 64 < D <= 128: the 36 lower tiles, nine per wave, held until every wave
+// has read A.  Tile (I, J) sums over k in [16 J, 16 I + 16), as for D <= 64.
+__device__ void lower_product_mfma8(const double* B, double* A, int D) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int kp = lane >> 4, q = lane & 15;
+    f64x4_t acc[9];
+#pragma unroll
+    for (int u = 0; u < 9; ++u) {
+        int I, J;
+        lower_tile8(wv + 4 * u, I, J);
+        f64x4_t c = {0.0, 0.0, 0.0, 0.0};
+        if (16 * I < D) {
+            for (int s = 4 * J; s < 4 * I + 4; ++s) {
+                const double x = packed_at(B, 16 * I + q, 4 * s + kp, D);
+                const double y = packed_at(A, 4 * s + kp, 16 * J + q, D);
+                c = __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, c, 0, 0, 0);
+            }
+        }
+        acc[u] = c;
+    }
+    __syncthreads();   // every wave has read A
+#pragma unroll
+    for (int u = 0; u < 9; ++u) {
+        int I, J;
+        lower_tile8(wv + 4 * u, I, J);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int i = 16 * I + 4 * v + kp, j = 16 * J + q;
+            if (j <= i && i < D) A[i * (i + 1) / 2 + j] = acc[u][v];
+        }
+    }
+    __syncthreads();
+}
+
+// G = J (R R^T) J for 64 < D <= 128, tile by tile as for D <= 64.
+__device__ void gram_rev_mfma8(const double* R, double* G, int D) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int kp = lane >> 4, q = lane & 15;
+    for (int t = wv; t < 36; t += 4) {
+        int I, J;
+        lower_tile8(t, I, J);
         if (16 * I >= D) continue;
         f64x4_t c = {0.0, 0.0, 0.0, 0.0};
         for (int s = 0; s < 4 * J + 4; ++s) {
@@ -504,6 +574,7 @@ __device__ void lower_inverse_packed(const double* L, double* X, int D) {
 // (k from j to i, in order) in registers, then all are stored after a barrier.
 __device__ void lower_product_packed(const double* B, double* A, int D) {
     if (D <= 64) return lower_product_mfma(B, A, D);
+    if (D <= 128) return lower_product_mfma8(B, A, D);
     constexpr int kMaxE = (128 * 129 / 2 + kThreads - 1) / kThreads;
     const int T = D * (D + 1) / 2;
     double acc[kMaxE];
@@ -563,6 +634,7 @@ __device__ void reverse_packed(double* X, int D) {
 // a = D-1-i <= b = D-1-j (k in increasing order).
 __device__ void gram_rev_packed(const double* R, double* G, int D) {
     if (D <= 64) return gram_rev_mfma(R, G, D);
+    if (D <= 128) return gram_rev_mfma8(R, G, D);
     const int T = D * (D + 1) / 2;
     for (int e = threadIdx.x; e < T; e += blockDim.x) {
         int i, j;

@@ -731,6 +731,7 @@ __global__ void __launch_bounds__(kThreads) k_prune_prep(const float* __restrict
     for (int itp = 0; itp < nit; ++itp) {
         double uj = 0.0;
         if (row < D)
+#pragma unroll 8
             for (int l = row + part; l < D; l += TPR) uj = fma((double)W[row * DS + l], v[l], uj);
         for (int o = 1; o < TPR; o <<= 1) uj += __shfl_xor(uj, o);
         if (row < D && part == 0) uu[row] = uj;
@@ -738,6 +739,7 @@ __global__ void __launch_bounds__(kThreads) k_prune_prep(const float* __restrict
         else __syncthreads();                                                      // publishes uu
         double zi = 0.0;
         if (row < D)
+#pragma unroll 8
             for (int j = part; j <= row; j += TPR) zi = fma((double)W[j * DS + row], uu[j], zi);
         for (int o = 1; o < TPR; o <<= 1) zi += __shfl_xor(zi, o);
         const double nz = block_sum(row < D && part == 0 ? zi * zi : 0.0, red);
@@ -753,6 +755,7 @@ __global__ void __launch_bounds__(kThreads) k_prune_prep(const float* __restrict
             int i, j;
             tri_ij(e, i, j);   // j <= i
             double lam = 0.0;
+#pragma unroll 8
             for (int l = 0; l <= j; ++l) lam = fma((double)W[l * DS + i], (double)W[l * DS + j], lam);
             M[e] = (i == j ? tt : 0.0) - lam;
         }
@@ -864,6 +867,7 @@ __global__ void __launch_bounds__(kThreads) k_prune_prep(const float* __restrict
         double sy = 0.0, se = 0.0;
         for (int j = lane; j < D; j += 64) {
             double y = 0.0;
+#pragma unroll 8
             for (int l = j; l < D; ++l) y = fma((double)W[j * DS + l], dv[l], y);
             sy = fma(y, y, sy);
             se = fma(dv[j], dv[j], se);

@@ -80,15 +80,18 @@ def test_bad_parameters():
     dev.close()
 
 
-def test_non_spd_posterior_gives_nan_and_the_sampler_reports_it():
-    X, _ = _setup()
-    D = X.shape[1]
+# D = 4 and 64 take the one-row register factorisations, D = 128 the two-row ones; "mid" puts the one
+# negative pivot in the middle (a later panel, another wave's), also for the reversed factorisation
+@pytest.mark.parametrize("D,where", [(4, "all"), (4, "mid"), (64, "mid"), (128, "all"), (128, "mid")])
+def test_non_spd_posterior_gives_nan_and_the_sampler_reports_it(D, where):
+    X, _ = _setup(D=D)
     dev = dp.DeviceSweep(X, dp.prior_type.Gaussian, seed=1)
     dev.set_prior(dp.niw_hyperparams(1.0, np.zeros(D), D + 3.0, np.eye(D)))
-    say("statistics whose posterior is not SPD")
+    say(f"statistics whose posterior is not SPD, D={D} {where}")
     Nsub = np.array([10, 10], np.int64)
     sums = np.zeros((2, D))
-    S = np.stack([-1e6 * np.eye(D), np.eye(D)])
+    bad = -1e6 * np.eye(D) if where == "all" else np.diag(np.where(np.arange(D) == D // 2, -1e6, 1.0))
+    S = np.stack([bad, np.eye(D)])
     dev.set_statistics(Nsub, sums, S)
     L, ldet = dev.log_marginal_batch(np.array([[0, -1, -1, -1], [1, -1, -1, -1], [0, 1, -1, -1]]))
     assert np.isnan(L[0]) and np.isfinite(L[1]) and np.isnan(L[2])
